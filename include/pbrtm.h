@@ -1,0 +1,133 @@
+/* pbrtm — MI355X (gfx950) HLBVH construction and traversal behind a C ABI.
+ *
+ * Every function returns 0 (PBRTM_OK) on success or a negative PBRTM_ERR_*
+ * code; the message is then available from pbrtm_last_error(ctx). No C++
+ * exception crosses this boundary and nothing calls exit(): the reference's
+ * CHECK_CUDA_ERROR / REPORT_FATAL_ERROR abort paths (gpu/macro.h:19-51)
+ * become error codes here.
+ *
+ * Pointers are host pointers unless PBRTM_DEVICE_PTRS is set in `flags`, in
+ * which case they are device pointers on the context's device; every call is
+ * complete (stream synchronised) when it returns. A context is used by one
+ * host thread at a time and owns one HIP stream.
+ *
+ * Data formats
+ *   triangles  [n][3][3] float: three vertices per triangle (an unindexed soup),
+ *              or vertices [nv][3] float + indices [n][3] uint32 (a TriangleMesh,
+ *              shapes/triangle_mesh.cu:39-106, vertices already in render space).
+ *   rays       [n][8] float: o.x o.y o.z t_max d.x d.y d.z unused. t_max is the
+ *              reference's intersect(ray, t_max) argument; a ray with
+ *              !(t_max > 0) is inactive and reports a miss.
+ *   hits       prim id uint32 (0xFFFFFFFF = miss), t float, barycentrics
+ *              (b0, b1, b2) float — the reference's ShapeIntersection::t_hit and
+ *              TriangleIntersection (shapes/triangle.h:15-22).
+ */
+#ifndef PBRTM_H
+#define PBRTM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBRTM_OK 0
+#define PBRTM_ERR_INVALID (-1)   /* bad argument */
+#define PBRTM_ERR_HIP (-2)       /* HIP runtime error */
+#define PBRTM_ERR_DEVICE (-3)    /* a kernel reported a condition (see message) */
+#define PBRTM_ERR_NOMEM (-4)
+#define PBRTM_ERR_UNSUPPORTED (-5)
+
+#define PBRTM_DEVICE_PTRS 1u
+
+#define PBRTM_NO_HIT 0xFFFFFFFFu
+
+typedef struct pbrtm_ctx pbrtm_ctx;
+typedef struct pbrtm_bvh pbrtm_bvh;
+
+/* Per-phase build timings (HIP events on the context stream) and shape. */
+typedef struct pbrtm_build_stats {
+    double ms_total;      /* host wall clock of the whole call */
+    double ms_sort;       /* centroid bounds + Morton codes + radix sort */
+    double ms_treelets;   /* sorted records, treelet emission */
+    double ms_top;        /* SAH over treelets */
+    double ms_bottom;     /* midpoint splits inside treelets */
+    double ms_layout;     /* traversal layout (leaf-ordered triangles) */
+    uint64_t num_prims;
+    uint64_t num_treelets;
+    uint64_t top_levels;      /* level-synchronous steps for nodes > 64 treelets */
+    uint64_t bottom_levels;   /* level-synchronous steps for nodes > 64 primitives */
+    uint64_t small_top;       /* wave-finished top subtrees */
+    uint64_t small_bottom;    /* wave-finished bottom subtrees */
+} pbrtm_build_stats;
+
+/* No counterpart in the reference (its objects live in one process-wide CUDA
+ * context, main.cu:124-146). device >= 0 selects a HIP device. */
+int pbrtm_create(pbrtm_ctx **out, int device);
+void pbrtm_destroy(pbrtm_ctx *ctx);
+const char *pbrtm_last_error(const pbrtm_ctx *ctx);
+int pbrtm_device_count(int *count);
+/* The HIP stream the context launches on (a hipStream_t), for callers that
+ * time or order work around it. */
+void *pbrtm_stream(pbrtm_ctx *ctx);
+
+/* Morton codes and the canonical order. Replaces hlbvh.cu:101-141 (codes) and
+ * the treelet counting sort hlbvh.cu:457-502: codes_out[i] is the 30-bit code
+ * of the i-th triangle in sorted order and order_out[i] its input index; the
+ * order sorts by full code with ties by input index, which refines the
+ * reference's treelet-only order. */
+int pbrtm_morton_sort(pbrtm_ctx *ctx, const float *tri, uint64_t n, uint32_t flags, uint32_t *codes_out,
+                      uint32_t *order_out);
+
+/* Builds the aggregate. Replaces `static const HLBVH *HLBVH::create(const
+ * std::vector<const Primitive *> &, GPUMemoryAllocator &)` (accelerator/hlbvh.h:76-77,
+ * hlbvh.cu:171-177, build_bvh :384-624). idx == NULL means `verts` is a soup of
+ * 3*nt vertices. With PBRTM_DEVICE_PTRS and idx == NULL the handle keeps
+ * `verts` by pointer (as HLBVH keeps its primitives, hlbvh.h:118-121): it must
+ * outlive the handle. n == 0 builds an empty aggregate (hlbvh.cu:392-395). */
+int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint32_t *idx, uint64_t nt,
+                    uint32_t flags, pbrtm_bvh **out, pbrtm_build_stats *stats);
+void pbrtm_bvh_free(pbrtm_bvh *bvh);
+
+/* Bounds of the whole aggregate: lo.xyz, hi.xyz. Replaces HLBVH::bounds()
+ * (hlbvh.h:79-86); an empty aggregate gives the empty box. */
+int pbrtm_bvh_bounds(const pbrtm_bvh *bvh, float out6[6]);
+
+/* The tree as a canonical byte string, independent of node numbering and of
+ * primitive order inside leaves: preorder, left (= "< split" side) first; per
+ * node u32 tag (1 = leaf, else axis << 8), six u32 bound bit patterns (-0.0
+ * written as +0.0), u32 primitive count, and for a leaf its primitive ids
+ * ascending. buf == NULL returns the size in *len. No reference counterpart:
+ * it makes the reference's atomic-ordered tree (hlbvh.cu:65, :167, :796)
+ * comparable. */
+int pbrtm_bvh_export_canonical(const pbrtm_bvh *bvh, uint8_t *buf, uint64_t *len);
+
+/* Batched ray queries. any_hit = 0 replaces HLBVH::intersect (closest hit,
+ * hlbvh.cu:231-289); any_hit = 1 replaces HLBVH::fast_intersect (hlbvh.cu:179-229),
+ * whose bool is prim_out != PBRTM_NO_HIT. t_out and bary_out ([n][3]) may be NULL. */
+int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_hit, uint32_t flags,
+                        uint32_t *prim_out, float *t_out, float *bary_out);
+
+/* From closest hits (prim, barycentrics), one cosine-weighted diffuse bounce
+ * per ray, as integrators/ambient_occlusion.cu:36-52 spawns it: u = the first
+ * two PCG32 draws of sequence = ray index, frame around the face-forwarded
+ * geometric normal, origin offset past the hit's error bounds (base/ray.h:22-43).
+ * Misses and zero-pdf samples give inactive rays. */
+int pbrtm_ao_bounce(pbrtm_bvh *bvh, const float *rays, const uint32_t *prim, const float *bary, uint64_t n,
+                    uint32_t flags, float *rays_out);
+
+/* Workload generators (host, deterministic, thread-count independent).
+ * Triangle soup: centroid c ~ U[0,1)^3, each vertex c + l*u with
+ * l ~ LogNormal(mu, sigma) (Box-Muller) and u uniform on the sphere; triangle i
+ * uses draws 16i.. of the PCG32 sequence `seed`. */
+int pbrtm_gen_soup(uint64_t seed, uint64_t n, float mu, float sigma, int threads, float *tri_out);
+/* One pinhole ray per pixel centre, row-major from the top-left pixel,
+ * vertical field of view fov_deg, t_max = +inf. */
+int pbrtm_gen_camera_rays(uint32_t width, uint32_t height, const float eye[3], const float look[3],
+                          const float up[3], float fov_deg, float *rays_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBRTM_H */

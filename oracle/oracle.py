@@ -1,0 +1,142 @@
+"""TEST INFRASTRUCTURE ONLY — ctypes view of the CPU oracle (liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module, and only as the checker or the timed CPU baseline. The product
+library never calls it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_float, c_int, c_int64, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+NO_HIT = 0xFFFFFFFF
+
+_lib = None
+
+
+class BuildInfo(ctypes.Structure):
+    _fields_ = [("num_treelets", c_uint64), ("top_nodes", c_uint64), ("total_nodes", c_uint64),
+                ("max_treelet", c_uint64)]
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} missing; run `make -C oracle`")
+    lib = ctypes.CDLL(LIB_PATH)
+    p = c_void_p
+    sig = {
+        "orc_morton_codes": (c_int, [p, c_uint64, p]),
+        "orc_morton_sort": (c_int, [p, c_uint64, p, p]),
+        "orc_bvh_build": (c_int, [p, c_uint64, POINTER(p), POINTER(BuildInfo)]),
+        "orc_bvh_error": (c_char_p, [p]),
+        "orc_bvh_free": (None, [p]),
+        "orc_bvh_export_canonical": (c_int, [p, p, POINTER(c_uint64)]),
+        "orc_bvh_intersect": (c_int, [p, p, c_uint64, c_int, c_int, p, p, p, p]),
+        "orc_brute_intersect": (c_int, [p, c_uint64, p, c_uint64, p, p]),
+        "orc_intersect_triangle": (c_int, [p, p, c_float, p]),
+        "orc_ao_bounce": (c_int, [p, p, p, p, c_uint64, p]),
+        "orc_pcg32_sequence": (c_uint32, [c_uint64, c_uint64, c_int, c_int64, p, c_uint64]),
+        "orc_murmur64a": (c_uint64, [p, c_uint64, c_uint64]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def morton_codes(tri: np.ndarray) -> np.ndarray:
+    out = np.empty(tri.shape[0], np.uint32)
+    load().orc_morton_codes(_p(tri), tri.shape[0], _p(out))
+    return out
+
+
+def morton_sort(tri: np.ndarray):
+    n = tri.shape[0]
+    codes, order = np.empty(n, np.uint32), np.empty(n, np.uint32)
+    load().orc_morton_sort(_p(tri), n, _p(codes), _p(order))
+    return codes, order
+
+
+class OracleBvh:
+    def __init__(self, tri: np.ndarray):
+        self.tri = np.ascontiguousarray(tri, np.float32)
+        self.lib = load()
+        self.h = c_void_p()
+        self.info = BuildInfo()
+        rc = self.lib.orc_bvh_build(_p(self.tri), self.tri.shape[0], ctypes.byref(self.h), ctypes.byref(self.info))
+        if rc != 0:
+            msg = self.lib.orc_bvh_error(self.h).decode()
+            self.lib.orc_bvh_free(self.h)
+            self.h = None
+            raise RuntimeError(f"oracle build failed ({rc}): {msg}")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_bvh_free(self.h)
+            self.h = None
+
+    def export_canonical(self) -> bytes:
+        ln = c_uint64(0)
+        self.lib.orc_bvh_export_canonical(self.h, None, ctypes.byref(ln))
+        buf = np.empty(ln.value, np.uint8)
+        self.lib.orc_bvh_export_canonical(self.h, _p(buf), ctypes.byref(ln))
+        return buf.tobytes()
+
+    def intersect(self, rays: np.ndarray, any_hit=False, threads=8):
+        n = rays.shape[0]
+        prim = np.empty(n, np.uint32)
+        t = np.empty(n, np.float32)
+        bary = np.empty((n, 3), np.float32)
+        stats = np.zeros(2, np.uint64)
+        rc = self.lib.orc_bvh_intersect(self.h, _p(rays), n, int(any_hit), threads, _p(prim), _p(t), _p(bary),
+                                        _p(stats))
+        if rc != 0:
+            raise RuntimeError("oracle traversal stack overflow")
+        return prim, t, bary, stats
+
+
+def brute_intersect(tri: np.ndarray, rays: np.ndarray):
+    n = rays.shape[0]
+    prim = np.empty(n, np.uint32)
+    t = np.empty(n, np.float32)
+    load().orc_brute_intersect(_p(tri), tri.shape[0], _p(rays), n, _p(prim), _p(t))
+    return prim, t
+
+
+def intersect_triangle(ray: np.ndarray, tri9: np.ndarray, t_max: float):
+    out = np.zeros(4, np.float32)
+    hit = load().orc_intersect_triangle(_p(np.ascontiguousarray(ray, np.float32)),
+                                        _p(np.ascontiguousarray(tri9, np.float32)), t_max, _p(out))
+    return bool(hit), out
+
+
+def ao_bounce(tri: np.ndarray, rays: np.ndarray, prim: np.ndarray, bary: np.ndarray) -> np.ndarray:
+    n = rays.shape[0]
+    out = np.empty((n, 8), np.float32)
+    load().orc_ao_bounce(_p(tri), _p(rays), _p(prim), _p(np.ascontiguousarray(bary, np.float32)), n, _p(out))
+    return out
+
+
+def pcg32(count: int, seq: int = 0, offset: int = 0, mode: int = 1, advance: int = 0) -> np.ndarray:
+    out = np.empty(count, np.uint32)
+    load().orc_pcg32_sequence(seq, offset, mode, advance, _p(out), count)
+    return out
+
+
+def murmur64a(data: bytes, seed: int = 0) -> int:
+    buf = np.frombuffer(data, np.uint8).copy() if data else np.zeros(1, np.uint8)
+    return int(load().orc_murmur64a(_p(buf), len(data), seed))

@@ -1,0 +1,514 @@
+// C ABI of libpbrtm.so (include/pbrtm.h): contexts, aggregate handles, error
+// reporting, host<->device staging and the deterministic workload generators.
+#include "../../include/pbrtm.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "hlbvh_build.h"
+#include "pbrtm_geometry.h"
+#include "traverse.h"
+
+using namespace pbrtm;
+
+struct pbrtm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    BuildScratch scratch;
+    DevBuf stage_in, stage_a, stage_b, stage_c, dev_err;
+    hipEvent_t ev[8] = {};
+    std::string error;
+};
+
+struct pbrtm_bvh {
+    pbrtm_ctx *ctx = nullptr;
+    uint32_t n = 0;
+    BvhDevice dev;
+    DevBuf tri_owned, tri_leaf;
+    const float *tri = nullptr;  // device, input order (owned or caller's)
+    float bounds[6];
+};
+
+namespace pbrtm {
+
+hipError_t DevBuf::reserve(size_t need) {
+    if (need <= bytes && ptr) return hipSuccess;
+    release();
+    size_t sz = std::max<size_t>(need, 256);
+    hipError_t e = hipMalloc(&ptr, sz);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        return e;
+    }
+    bytes = sz;
+    return hipSuccess;
+}
+
+void DevBuf::release() {
+    if (ptr) hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+std::string device_error_text(uint32_t bits) {
+    std::string s;
+    auto add = [&](uint32_t b, const char *t) {
+        if (bits & b) {
+            if (!s.empty()) s += "; ";
+            s += t;
+        }
+    };
+    add(kErrNonFinite, "non-finite vertex coordinate or vertex index out of range");
+    add(kErrDegenerateCentroids, "treelet centroids coincide on the split axis (hlbvh.cu:670-674)");
+    add(kErrSpin, "radix sort look-back did not converge");
+    add(kErrStack, "traversal stack overflow");
+    add(kErrSplit, "SAH split left one side empty");
+    add(kErrCapacity, "build work list overflow");
+    return s;
+}
+
+}  // namespace pbrtm
+
+namespace {
+
+int fail(pbrtm_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->error = msg;
+    return code;
+}
+
+int hip_fail(pbrtm_ctx *ctx, hipError_t e, const char *where) {
+    return fail(ctx, e == hipErrorOutOfMemory ? PBRTM_ERR_NOMEM : PBRTM_ERR_HIP,
+                std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define CK(ctx, expr, where)                           \
+    do {                                               \
+        hipError_t _e = (expr);                        \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, where); \
+    } while (0)
+
+// Host pointer -> device (staged in `buf`), or the device pointer as is.
+hipError_t stage_in(pbrtm_ctx *ctx, DevBuf &buf, const void *p, size_t bytes, uint32_t flags, const void **out) {
+    if (flags & PBRTM_DEVICE_PTRS) {
+        *out = p;
+        return hipSuccess;
+    }
+    hipError_t e = buf.reserve(bytes);
+    if (e != hipSuccess) return e;
+    if (bytes) e = hipMemcpyAsync(buf.ptr, p, bytes, hipMemcpyHostToDevice, ctx->stream);
+    *out = buf.ptr;
+    return e;
+}
+
+double event_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return ms;
+}
+
+uint32_t canon_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u == 0x80000000u ? 0u : u;
+}
+
+void put(std::vector<uint8_t> &out, uint32_t v) {
+    uint8_t b[4];
+    std::memcpy(b, &v, 4);
+    out.insert(out.end(), b, b + 4);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pbrtm_device_count(int *count) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return PBRTM_OK;
+}
+
+int pbrtm_create(pbrtm_ctx **out, int device) {
+    if (!out) return PBRTM_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return PBRTM_ERR_UNSUPPORTED;
+    if (device < 0 || device >= count) return PBRTM_ERR_INVALID;
+    auto *ctx = new (std::nothrow) pbrtm_ctx();
+    if (!ctx) return PBRTM_ERR_NOMEM;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return PBRTM_ERR_HIP;
+    }
+    for (auto &e : ctx->ev) hipEventCreate(&e);
+    *out = ctx;
+    return PBRTM_OK;
+}
+
+void pbrtm_destroy(pbrtm_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    ctx->scratch.release();
+    ctx->stage_in.release();
+    ctx->stage_a.release();
+    ctx->stage_b.release();
+    ctx->stage_c.release();
+    ctx->dev_err.release();
+    for (auto &e : ctx->ev) hipEventDestroy(e);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *pbrtm_last_error(const pbrtm_ctx *ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+void *pbrtm_stream(pbrtm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int pbrtm_morton_sort(pbrtm_ctx *ctx, const float *tri, uint64_t n, uint32_t flags, uint32_t *codes_out,
+                      uint32_t *order_out) {
+    if (!ctx) return PBRTM_ERR_INVALID;
+    if (n >= (1ull << 29)) return fail(ctx, PBRTM_ERR_INVALID, "at most 2^29 - 1 triangles");
+    if (n == 0) return PBRTM_OK;
+    CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    const void *d_tri = nullptr;
+    CK(ctx, stage_in(ctx, ctx->stage_in, tri, n * 36, flags, &d_tri), "upload triangles");
+    uint32_t *K = nullptr, *V = nullptr;
+    CK(ctx, morton_sort_device((const float *)d_tri, uint32_t(n), ctx->scratch, &K, &V, ctx->stream), "morton sort");
+    const hipMemcpyKind kind = (flags & PBRTM_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    CK(ctx, hipMemcpyAsync(codes_out, K, n * 4, kind, ctx->stream), "copy codes");
+    CK(ctx, hipMemcpyAsync(order_out, V, n * 4, kind, ctx->stream), "copy order");
+    uint32_t err = 0;
+    CK(ctx, hipMemcpyAsync(&err, ctx->scratch.counters.as<uint32_t>() + kCntErr, 4, hipMemcpyDeviceToHost,
+                           ctx->stream), "copy error");
+    CK(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (err) return fail(ctx, PBRTM_ERR_DEVICE, device_error_text(err));
+    return PBRTM_OK;
+}
+
+int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint32_t *idx, uint64_t nt,
+                    uint32_t flags, pbrtm_bvh **out, pbrtm_build_stats *stats) {
+    if (!ctx || !out) return PBRTM_ERR_INVALID;
+    *out = nullptr;
+    if (nt >= (1ull << 29)) return fail(ctx, PBRTM_ERR_INVALID, "at most 2^29 - 1 triangles");
+    if (!idx && nv != 3 * nt) return fail(ctx, PBRTM_ERR_INVALID, "a soup needs nv == 3 * nt");
+    if (idx && nv >= (1ull << 32)) return fail(ctx, PBRTM_ERR_INVALID, "too many vertices");
+    CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    const auto t0 = std::chrono::steady_clock::now();
+    auto *bvh = new (std::nothrow) pbrtm_bvh();
+    if (!bvh) return fail(ctx, PBRTM_ERR_NOMEM, "handle");
+    bvh->ctx = ctx;
+    bvh->n = uint32_t(nt);
+    const float mx = 3.402823466e+38f;
+    const float empty[6] = {mx, mx, mx, -mx, -mx, -mx};
+    std::memcpy(bvh->bounds, empty, sizeof(empty));
+    auto bail = [&](int code) {
+        pbrtm_bvh_free(bvh);
+        return code;
+    };
+    if (nt > 0) {
+        if (idx) {
+            const void *dv = nullptr, *di = nullptr;
+            hipError_t e = stage_in(ctx, ctx->stage_in, verts, nv * 12, flags, &dv);
+            if (e == hipSuccess) e = stage_in(ctx, ctx->stage_a, idx, nt * 12, flags, &di);
+            if (e == hipSuccess) e = bvh->tri_owned.reserve(nt * 36);
+            if (e == hipSuccess) e = ctx->dev_err.reserve(4);
+            if (e == hipSuccess) e = hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream);
+            if (e == hipSuccess)
+                e = deindex_triangles((const float *)dv, (const uint32_t *)di, uint32_t(nt), uint32_t(nv),
+                                      bvh->tri_owned.as<float>(), ctx->dev_err.as<uint32_t>(), ctx->stream);
+            uint32_t derr = 0;
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(&derr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) return bail(hip_fail(ctx, e, "mesh upload"));
+            if (derr) return bail(fail(ctx, PBRTM_ERR_INVALID, "vertex index out of range"));
+            bvh->tri = bvh->tri_owned.as<float>();
+        } else if (flags & PBRTM_DEVICE_PTRS) {
+            bvh->tri = verts;
+        } else {
+            hipError_t e = bvh->tri_owned.reserve(nt * 36);
+            if (e == hipSuccess) e = hipMemcpyAsync(bvh->tri_owned.ptr, verts, nt * 36, hipMemcpyHostToDevice, ctx->stream);
+            if (e != hipSuccess) return bail(hip_fail(ctx, e, "triangle upload"));
+            bvh->tri = bvh->tri_owned.as<float>();
+        }
+        BuildPhaseTimes times;
+        for (int k = 0; k < 6; ++k) times.ev[k] = ctx->ev[k];
+        uint32_t derr = 0;
+        hipError_t e = build_hlbvh_device(bvh->tri, uint32_t(nt), ctx->scratch, bvh->dev, &derr, ctx->stream, &times);
+        if (e != hipSuccess) return bail(hip_fail(ctx, e, "build"));
+        if (derr) return bail(fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr)));
+        e = bvh->tri_leaf.reserve(nt * 36);
+        if (e == hipSuccess)
+            e = make_leaf_triangles(bvh->tri, bvh->dev.prim_order.as<uint32_t>(), uint32_t(nt),
+                                    bvh->tri_leaf.as<float>(), ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev[6], ctx->stream);
+        BNode root;
+        if (e == hipSuccess) e = hipMemcpyAsync(&root, bvh->dev.nodes.ptr, sizeof(BNode), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return bail(hip_fail(ctx, e, "layout"));
+        for (int k = 0; k < 3; ++k) {
+            bvh->bounds[k] = root.lo[k];
+            bvh->bounds[3 + k] = root.hi[k];
+        }
+        if (stats) {
+            stats->ms_sort = event_ms(ctx->ev[0], ctx->ev[1]);
+            stats->ms_treelets = event_ms(ctx->ev[1], ctx->ev[2]);
+            stats->ms_top = event_ms(ctx->ev[2], ctx->ev[3]);
+            stats->ms_bottom = event_ms(ctx->ev[3], ctx->ev[4]);
+            stats->ms_layout = event_ms(ctx->ev[5], ctx->ev[6]);
+        }
+    }
+    if (stats) {
+        if (nt == 0) {
+            stats->ms_sort = stats->ms_treelets = stats->ms_top = stats->ms_bottom = stats->ms_layout = 0.0;
+        }
+        stats->num_prims = nt;
+        stats->num_treelets = bvh->dev.num_treelets;
+        stats->top_levels = bvh->dev.top_levels;
+        stats->bottom_levels = bvh->dev.bottom_levels;
+        stats->small_top = bvh->dev.small_top;
+        stats->small_bottom = bvh->dev.small_bottom;
+        stats->ms_total =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    *out = bvh;
+    return PBRTM_OK;
+}
+
+void pbrtm_bvh_free(pbrtm_bvh *bvh) {
+    if (!bvh) return;
+    if (bvh->ctx) {
+        hipSetDevice(bvh->ctx->device);
+        hipStreamSynchronize(bvh->ctx->stream);
+    }
+    bvh->dev.nodes.release();
+    bvh->dev.prim_order.release();
+    bvh->tri_owned.release();
+    bvh->tri_leaf.release();
+    delete bvh;
+}
+
+int pbrtm_bvh_bounds(const pbrtm_bvh *bvh, float out6[6]) {
+    if (!bvh || !out6) return PBRTM_ERR_INVALID;
+    std::memcpy(out6, bvh->bounds, sizeof(bvh->bounds));
+    return PBRTM_OK;
+}
+
+int pbrtm_bvh_export_canonical(const pbrtm_bvh *bvh, uint8_t *buf, uint64_t *len) {
+    if (!bvh || !len) return PBRTM_ERR_INVALID;
+    pbrtm_ctx *ctx = bvh->ctx;
+    std::vector<uint8_t> out;
+    const uint32_t n = bvh->n;
+    if (n > 0) {
+        CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+        std::vector<BNode> nodes(size_t(2) * n - 1);
+        std::vector<uint32_t> order(n);
+        CK(ctx, hipMemcpyAsync(nodes.data(), bvh->dev.nodes.ptr, nodes.size() * sizeof(BNode),
+                               hipMemcpyDeviceToHost, ctx->stream), "copy nodes");
+        CK(ctx, hipMemcpyAsync(order.data(), bvh->dev.prim_order.ptr, size_t(n) * 4, hipMemcpyDeviceToHost,
+                               ctx->stream), "copy order");
+        CK(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        struct Frame {
+            uint32_t id, prims;
+        };
+        std::vector<Frame> stack{{0u, n}};
+        std::vector<uint32_t> ids;
+        while (!stack.empty()) {
+            Frame f = stack.back();
+            stack.pop_back();
+            if (f.id >= nodes.size()) return fail(ctx, PBRTM_ERR_DEVICE, "node id out of range");
+            const BNode &nd = nodes[f.id];
+            const bool leaf = (nd.a & kLeafFlag) != 0;
+            put(out, leaf ? 1u : ((nd.b >> 30) << 8));
+            for (int k = 0; k < 3; ++k) put(out, canon_bits(nd.lo[k]));
+            for (int k = 0; k < 3; ++k) put(out, canon_bits(nd.hi[k]));
+            put(out, f.prims);
+            if (leaf) {
+                const uint32_t first = nd.a & ~kLeafFlag;
+                if (nd.b != f.prims || uint64_t(first) + nd.b > n)
+                    return fail(ctx, PBRTM_ERR_DEVICE, "leaf range inconsistent");
+                ids.assign(order.begin() + first, order.begin() + first + nd.b);
+                std::sort(ids.begin(), ids.end());
+                for (uint32_t v : ids) put(out, v);
+                continue;
+            }
+            const uint32_t right = nd.b & 0x3FFFFFFFu;
+            const uint32_t lp = (right - f.id) / 2;
+            if (nd.a != f.id + 1 || lp == 0 || lp >= f.prims)
+                return fail(ctx, PBRTM_ERR_DEVICE, "interior node inconsistent");
+            stack.push_back({right, f.prims - lp});
+            stack.push_back({nd.a, lp});
+        }
+    }
+    if (!buf) {
+        *len = out.size();
+        return PBRTM_OK;
+    }
+    if (*len < out.size()) return fail(ctx, PBRTM_ERR_INVALID, "buffer too small");
+    std::memcpy(buf, out.data(), out.size());
+    *len = out.size();
+    return PBRTM_OK;
+}
+
+int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_hit, uint32_t flags,
+                        uint32_t *prim_out, float *t_out, float *bary_out) {
+    if (!bvh || !prim_out) return PBRTM_ERR_INVALID;
+    pbrtm_ctx *ctx = bvh->ctx;
+    if (n >= (1ull << 32)) return fail(ctx, PBRTM_ERR_INVALID, "too many rays");
+    if (n == 0) return PBRTM_OK;
+    CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    const bool dev = flags & PBRTM_DEVICE_PTRS;
+    const void *d_rays = nullptr;
+    CK(ctx, stage_in(ctx, ctx->stage_in, rays, n * 32, flags, &d_rays), "upload rays");
+    uint32_t *d_prim = prim_out;
+    float *d_t = t_out, *d_b = bary_out;
+    if (!dev) {
+        CK(ctx, ctx->stage_a.reserve(n * 4), "alloc");
+        CK(ctx, ctx->stage_b.reserve(n * 4), "alloc");
+        CK(ctx, ctx->stage_c.reserve(n * 12), "alloc");
+        d_prim = ctx->stage_a.as<uint32_t>();
+        d_t = t_out ? ctx->stage_b.as<float>() : nullptr;
+        d_b = bary_out ? ctx->stage_c.as<float>() : nullptr;
+    }
+    CK(ctx, ctx->dev_err.reserve(4), "alloc");
+    CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream), "memset");
+    if (bvh->n == 0) {
+        CK(ctx, hipMemsetAsync(d_prim, 0xFF, n * 4, ctx->stream), "memset");
+        if (d_t) CK(ctx, hipMemsetAsync(d_t, 0, n * 4, ctx->stream), "memset");
+        if (d_b) CK(ctx, hipMemsetAsync(d_b, 0, n * 12, ctx->stream), "memset");
+    } else {
+        BvhTrace B{bvh->dev.nodes.as<BNode>(), bvh->tri_leaf.as<float>(), bvh->dev.prim_order.as<uint32_t>(), bvh->n};
+        CK(ctx, trace_binary(B, (const float *)d_rays, uint32_t(n), any_hit, d_prim, d_t, d_b,
+                             ctx->dev_err.as<uint32_t>(), ctx->stream), "trace");
+    }
+    if (!dev) {
+        CK(ctx, hipMemcpyAsync(prim_out, d_prim, n * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+        if (t_out) CK(ctx, hipMemcpyAsync(t_out, d_t, n * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+        if (bary_out) CK(ctx, hipMemcpyAsync(bary_out, d_b, n * 12, hipMemcpyDeviceToHost, ctx->stream), "download");
+    }
+    uint32_t derr = 0;
+    CK(ctx, hipMemcpyAsync(&derr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+    CK(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (derr) return fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr));
+    return PBRTM_OK;
+}
+
+int pbrtm_ao_bounce(pbrtm_bvh *bvh, const float *rays, const uint32_t *prim, const float *bary, uint64_t n,
+                    uint32_t flags, float *rays_out) {
+    if (!bvh || !rays || !prim || !bary || !rays_out) return PBRTM_ERR_INVALID;
+    pbrtm_ctx *ctx = bvh->ctx;
+    if (n >= (1ull << 32)) return fail(ctx, PBRTM_ERR_INVALID, "too many rays");
+    if (n == 0) return PBRTM_OK;
+    CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    for (uint64_t i = 0; !(flags & PBRTM_DEVICE_PTRS) && i < n; ++i)
+        if (prim[i] != PBRTM_NO_HIT && prim[i] >= bvh->n) return fail(ctx, PBRTM_ERR_INVALID, "prim id out of range");
+    const void *d_rays = nullptr, *d_prim = nullptr, *d_bary = nullptr;
+    CK(ctx, stage_in(ctx, ctx->stage_in, rays, n * 32, flags, &d_rays), "upload");
+    CK(ctx, stage_in(ctx, ctx->stage_a, prim, n * 4, flags, &d_prim), "upload");
+    CK(ctx, stage_in(ctx, ctx->stage_b, bary, n * 12, flags, &d_bary), "upload");
+    float *d_out = rays_out;
+    if (!(flags & PBRTM_DEVICE_PTRS)) {
+        CK(ctx, ctx->stage_c.reserve(n * 32), "alloc");
+        d_out = ctx->stage_c.as<float>();
+    }
+    CK(ctx, ao_bounce(bvh->tri, bvh->n, (const float *)d_rays, (const uint32_t *)d_prim, (const float *)d_bary,
+                      uint32_t(n), d_out, ctx->stream), "bounce");
+    if (!(flags & PBRTM_DEVICE_PTRS))
+        CK(ctx, hipMemcpyAsync(rays_out, d_out, n * 32, hipMemcpyDeviceToHost, ctx->stream), "download");
+    CK(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    return PBRTM_OK;
+}
+
+int pbrtm_gen_soup(uint64_t seed, uint64_t n, float mu, float sigma, int threads, float *tri_out) {
+    if (!tri_out && n) return PBRTM_ERR_INVALID;
+    const Pcg base = pcg_seq(seed);
+    auto work = [&](uint64_t b, uint64_t e) {
+        const double two_pi = 6.283185307179586;
+        for (uint64_t i = b; i < e; ++i) {
+            // advance the stream to draw 16*i (Brown's log-time jump)
+            uint64_t cur_mult = 0x5851f42d4c957f2dULL, cur_plus = base.inc, acc_mult = 1u, acc_plus = 0u;
+            for (uint64_t delta = 16 * i; delta > 0; delta >>= 1) {
+                if (delta & 1) {
+                    acc_mult *= cur_mult;
+                    acc_plus = acc_plus * cur_mult + cur_plus;
+                }
+                cur_plus = (cur_mult + 1) * cur_plus;
+                cur_mult *= cur_mult;
+            }
+            Pcg r{acc_mult * base.state + acc_plus, base.inc};
+            const double cx = r.uniform(), cy = r.uniform(), cz = r.uniform();
+            float *t = tri_out + 9 * i;
+            for (int v = 0; v < 3; ++v) {
+                const double ua = r.uniform(), ub = r.uniform(), uc = r.uniform(), ud = r.uniform();
+                const double g = std::sqrt(-2.0 * std::log(1.0 - ua)) * std::cos(two_pi * ub);
+                const double len = std::exp(double(mu) + double(sigma) * g);
+                const double z = 1.0 - 2.0 * uc;
+                const double rr = std::sqrt(std::max(0.0, 1.0 - z * z));
+                const double phi = two_pi * ud;
+                t[3 * v] = float(cx + len * rr * std::cos(phi));
+                t[3 * v + 1] = float(cy + len * rr * std::sin(phi));
+                t[3 * v + 2] = float(cz + len * z);
+            }
+        }
+    };
+    threads = std::max(1, std::min(threads, 64));
+    if (threads == 1 || n < 65536) {
+        work(0, n);
+        return PBRTM_OK;
+    }
+    std::vector<std::thread> pool;
+    const uint64_t chunk = (n + threads - 1) / threads;
+    for (int k = 0; k < threads; ++k) {
+        const uint64_t b = std::min(n, k * chunk), e = std::min(n, b + chunk);
+        pool.emplace_back(work, b, e);
+    }
+    for (auto &t : pool) t.join();
+    return PBRTM_OK;
+}
+
+int pbrtm_gen_camera_rays(uint32_t width, uint32_t height, const float eye[3], const float look[3],
+                          const float up[3], float fov_deg, float *rays_out) {
+    if (!eye || !look || !up || !rays_out || width == 0 || height == 0) return PBRTM_ERR_INVALID;
+    auto norm = [](double v[3]) {
+        const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        v[0] /= l; v[1] /= l; v[2] /= l;
+    };
+    auto crs = [](const double a[3], const double b[3], double o[3]) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    double f[3] = {double(look[0]) - eye[0], double(look[1]) - eye[1], double(look[2]) - eye[2]};
+    norm(f);
+    double u0[3] = {up[0], up[1], up[2]}, r[3], u[3];
+    crs(u0, f, r);
+    norm(r);
+    crs(f, r, u);
+    const double th = std::tan(double(fov_deg) * 3.141592653589793 / 360.0);
+    const double aspect = double(width) / double(height);
+    for (uint32_t y = 0; y < height; ++y) {
+        for (uint32_t x = 0; x < width; ++x) {
+            const double sx = (2.0 * (x + 0.5) / width - 1.0) * th * aspect;
+            const double sy = (1.0 - 2.0 * (y + 0.5) / height) * th;
+            double d[3] = {f[0] + sx * r[0] + sy * u[0], f[1] + sx * r[1] + sy * u[1], f[2] + sx * r[2] + sy * u[2]};
+            norm(d);
+            float *o = rays_out + 8 * (uint64_t(y) * width + x);
+            o[0] = eye[0]; o[1] = eye[1]; o[2] = eye[2];
+            o[3] = INFINITY;
+            o[4] = float(d[0]); o[5] = float(d[1]); o[6] = float(d[2]);
+            o[7] = 0.f;
+        }
+    }
+    return PBRTM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,875 @@
+// HLBVH construction on gfx950, reproducing the tree of accelerator/hlbvh.cu
+// (reference): 30-bit Morton codes over cube-adjusted centroid bounds
+// (hlbvh.cu:127-141, :428-446), treelets = runs of equal code >> 9
+// (hlbvh.cu:36-42, :504-543), a 24-bucket SAH top level over treelets
+// (hlbvh.cu:649-826) and a midpoint bottom level inside each treelet
+// (hlbvh.cu:291-382, :828-857).
+//
+// Layout and scheduling are MI355X-native, not the reference's:
+//  * one fused pass computes the Morton code and all three radix histograms;
+//  * a stable onesweep radix sort orders (code, index), which refines the
+//    reference's treelet-only counting sort into a canonical order;
+//  * every split, top or bottom, runs on 32-byte records in two ping-pong
+//    buffers. Nodes with more than 64 elements split level by level with one
+//    workgroup per 2048-element chunk (atomic min/max of order-mapped floats,
+//    chunked stable partition); nodes with at most 64 elements finish their
+//    whole subtree inside one wave (LDS records, ballot partition);
+//  * node ids are preassigned in preorder from primitive counts (a node with P
+//    primitives owns 2P-1 ids: left = id + 1, right = id + 2*P_left), so the
+//    output does not depend on the order in which work items run.
+#include "hlbvh_build.h"
+#include "wave_ops.h"
+
+namespace pbrtm {
+
+namespace {
+
+constexpr int kChunk = 2048;       // elements per workgroup task
+constexpr int kChunkBlock = 256;
+constexpr int kChunkItems = kChunk / kChunkBlock;
+constexpr int kAccWords = 192;     // per-item accumulator (see Acc below)
+
+// Per large work item: centroid box, full box, then 24 buckets of (count, box),
+// floats stored order-mapped (f2ord) so integer atomics give exact min/max.
+struct Acc {
+    uint32_t cb[6];
+    uint32_t fb[6];
+    uint32_t bcnt[kNumBuckets];
+    uint32_t bbox[kNumBuckets][6];
+    uint32_t pad[kAccWords - 12 - kNumBuckets * 7];
+};
+static_assert(sizeof(Acc) == kAccWords * 4, "Acc layout");
+
+struct Split {
+    uint32_t axis;
+    uint32_t bucket;     // top: last bucket on the left side
+    float split_val;     // midpoint predicate value (bottom, and top fallback)
+    uint32_t mode;       // 0 = SAH bucket predicate, 1 = midpoint predicate
+    float base, span;    // top: bucket mapping
+    uint32_t left_prims; // top: primitives on the left side
+    uint32_t pad;
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ Rec *pick(Rec *const (&pair)[2], uint32_t b) { return b ? pair[1] : pair[0]; }
+
+__device__ __forceinline__ Box acc_box(const uint32_t *w) {
+    return {{ord2f(w[0]), ord2f(w[1]), ord2f(w[2])}, {ord2f(w[3]), ord2f(w[4]), ord2f(w[5])}};
+}
+
+// 24 * ((c - base) / span), truncated, 24 folded to 23 (hlbvh.cu:693-701).
+__device__ __forceinline__ uint32_t bucket_of(float c, float base, float span) {
+    uint32_t b = f2u_sat(float(kNumBuckets) * ((c - base) / span));
+    return b >= kNumBuckets ? kNumBuckets - 1 : b;
+}
+
+__device__ __forceinline__ Rec make_rec(Box b, uint32_t a, uint32_t cnt) {
+    Rec r;
+    r.lo[0] = b.lo.x; r.lo[1] = b.lo.y; r.lo[2] = b.lo.z;
+    r.hi[0] = b.hi.x; r.hi[1] = b.hi.y; r.hi[2] = b.hi.z;
+    r.a = a;
+    r.b = cnt;
+    return r;
+}
+
+__device__ __forceinline__ void write_leaf(BNode *nodes, uint32_t id, Box b, uint32_t first, uint32_t count) {
+    BNode nd;
+    nd.lo[0] = b.lo.x; nd.lo[1] = b.lo.y; nd.lo[2] = b.lo.z;
+    nd.hi[0] = b.hi.x; nd.hi[1] = b.hi.y; nd.hi[2] = b.hi.z;
+    nd.a = kLeafFlag | first;
+    nd.b = count;
+    nodes[id] = nd;
+}
+
+__device__ __forceinline__ void write_interior(BNode *nodes, uint32_t id, Box b, uint32_t left_prims,
+                                               uint32_t axis) {
+    BNode nd;
+    nd.lo[0] = b.lo.x; nd.lo[1] = b.lo.y; nd.lo[2] = b.lo.z;
+    nd.hi[0] = b.hi.x; nd.hi[1] = b.hi.y; nd.hi[2] = b.hi.z;
+    nd.a = id + 1;
+    nd.b = (id + 2 * left_prims) | (axis << 30);
+    nodes[id] = nd;
+}
+
+__device__ __forceinline__ void push_item(WorkItem *list, uint32_t *counter, uint32_t cap, WorkItem it,
+                                          uint32_t *err) {
+    uint32_t slot = atomicAdd(counter, 1u);
+    if (slot < cap) list[slot] = it;
+    else atomicOr(err, uint32_t(kErrCapacity));
+}
+
+// A treelet that became a top-level leaf is the root of a bottom-level build
+// over its primitive range (hlbvh.cu:565-607 expands such leaves).
+__device__ void emit_bottom_root(const BuildArgs &A, const Rec &treelet, uint32_t node) {
+    const uint32_t first = treelet.a, count = treelet.b;
+    if (count == 1) {
+        const Rec p = A.prec[0][first];
+        write_leaf(A.nodes, node, rec_box(p), first, 1);
+        A.prim_order[first] = p.a;
+        return;
+    }
+    WorkItem it{first, first + count, node, count, 0, {0, 0, 0}};
+    if (count <= kSmallNode) push_item(A.small_bot, A.counters + kCntSmallBot, A.cap_small_bot, it, A.err);
+    else push_item(A.large_bot[0], A.counters + kCntLargeBot0, A.cap_large_bot, it, A.err);
+}
+
+// ---------------------------------------------------------------- Morton pass
+
+__global__ __launch_bounds__(256) void centroid_bounds_kernel(const float *__restrict__ tri, uint32_t n,
+                                                              uint32_t *__restrict__ cbounds,
+                                                              uint32_t *__restrict__ err) {
+    uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+    bool bad = false;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float *t = tri + 9ull * i;
+        f3 p0{t[0], t[1], t[2]}, p1{t[3], t[4], t[5]}, p2{t[6], t[7], t[8]};
+#pragma unroll
+        for (int k = 0; k < 9; ++k) bad |= !isfinite(t[k]);
+        f3 c = box_centroid(tri_box(p0, p1, p2));
+        lo[0] = min(lo[0], f2ord(c.x)); lo[1] = min(lo[1], f2ord(c.y)); lo[2] = min(lo[2], f2ord(c.z));
+        hi[0] = max(hi[0], f2ord(c.x)); hi[1] = max(hi[1], f2ord(c.y)); hi[2] = max(hi[2], f2ord(c.z));
+    }
+    __shared__ uint32_t s[4][6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = wave_min_u32(lo[k]);
+        hi[k] = wave_max_u32(hi[k]);
+    }
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0) {
+        for (int k = 0; k < 3; ++k) { s[w][k] = lo[k]; s[w][3 + k] = hi[k]; }
+    }
+    if (__ballot(bad) && __lane_id() == 0) atomicOr(err, uint32_t(kErrNonFinite));
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        uint32_t v = s[0][threadIdx.x];
+        for (int ww = 1; ww < 4; ++ww)
+            v = threadIdx.x < 3 ? min(v, s[ww][threadIdx.x]) : max(v, s[ww][threadIdx.x]);
+        if (threadIdx.x < 3) atomicMin(&cbounds[threadIdx.x], v);
+        else atomicMax(&cbounds[threadIdx.x], v);
+    }
+}
+
+// Cube-adjusted centroid bounds (hlbvh.cu:428-446), identical in every thread.
+__device__ __forceinline__ Box morton_frame(const uint32_t *cbounds) {
+    Box cb = acc_box(cbounds);
+    const int md = box_max_dim(cb);
+    const float radius = (sel3(cb.hi, md) - sel3(cb.lo, md)) / 2;
+    const f3 c = box_centroid(cb);
+    f3 lo = cb.lo, hi = cb.hi;
+    if (md != 0) { lo.x = c.x - radius; hi.x = c.x + radius; }
+    if (md != 1) { lo.y = c.y - radius; hi.y = c.y + radius; }
+    if (md != 2) { lo.z = c.z - radius; hi.z = c.z + radius; }
+    return {min3(lo, hi), max3(lo, hi)};
+}
+
+__device__ __forceinline__ uint32_t morton_of(Box frame, f3 p) {
+    f3 o = sub3(p, frame.lo);
+    if (frame.hi.x > frame.lo.x) o.x /= (frame.hi.x - frame.lo.x);
+    if (frame.hi.y > frame.lo.y) o.y /= (frame.hi.y - frame.lo.y);
+    if (frame.hi.z > frame.lo.z) o.z /= (frame.hi.z - frame.lo.z);
+    o = mul3(1024.0f, o);
+    return morton3(f2u_sat(o.x), f2u_sat(o.y), f2u_sat(o.z));
+}
+
+// Morton code per primitive + the three 10-bit digit histograms of the sort.
+__global__ __launch_bounds__(256) void morton_kernel(const float *__restrict__ tri, uint32_t n,
+                                                     const uint32_t *__restrict__ cbounds,
+                                                     uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                     uint32_t *__restrict__ hist) {
+    __shared__ uint32_t s_hist[kSortPasses * kSortRadix];
+    for (int i = threadIdx.x; i < int(kSortPasses * kSortRadix); i += blockDim.x) s_hist[i] = 0u;
+    __syncthreads();
+    const Box frame = morton_frame(cbounds);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float *t = tri + 9ull * i;
+        f3 p0{t[0], t[1], t[2]}, p1{t[3], t[4], t[5]}, p2{t[6], t[7], t[8]};
+        const uint32_t code = morton_of(frame, box_centroid(tri_box(p0, p1, p2)));
+        keys[i] = code;
+        vals[i] = i;
+        atomicAdd(&s_hist[code & 1023u], 1u);
+        atomicAdd(&s_hist[kSortRadix + ((code >> 10) & 1023u)], 1u);
+        atomicAdd(&s_hist[2 * kSortRadix + ((code >> 20) & 1023u)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < int(kSortPasses * kSortRadix); i += blockDim.x)
+        if (s_hist[i]) atomicAdd(&hist[i], s_hist[i]);
+}
+
+// Sorted position -> primitive record (box of the triangle, id), and the
+// treelet-start flag (code >> 9 differs from the previous position).
+__global__ __launch_bounds__(256) void gather_kernel(const float *__restrict__ tri, uint32_t n,
+                                                     const uint32_t *__restrict__ keys,
+                                                     const uint32_t *__restrict__ vals, Rec *__restrict__ prec,
+                                                     uint32_t *__restrict__ flags) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t pid = vals[i] < n ? vals[i] : 0u;  // in range unless the sort failed (reported)
+    const float *t = tri + 9ull * pid;
+    f3 p0{t[0], t[1], t[2]}, p1{t[3], t[4], t[5]}, p2{t[6], t[7], t[8]};
+    prec[i] = make_rec(tri_box(p0, p1, p2), pid, 1u);
+    const uint32_t key = keys[i] >> kTreeletShift;
+    flags[i] = (i == 0 || (keys[i - 1] >> kTreeletShift) != key) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void treelet_start_kernel(const uint32_t *__restrict__ flags,
+                                                            const uint32_t *__restrict__ tidx, uint32_t n,
+                                                            uint32_t *__restrict__ tstart) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flags[i]) tstart[tidx[i]] = i;
+}
+
+// Treelet box = union of member boxes (hlbvh.cu:82-99).
+__global__ __launch_bounds__(256) void treelet_rec_kernel(const uint32_t *__restrict__ tstart, uint32_t d,
+                                                          uint32_t n, const Rec *__restrict__ prec,
+                                                          Rec *__restrict__ trec) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d) return;
+    const uint32_t s = tstart[t], e = (t + 1 < d) ? tstart[t + 1] : n;
+    Box b = box_empty();
+    for (uint32_t i = s; i < e; ++i) b = box_union(b, rec_box(prec[i]));
+    trec[t] = make_rec(b, s, e - s);
+}
+
+__global__ void root_kernel(BuildArgs A, uint32_t num_treelets, uint32_t n) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (num_treelets == 1) {
+        emit_bottom_root(A, A.trec[0][0], 0);
+        return;
+    }
+    WorkItem it{0, num_treelets, 0, n, 0, {0, 0, 0}};
+    if (num_treelets <= kSmallNode) push_item(A.small_top, A.counters + kCntSmallTop, A.cap_small_top, it, A.err);
+    else push_item(A.large_top[0], A.counters + kCntLargeTop0, A.cap_large_top, it, A.err);
+}
+
+// ------------------------------------------------- level step for large nodes
+
+__global__ __launch_bounds__(256) void acc_init_kernel(Acc *acc, uint32_t nitems) {
+    const uint32_t i = blockIdx.x;
+    if (i >= nitems) return;
+    uint32_t *w = reinterpret_cast<uint32_t *>(&acc[i]);
+    for (int k = threadIdx.x; k < kAccWords; k += blockDim.x) {
+        uint32_t v = 0u;
+        if (k < 3 || (k >= 6 && k < 9)) v = 0xFFFFFFFFu;  // cb.lo, fb.lo
+        if (k >= 12 + int(kNumBuckets)) {
+            int j = (k - 12 - kNumBuckets) % 6;
+            v = j < 3 ? 0xFFFFFFFFu : 0u;
+        }
+        w[k] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void task_count_kernel(const WorkItem *items, uint32_t nitems, uint32_t *counts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nitems) counts[i] = (items[i].end - items[i].begin + kChunk - 1) / kChunk;
+}
+
+__global__ __launch_bounds__(256) void task_fill_kernel(const WorkItem *items, uint32_t nitems,
+                                                        const uint32_t *task_off, uint2 *tasks) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitems) return;
+    const uint32_t nc = (items[i].end - items[i].begin + kChunk - 1) / kChunk;
+    for (uint32_t c = 0; c < nc; ++c) tasks[task_off[i] + c] = make_uint2(i, c);
+}
+
+// Block reduce of 6 order-mapped mins and 6 maxes, merged into global atomics.
+__device__ __forceinline__ void block_merge_boxes(uint32_t v[12], uint32_t *dst) {
+    __shared__ uint32_t s[kChunkBlock / kWave][12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        const bool is_min = (k % 6) < 3;
+        v[k] = is_min ? wave_min_u32(v[k]) : wave_max_u32(v[k]);
+    }
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0)
+        for (int k = 0; k < 12; ++k) s[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        const bool is_min = (k % 6) < 3;
+        uint32_t r = s[0][k];
+        for (int ww = 1; ww < kChunkBlock / kWave; ++ww) r = is_min ? min(r, s[ww][k]) : max(r, s[ww][k]);
+        if (is_min) atomicMin(&dst[k], r);
+        else atomicMax(&dst[k], r);
+    }
+}
+
+__global__ __launch_bounds__(kChunkBlock) void chunk_reduce_kernel(const uint2 *tasks, const WorkItem *items,
+                                                                   Rec *const *recs, Acc *acc) {
+    const uint2 task = tasks[blockIdx.x];
+    const WorkItem it = items[task.x];
+    const Rec *R = recs[it.buf];
+    const uint32_t s = it.begin + task.y * kChunk, e = min(it.end, s + kChunk);
+    uint32_t v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
+    for (uint32_t i = s + threadIdx.x; i < e; i += kChunkBlock) {
+        const Box b = rec_box(R[i]);
+        const f3 c = box_centroid(b);
+        v[0] = min(v[0], f2ord(c.x)); v[1] = min(v[1], f2ord(c.y)); v[2] = min(v[2], f2ord(c.z));
+        v[3] = max(v[3], f2ord(c.x)); v[4] = max(v[4], f2ord(c.y)); v[5] = max(v[5], f2ord(c.z));
+        v[6] = min(v[6], f2ord(b.lo.x)); v[7] = min(v[7], f2ord(b.lo.y)); v[8] = min(v[8], f2ord(b.lo.z));
+        v[9] = max(v[9], f2ord(b.hi.x)); v[10] = max(v[10], f2ord(b.hi.y)); v[11] = max(v[11], f2ord(b.hi.z));
+    }
+    block_merge_boxes(v, acc[task.x].cb);
+}
+
+// SAH bucket statistics per chunk (hlbvh.cu:690-706).
+__global__ __launch_bounds__(kChunkBlock) void chunk_buckets_kernel(const uint2 *tasks, const WorkItem *items,
+                                                                    Rec *const *recs, Acc *acc, uint32_t *err) {
+    __shared__ uint32_t s_cnt[kNumBuckets];
+    __shared__ uint32_t s_box[kNumBuckets][6];
+    const uint2 task = tasks[blockIdx.x];
+    const WorkItem it = items[task.x];
+    const Rec *R = recs[it.buf];
+    for (int k = threadIdx.x; k < int(kNumBuckets * 7); k += kChunkBlock) {
+        if (k < int(kNumBuckets)) s_cnt[k] = 0u;
+        else {
+            int j = k - kNumBuckets;
+            (&s_box[0][0])[j] = (j % 6) < 3 ? 0xFFFFFFFFu : 0u;
+        }
+    }
+    const Box cb = acc_box(acc[task.x].cb);
+    const int axis = box_max_dim(cb);
+    const float base = sel3(cb.lo, axis);
+    const float span = sel3(cb.hi, axis) - sel3(cb.lo, axis);
+    if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) {
+        if (threadIdx.x == 0) atomicOr(err, uint32_t(kErrDegenerateCentroids));
+        return;
+    }
+    __syncthreads();
+    const uint32_t s = it.begin + task.y * kChunk, e = min(it.end, s + kChunk);
+    for (uint32_t i = s + threadIdx.x; i < e; i += kChunkBlock) {
+        const Rec r = R[i];
+        const Box b = rec_box(r);
+        const uint32_t k = bucket_of(sel3(box_centroid(b), axis), base, span);
+        atomicAdd(&s_cnt[k], r.b);
+        atomicMin(&s_box[k][0], f2ord(b.lo.x)); atomicMin(&s_box[k][1], f2ord(b.lo.y));
+        atomicMin(&s_box[k][2], f2ord(b.lo.z));
+        atomicMax(&s_box[k][3], f2ord(b.hi.x)); atomicMax(&s_box[k][4], f2ord(b.hi.y));
+        atomicMax(&s_box[k][5], f2ord(b.hi.z));
+    }
+    __syncthreads();
+    Acc *a = &acc[task.x];
+    for (int k = threadIdx.x; k < int(kNumBuckets * 7); k += kChunkBlock) {
+        if (k < int(kNumBuckets)) {
+            if (s_cnt[k]) atomicAdd(&a->bcnt[k], s_cnt[k]);
+        } else {
+            const int j = k - kNumBuckets;
+            const uint32_t v = (&s_box[0][0])[j];
+            if ((j % 6) < 3) { if (v != 0xFFFFFFFFu) atomicMin(&(&a->bbox[0][0])[j], v); }
+            else if (v != 0u) atomicMax(&(&a->bbox[0][0])[j], v);
+        }
+    }
+}
+
+// SAH cost of splitting after bucket s (hlbvh.cu:711-731); left/right unions of
+// bucket boxes are exact, so prefix and suffix scans give the same floats.
+__device__ float sah_cost(const Box *bb, const uint32_t *bc, int split, float total_sa) {
+    Box bl = box_empty(), br = box_empty();
+    uint32_t cl = 0, cr = 0;
+    for (int k = 0; k <= split; ++k) { bl = box_union(bl, bb[k]); cl += bc[k]; }
+    for (int k = split + 1; k < int(kNumBuckets); ++k) { br = box_union(br, bb[k]); cr += bc[k]; }
+    return 0.125 + (float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa;
+}
+
+template <bool TOP>
+__global__ __launch_bounds__(64) void decide_kernel(const WorkItem *items, uint32_t nitems, const Acc *acc,
+                                                    Split *splits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitems) return;
+    const Acc &a = acc[i];
+    const Box cb = acc_box(a.cb);
+    Split sp{};
+    sp.axis = box_max_dim(cb);
+    sp.split_val = sel3(box_centroid(cb), sp.axis);
+    sp.mode = 1;
+    if (TOP) {
+        const Box fb = acc_box(a.fb);
+        Box bb[kNumBuckets];
+        for (int k = 0; k < int(kNumBuckets); ++k) bb[k] = acc_box(a.bbox[k]);
+        const float total_sa = box_area(fb);
+        float best = sah_cost(bb, a.bcnt, 0, total_sa);
+        uint32_t split = 0;
+        for (int s = 1; s < int(kNumBuckets) - 1; ++s) {
+            const float c = sah_cost(bb, a.bcnt, s, total_sa);
+            if (c < best) { best = c; split = s; }
+        }
+        uint32_t lp = 0, rp = 0;
+        for (int k = 0; k < int(kNumBuckets); ++k) (k <= int(split) ? lp : rp) += a.bcnt[k];
+        sp.bucket = split;
+        sp.base = sel3(cb.lo, sp.axis);
+        sp.span = sel3(cb.hi, sp.axis) - sel3(cb.lo, sp.axis);
+        sp.left_prims = lp;
+        sp.mode = (lp == 0 || rp == 0) ? 1u : 0u;  // midpoint fallback (hlbvh.cu:758-777)
+    }
+    splits[i] = sp;
+}
+
+template <bool TOP>
+__device__ __forceinline__ bool goes_left(const Split &sp, const Box &b) {
+    const float c = sel3(box_centroid(b), sp.axis);
+    if (TOP && sp.mode == 0) return bucket_of(c, sp.base, sp.span) <= sp.bucket;
+    return c < sp.split_val;
+}
+
+template <bool TOP>
+__global__ __launch_bounds__(kChunkBlock) void chunk_count_kernel(const uint2 *tasks, const WorkItem *items,
+                                                                  Rec *const *recs, const Split *splits,
+                                                                  uint32_t *task_left, uint32_t *task_lprims) {
+    __shared__ uint32_t s_w[kChunkBlock / kWave];
+    __shared__ uint32_t s_p[kChunkBlock / kWave];
+    const uint2 task = tasks[blockIdx.x];
+    const WorkItem it = items[task.x];
+    const Split sp = splits[task.x];
+    const Rec *R = recs[it.buf];
+    const uint32_t s = it.begin + task.y * kChunk, e = min(it.end, s + kChunk);
+    uint32_t cnt = 0, prims = 0;
+    for (uint32_t i = s + threadIdx.x; i < e; i += kChunkBlock) {
+        const Rec r = R[i];
+        if (goes_left<TOP>(sp, rec_box(r))) { ++cnt; prims += r.b; }
+    }
+    cnt = wave_sum(cnt);
+    prims = wave_sum(prims);
+    if (__lane_id() == 0) { s_w[threadIdx.x / kWave] = cnt; s_p[threadIdx.x / kWave] = prims; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0, p = 0;
+        for (int w = 0; w < kChunkBlock / kWave; ++w) { t += s_w[w]; p += s_p[w]; }
+        task_left[blockIdx.x] = t;
+        task_lprims[blockIdx.x] = p;
+    }
+}
+
+// Stable partition of one chunk into the other buffer.
+template <bool TOP>
+__global__ __launch_bounds__(kChunkBlock) void chunk_scatter_kernel(const uint2 *tasks, const WorkItem *items,
+                                                                    Rec *const *recs, const Split *splits,
+                                                                    const uint32_t *task_off,
+                                                                    const uint32_t *left_off) {
+    __shared__ uint32_t s_waves[kChunkBlock / kWave];
+    const uint32_t t = blockIdx.x;
+    const uint2 task = tasks[t];
+    const WorkItem it = items[task.x];
+    const Split sp = splits[task.x];
+    const Rec *R = recs[it.buf];
+    Rec *W = recs[1 - it.buf];
+    const uint32_t first_task = task_off[task.x];
+    const uint32_t nchunks = (it.end - it.begin + kChunk - 1) / kChunk;
+    const uint32_t left_total = left_off[first_task + nchunks] - left_off[first_task];
+    const uint32_t left_before = left_off[t] - left_off[first_task];
+    const uint32_t s = it.begin + task.y * kChunk;
+    const uint32_t right_before = (s - it.begin) - left_before;
+    const uint32_t e = min(it.end, s + kChunk);
+    Rec r[kChunkItems];
+    uint32_t mask = 0, nleft = 0;
+#pragma unroll
+    for (int k = 0; k < kChunkItems; ++k) {
+        const uint32_t i = s + threadIdx.x * kChunkItems + k;
+        if (i < e) {
+            r[k] = R[i];
+            if (goes_left<TOP>(sp, rec_box(r[k]))) { mask |= 1u << k; ++nleft; }
+        }
+    }
+    const uint32_t nvalid = (s + threadIdx.x * kChunkItems >= e)
+                                ? 0u
+                                : min(uint32_t(kChunkItems), e - (s + threadIdx.x * kChunkItems));
+    uint32_t tot;
+    const uint32_t lpre = block_exclusive_add<kChunkBlock>(nleft, s_waves, &tot);
+    const uint32_t rpre = threadIdx.x * kChunkItems - lpre;  // elements before this thread minus lefts
+    uint32_t li = 0, ri = 0;
+#pragma unroll
+    for (int k = 0; k < kChunkItems; ++k) {
+        if (uint32_t(k) >= nvalid) break;
+        if (mask & (1u << k)) W[it.begin + left_before + lpre + li++] = r[k];
+        else W[it.begin + left_total + right_before + rpre + ri++] = r[k];
+    }
+}
+
+template <bool TOP>
+__global__ __launch_bounds__(64) void children_kernel(BuildArgs A, const WorkItem *items, uint32_t nitems,
+                                                      const Split *splits, const Acc *acc,
+                                                      const uint32_t *task_off, const uint32_t *left_off,
+                                                      const uint32_t *lprims_off, WorkItem *next_large,
+                                                      uint32_t *next_counter) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitems) return;
+    const WorkItem it = items[i];
+    const Split sp = splits[i];
+    const uint32_t size = it.end - it.begin;
+    const uint32_t first_task = task_off[i];
+    const uint32_t nchunks = (size + kChunk - 1) / kChunk;
+    const uint32_t L = left_off[first_task + nchunks] - left_off[first_task];
+    const Box fb = acc_box(acc[i].fb);
+    Rec *const(&recs)[2] = TOP ? A.trec : A.prec;
+    if (L == 0 || L == size) {
+        if (TOP) {  // unreachable for finite input: both SAH end buckets are occupied
+            atomicOr(A.err, uint32_t(kErrSplit));
+            return;
+        }
+        write_leaf(A.nodes, it.node, fb, it.begin, size);  // all centroids on one side
+        const Rec *R = pick(recs, it.buf);
+        for (uint32_t k = it.begin; k < it.end; ++k) A.prim_order[k] = R[k].a;
+        return;
+    }
+    const uint32_t lp = TOP ? (lprims_off[first_task + nchunks] - lprims_off[first_task]) : L;
+    write_interior(A.nodes, it.node, fb, lp, sp.axis);
+    const uint32_t nb = 1 - it.buf;
+    WorkItem ch[2] = {{it.begin, it.begin + L, it.node + 1, lp, nb, {0, 0, 0}},
+                      {it.begin + L, it.end, it.node + 2 * lp, it.prims - lp, nb, {0, 0, 0}}};
+    for (int c = 0; c < 2; ++c) {
+        const uint32_t cs = ch[c].end - ch[c].begin;
+        if (cs == 1) {
+            const Rec r = pick(recs, nb)[ch[c].begin];
+            if (TOP) emit_bottom_root(A, r, ch[c].node);
+            else {
+                write_leaf(A.nodes, ch[c].node, rec_box(r), ch[c].begin, 1);
+                A.prim_order[ch[c].begin] = r.a;
+            }
+        } else if (cs <= kSmallNode) {
+            if (TOP) push_item(A.small_top, A.counters + kCntSmallTop, A.cap_small_top, ch[c], A.err);
+            else push_item(A.small_bot, A.counters + kCntSmallBot, A.cap_small_bot, ch[c], A.err);
+        } else {
+            push_item(next_large, next_counter, TOP ? A.cap_large_top : A.cap_large_bot, ch[c], A.err);
+        }
+    }
+}
+
+// --------------------------------------------- wave-per-node subtree finisher
+
+constexpr int kFinBlock = 256;
+constexpr int kFinWaves = kFinBlock / kWave;
+
+template <bool TOP>
+__global__ __launch_bounds__(kFinBlock) void finish_kernel(BuildArgs A, const WorkItem *items, uint32_t nitems) {
+    __shared__ Rec s_rec[kFinWaves][kSmallNode];
+    __shared__ Rec s_tmp[kFinWaves][kSmallNode];
+    __shared__ uint4 s_stack[kFinWaves][kSmallNode];
+    __shared__ uint32_t s_bkt[kFinWaves][kSmallNode];
+    __shared__ float s_cost[kFinWaves][kNumBuckets];
+    const int w = threadIdx.x / kWave;
+    const uint32_t lane = __lane_id();
+    const uint32_t wid = blockIdx.x * kFinWaves + w;
+    if (wid >= nitems) return;
+    const WorkItem it = items[wid];
+    Rec *const(&recs)[2] = TOP ? A.trec : A.prec;
+    const uint32_t m0 = it.end - it.begin;
+    if (lane < m0) s_rec[w][lane] = pick(recs, it.buf)[it.begin + lane];
+    int sp = 0;
+    if (lane == 0) s_stack[w][0] = make_uint4(0, m0, it.node, it.prims);
+    sp = 1;
+    wave_lds_sync();
+    while (sp > 0) {
+        --sp;
+        const uint4 nd = s_stack[w][sp];
+        const uint32_t lo = nd.x, hi = nd.y, id = nd.z, P = nd.w, m = hi - lo;
+        const bool active = lane >= lo && lane < hi;
+        Rec r;
+        if (active) r = s_rec[w][lane];
+        const Box b = active ? rec_box(r) : box_empty();
+        const f3 c = box_centroid(b);
+        uint32_t v[12];
+        v[0] = active ? f2ord(c.x) : 0xFFFFFFFFu; v[1] = active ? f2ord(c.y) : 0xFFFFFFFFu;
+        v[2] = active ? f2ord(c.z) : 0xFFFFFFFFu;
+        v[3] = active ? f2ord(c.x) : 0u; v[4] = active ? f2ord(c.y) : 0u; v[5] = active ? f2ord(c.z) : 0u;
+        v[6] = active ? f2ord(b.lo.x) : 0xFFFFFFFFu; v[7] = active ? f2ord(b.lo.y) : 0xFFFFFFFFu;
+        v[8] = active ? f2ord(b.lo.z) : 0xFFFFFFFFu;
+        v[9] = active ? f2ord(b.hi.x) : 0u; v[10] = active ? f2ord(b.hi.y) : 0u; v[11] = active ? f2ord(b.hi.z) : 0u;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? wave_min_u32(v[k]) : wave_max_u32(v[k]);
+        const Box cb = acc_box(v);
+        const Box fb = acc_box(v + 6);
+        const int axis = box_max_dim(cb);
+        bool left;
+        uint32_t lp;
+        if (TOP) {
+            if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) {
+                if (lane == 0) atomicOr(A.err, uint32_t(kErrDegenerateCentroids));
+                continue;
+            }
+            const float base = sel3(cb.lo, axis);
+            const float span = sel3(cb.hi, axis) - sel3(cb.lo, axis);
+            const uint32_t bk = bucket_of(sel3(c, axis), base, span);
+            if (active) s_bkt[w][lane] = bk;
+            wave_lds_sync();
+            const float total_sa = box_area(fb);
+            if (lane < kNumBuckets - 1) {  // lane k prices the split after bucket k
+                Box bl = box_empty(), br = box_empty();
+                uint32_t cl = 0, cr = 0;
+                for (uint32_t j = lo; j < hi; ++j) {
+                    const Rec q = s_rec[w][j];
+                    if (s_bkt[w][j] <= lane) { bl = box_union(bl, rec_box(q)); cl += q.b; }
+                    else { br = box_union(br, rec_box(q)); cr += q.b; }
+                }
+                s_cost[w][lane] = 0.125 + (float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa;
+            }
+            wave_lds_sync();
+            uint32_t split = 0;
+            float best = s_cost[w][0];
+            for (int k = 1; k < int(kNumBuckets) - 1; ++k) {
+                const float ck = s_cost[w][k];
+                if (ck < best) { best = ck; split = k; }
+            }
+            left = active && bk <= split;
+            const uint64_t lm = __ballot(left);
+            if (lm == 0 || __popcll(lm) == m) {  // midpoint fallback (hlbvh.cu:758-777)
+                const float mid = sel3(box_centroid(cb), axis);
+                left = active && sel3(c, axis) < mid;
+            }
+            lp = wave_sum(left ? r.b : 0u);
+        } else {
+            const float mid = sel3(box_centroid(cb), axis);
+            left = active && sel3(c, axis) < mid;
+            lp = 0;
+        }
+        const uint64_t lmask = __ballot(left);
+        const uint64_t rmask = __ballot(active && !left);
+        const uint32_t L = __popcll(lmask);
+        if (L == 0 || L == m) {
+            if (TOP) {
+                if (lane == 0) atomicOr(A.err, uint32_t(kErrSplit));
+                continue;
+            }
+            if (lane == 0) write_leaf(A.nodes, id, fb, it.begin + lo, m);
+            if (active) A.prim_order[it.begin + lane] = r.a;
+            continue;
+        }
+        if (!TOP) lp = L;
+        if (active) {
+            const uint32_t dst = left ? lo + __popcll(lmask & lanemask_lt())
+                                      : lo + L + __popcll(rmask & lanemask_lt());
+            s_tmp[w][dst] = r;
+        }
+        wave_lds_sync();
+        if (active) s_rec[w][lane] = s_tmp[w][lane];
+        if (lane == 0) write_interior(A.nodes, id, fb, lp, axis);
+        const uint4 ch[2] = {make_uint4(lo, lo + L, id + 1, lp), make_uint4(lo + L, hi, id + 2 * lp, P - lp)};
+        wave_lds_sync();
+        for (int k = 1; k >= 0; --k) {
+            const uint4 cn = ch[k];
+            if (cn.y - cn.x == 1) {
+                if (lane == 0) {
+                    const Rec q = s_rec[w][cn.x];
+                    if (TOP) emit_bottom_root(A, q, cn.z);
+                    else {
+                        write_leaf(A.nodes, cn.z, rec_box(q), it.begin + cn.x, 1);
+                        A.prim_order[it.begin + cn.x] = q.a;
+                    }
+                }
+            } else {
+                if (lane == 0) s_stack[w][sp] = cn;
+                ++sp;
+            }
+        }
+        wave_lds_sync();
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void zero_counters_kernel(uint32_t *c, int n) {
+    if (threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
+__global__ void init_cbounds_kernel(uint32_t *cb) {
+    if (threadIdx.x < 6) cb[threadIdx.x] = threadIdx.x < 3 ? 0xFFFFFFFFu : 0u;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host side
+
+#define PBRTM_TRY(x)                         \
+    do {                                     \
+        hipError_t _e = (x);                 \
+        if (_e != hipSuccess) return _e;     \
+    } while (0)
+
+static hipError_t read_counters(BuildScratch &S, hipStream_t stream) {
+    PBRTM_TRY(hipMemcpyAsync(S.host_counters, S.counters.ptr, kNumCounters * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, stream));
+    return hipStreamSynchronize(stream);
+}
+
+template <bool TOP>
+static hipError_t run_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, Rec *const *recs, int cnt0,
+                             int cnt1, WorkItem *list0, WorkItem *list1, uint32_t *levels) {
+    WorkItem *cur = list0, *nxt = list1;
+    int ccur = cnt0, cnxt = cnt1;
+    PBRTM_TRY(read_counters(S, stream));
+    uint32_t ncur = S.host_counters[ccur];
+    while (ncur > 0) {
+        if (++*levels > 100000) return hipErrorUnknown;  // every level strictly shrinks its nodes
+        PBRTM_TRY(S.acc.reserve(size_t(ncur) * sizeof(Acc)));
+        PBRTM_TRY(S.splits.reserve(size_t(ncur) * sizeof(Split)));
+        PBRTM_TRY(S.task_off.reserve(size_t(ncur + 1) * sizeof(uint32_t)));
+        uint32_t *task_off = S.task_off.as<uint32_t>();
+        task_count_kernel<<<(ncur + 255) / 256, 256, 0, stream>>>(cur, ncur, task_off);
+        PBRTM_TRY(exclusive_scan_u32(task_off, task_off, ncur, S.counters.as<uint32_t>() + kCntTasks,
+                                     S.scan_scratch, stream));
+        PBRTM_TRY(read_counters(S, stream));
+        const uint32_t ntasks = S.host_counters[kCntTasks];
+        PBRTM_TRY(S.tasks.reserve(size_t(ntasks) * sizeof(uint2)));
+        PBRTM_TRY(S.task_left.reserve(size_t(ntasks + 1) * sizeof(uint32_t)));
+        PBRTM_TRY(S.task_lprims.reserve(size_t(ntasks + 1) * sizeof(uint32_t)));
+        uint2 *tasks = S.tasks.as<uint2>();
+        uint32_t *tleft = S.task_left.as<uint32_t>();
+        uint32_t *tlp = S.task_lprims.as<uint32_t>();
+        Acc *acc = S.acc.as<Acc>();
+        Split *splits = S.splits.as<Split>();
+        task_fill_kernel<<<(ncur + 255) / 256, 256, 0, stream>>>(cur, ncur, task_off, tasks);
+        acc_init_kernel<<<ncur, 256, 0, stream>>>(acc, ncur);
+        chunk_reduce_kernel<<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, acc);
+        if (TOP) chunk_buckets_kernel<<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, acc, A.err);
+        decide_kernel<TOP><<<(ncur + 63) / 64, 64, 0, stream>>>(cur, ncur, acc, splits);
+        chunk_count_kernel<TOP><<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, splits, tleft, tlp);
+        PBRTM_TRY(exclusive_scan_u32(tleft, tleft, ntasks, tleft + ntasks, S.scan_scratch, stream));
+        if (TOP) PBRTM_TRY(exclusive_scan_u32(tlp, tlp, ntasks, tlp + ntasks, S.scan_scratch, stream));
+        chunk_scatter_kernel<TOP><<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, splits, task_off, tleft);
+        zero_counters_kernel<<<1, 64, 0, stream>>>(S.counters.as<uint32_t>() + cnxt, 1);
+        children_kernel<TOP><<<(ncur + 63) / 64, 64, 0, stream>>>(A, cur, ncur, splits, acc, task_off, tleft, tlp,
+                                                                  nxt, S.counters.as<uint32_t>() + cnxt);
+        PBRTM_TRY(hipGetLastError());
+        PBRTM_TRY(read_counters(S, stream));
+        std::swap(cur, nxt);
+        std::swap(ccur, cnxt);
+        ncur = S.host_counters[ccur];
+    }
+    return hipSuccess;
+}
+
+hipError_t morton_sort_device(const float *d_tri, uint32_t n, BuildScratch &S, uint32_t **keys_out,
+                              uint32_t **vals_out, hipStream_t stream) {
+    PBRTM_TRY(S.counters.reserve(kNumCounters * sizeof(uint32_t)));
+    for (int k = 0; k < 2; ++k) {
+        PBRTM_TRY(S.keys[k].reserve(size_t(n) * 4));
+        PBRTM_TRY(S.vals[k].reserve(size_t(n) * 4));
+    }
+    PBRTM_TRY(S.hist.reserve((kSortPasses * kSortRadix + 64) * 4));
+    uint32_t *counters = S.counters.as<uint32_t>();
+    uint32_t *hist = S.hist.as<uint32_t>();
+    uint32_t *cbounds = hist + kSortPasses * kSortRadix;
+    const uint32_t grid_n = (n + 255) / 256;
+    zero_counters_kernel<<<1, 64, 0, stream>>>(counters, kNumCounters);
+    *keys_out = S.keys[0].as<uint32_t>();
+    *vals_out = S.vals[0].as<uint32_t>();
+    if (n == 0) return hipGetLastError();
+    fill_u32_kernel<<<12, 256, 0, stream>>>(hist, kSortPasses * kSortRadix, 0u);
+    init_cbounds_kernel<<<1, 64, 0, stream>>>(cbounds);
+    centroid_bounds_kernel<<<min(grid_n, 1024u), 256, 0, stream>>>(d_tri, n, cbounds, counters + kCntErr);
+    morton_kernel<<<min(grid_n, 512u), 256, 0, stream>>>(d_tri, n, cbounds, S.keys[0].as<uint32_t>(),
+                                                          S.vals[0].as<uint32_t>(), hist);
+    return onesweep_sort_30(S.keys[0].as<uint32_t>(), S.vals[0].as<uint32_t>(), S.keys[1].as<uint32_t>(),
+                            S.vals[1].as<uint32_t>(), n, hist, counters + kCntErr, S.sort, stream, keys_out,
+                            vals_out);
+}
+
+hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, BvhDevice &out,
+                              uint32_t *device_error, hipStream_t stream, BuildPhaseTimes *times) {
+    out.n = n;
+    out.num_treelets = out.top_levels = out.bottom_levels = out.small_top = out.small_bottom = 0;
+    *device_error = 0;
+    if (n == 0) return hipSuccess;
+    PBRTM_TRY(S.counters.reserve(kNumCounters * sizeof(uint32_t)));
+    if (!S.host_counters) PBRTM_TRY(hipHostMalloc((void **)&S.host_counters, kNumCounters * sizeof(uint32_t)));
+    for (int k = 0; k < 2; ++k) {
+        PBRTM_TRY(S.keys[k].reserve(size_t(n) * 4));
+        PBRTM_TRY(S.vals[k].reserve(size_t(n) * 4));
+        PBRTM_TRY(S.prec[k].reserve(size_t(n) * sizeof(Rec)));
+    }
+    PBRTM_TRY(S.hist.reserve((kSortPasses * kSortRadix + 64) * 4));
+    PBRTM_TRY(out.nodes.reserve(size_t(2 * uint64_t(n) - 1) * sizeof(BNode)));
+    PBRTM_TRY(out.prim_order.reserve(size_t(n) * 4));
+    uint32_t *counters = S.counters.as<uint32_t>();
+    uint32_t *hist = S.hist.as<uint32_t>();
+    uint32_t *cbounds = hist + kSortPasses * kSortRadix;
+    const uint32_t grid_n = (n + 255) / 256;
+
+    if (times) hipEventRecord(times->ev[0], stream);
+    uint32_t *K = nullptr, *V = nullptr;
+    PBRTM_TRY(morton_sort_device(d_tri, n, S, &K, &V, stream));
+    if (times) hipEventRecord(times->ev[1], stream);
+    uint32_t *flags = (K == S.keys[0].as<uint32_t>()) ? S.keys[1].as<uint32_t>() : S.keys[0].as<uint32_t>();
+    uint32_t *tidx = (V == S.vals[0].as<uint32_t>()) ? S.vals[1].as<uint32_t>() : S.vals[0].as<uint32_t>();
+    Rec *prec0 = S.prec[0].as<Rec>();
+    gather_kernel<<<grid_n, 256, 0, stream>>>(d_tri, n, K, V, prec0, flags);
+    PBRTM_TRY(exclusive_scan_u32(flags, tidx, n, counters + kCntTreelets, S.scan_scratch, stream));
+    PBRTM_TRY(read_counters(S, stream));
+    const uint32_t D = S.host_counters[kCntTreelets];
+    out.num_treelets = D;
+    PBRTM_TRY(S.trec[0].reserve(size_t(D) * sizeof(Rec)));
+    PBRTM_TRY(S.trec[1].reserve(size_t(D) * sizeof(Rec)));
+    uint32_t *tstart = K;  // sorted keys are dead once flags exist
+    treelet_start_kernel<<<grid_n, 256, 0, stream>>>(flags, tidx, n, tstart);
+    treelet_rec_kernel<<<(D + 255) / 256, 256, 0, stream>>>(tstart, D, n, prec0, S.trec[0].as<Rec>());
+    if (times) hipEventRecord(times->ev[2], stream);
+
+    BuildArgs A{};
+    A.nodes = out.nodes.as<BNode>();
+    A.prim_order = out.prim_order.as<uint32_t>();
+    A.trec[0] = S.trec[0].as<Rec>();
+    A.trec[1] = S.trec[1].as<Rec>();
+    A.prec[0] = prec0;
+    A.prec[1] = S.prec[1].as<Rec>();
+    A.cap_small_top = D / 2 + 64;
+    A.cap_large_top = D / 64 + 64;
+    A.cap_small_bot = n / 2 + 64;
+    A.cap_large_bot = n / 64 + 64;
+    const size_t list_items =
+        size_t(A.cap_small_top) + A.cap_small_bot + 2 * size_t(A.cap_large_top) + 2 * size_t(A.cap_large_bot);
+    PBRTM_TRY(S.lists.reserve(list_items * sizeof(WorkItem)));
+    WorkItem *lp = S.lists.as<WorkItem>();
+    A.small_top = lp; lp += A.cap_small_top;
+    A.small_bot = lp; lp += A.cap_small_bot;
+    A.large_top[0] = lp; lp += A.cap_large_top;
+    A.large_top[1] = lp; lp += A.cap_large_top;
+    A.large_bot[0] = lp; lp += A.cap_large_bot;
+    A.large_bot[1] = lp;
+    A.counters = counters;
+    A.err = counters + kCntErr;
+    PBRTM_TRY(S.rec_ptrs.reserve(4 * sizeof(Rec *)));
+    Rec *ptrs[4] = {A.trec[0], A.trec[1], A.prec[0], A.prec[1]};
+    PBRTM_TRY(hipMemcpyAsync(S.rec_ptrs.ptr, ptrs, sizeof(ptrs), hipMemcpyHostToDevice, stream));
+    Rec *const *trec_ptrs = S.rec_ptrs.as<Rec *>();
+    Rec *const *prec_ptrs = S.rec_ptrs.as<Rec *>() + 2;
+
+    root_kernel<<<1, 64, 0, stream>>>(A, D, n);
+    PBRTM_TRY(run_levels<true>(A, S, stream, trec_ptrs, kCntLargeTop0, kCntLargeTop1, A.large_top[0],
+                               A.large_top[1], &out.top_levels));
+    PBRTM_TRY(read_counters(S, stream));
+    out.small_top = S.host_counters[kCntSmallTop];
+    if (out.small_top > 0)
+        finish_kernel<true><<<(out.small_top + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
+            A, A.small_top, out.small_top);
+    if (times) hipEventRecord(times->ev[3], stream);
+    PBRTM_TRY(run_levels<false>(A, S, stream, prec_ptrs, kCntLargeBot0, kCntLargeBot1, A.large_bot[0],
+                                A.large_bot[1], &out.bottom_levels));
+    PBRTM_TRY(read_counters(S, stream));
+    out.small_bottom = S.host_counters[kCntSmallBot];
+    if (out.small_bottom > 0)
+        finish_kernel<false><<<(out.small_bottom + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
+            A, A.small_bot, out.small_bottom);
+    if (times) hipEventRecord(times->ev[4], stream);
+    PBRTM_TRY(hipGetLastError());
+    PBRTM_TRY(read_counters(S, stream));
+    *device_error = S.host_counters[kCntErr];
+    if (times) hipEventRecord(times->ev[5], stream);
+    return hipSuccess;
+}
+
+void BuildScratch::release() {
+    DevBuf *all[] = {&counters, &hist, &keys[0], &keys[1], &vals[0], &vals[1], &prec[0], &prec[1],
+                     &trec[0], &trec[1], &lists, &acc, &splits, &tasks, &task_off, &task_left,
+                     &task_lprims, &rec_ptrs, &scan_scratch, &sort.status, &sort.offsets, &sort.counters};
+    for (DevBuf *b : all) b->release();
+    if (host_counters) hipHostFree(host_counters);
+    host_counters = nullptr;
+}
+
+}  // namespace pbrtm

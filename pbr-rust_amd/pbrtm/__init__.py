@@ -1,0 +1,234 @@
+"""ctypes binding of libpbrtm.so (include/pbrtm.h) for tests and the benchmark.
+
+The library is the product: every call below runs the HIP kernels for gfx950.
+There is no CPU fallback; loading fails loudly when libpbrtm.so is missing.
+Device-pointer calls take torch tensors (``.data_ptr()``) allocated on the
+context's device.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_uint8, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libpbrtm.so")
+
+OK = 0
+ERR_INVALID = -1
+ERR_HIP = -2
+ERR_DEVICE = -3
+ERR_NOMEM = -4
+ERR_UNSUPPORTED = -5
+DEVICE_PTRS = 1
+NO_HIT = 0xFFFFFFFF
+
+# Every symbol include/pbrtm.h declares.
+EXPORTS = (
+    "pbrtm_create", "pbrtm_destroy", "pbrtm_last_error", "pbrtm_device_count", "pbrtm_stream",
+    "pbrtm_morton_sort", "pbrtm_bvh_build", "pbrtm_bvh_free", "pbrtm_bvh_bounds",
+    "pbrtm_bvh_export_canonical", "pbrtm_bvh_intersect", "pbrtm_ao_bounce", "pbrtm_gen_soup",
+    "pbrtm_gen_camera_rays",
+)
+
+
+class BuildStats(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", c_double), ("ms_sort", c_double), ("ms_treelets", c_double), ("ms_top", c_double),
+        ("ms_bottom", c_double), ("ms_layout", c_double), ("num_prims", c_uint64),
+        ("num_treelets", c_uint64), ("top_levels", c_uint64), ("bottom_levels", c_uint64),
+        ("small_top", c_uint64), ("small_bottom", c_uint64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class PbrtmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pbrtm error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libpbrtm.so and declare the C signatures (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    p = c_void_p
+    sig = {
+        "pbrtm_create": (c_int, [POINTER(p), c_int]),
+        "pbrtm_destroy": (None, [p]),
+        "pbrtm_last_error": (c_char_p, [p]),
+        "pbrtm_device_count": (c_int, [POINTER(c_int)]),
+        "pbrtm_stream": (p, [p]),
+        "pbrtm_morton_sort": (c_int, [p, p, c_uint64, c_uint32, p, p]),
+        "pbrtm_bvh_build": (c_int, [p, p, c_uint64, p, c_uint64, c_uint32, POINTER(p), POINTER(BuildStats)]),
+        "pbrtm_bvh_free": (None, [p]),
+        "pbrtm_bvh_bounds": (c_int, [p, p]),
+        "pbrtm_bvh_export_canonical": (c_int, [p, p, POINTER(c_uint64)]),
+        "pbrtm_bvh_intersect": (c_int, [p, p, c_uint64, c_int, c_uint32, p, p, p]),
+        "pbrtm_ao_bounce": (c_int, [p, p, p, p, c_uint64, c_uint32, p]),
+        "pbrtm_gen_soup": (c_int, [c_uint64, c_uint64, c_float, c_float, c_int, p]),
+        "pbrtm_gen_camera_rays": (c_int, [c_uint32, c_uint32, p, p, p, c_float, p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data_as(c_void_p)
+    return c_void_p(int(a.data_ptr()))  # torch tensor on the device
+
+
+def gen_soup(n: int, seed: int = 0x5EED, mu: float = -6.0, sigma: float = 0.5, threads: int = 8) -> np.ndarray:
+    """Seeded triangle soup of the HLBVH microbenchmark, [n][3][3] float32."""
+    lib = load()
+    out = np.empty((n, 3, 3), np.float32)
+    rc = lib.pbrtm_gen_soup(seed, n, mu, sigma, threads, _ptr(out))
+    if rc != OK:
+        raise PbrtmError(rc, "gen_soup")
+    return out
+
+
+def gen_camera_rays(width: int, height: int, eye=(0.5, 0.5, -1.5), look=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0),
+                    fov_deg: float = 45.0) -> np.ndarray:
+    lib = load()
+    out = np.empty((height * width, 8), np.float32)
+    e, l, u = (np.asarray(v, np.float32) for v in (eye, look, up))
+    rc = lib.pbrtm_gen_camera_rays(width, height, _ptr(e), _ptr(l), _ptr(u), fov_deg, _ptr(out))
+    if rc != OK:
+        raise PbrtmError(rc, "gen_camera_rays")
+    return out
+
+
+def device_count() -> int:
+    c = c_int(0)
+    load().pbrtm_device_count(ctypes.byref(c))
+    return c.value
+
+
+class Context:
+    """One HIP device and stream (pbrtm_create)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = c_void_p()
+        rc = self.lib.pbrtm_create(ctypes.byref(h), device)
+        if rc != OK:
+            raise PbrtmError(rc, f"cannot create a context on device {device}")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.pbrtm_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return self.lib.pbrtm_stream(self.h) or 0
+
+    def check(self, rc):
+        if rc != OK:
+            raise PbrtmError(rc, self.lib.pbrtm_last_error(self.h).decode())
+
+    def morton_sort(self, tri, device_ptrs=False, codes=None, order=None):
+        n = tri.shape[0]
+        if not device_ptrs:
+            codes = np.empty(n, np.uint32)
+            order = np.empty(n, np.uint32)
+        self.check(self.lib.pbrtm_morton_sort(self.h, _ptr(tri), n, DEVICE_PTRS if device_ptrs else 0,
+                                              _ptr(codes), _ptr(order)))
+        return codes, order
+
+    def build(self, verts, idx=None, device_ptrs=False) -> "Bvh":
+        stats = BuildStats()
+        h = c_void_p()
+        if idx is None:
+            nt = verts.shape[0]
+            nv = 3 * nt
+        else:
+            nt = idx.shape[0]
+            nv = verts.shape[0]
+        self.check(self.lib.pbrtm_bvh_build(self.h, _ptr(verts), nv, _ptr(idx), nt,
+                                            DEVICE_PTRS if device_ptrs else 0, ctypes.byref(h),
+                                            ctypes.byref(stats)))
+        return Bvh(self, h, nt, stats, keep=verts if device_ptrs else None)
+
+
+class Bvh:
+    """An aggregate handle (pbrtm_bvh_build); mirrors HLBVH's query surface."""
+
+    def __init__(self, ctx: Context, h, n: int, stats: BuildStats, keep=None):
+        self.ctx, self.h, self.n, self.stats = ctx, h, n, stats
+        self._keep = keep  # caller's device triangles referenced by the handle
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.pbrtm_bvh_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def bounds(self) -> np.ndarray:
+        out = np.empty(6, np.float32)
+        self.ctx.check(self.ctx.lib.pbrtm_bvh_bounds(self.h, _ptr(out)))
+        return out
+
+    def export_canonical(self) -> bytes:
+        ln = c_uint64(0)
+        self.ctx.check(self.ctx.lib.pbrtm_bvh_export_canonical(self.h, None, ctypes.byref(ln)))
+        buf = np.empty(ln.value, np.uint8)
+        self.ctx.check(self.ctx.lib.pbrtm_bvh_export_canonical(self.h, _ptr(buf), ctypes.byref(ln)))
+        return buf.tobytes()
+
+    def intersect(self, rays, any_hit=False, device_ptrs=False, prim=None, t=None, bary=None, want_bary=True):
+        n = rays.shape[0]
+        if not device_ptrs:
+            prim = np.empty(n, np.uint32)
+            t = np.empty(n, np.float32)
+            bary = np.empty((n, 3), np.float32) if want_bary else None
+        self.ctx.check(self.ctx.lib.pbrtm_bvh_intersect(self.h, _ptr(rays), n, int(any_hit),
+                                                        DEVICE_PTRS if device_ptrs else 0, _ptr(prim),
+                                                        _ptr(t), _ptr(bary)))
+        return prim, t, bary
+
+    def ao_bounce(self, rays, prim, bary, device_ptrs=False, out=None):
+        n = rays.shape[0]
+        if not device_ptrs:
+            out = np.empty((n, 8), np.float32)
+        self.ctx.check(self.ctx.lib.pbrtm_ao_bounce(self.h, _ptr(rays), _ptr(prim), _ptr(bary), n,
+                                                    DEVICE_PTRS if device_ptrs else 0, _ptr(out)))
+        return out

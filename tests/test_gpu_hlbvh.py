@@ -1,0 +1,161 @@
+"""GPU parity of the HLBVH path through the C ABI against the CPU oracle.
+
+Bit-exact: Morton codes and canonical order; the canonical tree (node sets,
+split axes, bound bit patterns); closest-hit primitive ids, t and barycentrics
+on identical rays; any-hit booleans. Bounce rays (which go through cosf/sinf)
+are compared within 2e-6 relative, then traced bit-exactly as given.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import pbrtm
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def soup(n, seed):
+    return pbrtm.gen_soup(n, seed=seed)
+
+
+def clustered(n, seed, far=True):
+    """n triangles in a tiny cluster (+ one far triangle): few, huge treelets."""
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 1, 3)) * 1e-3
+    tri = (c + rng.normal(0, 1e-5, (n, 3, 3))).astype(np.float32)
+    if far:
+        tri[0] += np.float32(10.0)
+    return np.ascontiguousarray(tri)
+
+
+CASES = {
+    "n1": lambda: soup(1, 0),
+    "n2": lambda: soup(2, 1),
+    "n3_dup": lambda: np.repeat(soup(1, 2), 3, axis=0),
+    "n70": lambda: soup(70, 3),
+    "n5000": lambda: soup(5000, 4),
+    "n200k": lambda: soup(200_000, 5),
+    "cluster3000": lambda: clustered(3000, 6),
+    "cluster_dups": lambda: np.concatenate([clustered(500, 7), np.repeat(soup(1, 8), 200, axis=0)]),
+    "flat_z": lambda: (lambda t: (t.__setitem__((slice(None), slice(None), 2), np.float32(0.5)), t)[1])(soup(20000, 9)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_morton_sort_parity(ctx, case):
+    tri = CASES[case]()
+    log(f"morton_sort {case} n={len(tri)}")
+    codes, order = ctx.morton_sort(tri)
+    oc, oo = O.morton_sort(tri)
+    assert np.array_equal(codes, oc)
+    assert np.array_equal(order, oo)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_bvh_canonical_parity(ctx, case):
+    tri = CASES[case]()
+    log(f"bvh_build {case} n={len(tri)}")
+    bvh = ctx.build(tri)
+    got = bvh.export_canonical()
+    want = O.OracleBvh(tri).export_canonical()
+    assert len(got) == len(want)
+    assert got == want
+    lo = tri.min(axis=(0, 1))
+    hi = tri.max(axis=(0, 1))
+    assert np.array_equal(bvh.bounds(), np.r_[lo, hi])
+
+
+def test_bvh_canonical_parity_1m(ctx):
+    tri = soup(1_000_000, 0x5EED)
+    log("bvh_build 1M")
+    bvh = ctx.build(tri)
+    log(f"stats {bvh.stats.as_dict()}")
+    assert bvh.export_canonical() == O.OracleBvh(tri).export_canonical()
+
+
+def test_intersect_and_bounce_parity_1m(ctx):
+    tri = soup(1_000_000, 0x5EED)
+    rays = pbrtm.gen_camera_rays(960, 540)
+    log("intersect 1M primary")
+    bvh = ctx.build(tri)
+    ob = O.OracleBvh(tri)
+    prim, t, bary = bvh.intersect(rays)
+    op, ot, obary, _ = ob.intersect(rays)
+    assert (op != O.NO_HIT).mean() > 0.1
+    assert np.array_equal(prim, op)
+    assert np.array_equal(t, ot)
+    assert np.array_equal(bary, obary)
+    log("ao_bounce")
+    br = bvh.ao_bounce(rays, prim, bary)
+    obr = O.ao_bounce(tri, rays, op, obary)
+    active = obr[:, 3] > 0
+    assert np.array_equal(br[:, 3] > 0, active)
+    assert np.allclose(br[active, :3], obr[active, :3], rtol=2e-6, atol=1e-7)
+    assert np.allclose(br[active, 4:7], obr[active, 4:7], rtol=0, atol=2e-6)
+    log("intersect bounce rays")
+    p2, t2, b2 = bvh.intersect(br)
+    q2, u2, c2, _ = ob.intersect(br)
+    assert np.array_equal(p2, q2)
+    assert np.array_equal(t2, u2)
+    log("any hit")
+    pa, _, _ = bvh.intersect(br, any_hit=True, want_bary=False)
+    qa, _, _, _ = ob.intersect(br, any_hit=True)
+    assert np.array_equal(pa != O.NO_HIT, qa != O.NO_HIT)
+
+
+def test_intersect_brute_force_small(ctx):
+    tri = soup(3000, 21)
+    rng = np.random.default_rng(1)
+    k = rng.integers(0, 3000, 5000)
+    w = (rng.dirichlet((1.0, 1.0, 1.0), 5000) - 1 / 3) * 1.1 + 1 / 3
+    tgt = np.einsum("ij,ijk->ik", w, tri[k].astype(np.float64))
+    org = rng.random((5000, 3)) * 1.4 - 0.2
+    d = (tgt - org) / np.linalg.norm(tgt - org, axis=1, keepdims=True)
+    rays = np.zeros((5000, 8), np.float32)
+    rays[:, :3], rays[:, 3], rays[:, 4:7] = org, np.inf, d
+    rays[::7, 3] = 0.4  # finite t_max
+    rays[::11, 3] = 0.0  # inactive
+    bvh = ctx.build(tri)
+    prim, t, _ = bvh.intersect(rays)
+    bp, bt = O.brute_intersect(tri, rays)
+    assert np.array_equal(prim, bp)
+    assert np.array_equal(t, bt)
+
+
+def test_empty_aggregate(ctx):
+    tri = np.zeros((0, 3, 3), np.float32)
+    bvh = ctx.build(tri)
+    assert bvh.export_canonical() == b""
+    rays = pbrtm.gen_camera_rays(8, 8)
+    prim, t, _ = bvh.intersect(rays)
+    assert np.all(prim == pbrtm.NO_HIT)
+
+
+def test_nonfinite_vertex_is_an_error(ctx):
+    tri = soup(100, 3)
+    tri[17, 1, 2] = np.nan
+    with pytest.raises(pbrtm.PbrtmError) as e:
+        ctx.build(tri)
+    assert e.value.code == pbrtm.ERR_DEVICE
+    # the context stays usable after the error
+    assert ctx.build(soup(10, 3)).export_canonical() == O.OracleBvh(soup(10, 3)).export_canonical()
+
+
+def test_indexed_mesh_equals_soup(ctx):
+    rng = np.random.default_rng(5)
+    verts = rng.random((500, 3)).astype(np.float32)
+    idx = rng.integers(0, 500, (2000, 3)).astype(np.uint32)
+    idx[:, 1] = (idx[:, 0] + 1) % 500
+    idx[:, 2] = (idx[:, 0] + 2) % 500
+    tri = np.ascontiguousarray(verts[idx])
+    a = ctx.build(verts, idx=idx).export_canonical()
+    b = ctx.build(tri).export_canonical()
+    assert a == b
+    bad = idx.copy()
+    bad[5, 0] = 500
+    with pytest.raises(pbrtm.PbrtmError):
+        ctx.build(verts, idx=bad)
