@@ -1,0 +1,194 @@
+"""HLBVH build + traversal benchmark on one MI355X.
+
+Workload (one step): build the HLBVH over 10M seeded FP32 triangles already
+resident in HBM (centroids uniform in the unit cube, edge lengths log-normal
+mu=-6, sigma=0.5), trace 1920x1080 primary rays (closest hit), spawn one
+cosine-hemisphere diffuse bounce per hit, trace the bounce rays (closest hit).
+
+Prints one JSON line: the headline metric is the HLBVH build time at 10M
+triangles (lower is better); traversal throughput, per-phase times, the
+roofline of the dominant kernel and a CPU baseline (the oracle, a host-only
+restatement of the reference's build, timed on a bounded sample) ride along.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pbr-rust_amd"))
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 32      # binary BVH node (box + two words)
+TRI_BYTES = 36       # leaf-ordered triangle, 9 floats
+RAY_BYTES = 32
+BUILD_BYTES_PER_TRI = 216  # algorithmic bytes of an HLBVH build per triangle
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--tris", type=int, default=10_000_000)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--seed", type=int, default=0x5EED)
+    ap.add_argument("--cpu-sample-tris", type=int, default=2_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(args, pbrtm):
+    """The oracle's build on a bounded sample of the same generator, scaled
+    linearly to the benchmark size (the build is O(n log n); the scaled figure
+    slightly favours the CPU)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    n = min(args.cpu_sample_tris, args.tris)
+    tri = pbrtm.gen_soup(n, seed=args.seed)
+    t0 = time.perf_counter()
+    b = oracle.OracleBvh(tri)
+    ms = (time.perf_counter() - t0) * 1e3
+    del b
+    return {
+        "value": round(ms * args.tris / n, 1),
+        "unit": "ms",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle (host C++ restatement of hlbvh.cu) build of {n} triangles of the same generator, "
+                  f"{ms:.0f} ms, scaled x{args.tris / n:g} to {args.tris}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import pbrtm
+
+    dev = torch.device("cuda:0")
+    tri = pbrtm.gen_soup(args.tris, seed=args.seed, threads=16)
+    rays = pbrtm.gen_camera_rays(args.width, args.height)
+    nr = rays.shape[0]
+    ctx = pbrtm.Context(0)
+    stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
+    d_tri = torch.from_numpy(tri).to(dev)
+    d_rays = torch.from_numpy(rays).to(dev)
+    d_prim = torch.empty(nr, dtype=torch.int32, device=dev)
+    d_t = torch.empty(nr, dtype=torch.float32, device=dev)
+    d_bary = torch.empty((nr, 3), dtype=torch.float32, device=dev)
+    d_bounce = torch.empty((nr, 8), dtype=torch.float32, device=dev)
+    d_prim2 = torch.empty(nr, dtype=torch.int32, device=dev)
+    d_t2 = torch.empty(nr, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+
+    def step(counts=None):
+        bvh = ctx.build(d_tri, device_ptrs=True)
+        s1 = s2 = None
+        if counts is not None:
+            s1, s2 = np.zeros(2, np.uint64), np.zeros(2, np.uint64)
+        ev[0].record(stream)
+        bvh.intersect(d_rays, device_ptrs=True, prim=d_prim, t=d_t, bary=d_bary, stats=s1)
+        ev[1].record(stream)
+        bvh.ao_bounce(d_rays, d_prim, d_bary, device_ptrs=True, out=d_bounce)
+        ev[2].record(stream)
+        bvh.intersect(d_bounce, device_ptrs=True, prim=d_prim2, t=d_t2, bary=None, stats=s2)
+        ev[3].record(stream)
+        stats = bvh.stats.as_dict()
+        bvh.free()
+        if counts is not None:
+            counts.append((s1, s2))
+        return stats
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    builds, prim_ms, bounce_ms, sec_ms = [], [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = step()
+        torch.cuda.synchronize()
+        builds.append(st)
+        prim_ms.append(ev[0].elapsed_time(ev[1]))
+        bounce_ms.append(ev[1].elapsed_time(ev[2]))
+        sec_ms.append(ev[2].elapsed_time(ev[3]))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    # Untimed counting pass: the same traversals, with per-launch totals of box
+    # and triangle tests for the algorithmic byte count.
+    counts = []
+    step(counts)
+    torch.cuda.synchronize()
+    (v1, t1), (v2, t2) = counts[0]
+    active2 = int((d_bounce[:, 3] > 0).sum().item())
+    hits1 = int((d_prim != -1).sum().item())
+    hits2 = int((d_prim2 != -1).sum().item())
+
+    build_ms = float(np.mean([b["ms_total"] for b in builds]))
+    phase = {k: round(float(np.mean([b[k] for b in builds])), 3)
+             for k in ("ms_sort", "ms_treelets", "ms_top", "ms_bottom", "ms_layout")}
+    trace_ms = float(np.mean(prim_ms) + np.mean(sec_ms))
+    rays_traced = nr + active2
+    mrays = rays_traced / (trace_ms * 1e-3) / 1e6
+    # primary rays write prim + t + barycentrics (20 B), bounce rays prim + t (8 B)
+    bytes1 = nr * (RAY_BYTES + 20) + int(v1) * NODE_BYTES + int(t1) * TRI_BYTES
+    bytes2 = nr * (RAY_BYTES + 8) + int(v2) * NODE_BYTES + int(t2) * TRI_BYTES
+    trace_gbs = (bytes1 + bytes2) / (trace_ms * 1e-3) / 1e9
+    build_gbs = BUILD_BYTES_PER_TRI * args.tris / (build_ms * 1e-3) / 1e9
+
+    # Dominant kernel by measured time: the closest-hit traversal when the
+    # two traversals outweigh the build.
+    if trace_ms >= build_ms:
+        roof = {"kernel": "trace_binary_kernel", "bound": "hbm", "achieved": round(trace_gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(trace_gbs / HBM_PEAK_GBS, 4),
+                "traffic": None,
+                "algorithmic_bytes": "32 B ray + 20/8 B hit + 32 B per box test + 36 B per triangle test"}
+    else:
+        roof = {"kernel": "hlbvh build (all phases)", "bound": "hbm", "achieved": round(build_gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
+                "traffic": None, "algorithmic_bytes": f"{BUILD_BYTES_PER_TRI} B per triangle"}
+
+    line = {
+        "metric": "HLBVH build ms @10M tris",
+        "value": round(build_ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "hlbvh build + 1080p primary + 1 diffuse bounce", "tris": args.tris,
+                   "width": args.width, "height": args.height, "seed": args.seed, "mu": -6.0, "sigma": 0.5},
+        "build_phases_ms": phase,
+        "build_mtris_per_s": round(args.tris / build_ms / 1e3, 1),
+        "trace_ms": {"primary": round(float(np.mean(prim_ms)), 3), "bounce_spawn": round(float(np.mean(bounce_ms)), 3),
+                     "secondary": round(float(np.mean(sec_ms)), 3)},
+        "mrays_per_s": round(mrays, 1),
+        "rays": {"primary": nr, "primary_hits": hits1, "bounce": active2, "bounce_hits": hits2,
+                 "box_tests_per_ray": round((int(v1) + int(v2)) / rays_traced, 2),
+                 "tri_tests_per_ray": round((int(t1) + int(t2)) / rays_traced, 2)},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args, pbrtm)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

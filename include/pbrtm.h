@@ -104,9 +104,11 @@ int pbrtm_bvh_export_canonical(const pbrtm_bvh *bvh, uint8_t *buf, uint64_t *len
 
 /* Batched ray queries. any_hit = 0 replaces HLBVH::intersect (closest hit,
  * hlbvh.cu:231-289); any_hit = 1 replaces HLBVH::fast_intersect (hlbvh.cu:179-229),
- * whose bool is prim_out != PBRTM_NO_HIT. t_out and bary_out ([n][3]) may be NULL. */
+ * whose bool is prim_out != PBRTM_NO_HIT. t_out and bary_out ([n][3]) may be NULL.
+ * stats_out (host, may be NULL) receives [box tests, triangle tests] summed over
+ * the rays; it selects a counting instance of the same traversal. */
 int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_hit, uint32_t flags,
-                        uint32_t *prim_out, float *t_out, float *bary_out);
+                        uint32_t *prim_out, float *t_out, float *bary_out, uint64_t *stats_out);
 
 /* From closest hits (prim, barycentrics), one cosine-weighted diffuse bounce
  * per ray, as integrators/ambient_occlusion.cu:36-52 spawns it: u = the first

@@ -27,7 +27,8 @@ struct pbrtm_ctx {
 };
 
 struct pbrtm_bvh {
-    pbrtm_ctx *ctx = nullptr;
+    pbrtm_ctx *ctx = nullptr;  // must outlive the handle
+    int device = 0;
     uint32_t n = 0;
     BvhDevice dev;
     DevBuf tri_owned, tri_leaf;
@@ -206,6 +207,7 @@ int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint3
     auto *bvh = new (std::nothrow) pbrtm_bvh();
     if (!bvh) return fail(ctx, PBRTM_ERR_NOMEM, "handle");
     bvh->ctx = ctx;
+    bvh->device = ctx->device;
     bvh->n = uint32_t(nt);
     const float mx = 3.402823466e+38f;
     const float empty[6] = {mx, mx, mx, -mx, -mx, -mx};
@@ -286,10 +288,7 @@ int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint3
 
 void pbrtm_bvh_free(pbrtm_bvh *bvh) {
     if (!bvh) return;
-    if (bvh->ctx) {
-        hipSetDevice(bvh->ctx->device);
-        hipStreamSynchronize(bvh->ctx->stream);
-    }
+    hipSetDevice(bvh->device);  // hipFree waits for work that uses the memory
     bvh->dev.nodes.release();
     bvh->dev.prim_order.release();
     bvh->tri_owned.release();
@@ -360,7 +359,7 @@ int pbrtm_bvh_export_canonical(const pbrtm_bvh *bvh, uint8_t *buf, uint64_t *len
 }
 
 int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_hit, uint32_t flags,
-                        uint32_t *prim_out, float *t_out, float *bary_out) {
+                        uint32_t *prim_out, float *t_out, float *bary_out, uint64_t *stats_out) {
     if (!bvh || !prim_out) return PBRTM_ERR_INVALID;
     pbrtm_ctx *ctx = bvh->ctx;
     if (n >= (1ull << 32)) return fail(ctx, PBRTM_ERR_INVALID, "too many rays");
@@ -379,8 +378,9 @@ int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_h
         d_t = t_out ? ctx->stage_b.as<float>() : nullptr;
         d_b = bary_out ? ctx->stage_c.as<float>() : nullptr;
     }
-    CK(ctx, ctx->dev_err.reserve(4), "alloc");
-    CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream), "memset");
+    CK(ctx, ctx->dev_err.reserve(64), "alloc");
+    CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 64, ctx->stream), "memset");
+    unsigned long long *d_counts = stats_out ? (unsigned long long *)(ctx->dev_err.as<uint8_t>() + 16) : nullptr;
     if (bvh->n == 0) {
         CK(ctx, hipMemsetAsync(d_prim, 0xFF, n * 4, ctx->stream), "memset");
         if (d_t) CK(ctx, hipMemsetAsync(d_t, 0, n * 4, ctx->stream), "memset");
@@ -388,7 +388,7 @@ int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_h
     } else {
         BvhTrace B{bvh->dev.nodes.as<BNode>(), bvh->tri_leaf.as<float>(), bvh->dev.prim_order.as<uint32_t>(), bvh->n};
         CK(ctx, trace_binary(B, (const float *)d_rays, uint32_t(n), any_hit, d_prim, d_t, d_b,
-                             ctx->dev_err.as<uint32_t>(), ctx->stream), "trace");
+                             ctx->dev_err.as<uint32_t>(), d_counts, ctx->stream), "trace");
     }
     if (!dev) {
         CK(ctx, hipMemcpyAsync(prim_out, d_prim, n * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
@@ -397,6 +397,8 @@ int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_h
     }
     uint32_t derr = 0;
     CK(ctx, hipMemcpyAsync(&derr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+    if (stats_out)
+        CK(ctx, hipMemcpyAsync(stats_out, d_counts, 16, hipMemcpyDeviceToHost, ctx->stream), "download");
     CK(ctx, hipStreamSynchronize(ctx->stream), "sync");
     if (derr) return fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr));
     return PBRTM_OK;

@@ -102,6 +102,23 @@ __device__ __forceinline__ void push_item(WorkItem *list, uint32_t *counter, uin
     else atomicOr(err, uint32_t(kErrCapacity));
 }
 
+// Wave-aggregated append: one atomic per wave and list instead of one per item.
+// Every active lane of the wave must call it (it ballots).
+__device__ __forceinline__ void wave_push(bool want, const WorkItem &it, WorkItem *list, uint32_t *counter,
+                                          uint32_t cap, uint32_t *err) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (int(__lane_id()) == leader) base = atomicAdd(counter, uint32_t(__popcll(m)));
+    base = __shfl(base, leader, kWave);
+    if (want) {
+        const uint32_t slot = base + __popcll(m & lanemask_lt());
+        if (slot < cap) list[slot] = it;
+        else atomicOr(err, uint32_t(kErrCapacity));
+    }
+}
+
 // A treelet that became a top-level leaf is the root of a bottom-level build
 // over its primitive range (hlbvh.cu:565-607 expands such leaves).
 __device__ void emit_bottom_root(const BuildArgs &A, const Rec &treelet, uint32_t node) {
@@ -522,20 +539,33 @@ __global__ __launch_bounds__(64) void children_kernel(BuildArgs A, const WorkIte
     WorkItem ch[2] = {{it.begin, it.begin + L, it.node + 1, lp, nb, {0, 0, 0}},
                       {it.begin + L, it.end, it.node + 2 * lp, it.prims - lp, nb, {0, 0, 0}}};
     for (int c = 0; c < 2; ++c) {
-        const uint32_t cs = ch[c].end - ch[c].begin;
+        WorkItem cw = ch[c];
+        const uint32_t cs = cw.end - cw.begin;
+        // 0 = done here, 1 = small list, 2 = next large list, 3 = small bottom, 4 = large bottom
+        int cat = 0;
         if (cs == 1) {
-            const Rec r = pick(recs, nb)[ch[c].begin];
-            if (TOP) emit_bottom_root(A, r, ch[c].node);
-            else {
-                write_leaf(A.nodes, ch[c].node, rec_box(r), ch[c].begin, 1);
-                A.prim_order[ch[c].begin] = r.a;
+            const Rec r = pick(recs, nb)[cw.begin];
+            if (TOP) {  // a single treelet: root of a bottom-level build over its primitives
+                cw = WorkItem{r.a, r.a + r.b, cw.node, r.b, 0, {0, 0, 0}};
+                if (r.b == 1) {
+                    const Rec p = A.prec[0][r.a];
+                    write_leaf(A.nodes, cw.node, rec_box(p), r.a, 1);
+                    A.prim_order[r.a] = p.a;
+                } else {
+                    cat = r.b <= kSmallNode ? 3 : 4;
+                }
+            } else {
+                write_leaf(A.nodes, cw.node, rec_box(r), cw.begin, 1);
+                A.prim_order[cw.begin] = r.a;
             }
-        } else if (cs <= kSmallNode) {
-            if (TOP) push_item(A.small_top, A.counters + kCntSmallTop, A.cap_small_top, ch[c], A.err);
-            else push_item(A.small_bot, A.counters + kCntSmallBot, A.cap_small_bot, ch[c], A.err);
         } else {
-            push_item(next_large, next_counter, TOP ? A.cap_large_top : A.cap_large_bot, ch[c], A.err);
+            cat = cs <= kSmallNode ? 1 : 2;
         }
+        wave_push(cat == 1, cw, TOP ? A.small_top : A.small_bot,
+                  A.counters + (TOP ? kCntSmallTop : kCntSmallBot), TOP ? A.cap_small_top : A.cap_small_bot, A.err);
+        wave_push(cat == 2, cw, next_large, next_counter, TOP ? A.cap_large_top : A.cap_large_bot, A.err);
+        wave_push(cat == 3, cw, A.small_bot, A.counters + kCntSmallBot, A.cap_small_bot, A.err);
+        wave_push(cat == 4, cw, A.large_bot[0], A.counters + kCntLargeBot0, A.cap_large_bot, A.err);
     }
 }
 
@@ -544,32 +574,37 @@ __global__ __launch_bounds__(64) void children_kernel(BuildArgs A, const WorkIte
 constexpr int kFinBlock = 256;
 constexpr int kFinWaves = kFinBlock / kWave;
 
+// Per-wave LDS working set of the finisher.
+struct WaveLds {
+    Rec rec[kSmallNode];
+    Rec tmp[kSmallNode];
+    uint4 stack[kSmallNode];
+    uint32_t bkt[kSmallNode];
+    float cost[kNumBuckets];
+    uint2 leaf[kSmallNode];  // top level: (node id, wave-local index) of single-treelet leaves
+};
+
+// Builds the whole subtree of one node with m0 <= 64 elements inside one wave.
+// Elements sit one per lane in LDS; every split is a wave reduction plus a
+// ballot-ranked stable partition. Top level (SAH over treelets): single-treelet
+// leaves are recorded in L.leaf and their count returned. Bottom level
+// (midpoint over primitives): leaves are written, with their primitive ids.
 template <bool TOP>
-__global__ __launch_bounds__(kFinBlock) void finish_kernel(BuildArgs A, const WorkItem *items, uint32_t nitems) {
-    __shared__ Rec s_rec[kFinWaves][kSmallNode];
-    __shared__ Rec s_tmp[kFinWaves][kSmallNode];
-    __shared__ uint4 s_stack[kFinWaves][kSmallNode];
-    __shared__ uint32_t s_bkt[kFinWaves][kSmallNode];
-    __shared__ float s_cost[kFinWaves][kNumBuckets];
-    const int w = threadIdx.x / kWave;
+__device__ uint32_t wave_build(const BuildArgs &A, const Rec *src, uint32_t begin, uint32_t m0, uint32_t node0,
+                               uint32_t prims0, WaveLds &L) {
     const uint32_t lane = __lane_id();
-    const uint32_t wid = blockIdx.x * kFinWaves + w;
-    if (wid >= nitems) return;
-    const WorkItem it = items[wid];
-    Rec *const(&recs)[2] = TOP ? A.trec : A.prec;
-    const uint32_t m0 = it.end - it.begin;
-    if (lane < m0) s_rec[w][lane] = pick(recs, it.buf)[it.begin + lane];
-    int sp = 0;
-    if (lane == 0) s_stack[w][0] = make_uint4(0, m0, it.node, it.prims);
-    sp = 1;
+    if (lane < m0) L.rec[lane] = src[begin + lane];
+    if (lane == 0) L.stack[0] = make_uint4(0, m0, node0, prims0);
+    int sp = 1;
+    uint32_t nleaf = 0;
     wave_lds_sync();
     while (sp > 0) {
         --sp;
-        const uint4 nd = s_stack[w][sp];
+        const uint4 nd = L.stack[sp];
         const uint32_t lo = nd.x, hi = nd.y, id = nd.z, P = nd.w, m = hi - lo;
         const bool active = lane >= lo && lane < hi;
         Rec r;
-        if (active) r = s_rec[w][lane];
+        if (active) r = L.rec[lane];
         const Box b = active ? rec_box(r) : box_empty();
         const f3 c = box_centroid(b);
         uint32_t v[12];
@@ -585,7 +620,7 @@ __global__ __launch_bounds__(kFinBlock) void finish_kernel(BuildArgs A, const Wo
         const Box fb = acc_box(v + 6);
         const int axis = box_max_dim(cb);
         bool left;
-        uint32_t lp;
+        uint32_t lp = 0;
         if (TOP) {
             if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) {
                 if (lane == 0) atomicOr(A.err, uint32_t(kErrDegenerateCentroids));
@@ -594,24 +629,24 @@ __global__ __launch_bounds__(kFinBlock) void finish_kernel(BuildArgs A, const Wo
             const float base = sel3(cb.lo, axis);
             const float span = sel3(cb.hi, axis) - sel3(cb.lo, axis);
             const uint32_t bk = bucket_of(sel3(c, axis), base, span);
-            if (active) s_bkt[w][lane] = bk;
+            if (active) L.bkt[lane] = bk;
             wave_lds_sync();
             const float total_sa = box_area(fb);
             if (lane < kNumBuckets - 1) {  // lane k prices the split after bucket k
                 Box bl = box_empty(), br = box_empty();
                 uint32_t cl = 0, cr = 0;
                 for (uint32_t j = lo; j < hi; ++j) {
-                    const Rec q = s_rec[w][j];
-                    if (s_bkt[w][j] <= lane) { bl = box_union(bl, rec_box(q)); cl += q.b; }
+                    const Rec q = L.rec[j];
+                    if (L.bkt[j] <= lane) { bl = box_union(bl, rec_box(q)); cl += q.b; }
                     else { br = box_union(br, rec_box(q)); cr += q.b; }
                 }
-                s_cost[w][lane] = 0.125 + (float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa;
+                L.cost[lane] = 0.125 + (float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa;
             }
             wave_lds_sync();
             uint32_t split = 0;
-            float best = s_cost[w][0];
+            float best = L.cost[0];
             for (int k = 1; k < int(kNumBuckets) - 1; ++k) {
-                const float ck = s_cost[w][k];
+                const float ck = L.cost[k];
                 if (ck < best) { best = ck; split = k; }
             }
             left = active && bk <= split;
@@ -624,49 +659,100 @@ __global__ __launch_bounds__(kFinBlock) void finish_kernel(BuildArgs A, const Wo
         } else {
             const float mid = sel3(box_centroid(cb), axis);
             left = active && sel3(c, axis) < mid;
-            lp = 0;
         }
         const uint64_t lmask = __ballot(left);
         const uint64_t rmask = __ballot(active && !left);
-        const uint32_t L = __popcll(lmask);
-        if (L == 0 || L == m) {
+        const uint32_t Lc = __popcll(lmask);
+        if (Lc == 0 || Lc == m) {
             if (TOP) {
                 if (lane == 0) atomicOr(A.err, uint32_t(kErrSplit));
                 continue;
             }
-            if (lane == 0) write_leaf(A.nodes, id, fb, it.begin + lo, m);
-            if (active) A.prim_order[it.begin + lane] = r.a;
+            if (lane == 0) write_leaf(A.nodes, id, fb, begin + lo, m);  // all centroids on one side
+            if (active) A.prim_order[begin + lane] = r.a;
             continue;
         }
-        if (!TOP) lp = L;
+        if (!TOP) lp = Lc;
         if (active) {
             const uint32_t dst = left ? lo + __popcll(lmask & lanemask_lt())
-                                      : lo + L + __popcll(rmask & lanemask_lt());
-            s_tmp[w][dst] = r;
+                                      : lo + Lc + __popcll(rmask & lanemask_lt());
+            L.tmp[dst] = r;
         }
         wave_lds_sync();
-        if (active) s_rec[w][lane] = s_tmp[w][lane];
+        if (active) L.rec[lane] = L.tmp[lane];
         if (lane == 0) write_interior(A.nodes, id, fb, lp, axis);
-        const uint4 ch[2] = {make_uint4(lo, lo + L, id + 1, lp), make_uint4(lo + L, hi, id + 2 * lp, P - lp)};
+        const uint4 ch[2] = {make_uint4(lo, lo + Lc, id + 1, lp), make_uint4(lo + Lc, hi, id + 2 * lp, P - lp)};
         wave_lds_sync();
         for (int k = 1; k >= 0; --k) {
             const uint4 cn = ch[k];
             if (cn.y - cn.x == 1) {
-                if (lane == 0) {
-                    const Rec q = s_rec[w][cn.x];
-                    if (TOP) emit_bottom_root(A, q, cn.z);
-                    else {
-                        write_leaf(A.nodes, cn.z, rec_box(q), it.begin + cn.x, 1);
-                        A.prim_order[it.begin + cn.x] = q.a;
-                    }
+                if (TOP) {
+                    if (lane == 0) L.leaf[nleaf] = make_uint2(cn.z, cn.x);
+                    ++nleaf;
+                } else if (lane == 0) {
+                    const Rec q = L.rec[cn.x];
+                    write_leaf(A.nodes, cn.z, rec_box(q), begin + cn.x, 1);
+                    A.prim_order[begin + cn.x] = q.a;
                 }
             } else {
-                if (lane == 0) s_stack[w][sp] = cn;
+                if (lane == 0) L.stack[sp] = cn;
                 ++sp;
             }
         }
         wave_lds_sync();
     }
+    return nleaf;
+}
+
+// Top-level finisher: one wave per node of <= 64 treelets. After the SAH
+// subtree, each single-treelet leaf becomes the root of a midpoint subtree over
+// its primitives, built in the same wave when it has <= 64 of them (the common
+// case) and queued for the level-synchronous path otherwise.
+__global__ __launch_bounds__(kFinBlock) void finish_top_kernel(BuildArgs A, const WorkItem *items, uint32_t nitems) {
+    __shared__ WaveLds s_lds[kFinWaves];
+    const int w = threadIdx.x / kWave;
+    const uint32_t lane = __lane_id();
+    const uint32_t wid = blockIdx.x * kFinWaves + w;
+    if (wid >= nitems) return;
+    WaveLds &L = s_lds[w];
+    const WorkItem it = items[wid];
+    const uint32_t nleaf = wave_build<true>(A, pick(A.trec, it.buf), it.begin, it.end - it.begin, it.node, it.prims, L);
+    // lane k keeps leaf k: node id and the treelet's primitive range
+    uint32_t my_node = 0, my_first = 0, my_count = 0;
+    if (lane < nleaf) {
+        const uint2 lf = L.leaf[lane];
+        const Rec t = L.rec[lf.y];
+        my_node = lf.x;
+        my_first = t.a;
+        my_count = t.b;
+        if (my_count == 1) {
+            const Rec p = A.prec[0][my_first];
+            write_leaf(A.nodes, my_node, rec_box(p), my_first, 1);
+            A.prim_order[my_first] = p.a;
+        } else if (my_count > kSmallNode) {
+            push_item(A.large_bot[0], A.counters + kCntLargeBot0, A.cap_large_bot,
+                      WorkItem{my_first, my_first + my_count, my_node, my_count, 0, {0, 0, 0}}, A.err);
+        }
+    }
+    wave_lds_sync();
+    for (uint32_t k = 0; k < nleaf; ++k) {
+        const uint32_t node = __shfl(my_node, k, kWave);
+        const uint32_t first = __shfl(my_first, k, kWave);
+        const uint32_t count = __shfl(my_count, k, kWave);
+        if (count < 2 || count > kSmallNode) continue;
+        wave_build<false>(A, A.prec[0], first, count, node, count, L);
+        wave_lds_sync();
+    }
+}
+
+// Bottom-level finisher for nodes of <= 64 primitives left by the level path.
+__global__ __launch_bounds__(kFinBlock) void finish_bottom_kernel(BuildArgs A, const WorkItem *items, uint32_t nitems) {
+    __shared__ WaveLds s_lds[kFinWaves];
+    const int w = threadIdx.x / kWave;
+    const uint32_t wid = blockIdx.x * kFinWaves + w;
+    if (wid >= nitems) return;
+    const WorkItem it = items[wid];
+    wave_build<false>(A, pick(A.prec, it.buf), it.begin, it.end - it.begin, it.node, it.prims, s_lds[w]);
 }
 
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v) {
@@ -845,7 +931,7 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     PBRTM_TRY(read_counters(S, stream));
     out.small_top = S.host_counters[kCntSmallTop];
     if (out.small_top > 0)
-        finish_kernel<true><<<(out.small_top + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
+        finish_top_kernel<<<(out.small_top + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
             A, A.small_top, out.small_top);
     if (times) hipEventRecord(times->ev[3], stream);
     PBRTM_TRY(run_levels<false>(A, S, stream, prec_ptrs, kCntLargeBot0, kCntLargeBot1, A.large_bot[0],
@@ -853,7 +939,7 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     PBRTM_TRY(read_counters(S, stream));
     out.small_bottom = S.host_counters[kCntSmallBot];
     if (out.small_bottom > 0)
-        finish_kernel<false><<<(out.small_bottom + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
+        finish_bottom_kernel<<<(out.small_bottom + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
             A, A.small_bot, out.small_bottom);
     if (times) hipEventRecord(times->ev[4], stream);
     PBRTM_TRY(hipGetLastError());

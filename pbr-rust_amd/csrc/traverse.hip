@@ -20,11 +20,14 @@ constexpr int kStack = 64;
 
 __device__ __forceinline__ f3 load3(const float *p) { return {p[0], p[1], p[2]}; }
 
+// COUNT adds per-launch totals of box tests and triangle tests (for the
+// algorithmic-bytes roofline); the traversal itself is identical.
+template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void trace_binary_kernel(
     const BNode *__restrict__ nodes, uint32_t num_nodes, uint32_t num_prims, const float *__restrict__ tri_leaf,
     const uint32_t *__restrict__ prim_order, const float4 *__restrict__ rays, uint32_t n, int any_hit,
     uint32_t *__restrict__ prim_out, float *__restrict__ t_out, float *__restrict__ bary_out,
-    uint32_t *__restrict__ err) {
+    uint32_t *__restrict__ err, unsigned long long *__restrict__ counts) {
     __shared__ uint32_t s_stack[kStack][kTraceBlock];
     const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= n) return;
@@ -33,6 +36,7 @@ __global__ __launch_bounds__(kTraceBlock) void trace_binary_kernel(
     float best_t = r0.w;
     uint32_t best = kNoHit;
     TriHit hit{0.f, 0.f, 0.f, 0.f};
+    uint32_t visits = 0, tests = 0;
     if (best_t > 0.f) {
         const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
         const int neg[3] = {inv.x < 0.f, inv.y < 0.f, inv.z < 0.f};
@@ -44,6 +48,7 @@ __global__ __launch_bounds__(kTraceBlock) void trace_binary_kernel(
                 break;
             }
             const BNode nd = nodes[cur];
+            if (COUNT) ++visits;
             if (box_hit(nd.lo, nd.hi, o, inv, neg, best_t)) {
                 if (nd.a & kLeafFlag) {
                     const uint32_t first = nd.a & ~kLeafFlag;
@@ -54,6 +59,7 @@ __global__ __launch_bounds__(kTraceBlock) void trace_binary_kernel(
                     bool done = false;
                     for (uint32_t k = first; k < first + nd.b; ++k) {
                         const float *t = tri_leaf + 9ull * k;
+                        if (COUNT) ++tests;
                         TriHit h;
                         if (tri_hit(o, d, best_t, load3(t), load3(t + 3), load3(t + 6), &h)) {
                             best = prim_order[k];
@@ -78,6 +84,13 @@ __global__ __launch_bounds__(kTraceBlock) void trace_binary_kernel(
             }
             if (sp == 0) break;
             cur = s_stack[--sp][threadIdx.x];
+        }
+    }
+    if (COUNT) {
+        const unsigned long long v = wave_sum(visits), tt = wave_sum(tests);
+        if (__lane_id() == 0) {
+            atomicAdd(&counts[0], v);
+            atomicAdd(&counts[1], tt);
         }
     }
     prim_out[i] = best;
@@ -175,11 +188,19 @@ __global__ __launch_bounds__(256) void ao_bounce_kernel(const float *__restrict_
 }  // namespace
 
 hipError_t trace_binary(const BvhTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
-                        float *t_out, float *bary_out, uint32_t *err, hipStream_t stream) {
+                        float *t_out, float *bary_out, uint32_t *err, unsigned long long *counts,
+                        hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    trace_binary_kernel<<<(n + kTraceBlock - 1) / kTraceBlock, kTraceBlock, 0, stream>>>(
-        B.nodes, 2 * B.num_prims - 1, B.num_prims, B.tri_leaf, B.prim_order, reinterpret_cast<const float4 *>(rays),
-        n, any_hit, prim_out, t_out, bary_out, err);
+    const uint32_t grid = (n + kTraceBlock - 1) / kTraceBlock;
+    const float4 *r = reinterpret_cast<const float4 *>(rays);
+    if (counts)
+        trace_binary_kernel<true><<<grid, kTraceBlock, 0, stream>>>(B.nodes, 2 * B.num_prims - 1, B.num_prims,
+                                                                    B.tri_leaf, B.prim_order, r, n, any_hit,
+                                                                    prim_out, t_out, bary_out, err, counts);
+    else
+        trace_binary_kernel<false><<<grid, kTraceBlock, 0, stream>>>(B.nodes, 2 * B.num_prims - 1, B.num_prims,
+                                                                     B.tri_leaf, B.prim_order, r, n, any_hit,
+                                                                     prim_out, t_out, bary_out, err, nullptr);
     return hipGetLastError();
 }
 

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_uint8, c_uint32, c_uint64, c_void_p
 
 import numpy as np
@@ -75,7 +76,7 @@ def load(path: str = LIB_PATH):
         "pbrtm_bvh_free": (None, [p]),
         "pbrtm_bvh_bounds": (c_int, [p, p]),
         "pbrtm_bvh_export_canonical": (c_int, [p, p, POINTER(c_uint64)]),
-        "pbrtm_bvh_intersect": (c_int, [p, p, c_uint64, c_int, c_uint32, p, p, p]),
+        "pbrtm_bvh_intersect": (c_int, [p, p, c_uint64, c_int, c_uint32, p, p, p, p]),
         "pbrtm_ao_bounce": (c_int, [p, p, p, p, c_uint64, c_uint32, p]),
         "pbrtm_gen_soup": (c_int, [c_uint64, c_uint64, c_float, c_float, c_int, p]),
         "pbrtm_gen_camera_rays": (c_int, [c_uint32, c_uint32, p, p, p, c_float, p]),
@@ -134,9 +135,13 @@ class Context:
         if rc != OK:
             raise PbrtmError(rc, f"cannot create a context on device {device}")
         self.h = h
+        self._handles = weakref.WeakSet()
 
     def close(self):
+        """Frees every aggregate built on this context, then the context."""
         if self.h:
+            for b in list(self._handles):
+                b.free()
             self.lib.pbrtm_destroy(self.h)
             self.h = None
 
@@ -181,7 +186,9 @@ class Context:
         self.check(self.lib.pbrtm_bvh_build(self.h, _ptr(verts), nv, _ptr(idx), nt,
                                             DEVICE_PTRS if device_ptrs else 0, ctypes.byref(h),
                                             ctypes.byref(stats)))
-        return Bvh(self, h, nt, stats, keep=verts if device_ptrs else None)
+        bvh = Bvh(self, h, nt, stats, keep=verts if device_ptrs else None)
+        self._handles.add(bvh)
+        return bvh
 
 
 class Bvh:
@@ -214,7 +221,10 @@ class Bvh:
         self.ctx.check(self.ctx.lib.pbrtm_bvh_export_canonical(self.h, _ptr(buf), ctypes.byref(ln)))
         return buf.tobytes()
 
-    def intersect(self, rays, any_hit=False, device_ptrs=False, prim=None, t=None, bary=None, want_bary=True):
+    def intersect(self, rays, any_hit=False, device_ptrs=False, prim=None, t=None, bary=None, want_bary=True,
+                  stats=None):
+        """Closest (or any) hit per ray. `stats`, a uint64[2] numpy array, receives
+        [box tests, triangle tests] when given."""
         n = rays.shape[0]
         if not device_ptrs:
             prim = np.empty(n, np.uint32)
@@ -222,7 +232,7 @@ class Bvh:
             bary = np.empty((n, 3), np.float32) if want_bary else None
         self.ctx.check(self.ctx.lib.pbrtm_bvh_intersect(self.h, _ptr(rays), n, int(any_hit),
                                                         DEVICE_PTRS if device_ptrs else 0, _ptr(prim),
-                                                        _ptr(t), _ptr(bary)))
+                                                        _ptr(t), _ptr(bary), _ptr(stats)))
         return prim, t, bary
 
     def ao_bounce(self, rays, prim, bary, device_ptrs=False, out=None):

@@ -34,7 +34,7 @@ def test_null_arguments_are_errors_not_crashes():
     assert lib.pbrtm_bvh_bounds(None, None) == pbrtm.ERR_INVALID
     ln = ctypes.c_uint64(0)
     assert lib.pbrtm_bvh_export_canonical(None, None, ctypes.byref(ln)) == pbrtm.ERR_INVALID
-    assert lib.pbrtm_bvh_intersect(None, None, 0, 0, 0, None, None, None) == pbrtm.ERR_INVALID
+    assert lib.pbrtm_bvh_intersect(None, None, 0, 0, 0, None, None, None, None) == pbrtm.ERR_INVALID
     assert lib.pbrtm_last_error(None) == b"null context"
 
 

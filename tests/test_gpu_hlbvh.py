@@ -2,8 +2,10 @@
 
 Bit-exact: Morton codes and canonical order; the canonical tree (node sets,
 split axes, bound bit patterns); closest-hit primitive ids, t and barycentrics
-on identical rays; any-hit booleans. Bounce rays (which go through cosf/sinf)
-are compared within 2e-6 relative, then traced bit-exactly as given.
+on identical rays; any-hit booleans. Bounce rays go through cosf/sinf, whose
+results may differ by an ulp between libms: origins are compared within 2e-6
+relative and directions by angle (2e-5 rad unless grazing), then the GPU's
+bounce rays are traced by both sides and compared bit-exactly.
 """
 import numpy as np
 import pytest
@@ -32,6 +34,12 @@ def clustered(n, seed, far=True):
     return np.ascontiguousarray(tri)
 
 
+def flat(tri):
+    """All vertices on the plane z = 0.5: zero extent on one axis."""
+    tri[:, :, 2] = np.float32(0.5)
+    return tri
+
+
 CASES = {
     "n1": lambda: soup(1, 0),
     "n2": lambda: soup(2, 1),
@@ -41,7 +49,7 @@ CASES = {
     "n200k": lambda: soup(200_000, 5),
     "cluster3000": lambda: clustered(3000, 6),
     "cluster_dups": lambda: np.concatenate([clustered(500, 7), np.repeat(soup(1, 8), 200, axis=0)]),
-    "flat_z": lambda: (lambda t: (t.__setitem__((slice(None), slice(None), 2), np.float32(0.5)), t)[1])(soup(20000, 9)),
+    "flat_z": lambda: flat(soup(20000, 9)),
 }
 
 
@@ -83,9 +91,11 @@ def test_intersect_and_bounce_parity_1m(ctx):
     log("intersect 1M primary")
     bvh = ctx.build(tri)
     ob = O.OracleBvh(tri)
-    prim, t, bary = bvh.intersect(rays)
-    op, ot, obary, _ = ob.intersect(rays)
+    stats = np.zeros(2, np.uint64)
+    prim, t, bary = bvh.intersect(rays, stats=stats)
+    op, ot, obary, ostats = ob.intersect(rays)
     assert (op != O.NO_HIT).mean() > 0.1
+    assert np.array_equal(stats, ostats)  # same tree, same visit order
     assert np.array_equal(prim, op)
     assert np.array_equal(t, ot)
     assert np.array_equal(bary, obary)
@@ -95,7 +105,16 @@ def test_intersect_and_bounce_parity_1m(ctx):
     active = obr[:, 3] > 0
     assert np.array_equal(br[:, 3] > 0, active)
     assert np.allclose(br[active, :3], obr[active, :3], rtol=2e-6, atol=1e-7)
-    assert np.allclose(br[active, 4:7], obr[active, 4:7], rtol=0, atol=2e-6)
+    # directions go through cosf/sinf (1-ulp differences between libms) and
+    # z = sqrt(1 - x^2 - y^2), which amplifies them by x/z near grazing angles
+    diff = np.abs(br[active, 4:7].astype(np.float64) - obr[active, 4:7]).max(axis=1)
+    P = tri[op[active]].astype(np.float64)
+    n = np.cross(P[:, 0] - P[:, 2], P[:, 1] - P[:, 2])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    cos_n = np.abs(np.sum(n * obr[active, 4:7], axis=1))
+    log(f"bounce direction: max |diff| {diff.max():.3g}, {np.mean(diff > 0):.2e} not bit-equal")
+    assert diff[cos_n > 0.05].max() < 5e-6
+    assert diff.max() < 1e-3 and np.mean(diff > 1e-6) < 1e-3
     log("intersect bounce rays")
     p2, t2, b2 = bvh.intersect(br)
     q2, u2, c2, _ = ob.intersect(br)
