@@ -20,7 +20,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "pbr-rust_amd"))
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 32      # binary BVH node (box + two words)
+WNODE_BYTES = 260    # 8-wide node (256 B) + its octant order word (4 B), per visit
+BOX_BYTES = 24       # one child box re-read when a leaf shortened the ray
 TRI_BYTES = 36       # leaf-ordered triangle, 9 floats
 RAY_BYTES = 32
 BUILD_BYTES_PER_TRI = 216  # algorithmic bytes of an HLBVH build per triangle
@@ -96,7 +97,7 @@ def main():
         bvh = ctx.build(d_tri, device_ptrs=True)
         s1 = s2 = None
         if counts is not None:
-            s1, s2 = np.zeros(2, np.uint64), np.zeros(2, np.uint64)
+            s1, s2 = np.zeros(3, np.uint64), np.zeros(3, np.uint64)
         ev[0].record(stream)
         bvh.intersect(d_rays, device_ptrs=True, prim=d_prim, t=d_t, bary=d_bary, stats=s1)
         ev[1].record(stream)
@@ -130,7 +131,7 @@ def main():
     counts = []
     step(counts)
     torch.cuda.synchronize()
-    (v1, t1), (v2, t2) = counts[0]
+    (v1, b1, t1), (v2, b2, t2) = (tuple(int(x) for x in s) for s in counts[0])
     active2 = int((d_bounce[:, 3] > 0).sum().item())
     hits1 = int((d_prim != -1).sum().item())
     hits2 = int((d_prim2 != -1).sum().item())
@@ -142,18 +143,22 @@ def main():
     rays_traced = nr + active2
     mrays = rays_traced / (trace_ms * 1e-3) / 1e6
     # primary rays write prim + t + barycentrics (20 B), bounce rays prim + t (8 B)
-    bytes1 = nr * (RAY_BYTES + 20) + int(v1) * NODE_BYTES + int(t1) * TRI_BYTES
-    bytes2 = nr * (RAY_BYTES + 8) + int(v2) * NODE_BYTES + int(t2) * TRI_BYTES
+    def node_bytes(v, b):
+        return v * WNODE_BYTES + (b - 8 * v) * BOX_BYTES
+
+    bytes1 = nr * (RAY_BYTES + 20) + node_bytes(v1, b1) + t1 * TRI_BYTES
+    bytes2 = nr * (RAY_BYTES + 8) + node_bytes(v2, b2) + t2 * TRI_BYTES
     trace_gbs = (bytes1 + bytes2) / (trace_ms * 1e-3) / 1e9
     build_gbs = BUILD_BYTES_PER_TRI * args.tris / (build_ms * 1e-3) / 1e9
 
     # Dominant kernel by measured time: the closest-hit traversal when the
     # two traversals outweigh the build.
     if trace_ms >= build_ms:
-        roof = {"kernel": "trace_binary_kernel", "bound": "hbm", "achieved": round(trace_gbs, 1),
+        roof = {"kernel": "trace_wide_kernel", "bound": "hbm", "achieved": round(trace_gbs, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(trace_gbs / HBM_PEAK_GBS, 4),
                 "traffic": None,
-                "algorithmic_bytes": "32 B ray + 20/8 B hit + 32 B per box test + 36 B per triangle test"}
+                "algorithmic_bytes": "32 B ray + 20/8 B hit + 260 B per 8-wide node visit + 24 B per box "
+                                     "re-test + 36 B per triangle test"}
     else:
         roof = {"kernel": "hlbvh build (all phases)", "bound": "hbm", "achieved": round(build_gbs, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
@@ -180,8 +185,9 @@ def main():
                      "secondary": round(float(np.mean(sec_ms)), 3)},
         "mrays_per_s": round(mrays, 1),
         "rays": {"primary": nr, "primary_hits": hits1, "bounce": active2, "bounce_hits": hits2,
-                 "box_tests_per_ray": round((int(v1) + int(v2)) / rays_traced, 2),
-                 "tri_tests_per_ray": round((int(t1) + int(t2)) / rays_traced, 2)},
+                 "node_visits_per_ray": round((v1 + v2) / rays_traced, 2),
+                 "box_tests_per_ray": round((b1 + b2) / rays_traced, 2),
+                 "tri_tests_per_ray": round((t1 + t2) / rays_traced, 2)},
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

@@ -39,6 +39,10 @@ extern "C" {
 #define PBRTM_ERR_UNSUPPORTED (-5)
 
 #define PBRTM_DEVICE_PTRS 1u
+/* pbrtm_bvh_intersect: traverse the binary build tree in the reference's node
+ * order instead of the 8-wide layout (same hits; visit counts comparable with
+ * the reference's). */
+#define PBRTM_TRACE_BINARY 2u
 
 #define PBRTM_NO_HIT 0xFFFFFFFFu
 
@@ -52,13 +56,15 @@ typedef struct pbrtm_build_stats {
     double ms_treelets;   /* sorted records, treelet emission */
     double ms_top;        /* SAH over treelets */
     double ms_bottom;     /* midpoint splits inside treelets */
-    double ms_layout;     /* traversal layout (leaf-ordered triangles) */
+    double ms_layout;     /* traversal layout (leaf-ordered triangles, 8-wide collapse) */
     uint64_t num_prims;
     uint64_t num_treelets;
     uint64_t top_levels;      /* level-synchronous steps for nodes > 64 treelets */
     uint64_t bottom_levels;   /* level-synchronous steps for nodes > 64 primitives */
     uint64_t small_top;       /* wave-finished top subtrees */
     uint64_t small_bottom;    /* wave-finished bottom subtrees */
+    uint64_t wide_nodes;      /* 8-wide traversal nodes */
+    uint64_t wide_depth;      /* wide levels on the longest path */
 } pbrtm_build_stats;
 
 /* No counterpart in the reference (its objects live in one process-wide CUDA
@@ -105,8 +111,10 @@ int pbrtm_bvh_export_canonical(const pbrtm_bvh *bvh, uint8_t *buf, uint64_t *len
 /* Batched ray queries. any_hit = 0 replaces HLBVH::intersect (closest hit,
  * hlbvh.cu:231-289); any_hit = 1 replaces HLBVH::fast_intersect (hlbvh.cu:179-229),
  * whose bool is prim_out != PBRTM_NO_HIT. t_out and bary_out ([n][3]) may be NULL.
- * stats_out (host, may be NULL) receives [box tests, triangle tests] summed over
- * the rays; it selects a counting instance of the same traversal. */
+ * The 8-wide traversal tests triangles in exactly the reference's order, so hits
+ * (ties included) equal the binary traversal's. stats_out (host, may be NULL)
+ * receives [nodes visited, box tests, triangle tests] summed over the rays; it
+ * selects a counting instance of the same traversal. */
 int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_hit, uint32_t flags,
                         uint32_t *prim_out, float *t_out, float *bary_out, uint64_t *stats_out);
 

@@ -14,6 +14,7 @@
 #include "hlbvh_build.h"
 #include "pbrtm_geometry.h"
 #include "traverse.h"
+#include "wide_bvh.h"
 
 using namespace pbrtm;
 
@@ -21,7 +22,7 @@ struct pbrtm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     BuildScratch scratch;
-    DevBuf stage_in, stage_a, stage_b, stage_c, dev_err;
+    DevBuf stage_in, stage_a, stage_b, stage_c, dev_err, spill, collapse;
     hipEvent_t ev[8] = {};
     std::string error;
 };
@@ -31,6 +32,7 @@ struct pbrtm_bvh {
     int device = 0;
     uint32_t n = 0;
     BvhDevice dev;
+    WideBvh wide;
     DevBuf tri_owned, tri_leaf;
     const float *tri = nullptr;  // device, input order (owned or caller's)
     float bounds[6];
@@ -165,6 +167,8 @@ void pbrtm_destroy(pbrtm_ctx *ctx) {
     ctx->stage_b.release();
     ctx->stage_c.release();
     ctx->dev_err.release();
+    ctx->spill.release();
+    ctx->collapse.release();
     for (auto &e : ctx->ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -252,11 +256,19 @@ int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint3
         if (e == hipSuccess)
             e = make_leaf_triangles(bvh->tri, bvh->dev.prim_order.as<uint32_t>(), uint32_t(nt),
                                     bvh->tri_leaf.as<float>(), ctx->stream);
+        if (e == hipSuccess) e = ctx->dev_err.reserve(64);
+        if (e == hipSuccess) e = hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream);
+        if (e == hipSuccess)
+            e = collapse_wide(bvh->dev.nodes.as<BNode>(), uint32_t(nt), bvh->wide, ctx->collapse,
+                              ctx->dev_err.as<uint32_t>(), ctx->stream);
         if (e == hipSuccess) e = hipEventRecord(ctx->ev[6], ctx->stream);
+        uint32_t cerr = 0;
+        if (e == hipSuccess) e = hipMemcpyAsync(&cerr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream);
         BNode root;
         if (e == hipSuccess) e = hipMemcpyAsync(&root, bvh->dev.nodes.ptr, sizeof(BNode), hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) return bail(hip_fail(ctx, e, "layout"));
+        if (cerr) return bail(fail(ctx, PBRTM_ERR_DEVICE, device_error_text(cerr)));
         for (int k = 0; k < 3; ++k) {
             bvh->bounds[k] = root.lo[k];
             bvh->bounds[3 + k] = root.hi[k];
@@ -279,6 +291,8 @@ int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint3
         stats->bottom_levels = bvh->dev.bottom_levels;
         stats->small_top = bvh->dev.small_top;
         stats->small_bottom = bvh->dev.small_bottom;
+        stats->wide_nodes = bvh->wide.num_nodes;
+        stats->wide_depth = bvh->wide.max_depth;
         stats->ms_total =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -293,6 +307,8 @@ void pbrtm_bvh_free(pbrtm_bvh *bvh) {
     bvh->dev.prim_order.release();
     bvh->tri_owned.release();
     bvh->tri_leaf.release();
+    bvh->wide.nodes.release();
+    bvh->wide.order.release();
     delete bvh;
 }
 
@@ -381,14 +397,34 @@ int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_h
     CK(ctx, ctx->dev_err.reserve(64), "alloc");
     CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 64, ctx->stream), "memset");
     unsigned long long *d_counts = stats_out ? (unsigned long long *)(ctx->dev_err.as<uint8_t>() + 16) : nullptr;
+    const bool binary = (flags & PBRTM_TRACE_BINARY) != 0;
     if (bvh->n == 0) {
         CK(ctx, hipMemsetAsync(d_prim, 0xFF, n * 4, ctx->stream), "memset");
         if (d_t) CK(ctx, hipMemsetAsync(d_t, 0, n * 4, ctx->stream), "memset");
         if (d_b) CK(ctx, hipMemsetAsync(d_b, 0, n * 12, ctx->stream), "memset");
     } else {
-        BvhTrace B{bvh->dev.nodes.as<BNode>(), bvh->tri_leaf.as<float>(), bvh->dev.prim_order.as<uint32_t>(), bvh->n};
-        CK(ctx, trace_binary(B, (const float *)d_rays, uint32_t(n), any_hit, d_prim, d_t, d_b,
-                             ctx->dev_err.as<uint32_t>(), d_counts, ctx->stream), "trace");
+        if (binary) {
+            BvhTrace B{bvh->dev.nodes.as<BNode>(), bvh->tri_leaf.as<float>(), bvh->dev.prim_order.as<uint32_t>(),
+                       bvh->n};
+            CK(ctx, trace_binary(B, (const float *)d_rays, uint32_t(n), any_hit, d_prim, d_t, d_b,
+                                 ctx->dev_err.as<uint32_t>(), d_counts, ctx->stream), "trace");
+        } else {
+            WideTrace W{};
+            W.nodes = bvh->wide.nodes.as<WNode>();
+            W.order = bvh->wide.order.as<uint32_t>();
+            W.tri_leaf = bvh->tri_leaf.as<float>();
+            W.prim_order = bvh->dev.prim_order.as<uint32_t>();
+            W.num_nodes = bvh->wide.num_nodes;
+            W.num_prims = bvh->n;
+            W.max_depth = bvh->wide.max_depth;
+            W.spill_per_ray = W.max_depth > kLdsStack ? W.max_depth - kLdsStack + 1 : 0;
+            if (W.spill_per_ray) {
+                CK(ctx, ctx->spill.reserve(size_t(n) * W.spill_per_ray * sizeof(uint2)), "alloc stack spill");
+                W.spill = ctx->spill.as<uint2>();
+            }
+            CK(ctx, trace_wide(W, (const float *)d_rays, uint32_t(n), any_hit, d_prim, d_t, d_b,
+                               ctx->dev_err.as<uint32_t>(), d_counts, ctx->stream), "trace");
+        }
     }
     if (!dev) {
         CK(ctx, hipMemcpyAsync(prim_out, d_prim, n * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
@@ -398,7 +434,7 @@ int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_h
     uint32_t derr = 0;
     CK(ctx, hipMemcpyAsync(&derr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "download");
     if (stats_out)
-        CK(ctx, hipMemcpyAsync(stats_out, d_counts, 16, hipMemcpyDeviceToHost, ctx->stream), "download");
+        CK(ctx, hipMemcpyAsync(stats_out, d_counts, 24, hipMemcpyDeviceToHost, ctx->stream), "download");
     CK(ctx, hipStreamSynchronize(ctx->stream), "sync");
     if (derr) return fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr));
     return PBRTM_OK;

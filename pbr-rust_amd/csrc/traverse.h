@@ -16,7 +16,8 @@ struct BvhTrace {
 };
 
 // rays: [n][8] floats (o.xyz, t_max, d.xyz, unused); t_max <= 0 marks an inactive ray.
-// counts (device, nullable): adds [box tests, triangle tests] of this launch.
+// counts (device, nullable): adds [nodes visited, box tests, triangle tests] of this
+// launch; in the binary traversal a visit is one box test.
 hipError_t trace_binary(const BvhTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
                         float *t_out, float *bary_out, uint32_t *err, unsigned long long *counts,
                         hipStream_t stream);

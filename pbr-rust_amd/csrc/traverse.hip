@@ -90,7 +90,8 @@ __global__ __launch_bounds__(kTraceBlock) void trace_binary_kernel(
         const unsigned long long v = wave_sum(visits), tt = wave_sum(tests);
         if (__lane_id() == 0) {
             atomicAdd(&counts[0], v);
-            atomicAdd(&counts[1], tt);
+            atomicAdd(&counts[1], v);
+            atomicAdd(&counts[2], tt);
         }
     }
     prim_out[i] = best;

@@ -1,0 +1,55 @@
+// 8-wide BVH collapsed from the binary build tree, and its traversal.
+#pragma once
+
+#include "pbrtm_device.h"
+#include "pbrtm_internal.h"
+
+namespace pbrtm {
+
+constexpr int kWide = 8;
+constexpr uint32_t kWideLeaf = 0x80000000u;
+constexpr uint32_t kWideEmpty = 0xFFFFFFFFu;
+
+// 256 bytes = two 128-byte cache lines: child boxes as SoA planes, then one
+// (word0, word1) reference per child slot. Interior child: word0 = wide node
+// index, word1 = 0. Leaf child: word0 = kWideLeaf | first leaf position,
+// word1 = primitive count. Unused slot: word0 = kWideEmpty, empty box.
+// Slots hold the collapsed binary subtree's frontier in left-to-right order.
+struct alignas(128) WNode {
+    float lo_x[kWide], lo_y[kWide], lo_z[kWide];
+    float hi_x[kWide], hi_y[kWide], hi_z[kWide];
+    uint32_t ref[kWide][2];
+};
+static_assert(sizeof(WNode) == 256, "WNode is two cache lines");
+
+// Per wide node and ray octant (bit k = direction k negative): the order in
+// which the reference's depth-first, near-child-first binary traversal reaches
+// the slots, as 8 nibbles (position p -> slot), 0xF past the last slot.
+struct WideBvh {
+    DevBuf nodes, order;
+    uint32_t num_nodes = 0;
+    uint32_t max_depth = 0;   // wide levels on the longest root-to-leaf path
+    uint32_t collapse_levels = 0;
+};
+
+// Collapses the binary tree (preorder-reserved ids, root 0, n primitives).
+hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &scratch, uint32_t *err,
+                         hipStream_t stream);
+
+struct WideTrace {
+    const WNode *nodes;
+    const uint32_t *order;
+    const float *tri_leaf;
+    const uint32_t *prim_order;
+    uint32_t num_nodes, num_prims, max_depth;
+    uint2 *spill;           // per-ray stack overflow area (max_depth - kLdsStack entries per ray), may be null
+    uint32_t spill_per_ray;
+};
+
+// counts (device, nullable): adds [wide nodes visited, child box tests, triangle tests].
+hipError_t trace_wide(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
+                      float *t_out, float *bary_out, uint32_t *err, unsigned long long *counts, hipStream_t stream);
+
+constexpr uint32_t kLdsStack = 12;
+
+}  // namespace pbrtm

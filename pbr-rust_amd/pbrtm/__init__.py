@@ -24,6 +24,7 @@ ERR_DEVICE = -3
 ERR_NOMEM = -4
 ERR_UNSUPPORTED = -5
 DEVICE_PTRS = 1
+TRACE_BINARY = 2
 NO_HIT = 0xFFFFFFFF
 
 # Every symbol include/pbrtm.h declares.
@@ -40,7 +41,7 @@ class BuildStats(ctypes.Structure):
         ("ms_total", c_double), ("ms_sort", c_double), ("ms_treelets", c_double), ("ms_top", c_double),
         ("ms_bottom", c_double), ("ms_layout", c_double), ("num_prims", c_uint64),
         ("num_treelets", c_uint64), ("top_levels", c_uint64), ("bottom_levels", c_uint64),
-        ("small_top", c_uint64), ("small_bottom", c_uint64),
+        ("small_top", c_uint64), ("small_bottom", c_uint64), ("wide_nodes", c_uint64), ("wide_depth", c_uint64),
     ]
 
     def as_dict(self):
@@ -222,16 +223,17 @@ class Bvh:
         return buf.tobytes()
 
     def intersect(self, rays, any_hit=False, device_ptrs=False, prim=None, t=None, bary=None, want_bary=True,
-                  stats=None):
-        """Closest (or any) hit per ray. `stats`, a uint64[2] numpy array, receives
-        [box tests, triangle tests] when given."""
+                  stats=None, binary=False):
+        """Closest (or any) hit per ray. `stats`, a uint64[3] numpy array, receives
+        [nodes visited, box tests, triangle tests] when given. binary=True walks the
+        binary build tree in the reference's order instead of the 8-wide layout."""
         n = rays.shape[0]
         if not device_ptrs:
             prim = np.empty(n, np.uint32)
             t = np.empty(n, np.float32)
             bary = np.empty((n, 3), np.float32) if want_bary else None
-        self.ctx.check(self.ctx.lib.pbrtm_bvh_intersect(self.h, _ptr(rays), n, int(any_hit),
-                                                        DEVICE_PTRS if device_ptrs else 0, _ptr(prim),
+        flags = (DEVICE_PTRS if device_ptrs else 0) | (TRACE_BINARY if binary else 0)
+        self.ctx.check(self.ctx.lib.pbrtm_bvh_intersect(self.h, _ptr(rays), n, int(any_hit), flags, _ptr(prim),
                                                         _ptr(t), _ptr(bary), _ptr(stats)))
         return prim, t, bary
 

@@ -91,11 +91,18 @@ def test_intersect_and_bounce_parity_1m(ctx):
     log("intersect 1M primary")
     bvh = ctx.build(tri)
     ob = O.OracleBvh(tri)
-    stats = np.zeros(2, np.uint64)
+    stats, bstats = np.zeros(3, np.uint64), np.zeros(3, np.uint64)
     prim, t, bary = bvh.intersect(rays, stats=stats)
     op, ot, obary, ostats = ob.intersect(rays)
     assert (op != O.NO_HIT).mean() > 0.1
-    assert np.array_equal(stats, ostats)  # same tree, same visit order
+    # binary traversal: the same node visits and triangle tests as the oracle
+    bp, bt, bb = bvh.intersect(rays, stats=bstats, binary=True)
+    assert np.array_equal(bstats[[0, 2]], ostats)
+    assert np.array_equal(bp, op) and np.array_equal(bt, ot) and np.array_equal(bb, obary)
+    # 8-wide traversal: fewer node visits, exactly the same triangle tests
+    log(f"wide stats {stats.tolist()} binary {bstats.tolist()}")
+    assert stats[2] == ostats[1]
+    assert stats[0] < ostats[0] / 2
     assert np.array_equal(prim, op)
     assert np.array_equal(t, ot)
     assert np.array_equal(bary, obary)
@@ -116,10 +123,13 @@ def test_intersect_and_bounce_parity_1m(ctx):
     assert diff[cos_n > 0.05].max() < 5e-6
     assert diff.max() < 1e-3 and np.mean(diff > 1e-6) < 1e-3
     log("intersect bounce rays")
-    p2, t2, b2 = bvh.intersect(br)
-    q2, u2, c2, _ = ob.intersect(br)
+    s2 = np.zeros(3, np.uint64)
+    p2, t2, b2 = bvh.intersect(br, stats=s2)
+    q2, u2, c2, os2 = ob.intersect(br)
     assert np.array_equal(p2, q2)
     assert np.array_equal(t2, u2)
+    assert np.array_equal(b2, c2)
+    assert s2[2] == os2[1]
     log("any hit")
     pa, _, _ = bvh.intersect(br, any_hit=True, want_bary=False)
     qa, _, _, _ = ob.intersect(br, any_hit=True)
