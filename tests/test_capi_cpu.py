@@ -66,3 +66,15 @@ def test_gen_camera_rays_geometry():
     # vertical field of view 45 degrees on a pinhole: y/z of row 0 is (1 - 1/36) tan(22.5 deg)
     top = d.reshape(36, 64, 3)[0, 32]
     assert abs(top[1] / top[2] - (1 - 1 / 36) * np.tan(np.radians(22.5))) < 1e-6
+
+
+def test_cpp_interface_compiles_and_fails_cleanly_without_gpu():
+    """host/hlbvh.hpp compiles against the C ABI; without a device the example
+    reports an error instead of crashing."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "pbr-rust_amd", "hlbvh_example")
+    assert os.path.exists(exe)
+    if pbrtm.device_count() == 0:
+        out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+        assert out.returncode == 1 and "pbrtm_create failed" in out.stderr

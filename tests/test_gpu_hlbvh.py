@@ -188,3 +188,16 @@ def test_indexed_mesh_equals_soup(ctx):
     bad[5, 0] = 500
     with pytest.raises(pbrtm.PbrtmError):
         ctx.build(verts, idx=bad)
+
+
+def test_cpp_interface_example():
+    """The C++ interface (host/hlbvh.hpp) end to end: a unit cube, six rays."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(pbrtm.LIB_PATH), "hlbvh_example")
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.strip().splitlines()
+    assert lines[0] == "bounds 0 0 0 1 1 1"
+    assert lines[1:] == ["1 1 1", "3 1 1", "8 1 1", "11 1 1", "5 1 1", "-1 0 0"]
