@@ -22,7 +22,7 @@ struct pbrtm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     BuildScratch scratch;
-    DevBuf stage_in, stage_a, stage_b, stage_c, dev_err, spill, collapse;
+    DevBuf stage_in, stage_a, stage_b, stage_c, dev_err, spill, collapse, queue;
     hipEvent_t ev[8] = {};
     std::string error;
 };
@@ -169,6 +169,7 @@ void pbrtm_destroy(pbrtm_ctx *ctx) {
     ctx->dev_err.release();
     ctx->spill.release();
     ctx->collapse.release();
+    ctx->queue.release();
     for (auto &e : ctx->ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -252,7 +253,7 @@ int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint3
         hipError_t e = build_hlbvh_device(bvh->tri, uint32_t(nt), ctx->scratch, bvh->dev, &derr, ctx->stream, &times);
         if (e != hipSuccess) return bail(hip_fail(ctx, e, "build"));
         if (derr) return bail(fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr)));
-        e = bvh->tri_leaf.reserve(nt * 36);
+        e = bvh->tri_leaf.reserve(nt * 48);
         if (e == hipSuccess)
             e = make_leaf_triangles(bvh->tri, bvh->dev.prim_order.as<uint32_t>(), uint32_t(nt),
                                     bvh->tri_leaf.as<float>(), ctx->stream);
@@ -422,8 +423,10 @@ int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_h
                 CK(ctx, ctx->spill.reserve(size_t(n) * W.spill_per_ray * sizeof(uint2)), "alloc stack spill");
                 W.spill = ctx->spill.as<uint2>();
             }
+            CK(ctx, ctx->queue.reserve(size_t(n) * 4), "alloc ray queue");
             CK(ctx, trace_wide(W, (const float *)d_rays, uint32_t(n), any_hit, d_prim, d_t, d_b,
-                               ctx->dev_err.as<uint32_t>(), d_counts, ctx->stream), "trace");
+                               ctx->dev_err.as<uint32_t>(), d_counts, ctx->queue.as<uint32_t>(),
+                               ctx->dev_err.as<uint32_t>() + 1, ctx->stream), "trace");
         }
     }
     if (!dev) {

@@ -384,47 +384,83 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_buckets_kernel(const uint2 
     }
 }
 
-// SAH cost of splitting after bucket s (hlbvh.cu:711-731); left/right unions of
-// bucket boxes are exact, so prefix and suffix scans give the same floats.
-__device__ float sah_cost(const Box *bb, const uint32_t *bc, int split, float total_sa) {
-    Box bl = box_empty(), br = box_empty();
-    uint32_t cl = 0, cr = 0;
-    for (int k = 0; k <= split; ++k) { bl = box_union(bl, bb[k]); cl += bc[k]; }
-    for (int k = split + 1; k < int(kNumBuckets); ++k) { br = box_union(br, bb[k]); cr += bc[k]; }
-    return 0.125 + (float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa;
-}
+// Split choice of one large node. Top level: one wave per node, lane k holds
+// bucket k; inclusive prefix and suffix unions (exact min/max, so equal to the
+// reference's per-split loops, hlbvh.cu:711-731) give lane k the cost of
+// splitting after bucket k; lane 0 then takes the first strict minimum
+// (hlbvh.cu:734-741). Bottom level: the midpoint of the centroid box.
+__device__ __forceinline__ float shfl_up_f(float v, int d) { return __shfl_up(v, d, kWave); }
+__device__ __forceinline__ float shfl_down_f(float v, int d) { return __shfl_down(v, d, kWave); }
 
 template <bool TOP>
 __global__ __launch_bounds__(64) void decide_kernel(const WorkItem *items, uint32_t nitems, const Acc *acc,
                                                     Split *splits) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nitems) return;
-    const Acc &a = acc[i];
+    __shared__ float s_cost[kNumBuckets];
+    const uint32_t item = TOP ? blockIdx.x : blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems) return;
+    const Acc &a = acc[item];
     const Box cb = acc_box(a.cb);
     Split sp{};
     sp.axis = box_max_dim(cb);
     sp.split_val = sel3(box_centroid(cb), sp.axis);
     sp.mode = 1;
-    if (TOP) {
-        const Box fb = acc_box(a.fb);
-        Box bb[kNumBuckets];
-        for (int k = 0; k < int(kNumBuckets); ++k) bb[k] = acc_box(a.bbox[k]);
-        const float total_sa = box_area(fb);
-        float best = sah_cost(bb, a.bcnt, 0, total_sa);
-        uint32_t split = 0;
-        for (int s = 1; s < int(kNumBuckets) - 1; ++s) {
-            const float c = sah_cost(bb, a.bcnt, s, total_sa);
-            if (c < best) { best = c; split = s; }
+    if (!TOP) {
+        splits[item] = sp;
+        return;
+    }
+    const int lane = threadIdx.x;
+    const bool in = lane < int(kNumBuckets);
+    const uint32_t cnt = in ? a.bcnt[lane] : 0u;
+    // an untouched bucket holds the atomic identities, which decode to NaN: use the empty box
+    Box b = cnt > 0 ? acc_box(a.bbox[lane]) : box_empty();
+    // inclusive prefix (left) and suffix (right) unions and counts over buckets
+    float pl[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+    float sf[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+    uint32_t pc = cnt, sc = cnt;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        const bool up = lane >= d;
+        const bool dn = lane + d < int(kNumBuckets);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const float u = shfl_up_f(pl[k], d);
+            const float w = shfl_down_f(sf[k], d);
+            if (up) pl[k] = k < 3 ? smin(u, pl[k]) : smax(u, pl[k]);
+            if (dn) sf[k] = k < 3 ? smin(sf[k], w) : smax(sf[k], w);
         }
-        uint32_t lp = 0, rp = 0;
-        for (int k = 0; k < int(kNumBuckets); ++k) (k <= int(split) ? lp : rp) += a.bcnt[k];
+        const uint32_t uc = __shfl_up(pc, d, kWave), wc = __shfl_down(sc, d, kWave);
+        if (up) pc += uc;
+        if (dn) sc += wc;
+    }
+    // right side of split k = suffix starting at bucket k + 1
+    float r6[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r6[k] = shfl_down_f(sf[k], 1);
+    const uint32_t rc = __shfl_down(sc, 1, kWave);
+    const float total_sa = box_area(acc_box(a.fb));
+    if (lane < int(kNumBuckets) - 1) {
+        const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
+        const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
+        s_cost[lane] = 0.125 + (float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa;
+    }
+    __syncthreads();
+    uint32_t split = 0;
+    if (lane == 0) {
+        float best = s_cost[0];
+        for (int k = 1; k < int(kNumBuckets) - 1; ++k)
+            if (s_cost[k] < best) { best = s_cost[k]; split = k; }
+    }
+    split = __shfl(split, 0, kWave);
+    const uint32_t lp = __shfl(pc, int(split), kWave);
+    const uint32_t rp = __shfl(rc, int(split), kWave);
+    if (lane == 0) {
         sp.bucket = split;
         sp.base = sel3(cb.lo, sp.axis);
         sp.span = sel3(cb.hi, sp.axis) - sel3(cb.lo, sp.axis);
         sp.left_prims = lp;
         sp.mode = (lp == 0 || rp == 0) ? 1u : 0u;  // midpoint fallback (hlbvh.cu:758-777)
+        splits[item] = sp;
     }
-    splits[i] = sp;
 }
 
 template <bool TOP>
@@ -704,6 +740,81 @@ __device__ uint32_t wave_build(const BuildArgs &A, const Rec *src, uint32_t begi
     return nleaf;
 }
 
+// Midpoint subtree of one small treelet built serially by one lane (the common
+// case: a few primitives per treelet). Records are partitioned in place in the
+// primitive buffer, which this treelet owns; the pending-node stack lives in
+// four 64-bit registers as 16-bit entries (lo, hi, node offset).
+constexpr uint32_t kLaneBuild = 16;
+
+__device__ __forceinline__ Rec load_rec(const Rec *p) {
+    const float4 a = reinterpret_cast<const float4 *>(p)[0];
+    const float4 b = reinterpret_cast<const float4 *>(p)[1];
+    Rec r;
+    r.lo[0] = a.x; r.lo[1] = a.y; r.lo[2] = a.z; r.a = __builtin_bit_cast(uint32_t, a.w);
+    r.hi[0] = b.x; r.hi[1] = b.y; r.hi[2] = b.z; r.b = __builtin_bit_cast(uint32_t, b.w);
+    return r;
+}
+
+__device__ void lane_build_bottom(const BuildArgs &A, uint32_t first, uint32_t count, uint32_t node0) {
+    Rec *R = A.prec[0] + first;
+    uint64_t st[4] = {0, 0, 0, 0};
+    int sp = 0;
+    auto push = [&](uint32_t lo, uint32_t hi, uint32_t off) {
+        st[3] = (st[3] << 16) | (st[2] >> 48);
+        st[2] = (st[2] << 16) | (st[1] >> 48);
+        st[1] = (st[1] << 16) | (st[0] >> 48);
+        st[0] = (st[0] << 16) | uint64_t(lo | (hi << 5) | (off << 10));
+        ++sp;
+    };
+    push(0, count, 0);
+    while (sp > 0) {
+        const uint32_t e = uint32_t(st[0] & 0xFFFFu);
+        st[0] = (st[0] >> 16) | (st[1] << 48);
+        st[1] = (st[1] >> 16) | (st[2] << 48);
+        st[2] = (st[2] >> 16) | (st[3] << 48);
+        st[3] >>= 16;
+        --sp;
+        const uint32_t lo = e & 31u, hi = (e >> 5) & 31u, off = e >> 10;
+        const uint32_t id = node0 + off;
+        if (hi - lo == 1) {
+            const Rec q = load_rec(R + lo);
+            write_leaf(A.nodes, id, rec_box(q), first + lo, 1);
+            A.prim_order[first + lo] = q.a;
+            continue;
+        }
+        Box cb = box_empty(), fb = box_empty();
+        for (uint32_t k = lo; k < hi; ++k) {
+            const Box b = rec_box(load_rec(R + k));
+            cb = box_add(cb, box_centroid(b));
+            fb = box_union(fb, b);
+        }
+        const int axis = box_max_dim(cb);
+        const float mid = sel3(box_centroid(cb), axis);
+        // two-pointer partition on centroid[axis] < mid (hlbvh.cu:828-857 predicate)
+        uint32_t i = lo, j = hi;
+        while (i < j) {
+            const Rec ri = load_rec(R + i);
+            if (sel3(box_centroid(rec_box(ri)), axis) < mid) {
+                ++i;
+            } else {
+                --j;
+                const Rec rj = load_rec(R + j);
+                R[i] = rj;
+                R[j] = ri;
+            }
+        }
+        const uint32_t L = i - lo;
+        if (L == 0 || L == hi - lo) {  // all centroids on one side: a multi-primitive leaf
+            write_leaf(A.nodes, id, fb, first + lo, hi - lo);
+            for (uint32_t k = lo; k < hi; ++k) A.prim_order[first + k] = R[k].a;
+            continue;
+        }
+        write_interior(A.nodes, id, fb, L, axis);
+        push(lo + L, hi, off + 2 * L);
+        push(lo, lo + L, off + 1);
+    }
+}
+
 // Top-level finisher: one wave per node of <= 64 treelets. After the SAH
 // subtree, each single-treelet leaf becomes the root of a midpoint subtree over
 // its primitives, built in the same wave when it has <= 64 of them (the common
@@ -734,12 +845,17 @@ __global__ __launch_bounds__(kFinBlock) void finish_top_kernel(BuildArgs A, cons
                       WorkItem{my_first, my_first + my_count, my_node, my_count, 0, {0, 0, 0}}, A.err);
         }
     }
+    // small treelets: one lane each, all in parallel
+    if (lane < nleaf && my_count >= 2 && my_count <= kLaneBuild) lane_build_bottom(A, my_first, my_count, my_node);
     wave_lds_sync();
-    for (uint32_t k = 0; k < nleaf; ++k) {
+    // medium treelets: the whole wave, one after another
+    uint64_t medium = __ballot(lane < nleaf && my_count > kLaneBuild && my_count <= kSmallNode);
+    while (medium) {
+        const int k = __builtin_ctzll(medium);
+        medium &= medium - 1;
         const uint32_t node = __shfl(my_node, k, kWave);
         const uint32_t first = __shfl(my_first, k, kWave);
         const uint32_t count = __shfl(my_count, k, kWave);
-        if (count < 2 || count > kSmallNode) continue;
         wave_build<false>(A, A.prec[0], first, count, node, count, L);
         wave_lds_sync();
     }
@@ -813,7 +929,7 @@ static hipError_t run_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, 
         acc_init_kernel<<<ncur, 256, 0, stream>>>(acc, ncur);
         chunk_reduce_kernel<<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, acc);
         if (TOP) chunk_buckets_kernel<<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, acc, A.err);
-        decide_kernel<TOP><<<(ncur + 63) / 64, 64, 0, stream>>>(cur, ncur, acc, splits);
+        decide_kernel<TOP><<<TOP ? ncur : (ncur + 63) / 64, 64, 0, stream>>>(cur, ncur, acc, splits);
         chunk_count_kernel<TOP><<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, splits, tleft, tlp);
         PBRTM_TRY(exclusive_scan_u32(tleft, tleft, ntasks, tleft + ntasks, S.scan_scratch, stream));
         if (TOP) PBRTM_TRY(exclusive_scan_u32(tlp, tlp, ntasks, tlp + ntasks, S.scan_scratch, stream));

@@ -7,7 +7,7 @@
 namespace pbrtm {
 
 // Everything a traversal kernel reads: binary nodes, leaf-ordered triangles
-// (9 floats per leaf position) and the leaf position -> primitive id map.
+// (9 floats in a 12-float slot per leaf position) and the leaf position -> primitive id map.
 struct BvhTrace {
     const BNode *nodes;
     const float *tri_leaf;

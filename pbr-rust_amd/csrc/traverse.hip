@@ -58,10 +58,12 @@ __global__ __launch_bounds__(kTraceBlock) void trace_binary_kernel(
                     }
                     bool done = false;
                     for (uint32_t k = first; k < first + nd.b; ++k) {
-                        const float *t = tri_leaf + 9ull * k;
+                        const float4 *t4 = reinterpret_cast<const float4 *>(tri_leaf) + 3ull * k;
+                        const float4 q0 = t4[0], q1 = t4[1], q2 = t4[2];
                         if (COUNT) ++tests;
                         TriHit h;
-                        if (tri_hit(o, d, best_t, load3(t), load3(t + 3), load3(t + 6), &h)) {
+                        if (tri_hit(o, d, best_t, f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, f3{q1.z, q1.w, q2.x},
+                                    &h)) {
                             best = prim_order[k];
                             hit = h;
                             best_t = h.t;
@@ -103,7 +105,8 @@ __global__ __launch_bounds__(kTraceBlock) void trace_binary_kernel(
     }
 }
 
-// Triangles of leaf position k = input triangle prim_order[k].
+// Triangles of leaf position k = input triangle prim_order[k], 48-byte stride
+// (three 16-byte loads per triangle).
 __global__ __launch_bounds__(256) void leaf_triangles_kernel(const float *__restrict__ tri,
                                                              const uint32_t *__restrict__ prim_order, uint32_t n,
                                                              float *__restrict__ tri_leaf) {
@@ -111,9 +114,10 @@ __global__ __launch_bounds__(256) void leaf_triangles_kernel(const float *__rest
     if (k >= n) return;
     const uint32_t p = prim_order[k];
     const float *s = tri + 9ull * (p < n ? p : 0u);  // p < n for every complete tree
-    float *d = tri_leaf + 9ull * k;
-#pragma unroll
-    for (int j = 0; j < 9; ++j) d[j] = s[j];
+    float4 *d = reinterpret_cast<float4 *>(tri_leaf) + 3ull * k;
+    d[0] = make_float4(s[0], s[1], s[2], s[3]);
+    d[1] = make_float4(s[4], s[5], s[6], s[7]);
+    d[2] = make_float4(s[8], 0.f, 0.f, 0.f);
 }
 
 __global__ __launch_bounds__(256) void deindex_kernel(const float *__restrict__ verts, const uint32_t *__restrict__ idx,

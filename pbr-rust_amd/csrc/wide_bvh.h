@@ -47,8 +47,11 @@ struct WideTrace {
 };
 
 // counts (device, nullable): adds [wide nodes visited, child box tests, triangle tests].
+// queue (device, n entries) and queue_len (device, zeroed by the caller) hold the
+// compacted active rays.
 hipError_t trace_wide(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
-                      float *t_out, float *bary_out, uint32_t *err, unsigned long long *counts, hipStream_t stream);
+                      float *t_out, float *bary_out, uint32_t *err, unsigned long long *counts, uint32_t *queue,
+                      uint32_t *queue_len, hipStream_t stream);
 
 constexpr uint32_t kLdsStack = 12;
 

@@ -56,53 +56,79 @@ __global__ void wide_root_kernel(const BNode *bn, Job *jobs, uint32_t *counters,
     counters[1] = 1;
 }
 
-__global__ __launch_bounds__(64) void wide_collapse_kernel(const BNode *__restrict__ bn, const Job *__restrict__ jobs,
-                                                           uint32_t njobs, Job *__restrict__ next,
-                                                           uint32_t *__restrict__ counters, WNode *__restrict__ wn,
-                                                           uint32_t *__restrict__ order, uint32_t cap,
-                                                           uint32_t *__restrict__ err) {
+constexpr int kCollapseBlock = 64;
+
+// Frontier state lives in registers: every array below is indexed only with
+// compile-time indices (fully unrolled loops with selects).
+__global__ __launch_bounds__(kCollapseBlock) void wide_collapse_kernel(
+    const BNode *__restrict__ bn, const Job *__restrict__ jobs, uint32_t njobs, Job *__restrict__ next,
+    uint32_t *__restrict__ counters, WNode *__restrict__ wn, uint32_t *__restrict__ order, uint32_t cap,
+    uint32_t *__restrict__ err) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < njobs;
-    Job job = live ? jobs[i] : Job{0, 0, 0, 0};
-    uint32_t F[kWide];   // frontier binary ids, left to right
-    uint32_t E[kWide];   // opened binary interior nodes (the collapsed subtree's inner nodes)
+    const Job job = live ? jobs[i] : Job{0, 0, 0, 0};
+    uint32_t fid[kWide];      // frontier binary ids, left to right
+    float farea[kWide];       // surface area, or -1 for a leaf / unused slot
+    uint32_t oid[kWide - 1], ol[kWide - 1], orr[kWide - 1], oax[kWide - 1];  // opened nodes
+#pragma unroll
+    for (int k = 0; k < kWide; ++k) { fid[k] = 0u; farea[k] = -1.0f; }
+#pragma unroll
+    for (int k = 0; k < kWide - 1; ++k) { oid[k] = 0xFFFFFFFFu; ol[k] = orr[k] = oax[k] = 0u; }
     int nF = 0, nE = 0;
     if (live) {
         const BNode r = bn[job.bnode];
-        E[nE++] = job.bnode;
-        F[nF++] = r.a;
-        F[nF++] = r.b & 0x3FFFFFFFu;
-        while (nF < kWide) {
+        oid[0] = job.bnode; ol[0] = r.a; orr[0] = r.b & 0x3FFFFFFFu; oax[0] = r.b >> 30;
+        nE = 1;
+        const BNode c0 = bn[ol[0]], c1 = bn[orr[0]];
+        fid[0] = ol[0]; farea[0] = bnode_leaf(c0) ? -1.0f : box_area(bnode_box(c0));
+        fid[1] = orr[0]; farea[1] = bnode_leaf(c1) ? -1.0f : box_area(bnode_box(c1));
+        nF = 2;
+        while (nF < kWide) {  // open the frontier interior node of largest surface area
             int best = -1;
             float best_area = -1.0f;
-            for (int k = 0; k < nF; ++k) {
-                const BNode c = bn[F[k]];
-                if (bnode_leaf(c)) continue;
-                const float a = box_area(bnode_box(c));
-                if (a > best_area) { best_area = a; best = k; }
-            }
+#pragma unroll
+            for (int k = 0; k < kWide; ++k)
+                if (farea[k] > best_area) { best_area = farea[k]; best = k; }
             if (best < 0) break;
-            const BNode c = bn[F[best]];
-            E[nE++] = F[best];
-            for (int k = nF; k > best + 1; --k) F[k] = F[k - 1];
-            F[best] = c.a;
-            F[best + 1] = c.b & 0x3FFFFFFFu;
+            uint32_t bid = 0;
+#pragma unroll
+            for (int k = 0; k < kWide; ++k) bid = (k == best) ? fid[k] : bid;
+            const BNode c = bn[bid];
+            const uint32_t cl = c.a, cr = c.b & 0x3FFFFFFFu;
+            const BNode nl = bn[cl], nr = bn[cr];
+#pragma unroll
+            for (int k = 0; k < kWide - 1; ++k) {
+                const bool here = k == nE;
+                oid[k] = here ? bid : oid[k];
+                ol[k] = here ? cl : ol[k];
+                orr[k] = here ? cr : orr[k];
+                oax[k] = here ? (c.b >> 30) : oax[k];
+            }
+            ++nE;
+            const float al = bnode_leaf(nl) ? -1.0f : box_area(bnode_box(nl));
+            const float ar = bnode_leaf(nr) ? -1.0f : box_area(bnode_box(nr));
+#pragma unroll
+            for (int k = kWide - 1; k >= 0; --k) {  // shift right of best by one, then place the children
+                const uint32_t prev_id = k > 0 ? fid[k - 1] : 0u;
+                const float prev_a = k > 0 ? farea[k - 1] : -1.0f;
+                if (k > best + 1) { fid[k] = prev_id; farea[k] = prev_a; }
+                else if (k == best + 1) { fid[k] = cr; farea[k] = ar; }
+                else if (k == best) { fid[k] = cl; farea[k] = al; }
+            }
             ++nF;
         }
     }
     // interior children need wide node indices: one atomic per wave
     uint32_t n_int = 0;
-    bool is_int[kWide];
-    for (int s = 0; s < kWide; ++s) {
-        is_int[s] = live && s < nF && !bnode_leaf(bn[F[s]]);
-        n_int += is_int[s] ? 1u : 0u;
-    }
+#pragma unroll
+    for (int s = 0; s < kWide; ++s) n_int += (s < nF && farea[s] >= 0.0f) ? 1u : 0u;
     const uint32_t incl = wave_inclusive_add(n_int);
-    uint32_t base = 0;
-    if (__lane_id() == kWave - 1 && incl > 0) base = atomicAdd(&counters[1], incl);
+    uint32_t base = 0, jbase = 0;
+    if (__lane_id() == kWave - 1 && incl > 0) {
+        base = atomicAdd(&counters[1], incl);
+        jbase = atomicAdd(&counters[2], incl);
+    }
     base = __shfl(base, kWave - 1, kWave) + incl - n_int;
-    uint32_t jbase = 0;
-    if (__lane_id() == kWave - 1 && incl > 0) jbase = atomicAdd(&counters[2], incl);
     jbase = __shfl(jbase, kWave - 1, kWave) + incl - n_int;
     if (!live) return;
     if (base + n_int > cap) {
@@ -111,14 +137,15 @@ __global__ __launch_bounds__(64) void wide_collapse_kernel(const BNode *__restri
     }
     WNode w;
     uint32_t k_int = 0;
+#pragma unroll
     for (int s = 0; s < kWide; ++s) {
         if (s < nF) {
-            const BNode c = bn[F[s]];
+            const BNode c = bn[fid[s]];
             w.lo_x[s] = c.lo[0]; w.lo_y[s] = c.lo[1]; w.lo_z[s] = c.lo[2];
             w.hi_x[s] = c.hi[0]; w.hi_y[s] = c.hi[1]; w.hi_z[s] = c.hi[2];
-            if (is_int[s]) {
+            if (farea[s] >= 0.0f) {
                 const uint32_t idx = base + k_int;
-                next[jbase + k_int] = Job{F[s], idx, job.depth + 1, 0};
+                next[jbase + k_int] = Job{fid[s], idx, job.depth + 1, 0};
                 ++k_int;
                 w.ref[s][0] = idx;
                 w.ref[s][1] = 0u;
@@ -134,33 +161,49 @@ __global__ __launch_bounds__(64) void wide_collapse_kernel(const BNode *__restri
         }
     }
     wn[job.wnode] = w;
-    // depth-first, near-first slot order per octant (the binary traversal's order)
+    // depth-first, near-first slot order per octant (the binary traversal's order).
+    // Codes: 0..7 = frontier slot, 8..14 = opened node; the DFS stack is eight
+    // 4-bit codes packed in one register.
+    uint32_t lcode[kWide - 1], rcode[kWide - 1];
+#pragma unroll
+    for (int k = 0; k < kWide - 1; ++k) {
+        uint32_t lc = 0xFu, rc = 0xFu;
+#pragma unroll
+        for (int j = 0; j < kWide - 1; ++j) {
+            if (j < nE && oid[j] == ol[k]) lc = 8u + j;
+            if (j < nE && oid[j] == orr[k]) rc = 8u + j;
+        }
+#pragma unroll
+        for (int s = 0; s < kWide; ++s) {
+            if (s < nF && fid[s] == ol[k]) lc = s;
+            if (s < nF && fid[s] == orr[k]) rc = s;
+        }
+        lcode[k] = lc;
+        rcode[k] = rc;
+    }
     for (uint32_t oct = 0; oct < 8; ++oct) {
-        uint32_t st[kWide];
-        int sp = 0;
-        st[sp++] = job.bnode;
-        uint32_t perm = 0xFFFFFFFFu;
-        int pos = 0;
+        uint32_t st = 8u;  // root = opened node 0
+        int sp = 1;
+        uint32_t pm = 0xFFFFFFFFu;
+        int p = 0;
         while (sp > 0) {
-            const uint32_t x = st[--sp];
-            bool opened = false;
-            for (int k = 0; k < nE; ++k) opened |= E[k] == x;
-            if (opened) {
-                const BNode b = bn[x];
-                const uint32_t axis = b.b >> 30;
-                const bool neg = (oct >> axis) & 1u;
-                const uint32_t l = b.a, r = b.b & 0x3FFFFFFFu;
-                st[sp++] = neg ? l : r;  // far child waits
-                st[sp++] = neg ? r : l;
+            const uint32_t c = st & 0xFu;
+            st >>= 4;
+            --sp;
+            if (c >= 8u) {
+                uint32_t l = 0, r = 0, ax = 0;
+#pragma unroll
+                for (int k = 0; k < kWide - 1; ++k)
+                    if (uint32_t(k) == c - 8u) { l = lcode[k]; r = rcode[k]; ax = oax[k]; }
+                const bool neg = (oct >> ax) & 1u;
+                st = (st << 8) | ((neg ? r : l) & 0xFu) | (((neg ? l : r) & 0xFu) << 4);  // near on top
+                sp += 2;
             } else {
-                uint32_t slot = 0;
-                for (int k = 0; k < nF; ++k)
-                    if (F[k] == x) slot = k;
-                perm = (perm & ~(0xFu << (4 * pos))) | (slot << (4 * pos));
-                ++pos;
+                pm = (pm & ~(0xFu << (4 * p))) | (c << (4 * p));
+                ++p;
             }
         }
-        order[job.wnode * 8 + oct] = perm;
+        order[job.wnode * 8 + oct] = pm;
     }
     atomicMax(&counters[3], job.depth);
 }
@@ -191,17 +234,49 @@ __device__ __forceinline__ float box_entry(float lx, float ly, float lz, float h
 
 constexpr int kWideBlock = 256;
 
+// Ray queue compaction: active rays (t_max > 0) are appended to `queue` with
+// one atomic per workgroup (wave ballots + a workgroup prefix), inactive ones
+// get their miss written here. The traversal then runs on full waves.
+__global__ __launch_bounds__(kWideBlock) void compact_rays_kernel(const float4 *__restrict__ rays, uint32_t n,
+                                                                  uint32_t *__restrict__ queue,
+                                                                  uint32_t *__restrict__ count,
+                                                                  uint32_t *__restrict__ prim_out,
+                                                                  float *__restrict__ t_out,
+                                                                  float *__restrict__ bary_out) {
+    __shared__ uint32_t s_waves[kWideBlock / kWave];
+    __shared__ uint32_t s_base;
+    const uint32_t i = blockIdx.x * kWideBlock + threadIdx.x;
+    const bool in = i < n;
+    const bool act = in && rays[2 * i].w > 0.f;
+    if (in && !act) {
+        prim_out[i] = kNoHit;
+        if (t_out) t_out[i] = 0.f;
+        if (bary_out) {
+            bary_out[3 * i] = 0.f;
+            bary_out[3 * i + 1] = 0.f;
+            bary_out[3 * i + 2] = 0.f;
+        }
+    }
+    uint32_t tot;
+    const uint32_t pre = block_exclusive_add<kWideBlock>(act ? 1u : 0u, s_waves, &tot);
+    if (threadIdx.x == 0) s_base = tot ? atomicAdd(count, tot) : 0u;
+    __syncthreads();
+    if (act) queue[s_base + pre] = i;
+}
+
 template <bool COUNT>
 __global__ __launch_bounds__(kWideBlock) void trace_wide_kernel(WideTrace B, const float4 *__restrict__ rays,
-                                                                uint32_t n, int any_hit,
+                                                                const uint32_t *__restrict__ queue,
+                                                                const uint32_t *__restrict__ queue_len, int any_hit,
                                                                 uint32_t *__restrict__ prim_out,
                                                                 float *__restrict__ t_out,
                                                                 float *__restrict__ bary_out,
                                                                 uint32_t *__restrict__ err,
                                                                 unsigned long long *__restrict__ counts) {
     __shared__ uint2 s_stack[kLdsStack][kWideBlock];
-    const uint32_t i = blockIdx.x * kWideBlock + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t qi = blockIdx.x * kWideBlock + threadIdx.x;
+    if (qi >= *queue_len) return;
+    const uint32_t i = queue[qi];
     const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
     const f3 o = {r0.x, r0.y, r0.z}, d = {r1.x, r1.y, r1.z};
     float best_t = r0.w;
@@ -269,10 +344,11 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide_kernel(WideTrace B, con
                         break;
                     }
                     for (uint32_t k = first; k < first + cnt; ++k) {
-                        const float *t = B.tri_leaf + 9ull * k;
+                        const float4 *t4 = reinterpret_cast<const float4 *>(B.tri_leaf) + 3ull * k;
+                        const float4 q0 = t4[0], q1 = t4[1], q2 = t4[2];
                         if (COUNT) ++tests;
                         TriHit h;
-                        if (tri_hit(o, d, best_t, f3{t[0], t[1], t[2]}, f3{t[3], t[4], t[5]}, f3{t[6], t[7], t[8]},
+                        if (tri_hit(o, d, best_t, f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, f3{q1.z, q1.w, q2.x},
                                     &h)) {
                             best = B.prim_order[k];
                             hit = h;
@@ -287,7 +363,7 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide_kernel(WideTrace B, con
                     if (sp < kLdsStack) {
                         s_stack[sp][threadIdx.x] = make_uint2(node, pmask);
                     } else if (sp - kLdsStack < B.spill_per_ray) {
-                        B.spill[uint64_t(i) * B.spill_per_ray + (sp - kLdsStack)] = make_uint2(node, pmask);
+                        B.spill[uint64_t(qi) * B.spill_per_ray + (sp - kLdsStack)] = make_uint2(node, pmask);
                     } else {
                         atomicOr(err, uint32_t(kErrStack));
                         done = true;
@@ -304,7 +380,7 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide_kernel(WideTrace B, con
             if (sp == 0) break;
             --sp;
             const uint2 e = sp < kLdsStack ? s_stack[sp][threadIdx.x]
-                                           : B.spill[uint64_t(i) * B.spill_per_ray + (sp - kLdsStack)];
+                                           : B.spill[uint64_t(qi) * B.spill_per_ray + (sp - kLdsStack)];
             node = e.x;
             pmask = e.y;
         }
@@ -358,8 +434,8 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
         if (++out.collapse_levels > 100000) return hipErrorUnknown;
         e = hipMemsetAsync(counters + 2, 0, 4, stream);
         if (e != hipSuccess) return e;
-        wide_collapse_kernel<<<(nj + 63) / 64, 64, 0, stream>>>(bnodes, jobs[cur], nj, jobs[1 - cur], counters, wn,
-                                                                 order, cap, err);
+        wide_collapse_kernel<<<(nj + kCollapseBlock - 1) / kCollapseBlock, kCollapseBlock, 0, stream>>>(
+            bnodes, jobs[cur], nj, jobs[1 - cur], counters, wn, order, cap, err);
         e = hipMemcpyAsync(counters, counters + 2, 4, hipMemcpyDeviceToDevice, stream);
         if (e != hipSuccess) return e;
         cur = 1 - cur;
@@ -370,16 +446,18 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
 }
 
 hipError_t trace_wide(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
-                      float *t_out, float *bary_out, uint32_t *err, unsigned long long *counts, hipStream_t stream) {
+                      float *t_out, float *bary_out, uint32_t *err, unsigned long long *counts, uint32_t *queue,
+                      uint32_t *queue_len, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     const uint32_t grid = (n + kWideBlock - 1) / kWideBlock;
     const float4 *r = reinterpret_cast<const float4 *>(rays);
+    compact_rays_kernel<<<grid, kWideBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
     if (counts)
-        trace_wide_kernel<true><<<grid, kWideBlock, 0, stream>>>(B, r, n, any_hit, prim_out, t_out, bary_out, err,
-                                                                 counts);
+        trace_wide_kernel<true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
+                                                                 bary_out, err, counts);
     else
-        trace_wide_kernel<false><<<grid, kWideBlock, 0, stream>>>(B, r, n, any_hit, prim_out, t_out, bary_out, err,
-                                                                  nullptr);
+        trace_wide_kernel<false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
+                                                                  bary_out, err, nullptr);
     return hipGetLastError();
 }
 
