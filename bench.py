@@ -22,7 +22,10 @@ sys.path.insert(0, os.path.join(ROOT, "pbr-rust_amd"))
 HBM_PEAK_GBS = 8000.0
 WNODE_BYTES = 260    # 8-wide node (256 B) + its octant order word (4 B), per visit
 BOX_BYTES = 24       # one child box re-read when a leaf shortened the ray
-TRI_BYTES = 36       # leaf-ordered triangle, 9 floats
+TRI_BYTES = 48       # leaf-ordered triangle, 9 floats in a 48-byte slot
+# HBM bytes per trace_wide_kernel launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
+# scaled by a plain copy kernel; None until measured.
+TRAFFIC_BYTES_PER_LAUNCH = None
 RAY_BYTES = 32
 BUILD_BYTES_PER_TRI = 216  # algorithmic bytes of an HLBVH build per triangle
 
@@ -38,6 +41,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0x5EED)
     ap.add_argument("--cpu-sample-tris", type=int, default=2_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one plain 256 MiB device copy (for scaling profiler byte counters)")
     return ap.parse_args()
 
 
@@ -151,18 +156,15 @@ def main():
     trace_gbs = (bytes1 + bytes2) / (trace_ms * 1e-3) / 1e9
     build_gbs = BUILD_BYTES_PER_TRI * args.tris / (build_ms * 1e-3) / 1e9
 
-    # Dominant kernel by measured time: the closest-hit traversal when the
-    # two traversals outweigh the build.
-    if trace_ms >= build_ms:
-        roof = {"kernel": "trace_wide_kernel", "bound": "hbm", "achieved": round(trace_gbs, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(trace_gbs / HBM_PEAK_GBS, 4),
-                "traffic": None,
-                "algorithmic_bytes": "32 B ray + 20/8 B hit + 260 B per 8-wide node visit + 24 B per box "
-                                     "re-test + 36 B per triangle test"}
-    else:
-        roof = {"kernel": "hlbvh build (all phases)", "bound": "hbm", "achieved": round(build_gbs, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
-                "traffic": None, "algorithmic_bytes": f"{BUILD_BYTES_PER_TRI} B per triangle"}
+    # The dominant single kernel of the step (rocprof: trace_wide_kernel, ~2 ms per
+    # launch; the build is ~40 smaller kernels). Achieved = algorithmic bytes of the
+    # two launches / their HIP-event durations on the library stream.
+    roof = {"kernel": "trace_wide_kernel", "bound": "hbm", "achieved": round(trace_gbs, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(trace_gbs / HBM_PEAK_GBS, 4),
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+            "bytes_per_launch": int((bytes1 + bytes2) / 2),
+            "algorithmic_bytes": "32 B ray + 20/8 B hit + 260 B per 8-wide node visit + 24 B per box "
+                                 "re-test + 48 B per triangle test"}
 
     line = {
         "metric": "HLBVH build ms @10M tris",
@@ -180,6 +182,7 @@ def main():
         "config": {"workload": "hlbvh build + 1080p primary + 1 diffuse bounce", "tris": args.tris,
                    "width": args.width, "height": args.height, "seed": args.seed, "mu": -6.0, "sigma": 0.5},
         "build_phases_ms": phase,
+        "build_gbs_algorithmic": round(build_gbs, 1),
         "build_mtris_per_s": round(args.tris / build_ms / 1e3, 1),
         "trace_ms": {"primary": round(float(np.mean(prim_ms)), 3), "bounce_spawn": round(float(np.mean(bounce_ms)), 3),
                      "secondary": round(float(np.mean(sec_ms)), 3)},
@@ -190,6 +193,12 @@ def main():
                  "tri_tests_per_ray": round((t1 + t2) / rays_traced, 2)},
         "roofline": roof,
     }
+    if args.calibrate_copy:  # a plain copy kernel of known size, for counter scaling
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args, pbrtm)
     ctx.close()
