@@ -136,6 +136,88 @@ int pbrtm_gen_soup(uint64_t seed, uint64_t n, float mu, float sigma, int threads
 int pbrtm_gen_camera_rays(uint32_t width, uint32_t height, const float eye[3], const float look[3],
                           const float up[3], float fov_deg, float *rays_out);
 
+/* ------------------------------------------------------------------------
+ * Wavefront path tracer. A scene is the flat result of the reference's scene
+ * front end (scene/scene_builder.cu): world-space triangles with a material
+ * and an optional diffuse area light per triangle, a LookAt + perspective
+ * camera and the film resolution. Render space is camera-world
+ * (base/camera.h:26-45); the Mitchell filter (radius 2, B = C = 1/3), the
+ * stratified sampler and the sRGB RGB film are the reference's defaults
+ * (scene_builder.cu:210-218, :275-290; films/rgb_film.cu:19-60).
+ * ------------------------------------------------------------------------ */
+typedef struct pbrtm_scene pbrtm_scene;
+
+#define PBRTM_MATERIAL_DIFFUSE 0u   /* materials/diffuse_material.cu */
+#define PBRTM_MATERIAL_CONDUCTOR 1u /* materials/conductor_material.cu */
+#define PBRTM_METAL_AG 0u           /* named spectra "metal-Ag-eta"/"metal-Ag-k" (scene_builder.cu:137-168) */
+#define PBRTM_METAL_AL 1u
+#define PBRTM_METAL_AU 2u
+#define PBRTM_METAL_CU 3u
+
+typedef struct pbrtm_material {
+    uint32_t type;            /* PBRTM_MATERIAL_* */
+    float rgb[3];             /* diffuse "rgb reflectance" (each <= 1) */
+    uint32_t metal;           /* conductor eta/k: PBRTM_METAL_* */
+    float uroughness, vroughness;
+    uint32_t remap_roughness; /* "bool remaproughness" (default true) */
+} pbrtm_material;
+
+typedef struct pbrtm_area_light { /* AreaLightSource "diffuse" (lights/diffuse_area_light.cu:9-35) */
+    float rgb[3];                 /* "rgb L" */
+    float scale;                  /* "float scale" */
+    uint32_t two_sided;           /* "bool twosided" */
+} pbrtm_area_light;
+
+typedef struct pbrtm_scene_desc {
+    float eye[3], look[3], up[3]; /* LookAt */
+    float fov;                    /* Camera "perspective" "float fov" (degrees) */
+    uint32_t width, height;       /* Film "integer xresolution" / "yresolution" */
+    uint64_t num_tris;
+    const float *tri;             /* [n][3][3] world space (host) */
+    const uint32_t *tri_material; /* [n] index into materials */
+    const int32_t *tri_light;     /* [n] index into lights, -1 = not emissive; may be NULL */
+    const uint8_t *tri_reverse;   /* [n] ReverseOrientation; may be NULL */
+    uint32_t num_materials;
+    const pbrtm_material *materials;
+    uint32_t num_lights;
+    const pbrtm_area_light *lights;
+} pbrtm_scene_desc;
+
+typedef struct pbrtm_render_params {
+    uint32_t spp;       /* samples per pixel, a square (stratified) */
+    uint32_t max_depth; /* Integrator "path" "integer maxdepth" (default 5) */
+    uint32_t pool_size; /* path slots in flight; 0 = 2^21 as the reference (wavefront_path.cu:18) */
+    uint32_t flags;     /* PBRTM_DEVICE_PTRS: film_rgb / film_aux are device pointers */
+} pbrtm_render_params;
+
+typedef struct pbrtm_render_stats {
+    double ms_total, ms_trace, ms_shadow, ms_shade, ms_film;
+    uint64_t rounds, rays, shadow_rays, paths;
+} pbrtm_render_stats;
+
+/* Builds the scene tables and its aggregate on the context's device. Replaces
+ * SceneBuilder::preprocess + build_integrator's set-up (scene_builder.cu:267-314,
+ * :829-870): render-space geometry, spectra, camera, filter, lights and the
+ * power light sampler (light_samplers/power_light_sampler.cu:7-45). */
+int pbrtm_scene_create(pbrtm_ctx *ctx, const pbrtm_scene_desc *desc, pbrtm_scene **out);
+void pbrtm_scene_free(pbrtm_scene *scene);
+
+/* Renders with the wavefront path integrator. Replaces
+ * WavefrontPathIntegrator::create + render (integrators/wavefront_path.h:87-92,
+ * wavefront_path.cu:541-572, :626-707) and RGBFilm::get_pixel_rgb
+ * (films/rgb_film.cu:129-144): film_rgb [h][w][3] is the output-space RGB of
+ * every pixel; film_aux [h][w][4] (may be NULL) the sensor-RGB sum and the
+ * filter-weight sum. Paths, pixels and samples are the reference's: the same
+ * (pixel, sample) gives the same sampler stream. */
+int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *scene, const pbrtm_render_params *params, float *film_rgb,
+                 float *film_aux, pbrtm_render_stats *stats);
+
+/* Spectral set-up exposed for tests: the sigmoid-polynomial coefficients the
+ * reference's RGB-to-spectrum table gives an RGB (spectrum_util/rgb_to_spectrum_data.cu:361-400),
+ * and the sRGB colour space matrices and white point (rgb_color_space.h:12-44). */
+int pbrtm_rgb_to_sigmoid(const float rgb[3], float out3[3]);
+int pbrtm_srgb_matrices(float xyz_from_rgb9[9], float rgb_from_xyz9[9], float white_xy2[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -562,6 +562,21 @@ uint32_t orc_pcg32_sequence(uint64_t seq, uint64_t offset, int mode, int64_t adv
     return 0;
 }
 
+int orc_trace_ray(const orc_bvh *bvh, const float o[3], const float d[3], float t_max, int any_hit,
+                  uint32_t *prim, float *t, float *bary3) {
+    TriHit h{};
+    TraceStats st;
+    const int rc = trace(bvh, v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), t_max, any_hit != 0, prim, &h, &st);
+    if (rc < 0) return -1;
+    if (t) *t = rc ? h.t : 0.f;
+    if (bary3) {
+        bary3[0] = rc ? h.b0 : 0.f;
+        bary3[1] = rc ? h.b1 : 0.f;
+        bary3[2] = rc ? h.b2 : 0.f;
+    }
+    return rc;
+}
+
 uint64_t orc_murmur64a(const uint8_t *key, uint64_t len, uint64_t seed) {
     return murmur64a(key, len, seed);
 }

@@ -140,3 +140,134 @@ def pcg32(count: int, seq: int = 0, offset: int = 0, mode: int = 1, advance: int
 def murmur64a(data: bytes, seed: int = 0) -> int:
     buf = np.frombuffer(data, np.uint8).copy() if data else np.zeros(1, np.uint8)
     return int(load().orc_murmur64a(_p(buf), len(data), seed))
+
+
+# ----------------------------------------------------------------- path tracer
+class OrcMaterial(ctypes.Structure):
+    _fields_ = [("type", c_uint32), ("rgb", c_float * 3), ("metal", c_uint32), ("uroughness", c_float),
+                ("vroughness", c_float), ("remap_roughness", c_uint32)]
+
+
+class OrcLight(ctypes.Structure):
+    _fields_ = [("rgb", c_float * 3), ("scale", c_float), ("two_sided", c_uint32)]
+
+
+class OrcScene(ctypes.Structure):
+    _fields_ = [("eye", c_float * 3), ("look", c_float * 3), ("up", c_float * 3), ("fov", c_float),
+                ("width", c_uint32), ("height", c_uint32), ("spp", c_uint32), ("max_depth", c_uint32),
+                ("pool_size", c_uint32), ("num_tris", c_uint64), ("tri", c_void_p), ("tri_material", c_void_p),
+                ("tri_light", c_void_p), ("tri_reverse", c_void_p), ("num_materials", c_uint32),
+                ("materials", POINTER(OrcMaterial)), ("num_lights", c_uint32), ("lights", POINTER(OrcLight))]
+
+
+METALS = {"Ag": 0, "Al": 1, "Au": 2, "Cu": 3}
+
+
+def _pt_sigs(lib):
+    p = c_void_p
+    sig = {
+        "orc_render": (c_int, [POINTER(OrcScene), c_int, p, p, p]),
+        "orc_rgb_to_sigmoid": (c_int, [p, p]),
+        "orc_srgb_matrices": (c_int, [p, p, p]),
+        "orc_filter_sample": (c_int, [p, p]),
+        "orc_spectrum_eval": (c_float, [c_int, c_float]),
+        "orc_trace_ray": (c_int, [p, p, p, c_float, c_int, p, p, p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def _scene_struct(sc, pool_size):
+    """Flatten a scene dict (see tests/scenes.py) into the C description; returns
+    (struct, keepalive)."""
+    tri = np.ascontiguousarray(sc["tri"], dtype=np.float32).reshape(-1, 9)
+    mat = np.ascontiguousarray(sc["tri_material"], dtype=np.uint32)
+    lig = np.ascontiguousarray(sc["tri_light"], dtype=np.int32)
+    rev = np.ascontiguousarray(sc.get("tri_reverse", np.zeros(len(tri), np.uint8)), dtype=np.uint8)
+    mats = (OrcMaterial * len(sc["materials"]))()
+    for i, m in enumerate(sc["materials"]):
+        if m["type"] == "diffuse":
+            mats[i].type = 0
+            mats[i].rgb[:] = list(m["rgb"])
+        else:
+            mats[i].type = 1
+            mats[i].metal = METALS[m["metal"]]
+            mats[i].uroughness = m.get("uroughness", m.get("roughness", 0.0))
+            mats[i].vroughness = m.get("vroughness", m.get("roughness", 0.0))
+            mats[i].remap_roughness = int(m.get("remap", True))
+    lights = (OrcLight * max(1, len(sc["lights"])))()
+    for i, l in enumerate(sc["lights"]):
+        lights[i].rgb[:] = list(l["rgb"])
+        lights[i].scale = l.get("scale", 1.0)
+        lights[i].two_sided = int(l.get("two_sided", False))
+    s = OrcScene()
+    s.eye[:], s.look[:], s.up[:] = sc["eye"], sc["look"], sc["up"]
+    s.fov = sc["fov"]
+    s.width, s.height, s.spp, s.max_depth = sc["width"], sc["height"], sc["spp"], sc["max_depth"]
+    s.pool_size = pool_size
+    s.num_tris = len(tri)
+    s.tri, s.tri_material, s.tri_light, s.tri_reverse = _p(tri), _p(mat), _p(lig), _p(rev)
+    s.num_materials = len(sc["materials"])
+    s.materials = mats
+    s.num_lights = len(sc["lights"])
+    s.lights = lights
+    return s, (tri, mat, lig, rev, mats, lights)
+
+
+def render(sc, pool_size=2 * 1024 * 1024, threads=8):
+    """Render a scene dict; returns (film_rgb [h][w][3], aux [h][w][4], stats dict)."""
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        _pt_sigs(lib)
+        lib._pt_ready = True
+    s, keep = _scene_struct(sc, pool_size)
+    h, w = sc["height"], sc["width"]
+    rgb = np.zeros((h, w, 3), np.float32)
+    aux = np.zeros((h, w, 4), np.float32)
+    st = np.zeros(4, np.uint64)
+    rc = lib.orc_render(ctypes.byref(s), threads, _p(rgb), _p(aux), _p(st))
+    if rc != 0:
+        raise RuntimeError(f"orc_render failed ({rc})")
+    return rgb, aux, {"rounds": int(st[0]), "rays": int(st[1]), "shadow_rays": int(st[2]), "paths": int(st[3])}
+
+
+def rgb_to_sigmoid(rgb):
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        _pt_sigs(lib)
+        lib._pt_ready = True
+    a = np.asarray(rgb, np.float32)
+    out = np.zeros(3, np.float32)
+    lib.orc_rgb_to_sigmoid(_p(a), _p(out))
+    return out
+
+
+def srgb_matrices():
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        _pt_sigs(lib)
+        lib._pt_ready = True
+    a, b, w = np.zeros(9, np.float32), np.zeros(9, np.float32), np.zeros(2, np.float32)
+    lib.orc_srgb_matrices(_p(a), _p(b), _p(w))
+    return a.reshape(3, 3), b.reshape(3, 3), w
+
+
+def filter_sample(u0, u1):
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        _pt_sigs(lib)
+        lib._pt_ready = True
+    u = np.array([u0, u1], np.float32)
+    out = np.zeros(3, np.float32)
+    lib.orc_filter_sample(_p(u), _p(out))
+    return out
+
+
+def spectrum_eval(which, lam):
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        _pt_sigs(lib)
+        lib._pt_ready = True
+    return float(lib.orc_spectrum_eval(int(which), float(lam)))

@@ -1,0 +1,65 @@
+/* TEST INFRASTRUCTURE ONLY — the CPU oracle's wavefront path tracer.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * liboracle.so, and only as the checker or the timed CPU baseline.
+ *
+ * A host-only restatement of integrators/wavefront_path.cu and everything it
+ * calls for diffuse and conductor surfaces lit by diffuse area lights: the
+ * stratified sampler, the tabulated Mitchell filter, the perspective camera,
+ * spectral sampling and the RGB film. Paths are scheduled in rounds exactly as
+ * the reference's path pool schedules them, so the film receives its samples
+ * in the reference's order (round, then pixel, then sample).
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_material {
+    uint32_t type;          /* 0 = diffuse, 1 = conductor */
+    float rgb[3];           /* diffuse: RGB reflectance (albedo, each <= 1) */
+    uint32_t metal;         /* conductor: eta/k of 0 = Ag, 1 = Al, 2 = Au, 3 = Cu */
+    float uroughness, vroughness;
+    uint32_t remap_roughness;
+} orc_material;
+
+typedef struct orc_light {  /* AreaLightSource "diffuse" */
+    float rgb[3];           /* "rgb L" */
+    float scale;
+    uint32_t two_sided;
+} orc_light;
+
+typedef struct orc_scene {
+    float eye[3], look[3], up[3]; /* LookAt */
+    float fov;                    /* Camera "perspective" "float fov" */
+    uint32_t width, height;       /* Film "rgb" resolution */
+    uint32_t spp;                 /* stratified samples per pixel (a square) */
+    uint32_t max_depth;           /* Integrator "path" "integer maxdepth" */
+    uint32_t pool_size;           /* path slots in flight */
+    uint64_t num_tris;
+    const float *tri;             /* [n][3][3] world space */
+    const uint32_t *tri_material; /* [n] */
+    const int32_t *tri_light;     /* [n] index into lights or -1; nullable */
+    const uint8_t *tri_reverse;   /* [n] ReverseOrientation; nullable */
+    uint32_t num_materials;
+    const orc_material *materials;
+    uint32_t num_lights;
+    const orc_light *lights;
+} orc_scene;
+
+/* film_rgb: [height][width][3] = RGBFilm::get_pixel_rgb; aux (nullable):
+ * [height][width][4] = rgb_sum, weight_sum; stats (nullable): rounds, rays,
+ * shadow rays, finished paths. */
+int orc_render(const orc_scene *scene, int threads, float *film_rgb, float *aux, uint64_t *stats);
+
+/* Intermediate tables, for tests: the RGB-to-spectrum sigmoid coefficients of an
+ * albedo RGB, the sRGB rgb_from_xyz matrix, the filter sample of a 2D u. */
+int orc_rgb_to_sigmoid(const float rgb[3], float out3[3]);
+int orc_srgb_matrices(float xyz_from_rgb9[9], float rgb_from_xyz9[9], float white_xy2[2]);
+int orc_filter_sample(const float u[2], float out3[3]);
+float orc_spectrum_eval(int which, float lambda); /* 0-2 CIE XYZ, 3 D65 (normalised), 4+ metals eta/k */
+
+#ifdef __cplusplus
+}
+#endif

@@ -109,8 +109,9 @@ inline V3 sample_cosine_hemisphere(float u0, float u1) {
             float pi_over_2 = float(pi / 2.0);
             theta = pi_over_2 - pi_over_4 * (ox / oy);
         }
-        dx = r * std::cos(theta);
-        dy = r * std::sin(theta);
+        // transcendentals in double, rounded once to float (see pt_oracle.cpp)
+        dx = r * float(std::cos(double(theta)));
+        dy = r * float(std::sin(double(theta)));
     }
     float zz = float(1.0 - (dx * dx) - (dy * dy));
     float z = std::sqrt(std::max(0.0f, zz));

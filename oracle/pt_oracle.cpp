@@ -1,0 +1,1341 @@
+// TEST INFRASTRUCTURE ONLY — part of the CPU oracle (see oracle_pt.h).
+//
+// Host-only restatement of the reference's wavefront path tracer for diffuse
+// and conductor surfaces lit by diffuse area lights. Every FP32 expression keeps
+// the reference's operation order and its double-precision promotions; the
+// build uses -ffp-contract=off so a*b+c is two roundings unless std::fma is
+// written out. Transcendental functions (sin, cos, asin, atan2, atanh, cosh,
+// tan) are evaluated in double precision and rounded once to float, a
+// correctly rounded stand-in for the CUDA device library the reference calls.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "oracle.h"
+#include "oracle_math.h"
+#include "oracle_pt.h"
+#include "oracle_rng.h"
+#include "oracle_spectra.h"
+#include "oracle_tri.h"
+
+namespace orc {
+namespace {
+
+// ------------------------------------------------------------ small helpers
+inline float tsin(float x) { return float(std::sin(double(x))); }
+inline float tcos(float x) { return float(std::cos(double(x))); }
+inline float tasin(float x) { return float(std::asin(double(x))); }
+inline float tatan2(float y, float x) { return float(std::atan2(double(y), double(x))); }
+inline float tatanh(float x) { return float(std::atanh(double(x))); }
+inline float tcosh(float x) { return float(std::cosh(double(x))); }
+
+const float kPi = float(std::acos(-1.0));  // compute_pi(): acos(-1) rounded to float
+constexpr float kOneMinusEps = 0x1.fffffep-1f;
+
+inline float sqrf(float v) { return v * v; }
+inline float clampf(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+inline float safe_sqrt(float x) { return std::sqrt(std::max(0.0f, x)); }
+inline float safe_asin(float x) { return tasin(clampf(x, -1, 1)); }
+inline float length(V3 a) { return std::sqrt(sqlen(a)); }
+inline V3 vmul(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline float abs_dot(V3 a, V3 b) { return std::fabs(dot(a, b)); }
+inline bool v3_zero(V3 a) { return a.x == 0 && a.y == 0 && a.z == 0; }
+inline V3 face_forward(V3 n, V3 v) { return dot(n, v) < 0.0f ? neg(n) : n; }
+
+// vector3.h:233-240
+inline float angle_between(V3 v1, V3 v2) {
+    if (dot(v1, v2) < 0) return kPi - 2 * safe_asin(length(add(v1, v2)) / 2);
+    return 2 * safe_asin(length(sub(v2, v1)) / 2);
+}
+inline V3 gram_schmidt(V3 v, V3 w) { return sub(v, scale(dot(v, w), w)); }
+
+// ------------------------------------------------------------ spectra (4 wavelengths)
+struct S4 {
+    float v[kNS];
+};
+inline S4 s4(float a) { return {{a, a, a, a}}; }
+inline S4 operator*(const S4 &a, const S4 &b) {
+    S4 r;
+    for (int i = 0; i < kNS; ++i) r.v[i] = a.v[i] * b.v[i];
+    return r;
+}
+inline S4 operator*(const S4 &a, float s) {
+    S4 r;
+    for (int i = 0; i < kNS; ++i) r.v[i] = a.v[i] * s;
+    return r;
+}
+inline S4 operator/(const S4 &a, float s) {
+    S4 r;
+    for (int i = 0; i < kNS; ++i) r.v[i] = a.v[i] / s;
+    return r;
+}
+inline S4 operator+(const S4 &a, const S4 &b) {
+    S4 r;
+    for (int i = 0; i < kNS; ++i) r.v[i] = a.v[i] + b.v[i];
+    return r;
+}
+inline bool positive(const S4 &a) {
+    for (int i = 0; i < kNS; ++i)
+        if (a.v[i] > 0) return true;
+    return false;
+}
+inline float max_comp(const S4 &a) {
+    float m = a.v[0];
+    for (int i = 1; i < kNS; ++i) m = std::max(m, a.v[i]);
+    return m;
+}
+inline float average(const S4 &a) {
+    float s = 0;
+    for (int i = 0; i < kNS; ++i) s += a.v[i];
+    return s / kNS;
+}
+inline S4 safe_div(const S4 &a, const S4 &b) {
+    S4 r;
+    for (int i = 0; i < kNS; ++i) r.v[i] = b.v[i] == 0.0f ? 0.0f : a.v[i] / b.v[i];
+    return r;
+}
+
+struct Lambda {
+    float l[kNS], pdf[kNS];
+};
+// sampling.h:240-253, sampled_wavelengths.h:25-42
+inline Lambda sample_visible(float u) {
+    Lambda w;
+    for (int i = 0; i < kNS; ++i) {
+        float up = u + float(i) / kNS;
+        if (up > 1) up -= 1;
+        float l = 538 - 138.888889f * tatanh(0.85691062f - 1.82750197f * up);
+        w.l[i] = l;
+        w.pdf[i] = (l < kLambdaMin || l > kLambdaMax) ? 0.0f : 0.0039398042f / sqrf(tcosh(0.0072f * (l - 538)));
+    }
+    return w;
+}
+inline S4 sample_pl(const PLSpectrum &s, const Lambda &w) {
+    S4 r;
+    for (int i = 0; i < kNS; ++i) r.v[i] = s(w.l[i]);
+    return r;
+}
+inline S4 sample_sigmoid(const Sigmoid &s, const Lambda &w) {
+    S4 r;
+    for (int i = 0; i < kNS; ++i) r.v[i] = s(w.l[i]);
+    return r;
+}
+
+// ------------------------------------------------------------ 4x4 transforms
+struct M4 {
+    float m[4][4];
+};
+inline M4 m4_mul(const M4 &a, const M4 &b) {  // squared_matrix.h:98-109
+    M4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            r.m[i][j] = 0;
+            for (int k = 0; k < 4; ++k) r.m[i][j] = std::fma(a.m[i][k], b.m[k][j], r.m[i][j]);
+        }
+    return r;
+}
+inline float ip(std::initializer_list<float> t) {
+    std::vector<float> a, b;
+    auto it = t.begin();
+    while (it != t.end()) {
+        a.push_back(*it++);
+        b.push_back(*it++);
+    }
+    return inner_product_n(a.data(), b.data(), int(a.size()));
+}
+inline M4 m4_inverse(const M4 &A) {  // squared_matrix.cu (Eberly's Laplace expansion)
+    const auto &m = A.m;
+    float s0 = dop(m[0][0], m[1][1], m[1][0], m[0][1]);
+    float s1 = dop(m[0][0], m[1][2], m[1][0], m[0][2]);
+    float s2 = dop(m[0][0], m[1][3], m[1][0], m[0][3]);
+    float s3 = dop(m[0][1], m[1][2], m[1][1], m[0][2]);
+    float s4_ = dop(m[0][1], m[1][3], m[1][1], m[0][3]);
+    float s5 = dop(m[0][2], m[1][3], m[1][2], m[0][3]);
+    float c0 = dop(m[2][0], m[3][1], m[3][0], m[2][1]);
+    float c1 = dop(m[2][0], m[3][2], m[3][0], m[2][2]);
+    float c2 = dop(m[2][0], m[3][3], m[3][0], m[2][3]);
+    float c3 = dop(m[2][1], m[3][2], m[3][1], m[2][2]);
+    float c4 = dop(m[2][1], m[3][3], m[3][1], m[2][3]);
+    float c5 = dop(m[2][2], m[3][3], m[3][2], m[2][3]);
+    float det = ip({s0, c5, -s1, c4, s2, c3, s3, c2, s5, c0, -s4_, c1});
+    if (det == 0) throw std::runtime_error("singular 4x4 matrix");
+    float s = float(1.0 / double(det));
+    M4 r;
+    r.m[0][0] = s * ip({m[1][1], c5, m[1][3], c3, -m[1][2], c4});
+    r.m[0][1] = s * ip({-m[0][1], c5, m[0][2], c4, -m[0][3], c3});
+    r.m[0][2] = s * ip({m[3][1], s5, m[3][3], s3, -m[3][2], s4_});
+    r.m[0][3] = s * ip({-m[2][1], s5, m[2][2], s4_, -m[2][3], s3});
+    r.m[1][0] = s * ip({-m[1][0], c5, m[1][2], c2, -m[1][3], c1});
+    r.m[1][1] = s * ip({m[0][0], c5, m[0][3], c1, -m[0][2], c2});
+    r.m[1][2] = s * ip({-m[3][0], s5, m[3][2], s2, -m[3][3], s1});
+    r.m[1][3] = s * ip({m[2][0], s5, m[2][3], s1, -m[2][2], s2});
+    r.m[2][0] = s * ip({m[1][0], c4, m[1][3], c0, -m[1][1], c2});
+    r.m[2][1] = s * ip({-m[0][0], c4, m[0][1], c2, -m[0][3], c0});
+    r.m[2][2] = s * ip({m[3][0], s4_, m[3][3], s0, -m[3][1], s2});
+    r.m[2][3] = s * ip({-m[2][0], s4_, m[2][1], s2, -m[2][3], s0});
+    r.m[3][0] = s * ip({-m[1][0], c3, m[1][1], c1, -m[1][2], c0});
+    r.m[3][1] = s * ip({m[0][0], c3, m[0][2], c0, -m[0][1], c1});
+    r.m[3][2] = s * ip({-m[3][0], s3, m[3][1], s1, -m[3][2], s0});
+    r.m[3][3] = s * ip({m[2][0], s3, m[2][2], s0, -m[2][1], s1});
+    return r;
+}
+struct Xf {  // euclidean_space/transform.h
+    M4 m, inv;
+};
+inline M4 m4_diag(float a, float b, float c) {
+    M4 r{};
+    r.m[0][0] = a;
+    r.m[1][1] = b;
+    r.m[2][2] = c;
+    r.m[3][3] = 1;
+    return r;
+}
+inline Xf xf_from(const M4 &m) { return {m, m4_inverse(m)}; }
+inline Xf xf_mul(const Xf &a, const Xf &b) { return {m4_mul(a.m, b.m), m4_mul(b.inv, a.inv)}; }
+inline Xf xf_inverse(const Xf &a) { return {a.inv, a.m}; }
+inline Xf xf_scale(float x, float y, float z) {
+    return {m4_diag(x, y, z), m4_diag(float(1.0) / x, float(1.0) / y, float(1.0) / z)};
+}
+inline Xf xf_translate(float x, float y, float z) {
+    Xf t{m4_diag(1, 1, 1), m4_diag(1, 1, 1)};
+    t.m.m[0][3] = x;
+    t.m.m[1][3] = y;
+    t.m.m[2][3] = z;
+    t.inv.m[0][3] = -x;
+    t.inv.m[1][3] = -y;
+    t.inv.m[2][3] = -z;
+    return t;
+}
+inline V3 xf_point(const M4 &M, V3 p) {  // transform.h:254-265
+    const auto &m = M.m;
+    float xp = m[0][0] * p.x + m[0][1] * p.y + m[0][2] * p.z + m[0][3];
+    float yp = m[1][0] * p.x + m[1][1] * p.y + m[1][2] * p.z + m[1][3];
+    float zp = m[2][0] * p.x + m[2][1] * p.y + m[2][2] * p.z + m[2][3];
+    float wp = m[3][0] * p.x + m[3][1] * p.y + m[3][2] * p.z + m[3][3];
+    if (wp == 1.0f) return {xp, yp, zp};
+    return {xp / wp, yp / wp, zp / wp};
+}
+inline V3 xf_vector(const M4 &M, V3 v) {  // transform.h:303-307
+    const auto &m = M.m;
+    return {m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z, m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z,
+            m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z};
+}
+
+// transform.h:181-224 — camera_from_world with world_from_camera as its inverse.
+inline Xf look_at(V3 pos, V3 look, V3 up) {
+    M4 w{};
+    w.m[0][3] = pos.x;
+    w.m[1][3] = pos.y;
+    w.m[2][3] = pos.z;
+    w.m[3][3] = 1.0f;
+    V3 dir = normalize(sub(look, pos));
+    if (length(cross(normalize(up), dir)) == 0.0f) throw std::runtime_error("LookAt: up parallel to view");
+    V3 right = normalize(cross(normalize(up), dir));
+    V3 new_up = cross(dir, right);
+    w.m[0][0] = right.x;
+    w.m[1][0] = right.y;
+    w.m[2][0] = right.z;
+    w.m[0][1] = new_up.x;
+    w.m[1][1] = new_up.y;
+    w.m[2][1] = new_up.z;
+    w.m[0][2] = dir.x;
+    w.m[1][2] = dir.y;
+    w.m[2][2] = dir.z;
+    return {m4_inverse(w), w};
+}
+
+// ------------------------------------------------------------ camera
+struct Camera {
+    Xf render_from_camera, camera_from_raster;
+    Xf render_from_world;
+};
+// cameras/perspective.cu:6-61, base/camera.h CameraTransform (render space = camera-world).
+inline Camera make_camera(const orc_scene &s) {
+    Camera c;
+    Xf camera_from_world = look_at({s.eye[0], s.eye[1], s.eye[2]}, {s.look[0], s.look[1], s.look[2]},
+                                   {s.up[0], s.up[1], s.up[2]});
+    Xf world_from_camera = xf_inverse(camera_from_world);
+    V3 pc = xf_point(world_from_camera.m, {0, 0, 0});
+    Xf world_from_render = xf_translate(pc.x, pc.y, pc.z);
+    c.render_from_world = xf_inverse(world_from_render);
+    c.render_from_camera = xf_mul(c.render_from_world, world_from_camera);
+    float aspect = float(s.width) / float(s.height);
+    float sw[4];  // pmin.x, pmin.y, pmax.x, pmax.y
+    if (aspect > 1.0) {
+        sw[0] = -aspect, sw[1] = -1.0f, sw[2] = aspect, sw[3] = 1.0f;
+    } else {
+        sw[0] = -1.0f, sw[1] = float(-1.0 / aspect), sw[2] = 1.0f, sw[3] = float(1.0 / aspect);
+    }
+    Xf ndc_from_screen = xf_mul(xf_scale(float(1.0 / (sw[2] - sw[0])), float(1.0 / (sw[3] - sw[1])), 1.0f),
+                                xf_translate(-sw[0], -sw[3], 0.0f));
+    Xf raster_from_ndc = xf_scale(float(s.width), float(-int(s.height)), 1.0f);
+    Xf raster_from_screen = xf_mul(raster_from_ndc, ndc_from_screen);
+    Xf screen_from_raster = xf_inverse(raster_from_screen);
+    const float z_near = 1e-2f, z_far = 1000.0f;
+    M4 persp{};
+    persp.m[0][0] = 1;
+    persp.m[1][1] = 1;
+    persp.m[2][2] = z_far / (z_far - z_near);
+    persp.m[2][3] = -z_far * z_near / (z_far - z_near);
+    persp.m[3][2] = 1;
+    float deg = float(double(kPi) / 180.0 * double(s.fov));
+    double inv_tan = 1.0 / std::tan(double(deg) / 2.0);
+    Xf screen_from_camera = xf_mul(xf_scale(float(inv_tan), float(inv_tan), 1.0f), xf_from(persp));
+    c.camera_from_raster = xf_mul(xf_inverse(screen_from_camera), screen_from_raster);
+    return c;
+}
+
+// ------------------------------------------------------------ Mitchell filter sampler
+struct Pc1d {  // util/piecewise_constant_1d.h
+    std::vector<float> func, cdf;
+    float mn, mx, func_int;
+    void init(const float *f, size_t n, float lo, float hi) {
+        mn = lo;
+        mx = hi;
+        func.resize(n);
+        for (size_t i = 0; i < n; ++i) func[i] = std::fabs(f[i]);
+        cdf.assign(n + 1, 0.f);
+        for (size_t i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + func[i - 1] * (mx - mn) / float(n);
+        func_int = cdf[n];
+        if (func_int == 0)
+            for (size_t i = 1; i < n + 1; ++i) cdf[i] = float(i) / float(n);
+        else
+            for (size_t i = 1; i < n + 1; ++i) cdf[i] /= func_int;
+    }
+    float sample(float u, float *pdf, int *off) const {
+        int o = (int)find_interval(func.size() + 1, [&](size_t i) { return cdf[i] <= u; });
+        *off = o;
+        float du = u - cdf[o];
+        if (cdf[o + 1] - cdf[o] > 0) du /= cdf[o + 1] - cdf[o];
+        *pdf = (func_int > 0) ? func[o] / func_int : 0;
+        return lerpf((float(o) + du) / float(func.size()), mn, mx);
+    }
+};
+struct FilterSampler {  // base/filter.cu:146-161, filters/mitchell.{h,cu}
+    float radius = 2.f, b = 1.f / 3.f, c = 1.f / 3.f;
+    int nx = 64, ny = 64;
+    std::vector<float> f;
+    std::vector<Pc1d> cond;
+    Pc1d marg;
+    float mitchell_1d(float x) const {
+        x = std::fabs(x);
+        if (x <= 1)
+            return ((12 - 9 * b - 6 * c) * x * x * x + (-18 + 12 * b + 6 * c) * x * x + (6 - 2 * b)) * (1.f / 6.f);
+        if (x <= 2)
+            return ((-b - 6 * c) * x * x * x + (6 * b + 30 * c) * x * x + (-12 * b - 48 * c) * x + (8 * b + 24 * c)) *
+                   (1.f / 6.f);
+        return 0;
+    }
+    FilterSampler() {
+        nx = int(32 * radius);
+        ny = int(32 * radius);
+        f.resize(size_t(nx) * ny);
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                float px = lerpf((x + 0.5f) / nx, -radius, radius);
+                float py = lerpf((y + 0.5f) / ny, -radius, radius);
+                f[x + nx * y] = mitchell_1d(2 * px / radius) * mitchell_1d(2 * py / radius);
+            }
+        cond.resize(ny);
+        std::vector<float> mf;
+        for (int v = 0; v < ny; ++v) {
+            cond[v].init(&f[size_t(v) * nx], nx, -radius, radius);
+            mf.push_back(cond[v].func_int);
+        }
+        marg.init(mf.data(), mf.size(), -radius, radius);
+    }
+    void sample(float u0, float u1, float *px, float *py, float *w) const {
+        float pdf0, pdf1;
+        int o0, o1;
+        float d1 = marg.sample(u1, &pdf1, &o1);
+        float d0 = cond[o1].sample(u0, &pdf0, &o0);
+        *px = d0;
+        *py = d1;
+        *w = f[o0 + nx * o1] / (pdf0 * pdf1);
+    }
+};
+
+// ------------------------------------------------------------ stratified sampler
+inline uint64_t hash_u32(uint32_t a) { return murmur64a(reinterpret_cast<const unsigned char *>(&a), 4, 0); }
+inline uint64_t hash_u32x2(uint32_t a, uint32_t b) {
+    uint32_t buf[2] = {a, b};
+    return murmur64a(reinterpret_cast<const unsigned char *>(buf), 8, 0);
+}
+inline int permutation_element(uint32_t i, uint32_t l, uint32_t p) {  // util/permutation.h (Kensler 2013)
+    uint32_t w = l - 1;
+    w |= w >> 1;
+    w |= w >> 2;
+    w |= w >> 4;
+    w |= w >> 8;
+    w |= w >> 16;
+    do {
+        i ^= p;
+        i *= 0xe170893d;
+        i ^= p >> 16;
+        i ^= (i & w) >> 4;
+        i ^= p >> 8;
+        i *= 0x0929eb3f;
+        i ^= p >> 23;
+        i ^= (i & w) >> 1;
+        i *= 1 | p >> 27;
+        i *= 0x6935fa69;
+        i ^= (i & w) >> 11;
+        i *= 0x74dcb303;
+        i ^= (i & w) >> 2;
+        i *= 0x9e501cc3;
+        i ^= (i & w) >> 2;
+        i *= 0xc860a3df;
+        i &= w;
+        i ^= i >> 5;
+    } while (i >= l);
+    return int((i + p) % l);
+}
+struct Sampler {  // samplers/stratified.h
+    uint32_t spd, pixel, sample, dim;
+    Pcg32 rng;
+    void start(uint32_t px, uint32_t s, uint32_t d) {
+        pixel = px;
+        sample = s;
+        dim = d;
+        rng.set_sequence(hash_u32(px));
+        rng.advance(int64_t(uint64_t(s) * 65536ull + d));
+    }
+    float get_1d() {
+        uint64_t h = hash_u32x2(pixel, dim);
+        int stratum = permutation_element(sample, spd * spd, uint32_t(h));
+        dim += 1;
+        float delta = rng.next_float();
+        return (float(stratum) + delta) / float(spd * spd);
+    }
+    void get_2d(float *u0, float *u1) {
+        uint64_t h = hash_u32x2(pixel, dim);
+        int stratum = permutation_element(sample, spd * spd, uint32_t(h));
+        dim += 2;
+        uint32_t x = uint32_t(stratum) % spd, y = uint32_t(stratum) / spd;
+        float dx = rng.next_float();
+        float dy = rng.next_float();
+        *u0 = (float(x) + dx) / float(spd);
+        *u1 = (float(y) + dy) / float(spd);
+    }
+};
+
+// ------------------------------------------------------------ scene tables
+struct Tri {
+    V3 p[3];
+    uint32_t material;
+    int32_t light;
+    bool flip;
+};
+struct Material {
+    uint32_t type;
+    Sigmoid albedo;               // diffuse
+    const PLSpectrum *eta, *k;    // conductor
+    float ax, ay;
+};
+struct Light {
+    uint32_t tri;
+    float scale;                  // light scale / to_photometric
+    float rgb_scale;              // RGBIlluminantSpectrum::scale
+    Sigmoid rsp;
+    bool two_sided;
+};
+
+struct Scene {
+    const orc_scene *desc;
+    Camera cam;
+    FilterSampler filter;
+    std::vector<Tri> tris;
+    std::vector<float> soup;      // render space, for the oracle BVH
+    orc_bvh *bvh = nullptr;
+    std::vector<Material> mats;
+    std::vector<Light> lights;
+    std::vector<float> light_pmf, light_cdf;
+    Spectra *sp;
+    PLSpectrum metals[8];
+    M3 out_from_sensor;
+    ~Scene() {
+        if (bvh) orc_bvh_free(bvh);
+    }
+};
+
+Spectra &spectra() {
+    static Spectra s;
+    return s;
+}
+
+inline float tri_area(const Tri &t) { return length(cross(sub(t.p[1], t.p[0]), sub(t.p[2], t.p[0]))) * 0.5f; }
+
+// lights/diffuse_area_light.cu:37-46
+inline S4 light_L(const Scene &sc, const Light &L, V3 n, V3 w, const Lambda &lam) {
+    if (!L.two_sided && dot(n, w) < 0.0f) return s4(0.0f);
+    S4 s;
+    for (int i = 0; i < kNS; ++i) s.v[i] = L.rgb_scale * L.rsp(lam.l[i]);
+    return (sample_pl(sc.sp->d65, lam) * s) * L.scale;
+}
+
+void build_scene(Scene &sc, const orc_scene &d) {
+    sc.desc = &d;
+    sc.sp = &spectra();
+    sc.cam = make_camera(d);
+    const float *metal_data[8] = {pbrtm_data::kAgEta, pbrtm_data::kAgK, pbrtm_data::kAlEta, pbrtm_data::kAlK,
+                                  pbrtm_data::kAuEta, pbrtm_data::kAuK, pbrtm_data::kCuEta, pbrtm_data::kCuK};
+    const int metal_len[8] = {pbrtm_data::kAgEtaLen, pbrtm_data::kAgKLen, pbrtm_data::kAlEtaLen, pbrtm_data::kAlKLen,
+                              pbrtm_data::kAuEtaLen, pbrtm_data::kAuKLen, pbrtm_data::kCuEtaLen, pbrtm_data::kCuKLen};
+    for (int i = 0; i < 8; ++i) sc.metals[i] = pl_from_interleaved(metal_data[i], metal_len[i]);
+    // Materials (materials/diffuse_material.cu, materials/conductor_material.cu).
+    for (uint32_t i = 0; i < d.num_materials; ++i) {
+        const orc_material &m = d.materials[i];
+        Material mm{};
+        mm.type = m.type;
+        if (m.type == 0) {
+            if (m.rgb[0] > 1 || m.rgb[1] > 1 || m.rgb[2] > 1) throw std::runtime_error("albedo RGB > 1");
+            float c[3] = {std::max(0.f, m.rgb[0]), std::max(0.f, m.rgb[1]), std::max(0.f, m.rgb[2])};
+            mm.albedo = sc.sp->table.lookup(c);
+        } else if (m.type == 1) {
+            if (m.metal > 3) throw std::runtime_error("unknown metal");
+            mm.eta = &sc.metals[2 * m.metal];
+            mm.k = &sc.metals[2 * m.metal + 1];
+            float u = m.uroughness, v = m.vroughness;
+            if (m.remap_roughness) {
+                u = std::sqrt(u);
+                v = std::sqrt(v);
+            }
+            mm.ax = u;
+            mm.ay = v;
+            if (!(std::max(mm.ax, mm.ay) < 1e-3f)) throw std::runtime_error("rough conductor not in the oracle");
+        } else {
+            throw std::runtime_error("unknown material type");
+        }
+        sc.mats.push_back(mm);
+    }
+    // Geometry in render space (triangle_mesh.cu:39-57: render_from_object applied per point).
+    const M4 &rw = sc.cam.render_from_world.m;
+    sc.tris.resize(d.num_tris);
+    sc.soup.resize(d.num_tris * 9);
+    for (uint64_t i = 0; i < d.num_tris; ++i) {
+        Tri &t = sc.tris[i];
+        for (int v = 0; v < 3; ++v) {
+            t.p[v] = xf_point(rw, {d.tri[9 * i + 3 * v], d.tri[9 * i + 3 * v + 1], d.tri[9 * i + 3 * v + 2]});
+            sc.soup[9 * i + 3 * v] = t.p[v].x;
+            sc.soup[9 * i + 3 * v + 1] = t.p[v].y;
+            sc.soup[9 * i + 3 * v + 2] = t.p[v].z;
+        }
+        t.material = d.tri_material[i];
+        if (t.material >= d.num_materials) throw std::runtime_error("material index out of range");
+        t.light = d.tri_light ? d.tri_light[i] : -1;
+        t.flip = d.tri_reverse ? d.tri_reverse[i] != 0 : false;
+        if (t.light >= 0) {
+            if (uint32_t(t.light) >= d.num_lights) throw std::runtime_error("light index out of range");
+            t.light = int32_t(sc.lights.size());
+            const orc_light &L = d.lights[d.tri_light[i]];
+            Light l{};
+            l.tri = uint32_t(i);
+            l.two_sided = L.two_sided != 0;
+            l.rgb_scale = float(2.0 * double(std::max(std::max(L.rgb[0], L.rgb[1]), L.rgb[2])));
+            float c[3] = {0, 0, 0};
+            if (l.rgb_scale > 0.0f)
+                for (int q = 0; q < 3; ++q) c[q] = std::max(0.f, L.rgb[q] / l.rgb_scale);
+            l.rsp = sc.sp->table.lookup(c);
+            l.scale = L.scale / sc.sp->d65.inner(sc.sp->cie[1]);
+            sc.lights.push_back(l);
+        }
+    }
+    orc_build_info info;
+    if (orc_bvh_build(sc.soup.data(), d.num_tris, &sc.bvh, &info) != 0) throw std::runtime_error("oracle BVH build");
+    // PowerLightSampler::create (power_light_sampler.cu:7-45) + Distribution1D::build.
+    if (!sc.lights.empty()) {
+        Lambda lam = sample_visible(0.5f);
+        S4 pdf;
+        for (int i = 0; i < kNS; ++i) pdf.v[i] = lam.pdf[i];
+        std::vector<float> w;
+        for (const Light &l : sc.lights) {
+            S4 Ls;
+            for (int i = 0; i < kNS; ++i) Ls.v[i] = l.rgb_scale * l.rsp(lam.l[i]);
+            S4 L = (sample_pl(sc.sp->d65, lam) * Ls) * l.scale;
+            float k = kPi * float(l.two_sided ? 2 : 1) * tri_area(sc.tris[l.tri]);
+            S4 phi = L * k;
+            w.push_back(average(safe_div(phi, pdf)));
+        }
+        double sum = 0.0;
+        for (float x : w) sum += x;
+        size_t n = w.size();
+        sc.light_pmf.resize(n);
+        for (size_t i = 0; i < n; ++i) sc.light_pmf[i] = sum == 0.0 ? float(1.0 / n) : float(w[i] / sum);
+        sc.light_cdf.resize(n);
+        sc.light_cdf[0] = sc.light_pmf[0];
+        for (size_t i = 1; i < n; ++i) sc.light_cdf[i] = sc.light_cdf[i - 1] + sc.light_pmf[i];
+    }
+    sc.out_from_sensor = sc.sp->rgb_from_xyz;  // times the identity white balance
+}
+
+// distribution_1d.h:9-42
+inline uint32_t search_cdf(float u, const float *cdf, uint32_t len) {
+    if (u < cdf[0] || len == 1) return 0;
+    uint32_t start = 1, end = len;
+    while (true) {
+        if (end - start <= 10) {
+            uint32_t t = start;
+            for (uint32_t i = start; i < end; ++i)
+                if (u >= cdf[i - 1] && u < cdf[i]) {
+                    t = i;
+                    break;
+                }
+            return t;
+        }
+        uint32_t mid = (start + end) / 2;
+        if (u >= cdf[mid]) start = mid + 1;
+        else end = mid + 1;
+    }
+}
+
+// ------------------------------------------------------------ surface interaction
+struct Isect {
+    P3fi pi;
+    V3 n, ns, dpdu, wo;
+    uint32_t tri;
+};
+inline V3 p3fi_mid(const P3fi &p) { return p3fi_point(p); }
+
+// shapes/triangle.cu:330-470 for meshes without normals or uv.
+inline Isect make_isect(const Scene &sc, uint32_t prim, const TriHit &h, V3 ray_d) {
+    const Tri &t = sc.tris[prim];
+    V3 p0 = t.p[0], p1 = t.p[1], p2 = t.p[2];
+    // uv = (0,0), (1,0), (1,1): duv02 = (-1,-1), duv12 = (0,-1), determinant 1.
+    V3 dp02 = sub(p0, p2), dp12 = sub(p1, p2);
+    const float du02[2] = {-1.f, -1.f}, du12[2] = {0.f, -1.f};
+    float det = dop(du02[0], du12[1], du02[1], du12[0]);
+    V3 dpdu, dpdv;
+    bool degenerate = std::fabs(det) < 1e-9f;
+    if (!degenerate) {
+        float invdet = 1 / det;
+        dpdu = scale(invdet, V3{dop(du12[1], dp02.x, du02[1], dp12.x), dop(du12[1], dp02.y, du02[1], dp12.y),
+                                dop(du12[1], dp02.z, du02[1], dp12.z)});
+        dpdv = scale(invdet, V3{dop(du02[0], dp12.x, du12[0], dp02.x), dop(du02[0], dp12.y, du12[0], dp02.y),
+                                dop(du02[0], dp12.z, du12[0], dp02.z)});
+    }
+    if (degenerate || sqlen(cross(dpdu, dpdv)) == 0) {
+        V3 ng = cross(sub(p2, p0), sub(p1, p0));
+        coordinate_system(normalize(ng), &dpdu, &dpdv);
+    }
+    HitGeom g = hit_geometry(h, p0, p1, p2);
+    Isect is;
+    is.pi = g.pi;
+    is.wo = normalize(neg(ray_d));
+    is.n = normalize(cross(dp02, dp12));
+    if (t.flip) is.n = neg(is.n);
+    is.ns = is.n;
+    is.dpdu = dpdu;
+    is.tri = prim;
+    return is;
+}
+
+// ------------------------------------------------------------ BSDFs
+struct Frame {
+    V3 x, y, z;
+    V3 to_local(V3 v) const { return {dot(v, x), dot(v, y), dot(v, z)}; }
+    V3 from_local(V3 v) const { return add(add(scale(v.x, x), scale(v.y, y)), scale(v.z, z)); }
+};
+struct Bsdf {
+    Frame fr;
+    uint32_t type;  // 0 diffuse, 1 specular conductor
+    S4 r;           // diffuse reflectance
+    S4 eta, k;
+    bool diffuse_set;
+};
+const float kInvPi = float(1.0 / double(kPi));
+
+struct Cx {
+    float re, im;
+};
+inline Cx cx(float r) { return {r, 0.f}; }
+inline Cx operator+(Cx a, Cx b) { return {a.re + b.re, a.im + b.im}; }
+inline Cx operator-(Cx a, Cx b) { return {a.re - b.re, a.im - b.im}; }
+inline Cx operator*(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline Cx operator/(Cx a, Cx b) {
+    float s = 1 / (b.re * b.re + b.im * b.im);
+    return {s * (a.re * b.re + a.im * b.im), s * (a.im * b.re - a.re * b.im)};
+}
+inline float cnorm(Cx a) { return a.re * a.re + a.im * a.im; }
+inline Cx csqrt(Cx z) {  // util/complex.h
+    float n = std::sqrt(cnorm(z));
+    float t1 = std::sqrt(.5f * (n + std::fabs(z.re)));
+    float t2 = .5f * z.im / t1;
+    if (n == 0) return cx(0);
+    if (z.re >= 0) return {t1, t2};
+    return {std::fabs(t2), std::copysign(t1, z.im)};
+}
+// util/scattering.h:56-72
+inline float fr_complex(float cos_i, Cx eta) {
+    cos_i = clampf(cos_i, 0, 1);
+    float sin2_i = 1 - sqrf(cos_i);
+    Cx sin2_t = cx(sin2_i) / (eta * eta);
+    Cx cos_t = csqrt(cx(1) - sin2_t);
+    Cx r_parl = (eta * cx(cos_i) - cos_t) / (eta * cx(cos_i) + cos_t);
+    Cx r_perp = (cx(cos_i) - eta * cos_t) / (cx(cos_i) + eta * cos_t);
+    return (cnorm(r_parl) + cnorm(r_perp)) / 2;
+}
+
+inline Bsdf make_bsdf(const Scene &sc, const Isect &is, const Lambda &lam) {
+    const Material &m = sc.mats[sc.tris[is.tri].material];
+    Bsdf b;
+    V3 x = normalize(is.dpdu);
+    b.fr = {x, cross(is.ns, x), is.ns};
+    b.type = m.type;
+    if (m.type == 0) {
+        S4 r = sample_sigmoid(m.albedo, lam);
+        for (int i = 0; i < kNS; ++i) r.v[i] = clampf(r.v[i], 0.0f, 1.0f);
+        b.r = r;
+        b.diffuse_set = positive(r);
+    } else {
+        b.eta = sample_pl(*m.eta, lam);
+        b.k = sample_pl(*m.k, lam);
+    }
+    return b;
+}
+// flags: 1 = reflection, 4 = diffuse, 16 = specular (base/bxdf.h BxDFFlags)
+inline uint32_t bsdf_flags(const Bsdf &b) {
+    if (b.type == 0) return b.diffuse_set ? (1u | 4u) : 0u;
+    return 1u | 16u;
+}
+inline bool non_specular(uint32_t f) { return (f & (4u | 8u)) != 0; }
+
+inline S4 bsdf_f(const Bsdf &b, V3 wo_r, V3 wi_r) {
+    V3 wi = b.fr.to_local(wi_r), wo = b.fr.to_local(wo_r);
+    if (wo.z == 0) return s4(0.f);
+    if (b.type == 0) {
+        if (!(wo.z * wi.z > 0)) return s4(0.f);
+        return b.r * kInvPi;
+    }
+    return s4(0.f);  // specular conductor
+}
+inline float bsdf_pdf(const Bsdf &b, V3 wo_r, V3 wi_r) {
+    V3 wo = b.fr.to_local(wo_r), wi = b.fr.to_local(wi_r);
+    if (wo.z == 0) return 0;
+    if (b.type == 0) {
+        if (!(wo.z * wi.z > 0)) return 0;
+        return std::fabs(wi.z) / kPi;
+    }
+    return 0;
+}
+struct BS {
+    S4 f;
+    V3 wi;
+    float pdf;
+    bool specular;
+};
+inline bool bsdf_sample(const Bsdf &b, V3 wo_r, float uc, float u0, float u1, BS *out) {
+    (void)uc;
+    V3 wo = b.fr.to_local(wo_r);
+    if (wo.z == 0 || bsdf_flags(b) == 0) return false;
+    BS bs;
+    if (b.type == 0) {
+        V3 wi = sample_cosine_hemisphere(u0, u1);
+        if (wo.z < 0) wi.z *= -1;
+        bs.pdf = std::fabs(wi.z) / kPi;
+        bs.f = b.r * kInvPi;
+        bs.wi = wi;
+        bs.specular = false;
+    } else {
+        V3 wi = {-wo.x, -wo.y, wo.z};
+        float c = std::fabs(wi.z);
+        S4 F;
+        for (int i = 0; i < kNS; ++i) F.v[i] = fr_complex(c, Cx{b.eta.v[i], b.k.v[i]});
+        bs.f = F / c;
+        bs.wi = wi;
+        bs.pdf = 1;
+        bs.specular = true;
+    }
+    if (!positive(bs.f) || bs.pdf == 0 || bs.wi.z == 0) return false;
+    bs.wi = b.fr.from_local(bs.wi);
+    *out = bs;
+    return true;
+}
+
+// ------------------------------------------------------------ triangle light sampling
+inline float spherical_triangle_area(V3 a, V3 b, V3 c) {
+    return float(std::fabs(2.0 * double(tatan2(dot(a, cross(b, c)), 1 + dot(a, b) + dot(a, c) + dot(b, c)))));
+}
+inline float tri_solid_angle(const Tri &t, V3 p) {
+    return spherical_triangle_area(normalize(sub(t.p[0], p)), normalize(sub(t.p[1], p)), normalize(sub(t.p[2], p)));
+}
+inline float sample_linear(float u, float a, float b) {
+    if (u == 0 && a == 0) return 0;
+    float x = u * (a + b) / (a + std::sqrt(lerpf(u, sqrf(a), sqrf(b))));
+    return std::min(x, kOneMinusEps);
+}
+inline void sample_bilinear(float u0, float u1, const float w[4], float *p0, float *p1) {
+    float y = sample_linear(u1, w[0] + w[1], w[2] + w[3]);
+    float x = sample_linear(u0, lerpf(y, w[0], w[2]), lerpf(y, w[1], w[3]));
+    *p0 = x;
+    *p1 = y;
+}
+inline float bilinear_pdf(float p0, float p1, const float w[4]) {
+    if (p0 < 0 || p0 > 1 || p1 < 0 || p1 > 1) return 0;
+    if (w[0] + w[1] + w[2] + w[3] == 0) return 1;
+    float s = (1 - p0) * (1 - p1) * w[0] + p0 * (1 - p1) * w[1] + (1 - p0) * p1 * w[2] + p0 * p1 * w[3];
+    return float(4.0 * double(s) / double(w[0] + w[1] + w[2] + w[3]));
+}
+// util/sampling.h:109-201
+inline void sample_spherical_triangle(float out[3], float *pdf, const V3 v[3], V3 p, float u0, float u1) {
+    *pdf = 0;
+    V3 a = normalize(sub(v[0], p)), b = normalize(sub(v[1], p)), c = normalize(sub(v[2], p));
+    V3 n_ab = cross(a, b), n_bc = cross(b, c), n_ca = cross(c, a);
+    if (sqlen(n_ab) == 0 || sqlen(n_bc) == 0 || sqlen(n_ca) == 0) return;
+    n_ab = normalize(n_ab);
+    n_bc = normalize(n_bc);
+    n_ca = normalize(n_ca);
+    float alpha = angle_between(n_ab, neg(n_ca));
+    float beta = angle_between(n_bc, neg(n_ab));
+    float gamma = angle_between(n_ca, neg(n_bc));
+    float A_pi = alpha + beta + gamma;
+    float Ap_pi = lerpf(u0, kPi, A_pi);
+    float A = A_pi - kPi;
+    *pdf = (A <= 0) ? 0 : 1 / A;
+    float cosAlpha = tcos(alpha), sinAlpha = tsin(alpha);
+    float sinPhi = tsin(Ap_pi) * cosAlpha - tcos(Ap_pi) * sinAlpha;
+    float cosPhi = tcos(Ap_pi) * cosAlpha + tsin(Ap_pi) * sinAlpha;
+    float k1 = cosPhi + cosAlpha;
+    float k2 = sinPhi - sinAlpha * dot(a, b);
+    float cosBp = (k2 + (dop(k2, cosPhi, k1, sinPhi)) * cosAlpha) / ((sop(k2, sinPhi, k1, cosPhi)) * sinAlpha);
+    cosBp = clampf(cosBp, -1, 1);
+    float sinBp = safe_sqrt(1 - sqrf(cosBp));
+    V3 cp = add(scale(cosBp, a), scale(sinBp, normalize(gram_schmidt(c, a))));
+    float cosTheta = 1 - u1 * (1 - dot(cp, b));
+    float sinTheta = safe_sqrt(1 - sqrf(cosTheta));
+    V3 w = add(scale(cosTheta, b), scale(sinTheta, normalize(gram_schmidt(cp, b))));
+    V3 e1 = sub(v[1], v[0]), e2 = sub(v[2], v[0]);
+    V3 s1 = cross(w, e2);
+    float divisor = dot(s1, e1);
+    if (divisor == 0) {
+        out[0] = out[1] = out[2] = float(1.0 / 3.0);
+        return;
+    }
+    float inv = 1 / divisor;
+    V3 s = sub(p, v[0]);
+    float b1 = dot(s, s1) * inv;
+    float b2 = dot(w, cross(s, e1)) * inv;
+    b1 = clampf(b1, 0, 1);
+    b2 = clampf(b2, 0, 1);
+    if (b1 + b2 > 1) {
+        b1 /= b1 + b2;
+        b2 /= b1 + b2;
+    }
+    out[0] = 1.0f - b1 - b2;
+    out[1] = b1;
+    out[2] = b2;
+}
+// util/sampling.cu:41-103 (Arvo's inversion)
+inline void invert_spherical_triangle(const V3 v[3], V3 p, V3 w, float *u0, float *u1) {
+    V3 a = normalize(sub(v[0], p)), b = normalize(sub(v[1], p)), c = normalize(sub(v[2], p));
+    V3 n_ab = cross(a, b), n_bc = cross(b, c), n_ca = cross(c, a);
+    if (sqlen(n_ab) == 0 || sqlen(n_bc) == 0 || sqlen(n_ca) == 0) {
+        *u0 = *u1 = 0;
+        return;
+    }
+    n_ab = normalize(n_ab);
+    n_bc = normalize(n_bc);
+    n_ca = normalize(n_ca);
+    float alpha = angle_between(n_ab, neg(n_ca));
+    float beta = angle_between(n_bc, neg(n_ab));
+    float gamma = angle_between(n_ca, neg(n_bc));
+    V3 cp = normalize(cross(cross(b, w), cross(c, a)));
+    if (dot(cp, add(a, c)) < 0) cp = neg(cp);
+    float uu0;
+    if (dot(a, cp) > 0.99999847691f) {
+        uu0 = 0;
+    } else {
+        V3 n_cpb = cross(cp, b), n_acp = cross(a, cp);
+        if (sqlen(n_cpb) == 0 || sqlen(n_acp) == 0) {
+            *u0 = *u1 = 0.5f;
+            return;
+        }
+        n_cpb = normalize(n_cpb);
+        n_acp = normalize(n_acp);
+        float Ap = alpha + angle_between(n_ab, n_cpb) + angle_between(n_acp, neg(n_cpb)) - kPi;
+        float A = alpha + beta + gamma - kPi;
+        uu0 = Ap / A;
+    }
+    float uu1 = (1 - dot(w, b)) / (1 - dot(cp, b));
+    *u0 = clampf(uu0, 0, 1);
+    *u1 = clampf(uu1, 0, 1);
+}
+
+struct LightCtx {  // LightSampleContext / ShapeSampleContext
+    P3fi pi;
+    V3 n, ns;
+};
+struct ShapeSample {
+    P3fi pi;
+    V3 n;
+    float pdf;
+};
+constexpr float kMinSA = 3e-4f, kMaxSA = 6.22f;
+
+inline V3 light_normal(const Tri &t) {
+    V3 n = normalize(cross(sub(t.p[1], t.p[0]), sub(t.p[2], t.p[0])));
+    return t.flip ? neg(n) : n;
+}
+inline P3fi sample_point(const Tri &t, const float b[3], V3 *p_out) {
+    V3 pabs = add(add(vabs(scale(b[0], t.p[0])), vabs(scale(b[1], t.p[1]))), vabs(scale(1 - b[0] - b[1], t.p[2])));
+    V3 perr = scale(gammaf(6), pabs);
+    V3 p = add(add(scale(b[0], t.p[0]), scale(b[1], t.p[1])), scale(b[2], t.p[2]));
+    *p_out = p;
+    return make_p3fi(p, perr);
+}
+// shapes/triangle.cu:126-215 (with the area-sampling path of :78-124)
+inline bool tri_sample(const Tri &t, const LightCtx &ctx, float u0, float u1, ShapeSample *ss) {
+    V3 cp = p3fi_mid(ctx.pi);
+    float sa = tri_solid_angle(t, cp);
+    if (sa < kMinSA || sa > kMaxSA) {
+        float b[3];
+        if (u0 < u1) {
+            b[0] = u0 / 2;
+            b[1] = u1 - b[0];
+        } else {
+            b[1] = u1 / 2;
+            b[0] = u0 - b[1];
+        }
+        b[2] = 1 - b[0] - b[1];
+        V3 p;
+        ss->pi = sample_point(t, b, &p);
+        ss->n = light_normal(t);
+        ss->pdf = float(1.0) / tri_area(t);
+        V3 pp = p3fi_mid(ss->pi);
+        V3 wi = sub(pp, cp);
+        if (sqlen(wi) == 0) return false;
+        wi = normalize(wi);
+        ss->pdf /= abs_dot(ss->n, neg(wi)) / sqlen(sub(cp, pp));
+        if (std::isinf(ss->pdf)) return false;
+        return true;
+    }
+    float pdf = 1.0f;
+    if (!v3_zero(ctx.ns)) {
+        V3 wi[3] = {normalize(sub(t.p[0], cp)), normalize(sub(t.p[1], cp)), normalize(sub(t.p[2], cp))};
+        float w[4] = {std::max(0.01f, abs_dot(ctx.ns, wi[1])), std::max(0.01f, abs_dot(ctx.ns, wi[1])),
+                      std::max(0.01f, abs_dot(ctx.ns, wi[0])), std::max(0.01f, abs_dot(ctx.ns, wi[2]))};
+        float a0, a1;
+        sample_bilinear(u0, u1, w, &a0, &a1);
+        u0 = a0;
+        u1 = a1;
+        pdf = bilinear_pdf(u0, u1, w);
+    }
+    float b[3], tri_pdf;
+    sample_spherical_triangle(b, &tri_pdf, t.p, cp, u0, u1);
+    if (tri_pdf == 0.0f) return false;
+    pdf *= tri_pdf;
+    V3 p;
+    ss->pi = sample_point(t, b, &p);
+    ss->n = light_normal(t);
+    ss->pdf = pdf;
+    return true;
+}
+// ShapeSampleContext::offset_ray_origin (base/shape.h:31-53): |n| . err, not |n . err|.
+inline V3 ctx_offset_origin(const LightCtx &c, V3 w) {
+    V3 err = p3fi_error(c.pi);
+    float d = dot(vabs(c.n), err);
+    V3 off = scale(d, c.n);
+    if (dot(c.n, w) < 0) off = neg(off);
+    V3 po = add(p3fi_mid(c.pi), off);
+    for (int i = 0; i < 3; ++i) {
+        if (off[i] > 0) po[i] = next_float_up(po[i]);
+        else if (off[i] < 0) po[i] = next_float_down(po[i]);
+    }
+    return po;
+}
+// shapes/triangle.cu:20-76
+inline float tri_pdf(const Tri &t, const LightCtx &ctx, V3 wi) {
+    V3 cp = p3fi_mid(ctx.pi);
+    float sa = tri_solid_angle(t, cp);
+    if (sa < kMinSA || sa > kMaxSA) {
+        V3 o = ctx_offset_origin(ctx, wi);
+        TriHit h;
+        if (!intersect_triangle(o, wi, std::numeric_limits<float>::infinity(), t.p[0], t.p[1], t.p[2], &h)) return 0;
+        HitGeom g = hit_geometry(h, t.p[0], t.p[1], t.p[2]);
+        V3 n = g.n;
+        if (t.flip) n = neg(n);
+        V3 ip = p3fi_mid(g.pi);
+        float pdf = (1 / tri_area(t)) / (abs_dot(n, neg(wi)) / sqlen(sub(cp, ip)));
+        if (std::isinf(pdf)) pdf = 0;
+        return pdf;
+    }
+    float pdf = 1 / sa;
+    if (!v3_zero(ctx.ns)) {
+        float u0, u1;
+        invert_spherical_triangle(t.p, cp, wi, &u0, &u1);
+        V3 w3[3] = {normalize(sub(t.p[0], cp)), normalize(sub(t.p[1], cp)), normalize(sub(t.p[2], cp))};
+        float w[4] = {std::max(0.01f, abs_dot(ctx.ns, w3[1])), std::max(0.01f, abs_dot(ctx.ns, w3[1])),
+                      std::max(0.01f, abs_dot(ctx.ns, w3[0])), std::max(0.01f, abs_dot(ctx.ns, w3[2]))};
+        pdf *= bilinear_pdf(u0, u1, w);
+    }
+    return pdf;
+}
+
+inline float power_heuristic(float f, float g) {  // sampling.h:259-269 with nf = ng = 1
+    float ff = 1 * f, gg = 1 * g;
+    if (std::isinf(sqrf(ff))) return 1;
+    return sqrf(ff) / (sqrf(ff) + sqrf(gg));
+}
+
+// Ray::spawn_ray_to (base/ray.h:45-51)
+inline void spawn_ray_to(const P3fi &from, V3 n_from, const P3fi &to, V3 n_to, V3 *o, V3 *d) {
+    V3 pf = offset_ray_origin(from, n_from, sub(p3fi_mid(to), p3fi_mid(from)));
+    V3 pt = offset_ray_origin(to, n_to, sub(pf, p3fi_mid(to)));
+    *o = pf;
+    *d = sub(pt, pf);
+}
+
+// ------------------------------------------------------------ the wavefront renderer
+struct PathSlot {
+    bool active = false, free = true;
+    uint32_t pixel = 0, sample = 0, depth = 0;
+    Sampler smp;
+    Lambda lam;
+    S4 L, beta;
+    float filter_w = 0;
+    bool specular = true, any_non_specular = false;
+    float pdf_bsdf = NAN, eta_scale = 1;
+    LightCtx prev;
+    V3 ro, rd;
+    bool hit = false;
+    uint32_t prim = 0;
+    TriHit h;
+    // deferred NEE: contribution and its two shadow rays
+    bool nee = false;
+    S4 nee_L;
+    V3 so[2], sd[2];
+};
+
+void parallel(size_t n, int threads, const std::function<void(size_t)> &fn) {
+    if (threads <= 1 || n < 256) {
+        for (size_t i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> ts;
+    for (int t = 0; t < threads; ++t)
+        ts.emplace_back([&] {
+            for (;;) {
+                size_t b = next.fetch_add(1024);
+                if (b >= n) break;
+                size_t e = std::min(n, b + 1024);
+                for (size_t i = b; i < e; ++i) fn(i);
+            }
+        });
+    for (auto &t : ts) t.join();
+}
+
+struct FilmRec {
+    uint32_t pixel, sample;
+    float rgb[3], w;
+};
+
+// PixelSensor::to_sensor_rgb (films/pixel_sensor.h:33-40) with imaging ratio 1.
+inline void sensor_rgb(const Scene &sc, const S4 &L, const Lambda &lam, float rgb[3]) {
+    S4 pdf;
+    for (int i = 0; i < kNS; ++i) pdf.v[i] = lam.pdf[i];
+    S4 s = safe_div(L, pdf);
+    for (int c = 0; c < 3; ++c) rgb[c] = 1.0f * average(sample_pl(sc.sp->cie[c], lam) * s);
+}
+
+void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64_t *stats) {
+    Scene sc;
+    build_scene(sc, d);
+    const uint32_t W = d.width, H = d.height;
+    const uint32_t spd = uint32_t(std::sqrt(double(d.spp)));
+    if (spd * spd != d.spp || d.spp == 0) throw std::runtime_error("spp must be a square");
+    const uint64_t total = uint64_t(W) * H * d.spp;
+    const uint32_t P = d.pool_size;
+    std::vector<PathSlot> pool(P);
+    std::vector<float> rgb_sum(size_t(W) * H * 3, 0.f), w_sum(size_t(W) * H, 0.f);
+    uint64_t next_id = 0, n_rays = 0, n_shadow = 0, n_done = 0, round = 0;
+    const float inf = std::numeric_limits<float>::infinity();
+
+    // generate_new_path (wavefront_path.cu:265-308)
+    auto gen = [&](std::vector<uint32_t> &slots) {
+        std::vector<std::pair<uint32_t, uint64_t>> work;
+        for (uint32_t s : slots) {
+            uint64_t id = next_id++;
+            if (id >= total) {
+                pool[s].active = false;
+                pool[s].free = false;
+                continue;
+            }
+            work.push_back({s, id});
+        }
+        parallel(work.size(), threads, [&](size_t k) {
+            PathSlot &p = pool[work[k].first];
+            uint64_t id = work[k].second;
+            p = PathSlot();
+            p.active = true;
+            p.free = false;
+            p.pixel = uint32_t(id % (uint64_t(W) * H));
+            p.sample = uint32_t(id / (uint64_t(W) * H));
+            p.smp.spd = spd;
+            p.smp.start(p.pixel, p.sample, 0);
+            float u0, u1;
+            p.smp.get_2d(&u0, &u1);
+            float fx, fy, fw;
+            sc.filter.sample(u0, u1, &fx, &fy, &fw);
+            float px = (float(p.pixel % W) + fx) + 0.5f, py = (float(p.pixel / W) + fy) + 0.5f;
+            p.filter_w = fw;
+            p.lam = sample_visible(p.smp.get_1d());
+            V3 pc = xf_point(sc.cam.camera_from_raster.m, {px, py, 0});
+            V3 dc = normalize(pc);
+            p.ro = xf_point(sc.cam.render_from_camera.m, {0, 0, 0});
+            p.rd = xf_vector(sc.cam.render_from_camera.m, dc);
+            p.L = s4(0);
+            p.beta = s4(1);
+            p.depth = 0;
+        });
+    };
+
+    std::vector<uint32_t> all(P);
+    for (uint32_t i = 0; i < P; ++i) all[i] = i;
+    gen(all);
+    std::vector<uint32_t> active_slots;
+    for (;;) {
+        active_slots.clear();
+        for (uint32_t i = 0; i < P; ++i)
+            if (pool[i].active) active_slots.push_back(i);
+        if (active_slots.empty()) break;
+        // ray_cast + the shadow rays of the previous round's NEE
+        parallel(active_slots.size(), threads, [&](size_t k) {
+            PathSlot &p = pool[active_slots[k]];
+            float t, b[3];
+            uint32_t prim;
+            int rc = orc_trace_ray(sc.bvh, &p.ro.x, &p.rd.x, inf, 0, &prim, &t, b);
+            if (rc < 0) throw std::runtime_error("oracle traversal stack overflow");
+            p.hit = rc == 1;
+            p.prim = prim;
+            p.h = TriHit{b[0], b[1], b[2], t};
+            if (p.nee) {
+                bool occluded = false;
+                for (int r = 0; r < 2 && !occluded; ++r) {
+                    uint32_t sp;
+                    int sr = orc_trace_ray(sc.bvh, &p.so[r].x, &p.sd[r].x, 0.6f, 1, &sp, nullptr, nullptr);
+                    occluded = sr == 1;
+                }
+                if (!occluded) p.L = p.L + p.nee_L;
+                p.nee = false;
+            }
+        });
+        for (uint32_t s : active_slots) {
+            n_rays++;
+            (void)s;
+        }
+        // control_logic (wavefront_path.cu:92-234)
+        std::vector<FilmRec> recs;
+        std::vector<uint32_t> freed, shade;
+        for (uint32_t s : active_slots) {
+            PathSlot &p = pool[s];
+            bool terminate = !p.hit || p.depth > d.max_depth || !positive(p.beta);
+            if (!terminate && p.depth > 8) {
+                float u = p.smp.get_1d();
+                S4 rr = p.beta * p.eta_scale;
+                if (max_comp(rr) < 1) {
+                    float q = clampf(1 - max_comp(rr), 0, 0.95f);
+                    if (u < q) {
+                        p.beta = s4(0.0f);
+                        terminate = true;
+                    } else {
+                        p.beta = p.beta / (1 - q);
+                    }
+                }
+            }
+            if (terminate) {
+                FilmRec r;
+                r.pixel = p.pixel;
+                r.sample = p.sample;
+                r.w = p.filter_w;
+                sensor_rgb(sc, p.L, p.lam, r.rgb);
+                recs.push_back(r);
+                p.active = false;
+                p.free = true;
+                freed.push_back(s);
+                n_done++;
+                continue;
+            }
+            const Tri &t = sc.tris[p.prim];
+            if (t.light >= 0) {
+                const Light &L = sc.lights[t.light];
+                Isect is = make_isect(sc, p.prim, p.h, p.rd);
+                S4 Le = light_L(sc, L, is.n, neg(p.rd), p.lam);
+                if (positive(Le)) {
+                    if (p.depth == 0 || p.specular) {
+                        p.L = p.L + p.beta * Le;
+                    } else {
+                        float pdf_light = sc.light_pmf[t.light] * tri_pdf(t, p.prev, p.rd);
+                        float wl = power_heuristic(p.pdf_bsdf, pdf_light);
+                        p.L = p.L + (p.beta * wl) * Le;
+                    }
+                }
+            }
+            p.depth += 1;
+            shade.push_back(s);
+        }
+        // write_frame_buffer: sorted by (pixel, sample), film add_sample (rgb_film.cu:85-102)
+        std::sort(recs.begin(), recs.end(), [](const FilmRec &a, const FilmRec &b) {
+            return a.pixel != b.pixel ? a.pixel < b.pixel : a.sample < b.sample;
+        });
+        for (const FilmRec &r : recs) {
+            if (r.w == 0) continue;
+            for (int c = 0; c < 3; ++c) rgb_sum[3 * r.pixel + c] += r.w * r.rgb[c];
+            w_sum[r.pixel] += r.w;
+        }
+        gen(freed);
+        // gpu_evaluate_material + sample_bsdf (wavefront_path.cu:310-431)
+        parallel(shade.size(), threads, [&](size_t k) {
+            PathSlot &p = pool[shade[k]];
+            Isect is = make_isect(sc, p.prim, p.h, p.rd);
+            Bsdf b = make_bsdf(sc, is, p.lam);
+            uint32_t flags = bsdf_flags(b);
+            if (non_specular(flags)) {
+                // sample_ld (wavefront_path.cu:574-624)
+                LightCtx ctx{is.pi, is.n, is.ns};
+                bool refl = (flags & 1u) != 0, trans = (flags & 2u) != 0;
+                if (refl && !trans) {
+                    V3 po = offset_ray_origin(is.pi, is.n, is.wo);
+                    ctx.pi = make_p3fi(po, {0, 0, 0});
+                } else if (trans && !refl) {
+                    V3 po = offset_ray_origin(is.pi, is.n, neg(is.wo));
+                    ctx.pi = make_p3fi(po, {0, 0, 0});
+                }
+                float u = p.smp.get_1d();
+                float ul0, ul1;
+                p.smp.get_2d(&ul0, &ul1);
+                if (!sc.lights.empty()) {
+                    uint32_t li = search_cdf(u, sc.light_cdf.data(), uint32_t(sc.lights.size()));
+                    float pmf = sc.light_pmf[li];
+                    const Light &L = sc.lights[li];
+                    const Tri &lt = sc.tris[L.tri];
+                    ShapeSample ss;
+                    bool ok = tri_sample(lt, ctx, ul0, ul1, &ss);
+                    V3 cpt = p3fi_mid(ctx.pi);
+                    V3 sp = p3fi_mid(ss.pi);
+                    if (ok && !(ss.pdf == 0 || sqlen(sub(sp, cpt)) == 0)) {
+                        V3 wi = normalize(sub(sp, cpt));
+                        S4 Le = light_L(sc, L, ss.n, neg(wi), p.lam);
+                        if (positive(Le) && ss.pdf != 0) {
+                            S4 f = bsdf_f(b, is.wo, wi) * abs_dot(wi, is.ns);
+                            if (positive(f)) {
+                                float pdf_light = pmf * ss.pdf;
+                                float pdf_b = bsdf_pdf(b, is.wo, wi);
+                                float wl = power_heuristic(pdf_light, pdf_b);
+                                S4 Ld = ((Le * wl) * f) / pdf_light;
+                                p.nee = true;
+                                p.nee_L = p.beta * Ld;
+                                spawn_ray_to(is.pi, is.n, ss.pi, ss.n, &p.so[0], &p.sd[0]);
+                                spawn_ray_to(ss.pi, ss.n, is.pi, is.n, &p.so[1], &p.sd[1]);
+                            }
+                        }
+                    }
+                }
+            }
+            V3 wo = neg(p.rd);
+            float uc = p.smp.get_1d();
+            float u0, u1;
+            p.smp.get_2d(&u0, &u1);
+            BS bs;
+            if (!bsdf_sample(b, wo, uc, u0, u1, &bs)) {
+                p.beta = s4(0.0f);
+                return;
+            }
+            p.beta = p.beta * ((bs.f * abs_dot(bs.wi, is.ns)) / bs.pdf);
+            p.pdf_bsdf = bs.pdf;
+            p.specular = bs.specular;
+            p.any_non_specular |= !bs.specular;
+            p.prev = LightCtx{is.pi, is.n, is.ns};
+            p.ro = offset_ray_origin(is.pi, is.n, bs.wi);
+            p.rd = bs.wi;
+        });
+        for (uint32_t s : shade)
+            if (pool[s].nee) n_shadow += 2;
+        round++;
+    }
+    // RGBFilm::get_pixel_rgb (rgb_film.cu:129-144)
+    for (uint32_t px = 0; px < W * H; ++px) {
+        float rgb[3] = {rgb_sum[3 * px], rgb_sum[3 * px + 1], rgb_sum[3 * px + 2]};
+        if (w_sum[px] != 0)
+            for (int c = 0; c < 3; ++c) rgb[c] /= w_sum[px];
+        auto o = m3_apply(sc.out_from_sensor, rgb);
+        for (int c = 0; c < 3; ++c) film_rgb[3 * size_t(px) + c] = o[c];
+        if (aux) {
+            for (int c = 0; c < 3; ++c) aux[4 * size_t(px) + c] = rgb_sum[3 * px + c];
+            aux[4 * size_t(px) + 3] = w_sum[px];
+        }
+    }
+    if (stats) {
+        stats[0] = round;
+        stats[1] = n_rays;
+        stats[2] = n_shadow;
+        stats[3] = n_done;
+    }
+}
+
+}  // namespace
+}  // namespace orc
+
+extern "C" {
+
+int orc_render(const orc_scene *scene, int threads, float *film_rgb, float *aux, uint64_t *stats) {
+    if (!scene || !film_rgb || !scene->tri || !scene->tri_material || scene->pool_size == 0) return -1;
+    try {
+        orc::render(*scene, threads, film_rgb, aux, stats);
+    } catch (const std::exception &) {
+        return -2;
+    }
+    return 0;
+}
+
+int orc_rgb_to_sigmoid(const float rgb[3], float out3[3]) {
+    orc::Sigmoid s = orc::spectra().table.lookup(rgb);
+    out3[0] = s.c0;
+    out3[1] = s.c1;
+    out3[2] = s.c2;
+    return 0;
+}
+
+int orc_srgb_matrices(float xyz_from_rgb9[9], float rgb_from_xyz9[9], float white_xy2[2]) {
+    const orc::Spectra &s = orc::spectra();
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            xyz_from_rgb9[3 * i + j] = s.xyz_from_rgb.m[i][j];
+            rgb_from_xyz9[3 * i + j] = s.rgb_from_xyz.m[i][j];
+        }
+    white_xy2[0] = s.white_xy[0];
+    white_xy2[1] = s.white_xy[1];
+    return 0;
+}
+
+int orc_filter_sample(const float u[2], float out3[3]) {
+    static orc::FilterSampler f;
+    f.sample(u[0], u[1], &out3[0], &out3[1], &out3[2]);
+    return 0;
+}
+
+float orc_spectrum_eval(int which, float lambda) {
+    const orc::Spectra &s = orc::spectra();
+    if (which >= 0 && which < 3) return s.cie[which](lambda);
+    if (which == 3) return s.d65(lambda);
+    static std::vector<orc::PLSpectrum> metals;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const float *d[8] = {pbrtm_data::kAgEta, pbrtm_data::kAgK, pbrtm_data::kAlEta, pbrtm_data::kAlK,
+                             pbrtm_data::kAuEta, pbrtm_data::kAuK, pbrtm_data::kCuEta, pbrtm_data::kCuK};
+        const int n[8] = {pbrtm_data::kAgEtaLen, pbrtm_data::kAgKLen, pbrtm_data::kAlEtaLen, pbrtm_data::kAlKLen,
+                          pbrtm_data::kAuEtaLen, pbrtm_data::kAuKLen, pbrtm_data::kCuEtaLen, pbrtm_data::kCuKLen};
+        for (int i = 0; i < 8; ++i) metals.push_back(orc::pl_from_interleaved(d[i], n[i]));
+    });
+    if (which >= 4 && which < 12) return metals[which - 4](lambda);
+    return NAN;
+}
+
+}  // extern "C"

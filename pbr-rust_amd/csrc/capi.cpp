@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "hlbvh_build.h"
+#include "pathtrace.h"
 #include "pbrtm_geometry.h"
 #include "traverse.h"
 #include "wide_bvh.h"
@@ -549,6 +550,190 @@ int pbrtm_gen_camera_rays(uint32_t width, uint32_t height, const float eye[3], c
             o[7] = 0.f;
         }
     }
+    return PBRTM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ path tracer
+
+struct pbrtm_scene {
+    pbrtm_ctx *ctx = nullptr;  // must outlive the handle
+    int device = 0;
+    pbrtm_bvh *bvh = nullptr;
+    pbrtm::pt::SceneHost host;
+    pbrtm::DevBuf tri, tri_info, mats, lights, pmf, cdf, spec_lam, spec_val, filt;
+    pbrtm::pt::SceneDev dev{};
+    pbrtm::pt::PathPool pool;
+    pbrtm::DevBuf film_rgb, film_aux;
+};
+
+extern "C" {
+
+void pbrtm_scene_free(pbrtm_scene *sc) {
+    if (!sc) return;
+    hipSetDevice(sc->device);
+    if (sc->bvh) pbrtm_bvh_free(sc->bvh);
+    for (pbrtm::DevBuf *b : {&sc->tri, &sc->tri_info, &sc->mats, &sc->lights, &sc->pmf, &sc->cdf, &sc->spec_lam,
+                             &sc->spec_val, &sc->filt, &sc->film_rgb, &sc->film_aux})
+        b->release();
+    auto &p = sc->pool;
+    for (pbrtm::DevBuf *b : {&p.path_id, &p.flags, &p.rng, &p.dim, &p.lam, &p.lam_pdf, &p.L, &p.beta, &p.nee_L,
+                             &p.scal, &p.prev, &p.rays, &p.prim, &p.thit, &p.bary, &p.srays, &p.sprim, &p.rec,
+                             &p.rec_round, &p.counters, &p.queue, &p.squeue, &p.spill})
+        b->release();
+    delete sc;
+}
+
+int pbrtm_scene_create(pbrtm_ctx *ctx, const pbrtm_scene_desc *desc, pbrtm_scene **out) {
+    if (!ctx || !desc || !out) return PBRTM_ERR_INVALID;
+    *out = nullptr;
+    if (desc->num_tris == 0) return fail(ctx, PBRTM_ERR_INVALID, "scene has no triangles");
+    CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    auto *sc = new (std::nothrow) pbrtm_scene();
+    if (!sc) return fail(ctx, PBRTM_ERR_NOMEM, "handle");
+    sc->ctx = ctx;
+    sc->device = ctx->device;
+    auto bail = [&](int code) {
+        pbrtm_scene_free(sc);
+        return code;
+    };
+    const std::string msg = pbrtm::pt::build_scene_host(desc, sc->host);
+    if (!msg.empty()) return bail(fail(ctx, PBRTM_ERR_INVALID, msg));
+    const auto &H = sc->host;
+    auto up = [&](pbrtm::DevBuf &b, const void *src, size_t bytes) -> hipError_t {
+        hipError_t e = b.reserve(std::max<size_t>(bytes, 4));
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+        return e;
+    };
+    hipError_t e = up(sc->tri, H.tri.data(), H.tri.size() * 4);
+    if (e == hipSuccess) e = up(sc->tri_info, H.tri_info.data(), H.tri_info.size() * sizeof(uint2));
+    if (e == hipSuccess) e = up(sc->mats, H.mats.data(), H.mats.size() * sizeof(pbrtm::pt::DMat));
+    if (e == hipSuccess) e = up(sc->lights, H.lights.data(), H.lights.size() * sizeof(pbrtm::pt::DLight));
+    if (e == hipSuccess) e = up(sc->pmf, H.light_pmf.data(), H.light_pmf.size() * 4);
+    if (e == hipSuccess) e = up(sc->cdf, H.light_cdf.data(), H.light_cdf.size() * 4);
+    if (e == hipSuccess) e = up(sc->spec_lam, H.spec_lam.data(), H.spec_lam.size() * 4);
+    if (e == hipSuccess) e = up(sc->spec_val, H.spec_val.data(), H.spec_val.size() * 4);
+    const size_t fn = H.filt_n;
+    std::vector<float> filt;
+    for (const auto *v : {&H.filt_f, &H.filt_cond_func, &H.filt_cond_cdf, &H.filt_cond_int, &H.filt_marg_func,
+                          &H.filt_marg_cdf})
+        filt.insert(filt.end(), v->begin(), v->end());
+    if (e == hipSuccess) e = up(sc->filt, filt.data(), filt.size() * 4);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return bail(hip_fail(ctx, e, "scene upload"));
+    pbrtm::pt::SceneDev &D = sc->dev;
+    D.tri = sc->tri.as<float>();
+    D.tri_info = sc->tri_info.as<uint2>();
+    D.mats = sc->mats.as<pbrtm::pt::DMat>();
+    D.lights = sc->lights.as<pbrtm::pt::DLight>();
+    D.light_pmf = sc->pmf.as<float>();
+    D.light_cdf = sc->cdf.as<float>();
+    D.num_tris = uint32_t(desc->num_tris);
+    D.num_lights = uint32_t(H.lights.size());
+    D.spec_lam = sc->spec_lam.as<float>();
+    D.spec_val = sc->spec_val.as<float>();
+    for (int k = 0; k < 3; ++k) D.cie_off[k] = H.cie_off[k];
+    D.cie_n = H.cie_n;
+    D.d65_off = H.d65_off;
+    D.d65_n = H.d65_n;
+    std::memcpy(D.cam_from_raster, H.cam_from_raster, sizeof(D.cam_from_raster));
+    std::memcpy(D.render_from_camera, H.render_from_camera, sizeof(D.render_from_camera));
+    std::memcpy(D.cam_origin, H.cam_origin, sizeof(D.cam_origin));
+    const float *fb = sc->filt.as<float>();
+    D.filt_f = fb;
+    D.filt_cond_func = fb + fn * fn;
+    D.filt_cond_cdf = D.filt_cond_func + fn * fn;
+    D.filt_cond_int = D.filt_cond_cdf + fn * (fn + 1);
+    D.filt_marg_func = D.filt_cond_int + fn;
+    D.filt_marg_cdf = D.filt_marg_func + fn;
+    D.filt_marg_int = H.filt_marg_int;
+    D.filt_radius = H.filt_radius;
+    D.filt_n = H.filt_n;
+    std::memcpy(D.out_from_sensor, H.out_from_sensor, sizeof(D.out_from_sensor));
+    D.width = H.width;
+    D.height = H.height;
+    // The aggregate over the render-space triangles (SceneBuilder::preprocess, scene_builder.cu:829-838).
+    const int rc = pbrtm_bvh_build(ctx, sc->tri.as<float>(), 3 * desc->num_tris, nullptr, desc->num_tris,
+                                   PBRTM_DEVICE_PTRS, &sc->bvh, nullptr);
+    if (rc != PBRTM_OK) return bail(rc);
+    *out = sc;
+    return PBRTM_OK;
+}
+
+int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *params, float *film_rgb,
+                 float *film_aux, pbrtm_render_stats *stats) {
+    if (!ctx || !sc || !params || !film_rgb) return PBRTM_ERR_INVALID;
+    if (sc->ctx != ctx) return fail(ctx, PBRTM_ERR_INVALID, "scene belongs to another context");
+    const uint32_t spp = params->spp;
+    const uint32_t spd = uint32_t(std::sqrt(double(spp)));
+    if (spp == 0 || spd * spd != spp) return fail(ctx, PBRTM_ERR_INVALID, "spp must be a positive square (stratified)");
+    if (spp > 1024) return fail(ctx, PBRTM_ERR_UNSUPPORTED, "at most 1024 samples per pixel");
+    const uint64_t wh = uint64_t(sc->host.width) * sc->host.height;
+    if (wh * spp >= 0xFFFFFFFFull) return fail(ctx, PBRTM_ERR_UNSUPPORTED, "at most 2^32 - 1 paths per render");
+    if (params->max_depth > 0xFFFE) return fail(ctx, PBRTM_ERR_INVALID, "max_depth too large");
+    const uint32_t pool = params->pool_size ? params->pool_size : (2u << 20);
+    CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    const bool dev = params->flags & PBRTM_DEVICE_PTRS;
+    float *d_rgb = film_rgb, *d_aux = film_aux;
+    if (!dev) {
+        CK(ctx, sc->film_rgb.reserve(wh * 12), "alloc film");
+        d_rgb = sc->film_rgb.as<float>();
+        if (film_aux) {
+            CK(ctx, sc->film_aux.reserve(wh * 16), "alloc film");
+            d_aux = sc->film_aux.as<float>();
+        }
+    }
+    pbrtm_bvh *bvh = sc->bvh;
+    WideTrace W{};
+    W.nodes = bvh->wide.nodes.as<WNode>();
+    W.order = bvh->wide.order.as<uint32_t>();
+    W.tri_leaf = bvh->tri_leaf.as<float>();
+    W.prim_order = bvh->dev.prim_order.as<uint32_t>();
+    W.num_nodes = bvh->wide.num_nodes;
+    W.num_prims = bvh->n;
+    W.max_depth = bvh->wide.max_depth;
+    W.spill_per_ray = W.max_depth > kLdsStack ? W.max_depth - kLdsStack + 1 : 0;
+    CK(ctx, ctx->dev_err.reserve(64), "alloc");
+    CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream), "memset");
+    pbrtm::pt::RenderStats rs;
+    CK(ctx, pbrtm::pt::render(sc->dev, W, spp, params->max_depth, pool, sc->pool, d_rgb, d_aux,
+                              ctx->dev_err.as<uint32_t>(), &rs, ctx->stream), "render");
+    uint32_t derr = 0;
+    CK(ctx, hipMemcpyAsync(&derr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+    if (!dev) {
+        CK(ctx, hipMemcpyAsync(film_rgb, d_rgb, wh * 12, hipMemcpyDeviceToHost, ctx->stream), "download");
+        if (film_aux) CK(ctx, hipMemcpyAsync(film_aux, d_aux, wh * 16, hipMemcpyDeviceToHost, ctx->stream), "download");
+    }
+    CK(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (derr) return fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr));
+    if (stats) {
+        stats->ms_total = rs.ms_total;
+        stats->ms_trace = rs.ms_trace;
+        stats->ms_shadow = rs.ms_shadow;
+        stats->ms_shade = rs.ms_shade;
+        stats->ms_film = rs.ms_film;
+        stats->rounds = rs.rounds;
+        stats->rays = rs.rays;
+        stats->shadow_rays = rs.shadow_rays;
+        stats->paths = rs.paths;
+    }
+    return PBRTM_OK;
+}
+
+int pbrtm_rgb_to_sigmoid(const float rgb[3], float out3[3]) {
+    if (!rgb || !out3) return PBRTM_ERR_INVALID;
+    try {
+        pbrtm::pt::rgb_to_sigmoid(rgb, out3);
+    } catch (const std::exception &) {
+        return PBRTM_ERR_INVALID;
+    }
+    return PBRTM_OK;
+}
+
+int pbrtm_srgb_matrices(float xyz_from_rgb9[9], float rgb_from_xyz9[9], float white_xy2[2]) {
+    if (!xyz_from_rgb9 || !rgb_from_xyz9 || !white_xy2) return PBRTM_ERR_INVALID;
+    pbrtm::pt::srgb_matrices(xyz_from_rgb9, rgb_from_xyz9, white_xy2);
     return PBRTM_OK;
 }
 

@@ -1,0 +1,107 @@
+// Wavefront path tracer: device scene tables, path-pool state and the render
+// driver. The pool holds `pool` path slots in structure-of-arrays form in HBM;
+// each round traces every live slot's ray (closest hit) and the previous
+// round's shadow rays (any hit) through the 8-wide BVH, then one shading kernel
+// per slot resolves next-event estimation, applies the integrator's control
+// logic (termination, emission with MIS), records finished paths for the film,
+// starts new paths in freed slots and samples the BSDF and a light.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "pbrtm_internal.h"
+#include "wide_bvh.h"
+
+namespace pbrtm {
+namespace pt {
+
+enum MatType : uint32_t { kDiffuse = 0, kConductor = 1 };
+
+struct DMat {  // 48 bytes
+    uint32_t type;
+    float c0, c1, c2;               // diffuse albedo sigmoid coefficients
+    uint32_t eta_off, eta_n, k_off, k_n;  // conductor eta/k in the spectrum pool
+    float ax, ay;                   // Trowbridge-Reitz alphas
+    uint32_t pad[2];
+};
+
+struct DLight {  // 32 bytes: a DiffuseAreaLight on one triangle
+    uint32_t tri;
+    float scale;      // "scale" / to_photometric(L)
+    float rgb_scale;  // RGBIlluminantSpectrum scale = 2 max(rgb)
+    float c0, c1, c2; // sigmoid of rgb / rgb_scale
+    uint32_t two_sided;
+    float pad;
+};
+
+// Everything the kernels read about the scene; passed by value.
+struct SceneDev {
+    const float *tri;          // [n][9] render space, input order
+    const uint2 *tri_info;     // x = material, y = light + 1 (0 = none) | flip << 31
+    const DMat *mats;
+    const DLight *lights;
+    const float *light_pmf, *light_cdf;
+    uint32_t num_tris, num_lights;
+    const float *spec_lam, *spec_val;  // piecewise-linear spectrum pool
+    uint32_t cie_off[3], cie_n, d65_off, d65_n;
+    float cam_from_raster[16], render_from_camera[16];
+    float cam_origin[3];
+    const float *filt_f;         // [64][64] filter values
+    const float *filt_cond_func; // [64][64] |f|
+    const float *filt_cond_cdf;  // [64][65]
+    const float *filt_cond_int;  // [64]
+    const float *filt_marg_func; // [64]
+    const float *filt_marg_cdf;  // [65]
+    float filt_marg_int, filt_radius;
+    uint32_t filt_n;
+    float out_from_sensor[9];
+    uint32_t width, height;
+};
+
+// Host-side scene set-up (render space, spectra, camera, filter, lights).
+struct SceneHost {
+    std::vector<float> tri;  // render space
+    std::vector<uint2> tri_info;
+    std::vector<DMat> mats;
+    std::vector<DLight> lights;
+    std::vector<float> light_pmf, light_cdf;
+    std::vector<float> spec_lam, spec_val;
+    uint32_t cie_off[3], cie_n, d65_off, d65_n;
+    float cam_from_raster[16], render_from_camera[16], cam_origin[3];
+    std::vector<float> filt_f, filt_cond_func, filt_cond_cdf, filt_cond_int, filt_marg_func, filt_marg_cdf;
+    float filt_marg_int, filt_radius;
+    uint32_t filt_n;
+    float out_from_sensor[9];
+    uint32_t width, height;
+};
+
+struct SceneDesc;  // the C-ABI description, see include/pbrtm.h
+
+// Builds the host tables; returns an empty string or an error message.
+std::string build_scene_host(const void *desc, SceneHost &out);
+
+// RGB-to-spectrum sigmoid coefficients of an albedo RGB, and the sRGB matrices (for tests).
+void rgb_to_sigmoid(const float rgb[3], float out[3]);
+void srgb_matrices(float xyz_from_rgb[9], float rgb_from_xyz[9], float white_xy[2]);
+
+struct PathPool {
+    DevBuf path_id, flags, rng, dim, lam, lam_pdf, L, beta, nee_L, scal, prev, rays, prim, thit, bary, srays, sprim,
+        rec, rec_round, counters, queue, squeue, spill;
+    uint32_t cap = 0;
+    uint64_t rec_cap = 0;
+};
+
+struct RenderStats {
+    double ms_total = 0, ms_trace = 0, ms_shadow = 0, ms_shade = 0, ms_film = 0;
+    uint64_t rounds = 0, rays = 0, shadow_rays = 0, paths = 0;
+};
+
+// Renders the whole image; film_rgb (device, [h][w][3]) and film_aux (device,
+// [h][w][4], nullable) receive RGBFilm::get_pixel_rgb and (rgb_sum, weight_sum).
+hipError_t render(const SceneDev &S, const WideTrace &W, uint32_t spp, uint32_t max_depth, uint32_t pool,
+                  PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
+                  hipStream_t stream);
+
+}  // namespace pt
+}  // namespace pbrtm

@@ -1,0 +1,733 @@
+// Wavefront path-tracer kernels and the round loop that drives them.
+//
+// Reference: integrators/wavefront_path.cu (render :626-707, generate_new_path
+// :265-308, control_logic :92-234, gpu_evaluate_material/sample_bsdf
+// :310-431, sample_ld :574-624, write_frame_buffer :236-255). Differences in
+// scheduling, none in results: control, material evaluation and new-path
+// generation run fused in one kernel per round; the two shadow rays of a
+// light sample are traced as a batch in the next round before the shading
+// kernel adds the deferred contribution (the reference traces them inline;
+// the contribution enters L at the same point of the sum); finished paths
+// keep their film contribution per path id and the film kernel adds them per
+// pixel in the reference's order (round, then sample).
+#include "pathtrace.h"
+#include "pt_device.h"
+#include "wave_ops.h"
+
+namespace pbrtm {
+namespace pt {
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kFlagSpecular = 1u << 16, kFlagAnyNonSpec = 1u << 17, kFlagNee = 1u << 18;
+constexpr uint32_t kDepthMask = 0xFFFFu;
+constexpr uint32_t kIdle = 0xFFFFFFFFu;
+
+// counters[] layout
+constexpr int kCntNextId = 0, kCntActive = 1, kCntQueue = 2, kCntSQueue = 3, kCntShadow = 4;
+
+struct State {
+    uint32_t *path_id, *flags, *dim;
+    uint64_t *rng;
+    float4 *lam, *lam_pdf, *L, *beta, *nee_L, *scal, *prev;  // scal = (filter_w, pdf_bsdf, eta_scale, -)
+    float4 *rays;                                             // [2 per slot]
+    const uint32_t *prim;
+    const float *thit, *bary;
+    float4 *srays;                                            // [4 per slot]
+    const uint32_t *sprim;                                    // [2 per slot]
+    float4 *rec;                                              // per path id: sensor RGB, filter weight
+    uint32_t *rec_round;
+    uint32_t *counters;
+    uint32_t *err;
+};
+
+struct Params {
+    uint64_t total;
+    uint32_t spd, spp, max_depth, round, pool;
+};
+
+__device__ __forceinline__ f3 ld3(const float *p) { return {p[0], p[1], p[2]}; }
+
+__device__ __forceinline__ f3 xf_point(const float *m, f3 p) {  // transform.h:254-265
+    const float xp = m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3];
+    const float yp = m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7];
+    const float zp = m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11];
+    const float wp = m[12] * p.x + m[13] * p.y + m[14] * p.z + m[15];
+    if (wp == 1.0f) return {xp, yp, zp};
+    return {xp / wp, yp / wp, zp / wp};
+}
+__device__ __forceinline__ f3 xf_vector(const float *m, f3 v) {  // transform.h:303-307
+    return {m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+            m[8] * v.x + m[9] * v.y + m[10] * v.z};
+}
+
+__device__ __forceinline__ float spec_eval(const SceneDev &S, uint32_t off, uint32_t n, float l) {
+    return pl_eval(S.spec_lam + off, S.spec_val + off, n, l);
+}
+
+// Piecewise-constant 1D sample (util/piecewise_constant_1d.h:61-86).
+__device__ __forceinline__ float pc1d_sample(const float *func, const float *cdf, uint32_t n, float func_int,
+                                             float mn, float mx, float u, float *pdf, int *off) {
+    const int o = int(find_interval_le(cdf, n + 1, u));
+    *off = o;
+    float du = u - cdf[o];
+    if (cdf[o + 1] - cdf[o] > 0) du /= cdf[o + 1] - cdf[o];
+    *pdf = (func_int > 0) ? func[o] / func_int : 0;
+    return lerpf((float(o) + du) / float(n), mn, mx);
+}
+
+// FilterSampler::sample (base/filter.h:66-73) over the tabulated Mitchell filter.
+__device__ __forceinline__ void filter_sample(const SceneDev &S, float u0, float u1, float *px, float *py, float *w) {
+    const uint32_t n = S.filt_n;
+    const float r = S.filt_radius;
+    float pdf0, pdf1;
+    int o0, o1;
+    const float d1 = pc1d_sample(S.filt_marg_func, S.filt_marg_cdf, n, S.filt_marg_int, -r, r, u1, &pdf1, &o1);
+    const float d0 = pc1d_sample(S.filt_cond_func + size_t(o1) * n, S.filt_cond_cdf + size_t(o1) * (n + 1), n,
+                                 S.filt_cond_int[o1], -r, r, u0, &pdf0, &o0);
+    *px = d0;
+    *py = d1;
+    *w = S.filt_f[o0 + n * o1] / (pdf0 * pdf1);
+}
+
+// distribution_1d.h:9-42
+__device__ __forceinline__ uint32_t search_cdf(float u, const float *cdf, uint32_t len) {
+    if (u < cdf[0] || len == 1) return 0;
+    uint32_t start = 1, end = len;
+    while (true) {
+        if (end - start <= 10) {
+            uint32_t t = start;
+            for (uint32_t i = start; i < end; ++i)
+                if (u >= cdf[i - 1] && u < cdf[i]) {
+                    t = i;
+                    break;
+                }
+            return t;
+        }
+        const uint32_t mid = (start + end) / 2;
+        if (u >= cdf[mid]) start = mid + 1;
+        else end = mid + 1;
+    }
+}
+
+struct Tri {
+    f3 p0, p1, p2;
+    bool flip;
+};
+__device__ __forceinline__ Tri load_tri(const SceneDev &S, uint32_t i) {
+    const float *t = S.tri + size_t(i) * 9;
+    return {ld3(t), ld3(t + 3), ld3(t + 6), (S.tri_info[i].y >> 31) != 0};
+}
+__device__ __forceinline__ float tri_area(const Tri &t) {
+    return len3(cross3(sub3(t.p1, t.p0), sub3(t.p2, t.p0))) * 0.5f;
+}
+__device__ __forceinline__ f3 tri_normal(const Tri &t) {  // shapes/triangle.cu:391-395
+    f3 n = norm3(cross3(sub3(t.p0, t.p2), sub3(t.p1, t.p2)));
+    return t.flip ? neg3(n) : n;
+}
+__device__ __forceinline__ f3 light_normal(const Tri &t) {  // shapes/triangle.cu:185-194
+    f3 n = norm3(cross3(sub3(t.p1, t.p0), sub3(t.p2, t.p0)));
+    return t.flip ? neg3(n) : n;
+}
+
+// DiffuseAreaLight::l (lights/diffuse_area_light.cu:37-46) with RGBIlluminantSpectrum.
+__device__ __forceinline__ S4 light_L(const SceneDev &S, const DLight &L, f3 n, f3 w, const S4 &lam) {
+    if (!L.two_sided && dot3(n, w) < 0.0f) return s4(0.0f);
+    S4 s, d;
+    for (int i = 0; i < kNS; ++i) {
+        s.v[i] = L.rgb_scale * sigmoid_eval(L.c0, L.c1, L.c2, lam.v[i]);
+        d.v[i] = spec_eval(S, S.d65_off, S.d65_n, lam.v[i]);
+    }
+    return s4scale(s4mul(d, s), L.scale);
+}
+
+struct ShapeSample {
+    P3i pi;
+    f3 n;
+    float pdf;
+};
+__device__ __forceinline__ P3i sample_point(const Tri &t, const float b[3]) {
+    const f3 pabs = add3(add3(abs3(mul3(b[0], t.p0)), abs3(mul3(b[1], t.p1))), abs3(mul3(1 - b[0] - b[1], t.p2)));
+    const f3 perr = mul3(gamma_n(6), pabs);
+    const f3 p = add3(add3(mul3(b[0], t.p0), mul3(b[1], t.p1)), mul3(b[2], t.p2));
+    return p3i_make(p, perr);
+}
+// Triangle::sample(ctx, u) (shapes/triangle.cu:126-215) and its area-sampling branch (:78-124).
+__device__ bool tri_sample(const Tri &t, const P3i &cpi, f3 cns, float u0, float u1, ShapeSample *ss) {
+    const f3 cp = p3i_mid(cpi);
+    const f3 v[3] = {t.p0, t.p1, t.p2};
+    const float sa = spherical_triangle_area(norm3(sub3(t.p0, cp)), norm3(sub3(t.p1, cp)), norm3(sub3(t.p2, cp)));
+    if (sa < kMinSolidAngle || sa > kMaxSolidAngle) {
+        float b[3];
+        if (u0 < u1) {
+            b[0] = u0 / 2;
+            b[1] = u1 - b[0];
+        } else {
+            b[1] = u1 / 2;
+            b[0] = u0 - b[1];
+        }
+        b[2] = 1 - b[0] - b[1];
+        ss->pi = sample_point(t, b);
+        ss->n = light_normal(t);
+        ss->pdf = float(1.0) / tri_area(t);
+        const f3 pp = p3i_mid(ss->pi);
+        f3 wi = sub3(pp, cp);
+        if (sqlen3(wi) == 0) return false;
+        wi = norm3(wi);
+        ss->pdf /= absdot3(ss->n, neg3(wi)) / sqlen3(sub3(cp, pp));
+        if (isinf(ss->pdf)) return false;
+        return true;
+    }
+    float pdf = 1.0f;
+    if (!zero3(cns)) {
+        const f3 w0 = norm3(sub3(t.p0, cp)), w1 = norm3(sub3(t.p1, cp)), w2 = norm3(sub3(t.p2, cp));
+        const float w[4] = {smax(0.01f, absdot3(cns, w1)), smax(0.01f, absdot3(cns, w1)), smax(0.01f, absdot3(cns, w0)),
+                            smax(0.01f, absdot3(cns, w2))};
+        const float y = sample_linear(u1, w[0] + w[1], w[2] + w[3]);
+        const float x = sample_linear(u0, lerpf(y, w[0], w[2]), lerpf(y, w[1], w[3]));
+        u0 = x;
+        u1 = y;
+        pdf = bilinear_pdf(u0, u1, w);
+    }
+    float b[3], tpdf;
+    sample_spherical_triangle(b, &tpdf, v, cp, u0, u1);
+    if (tpdf == 0.0f) return false;
+    pdf *= tpdf;
+    ss->pi = sample_point(t, b);
+    ss->n = light_normal(t);
+    ss->pdf = pdf;
+    return true;
+}
+// Triangle::pdf(ctx, wi) (shapes/triangle.cu:20-76).
+__device__ float tri_pdf(const Tri &t, const P3i &cpi, f3 cn, f3 cns, f3 wi) {
+    const f3 cp = p3i_mid(cpi);
+    const f3 v[3] = {t.p0, t.p1, t.p2};
+    const float sa = spherical_triangle_area(norm3(sub3(t.p0, cp)), norm3(sub3(t.p1, cp)), norm3(sub3(t.p2, cp)));
+    if (sa < kMinSolidAngle || sa > kMaxSolidAngle) {
+        const f3 o = ctx_offset_origin(cpi, cn, wi);
+        TriHit h;
+        if (!tri_hit(o, wi, __builtin_inff(), t.p0, t.p1, t.p2, &h)) return 0;
+        const HitGeom g = hit_geom(h, t.p0, t.p1, t.p2);
+        const f3 n = t.flip ? neg3(g.n) : g.n;
+        const f3 ip = {ival_mid(g.x), ival_mid(g.y), ival_mid(g.z)};
+        float pdf = (1 / tri_area(t)) / (absdot3(n, neg3(wi)) / sqlen3(sub3(cp, ip)));
+        if (isinf(pdf)) pdf = 0;
+        return pdf;
+    }
+    float pdf = 1 / sa;
+    if (!zero3(cns)) {
+        float u0, u1;
+        invert_spherical_triangle(v, cp, wi, &u0, &u1);
+        const f3 w0 = norm3(sub3(t.p0, cp)), w1 = norm3(sub3(t.p1, cp)), w2 = norm3(sub3(t.p2, cp));
+        const float w[4] = {smax(0.01f, absdot3(cns, w1)), smax(0.01f, absdot3(cns, w1)), smax(0.01f, absdot3(cns, w0)),
+                            smax(0.01f, absdot3(cns, w2))};
+        pdf *= bilinear_pdf(u0, u1, w);
+    }
+    return pdf;
+}
+
+// Ray::spawn_ray_to (base/ray.h:45-51): origin pf and unnormalised direction pt - pf.
+__device__ __forceinline__ void spawn_ray_to(const P3i &from, f3 n_from, const P3i &to, f3 n_to, f3 *o, f3 *d) {
+    const f3 pf = offset_ray_origin(from, n_from, sub3(p3i_mid(to), p3i_mid(from)));
+    const f3 pt = offset_ray_origin(to, n_to, sub3(pf, p3i_mid(to)));
+    *o = pf;
+    *d = sub3(pt, pf);
+}
+
+__device__ __forceinline__ void store_ray(float4 *r, f3 o, float tmax, f3 d) {
+    r[0] = make_float4(o.x, o.y, o.z, tmax);
+    r[1] = make_float4(d.x, d.y, d.z, 0.f);
+}
+
+// PixelSensor::to_sensor_rgb (films/pixel_sensor.h:33-40), imaging ratio 1.
+__device__ __forceinline__ float3 sensor_rgb(const SceneDev &S, const S4 &L, const S4 &lam, const S4 &pdf) {
+    const S4 s = s4safediv(L, pdf);
+    float c[3];
+    for (int k = 0; k < 3; ++k) {
+        S4 x;
+        for (int i = 0; i < kNS; ++i) x.v[i] = spec_eval(S, S.cie_off[k], S.cie_n, lam.v[i]);
+        c[k] = 1.0f * s4avg(s4mul(x, s));
+    }
+    return make_float3(c[0], c[1], c[2]);
+}
+
+// generate_new_path (wavefront_path.cu:265-308) for path `id` in slot `i`.
+__device__ void start_path(const SceneDev &S, const State &st, const Params &P, uint32_t i, uint64_t id) {
+    const uint64_t wh = uint64_t(S.width) * S.height;
+    const uint32_t pixel = uint32_t(id % wh), sample = uint32_t(id / wh);
+    Sampler smp{pcg_seq(hash_u32(pixel)), pixel, sample, 0, P.spd};
+    pcg_advance(smp.rng, uint64_t(sample) * 65536ull);
+    float u0, u1;
+    smp.get_2d(&u0, &u1);
+    float fx, fy, fw;
+    filter_sample(S, u0, u1, &fx, &fy, &fw);
+    const float px = (float(pixel % S.width) + fx) + 0.5f;
+    const float py = (float(pixel / S.width) + fy) + 0.5f;
+    S4 lam, pdf;
+    sample_wavelengths(smp.get_1d(), &lam, &pdf);
+    const f3 pc = xf_point(S.cam_from_raster, {px, py, 0.f});
+    const f3 dc = norm3(pc);
+    const f3 rd = xf_vector(S.render_from_camera, dc);
+    st.path_id[i] = uint32_t(id);
+    st.flags[i] = kFlagSpecular;
+    st.rng[i] = smp.rng.state;
+    st.dim[i] = smp.dim;
+    st.lam[i] = s4f(lam);
+    st.lam_pdf[i] = s4f(pdf);
+    st.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    st.beta[i] = make_float4(1.f, 1.f, 1.f, 1.f);
+    st.scal[i] = make_float4(fw, __builtin_nanf(""), 1.0f, 0.f);
+    store_ray(st.rays + 2 * size_t(i), ld3(S.cam_origin), __builtin_inff(), rd);
+}
+
+__device__ __forceinline__ void idle_slot(const State &st, uint32_t i) {
+    st.path_id[i] = kIdle;
+    st.rays[2 * size_t(i)] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kBlock) void init_pool_kernel(SceneDev S, State st, Params P) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= P.pool) return;
+    st.srays[4 * size_t(i)] = make_float4(0.f, 0.f, 0.f, 0.f);
+    st.srays[4 * size_t(i) + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (uint64_t(i) < P.total) start_path(S, st, P, i, i);
+    else idle_slot(st, i);
+}
+
+// One round for every slot: deferred NEE, control_logic, film record, new
+// path, material evaluation with sample_ld and sample_f.
+__global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Params P) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool in = i < P.pool;
+    const uint32_t pid = in ? st.path_id[i] : kIdle;
+    const bool live = pid != kIdle;
+    uint32_t fl = live ? st.flags[i] : 0u;
+    S4 L = live ? f4s(st.L[i]) : s4(0.f), beta = live ? f4s(st.beta[i]) : s4(0.f);
+    // 1. the previous round's light sample, if both shadow rays missed
+    if (live && (fl & kFlagNee)) {
+        if (st.sprim[2 * size_t(i)] == kNoHit && st.sprim[2 * size_t(i) + 1] == kNoHit) L = s4add(L, f4s(st.nee_L[i]));
+        fl &= ~kFlagNee;
+    }
+    const uint32_t prim = live ? st.prim[i] : kNoHit;
+    const uint32_t depth = fl & kDepthMask;
+    bool terminate = live && (prim == kNoHit || depth > P.max_depth || !s4pos(beta));
+    Sampler smp{};
+    S4 lam = s4(0.f), lpdf = s4(0.f);
+    float4 scal = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (live) {
+        smp.pixel = pid % (S.width * S.height);
+        smp.sample = pid / (S.width * S.height);
+        smp.spd = P.spd;
+        smp.rng.state = st.rng[i];
+        smp.rng.inc = sampler_inc(smp.pixel);
+        smp.dim = st.dim[i];
+        lam = f4s(st.lam[i]);
+        lpdf = f4s(st.lam_pdf[i]);
+        scal = st.scal[i];
+    }
+    // Russian roulette (wavefront_path.cu:121-138), only past 8 bounces.
+    if (live && !terminate && depth > 8) {
+        const float u = smp.get_1d();
+        const S4 rr = s4scale(beta, scal.z);
+        if (s4max(rr) < 1) {
+            const float q = clampf(1 - s4max(rr), 0, 0.95f);
+            if (u < q) {
+                beta = s4(0.0f);
+                terminate = true;
+            } else {
+                beta = s4div(beta, 1 - q);
+            }
+        }
+    }
+    // 2. finished paths: film record, then a new path in this slot
+    const uint64_t lanes = __ballot(terminate);
+    uint32_t base = 0;
+    if (lanes) {
+        const int leader = __ffsll((unsigned long long)lanes) - 1;
+        if (lane_id() == leader) base = atomicAdd(&st.counters[kCntNextId], uint32_t(__popcll(lanes)));
+        base = __shfl(base, leader, kWave);
+    }
+    if (terminate) {
+        const float3 rgb = sensor_rgb(S, L, lam, lpdf);
+        st.rec[pid] = make_float4(rgb.x, rgb.y, rgb.z, scal.x);
+        st.rec_round[pid] = P.round;
+        const uint64_t nid = uint64_t(base) + __popcll(lanes & lanemask_lt());
+        if (nid < P.total) start_path(S, st, P, i, nid);
+        else idle_slot(st, i);
+        st.srays[4 * size_t(i)].w = 0.f;
+        st.srays[4 * size_t(i) + 2].w = 0.f;
+        return;
+    }
+    if (!live) return;
+    const float4 r1 = st.rays[2 * size_t(i) + 1];
+    const f3 rd = {r1.x, r1.y, r1.z};
+    const Tri tri = load_tri(S, prim);
+    const uint2 info = S.tri_info[prim];
+    // surface interaction (shapes/triangle.cu:330-470, default uv, no shading normals)
+    const f3 dp02 = sub3(tri.p0, tri.p2), dp12 = sub3(tri.p1, tri.p2);
+    f3 dpdu = {dop(-1.f, dp02.x, -1.f, dp12.x), dop(-1.f, dp02.y, -1.f, dp12.y), dop(-1.f, dp02.z, -1.f, dp12.z)};
+    f3 dpdv = {dop(-1.f, dp12.x, 0.f, dp02.x), dop(-1.f, dp12.y, 0.f, dp02.y), dop(-1.f, dp12.z, 0.f, dp02.z)};
+    dpdu = mul3(1.0f, dpdu);
+    dpdv = mul3(1.0f, dpdv);
+    if (sqlen3(cross3(dpdu, dpdv)) == 0) basis(norm3(cross3(sub3(tri.p2, tri.p0), sub3(tri.p1, tri.p0))), &dpdu, &dpdv);
+    const TriHit th{st.bary[3 * size_t(i)], st.bary[3 * size_t(i) + 1], st.bary[3 * size_t(i) + 2], st.thit[i]};
+    const HitGeom g = hit_geom(th, tri.p0, tri.p1, tri.p2);
+    const P3i ipi{g.x, g.y, g.z};
+    const f3 n = tri.flip ? neg3(g.n) : g.n;
+    const f3 ns = n;
+    const f3 wo_n = norm3(neg3(rd));
+    // emission with MIS (wavefront_path.cu:179-196)
+    const uint32_t light = info.y & 0x7FFFFFFFu;
+    if (light) {
+        const DLight Ld = S.lights[light - 1];
+        const S4 Le = light_L(S, Ld, n, neg3(rd), lam);
+        if (s4pos(Le)) {
+            if (depth == 0 || (fl & kFlagSpecular)) {
+                L = s4add(L, s4mul(beta, Le));
+            } else {
+                const float4 q0 = st.prev[3 * size_t(i)], q1 = st.prev[3 * size_t(i) + 1], q2 = st.prev[3 * size_t(i) + 2];
+                const P3i ppi{{q0.x, q0.y}, {q0.z, q0.w}, {q1.x, q1.y}};
+                const f3 pn = {q1.z, q1.w, q2.x}, pns = {q2.y, q2.z, q2.w};
+                const float pdf_light = S.light_pmf[light - 1] * tri_pdf(tri, ppi, pn, pns, rd);
+                const float wl = power_heuristic(scal.y, pdf_light);
+                L = s4add(L, s4mul(s4scale(beta, wl), Le));
+            }
+        }
+    }
+    fl = (fl & ~kDepthMask) | ((depth + 1) & kDepthMask);
+    // 3. material: BSDF (interaction.cu:77-89, bsdf.cu), sample_ld, sample_f
+    const DMat m = S.mats[info.x];
+    const f3 fx = norm3(dpdu);
+    const f3 fy = cross3(ns, fx);
+    S4 r = s4(0.f), eta = s4(0.f), kk = s4(0.f);
+    uint32_t bflags;  // BxDFFlags: 1 reflection, 4 diffuse, 16 specular
+    if (m.type == kDiffuse) {
+        for (int k = 0; k < kNS; ++k) r.v[k] = clampf(sigmoid_eval(m.c0, m.c1, m.c2, lam.v[k]), 0.0f, 1.0f);
+        bflags = s4pos(r) ? (1u | 4u) : 0u;
+    } else {
+        for (int k = 0; k < kNS; ++k) {
+            eta.v[k] = spec_eval(S, m.eta_off, m.eta_n, lam.v[k]);
+            kk.v[k] = spec_eval(S, m.k_off, m.k_n, lam.v[k]);
+        }
+        bflags = 1u | 16u;
+    }
+    bool nee = false;
+    if (bflags & (4u | 8u)) {
+        // sample_ld (wavefront_path.cu:574-624)
+        P3i cpi = ipi;
+        const bool refl = (bflags & 1u) != 0, trans = (bflags & 2u) != 0;
+        if (refl && !trans) cpi = p3i_exact(offset_ray_origin(ipi, n, wo_n));
+        else if (trans && !refl) cpi = p3i_exact(offset_ray_origin(ipi, n, neg3(wo_n)));
+        const float u = smp.get_1d();
+        float ul0, ul1;
+        smp.get_2d(&ul0, &ul1);
+        if (S.num_lights > 0) {
+            const uint32_t li = search_cdf(u, S.light_cdf, S.num_lights);
+            const float pmf = S.light_pmf[li];
+            const DLight Lg = S.lights[li];
+            const Tri lt = load_tri(S, Lg.tri);
+            ShapeSample ss;
+            const bool ok = tri_sample(lt, cpi, ns, ul0, ul1, &ss);
+            const f3 cpt = p3i_mid(cpi);
+            const f3 sp = p3i_mid(ss.pi);
+            if (ok && !(ss.pdf == 0 || sqlen3(sub3(sp, cpt)) == 0)) {
+                const f3 wi = norm3(sub3(sp, cpt));
+                const S4 Le = light_L(S, Lg, ss.n, neg3(wi), lam);
+                if (s4pos(Le) && ss.pdf != 0) {
+                    // BSDF::f for the diffuse lobe (only non-specular BSDFs reach here)
+                    const f3 wil = {dot3(wi, fx), dot3(wi, fy), dot3(wi, ns)};
+                    const f3 wol = {dot3(wo_n, fx), dot3(wo_n, fy), dot3(wo_n, ns)};
+                    S4 f = s4(0.f);
+                    float pdf_b = 0.f;
+                    if (wol.z != 0 && wol.z * wil.z > 0) {
+                        f = s4scale(r, kInvPi);
+                        pdf_b = fabsf(wil.z) / kPi;
+                    }
+                    f = s4scale(f, absdot3(wi, ns));
+                    if (s4pos(f)) {
+                        const float pdf_light = pmf * ss.pdf;
+                        const float wl = power_heuristic(pdf_light, pdf_b);
+                        const S4 Ld = s4div(s4mul(s4scale(Le, wl), f), pdf_light);
+                        st.nee_L[i] = s4f(s4mul(beta, Ld));
+                        nee = true;
+                        f3 o, d;
+                        spawn_ray_to(ipi, n, ss.pi, ss.n, &o, &d);
+                        store_ray(st.srays + 4 * size_t(i), o, 0.6f, d);
+                        spawn_ray_to(ss.pi, ss.n, ipi, n, &o, &d);
+                        store_ray(st.srays + 4 * size_t(i) + 2, o, 0.6f, d);
+                    }
+                }
+            }
+        }
+    }
+    if (nee) fl |= kFlagNee;
+    else {
+        st.srays[4 * size_t(i)].w = 0.f;
+        st.srays[4 * size_t(i) + 2].w = 0.f;
+    }
+    // BSDF::sample_f (bsdf.cu:57-80)
+    const f3 wo = neg3(rd);
+    (void)smp.get_1d();
+    float u0, u1;
+    smp.get_2d(&u0, &u1);
+    const f3 wol = {dot3(wo, fx), dot3(wo, fy), dot3(wo, ns)};
+    bool ok = !(wol.z == 0 || bflags == 0);
+    S4 f = s4(0.f);
+    f3 wil = {0.f, 0.f, 0.f};
+    float pdf = 0.f;
+    bool specular = false;
+    if (ok) {
+        if (m.type == kDiffuse) {
+            wil = cosine_hemisphere(u0, u1);
+            if (wol.z < 0) wil.z *= -1;
+            pdf = fabsf(wil.z) / kPi;
+            f = s4scale(r, kInvPi);
+        } else {
+            wil = {-wol.x, -wol.y, wol.z};
+            const float c = fabsf(wil.z);
+            S4 F;
+            for (int k = 0; k < kNS; ++k) F.v[k] = fr_complex(c, eta.v[k], kk.v[k]);
+            f = s4div(F, c);
+            pdf = 1;
+            specular = true;
+        }
+        ok = s4pos(f) && pdf != 0 && wil.z != 0;
+    }
+    if (!ok) {
+        beta = s4(0.0f);  // the stale ray is traced again and the path ends next round
+    } else {
+        const f3 wi = add3(add3(mul3(wil.x, fx), mul3(wil.y, fy)), mul3(wil.z, ns));
+        beta = s4mul(beta, s4div(s4scale(f, absdot3(wi, ns)), pdf));
+        scal.y = pdf;
+        fl = specular ? (fl | kFlagSpecular) : (fl & ~kFlagSpecular);
+        if (!specular) fl |= kFlagAnyNonSpec;
+        st.prev[3 * size_t(i)] = make_float4(ipi.x.lo, ipi.x.hi, ipi.y.lo, ipi.y.hi);
+        st.prev[3 * size_t(i) + 1] = make_float4(ipi.z.lo, ipi.z.hi, n.x, n.y);
+        st.prev[3 * size_t(i) + 2] = make_float4(n.z, ns.x, ns.y, ns.z);
+        store_ray(st.rays + 2 * size_t(i), offset_ray_origin(ipi, n, wi), __builtin_inff(), wi);
+    }
+    st.flags[i] = fl;
+    st.L[i] = s4f(L);
+    st.beta[i] = s4f(beta);
+    st.scal[i] = scal;
+    st.rng[i] = smp.rng.state;
+    st.dim[i] = smp.dim;
+}
+
+__global__ __launch_bounds__(kBlock) void count_live_kernel(const uint32_t *path_id, uint32_t pool, uint32_t *count,
+                                                            const float4 *srays, uint32_t *shadow) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < pool && path_id[i] != kIdle;
+    const bool sh = i < pool && srays[4 * size_t(i)].w > 0.f;
+    const uint32_t a = wave_sum(live ? 1u : 0u), b = wave_sum(sh ? 2u : 0u);
+    if (lane_id() == 0) {
+        if (a) atomicAdd(count, a);
+        if (b) atomicAdd(shadow, b);
+    }
+}
+
+// write_frame_buffer + RGBFilm::add_sample/get_pixel_rgb (rgb_film.cu:85-144):
+// one wave per pixel ranks its samples by (round, sample), sums them in that
+// order, divides by the weight sum and converts sensor RGB to output RGB with
+// the compensated 3x3 product.
+__device__ __forceinline__ float ip3(const float *m, const float *v) {
+    float vv = m[2] * v[2];
+    float err = fmaf(m[2], v[2], -vv);
+    for (int k = 1; k >= 0; --k) {
+        const float ab = m[k] * v[k];
+        const float ab_err = fmaf(m[k], v[k], -ab);
+        const float s = ab + vv;
+        const float delta = s - ab;
+        const float s_err = (ab - (s - delta)) + (vv - delta);
+        err = ab_err + (err + s_err);
+        vv = s;
+    }
+    return vv + err;
+}
+
+constexpr uint32_t kFilmWaves = 4;
+constexpr uint32_t kMaxSpp = 1024;
+
+__global__ __launch_bounds__(kFilmWaves * kWave) void film_kernel(SceneDev S, const float4 *__restrict__ rec,
+                                                                  const uint32_t *__restrict__ rec_round, uint32_t spp,
+                                                                  float *film_rgb, float *film_aux) {
+    __shared__ uint32_t s_key[kFilmWaves][kMaxSpp];
+    __shared__ uint16_t s_order[kFilmWaves][kMaxSpp];
+    const uint32_t wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const uint32_t wh = S.width * S.height;
+    const uint32_t px = blockIdx.x * kFilmWaves + wv;
+    if (px >= wh) return;
+    for (uint32_t s = lane; s < spp; s += kWave) s_key[wv][s] = rec_round[size_t(s) * wh + px];
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t s = lane; s < spp; s += kWave) {
+        const uint32_t k = s_key[wv][s];
+        uint32_t rank = 0;
+        for (uint32_t t = 0; t < spp; ++t) {
+            const uint32_t kt = s_key[wv][t];
+            rank += (kt < k || (kt == k && t < s)) ? 1u : 0u;
+        }
+        s_order[wv][rank] = uint16_t(s);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane != 0) return;
+    float sum[3] = {0.f, 0.f, 0.f}, wsum = 0.f;
+    for (uint32_t j = 0; j < spp; ++j) {
+        const float4 r = rec[size_t(s_order[wv][j]) * wh + px];
+        if (r.w == 0) continue;
+        sum[0] += r.w * r.x;
+        sum[1] += r.w * r.y;
+        sum[2] += r.w * r.z;
+        wsum += r.w;
+    }
+    float rgb[3] = {sum[0], sum[1], sum[2]};
+    if (wsum != 0)
+        for (int c = 0; c < 3; ++c) rgb[c] /= wsum;
+    for (int c = 0; c < 3; ++c) film_rgb[3 * size_t(px) + c] = ip3(S.out_from_sensor + 3 * c, rgb);
+    if (film_aux) {
+        for (int c = 0; c < 3; ++c) film_aux[4 * size_t(px) + c] = sum[c];
+        film_aux[4 * size_t(px) + 3] = wsum;
+    }
+}
+
+}  // namespace
+
+hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t max_depth, uint32_t pool,
+                  PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
+                  hipStream_t stream) {
+    const uint64_t total = uint64_t(S.width) * S.height * spp;
+    const uint32_t spd = uint32_t(sqrt(double(spp)));
+    if (spd * spd != spp || spp == 0 || spp > kMaxSpp) return hipErrorInvalidValue;
+    if (total >= 0xFFFFFFFFull || pool == 0) return hipErrorInvalidValue;
+    hipError_t e;
+#define PT_CK(x)                          \
+    do {                                  \
+        if ((e = (x)) != hipSuccess) return e; \
+    } while (0)
+    if (pp.cap < pool) {
+        PT_CK(pp.path_id.reserve(size_t(pool) * 4));
+        PT_CK(pp.flags.reserve(size_t(pool) * 4));
+        PT_CK(pp.dim.reserve(size_t(pool) * 4));
+        PT_CK(pp.rng.reserve(size_t(pool) * 8));
+        for (DevBuf *b : {&pp.lam, &pp.lam_pdf, &pp.L, &pp.beta, &pp.nee_L, &pp.scal}) PT_CK(b->reserve(size_t(pool) * 16));
+        PT_CK(pp.prev.reserve(size_t(pool) * 48));
+        PT_CK(pp.rays.reserve(size_t(pool) * 32));
+        PT_CK(pp.prim.reserve(size_t(pool) * 4));
+        PT_CK(pp.thit.reserve(size_t(pool) * 4));
+        PT_CK(pp.bary.reserve(size_t(pool) * 12));
+        PT_CK(pp.srays.reserve(size_t(pool) * 64));
+        PT_CK(pp.sprim.reserve(size_t(pool) * 8));
+        PT_CK(pp.queue.reserve(size_t(pool) * 4));
+        PT_CK(pp.squeue.reserve(size_t(pool) * 8));
+        pp.cap = pool;
+    }
+    if (pp.rec_cap < total) {
+        PT_CK(pp.rec.reserve(total * 16));
+        PT_CK(pp.rec_round.reserve(total * 4));
+        pp.rec_cap = total;
+    }
+    PT_CK(pp.counters.reserve(64));
+    WideTrace W = W0;
+    const uint32_t spill_need = W.spill_per_ray;
+    if (spill_need) {
+        PT_CK(pp.spill.reserve(size_t(pool) * 2 * spill_need * sizeof(uint2)));
+        W.spill = pp.spill.as<uint2>();
+    }
+    State st;
+    st.path_id = pp.path_id.as<uint32_t>();
+    st.flags = pp.flags.as<uint32_t>();
+    st.dim = pp.dim.as<uint32_t>();
+    st.rng = pp.rng.as<uint64_t>();
+    st.lam = pp.lam.as<float4>();
+    st.lam_pdf = pp.lam_pdf.as<float4>();
+    st.L = pp.L.as<float4>();
+    st.beta = pp.beta.as<float4>();
+    st.nee_L = pp.nee_L.as<float4>();
+    st.scal = pp.scal.as<float4>();
+    st.prev = pp.prev.as<float4>();
+    st.rays = pp.rays.as<float4>();
+    st.prim = pp.prim.as<uint32_t>();
+    st.thit = pp.thit.as<float>();
+    st.bary = pp.bary.as<float>();
+    st.srays = pp.srays.as<float4>();
+    st.sprim = pp.sprim.as<uint32_t>();
+    st.rec = pp.rec.as<float4>();
+    st.rec_round = pp.rec_round.as<uint32_t>();
+    st.counters = pp.counters.as<uint32_t>();
+    st.err = err;
+    Params P{total, spd, spp, max_depth, 0u, pool};
+    const uint32_t grid = (pool + kBlock - 1) / kBlock;
+    hipEvent_t ev[6];
+    for (auto &x : ev) PT_CK(hipEventCreate(&x));
+    struct EvGuard {
+        hipEvent_t *e;
+        ~EvGuard() {
+            for (int k = 0; k < 6; ++k) hipEventDestroy(e[k]);
+        }
+    } guard{ev};
+    PT_CK(hipMemsetAsync(st.counters, 0, 64, stream));
+    uint32_t first = uint32_t(std::min<uint64_t>(total, pool));
+    PT_CK(hipMemcpyAsync(st.counters + kCntNextId, &first, 4, hipMemcpyHostToDevice, stream));
+    PT_CK(hipEventRecord(ev[0], stream));
+    init_pool_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);
+    PT_CK(hipGetLastError());
+    RenderStats rs;
+    uint32_t host_cnt[16];
+    uint32_t live = first;
+    float t_trace = 0, t_shadow = 0, t_shade = 0;
+    while (live > 0) {
+        PT_CK(hipMemsetAsync(st.counters + kCntActive, 0, 16, stream));
+        PT_CK(hipEventRecord(ev[1], stream));
+        PT_CK(trace_wide(W, reinterpret_cast<const float *>(st.rays), pool, 0, pp.prim.as<uint32_t>(),
+                         pp.thit.as<float>(), pp.bary.as<float>(), err, nullptr, pp.queue.as<uint32_t>(),
+                         st.counters + kCntQueue, stream));
+        PT_CK(hipEventRecord(ev[2], stream));
+        if (spill_need) W.spill = pp.spill.as<uint2>();
+        PT_CK(trace_wide(W, reinterpret_cast<const float *>(st.srays), 2 * pool, 1, pp.sprim.as<uint32_t>(), nullptr,
+                         nullptr, err, nullptr, pp.squeue.as<uint32_t>(), st.counters + kCntSQueue, stream));
+        PT_CK(hipEventRecord(ev[3], stream));
+        shade_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);
+        PT_CK(hipGetLastError());
+        count_live_kernel<<<grid, kBlock, 0, stream>>>(st.path_id, pool, st.counters + kCntActive, st.srays,
+                                                       st.counters + kCntShadow);
+        PT_CK(hipGetLastError());
+        PT_CK(hipEventRecord(ev[4], stream));
+        PT_CK(hipMemcpyAsync(host_cnt, st.counters, 32, hipMemcpyDeviceToHost, stream));
+        PT_CK(hipStreamSynchronize(stream));
+        float a, b, c;
+        hipEventElapsedTime(&a, ev[1], ev[2]);
+        hipEventElapsedTime(&b, ev[2], ev[3]);
+        hipEventElapsedTime(&c, ev[3], ev[4]);
+        t_trace += a;
+        t_shadow += b;
+        t_shade += c;
+        rs.rays += host_cnt[kCntQueue];
+        rs.shadow_rays += host_cnt[kCntSQueue];
+        live = host_cnt[kCntActive];
+        P.round++;
+        if (P.round > 100000) return hipErrorUnknown;
+    }
+    PT_CK(hipEventRecord(ev[5], stream));
+    const uint32_t wh = S.width * S.height;
+    film_kernel<<<(wh + kFilmWaves - 1) / kFilmWaves, kFilmWaves * kWave, 0, stream>>>(
+        S, st.rec, st.rec_round, spp, film_rgb, film_aux);
+    PT_CK(hipGetLastError());
+    PT_CK(hipEventRecord(ev[4], stream));
+    PT_CK(hipStreamSynchronize(stream));
+    float tf = 0, tt = 0;
+    hipEventElapsedTime(&tf, ev[5], ev[4]);
+    hipEventElapsedTime(&tt, ev[0], ev[4]);
+    rs.ms_total = tt;
+    rs.ms_trace = t_trace;
+    rs.ms_shadow = t_shadow;
+    rs.ms_shade = t_shade;
+    rs.ms_film = tf;
+    rs.rounds = P.round;
+    rs.paths = total;
+    if (stats) *stats = rs;
+    return hipSuccess;
+#undef PT_CK
+}
+
+}  // namespace pt
+}  // namespace pbrtm

@@ -32,7 +32,8 @@ EXPORTS = (
     "pbrtm_create", "pbrtm_destroy", "pbrtm_last_error", "pbrtm_device_count", "pbrtm_stream",
     "pbrtm_morton_sort", "pbrtm_bvh_build", "pbrtm_bvh_free", "pbrtm_bvh_bounds",
     "pbrtm_bvh_export_canonical", "pbrtm_bvh_intersect", "pbrtm_ao_bounce", "pbrtm_gen_soup",
-    "pbrtm_gen_camera_rays",
+    "pbrtm_gen_camera_rays", "pbrtm_scene_create", "pbrtm_scene_free", "pbrtm_render", "pbrtm_rgb_to_sigmoid",
+    "pbrtm_srgb_matrices",
 )
 
 
@@ -46,6 +47,39 @@ class BuildStats(ctypes.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("type", c_uint32), ("rgb", c_float * 3), ("metal", c_uint32), ("uroughness", c_float),
+                ("vroughness", c_float), ("remap_roughness", c_uint32)]
+
+
+class AreaLight(ctypes.Structure):
+    _fields_ = [("rgb", c_float * 3), ("scale", c_float), ("two_sided", c_uint32)]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("eye", c_float * 3), ("look", c_float * 3), ("up", c_float * 3), ("fov", c_float),
+                ("width", c_uint32), ("height", c_uint32), ("num_tris", c_uint64), ("tri", c_void_p),
+                ("tri_material", c_void_p), ("tri_light", c_void_p), ("tri_reverse", c_void_p),
+                ("num_materials", c_uint32), ("materials", POINTER(Material)), ("num_lights", c_uint32),
+                ("lights", POINTER(AreaLight))]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("spp", c_uint32), ("max_depth", c_uint32), ("pool_size", c_uint32), ("flags", c_uint32)]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [("ms_total", c_double), ("ms_trace", c_double), ("ms_shadow", c_double), ("ms_shade", c_double),
+                ("ms_film", c_double), ("rounds", c_uint64), ("rays", c_uint64), ("shadow_rays", c_uint64),
+                ("paths", c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+METALS = {"Ag": 0, "Al": 1, "Au": 2, "Cu": 3}
 
 
 class PbrtmError(RuntimeError):
@@ -81,6 +115,11 @@ def load(path: str = LIB_PATH):
         "pbrtm_ao_bounce": (c_int, [p, p, p, p, c_uint64, c_uint32, p]),
         "pbrtm_gen_soup": (c_int, [c_uint64, c_uint64, c_float, c_float, c_int, p]),
         "pbrtm_gen_camera_rays": (c_int, [c_uint32, c_uint32, p, p, p, c_float, p]),
+        "pbrtm_scene_create": (c_int, [p, POINTER(SceneDesc), POINTER(p)]),
+        "pbrtm_scene_free": (None, [p]),
+        "pbrtm_render": (c_int, [p, p, POINTER(RenderParams), p, p, POINTER(RenderStats)]),
+        "pbrtm_rgb_to_sigmoid": (c_int, [p, p]),
+        "pbrtm_srgb_matrices": (c_int, [p, p, p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -175,6 +214,9 @@ class Context:
                                               _ptr(codes), _ptr(order)))
         return codes, order
 
+    def scene(self, sc) -> "Scene":
+        return Scene(self, sc)
+
     def build(self, verts, idx=None, device_ptrs=False) -> "Bvh":
         stats = BuildStats()
         h = c_void_p()
@@ -190,6 +232,99 @@ class Context:
         bvh = Bvh(self, h, nt, stats, keep=verts if device_ptrs else None)
         self._handles.add(bvh)
         return bvh
+
+
+def rgb_to_sigmoid(rgb) -> np.ndarray:
+    """Sigmoid-polynomial coefficients (c0, c1, c2) of an RGB (host only, no GPU needed)."""
+    lib = load()
+    a = np.ascontiguousarray(rgb, np.float32)
+    out = np.zeros(3, np.float32)
+    rc = lib.pbrtm_rgb_to_sigmoid(_ptr(a), _ptr(out))
+    if rc != OK:
+        raise PbrtmError(rc, "rgb_to_sigmoid")
+    return out
+
+
+def srgb_matrices():
+    """(xyz_from_rgb, rgb_from_xyz, white xy) of the sRGB colour space (host only)."""
+    lib = load()
+    a, b, w = np.zeros(9, np.float32), np.zeros(9, np.float32), np.zeros(2, np.float32)
+    lib.pbrtm_srgb_matrices(_ptr(a), _ptr(b), _ptr(w))
+    return a.reshape(3, 3), b.reshape(3, 3), w
+
+
+def scene_desc(sc):
+    """Flatten a scene dict {eye, look, up, fov, width, height, tri [n][9], tri_material, tri_light,
+    [tri_reverse], materials [{type: diffuse|conductor, ...}], lights [{rgb, scale, two_sided}]} into
+    pbrtm_scene_desc; returns (desc, keepalive)."""
+    tri = np.ascontiguousarray(sc["tri"], dtype=np.float32).reshape(-1, 9)
+    mat = np.ascontiguousarray(sc["tri_material"], dtype=np.uint32)
+    lig = np.ascontiguousarray(sc["tri_light"], dtype=np.int32)
+    rev = np.ascontiguousarray(sc.get("tri_reverse", np.zeros(len(tri), np.uint8)), dtype=np.uint8)
+    mats = (Material * max(1, len(sc["materials"])))()
+    for i, m in enumerate(sc["materials"]):
+        if m["type"] == "diffuse":
+            mats[i].type = 0
+            mats[i].rgb[:] = list(m["rgb"])
+        elif m["type"] == "conductor":
+            mats[i].type = 1
+            mats[i].metal = METALS[m["metal"]]
+            mats[i].uroughness = m.get("uroughness", m.get("roughness", 0.0))
+            mats[i].vroughness = m.get("vroughness", m.get("roughness", 0.0))
+            mats[i].remap_roughness = int(m.get("remap", True))
+        else:
+            raise ValueError(f"unknown material type {m['type']}")
+    lights = (AreaLight * max(1, len(sc["lights"])))()
+    for i, l in enumerate(sc["lights"]):
+        lights[i].rgb[:] = list(l["rgb"])
+        lights[i].scale = l.get("scale", 1.0)
+        lights[i].two_sided = int(l.get("two_sided", False))
+    d = SceneDesc()
+    d.eye[:], d.look[:], d.up[:] = sc["eye"], sc["look"], sc["up"]
+    d.fov = sc["fov"]
+    d.width, d.height = sc["width"], sc["height"]
+    d.num_tris = len(tri)
+    d.tri, d.tri_material, d.tri_light, d.tri_reverse = _ptr(tri), _ptr(mat), _ptr(lig), _ptr(rev)
+    d.num_materials = len(sc["materials"])
+    d.materials = mats
+    d.num_lights = len(sc["lights"])
+    d.lights = lights
+    return d, (tri, mat, lig, rev, mats, lights)
+
+
+class Scene:
+    """A scene on a context (pbrtm_scene_create); render() runs the wavefront path integrator."""
+
+    def __init__(self, ctx: "Context", sc):
+        self.ctx = ctx
+        self.width, self.height = sc["width"], sc["height"]
+        d, keep = scene_desc(sc)
+        h = c_void_p()
+        ctx.check(ctx.lib.pbrtm_scene_create(ctx.h, ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        ctx._handles.add(self)
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.pbrtm_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def render(self, spp=4, max_depth=5, pool_size=0, film_rgb=None, film_aux=None, device_ptrs=False):
+        """Returns (film_rgb [h][w][3], film_aux [h][w][4], stats dict)."""
+        p = RenderParams(spp, max_depth, pool_size, DEVICE_PTRS if device_ptrs else 0)
+        st = RenderStats()
+        if not device_ptrs:
+            film_rgb = np.zeros((self.height, self.width, 3), np.float32)
+            film_aux = np.zeros((self.height, self.width, 4), np.float32)
+        self.ctx.check(self.ctx.lib.pbrtm_render(self.ctx.h, self.h, ctypes.byref(p), _ptr(film_rgb), _ptr(film_aux),
+                                                 ctypes.byref(st)))
+        return film_rgb, film_aux, st.as_dict()
 
 
 class Bvh:
