@@ -172,6 +172,12 @@ def _pt_sigs(lib):
         "orc_filter_sample": (c_int, [p, p]),
         "orc_spectrum_eval": (c_float, [c_int, c_float]),
         "orc_trace_ray": (c_int, [p, p, p, c_float, c_int, p, p, p]),
+        "orc_sample_visible": (c_int, [c_float, p, p]),
+        "orc_sampler_sequence": (c_int, [c_uint32, c_uint32, c_uint32, p, c_int, p]),
+        "orc_camera_ray": (c_int, [POINTER(OrcScene), c_float, c_float, p]),
+        "orc_spherical_triangle": (c_int, [p, p, p, p, POINTER(c_float)]),
+        "orc_invert_spherical_triangle": (c_int, [p, p, p, p]),
+        "orc_fr_complex": (c_float, [c_float, c_float, c_float]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -271,3 +277,55 @@ def spectrum_eval(which, lam):
         _pt_sigs(lib)
         lib._pt_ready = True
     return float(lib.orc_spectrum_eval(int(which), float(lam)))
+
+
+def _pt():
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        _pt_sigs(lib)
+        lib._pt_ready = True
+    return lib
+
+
+def sample_visible(u):
+    lam, pdf = np.zeros(4, np.float32), np.zeros(4, np.float32)
+    _pt().orc_sample_visible(float(u), _p(lam), _p(pdf))
+    return lam, pdf
+
+
+def sampler_sequence(spd, pixel, sample, kinds):
+    k = np.asarray(kinds, np.int32)
+    out = np.zeros(2 * len(k), np.float32)
+    n = _pt().orc_sampler_sequence(spd, pixel, sample, _p(k), len(k), _p(out))
+    return out[:n]
+
+
+def camera_ray(sc, px, py):
+    s, keep = _scene_struct(sc, 1)
+    out = np.zeros(6, np.float32)
+    if _pt().orc_camera_ray(ctypes.byref(s), float(px), float(py), _p(out)) != 0:
+        raise RuntimeError("camera")
+    return out[:3], out[3:]
+
+
+def spherical_triangle(v, p, u):
+    v = np.ascontiguousarray(v, np.float32).reshape(9)
+    p = np.ascontiguousarray(p, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    b = np.zeros(3, np.float32)
+    pdf = c_float(0)
+    _pt().orc_spherical_triangle(_p(v), _p(p), _p(u), _p(b), ctypes.byref(pdf))
+    return b, pdf.value
+
+
+def invert_spherical_triangle(v, p, w):
+    v = np.ascontiguousarray(v, np.float32).reshape(9)
+    p = np.ascontiguousarray(p, np.float32)
+    w = np.ascontiguousarray(w, np.float32)
+    u = np.zeros(2, np.float32)
+    _pt().orc_invert_spherical_triangle(_p(v), _p(p), _p(w), _p(u))
+    return u
+
+
+def fr_complex(cos_i, eta, k):
+    return float(_pt().orc_fr_complex(float(cos_i), float(eta), float(k)))

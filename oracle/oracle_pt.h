@@ -59,6 +59,15 @@ int orc_rgb_to_sigmoid(const float rgb[3], float out3[3]);
 int orc_srgb_matrices(float xyz_from_rgb9[9], float rgb_from_xyz9[9], float white_xy2[2]);
 int orc_filter_sample(const float u[2], float out3[3]);
 float orc_spectrum_eval(int which, float lambda); /* 0-2 CIE XYZ, 3 D65 (normalised), 4+ metals eta/k */
+int orc_sample_visible(float u, float lam4[4], float pdf4[4]);
+/* A stratified sampler started at (pixel, sample, dimension 0); kinds[i] = 1
+ * draws get_1d, 2 draws get_2d; returns the number of floats written. */
+int orc_sampler_sequence(uint32_t spd, uint32_t pixel, uint32_t sample, const int *kinds, int n, float *out);
+/* The camera ray through raster point (px, py): world-space origin, render-space direction. */
+int orc_camera_ray(const orc_scene *scene, float px, float py, float out6[6]);
+int orc_spherical_triangle(const float v9[9], const float p[3], const float u[2], float b3[3], float *pdf);
+int orc_invert_spherical_triangle(const float v9[9], const float p[3], const float w3[3], float u2[2]);
+float orc_fr_complex(float cos_i, float eta, float k);
 
 #ifdef __cplusplus
 }

@@ -1338,4 +1338,54 @@ float orc_spectrum_eval(int which, float lambda) {
     return NAN;
 }
 
+int orc_sample_visible(float u, float lam4[4], float pdf4[4]) {
+    orc::Lambda w = orc::sample_visible(u);
+    for (int i = 0; i < 4; ++i) lam4[i] = w.l[i], pdf4[i] = w.pdf[i];
+    return 0;
+}
+
+int orc_sampler_sequence(uint32_t spd, uint32_t pixel, uint32_t sample, const int *kinds, int n, float *out) {
+    orc::Sampler s;
+    s.spd = spd;
+    s.start(pixel, sample, 0);
+    int k = 0;
+    for (int i = 0; i < n; ++i) {
+        if (kinds[i] == 1) {
+            out[k++] = s.get_1d();
+        } else {
+            s.get_2d(&out[k], &out[k + 1]);
+            k += 2;
+        }
+    }
+    return k;
+}
+
+int orc_camera_ray(const orc_scene *sc, float px, float py, float out6[6]) {
+    try {
+        orc::Camera c = orc::make_camera(*sc);
+        orc::V3 pc = orc::xf_point(c.camera_from_raster.m, {px, py, 0});
+        orc::V3 o = orc::xf_point(c.render_from_camera.m, {0, 0, 0});
+        orc::V3 d = orc::xf_vector(c.render_from_camera.m, orc::normalize(pc));
+        orc::V3 ow = orc::xf_point(orc::xf_inverse(c.render_from_world).m, o);
+        out6[0] = ow.x, out6[1] = ow.y, out6[2] = ow.z, out6[3] = d.x, out6[4] = d.y, out6[5] = d.z;
+    } catch (const std::exception &) {
+        return -1;
+    }
+    return 0;
+}
+
+int orc_spherical_triangle(const float v9[9], const float p[3], const float u[2], float b3[3], float *pdf) {
+    orc::V3 v[3] = {{v9[0], v9[1], v9[2]}, {v9[3], v9[4], v9[5]}, {v9[6], v9[7], v9[8]}};
+    orc::sample_spherical_triangle(b3, pdf, v, {p[0], p[1], p[2]}, u[0], u[1]);
+    return 0;
+}
+
+int orc_invert_spherical_triangle(const float v9[9], const float p[3], const float w3[3], float u2[2]) {
+    orc::V3 v[3] = {{v9[0], v9[1], v9[2]}, {v9[3], v9[4], v9[5]}, {v9[6], v9[7], v9[8]}};
+    orc::invert_spherical_triangle(v, {p[0], p[1], p[2]}, {w3[0], w3[1], w3[2]}, &u2[0], &u2[1]);
+    return 0;
+}
+
+float orc_fr_complex(float cos_i, float eta, float k) { return orc::fr_complex(cos_i, orc::Cx{eta, k}); }
+
 }  // extern "C"

@@ -172,8 +172,9 @@ __global__ __launch_bounds__(256) void ao_bounce_kernel(const float *__restrict_
                 rr = oy;
                 theta = pi / 2.0f - pi_over_4 * (ox / oy);
             }
-            dx = rr * cosf(theta);
-            dy = rr * sinf(theta);
+            // cos and sin in double, rounded once to float (the oracle does the same)
+            dx = rr * float(cos(double(theta)));
+            dy = rr * float(sin(double(theta)));
         }
         const float zz = float(1.0 - double(dx * dx) - double(dy * dy));
         const float z = sqrtf(zz > 0.0f ? zz : 0.0f);

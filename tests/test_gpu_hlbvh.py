@@ -2,10 +2,12 @@
 
 Bit-exact: Morton codes and canonical order; the canonical tree (node sets,
 split axes, bound bit patterns); closest-hit primitive ids, t and barycentrics
-on identical rays; any-hit booleans. Bounce rays go through cosf/sinf, whose
-results may differ by an ulp between libms: origins are compared within 2e-6
-relative and directions by angle (2e-5 rad unless grazing), then the GPU's
-bounce rays are traced by both sides and compared bit-exactly.
+on identical rays; any-hit booleans. Bounce directions go through cos and sin,
+evaluated in double and rounded once to float on both sides, so they are
+bit-identical unless the two double-precision libraries straddle a float
+rounding boundary (a handful of rays at most); origins and directions are
+also held to small tolerances, then the GPU's bounce rays are traced by both
+sides and compared bit-exactly.
 """
 import numpy as np
 import pytest
@@ -112,8 +114,8 @@ def test_intersect_and_bounce_parity_1m(ctx):
     active = obr[:, 3] > 0
     assert np.array_equal(br[:, 3] > 0, active)
     assert np.allclose(br[active, :3], obr[active, :3], rtol=2e-6, atol=1e-7)
-    # directions go through cosf/sinf (1-ulp differences between libms) and
-    # z = sqrt(1 - x^2 - y^2), which amplifies them by x/z near grazing angles
+    # directions go through cos/sin rounded from double; an ulp difference in a
+    # rare case is amplified by z = sqrt(1 - x^2 - y^2) near grazing angles
     diff = np.abs(br[active, 4:7].astype(np.float64) - obr[active, 4:7]).max(axis=1)
     P = tri[op[active]].astype(np.float64)
     n = np.cross(P[:, 0] - P[:, 2], P[:, 1] - P[:, 2])
@@ -121,7 +123,7 @@ def test_intersect_and_bounce_parity_1m(ctx):
     cos_n = np.abs(np.sum(n * obr[active, 4:7], axis=1))
     log(f"bounce direction: max |diff| {diff.max():.3g}, {np.mean(diff > 0):.2e} not bit-equal")
     assert diff[cos_n > 0.05].max() < 5e-6
-    assert diff.max() < 1e-3 and np.mean(diff > 1e-6) < 1e-3
+    assert diff.max() < 1e-3 and np.mean(diff > 0) < 1e-5
     log("intersect bounce rays")
     s2 = np.zeros(3, np.uint64)
     p2, t2, b2 = bvh.intersect(br, stats=s2)
