@@ -147,8 +147,9 @@ int pbrtm_gen_camera_rays(uint32_t width, uint32_t height, const float eye[3], c
  * ------------------------------------------------------------------------ */
 typedef struct pbrtm_scene pbrtm_scene;
 
-#define PBRTM_MATERIAL_DIFFUSE 0u   /* materials/diffuse_material.cu */
-#define PBRTM_MATERIAL_CONDUCTOR 1u /* materials/conductor_material.cu */
+#define PBRTM_MATERIAL_DIFFUSE 0u    /* materials/diffuse_material.cu */
+#define PBRTM_MATERIAL_CONDUCTOR 1u  /* materials/conductor_material.cu */
+#define PBRTM_MATERIAL_DIELECTRIC 2u /* materials/dielectric_material.cu */
 #define PBRTM_METAL_AG 0u           /* named spectra "metal-Ag-eta"/"metal-Ag-k" (scene_builder.cu:137-168) */
 #define PBRTM_METAL_AL 1u
 #define PBRTM_METAL_AU 2u
@@ -158,8 +159,10 @@ typedef struct pbrtm_material {
     uint32_t type;            /* PBRTM_MATERIAL_* */
     float rgb[3];             /* diffuse "rgb reflectance" (each <= 1) */
     uint32_t metal;           /* conductor eta/k: PBRTM_METAL_* */
-    float uroughness, vroughness;
+    float uroughness, vroughness; /* conductor and dielectric */
     uint32_t remap_roughness; /* "bool remaproughness" (default true) */
+    float eta;                /* dielectric "float eta" (default 1.5) */
+    uint32_t eta_named;       /* dielectric: 0 = the constant eta, 1 = "spectrum eta" "glass-BK7" */
 } pbrtm_material;
 
 typedef struct pbrtm_area_light { /* AreaLightSource "diffuse" (lights/diffuse_area_light.cu:9-35) */

@@ -145,7 +145,7 @@ def murmur64a(data: bytes, seed: int = 0) -> int:
 # ----------------------------------------------------------------- path tracer
 class OrcMaterial(ctypes.Structure):
     _fields_ = [("type", c_uint32), ("rgb", c_float * 3), ("metal", c_uint32), ("uroughness", c_float),
-                ("vroughness", c_float), ("remap_roughness", c_uint32)]
+                ("vroughness", c_float), ("remap_roughness", c_uint32), ("eta", c_float), ("eta_named", c_uint32)]
 
 
 class OrcLight(ctypes.Structure):
@@ -198,8 +198,13 @@ def _scene_struct(sc, pool_size):
             mats[i].type = 0
             mats[i].rgb[:] = list(m["rgb"])
         else:
-            mats[i].type = 1
-            mats[i].metal = METALS[m["metal"]]
+            mats[i].type = 1 if m["type"] == "conductor" else 2
+            if m["type"] == "conductor":
+                mats[i].metal = METALS[m["metal"]]
+            else:
+                eta = m.get("eta", 1.5)
+                mats[i].eta_named = 1 if eta == "glass-BK7" else 0
+                mats[i].eta = 0.0 if isinstance(eta, str) else float(eta)
             mats[i].uroughness = m.get("uroughness", m.get("roughness", 0.0))
             mats[i].vroughness = m.get("vroughness", m.get("roughness", 0.0))
             mats[i].remap_roughness = int(m.get("remap", True))

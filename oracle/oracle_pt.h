@@ -3,7 +3,7 @@
  * liboracle.so, and only as the checker or the timed CPU baseline.
  *
  * A host-only restatement of integrators/wavefront_path.cu and everything it
- * calls for diffuse and conductor surfaces lit by diffuse area lights: the
+ * calls for diffuse, conductor and dielectric surfaces lit by diffuse area lights: the
  * stratified sampler, the tabulated Mitchell filter, the perspective camera,
  * spectral sampling and the RGB film. Paths are scheduled in rounds exactly as
  * the reference's path pool schedules them, so the film receives its samples
@@ -17,11 +17,13 @@ extern "C" {
 #endif
 
 typedef struct orc_material {
-    uint32_t type;          /* 0 = diffuse, 1 = conductor */
+    uint32_t type;          /* 0 = diffuse, 1 = conductor, 2 = dielectric */
     float rgb[3];           /* diffuse: RGB reflectance (albedo, each <= 1) */
     uint32_t metal;         /* conductor: eta/k of 0 = Ag, 1 = Al, 2 = Au, 3 = Cu */
     float uroughness, vroughness;
     uint32_t remap_roughness;
+    float eta;              /* dielectric: constant index of refraction */
+    uint32_t eta_named;     /* dielectric: 0 = constant eta, 1 = "glass-BK7" */
 } orc_material;
 
 typedef struct orc_light {  /* AreaLightSource "diffuse" */

@@ -438,6 +438,8 @@ struct Material {
     uint32_t type;
     Sigmoid albedo;               // diffuse
     const PLSpectrum *eta, *k;    // conductor
+    float eta_c;                  // dielectric, constant eta
+    const PLSpectrum *eta_pl;     // dielectric, spectral eta (secondary wavelengths terminated)
     float ax, ay;
 };
 struct Light {
@@ -459,7 +461,7 @@ struct Scene {
     std::vector<Light> lights;
     std::vector<float> light_pmf, light_cdf;
     Spectra *sp;
-    PLSpectrum metals[8];
+    PLSpectrum metals[8], glass_bk7;
     M3 out_from_sensor;
     ~Scene() {
         if (bvh) orc_bvh_free(bvh);
@@ -490,6 +492,7 @@ void build_scene(Scene &sc, const orc_scene &d) {
     const int metal_len[8] = {pbrtm_data::kAgEtaLen, pbrtm_data::kAgKLen, pbrtm_data::kAlEtaLen, pbrtm_data::kAlKLen,
                               pbrtm_data::kAuEtaLen, pbrtm_data::kAuKLen, pbrtm_data::kCuEtaLen, pbrtm_data::kCuKLen};
     for (int i = 0; i < 8; ++i) sc.metals[i] = pl_from_interleaved(metal_data[i], metal_len[i]);
+    sc.glass_bk7 = pl_from_interleaved(pbrtm_data::kGlassBK7Eta, pbrtm_data::kGlassBK7EtaLen);
     // Materials (materials/diffuse_material.cu, materials/conductor_material.cu).
     for (uint32_t i = 0; i < d.num_materials; ++i) {
         const orc_material &m = d.materials[i];
@@ -510,7 +513,17 @@ void build_scene(Scene &sc, const orc_scene &d) {
             }
             mm.ax = u;
             mm.ay = v;
-            if (!(std::max(mm.ax, mm.ay) < 1e-3f)) throw std::runtime_error("rough conductor not in the oracle");
+        } else if (m.type == 2) {
+            mm.eta_c = m.eta;
+            mm.eta_pl = m.eta_named == 1 ? &sc.glass_bk7 : nullptr;
+            if (m.eta_named > 1) throw std::runtime_error("unknown named eta");
+            float u = m.uroughness, v = m.vroughness;
+            if (m.remap_roughness) {
+                u = std::sqrt(u);
+                v = std::sqrt(v);
+            }
+            mm.ax = u;
+            mm.ay = v;
         } else {
             throw std::runtime_error("unknown material type");
         }
@@ -643,13 +656,6 @@ struct Frame {
     V3 to_local(V3 v) const { return {dot(v, x), dot(v, y), dot(v, z)}; }
     V3 from_local(V3 v) const { return add(add(scale(v.x, x), scale(v.y, y)), scale(v.z, z)); }
 };
-struct Bsdf {
-    Frame fr;
-    uint32_t type;  // 0 diffuse, 1 specular conductor
-    S4 r;           // diffuse reflectance
-    S4 eta, k;
-    bool diffuse_set;
-};
 const float kInvPi = float(1.0 / double(kPi));
 
 struct Cx {
@@ -682,76 +688,317 @@ inline float fr_complex(float cos_i, Cx eta) {
     Cx r_perp = (cx(cos_i) - eta * cos_t) / (cx(cos_i) + eta * cos_t);
     return (cnorm(r_parl) + cnorm(r_perp)) / 2;
 }
+// util/scattering.h:82-102
+inline float fr_dielectric(float cos_i, float eta) {
+    cos_i = clampf(cos_i, -1, 1);
+    if (cos_i < 0) {
+        eta = 1 / eta;
+        cos_i = -cos_i;
+    }
+    float sin2_i = 1 - sqrf(cos_i);
+    float sin2_t = sin2_i / sqrf(eta);
+    if (sin2_t >= 1) return 1.0f;
+    float cos_t = safe_sqrt(1 - sin2_t);
+    float r_parl = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
+    float r_perp = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
+    return (sqrf(r_parl) + sqrf(r_perp)) / 2;
+}
+// util/scattering.h:15-44
+inline bool refract(V3 wi, V3 n, float eta, float *etap, V3 *wt) {
+    float cos_i = dot(n, wi);
+    if (cos_i < 0) {
+        eta = 1 / eta;
+        cos_i = -cos_i;
+        n = neg(n);
+    }
+    float sin2_i = std::max(0.0f, 1 - sqrf(cos_i));
+    float sin2_t = sin2_i / sqrf(eta);
+    if (sin2_t >= 1) return false;
+    float cos_t = std::sqrt(1 - sin2_t);
+    *wt = add(divs(neg(wi), eta), scale(cos_i / eta - cos_t, n));
+    *etap = eta;
+    return true;
+}
+inline V3 reflect(V3 wo, V3 n) { return add(neg(wo), scale(2 * dot(wo, n), n)); }
 
-inline Bsdf make_bsdf(const Scene &sc, const Isect &is, const Lambda &lam) {
+// Local-frame trigonometry (euclidean_space/vector3.h:173-209).
+inline float cos2_theta(V3 w) { return sqrf(w.z); }
+inline float sin2_theta(V3 w) { return std::max(0.0f, float(1.0 - double(cos2_theta(w)))); }
+inline float tan2_theta(V3 w) { return sin2_theta(w) / cos2_theta(w); }
+inline float sin_theta(V3 w) { return std::sqrt(sin2_theta(w)); }
+inline float cos_phi(V3 w) {
+    float s = sin_theta(w);
+    return (s == 0) ? 1 : clampf(w.x / s, -1, 1);
+}
+inline float sin_phi(V3 w) {
+    float s = sin_theta(w);
+    return (s == 0) ? 0 : clampf(w.y / s, -1, 1);
+}
+inline bool same_hemisphere(V3 a, V3 b) { return a.z * b.z > 0; }
+
+// Trowbridge-Reitz microfacet distribution (util/scattering.h:105-229).
+struct TR {
+    float ax, ay;
+    bool smooth() const { return std::max(ax, ay) < 1e-3f; }
+    float D(V3 wm) const {
+        float t2 = tan2_theta(wm);
+        if (std::isinf(t2)) return 0;
+        float cos4 = sqrf(cos2_theta(wm));
+        if (cos4 < 1e-16f) return 0;
+        float e = t2 * (sqrf(cos_phi(wm) / ax) + sqrf(sin_phi(wm) / ay));
+        return 1 / (kPi * ax * ay * cos4 * sqrf(1 + e));
+    }
+    float lambda(V3 w) const {
+        float t2 = tan2_theta(w);
+        if (std::isinf(t2)) return 0;
+        float a2 = sqrf(cos_phi(w) * ax) + sqrf(sin_phi(w) * ay);
+        return (std::sqrt(1 + a2 * t2) - 1) / 2;
+    }
+    float G1(V3 w) const { return 1 / (1 + lambda(w)); }
+    float G(V3 wo, V3 wi) const { return 1 / (1 + lambda(wo) + lambda(wi)); }
+    float D(V3 w, V3 wm) const { return G1(w) / std::fabs(w.z) * D(wm) * abs_dot(w, wm); }
+    float pdf(V3 w, V3 wm) const { return D(w, wm); }
+    V3 sample_wm(V3 w, float u0, float u1) const {
+        V3 wh = normalize({ax * w.x, ay * w.y, w.z});
+        if (wh.z < 0) wh = neg(wh);
+        V3 T1 = (wh.z < 0.99999f) ? normalize(cross({0, 0, 1}, wh)) : V3{1, 0, 0};
+        V3 T2 = cross(wh, T1);
+        float r = std::sqrt(u0), th = 2 * kPi * u1;  // sampling.h:85-90, polar disk
+        float px = r * tcos(th), py = r * tsin(th);
+        float h = std::sqrt(1 - sqrf(px));
+        py = lerpf((1 + wh.z) / 2, h, py);
+        float pz = std::sqrt(std::max(0.0f, 1 - (px * px + py * py)));
+        V3 nh = add(add(scale(px, T1), scale(py, T2)), scale(pz, wh));
+        return normalize({ax * nh.x, ay * nh.y, std::max(1e-6f, nh.z)});
+    }
+};
+
+enum : uint32_t { kDiffuse = 0, kConductor = 1, kDielectric = 2 };
+// BxDFFlags (base/bxdf.h): reflection 1, transmission 2, diffuse 4, glossy 8, specular 16
+struct Bsdf {
+    Frame fr;
+    uint32_t type;
+    S4 r;          // diffuse reflectance
+    S4 eta, k;     // conductor
+    float eta_d;   // dielectric
+    TR tr;
+    bool diffuse_set;
+};
+
+inline Bsdf make_bsdf(const Scene &sc, const Isect &is, Lambda &lam) {
     const Material &m = sc.mats[sc.tris[is.tri].material];
     Bsdf b;
     V3 x = normalize(is.dpdu);
     b.fr = {x, cross(is.ns, x), is.ns};
     b.type = m.type;
-    if (m.type == 0) {
+    b.tr = TR{m.ax, m.ay};
+    if (!b.tr.smooth()) b.tr = TR{std::max(m.ax, 1e-4f), std::max(m.ay, 1e-4f)};
+    if (m.type == kDiffuse) {
         S4 r = sample_sigmoid(m.albedo, lam);
         for (int i = 0; i < kNS; ++i) r.v[i] = clampf(r.v[i], 0.0f, 1.0f);
         b.r = r;
         b.diffuse_set = positive(r);
-    } else {
+    } else if (m.type == kConductor) {
         b.eta = sample_pl(*m.eta, lam);
         b.k = sample_pl(*m.k, lam);
+    } else {
+        // materials/dielectric_material.cu:37-70
+        float e = m.eta_pl ? (*m.eta_pl)(lam.l[0]) : m.eta_c;
+        if (m.eta_pl) {  // terminate_secondary (sampled_wavelengths.h:84-95)
+            bool done = true;
+            for (int i = 1; i < kNS; ++i) done = done && lam.pdf[i] == 0;
+            if (!done) {
+                for (int i = 1; i < kNS; ++i) lam.pdf[i] = 0;
+                lam.pdf[0] /= kNS;
+            }
+        }
+        if (e == 0) e = 1;
+        b.eta_d = e;
     }
     return b;
 }
-// flags: 1 = reflection, 4 = diffuse, 16 = specular (base/bxdf.h BxDFFlags)
 inline uint32_t bsdf_flags(const Bsdf &b) {
-    if (b.type == 0) return b.diffuse_set ? (1u | 4u) : 0u;
-    return 1u | 16u;
+    if (b.type == kDiffuse) return b.diffuse_set ? (1u | 4u) : 0u;
+    if (b.type == kConductor) return b.tr.smooth() ? (1u | 16u) : (1u | 8u);
+    uint32_t f = b.eta_d == 1 ? 2u : (1u | 2u);
+    return f | (b.tr.smooth() ? 16u : 8u);
 }
 inline bool non_specular(uint32_t f) { return (f & (4u | 8u)) != 0; }
 
+// BxDF::f in the local frame; `valid` is false where the reference returns an
+// unset (NaN) spectrum, which callers treat as zero.
+inline S4 bxdf_f(const Bsdf &b, V3 wo, V3 wi) {
+    if (b.type == kDiffuse) {
+        if (!same_hemisphere(wo, wi)) return s4(0.f);
+        return b.r * kInvPi;
+    }
+    if (b.type == kConductor) {  // bxdfs/conductor_bxdf.cu:54-83
+        if (!same_hemisphere(wo, wi) || b.tr.smooth()) return s4(0.f);
+        float co = std::fabs(wo.z), ci = std::fabs(wi.z);
+        if (ci == 0 || co == 0) return s4(0.f);
+        V3 wm = add(wi, wo);
+        if (sqlen(wm) == 0) return s4(0.f);
+        wm = normalize(wm);
+        S4 F;
+        for (int i = 0; i < kNS; ++i) F.v[i] = fr_complex(abs_dot(wo, wm), Cx{b.eta.v[i], b.k.v[i]});
+        return ((F * b.tr.D(wm)) * b.tr.G(wo, wi)) / (4 * ci * co);
+    }
+    // bxdfs/dielectric_bxdf.cu:113-164
+    if (b.eta_d == 1 || b.tr.smooth()) return s4(0.f);
+    float co = wo.z, ci = wi.z;
+    bool refl = ci * co > 0;
+    float etap = 1;
+    if (!refl) etap = co > 0 ? b.eta_d : (1 / b.eta_d);
+    V3 wm = add(scale(etap, wi), wo);
+    if (ci == 0 || co == 0 || sqlen(wm) == 0) return s4(0.f);
+    wm = normalize(face_forward(wm, {0, 0, 1}));
+    if (dot(wm, wi) * ci < 0 || dot(wm, wo) * co < 0) return s4(0.f);
+    float F = fr_dielectric(dot(wo, wm), b.eta_d);
+    if (refl) return s4(b.tr.D(wm) * b.tr.G(wo, wi) * F / std::fabs(4 * ci * co));
+    float denom = sqrf(dot(wi, wm) + dot(wo, wm) / etap) * ci * co;
+    float ft = b.tr.D(wm) * (1 - F) * b.tr.G(wo, wi) * std::fabs(dot(wi, wm) * dot(wo, wm) / denom);
+    ft /= sqrf(etap);
+    return s4(ft);
+}
+inline float bxdf_pdf(const Bsdf &b, V3 wo, V3 wi) {
+    if (b.type == kDiffuse) {
+        if (!same_hemisphere(wo, wi)) return 0;
+        return std::fabs(wi.z) / kPi;
+    }
+    if (b.type == kConductor) {  // conductor_bxdf.cu:85-106
+        if (!same_hemisphere(wo, wi) || b.tr.smooth()) return 0;
+        V3 wm = add(wo, wi);
+        if (sqlen(wm) == 0) return 0;
+        wm = face_forward(normalize(wm), {0, 0, 1});
+        return b.tr.pdf(wo, wm) / (4 * abs_dot(wo, wm));
+    }
+    // dielectric_bxdf.cu:166-230
+    if (b.eta_d == 1 || b.tr.smooth()) return 0;
+    float co = wo.z, ci = wi.z;
+    bool refl = ci * co > 0;
+    float etap = 1;
+    if (!refl) etap = co > 0 ? b.eta_d : (1 / b.eta_d);
+    V3 wm = add(scale(etap, wi), wo);
+    if (ci == 0 || co == 0 || sqlen(wm) == 0) return 0;
+    wm = normalize(face_forward(wm, {0, 0, 1}));
+    if (dot(wm, wi) * ci < 0 || dot(wm, wo) * co < 0) return 0;
+    float R = fr_dielectric(dot(wo, wm), b.eta_d);
+    float T = 1 - R;
+    float pr = R, pt = T;
+    if (pr == 0 && pt == 0) return 0;
+    if (refl) return b.tr.pdf(wo, wm) / (4 * abs_dot(wo, wm)) * pr / (pr + pt);
+    float denom = sqrf(dot(wi, wm) + dot(wo, wm) / etap);
+    float dwm_dwi = abs_dot(wi, wm) / denom;
+    return b.tr.pdf(wo, wm) * dwm_dwi * pt / (pr + pt);
+}
+// BSDF::f / BSDF::pdf (base/bsdf.cu:44-93): render space in, local evaluation.
 inline S4 bsdf_f(const Bsdf &b, V3 wo_r, V3 wi_r) {
     V3 wi = b.fr.to_local(wi_r), wo = b.fr.to_local(wo_r);
     if (wo.z == 0) return s4(0.f);
-    if (b.type == 0) {
-        if (!(wo.z * wi.z > 0)) return s4(0.f);
-        return b.r * kInvPi;
-    }
-    return s4(0.f);  // specular conductor
+    return bxdf_f(b, wo, wi);
 }
 inline float bsdf_pdf(const Bsdf &b, V3 wo_r, V3 wi_r) {
     V3 wo = b.fr.to_local(wo_r), wi = b.fr.to_local(wi_r);
     if (wo.z == 0) return 0;
-    if (b.type == 0) {
-        if (!(wo.z * wi.z > 0)) return 0;
-        return std::fabs(wi.z) / kPi;
-    }
-    return 0;
+    return bxdf_pdf(b, wo, wi);
 }
 struct BS {
     S4 f;
     V3 wi;
-    float pdf;
-    bool specular;
+    float pdf, eta;
+    bool specular, transmission;
 };
+// BxDF::sample_f for each lobe, then BSDF::sample_f's checks (base/bsdf.cu:57-80).
 inline bool bsdf_sample(const Bsdf &b, V3 wo_r, float uc, float u0, float u1, BS *out) {
-    (void)uc;
     V3 wo = b.fr.to_local(wo_r);
     if (wo.z == 0 || bsdf_flags(b) == 0) return false;
     BS bs;
-    if (b.type == 0) {
+    bs.eta = 1;
+    bs.transmission = false;
+    if (b.type == kDiffuse) {
         V3 wi = sample_cosine_hemisphere(u0, u1);
         if (wo.z < 0) wi.z *= -1;
         bs.pdf = std::fabs(wi.z) / kPi;
         bs.f = b.r * kInvPi;
         bs.wi = wi;
         bs.specular = false;
-    } else {
-        V3 wi = {-wo.x, -wo.y, wo.z};
-        float c = std::fabs(wi.z);
-        S4 F;
-        for (int i = 0; i < kNS; ++i) F.v[i] = fr_complex(c, Cx{b.eta.v[i], b.k.v[i]});
-        bs.f = F / c;
-        bs.wi = wi;
-        bs.pdf = 1;
-        bs.specular = true;
+    } else if (b.type == kConductor) {
+        if (b.tr.smooth()) {
+            V3 wi = {-wo.x, -wo.y, wo.z};
+            float c = std::fabs(wi.z);
+            S4 F;
+            for (int i = 0; i < kNS; ++i) F.v[i] = fr_complex(c, Cx{b.eta.v[i], b.k.v[i]});
+            bs.f = F / c;
+            bs.wi = wi;
+            bs.pdf = 1;
+            bs.specular = true;
+        } else {  // conductor_bxdf.cu:24-52
+            if (wo.z == 0) return false;
+            V3 wm = b.tr.sample_wm(wo, u0, u1);
+            V3 wi = reflect(wo, wm);
+            if (!same_hemisphere(wo, wi)) return false;
+            float pdf = b.tr.pdf(wo, wm) / (4 * abs_dot(wo, wm));
+            float co = std::fabs(wo.z), ci = std::fabs(wi.z);
+            if (ci == 0 || co == 0) return false;
+            S4 F;
+            for (int i = 0; i < kNS; ++i) F.v[i] = fr_complex(abs_dot(wo, wm), Cx{b.eta.v[i], b.k.v[i]});
+            bs.f = ((F * b.tr.D(wm)) * b.tr.G(wo, wi)) / (4 * ci * co);
+            bs.wi = wi;
+            bs.pdf = pdf;
+            bs.specular = false;
+        }
+    } else if (b.eta_d == 1 || b.tr.smooth()) {  // dielectric_bxdf.cu:4-49
+        float R = fr_dielectric(wo.z, b.eta_d), T = 1 - R;
+        float pr = R, pt = T;
+        if (pr == 0 && pt == 0) return false;
+        if (uc < pr / (pr + pt)) {
+            V3 wi = {-wo.x, -wo.y, wo.z};
+            bs.f = s4(R / std::fabs(wi.z));
+            bs.wi = wi;
+            bs.pdf = pr / (pr + pt);
+            bs.specular = true;
+        } else {
+            V3 wi;
+            float etap;
+            if (!refract(wo, {0, 0, 1}, b.eta_d, &etap, &wi)) return false;
+            S4 ft = s4(T / std::fabs(wi.z));
+            ft = ft / sqrf(etap);
+            bs.f = ft;
+            bs.wi = wi;
+            bs.pdf = pt / (pr + pt);
+            bs.specular = true;
+            bs.transmission = true;
+            bs.eta = etap;
+        }
+    } else {  // rough dielectric (dielectric_bxdf.cu:50-110)
+        V3 wm = b.tr.sample_wm(wo, u0, u1);
+        float R = fr_dielectric(dot(wo, wm), b.eta_d), T = 1 - R;
+        float pr = R, pt = T;
+        if (pr == 0 && pt == 0) return false;
+        if (uc < pr / (pr + pt)) {
+            V3 wi = reflect(wo, wm);
+            if (!same_hemisphere(wo, wi)) return false;
+            bs.pdf = b.tr.pdf(wo, wm) / (4 * abs_dot(wo, wm)) * pr / (pr + pt);
+            bs.f = s4(b.tr.D(wm) * b.tr.G(wo, wi) * R / (4 * wi.z * wo.z));
+            bs.wi = wi;
+            bs.specular = false;
+        } else {
+            float etap;
+            V3 wi;
+            bool tir = !refract(wo, wm, b.eta_d, &etap, &wi);
+            if (same_hemisphere(wo, wi) || wi.z == 0 || tir) return false;
+            float denom = sqrf(dot(wi, wm) + dot(wo, wm) / etap);
+            float dwm_dwi = abs_dot(wi, wm) / denom;
+            bs.pdf = b.tr.pdf(wo, wm) * dwm_dwi * pt / (pr + pt);
+            float ft = T * b.tr.D(wm) * b.tr.G(wo, wi) * std::fabs(dot(wi, wm) * dot(wo, wm) / (wi.z * wo.z * denom));
+            S4 f = s4(ft);
+            f = f / sqrf(etap);
+            bs.f = f;
+            bs.wi = wi;
+            bs.specular = false;
+            bs.transmission = true;
+            bs.eta = etap;
+        }
     }
     if (!positive(bs.f) || bs.pdf == 0 || bs.wi.z == 0) return false;
     bs.wi = b.fr.from_local(bs.wi);
@@ -1252,6 +1499,7 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
             p.pdf_bsdf = bs.pdf;
             p.specular = bs.specular;
             p.any_non_specular |= !bs.specular;
+            if (bs.transmission) p.eta_scale *= sqrf(bs.eta);
             p.prev = LightCtx{is.pi, is.n, is.ns};
             p.ro = offset_ray_origin(is.pi, is.n, bs.wi);
             p.rd = bs.wi;

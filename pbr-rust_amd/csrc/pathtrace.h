@@ -16,14 +16,15 @@
 namespace pbrtm {
 namespace pt {
 
-enum MatType : uint32_t { kDiffuse = 0, kConductor = 1 };
+enum MatType : uint32_t { kDiffuse = 0, kConductor = 1, kDielectric = 2 };
 
 struct DMat {  // 48 bytes
     uint32_t type;
     float c0, c1, c2;               // diffuse albedo sigmoid coefficients
-    uint32_t eta_off, eta_n, k_off, k_n;  // conductor eta/k in the spectrum pool
-    float ax, ay;                   // Trowbridge-Reitz alphas
-    uint32_t pad[2];
+    uint32_t eta_off, eta_n, k_off, k_n;  // conductor eta/k, or a dielectric's spectral eta (eta_n > 0)
+    float ax, ay;                   // Trowbridge-Reitz alphas after the roughness remap
+    float eta_c;                    // dielectric constant eta (when eta_n == 0)
+    uint32_t pad;
 };
 
 struct DLight {  // 32 bytes: a DiffuseAreaLight on one triangle

@@ -400,18 +400,34 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Par
     const DMat m = S.mats[info.x];
     const f3 fx = norm3(dpdu);
     const f3 fy = cross3(ns, fx);
-    S4 r = s4(0.f), eta = s4(0.f), kk = s4(0.f);
-    uint32_t bflags;  // BxDFFlags: 1 reflection, 4 diffuse, 16 specular
+    Bxdf bx;
+    bx.type = m.type;
+    bx.tr = TR{m.ax, m.ay};
+    if (!bx.tr.smooth()) bx.tr = TR{smax(m.ax, 1e-4f), smax(m.ay, 1e-4f)};
+    bx.r = s4(0.f);
+    bx.eta = s4(0.f);
+    bx.k = s4(0.f);
+    bx.eta_d = 1.f;
     if (m.type == kDiffuse) {
-        for (int k = 0; k < kNS; ++k) r.v[k] = clampf(sigmoid_eval(m.c0, m.c1, m.c2, lam.v[k]), 0.0f, 1.0f);
-        bflags = s4pos(r) ? (1u | 4u) : 0u;
-    } else {
+        for (int k = 0; k < kNS; ++k) bx.r.v[k] = clampf(sigmoid_eval(m.c0, m.c1, m.c2, lam.v[k]), 0.0f, 1.0f);
+    } else if (m.type == kConductor) {
         for (int k = 0; k < kNS; ++k) {
-            eta.v[k] = spec_eval(S, m.eta_off, m.eta_n, lam.v[k]);
-            kk.v[k] = spec_eval(S, m.k_off, m.k_n, lam.v[k]);
+            bx.eta.v[k] = spec_eval(S, m.eta_off, m.eta_n, lam.v[k]);
+            bx.k.v[k] = spec_eval(S, m.k_off, m.k_n, lam.v[k]);
         }
-        bflags = 1u | 16u;
+    } else {  // materials/dielectric_material.cu:37-70
+        float e = m.eta_c;
+        if (m.eta_n) {
+            e = spec_eval(S, m.eta_off, m.eta_n, lam.v[0]);
+            if (!(lpdf.v[1] == 0 && lpdf.v[2] == 0 && lpdf.v[3] == 0)) {  // terminate_secondary
+                lpdf.v[1] = lpdf.v[2] = lpdf.v[3] = 0;
+                lpdf.v[0] /= kNS;
+                st.lam_pdf[i] = s4f(lpdf);
+            }
+        }
+        bx.eta_d = e == 0 ? 1.f : e;
     }
+    const uint32_t bflags = bx.flags();
     bool nee = false;
     if (bflags & (4u | 8u)) {
         // sample_ld (wavefront_path.cu:574-624)
@@ -435,17 +451,13 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Par
                 const f3 wi = norm3(sub3(sp, cpt));
                 const S4 Le = light_L(S, Lg, ss.n, neg3(wi), lam);
                 if (s4pos(Le) && ss.pdf != 0) {
-                    // BSDF::f for the diffuse lobe (only non-specular BSDFs reach here)
+                    // BSDF::f and BSDF::pdf (bsdf.cu:44-93) in the shading frame
                     const f3 wil = {dot3(wi, fx), dot3(wi, fy), dot3(wi, ns)};
                     const f3 wol = {dot3(wo_n, fx), dot3(wo_n, fy), dot3(wo_n, ns)};
-                    S4 f = s4(0.f);
-                    float pdf_b = 0.f;
-                    if (wol.z != 0 && wol.z * wil.z > 0) {
-                        f = s4scale(r, kInvPi);
-                        pdf_b = fabsf(wil.z) / kPi;
-                    }
+                    S4 f = wol.z == 0 ? s4(0.f) : bx.f(wol, wil);
                     f = s4scale(f, absdot3(wi, ns));
                     if (s4pos(f)) {
+                        const float pdf_b = wol.z == 0 ? 0.f : bx.pdf(wol, wil);
                         const float pdf_light = pmf * ss.pdf;
                         const float wl = power_heuristic(pdf_light, pdf_b);
                         const S4 Ld = s4div(s4mul(s4scale(Le, wl), f), pdf_light);
@@ -468,40 +480,20 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Par
     }
     // BSDF::sample_f (bsdf.cu:57-80)
     const f3 wo = neg3(rd);
-    (void)smp.get_1d();
+    const float uc = smp.get_1d();
     float u0, u1;
     smp.get_2d(&u0, &u1);
     const f3 wol = {dot3(wo, fx), dot3(wo, fy), dot3(wo, ns)};
-    bool ok = !(wol.z == 0 || bflags == 0);
-    S4 f = s4(0.f);
-    f3 wil = {0.f, 0.f, 0.f};
-    float pdf = 0.f;
-    bool specular = false;
-    if (ok) {
-        if (m.type == kDiffuse) {
-            wil = cosine_hemisphere(u0, u1);
-            if (wol.z < 0) wil.z *= -1;
-            pdf = fabsf(wil.z) / kPi;
-            f = s4scale(r, kInvPi);
-        } else {
-            wil = {-wol.x, -wol.y, wol.z};
-            const float c = fabsf(wil.z);
-            S4 F;
-            for (int k = 0; k < kNS; ++k) F.v[k] = fr_complex(c, eta.v[k], kk.v[k]);
-            f = s4div(F, c);
-            pdf = 1;
-            specular = true;
-        }
-        ok = s4pos(f) && pdf != 0 && wil.z != 0;
-    }
-    if (!ok) {
+    BxSample bs;
+    if (!bxdf_sample(bx, wol, uc, u0, u1, &bs)) {
         beta = s4(0.0f);  // the stale ray is traced again and the path ends next round
     } else {
-        const f3 wi = add3(add3(mul3(wil.x, fx), mul3(wil.y, fy)), mul3(wil.z, ns));
-        beta = s4mul(beta, s4div(s4scale(f, absdot3(wi, ns)), pdf));
-        scal.y = pdf;
-        fl = specular ? (fl | kFlagSpecular) : (fl & ~kFlagSpecular);
-        if (!specular) fl |= kFlagAnyNonSpec;
+        const f3 wi = add3(add3(mul3(bs.wi.x, fx), mul3(bs.wi.y, fy)), mul3(bs.wi.z, ns));
+        beta = s4mul(beta, s4div(s4scale(bs.f, absdot3(wi, ns)), bs.pdf));
+        scal.y = bs.pdf;
+        fl = bs.specular ? (fl | kFlagSpecular) : (fl & ~kFlagSpecular);
+        if (!bs.specular) fl |= kFlagAnyNonSpec;
+        if (bs.transmission) scal.z *= sqrf(bs.eta);
         st.prev[3 * size_t(i)] = make_float4(ipi.x.lo, ipi.x.hi, ipi.y.lo, ipi.y.hi);
         st.prev[3 * size_t(i) + 1] = make_float4(ipi.z.lo, ipi.z.hi, n.x, n.y);
         st.prev[3 * size_t(i) + 2] = make_float4(n.z, ns.x, ns.y, ns.z);

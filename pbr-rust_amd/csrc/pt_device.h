@@ -399,5 +399,256 @@ PBRTM_HD f3 ctx_offset_origin(const P3i &pi, f3 n, f3 w) {
     return {round_away(po.x, off.x), round_away(po.y, off.y), round_away(po.z, off.z)};
 }
 
+// ------------------------------------------------------------ BSDFs in the local frame
+// util/scattering.h:82-102
+PBRTM_HD float fr_dielectric(float cos_i, float eta) {
+    cos_i = clampf(cos_i, -1, 1);
+    if (cos_i < 0) {
+        eta = 1 / eta;
+        cos_i = -cos_i;
+    }
+    const float sin2_i = 1 - sqrf(cos_i);
+    const float sin2_t = sin2_i / sqrf(eta);
+    if (sin2_t >= 1) return 1.0f;
+    const float cos_t = safe_sqrtf(1 - sin2_t);
+    const float r_parl = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
+    const float r_perp = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
+    return (sqrf(r_parl) + sqrf(r_perp)) / 2;
+}
+// util/scattering.h:15-44
+PBRTM_HD bool refract(f3 wi, f3 n, float eta, float *etap, f3 *wt) {
+    float cos_i = dot3(n, wi);
+    if (cos_i < 0) {
+        eta = 1 / eta;
+        cos_i = -cos_i;
+        n = neg3(n);
+    }
+    const float sin2_i = smax(0.0f, 1 - sqrf(cos_i));
+    const float sin2_t = sin2_i / sqrf(eta);
+    if (sin2_t >= 1) return false;
+    const float cos_t = sqrtf(1 - sin2_t);
+    const f3 a = neg3(wi);
+    *wt = add3(f3{a.x / eta, a.y / eta, a.z / eta}, mul3(cos_i / eta - cos_t, n));
+    *etap = eta;
+    return true;
+}
+PBRTM_HD f3 reflect3(f3 wo, f3 n) { return add3(neg3(wo), mul3(2 * dot3(wo, n), n)); }
+PBRTM_HD f3 face_fwd(f3 v, f3 w) { return dot3(v, w) < 0.0f ? neg3(v) : v; }
+PBRTM_HD bool same_hemi(f3 a, f3 b) { return a.z * b.z > 0; }
+PBRTM_HD float cos2_th(f3 w) { return sqrf(w.z); }
+PBRTM_HD float sin2_th(f3 w) { return smax(0.0f, float(1.0 - double(cos2_th(w)))); }
+PBRTM_HD float tan2_th(f3 w) { return sin2_th(w) / cos2_th(w); }
+PBRTM_HD float cos_ph(f3 w) {
+    const float s = sqrtf(sin2_th(w));
+    return (s == 0) ? 1 : clampf(w.x / s, -1, 1);
+}
+PBRTM_HD float sin_ph(f3 w) {
+    const float s = sqrtf(sin2_th(w));
+    return (s == 0) ? 0 : clampf(w.y / s, -1, 1);
+}
+
+// Trowbridge-Reitz distribution (util/scattering.h:105-229).
+struct TR {
+    float ax, ay;
+    PBRTM_HD bool smooth() const { return smax(ax, ay) < 1e-3f; }
+    PBRTM_HD float D(f3 wm) const {
+        const float t2 = tan2_th(wm);
+        if (isinf(t2)) return 0;
+        const float cos4 = sqrf(cos2_th(wm));
+        if (cos4 < 1e-16f) return 0;
+        const float e = t2 * (sqrf(cos_ph(wm) / ax) + sqrf(sin_ph(wm) / ay));
+        return 1 / (kPi * ax * ay * cos4 * sqrf(1 + e));
+    }
+    PBRTM_HD float lambda(f3 w) const {
+        const float t2 = tan2_th(w);
+        if (isinf(t2)) return 0;
+        const float a2 = sqrf(cos_ph(w) * ax) + sqrf(sin_ph(w) * ay);
+        return (sqrtf(1 + a2 * t2) - 1) / 2;
+    }
+    PBRTM_HD float G1(f3 w) const { return 1 / (1 + lambda(w)); }
+    PBRTM_HD float G(f3 wo, f3 wi) const { return 1 / (1 + lambda(wo) + lambda(wi)); }
+    PBRTM_HD float pdf(f3 w, f3 wm) const { return G1(w) / fabsf(w.z) * D(wm) * absdot3(w, wm); }
+    PBRTM_HD f3 sample_wm(f3 w, float u0, float u1) const {
+        f3 wh = norm3(f3{ax * w.x, ay * w.y, w.z});
+        if (wh.z < 0) wh = neg3(wh);
+        const f3 T1 = (wh.z < 0.99999f) ? norm3(cross3(f3{0.f, 0.f, 1.f}, wh)) : f3{1.f, 0.f, 0.f};
+        const f3 T2 = cross3(wh, T1);
+        const float r = sqrtf(u0), th = 2 * kPi * u1;  // sampling.h:85-90
+        const float px = r * tcos(th);
+        float py = r * tsin(th);
+        const float h = sqrtf(1 - sqrf(px));
+        py = lerpf((1 + wh.z) / 2, h, py);
+        const float pz = sqrtf(smax(0.0f, 1 - (px * px + py * py)));
+        const f3 nh = add3(add3(mul3(px, T1), mul3(py, T2)), mul3(pz, wh));
+        return norm3(f3{ax * nh.x, ay * nh.y, smax(1e-6f, nh.z)});
+    }
+};
+
+enum : uint32_t { kBxDiffuse = 0, kBxConductor = 1, kBxDielectric = 2 };
+// BxDFFlags (base/bxdf.h): reflection 1, transmission 2, diffuse 4, glossy 8, specular 16
+struct Bxdf {
+    uint32_t type;
+    S4 r, eta, k;  // diffuse reflectance; conductor eta, k
+    float eta_d;   // dielectric
+    TR tr;
+    PBRTM_HD uint32_t flags() const {
+        if (type == kBxDiffuse) return s4pos(r) ? (1u | 4u) : 0u;
+        if (type == kBxConductor) return tr.smooth() ? (1u | 16u) : (1u | 8u);
+        return (eta_d == 1 ? 2u : (1u | 2u)) | (tr.smooth() ? 16u : 8u);
+    }
+    // f; zero where the reference returns an unset spectrum (callers test is_positive)
+    PBRTM_HD S4 f(f3 wo, f3 wi) const {
+        if (type == kBxDiffuse) return same_hemi(wo, wi) ? s4scale(r, kInvPi) : s4(0.f);
+        if (type == kBxConductor) {  // bxdfs/conductor_bxdf.cu:54-83
+            if (!same_hemi(wo, wi) || tr.smooth()) return s4(0.f);
+            const float co = fabsf(wo.z), ci = fabsf(wi.z);
+            if (ci == 0 || co == 0) return s4(0.f);
+            f3 wm = add3(wi, wo);
+            if (sqlen3(wm) == 0) return s4(0.f);
+            wm = norm3(wm);
+            S4 F;
+            for (int i = 0; i < kNS; ++i) F.v[i] = fr_complex(absdot3(wo, wm), eta.v[i], k.v[i]);
+            return s4div(s4scale(s4scale(F, tr.D(wm)), tr.G(wo, wi)), 4 * ci * co);
+        }
+        // bxdfs/dielectric_bxdf.cu:113-164
+        if (eta_d == 1 || tr.smooth()) return s4(0.f);
+        const float co = wo.z, ci = wi.z;
+        const bool refl = ci * co > 0;
+        float etap = 1;
+        if (!refl) etap = co > 0 ? eta_d : (1 / eta_d);
+        f3 wm = add3(mul3(etap, wi), wo);
+        if (ci == 0 || co == 0 || sqlen3(wm) == 0) return s4(0.f);
+        wm = norm3(face_fwd(wm, f3{0.f, 0.f, 1.f}));
+        if (dot3(wm, wi) * ci < 0 || dot3(wm, wo) * co < 0) return s4(0.f);
+        const float F = fr_dielectric(dot3(wo, wm), eta_d);
+        if (refl) return s4(tr.D(wm) * tr.G(wo, wi) * F / fabsf(4 * ci * co));
+        const float denom = sqrf(dot3(wi, wm) + dot3(wo, wm) / etap) * ci * co;
+        float ft = tr.D(wm) * (1 - F) * tr.G(wo, wi) * fabsf(dot3(wi, wm) * dot3(wo, wm) / denom);
+        ft /= sqrf(etap);
+        return s4(ft);
+    }
+    PBRTM_HD float pdf(f3 wo, f3 wi) const {
+        if (type == kBxDiffuse) return same_hemi(wo, wi) ? fabsf(wi.z) / kPi : 0.f;
+        if (type == kBxConductor) {  // conductor_bxdf.cu:85-106
+            if (!same_hemi(wo, wi) || tr.smooth()) return 0;
+            f3 wm = add3(wo, wi);
+            if (sqlen3(wm) == 0) return 0;
+            wm = face_fwd(norm3(wm), f3{0.f, 0.f, 1.f});
+            return tr.pdf(wo, wm) / (4 * absdot3(wo, wm));
+        }
+        // dielectric_bxdf.cu:166-230
+        if (eta_d == 1 || tr.smooth()) return 0;
+        const float co = wo.z, ci = wi.z;
+        const bool refl = ci * co > 0;
+        float etap = 1;
+        if (!refl) etap = co > 0 ? eta_d : (1 / eta_d);
+        f3 wm = add3(mul3(etap, wi), wo);
+        if (ci == 0 || co == 0 || sqlen3(wm) == 0) return 0;
+        wm = norm3(face_fwd(wm, f3{0.f, 0.f, 1.f}));
+        if (dot3(wm, wi) * ci < 0 || dot3(wm, wo) * co < 0) return 0;
+        const float R = fr_dielectric(dot3(wo, wm), eta_d), T = 1 - R;
+        const float pr = R, pt = T;
+        if (pr == 0 && pt == 0) return 0;
+        if (refl) return tr.pdf(wo, wm) / (4 * absdot3(wo, wm)) * pr / (pr + pt);
+        const float denom = sqrf(dot3(wi, wm) + dot3(wo, wm) / etap);
+        const float dwm_dwi = absdot3(wi, wm) / denom;
+        return tr.pdf(wo, wm) * dwm_dwi * pt / (pr + pt);
+    }
+};
+
+struct BxSample {
+    S4 f;
+    f3 wi;
+    float pdf, eta;
+    bool specular, transmission;
+};
+// BxDF::sample_f of each lobe plus BSDF::sample_f's checks (base/bsdf.cu:57-80); wo local.
+PBRTM_HD bool bxdf_sample(const Bxdf &b, f3 wo, float uc, float u0, float u1, BxSample *out) {
+    if (wo.z == 0 || b.flags() == 0) return false;
+    BxSample bs;
+    bs.eta = 1;
+    bs.transmission = false;
+    bs.specular = false;
+    if (b.type == kBxDiffuse) {
+        f3 wi = cosine_hemisphere(u0, u1);
+        if (wo.z < 0) wi.z *= -1;
+        bs.pdf = fabsf(wi.z) / kPi;
+        bs.f = s4scale(b.r, kInvPi);
+        bs.wi = wi;
+    } else if (b.type == kBxConductor) {
+        if (b.tr.smooth()) {
+            const f3 wi = {-wo.x, -wo.y, wo.z};
+            const float c = fabsf(wi.z);
+            S4 F;
+            for (int i = 0; i < kNS; ++i) F.v[i] = fr_complex(c, b.eta.v[i], b.k.v[i]);
+            bs.f = s4div(F, c);
+            bs.wi = wi;
+            bs.pdf = 1;
+            bs.specular = true;
+        } else {  // conductor_bxdf.cu:24-52
+            const f3 wm = b.tr.sample_wm(wo, u0, u1);
+            const f3 wi = reflect3(wo, wm);
+            if (!same_hemi(wo, wi)) return false;
+            const float pdf = b.tr.pdf(wo, wm) / (4 * absdot3(wo, wm));
+            const float co = fabsf(wo.z), ci = fabsf(wi.z);
+            if (ci == 0 || co == 0) return false;
+            S4 F;
+            for (int i = 0; i < kNS; ++i) F.v[i] = fr_complex(absdot3(wo, wm), b.eta.v[i], b.k.v[i]);
+            bs.f = s4div(s4scale(s4scale(F, b.tr.D(wm)), b.tr.G(wo, wi)), 4 * ci * co);
+            bs.wi = wi;
+            bs.pdf = pdf;
+        }
+    } else if (b.eta_d == 1 || b.tr.smooth()) {  // dielectric_bxdf.cu:4-49
+        const float R = fr_dielectric(wo.z, b.eta_d), T = 1 - R;
+        const float pr = R, pt = T;
+        if (pr == 0 && pt == 0) return false;
+        if (uc < pr / (pr + pt)) {
+            const f3 wi = {-wo.x, -wo.y, wo.z};
+            bs.f = s4(R / fabsf(wi.z));
+            bs.wi = wi;
+            bs.pdf = pr / (pr + pt);
+            bs.specular = true;
+        } else {
+            f3 wi;
+            float etap;
+            if (!refract(wo, f3{0.f, 0.f, 1.f}, b.eta_d, &etap, &wi)) return false;
+            bs.f = s4div(s4(T / fabsf(wi.z)), sqrf(etap));
+            bs.wi = wi;
+            bs.pdf = pt / (pr + pt);
+            bs.specular = true;
+            bs.transmission = true;
+            bs.eta = etap;
+        }
+    } else {  // rough dielectric (dielectric_bxdf.cu:50-110)
+        const f3 wm = b.tr.sample_wm(wo, u0, u1);
+        const float R = fr_dielectric(dot3(wo, wm), b.eta_d), T = 1 - R;
+        const float pr = R, pt = T;
+        if (pr == 0 && pt == 0) return false;
+        if (uc < pr / (pr + pt)) {
+            const f3 wi = reflect3(wo, wm);
+            if (!same_hemi(wo, wi)) return false;
+            bs.pdf = b.tr.pdf(wo, wm) / (4 * absdot3(wo, wm)) * pr / (pr + pt);
+            bs.f = s4(b.tr.D(wm) * b.tr.G(wo, wi) * R / (4 * wi.z * wo.z));
+            bs.wi = wi;
+        } else {
+            float etap;
+            f3 wi;
+            const bool tir = !refract(wo, wm, b.eta_d, &etap, &wi);
+            if (same_hemi(wo, wi) || wi.z == 0 || tir) return false;
+            const float denom = sqrf(dot3(wi, wm) + dot3(wo, wm) / etap);
+            const float dwm_dwi = absdot3(wi, wm) / denom;
+            bs.pdf = b.tr.pdf(wo, wm) * dwm_dwi * pt / (pr + pt);
+            const float ft = T * b.tr.D(wm) * b.tr.G(wo, wi) * fabsf(dot3(wi, wm) * dot3(wo, wm) / (wi.z * wo.z * denom));
+            bs.f = s4div(s4(ft), sqrf(etap));
+            bs.wi = wi;
+            bs.transmission = true;
+            bs.eta = etap;
+        }
+    }
+    if (!s4pos(bs.f) || bs.pdf == 0 || bs.wi.z == 0) return false;
+    *out = bs;
+    return true;
+}
+
 }  // namespace pt
 }  // namespace pbrtm

@@ -395,7 +395,7 @@ class SpectrumTable {
 };
 
 struct Globals {
-    PL cie[3], d65, metals[8];
+    PL cie[3], d65, metals[8], glass_bk7;
     Mat3 xyz_from_rgb, rgb_from_xyz;
     float white[2];
     SpectrumTable table;
@@ -418,6 +418,7 @@ struct Globals {
         const int ml[8] = {pbrtm_data::kAgEtaLen, pbrtm_data::kAgKLen, pbrtm_data::kAlEtaLen, pbrtm_data::kAlKLen,
                            pbrtm_data::kAuEtaLen, pbrtm_data::kAuKLen, pbrtm_data::kCuEtaLen, pbrtm_data::kCuKLen};
         for (int i = 0; i < 8; ++i) metals[i] = interleaved(md[i], ml[i]);
+        glass_bk7 = interleaved(pbrtm_data::kGlassBK7Eta, pbrtm_data::kGlassBK7EtaLen);
         float w[3];
         for (int k = 0; k < 3; ++k) w[k] = d65.dot(cie[k]) / pbrtm_data::kCieYIntegral;
         white[0] = w[0] / (w[0] + w[1] + w[2]);
@@ -489,7 +490,7 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
         S.cie_n = uint32_t(G.cie[0].lam.size());
         S.d65_off = pool_add(S, G.d65);
         S.d65_n = uint32_t(G.d65.lam.size());
-        int metal_off[8];
+        int metal_off[8], glass_off = -1;
         for (int i = 0; i < 8; ++i) metal_off[i] = -1;
         // materials (materials/diffuse_material.cu:22-29, conductor_material.cu:10-76)
         S.mats.clear();
@@ -515,7 +516,17 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
                 float u = m.uroughness, v = m.vroughness;
                 if (m.remap_roughness) u = std::sqrt(u), v = std::sqrt(v);
                 dm.ax = u, dm.ay = v;
-                if (!(std::max(u, v) < 1e-3f)) return "rough conductors are not supported yet";
+            } else if (m.type == PBRTM_MATERIAL_DIELECTRIC) {
+                if (m.eta_named > 1) return "unknown named index of refraction";
+                dm.eta_c = m.eta;
+                if (m.eta_named == 1) {
+                    if (glass_off < 0) glass_off = int(pool_add(S, G.glass_bk7));
+                    dm.eta_off = uint32_t(glass_off);
+                    dm.eta_n = uint32_t(G.glass_bk7.lam.size());
+                }
+                float u = m.uroughness, v = m.vroughness;
+                if (m.remap_roughness) u = std::sqrt(u), v = std::sqrt(v);
+                dm.ax = u, dm.ay = v;
             } else {
                 return "unknown material type";
             }

@@ -51,7 +51,7 @@ class BuildStats(ctypes.Structure):
 
 class Material(ctypes.Structure):
     _fields_ = [("type", c_uint32), ("rgb", c_float * 3), ("metal", c_uint32), ("uroughness", c_float),
-                ("vroughness", c_float), ("remap_roughness", c_uint32)]
+                ("vroughness", c_float), ("remap_roughness", c_uint32), ("eta", c_float), ("eta_named", c_uint32)]
 
 
 class AreaLight(ctypes.Structure):
@@ -266,9 +266,14 @@ def scene_desc(sc):
         if m["type"] == "diffuse":
             mats[i].type = 0
             mats[i].rgb[:] = list(m["rgb"])
-        elif m["type"] == "conductor":
-            mats[i].type = 1
-            mats[i].metal = METALS[m["metal"]]
+        elif m["type"] in ("conductor", "dielectric"):
+            mats[i].type = 1 if m["type"] == "conductor" else 2
+            if m["type"] == "conductor":
+                mats[i].metal = METALS[m["metal"]]
+            else:
+                eta = m.get("eta", 1.5)
+                mats[i].eta_named = 1 if eta == "glass-BK7" else 0
+                mats[i].eta = 0.0 if isinstance(eta, str) else float(eta)
             mats[i].uroughness = m.get("uroughness", m.get("roughness", 0.0))
             mats[i].vroughness = m.get("vroughness", m.get("roughness", 0.0))
             mats[i].remap_roughness = int(m.get("remap", True))

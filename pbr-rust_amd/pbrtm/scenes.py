@@ -1,0 +1,196 @@
+"""Synthetic scenes for the path tracer, as scene dicts for pbrtm.Scene (and the
+oracle). Deterministic: every random choice comes from PCG32 streams or a
+64-bit integer hash, never from numpy's generators.
+
+cornell34: the Cornell box of the reference's example scene
+(example/cornell-box-specular.pbrt:1-91) with its sphere and rotated block
+replaced by two axis-aligned boxes: 5 wall quads (10 triangles), a grey diffuse
+short box and a specular silver (metal-Ag, roughness 0) tall box (12 triangles
+each), and the example's emissive quad (rgb L 1, scale 20, 2 triangles).
+
+sphere_grid: n x n UV spheres (tess x tess quads, 2 tess^2 triangles each,
+radius 0.4, pitch 1.0) radially displaced by value noise of amplitude 0.1, on a
+ground plane; each sphere's material is drawn from {diffuse rgb U[0.1, 0.9]^3,
+conductor Cu roughness U[0.05, 0.4], dielectric eta 1.5 roughness 0}; 32
+emissive quads (64 triangles) at y = 5 with L = rgb(1, 1, 1), scale 10. At
+n = 32, tess = 32 it holds 2,097,218 triangles.
+"""
+import numpy as np
+
+GREY, RED, GREEN = (0.73, 0.73, 0.73), (0.65, 0.05, 0.05), (0.12, 0.45, 0.15)
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+class Pcg32:
+    """PCG32 (O'Neill), the reference's RNG (util/rng.h), scalar."""
+
+    MULT = 0x5851F42D4C957F2D
+
+    def __init__(self, seq, offset=None):
+        def mix(v):
+            v ^= v >> 31
+            v = (v * 0x7FB5D329728EA185) & 0xFFFFFFFFFFFFFFFF
+            v ^= v >> 27
+            v = (v * 0x81DADEF4BC2DD44D) & 0xFFFFFFFFFFFFFFFF
+            return v ^ (v >> 33)
+
+        self.state = 0
+        self.inc = ((seq << 1) | 1) & 0xFFFFFFFFFFFFFFFF
+        self.next_u32()
+        self.state = (self.state + (mix(seq) if offset is None else offset)) & 0xFFFFFFFFFFFFFFFF
+        self.next_u32()
+
+    def next_u32(self):
+        old = self.state
+        self.state = (old * self.MULT + self.inc) & 0xFFFFFFFFFFFFFFFF
+        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def uniform(self):
+        return min(float(np.float32(self.next_u32() * 2.0 ** -32)), float.fromhex('0x1.fffffep-1'))
+
+
+def quad(a, b, c, d):
+    """Triangles of indices [0 1 2 2 3 0] over the quad a b c d."""
+    return [a + b + c, c + d + a]
+
+
+def box(x0, x1, y0, y1, z0, z1):
+    p = lambda x, y, z: (x, y, z)
+    faces = [
+        (p(x0, y0, z0), p(x1, y0, z0), p(x1, y0, z1), p(x0, y0, z1)),  # bottom
+        (p(x0, y1, z0), p(x0, y1, z1), p(x1, y1, z1), p(x1, y1, z0)),  # top
+        (p(x0, y0, z0), p(x0, y1, z0), p(x1, y1, z0), p(x1, y0, z0)),  # front (z0)
+        (p(x0, y0, z1), p(x1, y0, z1), p(x1, y1, z1), p(x0, y1, z1)),  # back (z1)
+        (p(x0, y0, z0), p(x0, y0, z1), p(x0, y1, z1), p(x0, y1, z0)),  # left (x0)
+        (p(x1, y0, z0), p(x1, y1, z0), p(x1, y1, z1), p(x1, y0, z1)),  # right (x1)
+    ]
+    out = []
+    for f in faces:
+        out += quad(*f)
+    return out
+
+
+def cornell34(width=256, height=256, spp=4, max_depth=5):
+    tris, mats, lights = [], [], []
+    materials = [  # 0 grey, 1 red, 2 green, 3 silver
+        {"type": "diffuse", "rgb": GREY},
+        {"type": "diffuse", "rgb": RED},
+        {"type": "diffuse", "rgb": GREEN},
+        {"type": "conductor", "metal": "Ag", "roughness": 0.0, "remap": True},
+    ]
+
+    def add(ts, m, light=-1):
+        for t in ts:
+            tris.append(t)
+            mats.append(m)
+            lights.append(light)
+
+    # light (example :18-29), grey material, emitter 0
+    add(quad((343.0, 548.7, 227.0), (343.0, 548.7, 332.0), (213.0, 548.7, 332.0), (213.0, 548.7, 227.0)), 0, 0)
+    add(quad((552.8, 0.0, 0.0), (549.6, 0.0, 559.2), (556.0, 548.8, 559.2), (556.0, 548.8, 0.0)), 1)  # red
+    add(quad((0.0, 0.0, 559.2), (0.0, 0.0, 0.0), (0.0, 548.8, 0.0), (0.0, 548.8, 559.2)), 2)  # green
+    add(quad((552.8, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 559.2), (549.6, 0.0, 559.2)), 0)  # floor
+    add(quad((556.0, 548.8, 0.0), (556.0, 548.8, 559.2), (0.0, 548.8, 559.2), (0.0, 548.8, 0.0)), 0)  # ceiling
+    add(quad((549.6, 0.0, 559.2), (0.0, 0.0, 559.2), (0.0, 548.8, 559.2), (556.0, 548.8, 559.2)), 0)  # back
+    add(box(130.0, 290.0, 0.0, 165.0, 65.0, 225.0), 0)  # short box, grey
+    add(box(265.0, 430.0, 0.0, 330.0, 270.0, 435.0), 3)  # tall box, silver
+    return {
+        "eye": (278.0, 273.0, -800.0), "look": (278.0, 273.0, 0.0), "up": (0.0, 1.0, 0.0), "fov": 37.5,
+        "width": width, "height": height, "spp": spp, "max_depth": max_depth,
+        "tri": np.array(tris, np.float32).reshape(-1, 9),
+        "tri_material": np.array(mats, np.uint32), "tri_light": np.array(lights, np.int32),
+        "materials": materials, "lights": [{"rgb": (1.0, 1.0, 1.0), "scale": 20.0}],
+    }
+
+
+def _hash01(ix, iy, iz, seed):
+    """Lattice hash -> [0, 1): a SplitMix64 finaliser over packed coordinates (uint64 numpy)."""
+    with np.errstate(over="ignore"):
+        k = (ix.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) ^ (iy.astype(np.uint64) * np.uint64(0xC2B2AE3D27D4EB4F)) \
+            ^ (iz.astype(np.uint64) * np.uint64(0x165667B19E3779F9)) ^ np.uint64(seed)
+        k ^= k >> np.uint64(30)
+        k *= np.uint64(0xBF58476D1CE4E5B9)
+        k ^= k >> np.uint64(27)
+        k *= np.uint64(0x94D049BB133111EB)
+        k ^= k >> np.uint64(31)
+    return (k >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def _value_noise(p, seed, freq=3.0):
+    """Trilinear value noise with smoothstep weights, in [0, 1)."""
+    q = p * freq
+    i0 = np.floor(q).astype(np.int64)
+    f = q - i0
+    w = f * f * (3 - 2 * f)
+    out = np.zeros(len(p))
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                h = _hash01(i0[:, 0] + dx + (1 << 20), i0[:, 1] + dy + (1 << 20), i0[:, 2] + dz + (1 << 20), seed)
+                wx = w[:, 0] if dx else 1 - w[:, 0]
+                wy = w[:, 1] if dy else 1 - w[:, 1]
+                wz = w[:, 2] if dz else 1 - w[:, 2]
+                out += wx * wy * wz * h
+    return out
+
+
+def uv_sphere(tess):
+    """Unit-sphere vertices [tess+1][tess+1][3] and the triangle index list (2 tess^2)."""
+    th = np.linspace(0, np.pi, tess + 1)
+    ph = np.linspace(0, 2 * np.pi, tess + 1)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    v = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], axis=-1)
+    idx = []
+    for i in range(tess):
+        for j in range(tess):
+            a, b = i * (tess + 1) + j, i * (tess + 1) + j + 1
+            c, d = (i + 1) * (tess + 1) + j + 1, (i + 1) * (tess + 1) + j
+            idx += [(a, b, c), (c, d, a)]
+    return v.reshape(-1, 3), np.array(idx, np.int64)
+
+
+def sphere_grid(width=1920, height=1080, n=32, tess=32, seed=0xC0FFEE, spp=64, max_depth=8):
+    rng = Pcg32(seed)
+    unit, idx = uv_sphere(tess)
+    materials, tris, tri_mat = [], [], []
+    for gi in range(n):
+        for gj in range(n):
+            c = np.array([(gi - (n - 1) / 2) * 1.0, 0.5, (gj - (n - 1) / 2) * 1.0])
+            kind = min(int(rng.uniform() * 3), 2)
+            if kind == 0:
+                materials.append({"type": "diffuse", "rgb": tuple(0.1 + 0.8 * rng.uniform() for _ in range(3))})
+            elif kind == 1:
+                materials.append({"type": "conductor", "metal": "Cu", "roughness": 0.05 + 0.35 * rng.uniform(),
+                                  "remap": True})
+            else:
+                materials.append({"type": "dielectric", "eta": 1.5, "roughness": 0.0, "remap": True})
+            r = 0.4 + 0.1 * (_value_noise(unit + c, seed + gi * n + gj) - 0.5)
+            verts = c + unit * r[:, None]
+            tris.append(verts[idx].reshape(-1, 9))
+            tri_mat.append(np.full(len(idx), len(materials) - 1, np.uint32))
+    tri = np.concatenate(tris).astype(np.float32)
+    tri_mat = np.concatenate(tri_mat)
+    ground = len(materials)
+    materials.append({"type": "diffuse", "rgb": (0.5, 0.5, 0.5)})
+    g = n * 1.5
+    gt = np.array(quad((-g, 0.0, -g), (-g, 0.0, g), (g, 0.0, g), (g, 0.0, -g)), np.float32)
+    lights_t = []
+    for li in range(8):
+        for lj in range(4):
+            x0, z0 = -12.0 + 3.0 * li, -8.0 + 5.0 * lj
+            # wound so the geometric normal faces down (-y)
+            lights_t += quad((x0, 5.0, z0), (x0 + 1.0, 5.0, z0), (x0 + 1.0, 5.0, z0 + 1.0), (x0, 5.0, z0 + 1.0))
+    lt = np.array(lights_t, np.float32)
+    tri = np.concatenate([tri, gt, lt])
+    tri_mat = np.concatenate([tri_mat, np.full(2 + len(lt), ground, np.uint32)])
+    tri_light = np.full(len(tri), -1, np.int32)
+    tri_light[-len(lt):] = 0
+    return {
+        "eye": (0.0, 3.5, -21.0), "look": (0.0, 0.0, -4.0), "up": (0.0, 1.0, 0.0), "fov": 45.0,
+        "width": width, "height": height, "spp": spp, "max_depth": max_depth,
+        "tri": np.ascontiguousarray(tri), "tri_material": tri_mat, "tri_light": tri_light,
+        "materials": materials, "lights": [{"rgb": (1.0, 1.0, 1.0), "scale": 10.0}],
+    }

@@ -114,3 +114,35 @@ def test_render_errors(ctx):
     bad = dict(sc, materials=[{"type": "diffuse", "rgb": (1.5, 0.2, 0.2)}] + sc["materials"][1:])
     with pytest.raises(pbrtm.PbrtmError):
         ctx.scene(bad)
+
+
+def cornell_materials(variant):
+    """The Cornell scene with the boxes' materials swapped for rough and refractive ones."""
+    sc = scenes.cornell34(48, 48)
+    mats = list(sc["materials"])
+    if variant == "rough":
+        mats[3] = {"type": "conductor", "metal": "Cu", "roughness": 0.2, "remap": True}
+        mats[0] = {"type": "dielectric", "eta": 1.5, "roughness": 0.1, "remap": True}
+    elif variant == "glass":
+        mats[3] = {"type": "dielectric", "eta": "glass-BK7", "roughness": 0.0}
+        mats[0] = {"type": "conductor", "metal": "Au", "uroughness": 0.05, "vroughness": 0.3, "remap": False}
+    elif variant == "smooth_glass":
+        mats[3] = {"type": "dielectric", "eta": 1.5, "roughness": 0.0}
+    sc["materials"] = mats
+    return sc
+
+
+@pytest.mark.parametrize("variant", ["rough", "glass", "smooth_glass"])
+def test_rough_and_dielectric_materials(ctx, variant):
+    sc = cornell_materials(variant)
+    rgb, aux, orgb, oaux = render_both(ctx, sc, 4, 6)
+    compare(aux, oaux, f"aux {variant}")
+    compare(rgb, orgb, f"film {variant}")
+
+
+def test_sphere_grid_small(ctx):
+    """The 2M-triangle displaced-sphere grid (diffuse, rough copper, glass) at 64x36, 4 spp, depth 8."""
+    sc = scenes.sphere_grid(width=64, height=36, spp=4)
+    rgb, aux, orgb, oaux = render_both(ctx, sc, 4, 8)
+    compare(aux, oaux, "aux sphere grid")
+    compare(rgb, orgb, "film sphere grid")
