@@ -9,6 +9,13 @@ Prints one JSON line: the headline metric is the HLBVH build time at 10M
 triangles (lower is better); traversal throughput, per-phase times, the
 roofline of the dominant kernel and a CPU baseline (the oracle, a host-only
 restatement of the reference's build, timed on a bounded sample) ride along.
+
+The "pathtracer" object measures the wavefront path tracer on the
+displaced-sphere grid (2,097,218 triangles; diffuse, rough copper and glass;
+64 emissive triangles) at 1920x1080, 64 spp stratified, max depth 8:
+Msamples/s = camera samples / render time, the scene and its HLBVH resident
+in HBM before the timed renders, with the oracle's render of the same
+workload at 128x72 as its CPU baseline.
 """
 import argparse
 import json
@@ -41,6 +48,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=0x5EED)
     ap.add_argument("--cpu-sample-tris", type=int, default=2_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pt", action="store_true", help="skip the path-tracer measurement")
+    ap.add_argument("--pt-steps", type=int, default=2)
+    ap.add_argument("--pt-spp", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (for scaling profiler byte counters)")
     return ap.parse_args()
@@ -67,6 +77,56 @@ def cpu_baseline(args, pbrtm):
         "sample": f"oracle (host C++ restatement of hlbvh.cu) build of {n} triangles of the same generator, "
                   f"{ms:.0f} ms, scaled x{args.tris / n:g} to {args.tris}",
     }
+
+
+def pathtracer(args, ctx, pbrtm):
+    """Msamples/s of the wavefront path tracer at 1920x1080, 64 spp, depth 8 (sphere grid)."""
+    import numpy as np
+    from pbrtm import scenes
+
+    W, H, spp, depth = 1920, 1080, args.pt_spp, 8
+    sc = scenes.sphere_grid(width=W, height=H, spp=spp, max_depth=depth)
+    scene = ctx.scene(sc)
+    try:
+        scene.render(spp=spp, max_depth=depth)  # warm-up: allocates the path pool and records
+        runs = []
+        t0 = time.perf_counter()
+        for _ in range(args.pt_steps):
+            _, _, st = scene.render(spp=spp, max_depth=depth)
+            runs.append(st)
+        wall = (time.perf_counter() - t0) / args.pt_steps
+    finally:
+        scene.free()
+    ms = float(np.mean([r["ms_total"] for r in runs]))
+    last = runs[-1]
+    out = {
+        "metric": "Msamples/s at 1080p 64spp",
+        "value": round(W * H * spp / (ms * 1e3), 2),
+        "unit": "Msamples/s",
+        "higher_is_better": True,
+        "ms_per_image": round(ms, 2),
+        "wall_ms_per_image": round(wall * 1e3, 2),
+        "stages_ms": {k: round(float(np.mean([r[k] for r in runs])), 2)
+                      for k in ("ms_trace", "ms_shadow", "ms_shade", "ms_film")},
+        "rounds": last["rounds"], "rays": last["rays"], "shadow_rays": last["shadow_rays"],
+        "mrays_per_s": round((last["rays"] + last["shadow_rays"]) / (ms * 1e3), 1),
+        "config": {"workload": "sphere grid 32x32 UV spheres, 2,097,218 tris, 64 emitters", "width": W,
+                   "height": H, "spp": spp, "max_depth": depth, "seed": 0xC0FFEE, "dtype": "f32"},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        cw, ch = 128, 72
+        small = scenes.sphere_grid(width=cw, height=ch, spp=spp, max_depth=depth)
+        _, _, ost = oracle.render(small, threads=16)
+        out["cpu_baseline"] = {
+            "value": round(cw * ch * spp / (ost["render_ms"] * 1e3), 4),
+            "unit": "Msamples/s", "cores": 16, "kind": "port",
+            "sample": f"oracle wavefront render of the same scene at {cw}x{ch}, {spp} spp, depth {depth} "
+                      f"({cw * ch * spp} samples, {ost['render_ms']:.0f} ms, set-up and BVH build excluded)",
+        }
+    return out
 
 
 def main():
@@ -201,6 +261,10 @@ def main():
         del src, dst
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args, pbrtm)
+    if not args.no_pt:
+        del d_tri, d_rays, d_prim, d_t, d_bary, d_bounce, d_prim2, d_t2
+        torch.cuda.empty_cache()
+        line["pathtracer"] = pathtracer(args, ctx, pbrtm)
     ctx.close()
     print(json.dumps(line), flush=True)
 

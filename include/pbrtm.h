@@ -221,6 +221,13 @@ int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *scene, const pbrtm_render_params *
 int pbrtm_rgb_to_sigmoid(const float rgb[3], float out3[3]);
 int pbrtm_srgb_matrices(float xyz_from_rgb9[9], float rgb_from_xyz9[9], float white_xy2[2]);
 
+/* Image output. Replaces Film::write_to_png (base/film.cu:142-181): each linear
+ * output-space RGB value goes through SRGBColorEncoding::from_linear
+ * (spectrum_util/color_encoding.h:24-60) to 8 bits, alpha 255, and the image is
+ * written as an RGBA8 PNG. pbrtm_srgb8 returns the 8-bit pixels ([n][4]). */
+int pbrtm_srgb8(const float *rgb, uint64_t n, uint8_t *rgba_out);
+int pbrtm_write_png(const float *rgb, uint32_t width, uint32_t height, const char *path);
+
 #ifdef __cplusplus
 }
 #endif

@@ -178,6 +178,7 @@ def _pt_sigs(lib):
         "orc_spherical_triangle": (c_int, [p, p, p, p, POINTER(c_float)]),
         "orc_invert_spherical_triangle": (c_int, [p, p, p, p]),
         "orc_fr_complex": (c_float, [c_float, c_float, c_float]),
+        "orc_srgb8": (c_int, [p, c_uint64, p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -237,11 +238,12 @@ def render(sc, pool_size=2 * 1024 * 1024, threads=8):
     h, w = sc["height"], sc["width"]
     rgb = np.zeros((h, w, 3), np.float32)
     aux = np.zeros((h, w, 4), np.float32)
-    st = np.zeros(4, np.uint64)
+    st = np.zeros(5, np.uint64)
     rc = lib.orc_render(ctypes.byref(s), threads, _p(rgb), _p(aux), _p(st))
     if rc != 0:
         raise RuntimeError(f"orc_render failed ({rc})")
-    return rgb, aux, {"rounds": int(st[0]), "rays": int(st[1]), "shadow_rays": int(st[2]), "paths": int(st[3])}
+    return rgb, aux, {"rounds": int(st[0]), "rays": int(st[1]), "shadow_rays": int(st[2]), "paths": int(st[3]),
+                      "render_ms": int(st[4]) / 1e3}
 
 
 def rgb_to_sigmoid(rgb):
@@ -334,3 +336,10 @@ def invert_spherical_triangle(v, p, w):
 
 def fr_complex(cos_i, eta, k):
     return float(_pt().orc_fr_complex(float(cos_i), float(eta), float(k)))
+
+
+def srgb8(rgb):
+    a = np.ascontiguousarray(rgb, np.float32)
+    out = np.zeros(a.shape[:-1] + (4,), np.uint8)
+    _pt().orc_srgb8(_p(a), a.size // 3, _p(out))
+    return out

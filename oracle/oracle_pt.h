@@ -51,8 +51,9 @@ typedef struct orc_scene {
 } orc_scene;
 
 /* film_rgb: [height][width][3] = RGBFilm::get_pixel_rgb; aux (nullable):
- * [height][width][4] = rgb_sum, weight_sum; stats (nullable): rounds, rays,
- * shadow rays, finished paths. */
+ * [height][width][4] = rgb_sum, weight_sum; stats (nullable, 5 entries): rounds,
+ * rays, shadow rays, finished paths, microseconds spent rendering (scene set-up
+ * and the BVH build excluded). */
 int orc_render(const orc_scene *scene, int threads, float *film_rgb, float *aux, uint64_t *stats);
 
 /* Intermediate tables, for tests: the RGB-to-spectrum sigmoid coefficients of an
@@ -70,6 +71,8 @@ int orc_camera_ray(const orc_scene *scene, float px, float py, float out6[6]);
 int orc_spherical_triangle(const float v9[9], const float p[3], const float u[2], float b3[3], float *pdf);
 int orc_invert_spherical_triangle(const float v9[9], const float p[3], const float w3[3], float u2[2]);
 float orc_fr_complex(float cos_i, float eta, float k);
+/* Film::write_to_png's per-channel encoding: linear RGB [n][3] -> sRGB RGBA8 [n][4]. */
+int orc_srgb8(const float *rgb, uint64_t n, uint8_t *rgba);
 
 #ifdef __cplusplus
 }

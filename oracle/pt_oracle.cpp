@@ -9,6 +9,7 @@
 // correctly rounded stand-in for the CUDA device library the reference calls.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -1344,6 +1345,7 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
         });
     };
 
+    const auto t_start = std::chrono::steady_clock::now();
     std::vector<uint32_t> all(P);
     for (uint32_t i = 0; i < P; ++i) all[i] = i;
     gen(all);
@@ -1525,6 +1527,8 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
         stats[1] = n_rays;
         stats[2] = n_shadow;
         stats[3] = n_done;
+        stats[4] = uint64_t(std::chrono::duration_cast<std::chrono::microseconds>(
+                                std::chrono::steady_clock::now() - t_start).count());
     }
 }
 
@@ -1635,5 +1639,32 @@ int orc_invert_spherical_triangle(const float v9[9], const float p[3], const flo
 }
 
 float orc_fr_complex(float cos_i, float eta, float k) { return orc::fr_complex(cos_i, orc::Cx{eta, k}); }
+
+// SRGBColorEncoding::from_linear (spectrum_util/color_encoding.h:7-60).
+int orc_srgb8(const float *rgb, uint64_t n, uint8_t *rgba) {
+    auto to_srgb = [](float v) -> float {
+        if (v <= 0.0031308f) return 12.92f * v;
+        float sq = std::sqrt(std::max(0.0f, v));
+        float p = std::fma(sq, std::fma(sq, std::fma(sq, std::fma(sq, std::fma(sq, -0.016202083165206348f,
+                  0.7551545191665577f), 2.0041169284241644f), 0.7642611304733891f), 0.03453868659826638f),
+                  -0.0016829072605308378f);
+        float q = std::fma(sq, std::fma(sq, std::fma(sq, std::fma(sq, std::fma(sq, 1.f, 1.8970238036421054f),
+                  0.6085338522168684f), 0.03467195408529984f), -0.00004375359692957097f), 4.178892964897981e-7f);
+        return p / q * v;
+    };
+    for (uint64_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            float v = rgb[3 * i + c];
+            uint8_t o;
+            if (v <= 0) o = 0;
+            else if (v >= 1) o = 255;
+            else if (std::isnan(v)) o = 0;
+            else o = uint8_t(std::min(255.0, std::max(0.0, std::round(255.0 * double(to_srgb(v))))));
+            rgba[4 * i + c] = o;
+        }
+        rgba[4 * i + 3] = 255;
+    }
+    return 0;
+}
 
 }  // extern "C"

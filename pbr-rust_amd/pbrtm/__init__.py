@@ -33,7 +33,7 @@ EXPORTS = (
     "pbrtm_morton_sort", "pbrtm_bvh_build", "pbrtm_bvh_free", "pbrtm_bvh_bounds",
     "pbrtm_bvh_export_canonical", "pbrtm_bvh_intersect", "pbrtm_ao_bounce", "pbrtm_gen_soup",
     "pbrtm_gen_camera_rays", "pbrtm_scene_create", "pbrtm_scene_free", "pbrtm_render", "pbrtm_rgb_to_sigmoid",
-    "pbrtm_srgb_matrices",
+    "pbrtm_srgb_matrices", "pbrtm_srgb8", "pbrtm_write_png",
 )
 
 
@@ -120,6 +120,8 @@ def load(path: str = LIB_PATH):
         "pbrtm_render": (c_int, [p, p, POINTER(RenderParams), p, p, POINTER(RenderStats)]),
         "pbrtm_rgb_to_sigmoid": (c_int, [p, p]),
         "pbrtm_srgb_matrices": (c_int, [p, p, p]),
+        "pbrtm_srgb8": (c_int, [p, c_uint64, p]),
+        "pbrtm_write_png": (c_int, [p, c_uint32, c_uint32, c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -243,6 +245,28 @@ def rgb_to_sigmoid(rgb) -> np.ndarray:
     if rc != OK:
         raise PbrtmError(rc, "rgb_to_sigmoid")
     return out
+
+
+def srgb8(rgb) -> np.ndarray:
+    """Linear RGB [..][3] -> sRGB RGBA8 [..][4] (the film's PNG encoding; host only)."""
+    lib = load()
+    a = np.ascontiguousarray(rgb, np.float32)
+    n = a.size // 3
+    out = np.zeros(a.shape[:-1] + (4,), np.uint8)
+    rc = lib.pbrtm_srgb8(_ptr(a), n, _ptr(out))
+    if rc != OK:
+        raise PbrtmError(rc, "srgb8")
+    return out
+
+
+def write_png(rgb, path):
+    """Write a linear RGB film [h][w][3] as an sRGB RGBA8 PNG (host only)."""
+    lib = load()
+    a = np.ascontiguousarray(rgb, np.float32)
+    h, w = a.shape[:2]
+    rc = lib.pbrtm_write_png(_ptr(a), w, h, os.fsencode(path))
+    if rc != OK:
+        raise PbrtmError(rc, f"write_png {path}")
 
 
 def srgb_matrices():

@@ -239,3 +239,26 @@ def test_no_emitters_is_black():
     sc["lights"] = []
     img, _, _ = O.render(sc, threads=8)
     assert np.all(img == 0)
+
+
+def test_srgb8_encoding_and_png(tmp_path):
+    """Film::write_to_png's 8-bit encoding: the oracle against the exact sRGB curve
+    (the reference's rational fit may differ by one code at rounding boundaries),
+    the library against the oracle bit for bit, and the PNG decodes to those bytes."""
+    pbrtm = pytest.importorskip("pbrtm")
+    rng = np.random.default_rng(3)
+    v = np.concatenate([np.linspace(-0.1, 1.1, 4001), rng.random(20000) ** 3, [np.nan, np.inf, -np.inf, 0.0031308]])
+    rgb = np.stack([v, v[::-1], np.roll(v, 7)], axis=1).astype(np.float32)
+    ours = O.srgb8(rgb)
+    x = np.clip(np.nan_to_num(rgb.astype(np.float64), nan=0.0, posinf=1.0, neginf=0.0), 0, 1)
+    exact = np.where(x <= 0.0031308, 12.92 * x, 1.055 * np.power(x, 1 / 2.4) - 0.055)
+    assert np.abs(ours[:, :3].astype(int) - np.round(255 * exact).astype(int)).max() <= 1
+    assert np.all(ours[:, 3] == 255)
+    lib = pbrtm.srgb8(rgb)
+    assert np.array_equal(lib, ours)
+    img = rgb[:24000].reshape(120, 200, 3)
+    path = tmp_path / "film.png"
+    pbrtm.write_png(img, str(path))
+    from PIL import Image
+    dec = np.asarray(Image.open(path).convert("RGBA"))
+    assert np.array_equal(dec, O.srgb8(img))
