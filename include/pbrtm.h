@@ -138,8 +138,8 @@ int pbrtm_gen_camera_rays(uint32_t width, uint32_t height, const float eye[3], c
 
 /* ------------------------------------------------------------------------
  * Wavefront path tracer. A scene is the flat result of the reference's scene
- * front end (scene/scene_builder.cu): world-space triangles with a material
- * and an optional diffuse area light per triangle, a LookAt + perspective
+ * front end (scene/scene_builder.cu): object-space triangles with their
+ * object-to-world transform, a material and an optional diffuse area light per triangle, a LookAt + perspective
  * camera and the film resolution. Render space is camera-world
  * (base/camera.h:26-45); the Mitchell filter (radius 2, B = C = 1/3), the
  * stratified sampler and the sRGB RGB film are the reference's defaults
@@ -184,6 +184,9 @@ typedef struct pbrtm_scene_desc {
     const pbrtm_material *materials;
     uint32_t num_lights;
     const pbrtm_area_light *lights;
+    uint32_t num_xforms;          /* object-to-world matrices (the CTM at each Shape) */
+    const float *xforms;          /* [num_xforms][4][4] row-major; may be NULL */
+    const uint32_t *tri_xform;    /* [n] index into xforms, UINT32_MAX = identity; NULL = all identity */
 } pbrtm_scene_desc;
 
 typedef struct pbrtm_render_params {
@@ -220,6 +223,19 @@ int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *scene, const pbrtm_render_params *
  * and the sRGB colour space matrices and white point (rgb_color_space.h:12-44). */
 int pbrtm_rgb_to_sigmoid(const float rgb[3], float out3[3]);
 int pbrtm_srgb_matrices(float xyz_from_rgb9[9], float rgb_from_xyz9[9], float white_xy2[2]);
+
+/* The .pbrt front end (host only). Replaces SceneBuilder's parse
+ * (scene/scene_builder.cu:316-720) for the directives the path tracer renders;
+ * anything else that would change the image is an error (message in err).
+ * The handle owns the description's arrays; pbrtm_pbrt_render_params gives the
+ * file's samples per pixel (rounded to a square, scene_builder.cu:283-287) and
+ * maxdepth, pbrtm_pbrt_output the Film filename. */
+typedef struct pbrtm_pbrt_file pbrtm_pbrt_file;
+int pbrtm_pbrt_parse(const char *path, pbrtm_pbrt_file **out, char *err, uint64_t errlen);
+const pbrtm_scene_desc *pbrtm_pbrt_desc(const pbrtm_pbrt_file *file);
+int pbrtm_pbrt_render_params(const pbrtm_pbrt_file *file, pbrtm_render_params *out);
+const char *pbrtm_pbrt_output(const pbrtm_pbrt_file *file);
+void pbrtm_pbrt_free(pbrtm_pbrt_file *file);
 
 /* Image output. Replaces Film::write_to_png (base/film.cu:142-181): each linear
  * output-space RGB value goes through SRGBColorEncoding::from_linear

@@ -157,7 +157,8 @@ class OrcScene(ctypes.Structure):
                 ("width", c_uint32), ("height", c_uint32), ("spp", c_uint32), ("max_depth", c_uint32),
                 ("pool_size", c_uint32), ("num_tris", c_uint64), ("tri", c_void_p), ("tri_material", c_void_p),
                 ("tri_light", c_void_p), ("tri_reverse", c_void_p), ("num_materials", c_uint32),
-                ("materials", POINTER(OrcMaterial)), ("num_lights", c_uint32), ("lights", POINTER(OrcLight))]
+                ("materials", POINTER(OrcMaterial)), ("num_lights", c_uint32), ("lights", POINTER(OrcLight)),
+                ("num_xforms", c_uint32), ("xforms", c_void_p), ("tri_xform", c_void_p)]
 
 
 METALS = {"Ag": 0, "Al": 1, "Au": 2, "Cu": 3}
@@ -225,7 +226,13 @@ def _scene_struct(sc, pool_size):
     s.materials = mats
     s.num_lights = len(sc["lights"])
     s.lights = lights
-    return s, (tri, mat, lig, rev, mats, lights)
+    xf = np.ascontiguousarray(sc.get("xforms", np.zeros((0, 4, 4))), dtype=np.float32).reshape(-1, 16)
+    txf = sc.get("tri_xform")
+    txf = None if txf is None else np.ascontiguousarray(txf, dtype=np.uint32)
+    s.num_xforms = len(xf)
+    s.xforms = _p(xf) if len(xf) else None
+    s.tri_xform = _p(txf)
+    return s, (tri, mat, lig, rev, mats, lights, xf, txf)
 
 
 def render(sc, pool_size=2 * 1024 * 1024, threads=8):

@@ -40,7 +40,7 @@ typedef struct orc_scene {
     uint32_t max_depth;           /* Integrator "path" "integer maxdepth" */
     uint32_t pool_size;           /* path slots in flight */
     uint64_t num_tris;
-    const float *tri;             /* [n][3][3] world space */
+    const float *tri;             /* [n][3][3] object space (world space without xforms) */
     const uint32_t *tri_material; /* [n] */
     const int32_t *tri_light;     /* [n] index into lights or -1; nullable */
     const uint8_t *tri_reverse;   /* [n] ReverseOrientation; nullable */
@@ -48,6 +48,9 @@ typedef struct orc_scene {
     const orc_material *materials;
     uint32_t num_lights;
     const orc_light *lights;
+    uint32_t num_xforms;          /* object-to-world 4x4 row-major matrices */
+    const float *xforms;
+    const uint32_t *tri_xform;    /* [n] index or UINT32_MAX = identity; nullable */
 } orc_scene;
 
 /* film_rgb: [height][width][3] = RGBFilm::get_pixel_rgb; aux (nullable):

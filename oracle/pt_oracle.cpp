@@ -532,12 +532,22 @@ void build_scene(Scene &sc, const orc_scene &d) {
     }
     // Geometry in render space (triangle_mesh.cu:39-57: render_from_object applied per point).
     const M4 &rw = sc.cam.render_from_world.m;
+    // render_from_object = render_from_world * CTM (scene_builder.cu get_render_from_object)
+    std::vector<M4> rfo(d.num_xforms);
+    for (uint32_t k = 0; k < d.num_xforms; ++k) {
+        M4 x;
+        std::memcpy(x.m, d.xforms + 16 * size_t(k), sizeof(x.m));
+        rfo[k] = m4_mul(rw, x);
+    }
     sc.tris.resize(d.num_tris);
     sc.soup.resize(d.num_tris * 9);
     for (uint64_t i = 0; i < d.num_tris; ++i) {
         Tri &t = sc.tris[i];
+        const uint32_t xk = d.tri_xform ? d.tri_xform[i] : 0xFFFFFFFFu;
+        if (xk != 0xFFFFFFFFu && xk >= d.num_xforms) throw std::runtime_error("transform index out of range");
+        const M4 &m = xk == 0xFFFFFFFFu ? rw : rfo[xk];
         for (int v = 0; v < 3; ++v) {
-            t.p[v] = xf_point(rw, {d.tri[9 * i + 3 * v], d.tri[9 * i + 3 * v + 1], d.tri[9 * i + 3 * v + 2]});
+            t.p[v] = xf_point(m, {d.tri[9 * i + 3 * v], d.tri[9 * i + 3 * v + 1], d.tri[9 * i + 3 * v + 2]});
             sc.soup[9 * i + 3 * v] = t.p[v].x;
             sc.soup[9 * i + 3 * v + 1] = t.p[v].y;
             sc.soup[9 * i + 3 * v + 2] = t.p[v].z;

@@ -573,13 +573,23 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
         // geometry in render space and per-triangle lights (triangle_mesh.cu:39-57,
         // scene_builder.cu:620-630 one DiffuseAreaLight per emissive triangle)
         const uint64_t n = d.num_tris;
+        // render_from_object = render_from_world * CTM (scene_builder.cu get_render_from_object)
+        if (d.num_xforms > 0 && !d.xforms) return "transforms missing";
+        std::vector<Mat4> rfo(d.num_xforms);
+        for (uint32_t k = 0; k < d.num_xforms; ++k) {
+            Mat4 x;
+            std::memcpy(x.m, d.xforms + 16 * size_t(k), sizeof(x.m));
+            rfo[k] = mat4_mul(render_from_world.m, x);
+        }
         S.tri.resize(n * 9);
         S.tri_info.resize(n);
         S.lights.clear();
         for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t xk = d.tri_xform ? d.tri_xform[i] : 0xFFFFFFFFu;
+            if (xk != 0xFFFFFFFFu && xk >= d.num_xforms) return "triangle transform index out of range";
+            const Mat4 &rm = xk == 0xFFFFFFFFu ? render_from_world.m : rfo[xk];
             for (int v = 0; v < 3; ++v) {
-                const V p = xpoint(render_from_world.m, {d.tri[9 * i + 3 * v], d.tri[9 * i + 3 * v + 1],
-                                                         d.tri[9 * i + 3 * v + 2]});
+                const V p = xpoint(rm, {d.tri[9 * i + 3 * v], d.tri[9 * i + 3 * v + 1], d.tri[9 * i + 3 * v + 2]});
                 S.tri[9 * i + 3 * v] = p.x, S.tri[9 * i + 3 * v + 1] = p.y, S.tri[9 * i + 3 * v + 2] = p.z;
             }
             if (d.tri_material[i] >= d.num_materials) return "triangle material index out of range";

@@ -33,7 +33,8 @@ EXPORTS = (
     "pbrtm_morton_sort", "pbrtm_bvh_build", "pbrtm_bvh_free", "pbrtm_bvh_bounds",
     "pbrtm_bvh_export_canonical", "pbrtm_bvh_intersect", "pbrtm_ao_bounce", "pbrtm_gen_soup",
     "pbrtm_gen_camera_rays", "pbrtm_scene_create", "pbrtm_scene_free", "pbrtm_render", "pbrtm_rgb_to_sigmoid",
-    "pbrtm_srgb_matrices", "pbrtm_srgb8", "pbrtm_write_png",
+    "pbrtm_srgb_matrices", "pbrtm_srgb8", "pbrtm_write_png", "pbrtm_pbrt_parse", "pbrtm_pbrt_desc",
+    "pbrtm_pbrt_render_params", "pbrtm_pbrt_output", "pbrtm_pbrt_free",
 )
 
 
@@ -63,7 +64,8 @@ class SceneDesc(ctypes.Structure):
                 ("width", c_uint32), ("height", c_uint32), ("num_tris", c_uint64), ("tri", c_void_p),
                 ("tri_material", c_void_p), ("tri_light", c_void_p), ("tri_reverse", c_void_p),
                 ("num_materials", c_uint32), ("materials", POINTER(Material)), ("num_lights", c_uint32),
-                ("lights", POINTER(AreaLight))]
+                ("lights", POINTER(AreaLight)), ("num_xforms", c_uint32), ("xforms", c_void_p),
+                ("tri_xform", c_void_p)]
 
 
 class RenderParams(ctypes.Structure):
@@ -122,6 +124,11 @@ def load(path: str = LIB_PATH):
         "pbrtm_srgb_matrices": (c_int, [p, p, p]),
         "pbrtm_srgb8": (c_int, [p, c_uint64, p]),
         "pbrtm_write_png": (c_int, [p, c_uint32, c_uint32, c_char_p]),
+        "pbrtm_pbrt_parse": (c_int, [c_char_p, POINTER(p), c_char_p, c_uint64]),
+        "pbrtm_pbrt_desc": (POINTER(SceneDesc), [p]),
+        "pbrtm_pbrt_render_params": (c_int, [p, POINTER(RenderParams)]),
+        "pbrtm_pbrt_output": (c_char_p, [p]),
+        "pbrtm_pbrt_free": (None, [p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -218,6 +225,10 @@ class Context:
 
     def scene(self, sc) -> "Scene":
         return Scene(self, sc)
+
+    def scene_from_file(self, f: "PbrtFile") -> "Scene":
+        """A scene straight from a parsed .pbrt file's description (no Python copy)."""
+        return Scene(self, f)
 
     def build(self, verts, idx=None, device_ptrs=False) -> "Bvh":
         stats = BuildStats()
@@ -318,7 +329,79 @@ def scene_desc(sc):
     d.materials = mats
     d.num_lights = len(sc["lights"])
     d.lights = lights
-    return d, (tri, mat, lig, rev, mats, lights)
+    xf = np.ascontiguousarray(sc.get("xforms", np.zeros((0, 4, 4))), dtype=np.float32).reshape(-1, 16)
+    txf = sc.get("tri_xform")
+    txf = None if txf is None else np.ascontiguousarray(txf, dtype=np.uint32)
+    d.num_xforms = len(xf)
+    d.xforms = _ptr(xf) if len(xf) else None
+    d.tri_xform = _ptr(txf)
+    return d, (tri, mat, lig, rev, mats, lights, xf, txf)
+
+
+class PbrtFile:
+    """A parsed .pbrt file (pbrtm_pbrt_parse): its scene description, render
+    parameters and output name. to_dict() gives the same dict form Scene takes."""
+
+    def __init__(self, path):
+        self.lib = load()
+        h = c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = self.lib.pbrtm_pbrt_parse(os.fsencode(path), ctypes.byref(h), err, 512)
+        if rc != OK:
+            raise PbrtmError(rc, err.value.decode())
+        self.h = h
+        p = RenderParams()
+        self.lib.pbrtm_pbrt_render_params(h, ctypes.byref(p))
+        self.spp, self.max_depth = p.spp, p.max_depth
+        self.output = self.lib.pbrtm_pbrt_output(h).decode()
+
+    def free(self):
+        if self.h:
+            self.lib.pbrtm_pbrt_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def to_dict(self):
+        d = self.lib.pbrtm_pbrt_desc(self.h).contents
+        n = d.num_tris
+
+        def arr(ptr, ctype, count, dtype):
+            if not ptr or count == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, POINTER(ctype)), shape=(count,)).astype(dtype).copy()
+
+        mats = []
+        names = {v: k for k, v in METALS.items()}
+        for i in range(d.num_materials):
+            m = d.materials[i]
+            if m.type == 0:
+                mats.append({"type": "diffuse", "rgb": tuple(m.rgb)})
+            elif m.type == 1:
+                mats.append({"type": "conductor", "metal": names[m.metal], "uroughness": m.uroughness,
+                             "vroughness": m.vroughness, "remap": bool(m.remap_roughness)})
+            else:
+                mats.append({"type": "dielectric", "eta": "glass-BK7" if m.eta_named else m.eta,
+                             "uroughness": m.uroughness, "vroughness": m.vroughness, "remap": bool(m.remap_roughness)})
+        lights = [{"rgb": tuple(d.lights[i].rgb), "scale": d.lights[i].scale, "two_sided": bool(d.lights[i].two_sided)}
+                  for i in range(d.num_lights)]
+        out = {
+            "eye": tuple(d.eye), "look": tuple(d.look), "up": tuple(d.up), "fov": d.fov,
+            "width": d.width, "height": d.height, "spp": self.spp, "max_depth": self.max_depth,
+            "tri": arr(d.tri, c_float, 9 * n, np.float32).reshape(-1, 9),
+            "tri_material": arr(d.tri_material, c_uint32, n, np.uint32),
+            "tri_light": arr(d.tri_light, ctypes.c_int32, n, np.int32),
+            "tri_reverse": arr(d.tri_reverse, ctypes.c_uint8, n, np.uint8),
+            "materials": mats, "lights": lights,
+        }
+        if d.num_xforms:
+            out["xforms"] = arr(d.xforms, c_float, 16 * d.num_xforms, np.float32).reshape(-1, 4, 4)
+            out["tri_xform"] = arr(d.tri_xform, c_uint32, n, np.uint32)
+        return out
 
 
 class Scene:
@@ -326,10 +409,15 @@ class Scene:
 
     def __init__(self, ctx: "Context", sc):
         self.ctx = ctx
-        self.width, self.height = sc["width"], sc["height"]
-        d, keep = scene_desc(sc)
         h = c_void_p()
-        ctx.check(ctx.lib.pbrtm_scene_create(ctx.h, ctypes.byref(d), ctypes.byref(h)))
+        if isinstance(sc, PbrtFile):
+            dp = ctx.lib.pbrtm_pbrt_desc(sc.h)
+            self.width, self.height = dp.contents.width, dp.contents.height
+            ctx.check(ctx.lib.pbrtm_scene_create(ctx.h, dp, ctypes.byref(h)))
+        else:
+            self.width, self.height = sc["width"], sc["height"]
+            d, keep = scene_desc(sc)
+            ctx.check(ctx.lib.pbrtm_scene_create(ctx.h, ctypes.byref(d), ctypes.byref(h)))
         self.h = h
         ctx._handles.add(self)
 

@@ -146,3 +146,27 @@ def test_sphere_grid_small(ctx):
     rgb, aux, orgb, oaux = render_both(ctx, sc, 4, 8)
     compare(aux, oaux, "aux sphere grid")
     compare(rgb, orgb, "film sphere grid")
+
+
+def test_render_from_pbrt_file(ctx, tmp_path):
+    """The fixture file through the .pbrt front end, then the same with a rotated and
+    translated copy of the tall box (object-to-world transforms at Shape time)."""
+    import os
+    fixture = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fixtures", "cornell34.pbrt")
+    text = open(fixture).read().replace("256", "64")
+    extra = ('AttributeBegin\nTranslate 40 0 -30\nRotate 17 0 1 0\nScale 0.5 0.7 0.5\nMaterial "dielectric" '
+             '"float eta" 1.33\nShape "trianglemesh" "integer indices" [0 1 2 2 3 0] "point3 P" '
+             '[100 10 100  200 10 100  200 120 150  100 120 150]\nAttributeEnd\n')
+    for label, body in (("fixture", text), ("transformed", text + extra)):
+        p = tmp_path / f"{label}.pbrt"
+        p.write_text(body)
+        f = pbrtm.PbrtFile(str(p))
+        s = ctx.scene_from_file(f)
+        try:
+            rgb, aux, st = s.render(spp=f.spp, max_depth=f.max_depth)
+        finally:
+            s.free()
+        d = f.to_dict()
+        orgb, oaux, ost = O.render(d, threads=16)
+        assert st["rays"] == ost["rays"]
+        compare(rgb, orgb, f"film from file ({label})")
