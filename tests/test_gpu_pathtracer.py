@@ -170,3 +170,22 @@ def test_render_from_pbrt_file(ctx, tmp_path):
         orgb, oaux, ost = O.render(d, threads=16)
         assert st["rays"] == ost["rays"]
         compare(rgb, orgb, f"film from file ({label})")
+
+
+def test_cli_renders_fixture_png(tmp_path):
+    """pbrtm-render (the reference's main.cu over the C ABI) on the fixture: the PNG
+    decodes to the sRGB encoding of the oracle's film."""
+    import os
+    import subprocess
+    from PIL import Image
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "pbr-rust_amd", "pbrtm-render")
+    fixture = os.path.join(root, "tests", "fixtures", "cornell34.pbrt")
+    out = tmp_path / "c.png"
+    r = subprocess.run([exe, fixture, "--spp", "4", "--outfile", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    log(r.stdout.strip())
+    assert r.returncode == 0, r.stderr
+    img = np.asarray(Image.open(out).convert("RGBA"))
+    orgb, _, _ = O.render(scenes.cornell34(256, 256), threads=16)
+    assert np.array_equal(img, O.srgb8(orgb))
