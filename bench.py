@@ -51,6 +51,10 @@ def parse():
     ap.add_argument("--no-pt", action="store_true", help="skip the path-tracer measurement")
     ap.add_argument("--pt-steps", type=int, default=2)
     ap.add_argument("--pt-spp", type=int, default=64)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-render measurement")
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--batch-pools", type=str, default="1048576,2097152,4194304,8388608",
+                    help="shared pool sizes the batch is tuned over")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (for scaling profiler byte counters)")
     return ap.parse_args()
@@ -127,6 +131,66 @@ def pathtracer(args, ctx, pbrtm):
                       f"({cw * ch * spp} samples, {ost['render_ms']:.0f} ms, set-up and BVH build excluded)",
         }
     return out
+
+
+def batch(args, ctx, pbrtm):
+    """64 mixed jobs: one by one (pbrtm_render each) against one batched launch
+    (pbrtm_render_batch), after checking that every batched film equals its
+    one-by-one film bit for bit. The shared pool size is tuned over --batch-pools."""
+    import numpy as np
+    from pbrtm import scenes
+
+    t0 = time.perf_counter()
+    spec = scenes.batch_jobs(args.batch_jobs)
+    handles = [ctx.scene(sc) for sc, _, _ in spec]
+    setup_s = time.perf_counter() - t0
+    jobs = [(h, spp, depth, 0) for h, (_, spp, depth) in zip(handles, spec)]
+    paths = sum(sc["width"] * sc["height"] * spp for sc, spp, _ in spec)
+    try:
+        for h, spp, depth, _ in jobs:  # warm-up: per-scene pools and records
+            h.render(spp=spp, max_depth=depth)
+        ref, one_ms = [], 0.0
+        t0 = time.perf_counter()
+        for h, spp, depth, _ in jobs:
+            rgb, _, st = h.render(spp=spp, max_depth=depth)
+            ref.append((rgb, st["rounds"]))
+            one_ms += st["ms_total"]
+        one_wall = (time.perf_counter() - t0) * 1e3
+        tune = {}
+        for pool in (int(x) for x in args.batch_pools.split(",")):
+            ctx.render_batch(jobs, pool_size=pool)  # warm-up at this pool size
+            rgbs, _, rounds, st = ctx.render_batch(jobs, pool_size=pool)
+            same = all(np.array_equal(a.view(np.uint32), r[0].view(np.uint32)) and n == r[1]
+                       for a, n, r in zip(rgbs, rounds, ref))
+            tune[pool] = (st, same)
+        best = min(tune, key=lambda k: tune[k][0]["ms_total"])
+        t0 = time.perf_counter()
+        rgbs, _, rounds, st = ctx.render_batch(jobs, pool_size=best)
+        wall = (time.perf_counter() - t0) * 1e3
+    finally:
+        for h in handles:
+            h.free()
+    return {
+        "metric": "batched render throughput (64 mixed jobs)",
+        "value": round(paths / (st["ms_total"] * 1e3), 2),
+        "unit": "Msamples/s",
+        "higher_is_better": True,
+        "ms_batch": round(st["ms_total"], 2),
+        "wall_ms_batch": round(wall, 2),
+        "ms_one_by_one": round(one_ms, 2),
+        "wall_ms_one_by_one": round(one_wall, 2),
+        "speedup_vs_one_by_one": round(one_ms / st["ms_total"], 3),
+        "films_identical": all(v[1] for v in tune.values()),
+        "pool_size": best,
+        "tuning_ms": {str(k): round(v[0]["ms_total"], 2) for k, v in tune.items()},
+        "stages_ms": {"rounds": round(st["ms_rounds"], 2), "schedule": round(st["ms_schedule"], 2),
+                      "film": round(st["ms_film"], 2)},
+        "physical_rounds": st["rounds"], "graph_launches": st["launches"], "paths": paths,
+        "scene_setup_s": round(setup_s, 2),
+        "config": {"workload": "64 jobs: square resolution U[64, 1024], spp in {1, 4, 16, 64}, max depth 1..8, "
+                               "soup_room scenes of 1e4..1e6 triangles (log-uniform)", "seed": 0xBA7C4,
+                   "dtype": "f32"},
+    }
 
 
 def main():
@@ -265,6 +329,8 @@ def main():
         del d_tri, d_rays, d_prim, d_t, d_bary, d_bounce, d_prim2, d_t2
         torch.cuda.empty_cache()
         line["pathtracer"] = pathtracer(args, ctx, pbrtm)
+    if not args.no_batch:
+        line["batch"] = batch(args, ctx, pbrtm)
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -218,6 +218,41 @@ void pbrtm_scene_free(pbrtm_scene *scene);
 int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *scene, const pbrtm_render_params *params, float *film_rgb,
                  float *film_aux, pbrtm_render_stats *stats);
 
+/* Batched rendering (no counterpart in the reference, which renders one scene
+ * per process): every job's paths share one pool of path slots, so many small
+ * renders fill the device together. Each job's film is bit-identical to
+ * pbrtm_render of the same scene and parameters with pool_size = the job's
+ * pool_size (0 = 2^21): finished paths keep their length in rounds, and a
+ * per-job pass recovers the round each path would have finished in that
+ * one-by-one schedule, which orders the film's per-pixel sums. */
+typedef struct pbrtm_job {
+    pbrtm_scene *scene;            /* scenes may repeat across jobs */
+    pbrtm_render_params params;    /* spp, max_depth, pool_size as pbrtm_render; flags unused */
+    float *film_rgb;               /* [h][w][3], required */
+    float *film_aux;               /* [h][w][4] or NULL */
+    uint32_t rounds;               /* out: rounds of the job's one-by-one render */
+    uint32_t reserved;
+} pbrtm_job;
+
+/* pbrtm_batch_params.flags */
+#define PBRTM_BATCH_NO_GRAPH 2u    /* enqueue each round's kernels instead of replaying a captured graph */
+
+typedef struct pbrtm_batch_params {
+    uint32_t pool_size;           /* slots shared by the batch; 0 = 2^22 (capped at the batch's path count) */
+    uint32_t rounds_per_launch;   /* rounds per graph launch between done checks; 0 = 8 */
+    uint32_t flags;               /* PBRTM_DEVICE_PTRS (films are device pointers), PBRTM_BATCH_NO_GRAPH */
+    uint32_t reserved;
+} pbrtm_batch_params;
+
+typedef struct pbrtm_batch_stats {
+    double ms_total, ms_rounds, ms_schedule, ms_film;
+    uint64_t rounds, rays, shadow_rays, paths;
+    uint32_t pool_size, launches;
+} pbrtm_batch_stats;
+
+int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pbrtm_batch_params *params,
+                       pbrtm_batch_stats *stats);
+
 /* Spectral set-up exposed for tests: the sigmoid-polynomial coefficients the
  * reference's RGB-to-spectrum table gives an RGB (spectrum_util/rgb_to_spectrum_data.cu:361-400),
  * and the sRGB colour space matrices and white point (rgb_color_space.h:12-44). */

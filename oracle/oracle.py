@@ -168,6 +168,7 @@ def _pt_sigs(lib):
     p = c_void_p
     sig = {
         "orc_render": (c_int, [POINTER(OrcScene), c_int, p, p, p]),
+        "orc_render_paths": (c_int, [POINTER(OrcScene), c_int, p, p, p]),
         "orc_rgb_to_sigmoid": (c_int, [p, p]),
         "orc_srgb_matrices": (c_int, [p, p, p]),
         "orc_filter_sample": (c_int, [p, p]),
@@ -251,6 +252,24 @@ def render(sc, pool_size=2 * 1024 * 1024, threads=8):
         raise RuntimeError(f"orc_render failed ({rc})")
     return rgb, aux, {"rounds": int(st[0]), "rays": int(st[1]), "shadow_rays": int(st[2]), "paths": int(st[3]),
                       "render_ms": int(st[4]) / 1e3}
+
+
+def render_paths(sc, pool_size=2 * 1024 * 1024, threads=8):
+    """Render a scene dict; returns per path id (sample * W * H + pixel) the round its
+    first ray is traced in and the round it reaches the film, and the stats dict."""
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        _pt_sigs(lib)
+        lib._pt_ready = True
+    s, keep = _scene_struct(sc, pool_size)
+    n = sc["width"] * sc["height"] * sc["spp"]
+    start = np.zeros(n, np.uint32)
+    finish = np.zeros(n, np.uint32)
+    st = np.zeros(5, np.uint64)
+    rc = lib.orc_render_paths(ctypes.byref(s), threads, _p(start), _p(finish), _p(st))
+    if rc != 0:
+        raise RuntimeError(f"orc_render_paths failed ({rc})")
+    return start, finish, {"rounds": int(st[0]), "rays": int(st[1]), "shadow_rays": int(st[2]), "paths": int(st[3])}
 
 
 def rgb_to_sigmoid(rgb):

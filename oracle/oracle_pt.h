@@ -58,6 +58,10 @@ typedef struct orc_scene {
  * rays, shadow rays, finished paths, microseconds spent rendering (scene set-up
  * and the BVH build excluded). */
 int orc_render(const orc_scene *scene, int threads, float *film_rgb, float *aux, uint64_t *stats);
+/* The same render, reporting per path id (sample * W * H + pixel) the round its
+ * first ray is traced in and the round it is recorded in the film. */
+int orc_render_paths(const orc_scene *scene, int threads, uint32_t *path_start, uint32_t *path_finish,
+                     uint64_t *stats);
 
 /* Intermediate tables, for tests: the RGB-to-spectrum sigmoid coefficients of an
  * albedo RGB, the sRGB rgb_from_xyz matrix, the filter sample of a 2D u. */

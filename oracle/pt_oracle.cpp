@@ -1303,7 +1303,10 @@ inline void sensor_rgb(const Scene &sc, const S4 &L, const Lambda &lam, float rg
     for (int c = 0; c < 3; ++c) rgb[c] = 1.0f * average(sample_pl(sc.sp->cie[c], lam) * s);
 }
 
-void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64_t *stats) {
+// path_start / path_finish (nullable, [W*H*spp] by path id): the round a path's
+// first ray is traced in and the round it is recorded in the film.
+void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64_t *stats,
+            uint32_t *path_start = nullptr, uint32_t *path_finish = nullptr) {
     Scene sc;
     build_scene(sc, d);
     const uint32_t W = d.width, H = d.height;
@@ -1314,6 +1317,7 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
     std::vector<PathSlot> pool(P);
     std::vector<float> rgb_sum(size_t(W) * H * 3, 0.f), w_sum(size_t(W) * H, 0.f);
     uint64_t next_id = 0, n_rays = 0, n_shadow = 0, n_done = 0, round = 0;
+    uint32_t gen_round = 0;  // the round whose ray cast first traces the paths gen() starts
     const float inf = std::numeric_limits<float>::infinity();
 
     // generate_new_path (wavefront_path.cu:265-308)
@@ -1327,6 +1331,7 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                 continue;
             }
             work.push_back({s, id});
+            if (path_start) path_start[id] = gen_round;
         }
         parallel(work.size(), threads, [&](size_t k) {
             PathSlot &p = pool[work[k].first];
@@ -1415,6 +1420,7 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                 r.sample = p.sample;
                 r.w = p.filter_w;
                 sensor_rgb(sc, p.L, p.lam, r.rgb);
+                if (path_finish) path_finish[uint64_t(p.sample) * W * H + p.pixel] = uint32_t(round);
                 recs.push_back(r);
                 p.active = false;
                 p.free = true;
@@ -1449,6 +1455,7 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
             for (int c = 0; c < 3; ++c) rgb_sum[3 * r.pixel + c] += r.w * r.rgb[c];
             w_sum[r.pixel] += r.w;
         }
+        gen_round = uint32_t(round + 1);
         gen(freed);
         // gpu_evaluate_material + sample_bsdf (wavefront_path.cu:310-431)
         parallel(shade.size(), threads, [&](size_t k) {
@@ -1551,6 +1558,18 @@ int orc_render(const orc_scene *scene, int threads, float *film_rgb, float *aux,
     if (!scene || !film_rgb || !scene->tri || !scene->tri_material || scene->pool_size == 0) return -1;
     try {
         orc::render(*scene, threads, film_rgb, aux, stats);
+    } catch (const std::exception &) {
+        return -2;
+    }
+    return 0;
+}
+
+int orc_render_paths(const orc_scene *scene, int threads, uint32_t *path_start, uint32_t *path_finish,
+                     uint64_t *stats) {
+    if (!scene || !path_start || !path_finish || !scene->tri || scene->pool_size == 0) return -1;
+    try {
+        std::vector<float> rgb(size_t(scene->width) * scene->height * 3);
+        orc::render(*scene, threads, rgb.data(), nullptr, stats, path_start, path_finish);
     } catch (const std::exception &) {
         return -2;
     }

@@ -24,6 +24,8 @@ struct pbrtm_ctx {
     hipStream_t stream = nullptr;
     BuildScratch scratch;
     DevBuf stage_in, stage_a, stage_b, stage_c, dev_err, spill, collapse, queue;
+    pbrtm::pt::PathPool batch_pool;
+    DevBuf batch_jobs, batch_films;
     hipEvent_t ev[8] = {};
     std::string error;
 };
@@ -171,6 +173,12 @@ void pbrtm_destroy(pbrtm_ctx *ctx) {
     ctx->spill.release();
     ctx->collapse.release();
     ctx->queue.release();
+    auto &p = ctx->batch_pool;
+    for (DevBuf *b : {&p.path_id, &p.flags, &p.rng, &p.dim, &p.lam, &p.lam_pdf, &p.L, &p.beta, &p.nee_L, &p.scal,
+                      &p.prev, &p.rays, &p.prim, &p.thit, &p.bary, &p.srays, &p.sprim, &p.rec, &p.rec_round,
+                      &p.counters, &p.queue, &p.squeue, &p.spill, &p.job, &p.sround, &ctx->batch_jobs,
+                      &ctx->batch_films})
+        b->release();
     for (auto &e : ctx->ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -580,7 +588,7 @@ void pbrtm_scene_free(pbrtm_scene *sc) {
     auto &p = sc->pool;
     for (pbrtm::DevBuf *b : {&p.path_id, &p.flags, &p.rng, &p.dim, &p.lam, &p.lam_pdf, &p.L, &p.beta, &p.nee_L,
                              &p.scal, &p.prev, &p.rays, &p.prim, &p.thit, &p.bary, &p.srays, &p.sprim, &p.rec,
-                             &p.rec_round, &p.counters, &p.queue, &p.squeue, &p.spill})
+                             &p.rec_round, &p.counters, &p.queue, &p.squeue, &p.spill, &p.job, &p.sround})
         b->release();
     delete sc;
 }
@@ -661,17 +669,53 @@ int pbrtm_scene_create(pbrtm_ctx *ctx, const pbrtm_scene_desc *desc, pbrtm_scene
     return PBRTM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+WideTrace scene_trace(const pbrtm_scene *sc) {
+    const pbrtm_bvh *bvh = sc->bvh;
+    WideTrace W{};
+    W.nodes = bvh->wide.nodes.as<WNode>();
+    W.order = bvh->wide.order.as<uint32_t>();
+    W.tri_leaf = bvh->tri_leaf.as<float>();
+    W.prim_order = bvh->dev.prim_order.as<uint32_t>();
+    W.num_nodes = bvh->wide.num_nodes;
+    W.num_prims = bvh->n;
+    W.max_depth = bvh->wide.max_depth;
+    W.spill_per_ray = W.max_depth > kLdsStack ? W.max_depth - kLdsStack + 1 : 0;
+    return W;
+}
+
+// Shared checks of pbrtm_render and pbrtm_render_batch; an empty string when valid.
+const char *check_render_params(const pbrtm_scene *sc, const pbrtm_render_params *params, int *code) {
+    const uint32_t spp = params->spp;
+    const uint32_t spd = uint32_t(std::sqrt(double(spp)));
+    *code = PBRTM_ERR_INVALID;
+    if (spp == 0 || spd * spd != spp) return "spp must be a positive square (stratified)";
+    *code = PBRTM_ERR_UNSUPPORTED;
+    if (spp > 1024) return "at most 1024 samples per pixel";
+    const uint64_t wh = uint64_t(sc->host.width) * sc->host.height;
+    if (wh * spp + (params->pool_size ? params->pool_size : (2u << 20)) >= 0xFFFFFFFFull)
+        return "at most 2^32 - 1 - pool_size paths per render";
+    *code = PBRTM_ERR_INVALID;
+    if (params->max_depth > 0xFFFE) return "max_depth too large";
+    return "";
+}
+
+}  // namespace
+
+extern "C" {
+
 int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *params, float *film_rgb,
                  float *film_aux, pbrtm_render_stats *stats) {
     if (!ctx || !sc || !params || !film_rgb) return PBRTM_ERR_INVALID;
     if (sc->ctx != ctx) return fail(ctx, PBRTM_ERR_INVALID, "scene belongs to another context");
+    int code = 0;
+    const char *why = check_render_params(sc, params, &code);
+    if (*why) return fail(ctx, code, why);
     const uint32_t spp = params->spp;
-    const uint32_t spd = uint32_t(std::sqrt(double(spp)));
-    if (spp == 0 || spd * spd != spp) return fail(ctx, PBRTM_ERR_INVALID, "spp must be a positive square (stratified)");
-    if (spp > 1024) return fail(ctx, PBRTM_ERR_UNSUPPORTED, "at most 1024 samples per pixel");
     const uint64_t wh = uint64_t(sc->host.width) * sc->host.height;
-    if (wh * spp >= 0xFFFFFFFFull) return fail(ctx, PBRTM_ERR_UNSUPPORTED, "at most 2^32 - 1 paths per render");
-    if (params->max_depth > 0xFFFE) return fail(ctx, PBRTM_ERR_INVALID, "max_depth too large");
     const uint32_t pool = params->pool_size ? params->pool_size : (2u << 20);
     CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     const bool dev = params->flags & PBRTM_DEVICE_PTRS;
@@ -684,16 +728,7 @@ int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *par
             d_aux = sc->film_aux.as<float>();
         }
     }
-    pbrtm_bvh *bvh = sc->bvh;
-    WideTrace W{};
-    W.nodes = bvh->wide.nodes.as<WNode>();
-    W.order = bvh->wide.order.as<uint32_t>();
-    W.tri_leaf = bvh->tri_leaf.as<float>();
-    W.prim_order = bvh->dev.prim_order.as<uint32_t>();
-    W.num_nodes = bvh->wide.num_nodes;
-    W.num_prims = bvh->n;
-    W.max_depth = bvh->wide.max_depth;
-    W.spill_per_ray = W.max_depth > kLdsStack ? W.max_depth - kLdsStack + 1 : 0;
+    const WideTrace W = scene_trace(sc);
     CK(ctx, ctx->dev_err.reserve(64), "alloc");
     CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream), "memset");
     pbrtm::pt::RenderStats rs;
@@ -717,6 +752,86 @@ int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *par
         stats->rays = rs.rays;
         stats->shadow_rays = rs.shadow_rays;
         stats->paths = rs.paths;
+    }
+    return PBRTM_OK;
+}
+
+int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pbrtm_batch_params *params,
+                       pbrtm_batch_stats *stats) {
+    if (!ctx || !jobs || !params || njobs == 0) return PBRTM_ERR_INVALID;
+    const bool dev = params->flags & PBRTM_DEVICE_PTRS;
+    uint64_t film_floats = 0, paths = 0;
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const pbrtm_job &jb = jobs[j];
+        if (!jb.scene || !jb.film_rgb) return fail(ctx, PBRTM_ERR_INVALID, "job without scene or film");
+        if (jb.scene->ctx != ctx) return fail(ctx, PBRTM_ERR_INVALID, "scene belongs to another context");
+        int code = 0;
+        const char *why = check_render_params(jb.scene, &jb.params, &code);
+        if (*why) return fail(ctx, code, why);
+        if (jb.params.max_depth > 4000) return fail(ctx, PBRTM_ERR_UNSUPPORTED, "batched jobs allow max_depth <= 4000");
+        const uint64_t wh = uint64_t(jb.scene->host.width) * jb.scene->host.height;
+        film_floats += wh * (jb.film_aux ? 7 : 3);
+        paths += wh * jb.params.spp;
+    }
+    const uint64_t slots = std::min<uint64_t>(params->pool_size ? params->pool_size : (4u << 20), paths);
+    if (paths + slots >= 0xFFFFFFFFull)
+        return fail(ctx, PBRTM_ERR_UNSUPPORTED, "at most 2^32 - 1 - pool_size paths per batch");
+    CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    if (!dev) CK(ctx, ctx->batch_films.reserve(film_floats * 4), "alloc films");
+    std::vector<pbrtm::pt::BatchJob> bj(njobs);
+    float *f = dev ? nullptr : ctx->batch_films.as<float>();
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const pbrtm_job &jb = jobs[j];
+        const uint64_t wh = uint64_t(jb.scene->host.width) * jb.scene->host.height;
+        bj[j].S = jb.scene->dev;
+        bj[j].W = scene_trace(jb.scene);
+        bj[j].spp = jb.params.spp;
+        bj[j].max_depth = jb.params.max_depth;
+        bj[j].vpool = jb.params.pool_size ? jb.params.pool_size : (2u << 20);
+        if (dev) {
+            bj[j].film_rgb = jb.film_rgb;
+            bj[j].film_aux = jb.film_aux;
+        } else {
+            bj[j].film_rgb = f;
+            f += wh * 3;
+            bj[j].film_aux = jb.film_aux ? f : nullptr;
+            if (jb.film_aux) f += wh * 4;
+        }
+    }
+    const uint32_t pool = params->pool_size ? params->pool_size : (4u << 20);
+    const uint32_t rpl = params->rounds_per_launch ? params->rounds_per_launch : 8u;
+    CK(ctx, ctx->dev_err.reserve(64), "alloc");
+    CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream), "memset");
+    pbrtm::pt::BatchStats bs;
+    CK(ctx, pbrtm::pt::render_batch(bj, pool, rpl, !(params->flags & PBRTM_BATCH_NO_GRAPH), ctx->batch_pool,
+                                    ctx->batch_jobs, ctx->dev_err.as<uint32_t>(), &bs, ctx->stream),
+       "render batch");
+    uint32_t derr = 0;
+    CK(ctx, hipMemcpyAsync(&derr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+    if (!dev) {
+        for (uint32_t j = 0; j < njobs; ++j) {
+            const uint64_t wh = uint64_t(jobs[j].scene->host.width) * jobs[j].scene->host.height;
+            CK(ctx, hipMemcpyAsync(jobs[j].film_rgb, bj[j].film_rgb, wh * 12, hipMemcpyDeviceToHost, ctx->stream),
+               "download");
+            if (jobs[j].film_aux)
+                CK(ctx, hipMemcpyAsync(jobs[j].film_aux, bj[j].film_aux, wh * 16, hipMemcpyDeviceToHost, ctx->stream),
+                   "download");
+        }
+    }
+    CK(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (derr) return fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr));
+    for (uint32_t j = 0; j < njobs; ++j) jobs[j].rounds = bs.job_rounds[j];
+    if (stats) {
+        stats->ms_total = bs.ms_total;
+        stats->ms_rounds = bs.ms_rounds;
+        stats->ms_schedule = bs.ms_schedule;
+        stats->ms_film = bs.ms_film;
+        stats->rounds = bs.rounds;
+        stats->rays = bs.rays;
+        stats->shadow_rays = bs.shadow_rays;
+        stats->paths = bs.paths;
+        stats->pool_size = uint32_t(std::min<uint64_t>(pool, paths));
+        stats->launches = bs.launches;
     }
     return PBRTM_OK;
 }

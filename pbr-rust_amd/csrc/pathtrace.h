@@ -60,6 +60,13 @@ struct SceneDev {
     uint32_t width, height;
 };
 
+// One job of a batched render, as the kernels read it.
+struct JobDev {
+    SceneDev S;
+    uint64_t base, total;  // first global path id, paths (W * H * spp)
+    uint32_t spd, spp, max_depth, vpool;  // vpool: the one-by-one pool size whose schedule the film reproduces
+};
+
 // Host-side scene set-up (render space, spectra, camera, filter, lights).
 struct SceneHost {
     std::vector<float> tri;  // render space
@@ -88,7 +95,7 @@ void srgb_matrices(float xyz_from_rgb[9], float rgb_from_xyz[9], float white_xy[
 
 struct PathPool {
     DevBuf path_id, flags, rng, dim, lam, lam_pdf, L, beta, nee_L, scal, prev, rays, prim, thit, bary, srays, sprim,
-        rec, rec_round, counters, queue, squeue, spill;
+        rec, rec_round, counters, queue, squeue, spill, job, sround;
     uint32_t cap = 0;
     uint64_t rec_cap = 0;
 };
@@ -103,6 +110,27 @@ struct RenderStats {
 hipError_t render(const SceneDev &S, const WideTrace &W, uint32_t spp, uint32_t max_depth, uint32_t pool,
                   PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
                   hipStream_t stream);
+
+struct BatchJob {
+    SceneDev S;
+    WideTrace W;
+    uint32_t spp, max_depth, vpool;
+    float *film_rgb, *film_aux;  // device; film_aux nullable
+};
+
+struct BatchStats {
+    double ms_total = 0, ms_rounds = 0, ms_schedule = 0, ms_film = 0;
+    uint64_t rounds = 0, rays = 0, shadow_rays = 0, paths = 0;
+    uint32_t launches = 0;
+    std::vector<uint32_t> job_rounds;  // each job's round count in its one-by-one schedule
+};
+
+// Renders every job in one shared pool of `pool` slots; `rounds_per_launch`
+// rounds are enqueued (as one captured graph when use_graph) between checks
+// of the device's done flag. Each job's film equals its one-by-one render
+// with pool size vpool.
+hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32_t rounds_per_launch, bool use_graph,
+                        PathPool &pp, DevBuf &jobbuf, uint32_t *err, BatchStats *stats, hipStream_t stream);
 
 }  // namespace pt
 }  // namespace pbrtm

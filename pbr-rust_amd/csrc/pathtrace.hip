@@ -14,6 +14,9 @@
 #include "pt_device.h"
 #include "wave_ops.h"
 
+#include <algorithm>
+#include <cstring>
+
 namespace pbrtm {
 namespace pt {
 
@@ -24,8 +27,9 @@ constexpr uint32_t kFlagSpecular = 1u << 16, kFlagAnyNonSpec = 1u << 17, kFlagNe
 constexpr uint32_t kDepthMask = 0xFFFFu;
 constexpr uint32_t kIdle = 0xFFFFFFFFu;
 
-// counters[] layout
-constexpr int kCntNextId = 0, kCntActive = 1, kCntQueue = 2, kCntSQueue = 3, kCntShadow = 4;
+// counters[] layout (u32 words); 8..15 hold u64 accumulators of a batched render
+constexpr int kCntNextId = 0, kCntActive = 1, kCntQueue = 2, kCntSQueue = 3, kCntShadow = 4, kCntRound = 5,
+              kCntDone = 6, kCntAccRays = 8, kCntAccShadow = 10;
 
 struct State {
     uint32_t *path_id, *flags, *dim;
@@ -37,15 +41,30 @@ struct State {
     float4 *srays;                                            // [4 per slot]
     const uint32_t *sprim;                                    // [2 per slot]
     float4 *rec;                                              // per path id: sensor RGB, filter weight
-    uint32_t *rec_round;
+    uint32_t *rec_round;     // finishing round; in a batch, rounds since the path started
+    uint32_t *job, *sround;  // batch only: the slot's job and the round its path started
     uint32_t *counters;
     uint32_t *err;
 };
 
 struct Params {
-    uint64_t total;
+    uint64_t total;  // paths of the render, or of the whole batch
     uint32_t spd, spp, max_depth, round, pool;
+    const JobDev *jobs;        // batch only
+    const uint64_t *job_base;  // batch only: first path id of each job, then the batch total
+    uint32_t njobs;
 };
+
+// The job whose path-id range holds global path id `gid`.
+__device__ __forceinline__ uint32_t find_job(const Params &P, uint64_t gid) {
+    uint32_t lo = 0, hi = P.njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (P.job_base[mid] <= gid) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 __device__ __forceinline__ f3 ld3(const float *p) { return {p[0], p[1], p[2]}; }
 
@@ -252,11 +271,12 @@ __device__ __forceinline__ float3 sensor_rgb(const SceneDev &S, const S4 &L, con
     return make_float3(c[0], c[1], c[2]);
 }
 
-// generate_new_path (wavefront_path.cu:265-308) for path `id` in slot `i`.
-__device__ void start_path(const SceneDev &S, const State &st, const Params &P, uint32_t i, uint64_t id) {
+// generate_new_path (wavefront_path.cu:265-308) for path `id` of the render
+// (global id `gid` in a batch) in slot `i`.
+__device__ void start_path(const SceneDev &S, const State &st, uint32_t spd, uint32_t i, uint64_t id, uint64_t gid) {
     const uint64_t wh = uint64_t(S.width) * S.height;
     const uint32_t pixel = uint32_t(id % wh), sample = uint32_t(id / wh);
-    Sampler smp{pcg_seq(hash_u32(pixel)), pixel, sample, 0, P.spd};
+    Sampler smp{pcg_seq(hash_u32(pixel)), pixel, sample, 0, spd};
     pcg_advance(smp.rng, uint64_t(sample) * 65536ull);
     float u0, u1;
     smp.get_2d(&u0, &u1);
@@ -269,7 +289,7 @@ __device__ void start_path(const SceneDev &S, const State &st, const Params &P, 
     const f3 pc = xf_point(S.cam_from_raster, {px, py, 0.f});
     const f3 dc = norm3(pc);
     const f3 rd = xf_vector(S.render_from_camera, dc);
-    st.path_id[i] = uint32_t(id);
+    st.path_id[i] = uint32_t(gid);
     st.flags[i] = kFlagSpecular;
     st.rng[i] = smp.rng.state;
     st.dim[i] = smp.dim;
@@ -291,16 +311,34 @@ __global__ __launch_bounds__(kBlock) void init_pool_kernel(SceneDev S, State st,
     if (i >= P.pool) return;
     st.srays[4 * size_t(i)] = make_float4(0.f, 0.f, 0.f, 0.f);
     st.srays[4 * size_t(i) + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (uint64_t(i) < P.total) start_path(S, st, P, i, i);
+    if (uint64_t(i) < P.total) start_path(S, st, P.spd, i, i, i);
     else idle_slot(st, i);
 }
 
-// One round for every slot: deferred NEE, control_logic, film record, new
-// path, material evaluation with sample_ld and sample_f.
-__global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Params P) {
+__global__ __launch_bounds__(kBlock) void init_pool_multi_kernel(State st, Params P) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const bool in = i < P.pool;
-    const uint32_t pid = in ? st.path_id[i] : kIdle;
+    if (i >= P.pool) return;
+    st.srays[4 * size_t(i)] = make_float4(0.f, 0.f, 0.f, 0.f);
+    st.srays[4 * size_t(i) + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (uint64_t(i) < P.total) {
+        const uint32_t j = find_job(P, i);
+        const JobDev &J = P.jobs[j];
+        start_path(J.S, st, J.spd, i, i - J.base, i);
+        st.job[i] = j;
+        st.sround[i] = 0;
+    } else {
+        idle_slot(st, i);
+    }
+}
+
+// One round for slot i: deferred NEE, control_logic, film record, new path,
+// material evaluation with sample_ld and sample_f. pid is the slot's path id
+// (global in a batch), lid its id within its render, S its scene; every lane
+// of the wave calls this (the new-path ballot is wave-wide).
+template <bool MULTI>
+__device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, const Params &P, uint32_t i,
+                                           uint32_t pid, uint32_t lid, uint32_t spd, uint32_t max_depth,
+                                           uint32_t round) {
     const bool live = pid != kIdle;
     uint32_t fl = live ? st.flags[i] : 0u;
     S4 L = live ? f4s(st.L[i]) : s4(0.f), beta = live ? f4s(st.beta[i]) : s4(0.f);
@@ -311,14 +349,14 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Par
     }
     const uint32_t prim = live ? st.prim[i] : kNoHit;
     const uint32_t depth = fl & kDepthMask;
-    bool terminate = live && (prim == kNoHit || depth > P.max_depth || !s4pos(beta));
+    bool terminate = live && (prim == kNoHit || depth > max_depth || !s4pos(beta));
     Sampler smp{};
     S4 lam = s4(0.f), lpdf = s4(0.f);
     float4 scal = make_float4(0.f, 0.f, 0.f, 0.f);
     if (live) {
-        smp.pixel = pid % (S.width * S.height);
-        smp.sample = pid / (S.width * S.height);
-        smp.spd = P.spd;
+        smp.pixel = lid % (S.width * S.height);
+        smp.sample = lid / (S.width * S.height);
+        smp.spd = spd;
         smp.rng.state = st.rng[i];
         smp.rng.inc = sampler_inc(smp.pixel);
         smp.dim = st.dim[i];
@@ -351,10 +389,21 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Par
     if (terminate) {
         const float3 rgb = sensor_rgb(S, L, lam, lpdf);
         st.rec[pid] = make_float4(rgb.x, rgb.y, rgb.z, scal.x);
-        st.rec_round[pid] = P.round;
+        st.rec_round[pid] = MULTI ? round - st.sround[i] : round;
         const uint64_t nid = uint64_t(base) + __popcll(lanes & lanemask_lt());
-        if (nid < P.total) start_path(S, st, P, i, nid);
-        else idle_slot(st, i);
+        if (nid < P.total) {
+            if (MULTI) {
+                const uint32_t j = find_job(P, nid);
+                const JobDev &J = P.jobs[j];
+                start_path(J.S, st, J.spd, i, nid - J.base, nid);
+                st.job[i] = j;
+                st.sround[i] = round + 1;
+            } else {
+                start_path(S, st, spd, i, nid, nid);
+            }
+        } else {
+            idle_slot(st, i);
+        }
         st.srays[4 * size_t(i)].w = 0.f;
         st.srays[4 * size_t(i) + 2].w = 0.f;
         return;
@@ -507,6 +556,99 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Par
     st.dim[i] = smp.dim;
 }
 
+__global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Params P) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t pid = i < P.pool ? st.path_id[i] : kIdle;
+    shade_impl<false>(S, st, P, i, pid, pid, P.spd, P.max_depth, P.round);
+}
+
+// Batched render: each slot reads its job's scene and parameters; the round
+// number comes from the device counter so the round can be replayed in a graph.
+__global__ __launch_bounds__(kBlock) void shade_multi_kernel(State st, Params P) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t pid = i < P.pool ? st.path_id[i] : kIdle;
+    const uint32_t j = pid != kIdle ? st.job[i] : 0u;
+    const JobDev &J = P.jobs[j];
+    const uint32_t lid = pid != kIdle ? uint32_t(pid - J.base) : kIdle;
+    shade_impl<true>(J.S, st, P, i, pid, lid, J.spd, J.max_depth, st.counters[kCntRound]);
+}
+
+// Start and end of a batched round (kernels, so that a captured graph holds no memset).
+__global__ void round_begin_kernel(uint32_t *counters) {
+    if (threadIdx.x == 0) {
+        counters[kCntActive] = 0;
+        counters[kCntQueue] = 0;
+        counters[kCntSQueue] = 0;
+        counters[kCntShadow] = 0;
+    }
+}
+__global__ void round_end_kernel(uint32_t *counters) {
+    if (threadIdx.x != 0 || counters[kCntDone]) return;
+    unsigned long long *acc = reinterpret_cast<unsigned long long *>(counters + kCntAccRays);
+    acc[0] += counters[kCntQueue];
+    acc[1] += counters[kCntSQueue];
+    counters[kCntRound] += 1;
+    if (counters[kCntActive] == 0) counters[kCntDone] = 1;
+}
+
+// The reference's one-by-one pool schedule for one job (one workgroup per
+// job), recovered from each path's length in rounds: ids [0, pool) start in
+// round 0; the ids started at the end of round r are the next T_r ids, T_r =
+// the number of paths finishing in round r (wavefront_path.cu:265-285,
+// :641-700). rec_round holds lengths on entry and finishing rounds on exit;
+// rounds[j] receives the job's round count. Lengths never exceed
+// max_depth + 1, so finishing counts live in a ring of kRing rounds.
+constexpr uint32_t kSchedBlock = 1024, kRing = 4096;
+__global__ __launch_bounds__(kSchedBlock) void schedule_kernel(const JobDev *jobs, uint32_t *rec_round,
+                                                               uint32_t *rounds, uint32_t *err) {
+    __shared__ uint32_t ring[kRing];
+    __shared__ uint32_t s_max, s_bad;
+    const JobDev &J = jobs[blockIdx.x];
+    for (uint32_t k = threadIdx.x; k < kRing; k += kSchedBlock) ring[k] = 0;
+    if (threadIdx.x == 0) {
+        s_max = 0;
+        s_bad = 0;
+    }
+    __syncthreads();
+    uint32_t *rr = rec_round + J.base;
+    const uint64_t total = J.total;
+    const uint32_t lmax = J.max_depth + 1;
+    const uint64_t cap = (total + 1) * (uint64_t(J.max_depth) + 3);
+    uint64_t lo = 0, hi = total < J.vpool ? total : J.vpool;
+    uint64_t r = 0;
+    uint32_t my_max = 0;
+    for (;;) {
+        for (uint64_t k = lo + threadIdx.x; k < hi; k += kSchedBlock) {
+            const uint32_t len = rr[k];
+            if (len > lmax) {
+                s_bad = 1;
+                continue;
+            }
+            const uint32_t f = uint32_t(r) + len;
+            rr[k] = f;
+            my_max = f > my_max ? f : my_max;
+            atomicAdd(&ring[f % kRing], 1u);
+        }
+        __syncthreads();
+        const uint32_t t = ring[r % kRing];
+        __syncthreads();
+        if (threadIdx.x == 0) ring[r % kRing] = 0;
+        if (hi == total || s_bad) break;
+        lo = hi;
+        hi = hi + t < total ? hi + t : total;
+        if (++r > cap) {
+            if (threadIdx.x == 0) s_bad = 1;
+            break;
+        }
+    }
+    atomicMax(&s_max, my_max);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        rounds[blockIdx.x] = s_max + 1;
+        if (s_bad) atomicOr(err, uint32_t(kErrCapacity));
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void count_live_kernel(const uint32_t *path_id, uint32_t pool, uint32_t *count,
                                                             const float4 *srays, uint32_t *shadow) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -584,13 +726,9 @@ __global__ __launch_bounds__(kFilmWaves * kWave) void film_kernel(SceneDev S, co
 
 }  // namespace
 
-hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t max_depth, uint32_t pool,
-                  PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
-                  hipStream_t stream) {
-    const uint64_t total = uint64_t(S.width) * S.height * spp;
-    const uint32_t spd = uint32_t(sqrt(double(spp)));
-    if (spd * spd != spp || spp == 0 || spp > kMaxSpp) return hipErrorInvalidValue;
-    if (total >= 0xFFFFFFFFull || pool == 0) return hipErrorInvalidValue;
+namespace {
+
+hipError_t reserve_pool(PathPool &pp, uint32_t pool, uint64_t total, bool batch) {
     hipError_t e;
 #define PT_CK(x)                          \
     do {                                  \
@@ -613,18 +751,21 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         PT_CK(pp.squeue.reserve(size_t(pool) * 8));
         pp.cap = pool;
     }
+    if (batch) {
+        PT_CK(pp.job.reserve(size_t(pool) * 4));
+        PT_CK(pp.sround.reserve(size_t(pool) * 4));
+    }
     if (pp.rec_cap < total) {
         PT_CK(pp.rec.reserve(total * 16));
         PT_CK(pp.rec_round.reserve(total * 4));
         pp.rec_cap = total;
     }
     PT_CK(pp.counters.reserve(64));
-    WideTrace W = W0;
-    const uint32_t spill_need = W.spill_per_ray;
-    if (spill_need) {
-        PT_CK(pp.spill.reserve(size_t(pool) * 2 * spill_need * sizeof(uint2)));
-        W.spill = pp.spill.as<uint2>();
-    }
+    return hipSuccess;
+#undef PT_CK
+}
+
+State make_state(PathPool &pp, uint32_t *err) {
     State st;
     st.path_id = pp.path_id.as<uint32_t>();
     st.flags = pp.flags.as<uint32_t>();
@@ -645,9 +786,37 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
     st.sprim = pp.sprim.as<uint32_t>();
     st.rec = pp.rec.as<float4>();
     st.rec_round = pp.rec_round.as<uint32_t>();
+    st.job = pp.job.as<uint32_t>();
+    st.sround = pp.sround.as<uint32_t>();
     st.counters = pp.counters.as<uint32_t>();
     st.err = err;
-    Params P{total, spd, spp, max_depth, 0u, pool};
+    return st;
+}
+
+}  // namespace
+
+hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t max_depth, uint32_t pool,
+                  PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
+                  hipStream_t stream) {
+    const uint64_t total = uint64_t(S.width) * S.height * spp;
+    const uint32_t spd = uint32_t(sqrt(double(spp)));
+    if (spd * spd != spp || spp == 0 || spp > kMaxSpp) return hipErrorInvalidValue;
+    // the new-path counter runs up to total + pool
+    if (total + pool >= 0xFFFFFFFFull || pool == 0) return hipErrorInvalidValue;
+    hipError_t e;
+#define PT_CK(x)                          \
+    do {                                  \
+        if ((e = (x)) != hipSuccess) return e; \
+    } while (0)
+    PT_CK(reserve_pool(pp, pool, total, false));
+    WideTrace W = W0;
+    const uint32_t spill_need = W.spill_per_ray;
+    if (spill_need) {
+        PT_CK(pp.spill.reserve(size_t(pool) * 2 * spill_need * sizeof(uint2)));
+        W.spill = pp.spill.as<uint2>();
+    }
+    const State st = make_state(pp, err);
+    Params P{total, spd, spp, max_depth, 0u, pool, nullptr, nullptr, 0u};
     const uint32_t grid = (pool + kBlock - 1) / kBlock;
     hipEvent_t ev[6];
     for (auto &x : ev) PT_CK(hipEventCreate(&x));
@@ -717,6 +886,168 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
     rs.rounds = P.round;
     rs.paths = total;
     if (stats) *stats = rs;
+    return hipSuccess;
+#undef PT_CK
+}
+
+hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32_t rounds_per_launch, bool use_graph,
+                        PathPool &pp, DevBuf &jobbuf, uint32_t *err, BatchStats *stats, hipStream_t stream) {
+    const uint32_t nj = uint32_t(jobs.size());
+    if (nj == 0 || pool == 0 || rounds_per_launch == 0) return hipErrorInvalidValue;
+    std::vector<JobDev> jd(nj);
+    std::vector<WideTrace> jw(nj);
+    std::vector<uint64_t> base(nj + 1);
+    uint64_t total = 0;
+    uint32_t spill_need = 0;
+    for (uint32_t j = 0; j < nj; ++j) {
+        const BatchJob &b = jobs[j];
+        const uint32_t spd = uint32_t(sqrt(double(b.spp)));
+        if (b.spp == 0 || spd * spd != b.spp || b.spp > kMaxSpp || b.vpool == 0) return hipErrorInvalidValue;
+        if (b.max_depth + 3 > kRing) return hipErrorInvalidValue;
+        const uint64_t n = uint64_t(b.S.width) * b.S.height * b.spp;
+        jd[j].S = b.S;
+        jd[j].base = total;
+        jd[j].total = n;
+        jd[j].spd = spd;
+        jd[j].spp = b.spp;
+        jd[j].max_depth = b.max_depth;
+        jd[j].vpool = b.vpool;
+        jw[j] = b.W;
+        base[j] = total;
+        total += n;
+        spill_need = std::max(spill_need, b.W.spill_per_ray);
+    }
+    base[nj] = total;
+    if (uint64_t(pool) > total) pool = uint32_t(total);
+    if (total + pool >= 0xFFFFFFFFull) return hipErrorInvalidValue;  // the new-path counter runs up to total + pool
+    hipError_t e;
+#define PT_CK(x)                          \
+    do {                                  \
+        if ((e = (x)) != hipSuccess) return e; \
+    } while (0)
+    PT_CK(reserve_pool(pp, pool, total, true));
+    WideTrace W{};
+    W.spill_per_ray = spill_need;
+    if (spill_need) {
+        PT_CK(pp.spill.reserve(size_t(pool) * 2 * spill_need * sizeof(uint2)));
+        W.spill = pp.spill.as<uint2>();
+    }
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t off_jw = align(nj * sizeof(JobDev)), off_base = off_jw + align(nj * sizeof(WideTrace));
+    const size_t off_rounds = off_base + align((nj + 1) * sizeof(uint64_t));
+    PT_CK(jobbuf.reserve(off_rounds + align(nj * 4)));
+    char *jb = jobbuf.as<char>();
+    const JobDev *d_jobs = reinterpret_cast<const JobDev *>(jb);
+    const WideTrace *d_jw = reinterpret_cast<const WideTrace *>(jb + off_jw);
+    const uint64_t *d_base = reinterpret_cast<const uint64_t *>(jb + off_base);
+    uint32_t *d_rounds = reinterpret_cast<uint32_t *>(jb + off_rounds);
+    PT_CK(hipMemcpyAsync(jb, jd.data(), nj * sizeof(JobDev), hipMemcpyHostToDevice, stream));
+    PT_CK(hipMemcpyAsync(jb + off_jw, jw.data(), nj * sizeof(WideTrace), hipMemcpyHostToDevice, stream));
+    PT_CK(hipMemcpyAsync(jb + off_base, base.data(), (nj + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    const State st = make_state(pp, err);
+    Params P{total, 0u, 0u, 0u, 0u, pool, d_jobs, d_base, nj};
+    const uint32_t grid = (pool + kBlock - 1) / kBlock;
+    hipEvent_t ev[4];
+    for (auto &x : ev) PT_CK(hipEventCreate(&x));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    struct Guard {
+        hipEvent_t *e;
+        hipGraph_t *g;
+        hipGraphExec_t *x;
+        ~Guard() {
+            for (int k = 0; k < 4; ++k) hipEventDestroy(e[k]);
+            if (*x) hipGraphExecDestroy(*x);
+            if (*g) hipGraphDestroy(*g);
+        }
+    } guard{ev, &graph, &exec};
+    PT_CK(hipMemsetAsync(st.counters, 0, 64, stream));
+    const uint32_t first = pool;
+    PT_CK(hipMemcpyAsync(st.counters + kCntNextId, &first, 4, hipMemcpyHostToDevice, stream));
+    PT_CK(hipEventRecord(ev[0], stream));
+    init_pool_multi_kernel<<<grid, kBlock, 0, stream>>>(st, P);
+    PT_CK(hipGetLastError());
+    auto enqueue_rounds = [&]() -> hipError_t {
+        hipError_t e2;
+        for (uint32_t k = 0; k < rounds_per_launch; ++k) {
+            round_begin_kernel<<<1, 64, 0, stream>>>(st.counters);
+            if ((e2 = hipGetLastError()) != hipSuccess) return e2;
+            if ((e2 = trace_wide_multi(W, d_jw, st.job, 0, reinterpret_cast<const float *>(st.rays), pool, 0,
+                                       pp.prim.as<uint32_t>(), pp.thit.as<float>(), pp.bary.as<float>(), err,
+                                       pp.queue.as<uint32_t>(), st.counters + kCntQueue, stream)) != hipSuccess)
+                return e2;
+            if ((e2 = trace_wide_multi(W, d_jw, st.job, 1, reinterpret_cast<const float *>(st.srays), 2 * pool, 1,
+                                       pp.sprim.as<uint32_t>(), nullptr, nullptr, err, pp.squeue.as<uint32_t>(),
+                                       st.counters + kCntSQueue, stream)) != hipSuccess)
+                return e2;
+            shade_multi_kernel<<<grid, kBlock, 0, stream>>>(st, P);
+            count_live_kernel<<<grid, kBlock, 0, stream>>>(st.path_id, pool, st.counters + kCntActive, st.srays,
+                                                           st.counters + kCntShadow);
+            round_end_kernel<<<1, 64, 0, stream>>>(st.counters);
+            if ((e2 = hipGetLastError()) != hipSuccess) return e2;
+        }
+        return hipSuccess;
+    };
+    if (use_graph) {
+        PT_CK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t ec = enqueue_rounds();
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(stream, &g);
+        graph = g;
+        PT_CK(ec);
+        PT_CK(ee);
+        PT_CK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    uint32_t host_cnt[16] = {};
+    uint32_t launches = 0;
+    // Every path ends within max_depth + 2 rounds of its start, so the pool drains in
+    // at most (paths / pool + 1) * (max depth + 2) rounds.
+    uint32_t dmax = 0;
+    for (const auto &b : jobs) dmax = std::max(dmax, b.max_depth);
+    const uint64_t round_cap = (total / pool + 2) * (uint64_t(dmax) + 3) + 16;
+    for (;;) {
+        if (use_graph) PT_CK(hipGraphLaunch(exec, stream));
+        else PT_CK(enqueue_rounds());
+        ++launches;
+        PT_CK(hipMemcpyAsync(host_cnt, st.counters, 64, hipMemcpyDeviceToHost, stream));
+        PT_CK(hipStreamSynchronize(stream));
+        if (host_cnt[kCntDone]) break;
+        if (uint64_t(launches) * rounds_per_launch > round_cap) return hipErrorUnknown;
+    }
+    PT_CK(hipEventRecord(ev[1], stream));
+    schedule_kernel<<<nj, kSchedBlock, 0, stream>>>(d_jobs, st.rec_round, d_rounds, err);
+    PT_CK(hipGetLastError());
+    PT_CK(hipEventRecord(ev[2], stream));
+    for (uint32_t j = 0; j < nj; ++j) {
+        const uint32_t wh = jd[j].S.width * jd[j].S.height;
+        film_kernel<<<(wh + kFilmWaves - 1) / kFilmWaves, kFilmWaves * kWave, 0, stream>>>(
+            jd[j].S, st.rec + jd[j].base, st.rec_round + jd[j].base, jd[j].spp, jobs[j].film_rgb, jobs[j].film_aux);
+        PT_CK(hipGetLastError());
+    }
+    PT_CK(hipEventRecord(ev[3], stream));
+    if (stats) {
+        stats->job_rounds.assign(nj, 0);
+        PT_CK(hipMemcpyAsync(stats->job_rounds.data(), d_rounds, nj * 4, hipMemcpyDeviceToHost, stream));
+    }
+    PT_CK(hipStreamSynchronize(stream));
+    if (stats) {
+        float a = 0, b = 0, c = 0, t = 0;
+        hipEventElapsedTime(&a, ev[0], ev[1]);
+        hipEventElapsedTime(&b, ev[1], ev[2]);
+        hipEventElapsedTime(&c, ev[2], ev[3]);
+        hipEventElapsedTime(&t, ev[0], ev[3]);
+        stats->ms_rounds = a;
+        stats->ms_schedule = b;
+        stats->ms_film = c;
+        stats->ms_total = t;
+        stats->rounds = host_cnt[kCntRound];
+        uint64_t acc[2];
+        memcpy(acc, host_cnt + kCntAccRays, sizeof(acc));
+        stats->rays = acc[0];
+        stats->shadow_rays = acc[1];
+        stats->paths = total;
+        stats->launches = launches;
+    }
     return hipSuccess;
 #undef PT_CK
 }

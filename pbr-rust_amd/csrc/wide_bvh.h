@@ -53,6 +53,13 @@ hipError_t trace_wide(const WideTrace &B, const float *rays, uint32_t n, int any
                       float *t_out, float *bary_out, uint32_t *err, unsigned long long *counts, uint32_t *queue,
                       uint32_t *queue_len, hipStream_t stream);
 
+// As trace_wide, for rays of several jobs: ray i is traced against jobs[ray_job[i >> job_shift]]
+// (nodes, order, tri_leaf, prim_order, num_nodes, num_prims); B supplies the spill area.
+hipError_t trace_wide_multi(const WideTrace &B, const WideTrace *jobs, const uint32_t *ray_job, uint32_t job_shift,
+                            const float *rays, uint32_t n, int any_hit, uint32_t *prim_out, float *t_out,
+                            float *bary_out, uint32_t *err, uint32_t *queue, uint32_t *queue_len,
+                            hipStream_t stream);
+
 constexpr uint32_t kLdsStack = 12;
 
 }  // namespace pbrtm

@@ -264,7 +264,9 @@ __global__ __launch_bounds__(kWideBlock) void compact_rays_kernel(const float4 *
     if (act) queue[s_base + pre] = i;
 }
 
-template <bool COUNT>
+// MULTI: every ray selects its own tree, jw[ray_job[i >> job_shift]] (jobs of a
+// batched render); the spill area stays the launch's.
+template <bool COUNT, bool MULTI>
 __global__ __launch_bounds__(kWideBlock) void trace_wide_kernel(WideTrace B, const float4 *__restrict__ rays,
                                                                 const uint32_t *__restrict__ queue,
                                                                 const uint32_t *__restrict__ queue_len, int any_hit,
@@ -272,11 +274,28 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide_kernel(WideTrace B, con
                                                                 float *__restrict__ t_out,
                                                                 float *__restrict__ bary_out,
                                                                 uint32_t *__restrict__ err,
-                                                                unsigned long long *__restrict__ counts) {
+                                                                unsigned long long *__restrict__ counts,
+                                                                const WideTrace *__restrict__ jw,
+                                                                const uint32_t *__restrict__ ray_job,
+                                                                uint32_t job_shift) {
     __shared__ uint2 s_stack[kLdsStack][kWideBlock];
     const uint32_t qi = blockIdx.x * kWideBlock + threadIdx.x;
     if (qi >= *queue_len) return;
     const uint32_t i = queue[qi];
+    const WNode *nodes = B.nodes;
+    const uint32_t *order = B.order;
+    const float *tri_leaf = B.tri_leaf;
+    const uint32_t *prim_order = B.prim_order;
+    uint32_t num_nodes = B.num_nodes, num_prims = B.num_prims;
+    if (MULTI) {
+        const WideTrace &J = jw[ray_job[i >> job_shift]];
+        nodes = J.nodes;
+        order = J.order;
+        tri_leaf = J.tri_leaf;
+        prim_order = J.prim_order;
+        num_nodes = J.num_nodes;
+        num_prims = J.num_prims;
+    }
     const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
     const f3 o = {r0.x, r0.y, r0.z}, d = {r1.x, r1.y, r1.z};
     float best_t = r0.w;
@@ -291,12 +310,12 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide_kernel(WideTrace B, con
         uint32_t node = 0, pmask = 0xFFu;
         bool done = false;
         while (!done) {
-            if (node >= B.num_nodes) {  // only a corrupt layout gets here
+            if (node >= num_nodes) {  // only a corrupt layout gets here
                 atomicOr(err, uint32_t(kErrStack));
                 break;
             }
-            const WNode *W = B.nodes + node;
-            const uint32_t perm = B.order[node * 8 + oct];
+            const WNode *W = nodes + node;
+            const uint32_t perm = order[node * 8 + oct];
             if (COUNT) { ++visits; boxes += kWide; }
             // all eight child boxes (12 x 16-byte loads), hit mask in slot order
             const float4 *q = reinterpret_cast<const float4 *>(W);
@@ -338,19 +357,19 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide_kernel(WideTrace B, con
                 const uint32_t ref0 = W->ref[s][0];
                 if (ref0 & kWideLeaf) {
                     const uint32_t first = ref0 & ~kWideLeaf, cnt = W->ref[s][1];
-                    if (uint64_t(first) + cnt > B.num_prims) {
+                    if (uint64_t(first) + cnt > num_prims) {
                         atomicOr(err, uint32_t(kErrStack));
                         done = true;
                         break;
                     }
                     for (uint32_t k = first; k < first + cnt; ++k) {
-                        const float4 *t4 = reinterpret_cast<const float4 *>(B.tri_leaf) + 3ull * k;
+                        const float4 *t4 = reinterpret_cast<const float4 *>(tri_leaf) + 3ull * k;
                         const float4 q0 = t4[0], q1 = t4[1], q2 = t4[2];
                         if (COUNT) ++tests;
                         TriHit h;
                         if (tri_hit(o, d, best_t, f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, f3{q1.z, q1.w, q2.x},
                                     &h)) {
-                            best = B.prim_order[k];
+                            best = prim_order[k];
                             hit = h;
                             best_t = h.t;
                             if (any_hit) { done = true; break; }
@@ -453,11 +472,27 @@ hipError_t trace_wide(const WideTrace &B, const float *rays, uint32_t n, int any
     const float4 *r = reinterpret_cast<const float4 *>(rays);
     compact_rays_kernel<<<grid, kWideBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
     if (counts)
-        trace_wide_kernel<true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
-                                                                 bary_out, err, counts);
+        trace_wide_kernel<true, false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
+                                                                        t_out, bary_out, err, counts, nullptr,
+                                                                        nullptr, 0);
     else
-        trace_wide_kernel<false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
-                                                                  bary_out, err, nullptr);
+        trace_wide_kernel<false, false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
+                                                                         t_out, bary_out, err, nullptr, nullptr,
+                                                                         nullptr, 0);
+    return hipGetLastError();
+}
+
+hipError_t trace_wide_multi(const WideTrace &B, const WideTrace *jobs, const uint32_t *ray_job, uint32_t job_shift,
+                            const float *rays, uint32_t n, int any_hit, uint32_t *prim_out, float *t_out,
+                            float *bary_out, uint32_t *err, uint32_t *queue, uint32_t *queue_len,
+                            hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (n + kWideBlock - 1) / kWideBlock;
+    const float4 *r = reinterpret_cast<const float4 *>(rays);
+    compact_rays_kernel<<<grid, kWideBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
+    trace_wide_kernel<false, true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
+                                                                    bary_out, err, nullptr, jobs, ray_job,
+                                                                    job_shift);
     return hipGetLastError();
 }
 

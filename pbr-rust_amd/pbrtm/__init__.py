@@ -34,8 +34,9 @@ EXPORTS = (
     "pbrtm_bvh_export_canonical", "pbrtm_bvh_intersect", "pbrtm_ao_bounce", "pbrtm_gen_soup",
     "pbrtm_gen_camera_rays", "pbrtm_scene_create", "pbrtm_scene_free", "pbrtm_render", "pbrtm_rgb_to_sigmoid",
     "pbrtm_srgb_matrices", "pbrtm_srgb8", "pbrtm_write_png", "pbrtm_pbrt_parse", "pbrtm_pbrt_desc",
-    "pbrtm_pbrt_render_params", "pbrtm_pbrt_output", "pbrtm_pbrt_free",
+    "pbrtm_pbrt_render_params", "pbrtm_pbrt_output", "pbrtm_pbrt_free", "pbrtm_render_batch",
 )
+BATCH_NO_GRAPH = 2
 
 
 class BuildStats(ctypes.Structure):
@@ -76,6 +77,25 @@ class RenderStats(ctypes.Structure):
     _fields_ = [("ms_total", c_double), ("ms_trace", c_double), ("ms_shadow", c_double), ("ms_shade", c_double),
                 ("ms_film", c_double), ("rounds", c_uint64), ("rays", c_uint64), ("shadow_rays", c_uint64),
                 ("paths", c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("scene", c_void_p), ("params", RenderParams), ("film_rgb", c_void_p), ("film_aux", c_void_p),
+                ("rounds", c_uint32), ("reserved", c_uint32)]
+
+
+class BatchParams(ctypes.Structure):
+    _fields_ = [("pool_size", c_uint32), ("rounds_per_launch", c_uint32), ("flags", c_uint32),
+                ("reserved", c_uint32)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("ms_total", c_double), ("ms_rounds", c_double), ("ms_schedule", c_double), ("ms_film", c_double),
+                ("rounds", c_uint64), ("rays", c_uint64), ("shadow_rays", c_uint64), ("paths", c_uint64),
+                ("pool_size", c_uint32), ("launches", c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -129,6 +149,7 @@ def load(path: str = LIB_PATH):
         "pbrtm_pbrt_render_params": (c_int, [p, POINTER(RenderParams)]),
         "pbrtm_pbrt_output": (c_char_p, [p]),
         "pbrtm_pbrt_free": (None, [p]),
+        "pbrtm_render_batch": (c_int, [p, POINTER(Job), c_uint32, POINTER(BatchParams), POINTER(BatchStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -225,6 +246,25 @@ class Context:
 
     def scene(self, sc) -> "Scene":
         return Scene(self, sc)
+
+    def render_batch(self, jobs, pool_size=0, rounds_per_launch=0, graph=True, aux=False):
+        """jobs: [(Scene, spp, max_depth, pool_size)]. Returns ([film_rgb], [film_aux or None],
+        [one-by-one round count per job], stats dict)."""
+        arr = (Job * len(jobs))()
+        rgbs, auxs = [], []
+        for k, (scene, spp, depth, vpool) in enumerate(jobs):
+            rgb = np.zeros((scene.height, scene.width, 3), np.float32)
+            ax = np.zeros((scene.height, scene.width, 4), np.float32) if aux else None
+            rgbs.append(rgb)
+            auxs.append(ax)
+            arr[k].scene = scene.h
+            arr[k].params = RenderParams(spp, depth, vpool, 0)
+            arr[k].film_rgb = _ptr(rgb)
+            arr[k].film_aux = _ptr(ax)
+        bp = BatchParams(pool_size, rounds_per_launch, 0 if graph else BATCH_NO_GRAPH, 0)
+        st = BatchStats()
+        self.check(self.lib.pbrtm_render_batch(self.h, arr, len(jobs), ctypes.byref(bp), ctypes.byref(st)))
+        return rgbs, auxs, [arr[k].rounds for k in range(len(jobs))], st.as_dict()
 
     def scene_from_file(self, f: "PbrtFile") -> "Scene":
         """A scene straight from a parsed .pbrt file's description (no Python copy)."""

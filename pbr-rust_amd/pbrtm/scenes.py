@@ -14,6 +14,17 @@ ground plane; each sphere's material is drawn from {diffuse rgb U[0.1, 0.9]^3,
 conductor Cu roughness U[0.05, 0.4], dielectric eta 1.5 roughness 0}; 32
 emissive quads (64 triangles) at y = 5 with L = rgb(1, 1, 1), scale 10. At
 n = 32, tess = 32 it holds 2,097,218 triangles.
+
+soup_room: the HLBVH benchmark's seeded triangle soup (centroids uniform in the
+unit cube, log-normal edges) inside a closed room (x, y in [-0.5, 1.5], z in
+[-1.5, 1.5]; grey floor, ceiling and end walls, red left and green right wall)
+lit by one emissive quad under the ceiling (rgb L 1, scale 8); soup triangle k
+takes diffuse material k mod 8 (albedo from PCG32). The camera sits inside the
+room at z = -1.3 looking at the cube's centre.
+
+batch_jobs: the mixed batch of the batched-render benchmark (per job: a square
+resolution uniform in [lo, hi], spp from {1, 4, 16, 64}, max depth 1..8 and a
+soup_room of 10^4..10^6 triangles, log-uniform), drawn from one PCG32 stream.
 """
 import numpy as np
 
@@ -194,3 +205,40 @@ def sphere_grid(width=1920, height=1080, n=32, tess=32, seed=0xC0FFEE, spp=64, m
         "tri": np.ascontiguousarray(tri), "tri_material": tri_mat, "tri_light": tri_light,
         "materials": materials, "lights": [{"rgb": (1.0, 1.0, 1.0), "scale": 10.0}],
     }
+
+
+def soup_room(n_tris, width=256, height=256, seed=0x5EED, spp=4, max_depth=5):
+    from . import gen_soup
+    soup = gen_soup(n_tris, seed=seed).reshape(-1, 9)
+    rng = Pcg32(seed ^ 0x9E3779B9)
+    materials = [{"type": "diffuse", "rgb": tuple(0.1 + 0.8 * rng.uniform() for _ in range(3))} for _ in range(8)]
+    grey, red, green = len(materials), len(materials) + 1, len(materials) + 2
+    materials += [{"type": "diffuse", "rgb": GREY}, {"type": "diffuse", "rgb": RED}, {"type": "diffuse", "rgb": GREEN}]
+    walls = box(-0.5, 1.5, -0.5, 1.5, -1.5, 1.5)
+    wall_mat = [grey, grey, grey, grey, grey, grey, grey, grey, red, red, green, green]
+    light = quad((0.25, 1.49, 0.25), (0.75, 1.49, 0.25), (0.75, 1.49, 0.75), (0.25, 1.49, 0.75))
+    tri = np.concatenate([soup, np.array(walls, np.float32), np.array(light, np.float32)]).astype(np.float32)
+    tri_mat = np.concatenate([np.arange(n_tris, dtype=np.uint32) % 8, np.array(wall_mat, np.uint32),
+                              np.full(2, grey, np.uint32)])
+    tri_light = np.full(len(tri), -1, np.int32)
+    tri_light[-2:] = 0
+    return {
+        "eye": (0.5, 0.5, -1.3), "look": (0.5, 0.5, 0.5), "up": (0.0, 1.0, 0.0), "fov": 50.0,
+        "width": width, "height": height, "spp": spp, "max_depth": max_depth,
+        "tri": np.ascontiguousarray(tri), "tri_material": tri_mat, "tri_light": tri_light,
+        "materials": materials, "lights": [{"rgb": (1.0, 1.0, 1.0), "scale": 8.0}],
+    }
+
+
+def batch_jobs(n_jobs=64, seed=0xBA7C4, res=(64, 1024), tris=(10_000, 1_000_000)):
+    """[(scene dict, spp, max_depth)] of the mixed batch."""
+    rng = Pcg32(seed)
+    out = []
+    for k in range(n_jobs):
+        r = res[0] + min(int(rng.uniform() * (res[1] - res[0] + 1)), res[1] - res[0])
+        spp = (1, 4, 16, 64)[min(int(rng.uniform() * 4), 3)]
+        depth = 1 + min(int(rng.uniform() * 8), 7)
+        lo, hi = np.log(tris[0]), np.log(tris[1])
+        n = int(round(float(np.exp(lo + (hi - lo) * rng.uniform()))))
+        out.append((soup_room(n, r, r, seed=seed + 17 * k + 1, spp=spp, max_depth=depth), spp, depth))
+    return out
