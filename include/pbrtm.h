@@ -218,6 +218,27 @@ void pbrtm_scene_free(pbrtm_scene *scene);
 int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *scene, const pbrtm_render_params *params, float *film_rgb,
                  float *film_aux, pbrtm_render_stats *stats);
 
+/* Differentiable rendering (no counterpart in the reference): the loss
+ * sum over pixels and channels of (film - target)^2 of pbrtm_render's film and
+ * its gradient by path replay. The forward render keeps every path's final
+ * spectral radiance; a second render of the same paths (same sampler streams)
+ * accumulates dLoss/d(diffuse albedo RGB) per material (through the
+ * RGB-to-spectrum table) and dLoss/d(scale) per emitter (pbrtm_area_light).
+ * The per-sample estimator is differentiated with its MIS weights' light pmf
+ * attached and the light choice's 1/pmf held (the choice's boundary term
+ * cancels that derivative in expectation, so the result is what central
+ * differences over many samples measure); Russian roulette's probability is
+ * held. d_albedo [num_materials][3] (zero for non-diffuse materials) and
+ * d_scale [num_lights] may be NULL; film_rgb (may be NULL) receives the film.
+ * Host pointers; at most 255 emitters. */
+typedef struct pbrtm_grad_stats {
+    double ms_forward, ms_adjoint, ms_replay, ms_total;
+    uint64_t rounds, rays, shadow_rays, paths;
+} pbrtm_grad_stats;
+
+int pbrtm_render_grad(pbrtm_ctx *ctx, pbrtm_scene *scene, const pbrtm_render_params *params, const float *target_rgb,
+                      float *film_rgb, double *loss, float *d_albedo, float *d_scale, pbrtm_grad_stats *stats);
+
 /* Batched rendering (no counterpart in the reference, which renders one scene
  * per process): every job's paths share one pool of path slots, so many small
  * renders fill the device together. Each job's film is bit-identical to

@@ -158,7 +158,7 @@ class OrcScene(ctypes.Structure):
                 ("pool_size", c_uint32), ("num_tris", c_uint64), ("tri", c_void_p), ("tri_material", c_void_p),
                 ("tri_light", c_void_p), ("tri_reverse", c_void_p), ("num_materials", c_uint32),
                 ("materials", POINTER(OrcMaterial)), ("num_lights", c_uint32), ("lights", POINTER(OrcLight)),
-                ("num_xforms", c_uint32), ("xforms", c_void_p), ("tri_xform", c_void_p)]
+                ("num_xforms", c_uint32), ("xforms", c_void_p), ("tri_xform", c_void_p), ("select_scale", c_void_p)]
 
 
 METALS = {"Ag": 0, "Al": 1, "Au": 2, "Cu": 3}
@@ -169,6 +169,8 @@ def _pt_sigs(lib):
     sig = {
         "orc_render": (c_int, [POINTER(OrcScene), c_int, p, p, p]),
         "orc_render_paths": (c_int, [POINTER(OrcScene), c_int, p, p, p]),
+        "orc_rgb_to_sigmoid_jacobian": (c_int, [p, p]),
+        "orc_render_grad": (c_int, [POINTER(OrcScene), c_int, p, p, POINTER(ctypes.c_double), p, p, p, c_int]),
         "orc_rgb_to_sigmoid": (c_int, [p, p]),
         "orc_srgb_matrices": (c_int, [p, p, p]),
         "orc_filter_sample": (c_int, [p, p]),
@@ -195,8 +197,21 @@ def _scene_struct(sc, pool_size):
     mat = np.ascontiguousarray(sc["tri_material"], dtype=np.uint32)
     lig = np.ascontiguousarray(sc["tri_light"], dtype=np.int32)
     rev = np.ascontiguousarray(sc.get("tri_reverse", np.zeros(len(tri), np.uint8)), dtype=np.uint8)
-    mats = (OrcMaterial * len(sc["materials"]))()
-    for i, m in enumerate(sc["materials"]):
+    if sc.get("materials") is None:  # all diffuse, albedo [n][3]
+        alb = np.ascontiguousarray(sc["albedo"], dtype=np.float32).reshape(-1, 3)
+        marr = np.zeros(len(alb), dtype=np.dtype([("type", np.uint32), ("rgb", np.float32, 3), ("metal", np.uint32),
+                                                  ("uroughness", np.float32), ("vroughness", np.float32),
+                                                  ("remap_roughness", np.uint32), ("eta", np.float32),
+                                                  ("eta_named", np.uint32)]))
+        assert marr.dtype.itemsize == ctypes.sizeof(OrcMaterial)
+        marr["rgb"] = alb
+        mats = marr.ctypes.data_as(POINTER(OrcMaterial))
+        materials = []
+    else:
+        marr = None
+        materials = sc["materials"]
+        mats = (OrcMaterial * len(materials))()
+    for i, m in enumerate(materials):
         if m["type"] == "diffuse":
             mats[i].type = 0
             mats[i].rgb[:] = list(m["rgb"])
@@ -223,7 +238,7 @@ def _scene_struct(sc, pool_size):
     s.pool_size = pool_size
     s.num_tris = len(tri)
     s.tri, s.tri_material, s.tri_light, s.tri_reverse = _p(tri), _p(mat), _p(lig), _p(rev)
-    s.num_materials = len(sc["materials"])
+    s.num_materials = len(materials) if marr is None else len(marr)
     s.materials = mats
     s.num_lights = len(sc["lights"])
     s.lights = lights
@@ -233,7 +248,10 @@ def _scene_struct(sc, pool_size):
     s.num_xforms = len(xf)
     s.xforms = _p(xf) if len(xf) else None
     s.tri_xform = _p(txf)
-    return s, (tri, mat, lig, rev, mats, lights, xf, txf)
+    sel = sc.get("select_scale")
+    sel = None if sel is None else np.ascontiguousarray(sel, dtype=np.float32)
+    s.select_scale = _p(sel)
+    return s, (tri, mat, lig, rev, mats, lights, xf, txf, sel, marr)
 
 
 def render(sc, pool_size=2 * 1024 * 1024, threads=8):
@@ -270,6 +288,41 @@ def render_paths(sc, pool_size=2 * 1024 * 1024, threads=8):
     if rc != 0:
         raise RuntimeError(f"orc_render_paths failed ({rc})")
     return start, finish, {"rounds": int(st[0]), "rays": int(st[1]), "shadow_rays": int(st[2]), "paths": int(st[3])}
+
+
+def render_grad(sc, target, pool_size=2 * 1024 * 1024, threads=8, attach_selection=False):
+    """Loss sum((film - target)^2) and its FP64 path-replay gradient: returns
+    (loss, d_rgb [num_materials][3], d_scale [num_lights], film)."""
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        _pt_sigs(lib)
+        lib._pt_ready = True
+    s, keep = _scene_struct(sc, pool_size)
+    h, w = sc["height"], sc["width"]
+    tgt = np.ascontiguousarray(target, np.float32)
+    assert tgt.shape == (h, w, 3)
+    film = np.zeros((h, w, 3), np.float32)
+    loss = ctypes.c_double(0)
+    d_rgb = np.zeros((s.num_materials, 3), np.float64)
+    d_scale = np.zeros(len(sc["lights"]), np.float64)
+    st = np.zeros(5, np.uint64)
+    rc = lib.orc_render_grad(ctypes.byref(s), threads, _p(tgt), _p(film), ctypes.byref(loss), _p(d_rgb),
+                             _p(d_scale), _p(st), int(attach_selection))
+    if rc != 0:
+        raise RuntimeError(f"orc_render_grad failed ({rc})")
+    return loss.value, d_rgb, d_scale, film
+
+
+def rgb_to_sigmoid_jacobian(rgb):
+    """d(c0, c1, c2)/d(r, g, b) of the RGB-to-spectrum table lookup, [3][3]."""
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        _pt_sigs(lib)
+        lib._pt_ready = True
+    a = np.asarray(rgb, np.float32)
+    out = np.zeros(9, np.float64)
+    lib.orc_rgb_to_sigmoid_jacobian(_p(a), _p(out))
+    return out.reshape(3, 3)
 
 
 def rgb_to_sigmoid(rgb):

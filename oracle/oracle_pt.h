@@ -51,6 +51,9 @@ typedef struct orc_scene {
     uint32_t num_xforms;          /* object-to-world 4x4 row-major matrices */
     const float *xforms;
     const uint32_t *tri_xform;    /* [n] index or UINT32_MAX = identity; nullable */
+    const float *select_scale;    /* [num_lights] nullable: emitter scales the light choice's cdf
+                                     is built from (test knob: freezes the choice under a change of
+                                     the scales; pmf values in the estimator stay the scene's) */
 } orc_scene;
 
 /* film_rgb: [height][width][3] = RGBFilm::get_pixel_rgb; aux (nullable):
@@ -63,9 +66,21 @@ int orc_render(const orc_scene *scene, int threads, float *film_rgb, float *aux,
 int orc_render_paths(const orc_scene *scene, int threads, uint32_t *path_start, uint32_t *path_finish,
                      uint64_t *stats);
 
+/* Loss and gradient of the path-traced film (FP64 path replay): loss = sum over
+ * pixels and channels of (film - target)^2; d_rgb [num_materials][3] is
+ * dLoss/d(diffuse albedo RGB) (0 for other materials), d_scale [num_lights]
+ * dLoss/d(emitter scale). The fixed-seed estimator is differentiated with the
+ * MIS weights' light pmf attached and the light choice's 1/pmf held, which is
+ * what central differences over many samples measure (the choice's boundary
+ * crossings cancel that term in expectation); attach_selection != 0
+ * differentiates that 1/pmf as well. film_rgb receives the film. */
+int orc_render_grad(const orc_scene *scene, int threads, const float *target, float *film_rgb, double *loss,
+                    double *d_rgb, double *d_scale, uint64_t *stats, int attach_selection);
+
 /* Intermediate tables, for tests: the RGB-to-spectrum sigmoid coefficients of an
  * albedo RGB, the sRGB rgb_from_xyz matrix, the filter sample of a 2D u. */
 int orc_rgb_to_sigmoid(const float rgb[3], float out3[3]);
+int orc_rgb_to_sigmoid_jacobian(const float rgb[3], double J9[9]); /* dc_q/drgb_k at [3q + k] */
 int orc_srgb_matrices(float xyz_from_rgb9[9], float rgb_from_xyz9[9], float white_xy2[2]);
 int orc_filter_sample(const float u[2], float out3[3]);
 float orc_spectrum_eval(int which, float lambda); /* 0-2 CIE XYZ, 3 D65 (normalised), 4+ metals eta/k */

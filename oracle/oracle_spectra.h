@@ -374,6 +374,40 @@ class Table {
         return Sigmoid{c[0], c[1], c[2]};
     }
 
+    // d(c0, c1, c2)/d(r, g, b) of lookup(): J[q][k] = dc_q / drgb_k, from the
+    // derivative of the trilinear interpolation and of the cell coordinates
+    // x = rgb[(m+1)%3] (res-1) / z, y = rgb[(m+2)%3] (res-1) / z, z = rgb[m].
+    // Grey inputs take the general branch (the closed form holds only on the diagonal).
+    void jacobian(const float rgb[3], double J[3][3]) {
+        int mc = (rgb[0] > rgb[1]) ? ((rgb[0] > rgb[2]) ? 0 : 2) : ((rgb[1] > rgb[2]) ? 1 : 2);
+        double z = rgb[mc];
+        double x = double(rgb[(mc + 1) % 3] * (kRes - 1)) / z;
+        double y = double(rgb[(mc + 2) % 3] * (kRes - 1)) / z;
+        int xi = std::min((int)x, kRes - 2), yi = std::min((int)y, kRes - 2);
+        int zi = (int)find_interval(kRes, [&](size_t i) { return z_nodes_[i] < z; });
+        double dx = x - xi, dy = y - yi;
+        double dzn = z_nodes_[zi + 1] - z_nodes_[zi];
+        double dz = (z - z_nodes_[zi]) / dzn;
+        const auto &c00 = col(mc, yi, xi), &c10 = col(mc, yi, xi + 1);
+        const auto &c01 = col(mc, yi + 1, xi), &c11 = col(mc, yi + 1, xi + 1);
+        auto lerp = [](double t, double a, double b) { return (1 - t) * a + t * b; };
+        for (int q = 0; q < 3; ++q) {
+            auto co = [&](int ddx, int ddy, int ddz) -> double {
+                const auto &cc = ddy ? (ddx ? c11 : c01) : (ddx ? c10 : c00);
+                return double(float(cc[zi + ddz][q]));
+            };
+            const double gx = lerp(dz, lerp(dy, co(1, 0, 0) - co(0, 0, 0), co(1, 1, 0) - co(0, 1, 0)),
+                                   lerp(dy, co(1, 0, 1) - co(0, 0, 1), co(1, 1, 1) - co(0, 1, 1)));
+            const double gy = lerp(dz, lerp(dx, co(0, 1, 0) - co(0, 0, 0), co(1, 1, 0) - co(1, 0, 0)),
+                                   lerp(dx, co(0, 1, 1) - co(0, 0, 1), co(1, 1, 1) - co(1, 0, 1)));
+            const double gz = lerp(dy, lerp(dx, co(0, 0, 1), co(1, 0, 1)), lerp(dx, co(0, 1, 1), co(1, 1, 1))) -
+                              lerp(dy, lerp(dx, co(0, 0, 0), co(1, 0, 0)), lerp(dx, co(0, 1, 0), co(1, 1, 0)));
+            J[q][mc] = gx * (-x / z) + gy * (-y / z) + gz / dzn;
+            J[q][(mc + 1) % 3] = gx * (kRes - 1) / z;
+            J[q][(mc + 2) % 3] = gy * (kRes - 1) / z;
+        }
+    }
+
   private:
     Buffer buf_;
     double z_nodes_[kRes];

@@ -461,6 +461,7 @@ struct Scene {
     std::vector<Material> mats;
     std::vector<Light> lights;
     std::vector<float> light_pmf, light_cdf;
+    std::vector<float> select_cdf;  // the cdf the NEE light choice searches (= light_cdf unless overridden)
     Spectra *sp;
     PLSpectrum metals[8], glass_bk7;
     M3 out_from_sensor;
@@ -596,6 +597,18 @@ void build_scene(Scene &sc, const orc_scene &d) {
         sc.light_cdf.resize(n);
         sc.light_cdf[0] = sc.light_pmf[0];
         for (size_t i = 1; i < n; ++i) sc.light_cdf[i] = sc.light_cdf[i - 1] + sc.light_pmf[i];
+        sc.select_cdf = sc.light_cdf;
+        if (d.select_scale) {
+            double ssum = 0.0;
+            std::vector<double> ws(n);
+            for (size_t i = 0; i < n; ++i) {
+                const uint32_t g = uint32_t(d.tri_light[sc.lights[i].tri]);
+                ws[i] = double(w[i]) * double(d.select_scale[g]) / double(d.lights[g].scale);
+                ssum += ws[i];
+            }
+            float acc = 0;
+            for (size_t i = 0; i < n; ++i) sc.select_cdf[i] = acc = acc + float(ws[i] / ssum);
+        }
     }
     sc.out_from_sensor = sc.sp->rgb_from_xyz;  // times the identity white balance
 }
@@ -1269,6 +1282,33 @@ struct PathSlot {
     bool nee = false;
     S4 nee_L;
     V3 so[2], sd[2];
+    // gradient replay: dLoss/dL per wavelength, the path's final L, the pending
+    // NEE's adjoint-weighted value / MIS weight / light group, and this phase's
+    // records (reduced in slot order after each parallel phase)
+    double g[4] = {0, 0, 0, 0};
+    S4 Ltot;
+    double nee_G = 0, nee_w = 0;
+    int nee_group = -1;
+    int rec_group = -1;
+    double rec_a = 0, rec_k = 0;
+    int rec_mat = -1;
+    double rec_c[3] = {0, 0, 0};
+};
+
+// Gradient mode of render(). Forward pass: Lfin / pdffin receive each path's
+// final L and wavelength pdfs. Replay: A holds the per-pixel adjoint of the
+// sensor RGB (dLoss/d(rgb_sum / weight_sum) through rgb_from_xyz), and the
+// replay accumulates d_coef (per material, sigmoid coefficients), a (per light
+// group, scale-linear part) and k (light-pmf part).
+struct GradIO {
+    std::vector<S4> *Lfin = nullptr, *pdffin = nullptr;  // forward
+    const std::vector<S4> *Ltot = nullptr, *pdf = nullptr;  // replay
+    const std::vector<double> *A = nullptr;
+    std::vector<double> d_coef, a;
+    double k = 0;
+    std::vector<double> d_rgb, d_scale;  // per material [3], per light group
+    bool attach_selection = false;       // differentiate the light choice's 1/pmf too
+    double M[9] = {};                    // out_from_sensor, for the adjoint
 };
 
 void parallel(size_t n, int threads, const std::function<void(size_t)> &fn) {
@@ -1306,7 +1346,7 @@ inline void sensor_rgb(const Scene &sc, const S4 &L, const Lambda &lam, float rg
 // path_start / path_finish (nullable, [W*H*spp] by path id): the round a path's
 // first ray is traced in and the round it is recorded in the film.
 void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64_t *stats,
-            uint32_t *path_start = nullptr, uint32_t *path_finish = nullptr) {
+            uint32_t *path_start = nullptr, uint32_t *path_finish = nullptr, GradIO *gio = nullptr) {
     Scene sc;
     build_scene(sc, d);
     const uint32_t W = d.width, H = d.height;
@@ -1318,6 +1358,22 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
     std::vector<float> rgb_sum(size_t(W) * H * 3, 0.f), w_sum(size_t(W) * H, 0.f);
     uint64_t next_id = 0, n_rays = 0, n_shadow = 0, n_done = 0, round = 0;
     uint32_t gen_round = 0;  // the round whose ray cast first traces the paths gen() starts
+    const bool replay = gio && gio->A;
+    if (replay) {
+        gio->d_coef.assign(size_t(d.num_materials) * 3, 0.0);
+        gio->a.assign(d.num_lights, 0.0);
+        gio->k = 0;
+    }
+    if (gio && gio->Lfin) {
+        gio->Lfin->assign(total, s4(0));
+        gio->pdffin->assign(total, s4(0));
+    }
+    auto group_of = [&](uint32_t light) { return int(d.tri_light[sc.lights[light].tri]); };
+    auto dotg = [](const double *g, const S4 &c) {
+        double v = 0;
+        for (int i = 0; i < kNS; ++i) v += g[i] * double(c.v[i]);
+        return v;
+    };
     const float inf = std::numeric_limits<float>::infinity();
 
     // generate_new_path (wavefront_path.cu:265-308)
@@ -1357,6 +1413,18 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
             p.L = s4(0);
             p.beta = s4(1);
             p.depth = 0;
+            if (replay) {
+                // dLoss/dL_i = w / 4 * sum_c A_c x_c(lambda_i) / pdf_i (sensor_rgb, film add_sample)
+                const S4 &pf = (*gio->pdf)[id];
+                const double *A = &(*gio->A)[3 * size_t(p.pixel)];
+                for (int i = 0; i < kNS; ++i) {
+                    double v = 0;
+                    if (pf.v[i] != 0)
+                        for (int c = 0; c < 3; ++c) v += A[c] * double(sc.sp->cie[c](p.lam.l[i])) / double(pf.v[i]);
+                    p.g[i] = double(fw) * v / kNS;
+                }
+                p.Ltot = (*gio->Ltot)[id];
+            }
         });
     };
 
@@ -1387,13 +1455,35 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                     int sr = orc_trace_ray(sc.bvh, &p.so[r].x, &p.sd[r].x, 0.6f, 1, &sp, nullptr, nullptr);
                     occluded = sr == 1;
                 }
-                if (!occluded) p.L = p.L + p.nee_L;
+                if (!occluded) {
+                    p.L = p.L + p.nee_L;
+                    if (replay) {
+                        // NEE, C = beta f cos Le w(pmf p) / (pmf p). The MIS weight's pmf is
+                        // attached: dC/dpmf = 2 C (1 - w) / pmf, so a += G (3 - 2w),
+                        // k += 2 G (1 - w). The 1/pmf of the light choice is held (its
+                        // derivative is cancelled in expectation by the choice's boundary
+                        // term); attach_selection adds it: dC/dpmf -= C / pmf.
+                        p.rec_group = p.nee_group;
+                        if (gio->attach_selection) {
+                            p.rec_a = p.nee_G * 2.0 * (1.0 - p.nee_w);
+                            p.rec_k = p.nee_G * (1.0 - 2.0 * p.nee_w);
+                        } else {
+                            p.rec_a = p.nee_G * (3.0 - 2.0 * p.nee_w);
+                            p.rec_k = p.nee_G * 2.0 * (1.0 - p.nee_w);
+                        }
+                    }
+                }
                 p.nee = false;
             }
         });
         for (uint32_t s : active_slots) {
             n_rays++;
-            (void)s;
+            PathSlot &p = pool[s];
+            if (p.rec_group >= 0) {
+                gio->a[p.rec_group] += p.rec_a;
+                gio->k += p.rec_k;
+                p.rec_group = -1;
+            }
         }
         // control_logic (wavefront_path.cu:92-234)
         std::vector<FilmRec> recs;
@@ -1421,6 +1511,11 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                 r.w = p.filter_w;
                 sensor_rgb(sc, p.L, p.lam, r.rgb);
                 if (path_finish) path_finish[uint64_t(p.sample) * W * H + p.pixel] = uint32_t(round);
+                if (gio && gio->Lfin) {
+                    const uint64_t id = uint64_t(p.sample) * W * H + p.pixel;
+                    (*gio->Lfin)[id] = p.L;
+                    for (int i = 0; i < kNS; ++i) (*gio->pdffin)[id].v[i] = p.lam.pdf[i];
+                }
                 recs.push_back(r);
                 p.active = false;
                 p.free = true;
@@ -1436,10 +1531,16 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                 if (positive(Le)) {
                     if (p.depth == 0 || p.specular) {
                         p.L = p.L + p.beta * Le;
+                        if (replay) gio->a[group_of(t.light)] += dotg(p.g, p.beta * Le);
                     } else {
                         float pdf_light = sc.light_pmf[t.light] * tri_pdf(t, p.prev, p.rd);
                         float wl = power_heuristic(p.pdf_bsdf, pdf_light);
                         p.L = p.L + (p.beta * wl) * Le;
+                        if (replay) {  // emission with MIS: a += G (2w - 1), k += -2 G (1 - w)
+                            const double G = dotg(p.g, (p.beta * wl) * Le);
+                            gio->a[group_of(t.light)] += G * (2.0 * wl - 1.0);
+                            gio->k += -2.0 * G * (1.0 - wl);
+                        }
                     }
                 }
             }
@@ -1463,6 +1564,23 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
             Isect is = make_isect(sc, p.prim, p.h, p.rd);
             Bsdf b = make_bsdf(sc, is, p.lam);
             uint32_t flags = bsdf_flags(b);
+            if (replay && b.type == kDiffuse) {
+                // every later contribution is linear in this vertex's reflectance R:
+                // dLoss/dR_i = g_i (Ltot_i - L_i) / R_i, then R = sigmoid(c0 l^2 + c1 l + c2)
+                const Material &m = sc.mats[sc.tris[p.prim].material];
+                p.rec_mat = int(sc.tris[p.prim].material);
+                for (int q = 0; q < 3; ++q) p.rec_c[q] = 0;
+                for (int i = 0; i < kNS; ++i) {
+                    if (b.r.v[i] <= 0 || b.r.v[i] >= 1) continue;  // clamped
+                    const double l = p.lam.l[i];
+                    const double x = (double(m.albedo.c0) * l + double(m.albedo.c1)) * l + double(m.albedo.c2);
+                    const double ds = 0.5 / std::pow(1.0 + x * x, 1.5);
+                    const double v = p.g[i] * (double(p.Ltot.v[i]) - double(p.L.v[i])) / double(b.r.v[i]) * ds;
+                    p.rec_c[0] += v * l * l;
+                    p.rec_c[1] += v * l;
+                    p.rec_c[2] += v;
+                }
+            }
             if (non_specular(flags)) {
                 // sample_ld (wavefront_path.cu:574-624)
                 LightCtx ctx{is.pi, is.n, is.ns};
@@ -1478,7 +1596,7 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                 float ul0, ul1;
                 p.smp.get_2d(&ul0, &ul1);
                 if (!sc.lights.empty()) {
-                    uint32_t li = search_cdf(u, sc.light_cdf.data(), uint32_t(sc.lights.size()));
+                    uint32_t li = search_cdf(u, sc.select_cdf.data(), uint32_t(sc.lights.size()));
                     float pmf = sc.light_pmf[li];
                     const Light &L = sc.lights[li];
                     const Tri &lt = sc.tris[L.tri];
@@ -1498,6 +1616,11 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                                 S4 Ld = ((Le * wl) * f) / pdf_light;
                                 p.nee = true;
                                 p.nee_L = p.beta * Ld;
+                                if (replay) {
+                                    p.nee_G = dotg(p.g, p.nee_L);
+                                    p.nee_w = wl;
+                                    p.nee_group = group_of(li);
+                                }
                                 spawn_ray_to(is.pi, is.n, ss.pi, ss.n, &p.so[0], &p.sd[0]);
                                 spawn_ray_to(ss.pi, ss.n, is.pi, is.n, &p.so[1], &p.sd[1]);
                             }
@@ -1523,8 +1646,14 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
             p.ro = offset_ray_origin(is.pi, is.n, bs.wi);
             p.rd = bs.wi;
         });
-        for (uint32_t s : shade)
+        for (uint32_t s : shade) {
             if (pool[s].nee) n_shadow += 2;
+            PathSlot &p = pool[s];
+            if (p.rec_mat >= 0) {
+                for (int q = 0; q < 3; ++q) gio->d_coef[3 * size_t(p.rec_mat) + q] += p.rec_c[q];
+                p.rec_mat = -1;
+            }
+        }
         round++;
     }
     // RGBFilm::get_pixel_rgb (rgb_film.cu:129-144)
@@ -1537,6 +1666,26 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
         if (aux) {
             for (int c = 0; c < 3; ++c) aux[4 * size_t(px) + c] = rgb_sum[3 * px + c];
             aux[4 * size_t(px) + 3] = w_sum[px];
+        }
+    }
+    if (gio) {
+        for (int q = 0; q < 9; ++q) gio->M[q] = double(sc.out_from_sensor.m[q / 3][q % 3]);
+    }
+    if (replay) {
+        // light groups: dLoss/ds_m = (a_m - k P_m) / s_m, P_m = the group's total pmf
+        gio->d_scale.assign(d.num_lights, 0.0);
+        std::vector<double> P(d.num_lights, 0.0);
+        for (size_t t = 0; t < sc.lights.size(); ++t) P[group_of(uint32_t(t))] += double(sc.light_pmf[t]);
+        for (uint32_t m = 0; m < d.num_lights; ++m)
+            if (d.lights[m].scale != 0) gio->d_scale[m] = (gio->a[m] - gio->k * P[m]) / double(d.lights[m].scale);
+        // diffuse albedo RGB: d_rgb = J^T d_coef through the RGB-to-spectrum table
+        gio->d_rgb.assign(size_t(d.num_materials) * 3, 0.0);
+        for (uint32_t m = 0; m < d.num_materials; ++m) {
+            if (d.materials[m].type != 0) continue;
+            double J[3][3];
+            sc.sp->table.jacobian(d.materials[m].rgb, J);
+            for (int k = 0; k < 3; ++k)
+                for (int q = 0; q < 3; ++q) gio->d_rgb[3 * size_t(m) + k] += J[q][k] * gio->d_coef[3 * size_t(m) + q];
         }
     }
     if (stats) {
@@ -1576,11 +1725,63 @@ int orc_render_paths(const orc_scene *scene, int threads, uint32_t *path_start, 
     return 0;
 }
 
+int orc_render_grad(const orc_scene *scene, int threads, const float *target, float *film_rgb, double *loss,
+                    double *d_rgb, double *d_scale, uint64_t *stats, int attach_selection) {
+    if (!scene || !target || !film_rgb || !loss || !scene->tri || scene->pool_size == 0) return -1;
+    try {
+        const size_t npx = size_t(scene->width) * scene->height;
+        std::vector<float> aux(npx * 4);
+        std::vector<orc::S4> Lfin, pdffin;
+        orc::GradIO fwd;
+        fwd.Lfin = &Lfin;
+        fwd.pdffin = &pdffin;
+        orc::render(*scene, threads, film_rgb, aux.data(), stats, nullptr, nullptr, &fwd);
+        // loss = sum (film - target)^2; A = d loss / d(rgb_sum / weight_sum) per pixel,
+        // divided by the weight sum when the film divides by it
+        std::vector<double> A(npx * 3, 0.0);
+        double ls = 0;
+        for (size_t px = 0; px < npx; ++px) {
+            double gk[3];
+            for (int k = 0; k < 3; ++k) {
+                const double e = double(film_rgb[3 * px + k]) - double(target[3 * px + k]);
+                ls += e * e;
+                gk[k] = 2.0 * e;
+            }
+            const double W = aux[4 * px + 3];
+            for (int c = 0; c < 3; ++c) {
+                double v = 0;
+                for (int k = 0; k < 3; ++k) v += gk[k] * fwd.M[3 * k + c];
+                A[3 * px + c] = W != 0 ? v / W : v;
+            }
+        }
+        *loss = ls;
+        orc::GradIO rep;
+        rep.attach_selection = attach_selection != 0;
+        rep.Ltot = &Lfin;
+        rep.pdf = &pdffin;
+        rep.A = &A;
+        std::vector<float> film2(npx * 3);
+        orc::render(*scene, threads, film2.data(), nullptr, nullptr, nullptr, nullptr, &rep);
+        if (d_rgb) std::copy(rep.d_rgb.begin(), rep.d_rgb.end(), d_rgb);
+        if (d_scale) std::copy(rep.d_scale.begin(), rep.d_scale.end(), d_scale);
+    } catch (const std::exception &) {
+        return -2;
+    }
+    return 0;
+}
+
 int orc_rgb_to_sigmoid(const float rgb[3], float out3[3]) {
     orc::Sigmoid s = orc::spectra().table.lookup(rgb);
     out3[0] = s.c0;
     out3[1] = s.c1;
     out3[2] = s.c2;
+    return 0;
+}
+
+int orc_rgb_to_sigmoid_jacobian(const float rgb[3], double J9[9]) {
+    double J[3][3];
+    orc::spectra().table.jacobian(rgb, J);
+    for (int q = 0; q < 9; ++q) J9[q] = J[q / 3][q % 3];
     return 0;
 }
 

@@ -574,6 +574,7 @@ struct pbrtm_scene {
     pbrtm::pt::SceneDev dev{};
     pbrtm::pt::PathPool pool;
     pbrtm::DevBuf film_rgb, film_aux;
+    pbrtm::DevBuf g_lfin, g_pfin, g_target, g_adj, g_coef, g_acc, g_slot;  // gradient replay
 };
 
 extern "C" {
@@ -583,7 +584,8 @@ void pbrtm_scene_free(pbrtm_scene *sc) {
     hipSetDevice(sc->device);
     if (sc->bvh) pbrtm_bvh_free(sc->bvh);
     for (pbrtm::DevBuf *b : {&sc->tri, &sc->tri_info, &sc->mats, &sc->lights, &sc->pmf, &sc->cdf, &sc->spec_lam,
-                             &sc->spec_val, &sc->filt, &sc->film_rgb, &sc->film_aux})
+                             &sc->spec_val, &sc->filt, &sc->film_rgb, &sc->film_aux, &sc->g_lfin, &sc->g_pfin,
+                             &sc->g_target, &sc->g_adj, &sc->g_coef, &sc->g_acc, &sc->g_slot})
         b->release();
     auto &p = sc->pool;
     for (pbrtm::DevBuf *b : {&p.path_id, &p.flags, &p.rng, &p.dim, &p.lam, &p.lam_pdf, &p.L, &p.beta, &p.nee_L,
@@ -752,6 +754,107 @@ int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *par
         stats->rays = rs.rays;
         stats->shadow_rays = rs.shadow_rays;
         stats->paths = rs.paths;
+    }
+    return PBRTM_OK;
+}
+
+int pbrtm_render_grad(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *params, const float *target_rgb,
+                      float *film_rgb, double *loss, float *d_albedo, float *d_scale, pbrtm_grad_stats *stats) {
+    if (!ctx || !sc || !params || !target_rgb || !loss) return PBRTM_ERR_INVALID;
+    if (sc->ctx != ctx) return fail(ctx, PBRTM_ERR_INVALID, "scene belongs to another context");
+    int code = 0;
+    const char *why = check_render_params(sc, params, &code);
+    if (*why) return fail(ctx, code, why);
+    const pbrtm::pt::SceneHost &H = sc->host;
+    const uint32_t ng = uint32_t(H.group_scale.size());
+    if (ng > 255) return fail(ctx, PBRTM_ERR_UNSUPPORTED, "gradients support at most 255 emitters");
+    const uint32_t nm = uint32_t(H.mats.size());
+    const uint32_t spp = params->spp;
+    const uint64_t wh = uint64_t(H.width) * H.height, total = wh * spp;
+    const uint32_t pool = params->pool_size ? params->pool_size : (2u << 20);
+    const auto t0 = std::chrono::steady_clock::now();
+    CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    CK(ctx, sc->g_lfin.reserve(total * 16), "alloc");
+    CK(ctx, sc->g_pfin.reserve(total * 16), "alloc");
+    CK(ctx, sc->film_rgb.reserve(wh * 12), "alloc");
+    CK(ctx, sc->film_aux.reserve(wh * 16), "alloc");
+    CK(ctx, sc->g_target.reserve(wh * 12), "alloc");
+    CK(ctx, sc->g_adj.reserve(wh * 12), "alloc");
+    CK(ctx, sc->g_coef.reserve(size_t(std::max(nm, 1u)) * 12), "alloc");
+    CK(ctx, sc->g_acc.reserve(size_t(ng + 2) * 8), "alloc");
+    CK(ctx, sc->g_slot.reserve(size_t(pool) * 48), "alloc");
+    CK(ctx, ctx->dev_err.reserve(64), "alloc");
+    hipStream_t st = ctx->stream;
+    CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, st), "memset");
+    CK(ctx, hipMemcpyAsync(sc->g_target.ptr, target_rgb, wh * 12, hipMemcpyHostToDevice, st), "upload");
+    CK(ctx, hipMemsetAsync(sc->g_coef.ptr, 0, size_t(std::max(nm, 1u)) * 12, st), "memset");
+    CK(ctx, hipMemsetAsync(sc->g_acc.ptr, 0, size_t(ng + 2) * 8, st), "memset");
+    const WideTrace W = scene_trace(sc);
+    float *film = sc->film_rgb.as<float>(), *aux = sc->film_aux.as<float>();
+    float4 *lfin = sc->g_lfin.as<float4>(), *pfin = sc->g_pfin.as<float4>();
+    uint32_t *err = ctx->dev_err.as<uint32_t>();
+    pbrtm::pt::RenderStats fwd, rep;
+    CK(ctx, pbrtm::pt::render(sc->dev, W, spp, params->max_depth, pool, sc->pool, film, aux, err, &fwd, st, lfin, pfin),
+       "render");
+    const auto t1 = std::chrono::steady_clock::now();
+    double *acc = sc->g_acc.as<double>();
+    CK(ctx, pbrtm::pt::film_adjoint(sc->dev, film, aux, sc->g_target.as<float>(), sc->g_adj.as<float>(), acc + ng + 1,
+                                    st), "adjoint");
+    CK(ctx, hipStreamSynchronize(st), "sync");
+    const auto t2 = std::chrono::steady_clock::now();
+    pbrtm::pt::GradDev gd{};
+    gd.lfin = lfin;
+    gd.pfin = pfin;
+    gd.adj = sc->g_adj.as<float>();
+    gd.d_coef = sc->g_coef.as<float>();
+    gd.acc = acc;
+    gd.g = sc->g_slot.as<float4>();
+    gd.ltot = gd.g + pool;
+    gd.nee = gd.g + 2 * size_t(pool);
+    gd.ngroups = ng;
+    CK(ctx, pbrtm::pt::render(sc->dev, W, spp, params->max_depth, pool, sc->pool, nullptr, nullptr, err, &rep, st,
+                              nullptr, nullptr, &gd), "replay");
+    const auto t3 = std::chrono::steady_clock::now();
+    std::vector<float> coef(size_t(nm) * 3);
+    std::vector<double> hacc(ng + 2);
+    uint32_t derr = 0;
+    CK(ctx, hipMemcpyAsync(&derr, err, 4, hipMemcpyDeviceToHost, st), "download");
+    if (nm) CK(ctx, hipMemcpyAsync(coef.data(), gd.d_coef, coef.size() * 4, hipMemcpyDeviceToHost, st), "download");
+    CK(ctx, hipMemcpyAsync(hacc.data(), acc, hacc.size() * 8, hipMemcpyDeviceToHost, st), "download");
+    if (film_rgb) CK(ctx, hipMemcpyAsync(film_rgb, film, wh * 12, hipMemcpyDeviceToHost, st), "download");
+    CK(ctx, hipStreamSynchronize(st), "sync");
+    if (derr) return fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr));
+    *loss = hacc[ng + 1];
+    if (d_albedo) {  // J^T through the RGB-to-spectrum table, per diffuse material
+        for (uint32_t m = 0; m < nm; ++m) {
+            float *o = d_albedo + 3 * size_t(m);
+            o[0] = o[1] = o[2] = 0.f;
+            if (H.mats[m].type != pbrtm::pt::kDiffuse) continue;
+            double J[9];
+            pbrtm::pt::rgb_to_sigmoid_jacobian(&H.mat_rgb[3 * size_t(m)], J);
+            for (int k = 0; k < 3; ++k) {
+                double v = 0;
+                for (int q = 0; q < 3; ++q) v += J[3 * q + k] * double(coef[3 * size_t(m) + q]);
+                o[k] = float(v);
+            }
+        }
+    }
+    if (d_scale) {  // dLoss/ds_m = (a_m - k P_m) / s_m, P_m = the emitter's total pmf
+        std::vector<double> P(ng, 0.0);
+        for (size_t t = 0; t < H.lights.size(); ++t) P[H.lights[t].group] += double(H.light_pmf[t]);
+        for (uint32_t m = 0; m < ng; ++m)
+            d_scale[m] = H.group_scale[m] != 0.f ? float((hacc[m] - hacc[ng] * P[m]) / double(H.group_scale[m])) : 0.f;
+    }
+    if (stats) {
+        auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        stats->ms_forward = ms(t0, t1);
+        stats->ms_adjoint = ms(t1, t2);
+        stats->ms_replay = ms(t2, t3);
+        stats->ms_total = ms(t0, std::chrono::steady_clock::now());
+        stats->rounds = fwd.rounds;
+        stats->rays = fwd.rays;
+        stats->shadow_rays = fwd.shadow_rays;
+        stats->paths = fwd.paths;
     }
     return PBRTM_OK;
 }

@@ -33,7 +33,7 @@ struct DLight {  // 32 bytes: a DiffuseAreaLight on one triangle
     float rgb_scale;  // RGBIlluminantSpectrum scale = 2 max(rgb)
     float c0, c1, c2; // sigmoid of rgb / rgb_scale
     uint32_t two_sided;
-    float pad;
+    uint32_t group;   // the scene's emitter (pbrtm_area_light index) the triangle belongs to
 };
 
 // Everything the kernels read about the scene; passed by value.
@@ -60,6 +60,16 @@ struct SceneDev {
     uint32_t width, height;
 };
 
+// Gradient replay state (device pointers).
+struct GradDev {
+    const float4 *lfin, *pfin;  // per path id: final L and wavelength pdfs of the forward pass
+    const float *adj;           // per pixel [3]: dLoss/d(rgb_sum / weight_sum) through the output matrix
+    float *d_coef;              // per material [3]: dLoss/d(sigmoid coefficients), accumulated
+    double *acc;                // [ngroups + 1]: scale-linear part per emitter, then the pmf part
+    float4 *g, *ltot, *nee;     // per slot: dLoss/dL, the final L, pending NEE (G, w, group)
+    uint32_t ngroups;
+};
+
 // One job of a batched render, as the kernels read it.
 struct JobDev {
     SceneDev S;
@@ -82,6 +92,8 @@ struct SceneHost {
     uint32_t filt_n;
     float out_from_sensor[9];
     uint32_t width, height;
+    std::vector<float> mat_rgb;      // per material [3]: the diffuse albedo RGB (gradients)
+    std::vector<float> group_scale;  // per emitter: the scene's "scale" parameter (gradients)
 };
 
 struct SceneDesc;  // the C-ABI description, see include/pbrtm.h
@@ -91,6 +103,8 @@ std::string build_scene_host(const void *desc, SceneHost &out);
 
 // RGB-to-spectrum sigmoid coefficients of an albedo RGB, and the sRGB matrices (for tests).
 void rgb_to_sigmoid(const float rgb[3], float out[3]);
+// d(c0, c1, c2) / d(r, g, b) of rgb_to_sigmoid, row-major [q][k].
+void rgb_to_sigmoid_jacobian(const float rgb[3], double J[9]);
 void srgb_matrices(float xyz_from_rgb[9], float rgb_from_xyz[9], float white_xy[2]);
 
 struct PathPool {
@@ -107,9 +121,18 @@ struct RenderStats {
 
 // Renders the whole image; film_rgb (device, [h][w][3]) and film_aux (device,
 // [h][w][4], nullable) receive RGBFilm::get_pixel_rgb and (rgb_sum, weight_sum).
+// lfin / pfin (nullable, per path id) receive each path's final L and wavelength
+// pdfs; with gd the render is a gradient replay (film_rgb may then be null).
 hipError_t render(const SceneDev &S, const WideTrace &W, uint32_t spp, uint32_t max_depth, uint32_t pool,
                   PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
-                  hipStream_t stream);
+                  hipStream_t stream, float4 *lfin = nullptr, float4 *pfin = nullptr,
+                  const GradDev *gd = nullptr);
+
+// Loss sum((film - target)^2) per pixel into loss_px, and adj [h][w][3] =
+// dLoss/d(rgb_sum / weight_sum) through out_from_sensor (divided by the
+// weight sum where the film divides by it).
+hipError_t film_adjoint(const SceneDev &S, const float *film_rgb, const float *film_aux, const float *target,
+                        float *adj, double *loss, hipStream_t stream);
 
 struct BatchJob {
     SceneDev S;

@@ -27,9 +27,10 @@ constexpr uint32_t kFlagSpecular = 1u << 16, kFlagAnyNonSpec = 1u << 17, kFlagNe
 constexpr uint32_t kDepthMask = 0xFFFFu;
 constexpr uint32_t kIdle = 0xFFFFFFFFu;
 
-// counters[] layout (u32 words); 8..15 hold u64 accumulators of a batched render
-constexpr int kCntNextId = 0, kCntActive = 1, kCntQueue = 2, kCntSQueue = 3, kCntShadow = 4, kCntRound = 5,
-              kCntDone = 6, kCntAccRays = 8, kCntAccShadow = 10;
+// counters[] layout (u32 words); 8..15 hold u64 accumulators of a batched render.
+// A slot holds a live path exactly when its ray is active, so the closest-hit
+// compaction's count (kCntQueue) is the round's live-path count.
+constexpr int kCntNextId = 0, kCntQueue = 2, kCntSQueue = 3, kCntRound = 5, kCntDone = 6, kCntAccRays = 8;
 
 struct State {
     uint32_t *path_id, *flags, *dim;
@@ -43,6 +44,7 @@ struct State {
     float4 *rec;                                              // per path id: sensor RGB, filter weight
     uint32_t *rec_round;     // finishing round; in a batch, rounds since the path started
     uint32_t *job, *sround;  // batch only: the slot's job and the round its path started
+    float4 *lfin, *pfin;     // nullable: each path's final L and wavelength pdfs, by path id
     uint32_t *counters;
     uint32_t *err;
 };
@@ -53,6 +55,7 @@ struct Params {
     const JobDev *jobs;        // batch only
     const uint64_t *job_base;  // batch only: first path id of each job, then the batch total
     uint32_t njobs;
+    GradDev gd;                // gradient replay only
 };
 
 // The job whose path-id range holds global path id `gid`.
@@ -273,7 +276,10 @@ __device__ __forceinline__ float3 sensor_rgb(const SceneDev &S, const S4 &L, con
 
 // generate_new_path (wavefront_path.cu:265-308) for path `id` of the render
 // (global id `gid` in a batch) in slot `i`.
-__device__ void start_path(const SceneDev &S, const State &st, uint32_t spd, uint32_t i, uint64_t id, uint64_t gid) {
+template <bool GRAD = false>
+__device__ __forceinline__ void start_path(const SceneDev &S, const State &st, uint32_t spd, uint32_t i, uint64_t id, uint64_t gid,
+                           const float4 *g_lfin = nullptr, const float4 *g_pfin = nullptr, const float *g_adj = nullptr,
+                           float4 *g_g = nullptr, float4 *g_ltot = nullptr) {
     const uint64_t wh = uint64_t(S.width) * S.height;
     const uint32_t pixel = uint32_t(id % wh), sample = uint32_t(id / wh);
     Sampler smp{pcg_seq(hash_u32(pixel)), pixel, sample, 0, spd};
@@ -299,6 +305,23 @@ __device__ void start_path(const SceneDev &S, const State &st, uint32_t spd, uin
     st.beta[i] = make_float4(1.f, 1.f, 1.f, 1.f);
     st.scal[i] = make_float4(fw, __builtin_nanf(""), 1.0f, 0.f);
     store_ray(st.rays + 2 * size_t(i), ld3(S.cam_origin), __builtin_inff(), rd);
+    if (GRAD) {
+        // dLoss/dL_i = w / 4 * sum_c A_c x_c(lambda_i) / pdf_i, pdf_i the path's final one
+        // (PixelSensor::to_sensor_rgb, RGBFilm::add_sample and get_pixel_rgb)
+        const S4 pf = f4s(g_pfin[id]);
+        const float *A = g_adj + 3 * size_t(pixel);
+        S4 g;
+#pragma unroll
+        for (int k = 0; k < kNS; ++k) {
+            float v = 0.f;
+            if (pf.v[k] != 0.f)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) v += A[c] * spec_eval(S, S.cie_off[c], S.cie_n, lam.v[k]) / pf.v[k];
+            g.v[k] = fw * v * (1.0f / kNS);
+        }
+        g_g[i] = s4f(g);
+        g_ltot[i] = g_lfin[id];
+    }
 }
 
 __device__ __forceinline__ void idle_slot(const State &st, uint32_t i) {
@@ -306,12 +329,14 @@ __device__ __forceinline__ void idle_slot(const State &st, uint32_t i) {
     st.rays[2 * size_t(i)] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+template <bool GRAD>
 __global__ __launch_bounds__(kBlock) void init_pool_kernel(SceneDev S, State st, Params P) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= P.pool) return;
     st.srays[4 * size_t(i)] = make_float4(0.f, 0.f, 0.f, 0.f);
     st.srays[4 * size_t(i) + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (uint64_t(i) < P.total) start_path(S, st, P.spd, i, i, i);
+    if (uint64_t(i) < P.total)
+        start_path<GRAD>(S, st, P.spd, i, i, i, P.gd.lfin, P.gd.pfin, P.gd.adj, P.gd.g, P.gd.ltot);
     else idle_slot(st, i);
 }
 
@@ -335,16 +360,37 @@ __global__ __launch_bounds__(kBlock) void init_pool_multi_kernel(State st, Param
 // material evaluation with sample_ld and sample_f. pid is the slot's path id
 // (global in a batch), lid its id within its render, S its scene; every lane
 // of the wave calls this (the new-path ballot is wave-wide).
-template <bool MULTI>
+// Gradient replay accumulators: s_acc[group] gathers the scale-linear part of
+// dLoss/d(emitter scale) and s_acc[ngroups] the light-pmf part (shared memory,
+// flushed per block).
+__device__ __forceinline__ float dot4(const S4 &a, const S4 &b) {
+    float v = 0.f;
+#pragma unroll
+    for (int k = 0; k < kNS; ++k) v += a.v[k] * b.v[k];
+    return v;
+}
+
+template <bool MULTI, bool GRAD>
 __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, const Params &P, uint32_t i,
                                            uint32_t pid, uint32_t lid, uint32_t spd, uint32_t max_depth,
-                                           uint32_t round) {
+                                           uint32_t round, double *s_acc) {
     const bool live = pid != kIdle;
     uint32_t fl = live ? st.flags[i] : 0u;
     S4 L = live ? f4s(st.L[i]) : s4(0.f), beta = live ? f4s(st.beta[i]) : s4(0.f);
+    S4 gadj = s4(0.f);
+    if (GRAD && live) gadj = f4s(P.gd.g[i]);
     // 1. the previous round's light sample, if both shadow rays missed
     if (live && (fl & kFlagNee)) {
-        if (st.sprim[2 * size_t(i)] == kNoHit && st.sprim[2 * size_t(i) + 1] == kNoHit) L = s4add(L, f4s(st.nee_L[i]));
+        if (st.sprim[2 * size_t(i)] == kNoHit && st.sprim[2 * size_t(i) + 1] == kNoHit) {
+            L = s4add(L, f4s(st.nee_L[i]));
+            if (GRAD) {
+                // C = beta f cos Le w(pmf p) / (pmf p): the MIS weight's pmf attached,
+                // dC/dpmf = 2 C (1 - w) / pmf; the light choice's 1/pmf held
+                const float4 q = P.gd.nee[i];
+                atomicAdd(&s_acc[__float_as_uint(q.z)], double(q.x) * (3.0 - 2.0 * double(q.y)));
+                atomicAdd(&s_acc[P.gd.ngroups], double(q.x) * 2.0 * (1.0 - double(q.y)));
+            }
+        }
         fl &= ~kFlagNee;
     }
     const uint32_t prim = live ? st.prim[i] : kNoHit;
@@ -390,6 +436,10 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
         const float3 rgb = sensor_rgb(S, L, lam, lpdf);
         st.rec[pid] = make_float4(rgb.x, rgb.y, rgb.z, scal.x);
         st.rec_round[pid] = MULTI ? round - st.sround[i] : round;
+        if (st.lfin) {
+            st.lfin[pid] = s4f(L);
+            st.pfin[pid] = s4f(lpdf);
+        }
         const uint64_t nid = uint64_t(base) + __popcll(lanes & lanemask_lt());
         if (nid < P.total) {
             if (MULTI) {
@@ -399,7 +449,7 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
                 st.job[i] = j;
                 st.sround[i] = round + 1;
             } else {
-                start_path(S, st, spd, i, nid, nid);
+                start_path<GRAD>(S, st, spd, i, nid, nid, P.gd.lfin, P.gd.pfin, P.gd.adj, P.gd.g, P.gd.ltot);
             }
         } else {
             idle_slot(st, i);
@@ -434,6 +484,7 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
         if (s4pos(Le)) {
             if (depth == 0 || (fl & kFlagSpecular)) {
                 L = s4add(L, s4mul(beta, Le));
+                if (GRAD) atomicAdd(&s_acc[Ld.group], double(dot4(gadj, s4mul(beta, Le))));
             } else {
                 const float4 q0 = st.prev[3 * size_t(i)], q1 = st.prev[3 * size_t(i) + 1], q2 = st.prev[3 * size_t(i) + 2];
                 const P3i ppi{{q0.x, q0.y}, {q0.z, q0.w}, {q1.x, q1.y}};
@@ -441,6 +492,11 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
                 const float pdf_light = S.light_pmf[light - 1] * tri_pdf(tri, ppi, pn, pns, rd);
                 const float wl = power_heuristic(scal.y, pdf_light);
                 L = s4add(L, s4mul(s4scale(beta, wl), Le));
+                if (GRAD) {  // MIS weight attached: dC/dpmf = -2 C (1 - w) / pmf
+                    const double G = dot4(gadj, s4mul(s4scale(beta, wl), Le));
+                    atomicAdd(&s_acc[Ld.group], G * (2.0 * double(wl) - 1.0));
+                    atomicAdd(&s_acc[P.gd.ngroups], -2.0 * G * (1.0 - double(wl)));
+                }
             }
         }
     }
@@ -459,6 +515,28 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
     bx.eta_d = 1.f;
     if (m.type == kDiffuse) {
         for (int k = 0; k < kNS; ++k) bx.r.v[k] = clampf(sigmoid_eval(m.c0, m.c1, m.c2, lam.v[k]), 0.0f, 1.0f);
+        if (GRAD) {
+            // every later contribution is linear in this vertex's reflectance R:
+            // dLoss/dR_k = g_k (Ltot_k - L_k) / R_k, R = sigmoid(c0 l^2 + c1 l + c2)
+            const S4 lt = f4s(P.gd.ltot[i]);
+            float d0 = 0.f, d1 = 0.f, d2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < kNS; ++k) {
+                const float r = bx.r.v[k];
+                if (!(r > 0.f && r < 1.f)) continue;
+                const float l = lam.v[k];
+                const float x = fmaf(l, fmaf(l, m.c0, m.c1), m.c2);
+                const float q = 1.f + x * x;
+                const float v = gadj.v[k] * (lt.v[k] - L.v[k]) / r * (0.5f / (q * sqrtf(q)));
+                d0 += v * l * l;
+                d1 += v * l;
+                d2 += v;
+            }
+            float *dc = P.gd.d_coef + 3 * size_t(info.x);
+            if (d0 != 0.f) atomicAdd(dc, d0);
+            if (d1 != 0.f) atomicAdd(dc + 1, d1);
+            if (d2 != 0.f) atomicAdd(dc + 2, d2);
+        }
     } else if (m.type == kConductor) {
         for (int k = 0; k < kNS; ++k) {
             bx.eta.v[k] = spec_eval(S, m.eta_off, m.eta_n, lam.v[k]);
@@ -512,6 +590,8 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
                         const S4 Ld = s4div(s4mul(s4scale(Le, wl), f), pdf_light);
                         st.nee_L[i] = s4f(s4mul(beta, Ld));
                         nee = true;
+                        if (GRAD)
+                            P.gd.nee[i] = make_float4(dot4(gadj, s4mul(beta, Ld)), wl, __uint_as_float(Lg.group), 0.f);
                         f3 o, d;
                         spawn_ray_to(ipi, n, ss.pi, ss.n, &o, &d);
                         store_ray(st.srays + 4 * size_t(i), o, 0.6f, d);
@@ -559,7 +639,21 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
 __global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Params P) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pid = i < P.pool ? st.path_id[i] : kIdle;
-    shade_impl<false>(S, st, P, i, pid, pid, P.spd, P.max_depth, P.round);
+    shade_impl<false, false>(S, st, P, i, pid, pid, P.spd, P.max_depth, P.round, nullptr);
+}
+
+// The gradient replay of one round (same paths as the forward render).
+constexpr uint32_t kMaxGroups = 255;
+__global__ __launch_bounds__(kBlock) void shade_grad_kernel(SceneDev S, State st, Params P) {
+    __shared__ double s_acc[kMaxGroups + 1];
+    for (uint32_t k = threadIdx.x; k <= P.gd.ngroups; k += kBlock) s_acc[k] = 0.0;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t pid = i < P.pool ? st.path_id[i] : kIdle;
+    shade_impl<false, true>(S, st, P, i, pid, pid, P.spd, P.max_depth, P.round, s_acc);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k <= P.gd.ngroups; k += kBlock)
+        if (s_acc[k] != 0.0) atomicAdd(&P.gd.acc[k], s_acc[k]);
 }
 
 // Batched render: each slot reads its job's scene and parameters; the round
@@ -570,25 +664,28 @@ __global__ __launch_bounds__(kBlock) void shade_multi_kernel(State st, Params P)
     const uint32_t j = pid != kIdle ? st.job[i] : 0u;
     const JobDev &J = P.jobs[j];
     const uint32_t lid = pid != kIdle ? uint32_t(pid - J.base) : kIdle;
-    shade_impl<true>(J.S, st, P, i, pid, lid, J.spd, J.max_depth, st.counters[kCntRound]);
+    shade_impl<true, false>(J.S, st, P, i, pid, lid, J.spd, J.max_depth, st.counters[kCntRound], nullptr);
 }
 
 // Start and end of a batched round (kernels, so that a captured graph holds no memset).
 __global__ void round_begin_kernel(uint32_t *counters) {
     if (threadIdx.x == 0) {
-        counters[kCntActive] = 0;
         counters[kCntQueue] = 0;
         counters[kCntSQueue] = 0;
-        counters[kCntShadow] = 0;
     }
 }
+// A round that traced no ray had no live path: the render is over and the round
+// does not count.
 __global__ void round_end_kernel(uint32_t *counters) {
     if (threadIdx.x != 0 || counters[kCntDone]) return;
+    if (counters[kCntQueue] == 0) {
+        counters[kCntDone] = 1;
+        return;
+    }
     unsigned long long *acc = reinterpret_cast<unsigned long long *>(counters + kCntAccRays);
     acc[0] += counters[kCntQueue];
     acc[1] += counters[kCntSQueue];
     counters[kCntRound] += 1;
-    if (counters[kCntActive] == 0) counters[kCntDone] = 1;
 }
 
 // The reference's one-by-one pool schedule for one job (one workgroup per
@@ -646,18 +743,6 @@ __global__ __launch_bounds__(kSchedBlock) void schedule_kernel(const JobDev *job
     if (threadIdx.x == 0) {
         rounds[blockIdx.x] = s_max + 1;
         if (s_bad) atomicOr(err, uint32_t(kErrCapacity));
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void count_live_kernel(const uint32_t *path_id, uint32_t pool, uint32_t *count,
-                                                            const float4 *srays, uint32_t *shadow) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const bool live = i < pool && path_id[i] != kIdle;
-    const bool sh = i < pool && srays[4 * size_t(i)].w > 0.f;
-    const uint32_t a = wave_sum(live ? 1u : 0u), b = wave_sum(sh ? 2u : 0u);
-    if (lane_id() == 0) {
-        if (a) atomicAdd(count, a);
-        if (b) atomicAdd(shadow, b);
     }
 }
 
@@ -728,6 +813,49 @@ __global__ __launch_bounds__(kFilmWaves * kWave) void film_kernel(SceneDev S, co
 
 namespace {
 
+// Per pixel: loss terms (film - target)^2 and the adjoint of rgb_sum / weight_sum.
+__global__ __launch_bounds__(kBlock) void film_adjoint_kernel(SceneDev S, const float *__restrict__ film,
+                                                              const float *__restrict__ aux,
+                                                              const float *__restrict__ target,
+                                                              float *__restrict__ adj, double *loss) {
+    __shared__ double s_sum[kBlock / kWave];
+    const uint32_t px = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t wh = S.width * S.height;
+    double l = 0.0;
+    if (px < wh) {
+        float e[3];
+        for (int k = 0; k < 3; ++k) {
+            e[k] = film[3 * size_t(px) + k] - target[3 * size_t(px) + k];
+            l += double(e[k]) * double(e[k]);
+        }
+        const float w = aux[4 * size_t(px) + 3];
+        for (int j = 0; j < 3; ++j) {
+            float v = 0.f;
+            for (int k = 0; k < 3; ++k) v += 2.f * e[k] * S.out_from_sensor[3 * k + j];
+            adj[3 * size_t(px) + j] = w != 0.f ? v / w : v;
+        }
+    }
+    for (int d = kWave / 2; d > 0; d >>= 1) l += __shfl_xor(l, d, kWave);
+    if (lane_id() == 0) s_sum[threadIdx.x / kWave] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (uint32_t k = 0; k < kBlock / kWave; ++k) t += s_sum[k];
+        atomicAdd(loss, t);
+    }
+}
+
+}  // namespace
+
+hipError_t film_adjoint(const SceneDev &S, const float *film_rgb, const float *film_aux, const float *target,
+                        float *adj, double *loss, hipStream_t stream) {
+    const uint32_t wh = S.width * S.height;
+    film_adjoint_kernel<<<(wh + kBlock - 1) / kBlock, kBlock, 0, stream>>>(S, film_rgb, film_aux, target, adj, loss);
+    return hipGetLastError();
+}
+
+namespace {
+
 hipError_t reserve_pool(PathPool &pp, uint32_t pool, uint64_t total, bool batch) {
     hipError_t e;
 #define PT_CK(x)                          \
@@ -766,7 +894,7 @@ hipError_t reserve_pool(PathPool &pp, uint32_t pool, uint64_t total, bool batch)
 }
 
 State make_state(PathPool &pp, uint32_t *err) {
-    State st;
+    State st{};
     st.path_id = pp.path_id.as<uint32_t>();
     st.flags = pp.flags.as<uint32_t>();
     st.dim = pp.dim.as<uint32_t>();
@@ -788,6 +916,8 @@ State make_state(PathPool &pp, uint32_t *err) {
     st.rec_round = pp.rec_round.as<uint32_t>();
     st.job = pp.job.as<uint32_t>();
     st.sround = pp.sround.as<uint32_t>();
+    st.lfin = nullptr;
+    st.pfin = nullptr;
     st.counters = pp.counters.as<uint32_t>();
     st.err = err;
     return st;
@@ -797,7 +927,7 @@ State make_state(PathPool &pp, uint32_t *err) {
 
 hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t max_depth, uint32_t pool,
                   PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
-                  hipStream_t stream) {
+                  hipStream_t stream, float4 *lfin, float4 *pfin, const GradDev *gd) {
     const uint64_t total = uint64_t(S.width) * S.height * spp;
     const uint32_t spd = uint32_t(sqrt(double(spp)));
     if (spd * spd != spp || spp == 0 || spp > kMaxSpp) return hipErrorInvalidValue;
@@ -815,8 +945,14 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         PT_CK(pp.spill.reserve(size_t(pool) * 2 * spill_need * sizeof(uint2)));
         W.spill = pp.spill.as<uint2>();
     }
-    const State st = make_state(pp, err);
+    State st = make_state(pp, err);
+    st.lfin = lfin;
+    st.pfin = pfin;
     Params P{total, spd, spp, max_depth, 0u, pool, nullptr, nullptr, 0u};
+    if (gd) {
+        if (gd->ngroups > kMaxGroups) return hipErrorInvalidValue;
+        P.gd = *gd;
+    }
     const uint32_t grid = (pool + kBlock - 1) / kBlock;
     hipEvent_t ev[6];
     for (auto &x : ev) PT_CK(hipEventCreate(&x));
@@ -830,14 +966,14 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
     uint32_t first = uint32_t(std::min<uint64_t>(total, pool));
     PT_CK(hipMemcpyAsync(st.counters + kCntNextId, &first, 4, hipMemcpyHostToDevice, stream));
     PT_CK(hipEventRecord(ev[0], stream));
-    init_pool_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);
+    if (gd) init_pool_kernel<true><<<grid, kBlock, 0, stream>>>(S, st, P);
+    else init_pool_kernel<false><<<grid, kBlock, 0, stream>>>(S, st, P);
     PT_CK(hipGetLastError());
     RenderStats rs;
     uint32_t host_cnt[16];
-    uint32_t live = first;
     float t_trace = 0, t_shadow = 0, t_shade = 0;
-    while (live > 0) {
-        PT_CK(hipMemsetAsync(st.counters + kCntActive, 0, 16, stream));
+    for (;;) {
+        PT_CK(hipMemsetAsync(st.counters + kCntQueue, 0, 8, stream));
         PT_CK(hipEventRecord(ev[1], stream));
         PT_CK(trace_wide(W, reinterpret_cast<const float *>(st.rays), pool, 0, pp.prim.as<uint32_t>(),
                          pp.thit.as<float>(), pp.bary.as<float>(), err, nullptr, pp.queue.as<uint32_t>(),
@@ -847,14 +983,13 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         PT_CK(trace_wide(W, reinterpret_cast<const float *>(st.srays), 2 * pool, 1, pp.sprim.as<uint32_t>(), nullptr,
                          nullptr, err, nullptr, pp.squeue.as<uint32_t>(), st.counters + kCntSQueue, stream));
         PT_CK(hipEventRecord(ev[3], stream));
-        shade_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);
-        PT_CK(hipGetLastError());
-        count_live_kernel<<<grid, kBlock, 0, stream>>>(st.path_id, pool, st.counters + kCntActive, st.srays,
-                                                       st.counters + kCntShadow);
+        if (gd) shade_grad_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);
+        else shade_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);
         PT_CK(hipGetLastError());
         PT_CK(hipEventRecord(ev[4], stream));
         PT_CK(hipMemcpyAsync(host_cnt, st.counters, 32, hipMemcpyDeviceToHost, stream));
         PT_CK(hipStreamSynchronize(stream));
+        if (host_cnt[kCntQueue] == 0) break;  // no live path was left: the round does not count
         float a, b, c;
         hipEventElapsedTime(&a, ev[1], ev[2]);
         hipEventElapsedTime(&b, ev[2], ev[3]);
@@ -864,15 +999,16 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         t_shade += c;
         rs.rays += host_cnt[kCntQueue];
         rs.shadow_rays += host_cnt[kCntSQueue];
-        live = host_cnt[kCntActive];
         P.round++;
         if (P.round > 100000) return hipErrorUnknown;
     }
     PT_CK(hipEventRecord(ev[5], stream));
     const uint32_t wh = S.width * S.height;
-    film_kernel<<<(wh + kFilmWaves - 1) / kFilmWaves, kFilmWaves * kWave, 0, stream>>>(
-        S, st.rec, st.rec_round, spp, film_rgb, film_aux);
-    PT_CK(hipGetLastError());
+    if (film_rgb) {
+        film_kernel<<<(wh + kFilmWaves - 1) / kFilmWaves, kFilmWaves * kWave, 0, stream>>>(
+            S, st.rec, st.rec_round, spp, film_rgb, film_aux);
+        PT_CK(hipGetLastError());
+    }
     PT_CK(hipEventRecord(ev[4], stream));
     PT_CK(hipStreamSynchronize(stream));
     float tf = 0, tt = 0;
@@ -981,8 +1117,6 @@ hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32
                                        st.counters + kCntSQueue, stream)) != hipSuccess)
                 return e2;
             shade_multi_kernel<<<grid, kBlock, 0, stream>>>(st, P);
-            count_live_kernel<<<grid, kBlock, 0, stream>>>(st.path_id, pool, st.counters + kCntActive, st.srays,
-                                                           st.counters + kCntShadow);
             round_end_kernel<<<1, 64, 0, stream>>>(st.counters);
             if ((e2 = hipGetLastError()) != hipSuccess) return e2;
         }

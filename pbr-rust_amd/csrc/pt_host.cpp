@@ -12,6 +12,8 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../data/spectra_data.h"
 #include "../../include/pbrtm.h"
@@ -269,6 +271,36 @@ class SpectrumTable {
         return out;
     }
 
+    // J[q][k] = d c_q / d rgb_k of lookup(): the trilinear interpolation's
+    // derivative times that of the cell coordinates x = rgb[(m+1)%3] (res-1) / z,
+    // y = rgb[(m+2)%3] (res-1) / z, z = rgb[m] (grey inputs take this general branch).
+    void jacobian(const float rgb[3], double J[3][3]) {
+        const int mc = (rgb[0] > rgb[1]) ? ((rgb[0] > rgb[2]) ? 0 : 2) : ((rgb[1] > rgb[2]) ? 1 : 2);
+        const double z = rgb[mc];
+        const double x = double(rgb[(mc + 1) % 3] * (kRes - 1)) / z;
+        const double y = double(rgb[(mc + 2) % 3] * (kRes - 1)) / z;
+        const int xi = std::min((int)x, kRes - 2), yi = std::min((int)y, kRes - 2);
+        int zi = 0;
+        while (zi < kRes - 2 && z_[zi + 1] < z) ++zi;
+        const double dx = x - xi, dy = y - yi, h = z_[zi + 1] - z_[zi], dz = (z - z_[zi]) / h;
+        const Column &a = column(mc, yi, xi), &b = column(mc, yi, xi + 1);
+        const Column &c = column(mc, yi + 1, xi), &d = column(mc, yi + 1, xi + 1);
+        auto mix = [](double t, double u, double v) { return u + t * (v - u); };
+        for (int q = 0; q < 3; ++q) {
+            auto f = [&](const Column &col, int dk) { return double(float(col[zi + dk][q])); };
+            double gx[2], gy[2], v[2];
+            for (int k = 0; k < 2; ++k) {
+                gx[k] = mix(dy, f(b, k) - f(a, k), f(d, k) - f(c, k));
+                gy[k] = mix(dx, f(c, k) - f(a, k), f(d, k) - f(b, k));
+                v[k] = mix(dy, mix(dx, f(a, k), f(b, k)), mix(dx, f(c, k), f(d, k)));
+            }
+            const double ddx = mix(dz, gx[0], gx[1]), ddy = mix(dz, gy[0], gy[1]), ddz = v[1] - v[0];
+            J[q][mc] = -ddx * x / z - ddy * y / z + ddz / h;
+            J[q][(mc + 1) % 3] = ddx * (kRes - 1) / z;
+            J[q][(mc + 2) % 3] = ddy * (kRes - 1) / z;
+        }
+    }
+
   private:
     using Column = std::array<std::array<double, 3>, kRes>;
     static double interp(const double *d, double x) {
@@ -362,11 +394,57 @@ class SpectrumTable {
             if (rr < 1e-6) break;
         }
     }
+  public:
+    // Computes, on all host threads, the table columns that lookups of these RGBs
+    // ([n][3]) will read and that are not cached yet (the reference fills the
+    // whole table on a thread pool at start-up; here only what a scene needs).
+    void prefetch(const float *rgb, size_t n) {
+        std::vector<int> keys;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            std::vector<char> want(3 * kRes * kRes, 0);
+            for (size_t k = 0; k < n; ++k) {
+                const float *c = rgb + 3 * k;
+                if (c[0] == c[1] && c[1] == c[2]) continue;
+                const int mc = (c[0] > c[1]) ? ((c[0] > c[2]) ? 0 : 2) : ((c[1] > c[2]) ? 1 : 2);
+                const double z = c[mc];
+                if (!(z > 0)) continue;
+                const double x = double(c[(mc + 1) % 3] * (kRes - 1)) / z;
+                const double y = double(c[(mc + 2) % 3] * (kRes - 1)) / z;
+                const int xi = std::min((int)x, kRes - 2), yi = std::min((int)y, kRes - 2);
+                if (xi < 0 || yi < 0) continue;
+                for (int dy = 0; dy < 2; ++dy)
+                    for (int dx = 0; dx < 2; ++dx) want[(mc * kRes + yi + dy) * kRes + xi + dx] = 1;
+            }
+            for (int key = 0; key < int(want.size()); ++key)
+                if (want[key] && !cols_.count(key)) keys.push_back(key);
+        }
+        if (keys.empty()) return;
+        std::vector<Column> out(keys.size());
+        const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 64u));
+        std::vector<std::thread> ts;
+        for (unsigned t = 0; t < nt; ++t)
+            ts.emplace_back([&, t] {
+                for (size_t k = t; k < keys.size(); k += nt)
+                    out[k] = compute(keys[k] / (kRes * kRes), (keys[k] / kRes) % kRes, keys[k] % kRes);
+            });
+        for (auto &th : ts) th.join();
+        std::lock_guard<std::mutex> g(mu_);
+        for (size_t k = 0; k < keys.size(); ++k) cols_.emplace(keys[k], out[k]);
+    }
+
+  private:
     const Column &column(int l, int j, int i) {
         std::lock_guard<std::mutex> g(mu_);
         const int key = (l * kRes + j) * kRes + i;
         auto it = cols_.find(key);
         if (it != cols_.end()) return it->second;
+        return cols_.emplace(key, compute(l, j, i)).first->second;
+    }
+
+    // Column (l, j, i) of the table: Gauss-Newton fits along z, warm-started from
+    // z index kRes / 5 upwards, then from there downwards.
+    Column compute(int l, int j, int i) const {
         Column col;
         const double y = j / double(kRes - 1), x = i / double(kRes - 1);
         const double c0 = 360.0, c1 = 1.0 / (830.0 - 360.0);
@@ -386,7 +464,7 @@ class SpectrumTable {
                 put(k, c);
             }
         }
-        return cols_.emplace(key, col).first->second;
+        return col;
     }
 
     double lam_[kFine], tbl_[3][kFine], white_[3], z_[kRes];
@@ -461,6 +539,12 @@ float pc1d(const float *f, size_t n, float mn, float mx, float *func, float *cdf
 
 }  // namespace
 
+void rgb_to_sigmoid_jacobian(const float rgb[3], double J[9]) {
+    double m[3][3];
+    globals().table.jacobian(rgb, m);
+    for (int q = 0; q < 9; ++q) J[q] = m[q / 3][q % 3];
+}
+
 void rgb_to_sigmoid(const float rgb[3], float out[3]) {
     auto c = globals().table.lookup(rgb);
     out[0] = c[0], out[1] = c[1], out[2] = c[2];
@@ -493,11 +577,22 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
         int metal_off[8], glass_off = -1;
         for (int i = 0; i < 8; ++i) metal_off[i] = -1;
         // materials (materials/diffuse_material.cu:22-29, conductor_material.cu:10-76)
+        {
+            std::vector<float> alb;
+            for (uint32_t i = 0; i < d.num_materials; ++i)
+                if (d.materials[i].type == PBRTM_MATERIAL_DIFFUSE)
+                    for (int q = 0; q < 3; ++q) alb.push_back(std::max(0.f, d.materials[i].rgb[q]));
+            G.table.prefetch(alb.data(), alb.size() / 3);
+        }
         S.mats.clear();
+        S.mat_rgb.assign(size_t(d.num_materials) * 3, 0.f);
+        S.group_scale.assign(d.num_lights, 0.f);
+        for (uint32_t g = 0; g < d.num_lights && d.lights; ++g) S.group_scale[g] = d.lights[g].scale;
         for (uint32_t i = 0; i < d.num_materials; ++i) {
             const pbrtm_material &m = d.materials[i];
             DMat dm{};
             dm.type = m.type;
+            for (int q = 0; q < 3; ++q) S.mat_rgb[3 * size_t(i) + q] = m.rgb[q];
             if (m.type == PBRTM_MATERIAL_DIFFUSE) {
                 if (m.rgb[0] > 1 || m.rgb[1] > 1 || m.rgb[2] > 1) return "diffuse reflectance RGB above 1";
                 const float c[3] = {std::max(0.f, m.rgb[0]), std::max(0.f, m.rgb[1]), std::max(0.f, m.rgb[2])};
@@ -609,6 +704,7 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
                 auto s = G.table.lookup(c);
                 dl.c0 = s[0], dl.c1 = s[1], dl.c2 = s[2];
                 dl.scale = L.scale / G.d65.dot(G.cie[1]);
+                dl.group = uint32_t(li);
                 S.lights.push_back(dl);
                 light = uint32_t(S.lights.size());
             }

@@ -237,31 +237,49 @@ constexpr int kWideBlock = 256;
 // Ray queue compaction: active rays (t_max > 0) are appended to `queue` with
 // one atomic per workgroup (wave ballots + a workgroup prefix), inactive ones
 // get their miss written here. The traversal then runs on full waves.
-__global__ __launch_bounds__(kWideBlock) void compact_rays_kernel(const float4 *__restrict__ rays, uint32_t n,
-                                                                  uint32_t *__restrict__ queue,
-                                                                  uint32_t *__restrict__ count,
-                                                                  uint32_t *__restrict__ prim_out,
-                                                                  float *__restrict__ t_out,
-                                                                  float *__restrict__ bary_out) {
-    __shared__ uint32_t s_waves[kWideBlock / kWave];
+// Active rays (t_max > 0) into a dense queue, in ray order within each tile of
+// kCompactBlock * kCompactItems rays; inactive rays report a miss. One queue
+// reservation (atomic) per tile.
+constexpr uint32_t kCompactBlock = 1024, kCompactItems = 4;
+__global__ __launch_bounds__(kCompactBlock) void compact_rays_kernel(const float4 *__restrict__ rays, uint32_t n,
+                                                                     uint32_t *__restrict__ queue,
+                                                                     uint32_t *__restrict__ count,
+                                                                     uint32_t *__restrict__ prim_out,
+                                                                     float *__restrict__ t_out,
+                                                                     float *__restrict__ bary_out) {
+    __shared__ uint32_t s_waves[kCompactBlock / kWave];
     __shared__ uint32_t s_base;
-    const uint32_t i = blockIdx.x * kWideBlock + threadIdx.x;
-    const bool in = i < n;
-    const bool act = in && rays[2 * i].w > 0.f;
-    if (in && !act) {
-        prim_out[i] = kNoHit;
-        if (t_out) t_out[i] = 0.f;
-        if (bary_out) {
-            bary_out[3 * i] = 0.f;
-            bary_out[3 * i + 1] = 0.f;
-            bary_out[3 * i + 2] = 0.f;
+    const uint32_t tile = blockIdx.x * kCompactBlock * kCompactItems;
+    bool act[kCompactItems];
+    uint32_t mine = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kCompactItems; ++k) {
+        const uint32_t i = tile + k * kCompactBlock + threadIdx.x;
+        const bool in = i < n;
+        act[k] = in && rays[2 * size_t(i)].w > 0.f;
+        if (in && !act[k]) {
+            prim_out[i] = kNoHit;
+            if (t_out) t_out[i] = 0.f;
+            if (bary_out) {
+                bary_out[3 * size_t(i)] = 0.f;
+                bary_out[3 * size_t(i) + 1] = 0.f;
+                bary_out[3 * size_t(i) + 2] = 0.f;
+            }
         }
+        mine += act[k] ? 1u : 0u;
     }
     uint32_t tot;
-    const uint32_t pre = block_exclusive_add<kWideBlock>(act ? 1u : 0u, s_waves, &tot);
+    block_exclusive_add<kCompactBlock>(mine, s_waves, &tot);
     if (threadIdx.x == 0) s_base = tot ? atomicAdd(count, tot) : 0u;
     __syncthreads();
-    if (act) queue[s_base + pre] = i;
+    uint32_t run = s_base;
+#pragma unroll
+    for (uint32_t k = 0; k < kCompactItems; ++k) {
+        uint32_t part;
+        const uint32_t pre = block_exclusive_add<kCompactBlock>(act[k] ? 1u : 0u, s_waves, &part);
+        if (act[k]) queue[run + pre] = tile + k * kCompactBlock + threadIdx.x;
+        run += part;
+    }
 }
 
 // MULTI: every ray selects its own tree, jw[ray_job[i >> job_shift]] (jobs of a
@@ -469,8 +487,9 @@ hipError_t trace_wide(const WideTrace &B, const float *rays, uint32_t n, int any
                       uint32_t *queue_len, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     const uint32_t grid = (n + kWideBlock - 1) / kWideBlock;
+    const uint32_t cgrid = (n + kCompactBlock * kCompactItems - 1) / (kCompactBlock * kCompactItems);
     const float4 *r = reinterpret_cast<const float4 *>(rays);
-    compact_rays_kernel<<<grid, kWideBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
+    compact_rays_kernel<<<cgrid, kCompactBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
     if (counts)
         trace_wide_kernel<true, false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
                                                                         t_out, bary_out, err, counts, nullptr,
@@ -488,8 +507,9 @@ hipError_t trace_wide_multi(const WideTrace &B, const WideTrace *jobs, const uin
                             hipStream_t stream) {
     if (n == 0) return hipSuccess;
     const uint32_t grid = (n + kWideBlock - 1) / kWideBlock;
+    const uint32_t cgrid = (n + kCompactBlock * kCompactItems - 1) / (kCompactBlock * kCompactItems);
     const float4 *r = reinterpret_cast<const float4 *>(rays);
-    compact_rays_kernel<<<grid, kWideBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
+    compact_rays_kernel<<<cgrid, kCompactBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
     trace_wide_kernel<false, true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
                                                                     bary_out, err, nullptr, jobs, ray_job,
                                                                     job_shift);

@@ -35,6 +35,7 @@ EXPORTS = (
     "pbrtm_gen_camera_rays", "pbrtm_scene_create", "pbrtm_scene_free", "pbrtm_render", "pbrtm_rgb_to_sigmoid",
     "pbrtm_srgb_matrices", "pbrtm_srgb8", "pbrtm_write_png", "pbrtm_pbrt_parse", "pbrtm_pbrt_desc",
     "pbrtm_pbrt_render_params", "pbrtm_pbrt_output", "pbrtm_pbrt_free", "pbrtm_render_batch",
+    "pbrtm_render_grad",
 )
 BATCH_NO_GRAPH = 2
 
@@ -77,6 +78,14 @@ class RenderStats(ctypes.Structure):
     _fields_ = [("ms_total", c_double), ("ms_trace", c_double), ("ms_shadow", c_double), ("ms_shade", c_double),
                 ("ms_film", c_double), ("rounds", c_uint64), ("rays", c_uint64), ("shadow_rays", c_uint64),
                 ("paths", c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GradStats(ctypes.Structure):
+    _fields_ = [("ms_forward", c_double), ("ms_adjoint", c_double), ("ms_replay", c_double), ("ms_total", c_double),
+                ("rounds", c_uint64), ("rays", c_uint64), ("shadow_rays", c_uint64), ("paths", c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -150,6 +159,8 @@ def load(path: str = LIB_PATH):
         "pbrtm_pbrt_output": (c_char_p, [p]),
         "pbrtm_pbrt_free": (None, [p]),
         "pbrtm_render_batch": (c_int, [p, POINTER(Job), c_uint32, POINTER(BatchParams), POINTER(BatchStats)]),
+        "pbrtm_render_grad": (c_int, [p, p, POINTER(RenderParams), p, p, POINTER(c_double), p, p,
+                                      POINTER(GradStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -336,8 +347,41 @@ def scene_desc(sc):
     mat = np.ascontiguousarray(sc["tri_material"], dtype=np.uint32)
     lig = np.ascontiguousarray(sc["tri_light"], dtype=np.int32)
     rev = np.ascontiguousarray(sc.get("tri_reverse", np.zeros(len(tri), np.uint8)), dtype=np.uint8)
-    mats = (Material * max(1, len(sc["materials"])))()
-    for i, m in enumerate(sc["materials"]):
+    if sc.get("materials") is None:  # all diffuse, albedo [n][3]
+        alb = np.ascontiguousarray(sc["albedo"], dtype=np.float32).reshape(-1, 3)
+        arr = np.zeros(len(alb), dtype=_MATERIAL_DTYPE)
+        arr["rgb"] = alb
+        mats = arr.ctypes.data_as(POINTER(Material))
+        nmat = len(alb)
+    else:
+        mats, nmat, arr = _materials(sc["materials"]), len(sc["materials"]), None
+    lights = (AreaLight * max(1, len(sc["lights"])))()
+    for i, l in enumerate(sc["lights"]):
+        lights[i].rgb[:] = list(l["rgb"])
+        lights[i].scale = l.get("scale", 1.0)
+        lights[i].two_sided = int(l.get("two_sided", False))
+    d = SceneDesc()
+    d.eye[:], d.look[:], d.up[:] = sc["eye"], sc["look"], sc["up"]
+    d.fov = sc["fov"]
+    d.width, d.height = sc["width"], sc["height"]
+    d.num_tris = len(tri)
+    d.tri, d.tri_material, d.tri_light, d.tri_reverse = _ptr(tri), _ptr(mat), _ptr(lig), _ptr(rev)
+    d.num_materials = nmat
+    d.materials = mats
+    d.num_lights = len(sc["lights"])
+    d.lights = lights
+    return _scene_desc_tail(sc, d, (tri, mat, lig, rev, mats, lights, arr))
+
+
+_MATERIAL_DTYPE = np.dtype([("type", np.uint32), ("rgb", np.float32, 3), ("metal", np.uint32),
+                            ("uroughness", np.float32), ("vroughness", np.float32), ("remap_roughness", np.uint32),
+                            ("eta", np.float32), ("eta_named", np.uint32)])
+assert _MATERIAL_DTYPE.itemsize == ctypes.sizeof(Material)
+
+
+def _materials(materials):
+    mats = (Material * max(1, len(materials)))()
+    for i, m in enumerate(materials):
         if m["type"] == "diffuse":
             mats[i].type = 0
             mats[i].rgb[:] = list(m["rgb"])
@@ -354,28 +398,17 @@ def scene_desc(sc):
             mats[i].remap_roughness = int(m.get("remap", True))
         else:
             raise ValueError(f"unknown material type {m['type']}")
-    lights = (AreaLight * max(1, len(sc["lights"])))()
-    for i, l in enumerate(sc["lights"]):
-        lights[i].rgb[:] = list(l["rgb"])
-        lights[i].scale = l.get("scale", 1.0)
-        lights[i].two_sided = int(l.get("two_sided", False))
-    d = SceneDesc()
-    d.eye[:], d.look[:], d.up[:] = sc["eye"], sc["look"], sc["up"]
-    d.fov = sc["fov"]
-    d.width, d.height = sc["width"], sc["height"]
-    d.num_tris = len(tri)
-    d.tri, d.tri_material, d.tri_light, d.tri_reverse = _ptr(tri), _ptr(mat), _ptr(lig), _ptr(rev)
-    d.num_materials = len(sc["materials"])
-    d.materials = mats
-    d.num_lights = len(sc["lights"])
-    d.lights = lights
+    return mats
+
+
+def _scene_desc_tail(sc, d, keep):
     xf = np.ascontiguousarray(sc.get("xforms", np.zeros((0, 4, 4))), dtype=np.float32).reshape(-1, 16)
     txf = sc.get("tri_xform")
     txf = None if txf is None else np.ascontiguousarray(txf, dtype=np.uint32)
     d.num_xforms = len(xf)
     d.xforms = _ptr(xf) if len(xf) else None
     d.tri_xform = _ptr(txf)
-    return d, (tri, mat, lig, rev, mats, lights, xf, txf)
+    return d, keep + (xf, txf)
 
 
 class PbrtFile:
@@ -453,10 +486,12 @@ class Scene:
         if isinstance(sc, PbrtFile):
             dp = ctx.lib.pbrtm_pbrt_desc(sc.h)
             self.width, self.height = dp.contents.width, dp.contents.height
+            self.num_materials, self.num_lights = dp.contents.num_materials, dp.contents.num_lights
             ctx.check(ctx.lib.pbrtm_scene_create(ctx.h, dp, ctypes.byref(h)))
         else:
             self.width, self.height = sc["width"], sc["height"]
             d, keep = scene_desc(sc)
+            self.num_materials, self.num_lights = d.num_materials, d.num_lights
             ctx.check(ctx.lib.pbrtm_scene_create(ctx.h, ctypes.byref(d), ctypes.byref(h)))
         self.h = h
         ctx._handles.add(self)
@@ -471,6 +506,21 @@ class Scene:
             self.free()
         except Exception:
             pass
+
+    def render_grad(self, target, spp=4, max_depth=5, pool_size=0):
+        """Loss sum((film - target)^2) and its gradient (pbrtm_render_grad): returns
+        (loss, d_albedo [num_materials][3], d_scale [num_lights], film, stats dict)."""
+        tgt = np.ascontiguousarray(target, np.float32)
+        assert tgt.shape == (self.height, self.width, 3)
+        p = RenderParams(spp, max_depth, pool_size, 0)
+        film = np.zeros((self.height, self.width, 3), np.float32)
+        d_alb = np.zeros((self.num_materials, 3), np.float32)
+        d_sc = np.zeros(max(1, self.num_lights), np.float32)
+        loss = c_double(0)
+        st = GradStats()
+        self.ctx.check(self.ctx.lib.pbrtm_render_grad(self.ctx.h, self.h, ctypes.byref(p), _ptr(tgt), _ptr(film),
+                                                      ctypes.byref(loss), _ptr(d_alb), _ptr(d_sc), ctypes.byref(st)))
+        return loss.value, d_alb, d_sc[:self.num_lights], film, st.as_dict()
 
     def render(self, spp=4, max_depth=5, pool_size=0, film_rgb=None, film_aux=None, device_ptrs=False):
         """Returns (film_rgb [h][w][3], film_aux [h][w][4], stats dict)."""

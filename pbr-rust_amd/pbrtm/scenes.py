@@ -22,6 +22,12 @@ lit by one emissive quad under the ceiling (rgb L 1, scale 8); soup triangle k
 takes diffuse material k mod 8 (albedo from PCG32). The camera sits inside the
 room at z = -1.3 looking at the cube's centre.
 
+grad_room: the differentiable-rendering workload: the soup_room geometry with
+every triangle given its own diffuse material (albedo U[0.1, 0.9]^3 from a
+64-bit hash) and the ceiling light split into 8 emitters (quads in a 4 x 2
+grid, scales U[4, 8]); perturbed() scales every albedo and emitter scale by
+1 + U[-a, a] (albedos clamped to [0.01, 0.99]) to make the target.
+
 batch_jobs: the mixed batch of the batched-render benchmark (per job: a square
 resolution uniform in [lo, hi], spp from {1, 4, 16, 64}, max depth 1..8 and a
 soup_room of 10^4..10^6 triangles, log-uniform), drawn from one PCG32 stream.
@@ -241,4 +247,44 @@ def batch_jobs(n_jobs=64, seed=0xBA7C4, res=(64, 1024), tris=(10_000, 1_000_000)
         lo, hi = np.log(tris[0]), np.log(tris[1])
         n = int(round(float(np.exp(lo + (hi - lo) * rng.uniform()))))
         out.append((soup_room(n, r, r, seed=seed + 17 * k + 1, spp=spp, max_depth=depth), spp, depth))
+    return out
+
+
+def grad_room(n_tris, width=512, height=512, seed=0xD1FF, spp=16, max_depth=4):
+    from . import gen_soup
+    soup = gen_soup(n_tris, seed=seed).reshape(-1, 9)
+    walls = np.array(box(-0.5, 1.5, -0.5, 1.5, -1.5, 1.5), np.float32)
+    lights = []
+    for i in range(4):
+        for j in range(2):
+            x0, z0 = 0.05 + 0.25 * i, 0.1 + 0.45 * j
+            lights += quad((x0, 1.49, z0), (x0 + 0.2, 1.49, z0), (x0 + 0.2, 1.49, z0 + 0.35), (x0, 1.49, z0 + 0.35))
+    lt = np.array(lights, np.float32)
+    tri = np.ascontiguousarray(np.concatenate([soup, walls, lt]).astype(np.float32))
+    n = len(tri)
+    k = np.arange(n, dtype=np.uint64)
+    alb = np.stack([0.1 + 0.8 * _hash01(k, np.full(n, c, np.uint64), np.zeros(n, np.uint64), seed) for c in range(3)], 1)
+    rng = Pcg32(seed)
+    tri_light = np.full(n, -1, np.int32)
+    tri_light[-len(lt):] = np.repeat(np.arange(8, dtype=np.int32), 2)
+    return {
+        "eye": (0.5, 0.5, -1.3), "look": (0.5, 0.5, 0.5), "up": (0.0, 1.0, 0.0), "fov": 50.0,
+        "width": width, "height": height, "spp": spp, "max_depth": max_depth,
+        "tri": tri, "tri_material": np.arange(n, dtype=np.uint32), "tri_light": tri_light,
+        "albedo": alb.astype(np.float32),
+        "materials": None,
+        "lights": [{"rgb": (1.0, 1.0, 1.0), "scale": 4.0 + 4.0 * rng.uniform()} for _ in range(8)],
+    }
+
+
+def perturbed(sc, seed=0x7A27, amount=0.1):
+    """The scene with every albedo and emitter scale multiplied by 1 + U[-amount, amount]."""
+    n = len(sc["albedo"])
+    k = np.arange(n, dtype=np.uint64)
+    f = np.stack([1.0 + amount * (2.0 * _hash01(k, np.full(n, c, np.uint64), np.ones(n, np.uint64), seed) - 1.0)
+                  for c in range(3)], 1)
+    rng = Pcg32(seed)
+    out = dict(sc)
+    out["albedo"] = np.clip(sc["albedo"] * f, 0.01, 0.99).astype(np.float32)
+    out["lights"] = [dict(l, scale=l["scale"] * (1.0 + amount * (2.0 * rng.uniform() - 1.0))) for l in sc["lights"]]
     return out

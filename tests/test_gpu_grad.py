@@ -1,0 +1,105 @@
+"""pbrtm_render_grad (forward render + path replay on the GPU) against the
+oracle's FP64 path replay, and against central differences of the GPU's own
+loss at a larger size."""
+import numpy as np
+import pytest
+
+import oracle as O
+import pbrtm
+import scenes
+from pbrtm.scenes import grad_room, perturbed
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def close(gpu, ref, rel, label):
+    scale = float(np.abs(ref).max())
+    err = np.abs(gpu.astype(np.float64) - ref)
+    worst = float((err / (rel * np.abs(ref) + rel * scale)).max())
+    log(f"{label}: max |gpu - oracle| {float(err.max()):.3g} (scale {scale:.3g}), worst/tolerance {worst:.3g}")
+    assert worst <= 1.0, label
+
+
+@pytest.mark.parametrize("which", ["cornell", "room"])
+def test_gradient_matches_oracle(ctx, which):
+    if which == "cornell":
+        sc = scenes.cornell_emitters(32, 32, spp=16)
+    else:
+        sc = grad_room(300, 40, 40, spp=16, max_depth=4)
+    target, _, _ = O.render(perturbed(sc), threads=16)
+    loss_o, d_rgb_o, d_scale_o, film_o = O.render_grad(sc, target, threads=16)
+    s = ctx.scene(sc)
+    try:
+        loss, d_alb, d_sc, film, st = s.render_grad(target, spp=sc["spp"], max_depth=sc["max_depth"])
+    finally:
+        s.free()
+    log(f"{which}: loss gpu {loss:.9g} oracle {loss_o:.9g}; forward {st['ms_forward']:.2f} ms, "
+        f"replay {st['ms_replay']:.2f} ms")
+    assert np.array_equal(film.view(np.uint32), film_o.view(np.uint32))
+    assert abs(loss - loss_o) <= 1e-9 * loss_o
+    close(d_alb, d_rgb_o, 2e-3, f"{which} d_albedo")
+    close(d_sc, d_scale_o, 2e-3, f"{which} d_scale")
+
+
+def test_gradient_central_differences_on_gpu(ctx):
+    """64k-triangle room at 256x256, 16 spp, depth 4: the 12 wall albedos' 36
+    derivatives against central differences (h = 1e-3) of the GPU loss, and the 8
+    emitter scales' (h = 1 % of the scale). The emitter bound is looser and taken
+    over the 8 together: changing a scale moves the light choice's cdf, and the
+    samples whose choice flips add a boundary term that is right in expectation but
+    noisy (its noise falls as 1 / sqrt(h))."""
+    sc = grad_room(65536, 256, 256, spp=16, max_depth=4)
+    target_sc = perturbed(sc)
+    t = ctx.scene(target_sc)
+    target, _, _ = t.render(spp=16, max_depth=4)
+    t.free()
+
+    def loss_of(alb=None, lights=None):
+        s2 = ctx.scene(dict(sc, albedo=sc["albedo"] if alb is None else alb,
+                            lights=sc["lights"] if lights is None else lights))
+        f, _, _ = s2.render(spp=16, max_depth=4)
+        s2.free()
+        return float(np.sum((f.astype(np.float64) - target) ** 2))
+
+    s = ctx.scene(sc)
+    loss, d_alb, d_sc, _, st = s.render_grad(target, spp=16, max_depth=4)
+    s.free()
+    log(f"loss {loss:.6g}, forward {st['ms_forward']:.1f} ms, replay {st['ms_replay']:.1f} ms")
+    walls = range(65536, 65536 + 12)
+    gmax = float(np.abs(d_alb[65536:65548]).max())
+    for m in walls:
+        for c in range(3):
+            hi, lo = sc["albedo"].copy(), sc["albedo"].copy()
+            hi[m, c] += 1e-3
+            lo[m, c] -= 1e-3
+            fd = (loss_of(alb=hi) - loss_of(alb=lo)) / 2e-3
+            log(f"wall {m - 65536} c {c}: fd {fd:.6g} ad {d_alb[m, c]:.6g}")
+            assert abs(fd - d_alb[m, c]) <= 0.02 * abs(fd) + 2e-3 * gmax, (m, c)
+    smax = float(np.abs(d_sc).max())
+    fds = []
+    for li in range(8):
+        h = 1e-2 * sc["lights"][li]["scale"]
+        hi = [dict(l) for l in sc["lights"]]
+        lo = [dict(l) for l in sc["lights"]]
+        hi[li]["scale"] += h
+        lo[li]["scale"] -= h
+        fd = (loss_of(lights=hi) - loss_of(lights=lo)) / (2 * h)
+        fds.append(fd)
+        log(f"emitter {li}: fd {fd:.6g} ad {d_sc[li]:.6g}")
+        assert abs(fd - d_sc[li]) <= 0.05 * abs(fd) + 0.03 * smax, li
+    fds = np.array(fds)
+    assert np.linalg.norm(fds - d_sc) <= 0.05 * np.linalg.norm(fds)
+
+
+def test_gradient_errors(ctx):
+    sc = scenes.cornell_emitters(16, 16, spp=4)
+    s = ctx.scene(sc)
+    try:
+        with pytest.raises(pbrtm.PbrtmError):
+            s.render_grad(np.zeros((16, 16, 3), np.float32), spp=3)
+    finally:
+        s.free()
