@@ -224,11 +224,11 @@ int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *scene, const pbrtm_render_params *
  * spectral radiance; a second render of the same paths (same sampler streams)
  * accumulates dLoss/d(diffuse albedo RGB) per material (through the
  * RGB-to-spectrum table) and dLoss/d(scale) per emitter (pbrtm_area_light).
- * The per-sample estimator is differentiated with its MIS weights' light pmf
- * attached and the light choice's 1/pmf held (the choice's boundary term
- * cancels that derivative in expectation, so the result is what central
- * differences over many samples measure); Russian roulette's probability is
- * held. d_albedo [num_materials][3] (zero for non-diffuse materials) and
+ * Each sample's estimator (fixed sampler streams) is differentiated path by
+ * path, the light pmf attached wherever it appears (MIS weights, the light
+ * choice's 1/pmf) and every discrete choice held (which light, Russian
+ * roulette), so the result is the derivative of the fixed-seed loss between
+ * the parameter values where some sample's light choice flips. d_albedo [num_materials][3] (zero for non-diffuse materials) and
  * d_scale [num_lights] may be NULL; film_rgb (may be NULL) receives the film.
  * Host pointers; at most 255 emitters. */
 typedef struct pbrtm_grad_stats {

@@ -290,7 +290,7 @@ def render_paths(sc, pool_size=2 * 1024 * 1024, threads=8):
     return start, finish, {"rounds": int(st[0]), "rays": int(st[1]), "shadow_rays": int(st[2]), "paths": int(st[3])}
 
 
-def render_grad(sc, target, pool_size=2 * 1024 * 1024, threads=8, attach_selection=False):
+def render_grad(sc, target, pool_size=2 * 1024 * 1024, threads=8, hold_selection=False):
     """Loss sum((film - target)^2) and its FP64 path-replay gradient: returns
     (loss, d_rgb [num_materials][3], d_scale [num_lights], film)."""
     lib = load()
@@ -307,7 +307,7 @@ def render_grad(sc, target, pool_size=2 * 1024 * 1024, threads=8, attach_selecti
     d_scale = np.zeros(len(sc["lights"]), np.float64)
     st = np.zeros(5, np.uint64)
     rc = lib.orc_render_grad(ctypes.byref(s), threads, _p(tgt), _p(film), ctypes.byref(loss), _p(d_rgb),
-                             _p(d_scale), _p(st), int(attach_selection))
+                             _p(d_scale), _p(st), int(hold_selection))
     if rc != 0:
         raise RuntimeError(f"orc_render_grad failed ({rc})")
     return loss.value, d_rgb, d_scale, film

@@ -69,13 +69,14 @@ int orc_render_paths(const orc_scene *scene, int threads, uint32_t *path_start, 
 /* Loss and gradient of the path-traced film (FP64 path replay): loss = sum over
  * pixels and channels of (film - target)^2; d_rgb [num_materials][3] is
  * dLoss/d(diffuse albedo RGB) (0 for other materials), d_scale [num_lights]
- * dLoss/d(emitter scale). The fixed-seed estimator is differentiated with the
- * MIS weights' light pmf attached and the light choice's 1/pmf held, which is
- * what central differences over many samples measure (the choice's boundary
- * crossings cancel that term in expectation); attach_selection != 0
- * differentiates that 1/pmf as well. film_rgb receives the film. */
+ * dLoss/d(emitter scale). The fixed-seed estimator is differentiated path by
+ * path with the light pmf attached everywhere it appears (MIS weights and the
+ * light choice's 1/pmf); the discrete choices themselves are held, so this is
+ * the derivative central differences measure while no sample's light choice
+ * flips. hold_selection != 0 holds the choice's 1/pmf instead. film_rgb
+ * receives the film. */
 int orc_render_grad(const orc_scene *scene, int threads, const float *target, float *film_rgb, double *loss,
-                    double *d_rgb, double *d_scale, uint64_t *stats, int attach_selection);
+                    double *d_rgb, double *d_scale, uint64_t *stats, int hold_selection);
 
 /* Intermediate tables, for tests: the RGB-to-spectrum sigmoid coefficients of an
  * albedo RGB, the sRGB rgb_from_xyz matrix, the filter sample of a 2D u. */

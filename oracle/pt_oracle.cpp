@@ -1307,7 +1307,7 @@ struct GradIO {
     std::vector<double> d_coef, a;
     double k = 0;
     std::vector<double> d_rgb, d_scale;  // per material [3], per light group
-    bool attach_selection = false;       // differentiate the light choice's 1/pmf too
+    bool hold_selection = false;         // hold the light choice's 1/pmf (not differentiated)
     double M[9] = {};                    // out_from_sensor, for the adjoint
 };
 
@@ -1458,13 +1458,13 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                 if (!occluded) {
                     p.L = p.L + p.nee_L;
                     if (replay) {
-                        // NEE, C = beta f cos Le w(pmf p) / (pmf p). The MIS weight's pmf is
-                        // attached: dC/dpmf = 2 C (1 - w) / pmf, so a += G (3 - 2w),
-                        // k += 2 G (1 - w). The 1/pmf of the light choice is held (its
-                        // derivative is cancelled in expectation by the choice's boundary
-                        // term); attach_selection adds it: dC/dpmf -= C / pmf.
+                        // NEE, C = beta f cos Le w(pmf p) / (pmf p), w = g^2 / (g^2 + b^2),
+                        // g = pmf p. Both the MIS weight's pmf and the light choice's 1/pmf
+                        // attached: dC/dpmf = C (1 - 2w) / pmf, so a += G 2 (1 - w),
+                        // k += G (1 - 2w). hold_selection holds the choice's 1/pmf
+                        // (dC/dpmf = 2 C (1 - w) / pmf): a += G (3 - 2w), k += 2 G (1 - w).
                         p.rec_group = p.nee_group;
-                        if (gio->attach_selection) {
+                        if (!gio->hold_selection) {
                             p.rec_a = p.nee_G * 2.0 * (1.0 - p.nee_w);
                             p.rec_k = p.nee_G * (1.0 - 2.0 * p.nee_w);
                         } else {
@@ -1726,7 +1726,7 @@ int orc_render_paths(const orc_scene *scene, int threads, uint32_t *path_start, 
 }
 
 int orc_render_grad(const orc_scene *scene, int threads, const float *target, float *film_rgb, double *loss,
-                    double *d_rgb, double *d_scale, uint64_t *stats, int attach_selection) {
+                    double *d_rgb, double *d_scale, uint64_t *stats, int hold_selection) {
     if (!scene || !target || !film_rgb || !loss || !scene->tri || scene->pool_size == 0) return -1;
     try {
         const size_t npx = size_t(scene->width) * scene->height;
@@ -1756,7 +1756,7 @@ int orc_render_grad(const orc_scene *scene, int threads, const float *target, fl
         }
         *loss = ls;
         orc::GradIO rep;
-        rep.attach_selection = attach_selection != 0;
+        rep.hold_selection = hold_selection != 0;
         rep.Ltot = &Lfin;
         rep.pdf = &pdffin;
         rep.A = &A;

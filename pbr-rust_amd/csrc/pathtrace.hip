@@ -384,11 +384,12 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
         if (st.sprim[2 * size_t(i)] == kNoHit && st.sprim[2 * size_t(i) + 1] == kNoHit) {
             L = s4add(L, f4s(st.nee_L[i]));
             if (GRAD) {
-                // C = beta f cos Le w(pmf p) / (pmf p): the MIS weight's pmf attached,
-                // dC/dpmf = 2 C (1 - w) / pmf; the light choice's 1/pmf held
+                // C = beta f cos Le w(pmf p) / (pmf p) with w = g^2 / (g^2 + b^2), g = pmf p:
+                // dC/dpmf = C (1 - 2w) / pmf (the MIS weight's pmf and the light choice's
+                // 1/pmf both attached), so a += G 2 (1 - w) and k += G (1 - 2w)
                 const float4 q = P.gd.nee[i];
-                atomicAdd(&s_acc[__float_as_uint(q.z)], double(q.x) * (3.0 - 2.0 * double(q.y)));
-                atomicAdd(&s_acc[P.gd.ngroups], double(q.x) * 2.0 * (1.0 - double(q.y)));
+                atomicAdd(&s_acc[__float_as_uint(q.z)], double(q.x) * 2.0 * (1.0 - double(q.y)));
+                atomicAdd(&s_acc[P.gd.ngroups], double(q.x) * (1.0 - 2.0 * double(q.y)));
             }
         }
         fl &= ~kFlagNee;

@@ -47,11 +47,14 @@ def test_gradient_matches_oracle(ctx, which):
 
 def test_gradient_central_differences_on_gpu(ctx):
     """64k-triangle room at 256x256, 16 spp, depth 4: the 12 wall albedos' 36
-    derivatives against central differences (h = 1e-3) of the GPU loss, and the 8
-    emitter scales' (h = 1 % of the scale). The emitter bound is looser and taken
-    over the 8 together: changing a scale moves the light choice's cdf, and the
-    samples whose choice flips add a boundary term that is right in expectation but
-    noisy (its noise falls as 1 / sqrt(h))."""
+    derivatives against central differences (h = 1e-3) of the GPU loss. Albedos
+    move no discrete choice, so the differences measure the replayed derivative
+    itself. The 8 emitter scales' (h = 5 % of the scale) are only bounded loosely:
+    a scale moves the light choice's cdf, and the samples whose choice flips add
+    boundary terms (a jump in a pixel enters the L2 loss squared) that the
+    fixed-seed derivative leaves out by definition; the emitter derivative is
+    pinned against the oracle, and the oracle's against differences with the
+    choice frozen."""
     sc = grad_room(65536, 256, 256, spp=16, max_depth=4)
     target_sc = perturbed(sc)
     t = ctx.scene(target_sc)
@@ -82,7 +85,7 @@ def test_gradient_central_differences_on_gpu(ctx):
     smax = float(np.abs(d_sc).max())
     fds = []
     for li in range(8):
-        h = 1e-2 * sc["lights"][li]["scale"]
+        h = 5e-2 * sc["lights"][li]["scale"]
         hi = [dict(l) for l in sc["lights"]]
         lo = [dict(l) for l in sc["lights"]]
         hi[li]["scale"] += h
@@ -90,9 +93,9 @@ def test_gradient_central_differences_on_gpu(ctx):
         fd = (loss_of(lights=hi) - loss_of(lights=lo)) / (2 * h)
         fds.append(fd)
         log(f"emitter {li}: fd {fd:.6g} ad {d_sc[li]:.6g}")
-        assert abs(fd - d_sc[li]) <= 0.05 * abs(fd) + 0.03 * smax, li
+        assert abs(fd - d_sc[li]) <= 0.1 * smax, li
     fds = np.array(fds)
-    assert np.linalg.norm(fds - d_sc) <= 0.05 * np.linalg.norm(fds)
+    assert np.linalg.norm(fds - d_sc) <= 0.1 * np.linalg.norm(fds)
 
 
 def test_gradient_errors(ctx):

@@ -3,9 +3,10 @@
 Loss = sum over pixels and channels of (film - target)^2, target rendered from
 parameters perturbed by up to 10 %. Albedo derivatives pass through the
 RGB-to-spectrum table (piecewise trilinear) and every later path contribution;
-emitter-scale derivatives through the emitted radiance, the power light
-sampler's pmf inside the MIS weights and, with the light choice frozen (the
-select_scale knob), the choice's 1/pmf. A dropped term (the MIS-weight
+emitter-scale derivatives through the emitted radiance and the power light
+sampler's pmf inside the MIS weights and the light choice's 1/pmf, checked with
+the light choice frozen (the select_scale knob) so that no sample's discrete
+choice flips between the two renders of a difference. A dropped term (the MIS-weight
 derivative, the pmf's dependence on the other emitters, a vertex's own NEE)
 moves the gradient by far more than the tolerances below.
 """
@@ -74,13 +75,14 @@ def test_albedo_gradient_matches_central_differences():
 
 
 def test_emitter_gradient_with_frozen_light_choice():
-    """With the light choice frozen, central differences measure the pathwise
-    derivative with the choice's 1/pmf attached."""
+    """With the light choice frozen (no sample's choice flips), central differences
+    measure the fixed-seed estimator's pathwise derivative: the light pmf attached
+    in the MIS weights and in the choice's 1/pmf."""
     sc = scenes.cornell_emitters(32, 32, spp=16)
     sc["select_scale"] = np.array([l["scale"] for l in sc["lights"]], np.float32)
     target = target_for(dict(sc, select_scale=None))
-    _, _, d_att, _ = O.render_grad(sc, target, attach_selection=True)
-    _, _, d_held, _ = O.render_grad(sc, target)
+    _, _, d_att, _ = O.render_grad(sc, target)
+    _, _, d_held, _ = O.render_grad(sc, target, hold_selection=True)
     for li in range(3):
         h = H * sc["lights"][li]["scale"]
         hi = dict(sc, lights=[dict(l) for l in sc["lights"]])
@@ -89,7 +91,7 @@ def test_emitter_gradient_with_frozen_light_choice():
         lo["lights"][li]["scale"] -= h
         fd = (loss_of(hi, target) - loss_of(lo, target)) / (2 * h)
         assert abs(fd - d_att[li]) <= 2e-3 * abs(fd) + 1e-3 * float(np.abs(d_att).max()), (li, fd, d_att[li])
-    # the held choice differs by the choice's own pmf derivative
+    # holding the choice's 1/pmf drops a term the differences see
     assert np.abs(d_att - d_held).max() > 1e-2 * np.abs(d_att).max()
 
 
