@@ -30,9 +30,10 @@ HBM_PEAK_GBS = 8000.0
 WNODE_BYTES = 260    # 8-wide node (256 B) + its octant order word (4 B), per visit
 BOX_BYTES = 24       # one child box re-read when a leaf shortened the ray
 TRI_BYTES = 48       # leaf-ordered triangle, 9 floats in a 48-byte slot
-# HBM bytes per trace_wide_kernel launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
-# scaled by a plain copy kernel; None until measured.
-TRAFFIC_BYTES_PER_LAUNCH = None
+# HBM bytes per trace_wide_kernel launch of this workload (mean of the primary and
+# bounce launches): rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors
+# read off a 256 MiB elementwise copy (--calibrate-copy).
+TRAFFIC_BYTES_PER_LAUNCH = 2_485_557_760
 RAY_BYTES = 32
 BUILD_BYTES_PER_TRI = 216  # algorithmic bytes of an HLBVH build per triangle
 
