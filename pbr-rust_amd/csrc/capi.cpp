@@ -825,19 +825,11 @@ int pbrtm_render_grad(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params
     CK(ctx, hipStreamSynchronize(st), "sync");
     if (derr) return fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr));
     *loss = hacc[ng + 1];
-    if (d_albedo) {  // J^T through the RGB-to-spectrum table, per diffuse material
-        for (uint32_t m = 0; m < nm; ++m) {
-            float *o = d_albedo + 3 * size_t(m);
-            o[0] = o[1] = o[2] = 0.f;
-            if (H.mats[m].type != pbrtm::pt::kDiffuse) continue;
-            double J[9];
-            pbrtm::pt::rgb_to_sigmoid_jacobian(&H.mat_rgb[3 * size_t(m)], J);
-            for (int k = 0; k < 3; ++k) {
-                double v = 0;
-                for (int q = 0; q < 3; ++q) v += J[3 * q + k] * double(coef[3 * size_t(m) + q]);
-                o[k] = float(v);
-            }
-        }
+    if (d_albedo) {  // J^T through the RGB-to-spectrum table (non-diffuse materials accumulate nothing)
+        if (nm) pbrtm::pt::rgb_to_sigmoid_jacobian_transpose(H.mat_rgb.data(), coef.data(), nm, d_albedo);
+        for (uint32_t m = 0; m < nm; ++m)
+            if (H.mats[m].type != pbrtm::pt::kDiffuse) d_albedo[3 * size_t(m)] = d_albedo[3 * size_t(m) + 1] =
+                d_albedo[3 * size_t(m) + 2] = 0.f;
     }
     if (d_scale) {  // dLoss/ds_m = (a_m - k P_m) / s_m, P_m = the emitter's total pmf
         std::vector<double> P(ng, 0.0);

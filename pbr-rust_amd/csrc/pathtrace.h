@@ -105,6 +105,8 @@ std::string build_scene_host(const void *desc, SceneHost &out);
 void rgb_to_sigmoid(const float rgb[3], float out[3]);
 // d(c0, c1, c2) / d(r, g, b) of rgb_to_sigmoid, row-major [q][k].
 void rgb_to_sigmoid_jacobian(const float rgb[3], double J[9]);
+// out[m] = J(rgb[m])^T coef[m] for n RGBs (multi-threaded).
+void rgb_to_sigmoid_jacobian_transpose(const float *rgb, const float *coef, size_t n, float *out);
 void srgb_matrices(float xyz_from_rgb[9], float rgb_from_xyz[9], float white_xy[2]);
 
 struct PathPool {

@@ -274,17 +274,48 @@ class SpectrumTable {
     // J[q][k] = d c_q / d rgb_k of lookup(): the trilinear interpolation's
     // derivative times that of the cell coordinates x = rgb[(m+1)%3] (res-1) / z,
     // y = rgb[(m+2)%3] (res-1) / z, z = rgb[m] (grey inputs take this general branch).
-    void jacobian(const float rgb[3], double J[3][3]) {
+    void jacobian(const float rgb[3], double J[3][3]) { jacobian_impl(rgb, J, true); }
+
+    // out[m][k] = sum_q J_m[q][k] coef[m][q] for n RGBs, on all host threads (the
+    // columns are computed first, so the threads only read the cache).
+    void jacobian_transpose(const float *rgb, const float *coef, size_t n, float *out) {
+        prefetch(rgb, n, true);
+        const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 64u));
+        std::vector<std::thread> ts;
+        for (unsigned t = 0; t < nt; ++t)
+            ts.emplace_back([&, t] {
+                for (size_t m = t; m < n; m += nt) {
+                    double J[3][3];
+                    jacobian_impl(rgb + 3 * m, J, false);
+                    for (int k = 0; k < 3; ++k) {
+                        double v = 0;
+                        for (int q = 0; q < 3; ++q) v += J[q][k] * double(coef[3 * m + q]);
+                        out[3 * m + k] = float(v);
+                    }
+                }
+            });
+        for (auto &th : ts) th.join();
+    }
+
+  private:
+    void jacobian_impl(const float rgb[3], double J[3][3], bool locked) {
         const int mc = (rgb[0] > rgb[1]) ? ((rgb[0] > rgb[2]) ? 0 : 2) : ((rgb[1] > rgb[2]) ? 1 : 2);
         const double z = rgb[mc];
+        if (!(z > 0) || rgb[0] < 0 || rgb[1] < 0 || rgb[2] < 0) {  // black: no general-branch cell
+            for (int q = 0; q < 9; ++q) J[q / 3][q % 3] = 0;
+            return;
+        }
         const double x = double(rgb[(mc + 1) % 3] * (kRes - 1)) / z;
         const double y = double(rgb[(mc + 2) % 3] * (kRes - 1)) / z;
         const int xi = std::min((int)x, kRes - 2), yi = std::min((int)y, kRes - 2);
         int zi = 0;
         while (zi < kRes - 2 && z_[zi + 1] < z) ++zi;
         const double dx = x - xi, dy = y - yi, h = z_[zi + 1] - z_[zi], dz = (z - z_[zi]) / h;
-        const Column &a = column(mc, yi, xi), &b = column(mc, yi, xi + 1);
-        const Column &c = column(mc, yi + 1, xi), &d = column(mc, yi + 1, xi + 1);
+        auto get = [&](int l, int j, int i) -> const Column & {
+            return locked ? column(l, j, i) : cols_.find((l * kRes + j) * kRes + i)->second;
+        };
+        const Column &a = get(mc, yi, xi), &b = get(mc, yi, xi + 1);
+        const Column &c = get(mc, yi + 1, xi), &d = get(mc, yi + 1, xi + 1);
         auto mix = [](double t, double u, double v) { return u + t * (v - u); };
         for (int q = 0; q < 3; ++q) {
             auto f = [&](const Column &col, int dk) { return double(float(col[zi + dk][q])); };
@@ -300,6 +331,8 @@ class SpectrumTable {
             J[q][(mc + 2) % 3] = ddy * (kRes - 1) / z;
         }
     }
+
+  public:
 
   private:
     using Column = std::array<std::array<double, 3>, kRes>;
@@ -398,14 +431,14 @@ class SpectrumTable {
     // Computes, on all host threads, the table columns that lookups of these RGBs
     // ([n][3]) will read and that are not cached yet (the reference fills the
     // whole table on a thread pool at start-up; here only what a scene needs).
-    void prefetch(const float *rgb, size_t n) {
+    void prefetch(const float *rgb, size_t n, bool include_grey = false) {
         std::vector<int> keys;
         {
             std::lock_guard<std::mutex> g(mu_);
             std::vector<char> want(3 * kRes * kRes, 0);
             for (size_t k = 0; k < n; ++k) {
                 const float *c = rgb + 3 * k;
-                if (c[0] == c[1] && c[1] == c[2]) continue;
+                if (!include_grey && c[0] == c[1] && c[1] == c[2]) continue;
                 const int mc = (c[0] > c[1]) ? ((c[0] > c[2]) ? 0 : 2) : ((c[1] > c[2]) ? 1 : 2);
                 const double z = c[mc];
                 if (!(z > 0)) continue;
@@ -539,6 +572,10 @@ float pc1d(const float *f, size_t n, float mn, float mx, float *func, float *cdf
 
 }  // namespace
 
+void rgb_to_sigmoid_jacobian_transpose(const float *rgb, const float *coef, size_t n, float *out) {
+    globals().table.jacobian_transpose(rgb, coef, n, out);
+}
+
 void rgb_to_sigmoid_jacobian(const float rgb[3], double J[9]) {
     double m[3][3];
     globals().table.jacobian(rgb, m);
@@ -592,7 +629,7 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
             const pbrtm_material &m = d.materials[i];
             DMat dm{};
             dm.type = m.type;
-            for (int q = 0; q < 3; ++q) S.mat_rgb[3 * size_t(i) + q] = m.rgb[q];
+            for (int q = 0; q < 3; ++q) S.mat_rgb[3 * size_t(i) + q] = std::max(0.f, m.rgb[q]);
             if (m.type == PBRTM_MATERIAL_DIFFUSE) {
                 if (m.rgb[0] > 1 || m.rgb[1] > 1 || m.rgb[2] > 1) return "diffuse reflectance RGB above 1";
                 const float c[3] = {std::max(0.f, m.rgb[0]), std::max(0.f, m.rgb[1]), std::max(0.f, m.rgb[2])};

@@ -282,21 +282,18 @@ __global__ __launch_bounds__(kCompactBlock) void compact_rays_kernel(const float
     }
 }
 
-// MULTI: every ray selects its own tree, jw[ray_job[i >> job_shift]] (jobs of a
-// batched render); the spill area stays the launch's.
+// One ray per lane. MULTI: every ray selects its own tree, jw[ray_job[i >> job_shift]]
+// (jobs of a batched render); the spill area stays the launch's.
 template <bool COUNT, bool MULTI>
-__global__ __launch_bounds__(kWideBlock) void trace_wide_kernel(WideTrace B, const float4 *__restrict__ rays,
-                                                                const uint32_t *__restrict__ queue,
-                                                                const uint32_t *__restrict__ queue_len, int any_hit,
-                                                                uint32_t *__restrict__ prim_out,
-                                                                float *__restrict__ t_out,
-                                                                float *__restrict__ bary_out,
-                                                                uint32_t *__restrict__ err,
-                                                                unsigned long long *__restrict__ counts,
-                                                                const WideTrace *__restrict__ jw,
-                                                                const uint32_t *__restrict__ ray_job,
-                                                                uint32_t job_shift) {
-    __shared__ uint2 s_stack[kLdsStack][kWideBlock];
+__device__ __forceinline__ void trace_wide_body(const WideTrace &B, const float4 *__restrict__ rays,
+                                                const uint32_t *__restrict__ queue,
+                                                const uint32_t *__restrict__ queue_len, int any_hit,
+                                                uint32_t *__restrict__ prim_out, float *__restrict__ t_out,
+                                                float *__restrict__ bary_out, uint32_t *__restrict__ err,
+                                                unsigned long long *__restrict__ counts,
+                                                const WideTrace *__restrict__ jw,
+                                                const uint32_t *__restrict__ ray_job, uint32_t job_shift,
+                                                uint2 (*s_stack)[kWideBlock]) {
     const uint32_t qi = blockIdx.x * kWideBlock + threadIdx.x;
     if (qi >= *queue_len) return;
     const uint32_t i = queue[qi];
@@ -439,6 +436,30 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide_kernel(WideTrace B, con
     }
 }
 
+// The occupancy targets come from measurement: the single-tree traversal runs
+// fastest at 6 waves per SIMD (80 VGPRs, a 20-byte spill), the per-ray-tree one at
+// 5 (96 VGPRs, no spill); the LDS stack (24 KiB per block) allows 6.
+template <bool COUNT>
+__global__ __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void trace_wide_kernel(
+    WideTrace B, const float4 *__restrict__ rays, const uint32_t *__restrict__ queue,
+    const uint32_t *__restrict__ queue_len, int any_hit, uint32_t *__restrict__ prim_out,
+    float *__restrict__ t_out, float *__restrict__ bary_out, uint32_t *__restrict__ err,
+    unsigned long long *__restrict__ counts) {
+    __shared__ uint2 s_stack[kLdsStack][kWideBlock];
+    trace_wide_body<COUNT, false>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err, counts,
+                                  nullptr, nullptr, 0, s_stack);
+}
+
+__global__ __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void trace_wide_multi_kernel(
+    WideTrace B, const float4 *__restrict__ rays, const uint32_t *__restrict__ queue,
+    const uint32_t *__restrict__ queue_len, int any_hit, uint32_t *__restrict__ prim_out,
+    float *__restrict__ t_out, float *__restrict__ bary_out, uint32_t *__restrict__ err,
+    const WideTrace *__restrict__ jw, const uint32_t *__restrict__ ray_job, uint32_t job_shift) {
+    __shared__ uint2 s_stack[kLdsStack][kWideBlock];
+    trace_wide_body<false, true>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err, nullptr, jw,
+                                 ray_job, job_shift, s_stack);
+}
+
 }  // namespace
 
 hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &scratch, uint32_t *err,
@@ -491,13 +512,11 @@ hipError_t trace_wide(const WideTrace &B, const float *rays, uint32_t n, int any
     const float4 *r = reinterpret_cast<const float4 *>(rays);
     compact_rays_kernel<<<cgrid, kCompactBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
     if (counts)
-        trace_wide_kernel<true, false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
-                                                                        t_out, bary_out, err, counts, nullptr,
-                                                                        nullptr, 0);
+        trace_wide_kernel<true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
+                                                                 bary_out, err, counts);
     else
-        trace_wide_kernel<false, false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
-                                                                         t_out, bary_out, err, nullptr, nullptr,
-                                                                         nullptr, 0);
+        trace_wide_kernel<false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
+                                                                  bary_out, err, nullptr);
     return hipGetLastError();
 }
 
@@ -510,9 +529,8 @@ hipError_t trace_wide_multi(const WideTrace &B, const WideTrace *jobs, const uin
     const uint32_t cgrid = (n + kCompactBlock * kCompactItems - 1) / (kCompactBlock * kCompactItems);
     const float4 *r = reinterpret_cast<const float4 *>(rays);
     compact_rays_kernel<<<cgrid, kCompactBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
-    trace_wide_kernel<false, true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
-                                                                    bary_out, err, nullptr, jobs, ray_job,
-                                                                    job_shift);
+    trace_wide_multi_kernel<<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
+                                                             bary_out, err, jobs, ray_job, job_shift);
     return hipGetLastError();
 }
 
