@@ -780,14 +780,15 @@ int pbrtm_render_grad(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params
     CK(ctx, sc->film_aux.reserve(wh * 16), "alloc");
     CK(ctx, sc->g_target.reserve(wh * 12), "alloc");
     CK(ctx, sc->g_adj.reserve(wh * 12), "alloc");
-    CK(ctx, sc->g_coef.reserve(size_t(std::max(nm, 1u)) * 12), "alloc");
+    const size_t coef_bytes = size_t(std::max(nm, 1u)) * 12 * pbrtm::pt::kGradShards;
+    CK(ctx, sc->g_coef.reserve(coef_bytes), "alloc");
     CK(ctx, sc->g_acc.reserve(size_t(ng + 2) * 8), "alloc");
     CK(ctx, sc->g_slot.reserve(size_t(pool) * 48), "alloc");
     CK(ctx, ctx->dev_err.reserve(64), "alloc");
     hipStream_t st = ctx->stream;
     CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, st), "memset");
     CK(ctx, hipMemcpyAsync(sc->g_target.ptr, target_rgb, wh * 12, hipMemcpyHostToDevice, st), "upload");
-    CK(ctx, hipMemsetAsync(sc->g_coef.ptr, 0, size_t(std::max(nm, 1u)) * 12, st), "memset");
+    CK(ctx, hipMemsetAsync(sc->g_coef.ptr, 0, coef_bytes, st), "memset");
     CK(ctx, hipMemsetAsync(sc->g_acc.ptr, 0, size_t(ng + 2) * 8, st), "memset");
     const WideTrace W = scene_trace(sc);
     float *film = sc->film_rgb.as<float>(), *aux = sc->film_aux.as<float>();
@@ -807,6 +808,7 @@ int pbrtm_render_grad(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params
     gd.pfin = pfin;
     gd.adj = sc->g_adj.as<float>();
     gd.d_coef = sc->g_coef.as<float>();
+    gd.ncoef = 3 * nm;
     gd.acc = acc;
     gd.g = sc->g_slot.as<float4>();
     gd.ltot = gd.g + pool;

@@ -64,11 +64,15 @@ struct SceneDev {
 struct GradDev {
     const float4 *lfin, *pfin;  // per path id: final L and wavelength pdfs of the forward pass
     const float *adj;           // per pixel [3]: dLoss/d(rgb_sum / weight_sum) through the output matrix
-    float *d_coef;              // per material [3]: dLoss/d(sigmoid coefficients), accumulated
+    float *d_coef;              // kGradShards x [materials][3]: dLoss/d(sigmoid coefficients); the
+                                // render leaves the sum in the first copy
+    uint32_t ncoef;             // 3 x materials
     double *acc;                // [ngroups + 1]: scale-linear part per emitter, then the pmf part
     float4 *g, *ltot, *nee;     // per slot: dLoss/dL, the final L, pending NEE (G, w, group)
     uint32_t ngroups;
 };
+
+constexpr uint32_t kGradShards = 32;
 
 // One job of a batched render, as the kernels read it.
 struct JobDev {

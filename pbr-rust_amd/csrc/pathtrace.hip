@@ -373,7 +373,7 @@ __device__ __forceinline__ float dot4(const S4 &a, const S4 &b) {
 template <bool MULTI, bool GRAD>
 __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, const Params &P, uint32_t i,
                                            uint32_t pid, uint32_t lid, uint32_t spd, uint32_t max_depth,
-                                           uint32_t round, double *s_acc) {
+                                           uint32_t round, float4 *erec, float4 *grec) {
     const bool live = pid != kIdle;
     uint32_t fl = live ? st.flags[i] : 0u;
     S4 L = live ? f4s(st.L[i]) : s4(0.f), beta = live ? f4s(st.beta[i]) : s4(0.f);
@@ -388,8 +388,7 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
                 // dC/dpmf = C (1 - 2w) / pmf (the MIS weight's pmf and the light choice's
                 // 1/pmf both attached), so a += G 2 (1 - w) and k += G (1 - 2w)
                 const float4 q = P.gd.nee[i];
-                atomicAdd(&s_acc[__float_as_uint(q.z)], double(q.x) * 2.0 * (1.0 - double(q.y)));
-                atomicAdd(&s_acc[P.gd.ngroups], double(q.x) * (1.0 - 2.0 * double(q.y)));
+                erec[0] = make_float4(q.x * 2.f * (1.f - q.y), q.x * (1.f - 2.f * q.y), q.z, 0.f);
             }
         }
         fl &= ~kFlagNee;
@@ -485,7 +484,7 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
         if (s4pos(Le)) {
             if (depth == 0 || (fl & kFlagSpecular)) {
                 L = s4add(L, s4mul(beta, Le));
-                if (GRAD) atomicAdd(&s_acc[Ld.group], double(dot4(gadj, s4mul(beta, Le))));
+                if (GRAD) erec[1] = make_float4(dot4(gadj, s4mul(beta, Le)), 0.f, __uint_as_float(Ld.group), 0.f);
             } else {
                 const float4 q0 = st.prev[3 * size_t(i)], q1 = st.prev[3 * size_t(i) + 1], q2 = st.prev[3 * size_t(i) + 2];
                 const P3i ppi{{q0.x, q0.y}, {q0.z, q0.w}, {q1.x, q1.y}};
@@ -495,8 +494,8 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
                 L = s4add(L, s4mul(s4scale(beta, wl), Le));
                 if (GRAD) {  // MIS weight attached: dC/dpmf = -2 C (1 - w) / pmf
                     const double G = dot4(gadj, s4mul(s4scale(beta, wl), Le));
-                    atomicAdd(&s_acc[Ld.group], G * (2.0 * double(wl) - 1.0));
-                    atomicAdd(&s_acc[P.gd.ngroups], -2.0 * G * (1.0 - double(wl)));
+                    erec[1] = make_float4(float(G * (2.0 * double(wl) - 1.0)), float(-2.0 * G * (1.0 - double(wl))),
+                                          __uint_as_float(Ld.group), 0.f);
                 }
             }
         }
@@ -533,10 +532,7 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
                 d1 += v * l;
                 d2 += v;
             }
-            float *dc = P.gd.d_coef + 3 * size_t(info.x);
-            if (d0 != 0.f) atomicAdd(dc, d0);
-            if (d1 != 0.f) atomicAdd(dc + 1, d1);
-            if (d2 != 0.f) atomicAdd(dc + 2, d2);
+            *grec = make_float4(d0, d1, d2, __uint_as_float(info.x));  // added by the kernel, wave-aggregated
         }
     } else if (m.type == kConductor) {
         for (int k = 0; k < kNS; ++k) {
@@ -640,7 +636,7 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
 __global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Params P) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pid = i < P.pool ? st.path_id[i] : kIdle;
-    shade_impl<false, false>(S, st, P, i, pid, pid, P.spd, P.max_depth, P.round, nullptr);
+    shade_impl<false, false>(S, st, P, i, pid, pid, P.spd, P.max_depth, P.round, nullptr, nullptr);
 }
 
 // The gradient replay of one round (same paths as the forward render).
@@ -651,7 +647,68 @@ __global__ __launch_bounds__(kBlock) void shade_grad_kernel(SceneDev S, State st
     __syncthreads();
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pid = i < P.pool ? st.path_id[i] : kIdle;
-    shade_impl<false, true>(S, st, P, i, pid, pid, P.spd, P.max_depth, P.round, s_acc);
+    float4 grec = make_float4(0.f, 0.f, 0.f, __uint_as_float(kIdle));
+    float4 erec[2] = {make_float4(0.f, 0.f, __uint_as_float(kIdle), 0.f),
+                      make_float4(0.f, 0.f, __uint_as_float(kIdle), 0.f)};
+    shade_impl<false, true>(S, st, P, i, pid, pid, P.spd, P.max_depth, P.round, erec, &grec);
+    // Emitter sums: the pmf part over the whole wave, the scale part per emitter
+    // (lanes of one emitter summed first), into shared memory.
+    {
+        double kk = double(erec[0].y) + double(erec[1].y);
+        for (int d = kWave / 2; d > 0; d >>= 1) kk += __shfl_xor(kk, d, kWave);
+        if (lane_id() == 0 && kk != 0.0) atomicAdd(&s_acc[P.gd.ngroups], kk);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const uint32_t grp = __float_as_uint(erec[e].z);
+            bool act = grp != kIdle && erec[e].x != 0.f;
+            for (int r = 0; r < 4; ++r) {
+                const uint64_t pend = __ballot(act);
+                if (!pend) break;
+                const int leader = __ffsll((unsigned long long)pend) - 1;
+                const uint32_t g0 = __shfl(grp, leader, kWave);
+                const bool mine = act && grp == g0;
+                double v = mine ? double(erec[e].x) : 0.0;
+                for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+                if (lane_id() == leader) atomicAdd(&s_acc[g0], v);
+                if (mine) act = false;
+            }
+            if (act) atomicAdd(&s_acc[grp], double(erec[e].x));
+        }
+    }
+    // Coefficient derivatives: lanes of a wave that hit the same material (walls,
+    // large triangles) are summed first, for up to three materials per wave, the
+    // rest add directly; each block adds into one of kGradShards copies of the
+    // table (summed afterwards), so that the atomics on a few hot materials from
+    // every wave of the launch do not serialise on one address.
+    float *shard = P.gd.d_coef + size_t(blockIdx.x % kGradShards) * P.gd.ncoef;
+    uint32_t mat = __float_as_uint(grec.w);
+    bool act = mat != kIdle && (grec.x != 0.f || grec.y != 0.f || grec.z != 0.f);
+    for (int r = 0; r < 3; ++r) {
+        const uint64_t pend = __ballot(act);
+        if (!pend) break;
+        const int leader = __ffsll((unsigned long long)pend) - 1;
+        const uint32_t m0 = __shfl(mat, leader, kWave);
+        const bool mine = act && mat == m0;
+        float a = mine ? grec.x : 0.f, b = mine ? grec.y : 0.f, c = mine ? grec.z : 0.f;
+        for (int d = kWave / 2; d > 0; d >>= 1) {
+            a += __shfl_xor(a, d, kWave);
+            b += __shfl_xor(b, d, kWave);
+            c += __shfl_xor(c, d, kWave);
+        }
+        if (lane_id() == leader) {
+            float *dc = shard + 3 * size_t(m0);
+            atomicAdd(dc, a);
+            atomicAdd(dc + 1, b);
+            atomicAdd(dc + 2, c);
+        }
+        if (mine) act = false;
+    }
+    if (act) {
+        float *dc = shard + 3 * size_t(mat);
+        atomicAdd(dc, grec.x);
+        atomicAdd(dc + 1, grec.y);
+        atomicAdd(dc + 2, grec.z);
+    }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k <= P.gd.ngroups; k += kBlock)
         if (s_acc[k] != 0.0) atomicAdd(&P.gd.acc[k], s_acc[k]);
@@ -665,7 +722,17 @@ __global__ __launch_bounds__(kBlock) void shade_multi_kernel(State st, Params P)
     const uint32_t j = pid != kIdle ? st.job[i] : 0u;
     const JobDev &J = P.jobs[j];
     const uint32_t lid = pid != kIdle ? uint32_t(pid - J.base) : kIdle;
-    shade_impl<true, false>(J.S, st, P, i, pid, lid, J.spd, J.max_depth, st.counters[kCntRound], nullptr);
+    shade_impl<true, false>(J.S, st, P, i, pid, lid, J.spd, J.max_depth, st.counters[kCntRound], nullptr,
+                            nullptr);
+}
+
+// Sums the kGradShards copies of the coefficient-derivative table into the first.
+__global__ __launch_bounds__(kBlock) void reduce_shards_kernel(float *d, uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float v = d[i];
+    for (uint32_t k = 1; k < kGradShards; ++k) v += d[size_t(k) * n + i];
+    d[i] = v;
 }
 
 // Start and end of a batched round (kernels, so that a captured graph holds no memset).
@@ -1005,6 +1072,10 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
     }
     PT_CK(hipEventRecord(ev[5], stream));
     const uint32_t wh = S.width * S.height;
+    if (gd && gd->ncoef) {
+        reduce_shards_kernel<<<(gd->ncoef + kBlock - 1) / kBlock, kBlock, 0, stream>>>(gd->d_coef, gd->ncoef);
+        PT_CK(hipGetLastError());
+    }
     if (film_rgb) {
         film_kernel<<<(wh + kFilmWaves - 1) / kFilmWaves, kFilmWaves * kWave, 0, stream>>>(
             S, st.rec, st.rec_round, spp, film_rgb, film_aux);
