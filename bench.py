@@ -27,13 +27,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "pbr-rust_amd"))
 
 HBM_PEAK_GBS = 8000.0
-WNODE_BYTES = 260    # 8-wide node (256 B) + its octant order word (4 B), per visit
-BOX_BYTES = 24       # one child box re-read when a leaf shortened the ray
+WNODE_BYTES = 132    # 8-wide quantized node (128 B) + its octant order word (4 B), per visit
+BOX_BYTES = 22       # one child box re-read when a leaf shortened the ray (6 plane bytes + 16 B header)
 TRI_BYTES = 48       # leaf-ordered triangle, 9 floats in a 48-byte slot
 # HBM bytes per trace_wide_kernel launch of this workload (mean of the primary and
 # bounce launches): rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors
-# read off a 256 MiB elementwise copy (--calibrate-copy).
-TRAFFIC_BYTES_PER_LAUNCH = 2_485_557_760
+# read off a 256 MiB elementwise copy (--calibrate-copy); None until measured for
+# the current node layout.
+TRAFFIC_BYTES_PER_LAUNCH = None
 RAY_BYTES = 32
 BUILD_BYTES_PER_TRI = 216  # algorithmic bytes of an HLBVH build per triangle
 
@@ -329,7 +330,7 @@ def main():
             "traffic": TRAFFIC_BYTES_PER_LAUNCH if (args.tris, args.width, args.height, args.seed)
             == (10_000_000, 1920, 1080, 0x5EED) else None,
             "bytes_per_launch": int((bytes1 + bytes2) / 2),
-            "algorithmic_bytes": "32 B ray + 20/8 B hit + 260 B per 8-wide node visit + 24 B per box "
+            "algorithmic_bytes": "32 B ray + 20/8 B hit + 132 B per 8-wide node visit + 22 B per box "
                                  "re-test + 48 B per triangle test"}
 
     line = {

@@ -10,17 +10,22 @@ constexpr int kWide = 8;
 constexpr uint32_t kWideLeaf = 0x80000000u;
 constexpr uint32_t kWideEmpty = 0xFFFFFFFFu;
 
-// 256 bytes = two 128-byte cache lines: child boxes as SoA planes, then one
-// (word0, word1) reference per child slot. Interior child: word0 = wide node
-// index, word1 = 0. Leaf child: word0 = kWideLeaf | first leaf position,
-// word1 = primitive count. Unused slot: word0 = kWideEmpty, empty box.
-// Slots hold the collapsed binary subtree's frontier in left-to-right order.
+// 128 bytes = one cache line. Child boxes are quantized to 8 bits per plane
+// against the node's origin (the union's lower corner) and a power-of-two step
+// per axis: plane = fmaf(q, 2^exp, org), rounded outwards at encoding so that
+// every decoded box contains the child's exact FP32 box. Then one (word0,
+// word1) reference per child slot. Interior child: word0 = wide node index,
+// word1 = 0. Leaf child: word0 = kWideLeaf | first leaf position, word1 =
+// primitive count. Unused slot: word0 = kWideEmpty, inverted box. Slots hold
+// the collapsed binary subtree's frontier in left-to-right order.
 struct alignas(128) WNode {
-    float lo_x[kWide], lo_y[kWide], lo_z[kWide];
-    float hi_x[kWide], hi_y[kWide], hi_z[kWide];
+    float org[3];
+    int8_t exp[3];
+    uint8_t pad;
+    uint8_t q[6][kWide];  // lo_x, lo_y, lo_z, hi_x, hi_y, hi_z per slot
     uint32_t ref[kWide][2];
 };
-static_assert(sizeof(WNode) == 256, "WNode is two cache lines");
+static_assert(sizeof(WNode) == 128, "WNode is one cache line");
 
 // Per wide node and ray octant (bit k = direction k negative): the order in
 // which the reference's depth-first, near-child-first binary traversal reaches

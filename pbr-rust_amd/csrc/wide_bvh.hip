@@ -4,7 +4,8 @@
 // Collapse: each wide node stands for one binary interior node; its frontier
 // is grown by repeatedly opening the frontier interior node of largest surface
 // area until 8 slots are filled (the greedy collapse of Wald et al. 2008 /
-// Ylitie et al. 2017). Child boxes are the binary nodes' exact FP32 boxes.
+// Ylitie et al. 2017). Child boxes are the binary nodes' exact FP32 boxes,
+// quantized outwards to 8 bits per plane (one 128-byte node).
 //
 // Traversal keeps the reference's semantics exactly (accelerator/hlbvh.cu:179-289):
 // for every ray octant a wide node stores the order in which the binary
@@ -13,8 +14,11 @@
 // the moment the binary traversal would pop it. Box tests are monotone under
 // box inclusion, so skipping the collapsed inner binary boxes changes no
 // decision: the sequence of triangle tests, and therefore every hit (ties
-// included), equals the binary traversal's. Each visit costs one 256-byte node
-// (two cache lines) instead of up to seven dependent 32-byte fetches.
+// included), equals the binary traversal's. Quantized boxes only contain the
+// exact ones, so they add box entries whose subtrees hold no triangle the ray
+// hits before the current best t (the slab test is conservative): extra node
+// visits and triangle tests, never a different hit or tie winner. Each visit
+// costs one 128-byte node instead of up to seven dependent 32-byte fetches.
 #include "pbrtm_geometry.h"
 #include "wave_ops.h"
 #include "wide_bvh.h"
@@ -27,6 +31,57 @@ struct Job {
     uint32_t bnode, wnode, depth, pad;
 };
 
+// 2^e for e in [-126, 127], exactly.
+__device__ __forceinline__ float pow2f(int e) { return __uint_as_float(uint32_t(e + 127) << 23); }
+
+// Encodes n (<= 8) child boxes into w's origin, exponents and quantized planes;
+// unused slots get the inverted box (lo 255, hi 0). ulo / uhi: the union of the
+// child boxes (the collapsed binary node's own box); box(s, lo, hi) fetches child
+// s. Steps are powers of two, so the divisions below are exact products.
+template <typename BoxOf>
+__device__ __forceinline__ void encode_node(WNode *w, const float ulo[3], const float uhi[3], BoxOf box, int n) {
+    float sc[3], inv[3];
+    uint32_t ex = 0u;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float ext = uhi[a] - ulo[a];
+        int e = -126;
+        if (ext > 0.f) {  // smallest power-of-two step with ext / step <= 253
+            int e2;
+            frexpf(ext / 253.0f, &e2);
+            e = e2;
+            if (ldexpf(1.0f, e2 - 1) * 253.0f >= ext) e = e2 - 1;
+        }
+        e = max(-126, min(126, e));
+        sc[a] = pow2f(e);
+        inv[a] = pow2f(-e);
+        ex |= (uint32_t(e) & 0xFFu) << (8 * a);
+    }
+    uint64_t plo[3] = {0, 0, 0}, phi[3] = {0, 0, 0};
+#pragma unroll 1
+    for (int s = 0; s < kWide; ++s) {
+        float l[3] = {0.f, 0.f, 0.f}, h[3] = {0.f, 0.f, 0.f};
+        if (s < n) box(s, l, h);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            int ql = 255, qh = 0;
+            if (s < n) {
+                ql = int(fminf(255.f, fmaxf(0.f, floorf((l[a] - ulo[a]) * inv[a]))));
+                while (ql > 0 && fmaf(float(ql), sc[a], ulo[a]) > l[a]) --ql;
+                qh = int(fminf(255.f, fmaxf(0.f, ceilf((h[a] - ulo[a]) * inv[a]))));
+                while (qh < 255 && fmaf(float(qh), sc[a], ulo[a]) < h[a]) ++qh;
+            }
+            plo[a] |= uint64_t(ql) << (8 * s);
+            phi[a] |= uint64_t(qh) << (8 * s);
+        }
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(w);
+    dst[0] = make_uint4(__float_as_uint(ulo[0]), __float_as_uint(ulo[1]), __float_as_uint(ulo[2]), ex);
+    dst[1] = make_uint4(uint32_t(plo[0]), uint32_t(plo[0] >> 32), uint32_t(plo[1]), uint32_t(plo[1] >> 32));
+    dst[2] = make_uint4(uint32_t(plo[2]), uint32_t(plo[2] >> 32), uint32_t(phi[0]), uint32_t(phi[0] >> 32));
+    dst[3] = make_uint4(uint32_t(phi[1]), uint32_t(phi[1] >> 32), uint32_t(phi[2]), uint32_t(phi[2] >> 32));
+}
+
 __device__ __forceinline__ Box bnode_box(const BNode &b) {
     return {{b.lo[0], b.lo[1], b.lo[2]}, {b.hi[0], b.hi[1], b.hi[2]}};
 }
@@ -36,17 +91,14 @@ __global__ void wide_root_kernel(const BNode *bn, Job *jobs, uint32_t *counters,
     if (threadIdx.x != 0) return;
     const BNode r = bn[0];
     if (bnode_leaf(r)) {  // the whole tree is one leaf: a wide root with one leaf slot
-        WNode w;
+        encode_node(wn, r.lo, r.hi, [&](int, float *l, float *h) {
+            for (int a = 0; a < 3; ++a) l[a] = r.lo[a], h[a] = r.hi[a];
+        }, 1);
         for (int s = 0; s < kWide; ++s) {
             const bool use = s == 0;
-            w.lo_x[s] = use ? r.lo[0] : 3.402823466e+38f; w.lo_y[s] = use ? r.lo[1] : 3.402823466e+38f;
-            w.lo_z[s] = use ? r.lo[2] : 3.402823466e+38f;
-            w.hi_x[s] = use ? r.hi[0] : -3.402823466e+38f; w.hi_y[s] = use ? r.hi[1] : -3.402823466e+38f;
-            w.hi_z[s] = use ? r.hi[2] : -3.402823466e+38f;
-            w.ref[s][0] = use ? r.a : kWideEmpty;
-            w.ref[s][1] = use ? r.b : 0u;
+            wn[0].ref[s][0] = use ? r.a : kWideEmpty;
+            wn[0].ref[s][1] = use ? r.b : 0u;
         }
-        wn[0] = w;
         counters[1] = 1;  // nodes used
         counters[0] = 0;  // no jobs
         return;
@@ -60,7 +112,7 @@ constexpr int kCollapseBlock = 64;
 
 // Frontier state lives in registers: every array below is indexed only with
 // compile-time indices (fully unrolled loops with selects).
-__global__ __launch_bounds__(kCollapseBlock) void wide_collapse_kernel(
+__global__ __launch_bounds__(kCollapseBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void wide_collapse_kernel(
     const BNode *__restrict__ bn, const Job *__restrict__ jobs, uint32_t njobs, Job *__restrict__ next,
     uint32_t *__restrict__ counters, WNode *__restrict__ wn, uint32_t *__restrict__ order, uint32_t cap,
     uint32_t *__restrict__ err) {
@@ -135,32 +187,44 @@ __global__ __launch_bounds__(kCollapseBlock) void wide_collapse_kernel(
         atomicOr(err, uint32_t(kErrCapacity));
         return;
     }
-    WNode w;
+    uint32_t refw[kWide][2];
     uint32_t k_int = 0;
 #pragma unroll
     for (int s = 0; s < kWide; ++s) {
         if (s < nF) {
             const BNode c = bn[fid[s]];
-            w.lo_x[s] = c.lo[0]; w.lo_y[s] = c.lo[1]; w.lo_z[s] = c.lo[2];
-            w.hi_x[s] = c.hi[0]; w.hi_y[s] = c.hi[1]; w.hi_z[s] = c.hi[2];
             if (farea[s] >= 0.0f) {
                 const uint32_t idx = base + k_int;
                 next[jbase + k_int] = Job{fid[s], idx, job.depth + 1, 0};
                 ++k_int;
-                w.ref[s][0] = idx;
-                w.ref[s][1] = 0u;
+                refw[s][0] = idx;
+                refw[s][1] = 0u;
             } else {
-                w.ref[s][0] = c.a;  // kLeafFlag | first (same bit as kWideLeaf)
-                w.ref[s][1] = c.b;
+                refw[s][0] = c.a;  // kLeafFlag | first (same bit as kWideLeaf)
+                refw[s][1] = c.b;
             }
         } else {
-            w.lo_x[s] = w.lo_y[s] = w.lo_z[s] = 3.402823466e+38f;
-            w.hi_x[s] = w.hi_y[s] = w.hi_z[s] = -3.402823466e+38f;
-            w.ref[s][0] = kWideEmpty;
-            w.ref[s][1] = 0u;
+            refw[s][0] = kWideEmpty;
+            refw[s][1] = 0u;
         }
     }
-    wn[job.wnode] = w;
+    // child boxes re-read from the binary nodes (cached) rather than held in registers;
+    // their union is the collapsed node's own box
+    auto fidx = [&](int s) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < kWide; ++k) v = (k == s) ? fid[k] : v;
+        return v;
+    };
+    WNode *w = wn + job.wnode;
+    const BNode root = bn[job.bnode];
+    encode_node(w, root.lo, root.hi, [&](int s, float *l, float *h) {
+        const BNode c = bn[fidx(s)];
+        for (int a = 0; a < 3; ++a) l[a] = c.lo[a], h[a] = c.hi[a];
+    }, nF);
+    uint4 *wr = reinterpret_cast<uint4 *>(w->ref);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wr[k] = make_uint4(refw[2 * k][0], refw[2 * k][1], refw[2 * k + 1][0], refw[2 * k + 1][1]);
     // depth-first, near-first slot order per octant (the binary traversal's order).
     // Codes: 0..7 = frontier slot, 8..14 = opened node; the DFS stack is eight
     // 4-bit codes packed in one register.
@@ -332,17 +396,30 @@ __device__ __forceinline__ void trace_wide_body(const WideTrace &B, const float4
             const WNode *W = nodes + node;
             const uint32_t perm = order[node * 8 + oct];
             if (COUNT) { ++visits; boxes += kWide; }
-            // all eight child boxes (12 x 16-byte loads), hit mask in slot order
-            const float4 *q = reinterpret_cast<const float4 *>(W);
+            // all eight child boxes (4 x 16-byte loads: origin, exponents, 48 plane
+            // bytes), decoded to FP32 planes, hit mask in slot order
+            const uint4 *q = reinterpret_cast<const uint4 *>(W);
+            const uint4 hd = q[0];
+            const float org[3] = {__uint_as_float(hd.x), __uint_as_float(hd.y), __uint_as_float(hd.z)};
+            const float scl[3] = {pow2f(int(int8_t(hd.w & 0xFFu))), pow2f(int(int8_t((hd.w >> 8) & 0xFFu))),
+                                  pow2f(int(int8_t((hd.w >> 16) & 0xFFu)))};
+            uint32_t qw[12];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint4 v = q[1 + k];
+                qw[4 * k] = v.x;
+                qw[4 * k + 1] = v.y;
+                qw[4 * k + 2] = v.z;
+                qw[4 * k + 3] = v.w;
+            }
             float pl[6][kWide];
 #pragma unroll
-            for (int k = 0; k < 12; ++k) {
-                const float4 v = q[k];
-                pl[k / 2][(k % 2) * 4 + 0] = v.x;
-                pl[k / 2][(k % 2) * 4 + 1] = v.y;
-                pl[k / 2][(k % 2) * 4 + 2] = v.z;
-                pl[k / 2][(k % 2) * 4 + 3] = v.w;
-            }
+            for (int pp = 0; pp < 6; ++pp)
+#pragma unroll
+                for (int s = 0; s < kWide; ++s) {
+                    const uint32_t b = (qw[(pp * kWide + s) / 4] >> (8 * ((pp * kWide + s) % 4))) & 0xFFu;
+                    pl[pp][s] = fmaf(float(b), scl[pp % 3], org[pp % 3]);
+                }
             uint32_t hs = 0;
 #pragma unroll
             for (int s = 0; s < kWide; ++s) {
@@ -365,8 +442,10 @@ __device__ __forceinline__ void trace_wide_body(const WideTrace &B, const float4
                 const uint32_t s = (perm >> (4 * p)) & 0xFu;
                 if (best_t != t_at_visit) {  // a leaf of this node shortened the ray: re-test at pop time
                     if (COUNT) ++boxes;
-                    const float e = box_entry(W->lo_x[s], W->lo_y[s], W->lo_z[s], W->hi_x[s], W->hi_y[s],
-                                              W->hi_z[s], o, inv, neg);
+                    float bx[6];
+#pragma unroll
+                    for (int pp = 0; pp < 6; ++pp) bx[pp] = fmaf(float(W->q[pp][s]), scl[pp % 3], org[pp % 3]);
+                    const float e = box_entry(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], o, inv, neg);
                     if (!(e < best_t)) continue;
                 }
                 const uint32_t ref0 = W->ref[s][0];

@@ -101,9 +101,11 @@ def test_intersect_and_bounce_parity_1m(ctx):
     bp, bt, bb = bvh.intersect(rays, stats=bstats, binary=True)
     assert np.array_equal(bstats[[0, 2]], ostats)
     assert np.array_equal(bp, op) and np.array_equal(bt, ot) and np.array_equal(bb, obary)
-    # 8-wide traversal: fewer node visits, exactly the same triangle tests
+    # 8-wide traversal: fewer node visits; the quantized child boxes contain the
+    # exact ones, so the triangle tests are a superset of the binary traversal's
+    # (a few per cent more) and every hit is the same
     log(f"wide stats {stats.tolist()} binary {bstats.tolist()}")
-    assert stats[2] == ostats[1]
+    assert ostats[1] <= stats[2] <= 1.15 * ostats[1]
     assert stats[0] < ostats[0] / 2
     assert np.array_equal(prim, op)
     assert np.array_equal(t, ot)
@@ -131,7 +133,7 @@ def test_intersect_and_bounce_parity_1m(ctx):
     assert np.array_equal(p2, q2)
     assert np.array_equal(t2, u2)
     assert np.array_equal(b2, c2)
-    assert s2[2] == os2[1]
+    assert os2[1] <= s2[2] <= 1.15 * os2[1]
     log("any hit")
     pa, _, _ = bvh.intersect(br, any_hit=True, want_bary=False)
     qa, _, _, _ = ob.intersect(br, any_hit=True)
