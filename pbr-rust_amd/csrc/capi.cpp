@@ -647,6 +647,11 @@ int pbrtm_scene_create(pbrtm_ctx *ctx, const pbrtm_scene_desc *desc, pbrtm_scene
     D.cie_n = H.cie_n;
     D.d65_off = H.d65_off;
     D.d65_n = H.d65_n;
+    D.cie_uni = H.cie_uni;
+    D.d65_uni = H.d65_uni;
+    D.cie_shared = H.cie_shared;
+    D.cie_inv_step = H.cie_inv_step;
+    D.d65_inv_step = H.d65_inv_step;
     std::memcpy(D.cam_from_raster, H.cam_from_raster, sizeof(D.cam_from_raster));
     std::memcpy(D.render_from_camera, H.render_from_camera, sizeof(D.render_from_camera));
     std::memcpy(D.cam_origin, H.cam_origin, sizeof(D.cam_origin));

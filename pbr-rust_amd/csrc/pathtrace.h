@@ -46,6 +46,10 @@ struct SceneDev {
     uint32_t num_tris, num_lights;
     const float *spec_lam, *spec_val;  // piecewise-linear spectrum pool
     uint32_t cie_off[3], cie_n, d65_off, d65_n;
+    // uniform tails of the CIE and D65 wavelength grids (start index, 1 / step) for the
+    // interval guess; cie_shared: x, y and z use the same grid
+    uint32_t cie_uni, d65_uni, cie_shared;
+    float cie_inv_step, d65_inv_step;
     float cam_from_raster[16], render_from_camera[16];
     float cam_origin[3];
     const float *filt_f;         // [64][64] filter values
@@ -90,6 +94,8 @@ struct SceneHost {
     std::vector<float> light_pmf, light_cdf;
     std::vector<float> spec_lam, spec_val;
     uint32_t cie_off[3], cie_n, d65_off, d65_n;
+    uint32_t cie_uni, d65_uni, cie_shared;
+    float cie_inv_step, d65_inv_step;
     float cam_from_raster[16], render_from_camera[16], cam_origin[3];
     std::vector<float> filt_f, filt_cond_func, filt_cond_cdf, filt_cond_int, filt_marg_func, filt_marg_cdf;
     float filt_marg_int, filt_radius;

@@ -88,6 +88,38 @@ __device__ __forceinline__ float spec_eval(const SceneDev &S, uint32_t off, uint
     return pl_eval(S.spec_lam + off, S.spec_val + off, n, l);
 }
 
+// pl_eval with the interval found by find_interval_guess (same result).
+__device__ __forceinline__ float spec_eval_uni(const SceneDev &S, uint32_t off, uint32_t n, uint32_t a,
+                                               float inv_step, float l) {
+    const float *lam = S.spec_lam + off, *val = S.spec_val + off;
+    if (n == 0 || l < lam[0] || l > lam[n - 1]) return 0;
+    const size_t o = find_interval_guess(lam, n, a, inv_step, l);
+    const float t = (l - lam[o]) / (lam[o + 1] - lam[o]);
+    return lerpf(t, val[o], val[o + 1]);
+}
+
+// x, y, z of the CIE matching functions at l (one interval search when they share a grid).
+__device__ __forceinline__ void cie_eval(const SceneDev &S, float l, float (&xyz)[3]) {
+    if (!S.cie_shared) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xyz[c] = spec_eval(S, S.cie_off[c], S.cie_n, l);
+        return;
+    }
+    const uint32_t n = S.cie_n;
+    const float *lam = S.spec_lam + S.cie_off[0];
+    if (n == 0 || l < lam[0] || l > lam[n - 1]) {
+        xyz[0] = xyz[1] = xyz[2] = 0.f;
+        return;
+    }
+    const size_t o = find_interval_guess(lam, n, S.cie_uni, S.cie_inv_step, l);
+    const float t = (l - lam[o]) / (lam[o + 1] - lam[o]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *val = S.spec_val + S.cie_off[c];
+        xyz[c] = lerpf(t, val[o], val[o + 1]);
+    }
+}
+
 // Piecewise-constant 1D sample (util/piecewise_constant_1d.h:61-86).
 __device__ __forceinline__ float pc1d_sample(const float *func, const float *cdf, uint32_t n, float func_int,
                                              float mn, float mx, float u, float *pdf, int *off) {
@@ -159,7 +191,7 @@ __device__ __forceinline__ S4 light_L(const SceneDev &S, const DLight &L, f3 n, 
     S4 s, d;
     for (int i = 0; i < kNS; ++i) {
         s.v[i] = L.rgb_scale * sigmoid_eval(L.c0, L.c1, L.c2, lam.v[i]);
-        d.v[i] = spec_eval(S, S.d65_off, S.d65_n, lam.v[i]);
+        d.v[i] = spec_eval_uni(S, S.d65_off, S.d65_n, S.d65_uni, S.d65_inv_step, lam.v[i]);
     }
     return s4scale(s4mul(d, s), L.scale);
 }
@@ -176,7 +208,7 @@ __device__ __forceinline__ P3i sample_point(const Tri &t, const float b[3]) {
     return p3i_make(p, perr);
 }
 // Triangle::sample(ctx, u) (shapes/triangle.cu:126-215) and its area-sampling branch (:78-124).
-__device__ bool tri_sample(const Tri &t, const P3i &cpi, f3 cns, float u0, float u1, ShapeSample *ss) {
+__device__ __forceinline__ bool tri_sample(const Tri &t, const P3i &cpi, f3 cns, float u0, float u1, ShapeSample *ss) {
     const f3 cp = p3i_mid(cpi);
     const f3 v[3] = {t.p0, t.p1, t.p2};
     const float sa = spherical_triangle_area(norm3(sub3(t.p0, cp)), norm3(sub3(t.p1, cp)), norm3(sub3(t.p2, cp)));
@@ -222,7 +254,7 @@ __device__ bool tri_sample(const Tri &t, const P3i &cpi, f3 cns, float u0, float
     return true;
 }
 // Triangle::pdf(ctx, wi) (shapes/triangle.cu:20-76).
-__device__ float tri_pdf(const Tri &t, const P3i &cpi, f3 cn, f3 cns, f3 wi) {
+__device__ __forceinline__ float tri_pdf(const Tri &t, const P3i &cpi, f3 cn, f3 cns, f3 wi) {
     const f3 cp = p3i_mid(cpi);
     const f3 v[3] = {t.p0, t.p1, t.p2};
     const float sa = spherical_triangle_area(norm3(sub3(t.p0, cp)), norm3(sub3(t.p1, cp)), norm3(sub3(t.p2, cp)));
@@ -265,10 +297,15 @@ __device__ __forceinline__ void store_ray(float4 *r, f3 o, float tmax, f3 d) {
 // PixelSensor::to_sensor_rgb (films/pixel_sensor.h:33-40), imaging ratio 1.
 __device__ __forceinline__ float3 sensor_rgb(const SceneDev &S, const S4 &L, const S4 &lam, const S4 &pdf) {
     const S4 s = s4safediv(L, pdf);
+    float xyz[kNS][3];
+#pragma unroll
+    for (int i = 0; i < kNS; ++i) cie_eval(S, lam.v[i], xyz[i]);
     float c[3];
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
         S4 x;
-        for (int i = 0; i < kNS; ++i) x.v[i] = spec_eval(S, S.cie_off[k], S.cie_n, lam.v[i]);
+#pragma unroll
+        for (int i = 0; i < kNS; ++i) x.v[i] = xyz[i][k];
         c[k] = 1.0f * s4avg(s4mul(x, s));
     }
     return make_float3(c[0], c[1], c[2]);
@@ -314,9 +351,12 @@ __device__ __forceinline__ void start_path(const SceneDev &S, const State &st, u
 #pragma unroll
         for (int k = 0; k < kNS; ++k) {
             float v = 0.f;
-            if (pf.v[k] != 0.f)
+            if (pf.v[k] != 0.f) {
+                float xyz[3];
+                cie_eval(S, lam.v[k], xyz);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) v += A[c] * spec_eval(S, S.cie_off[c], S.cie_n, lam.v[k]) / pf.v[k];
+                for (int c = 0; c < 3; ++c) v += A[c] * xyz[c] / pf.v[k];
+            }
             g.v[k] = fw * v * (1.0f / kNS);
         }
         g_g[i] = s4f(g);

@@ -101,6 +101,18 @@ PBRTM_HD size_t find_interval_le(const float *lam, size_t sz, float l) {
     v = v > (long)sz - 2 ? (long)sz - 2 : v;
     return (size_t)v;
 }
+// find_interval_le for a grid whose entries from index a on are evenly spaced
+// (1 / step = inv_step): the guessed interval is checked against the grid, so the
+// result is the binary search's in every case.
+PBRTM_HD size_t find_interval_guess(const float *lam, size_t n, uint32_t a, float inv_step, float l) {
+    const float la = lam[a];
+    if (inv_step > 0.f && l >= la) {
+        long g = long(a) + long((l - la) * inv_step);
+        g = g > long(n) - 2 ? long(n) - 2 : g;
+        if (lam[g] <= l && (g == long(n) - 2 || lam[g + 1] > l)) return size_t(g);
+    }
+    return find_interval_le(lam, n, l);
+}
 PBRTM_HD float pl_eval(const float *lam, const float *val, uint32_t n, float l) {
     if (n == 0 || l < lam[0] || l > lam[n - 1]) return 0;
     const size_t o = find_interval_le(lam, n, l);

@@ -611,6 +611,22 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
         S.cie_n = uint32_t(G.cie[0].lam.size());
         S.d65_off = pool_add(S, G.d65);
         S.d65_n = uint32_t(G.d65.lam.size());
+        // the longest uniformly spaced tail of a grid: its first index and 1 / step
+        auto uniform_tail = [](const std::vector<float> &lam, uint32_t *start, float *inv_step) {
+            const size_t n = lam.size();
+            *start = uint32_t(n ? n - 1 : 0);
+            *inv_step = 0.f;
+            if (n < 3) return;
+            const float d = lam[n - 1] - lam[n - 2];
+            if (!(d > 0)) return;
+            size_t a = n - 2;
+            while (a > 0 && lam[a] - lam[a - 1] == d) --a;
+            *start = uint32_t(a);
+            *inv_step = 1.0f / d;
+        };
+        uniform_tail(G.cie[0].lam, &S.cie_uni, &S.cie_inv_step);
+        uniform_tail(G.d65.lam, &S.d65_uni, &S.d65_inv_step);
+        S.cie_shared = (G.cie[0].lam == G.cie[1].lam && G.cie[0].lam == G.cie[2].lam) ? 1u : 0u;
         int metal_off[8], glass_off = -1;
         for (int i = 0; i < 8; ++i) metal_off[i] = -1;
         // materials (materials/diffuse_material.cu:22-29, conductor_material.cu:10-76)
