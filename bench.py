@@ -116,6 +116,7 @@ def pathtracer(args, ctx, pbrtm):
         "stages_ms": {k: round(float(np.mean([r[k] for r in runs])), 2)
                       for k in ("ms_trace", "ms_shadow", "ms_shade", "ms_film")},
         "rounds": last["rounds"], "rays": last["rays"], "shadow_rays": last["shadow_rays"],
+        "ray_order": {"closest_sorted": bool(last["ray_order"] & 1), "shadow_sorted": bool(last["ray_order"] & 2)},
         "mrays_per_s": round((last["rays"] + last["shadow_rays"]) / (ms * 1e3), 1),
         "config": {"workload": "sphere grid 32x32 UV spheres, 2,097,218 tris, 64 emitters", "width": W,
                    "height": H, "spp": spp, "max_depth": depth, "seed": 0xC0FFEE, "dtype": "f32"},

@@ -199,6 +199,8 @@ typedef struct pbrtm_render_params {
 typedef struct pbrtm_render_stats {
     double ms_total, ms_trace, ms_shadow, ms_shade, ms_film;
     uint64_t rounds, rays, shadow_rays, paths;
+    uint32_t ray_order;  /* the ray-order autotuner's choice: bit 0 closest-hit rays sorted, bit 1 shadow rays */
+    uint32_t reserved;
 } pbrtm_render_stats;
 
 /* Builds the scene tables and its aggregate on the context's device. Replaces

@@ -177,7 +177,8 @@ void pbrtm_destroy(pbrtm_ctx *ctx) {
     for (DevBuf *b : {&p.path_id, &p.flags, &p.rng, &p.dim, &p.lam, &p.lam_pdf, &p.L, &p.beta, &p.nee_L, &p.scal,
                       &p.prev, &p.rays, &p.prim, &p.thit, &p.bary, &p.srays, &p.sprim, &p.rec, &p.rec_round,
                       &p.counters, &p.queue, &p.squeue, &p.spill, &p.job, &p.sround, &ctx->batch_jobs,
-                      &ctx->batch_films})
+                      &ctx->batch_films, &p.rsort.keys[0], &p.rsort.keys[1], &p.rsort.vals[0], &p.rsort.vals[1],
+                      &p.rsort.hist, &p.rsort.sort.status, &p.rsort.sort.offsets, &p.rsort.sort.counters})
         b->release();
     for (auto &e : ctx->ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
@@ -590,7 +591,9 @@ void pbrtm_scene_free(pbrtm_scene *sc) {
     auto &p = sc->pool;
     for (pbrtm::DevBuf *b : {&p.path_id, &p.flags, &p.rng, &p.dim, &p.lam, &p.lam_pdf, &p.L, &p.beta, &p.nee_L,
                              &p.scal, &p.prev, &p.rays, &p.prim, &p.thit, &p.bary, &p.srays, &p.sprim, &p.rec,
-                             &p.rec_round, &p.counters, &p.queue, &p.squeue, &p.spill, &p.job, &p.sround})
+                             &p.rec_round, &p.counters, &p.queue, &p.squeue, &p.spill, &p.job, &p.sround,
+                             &p.rsort.keys[0], &p.rsort.keys[1], &p.rsort.vals[0], &p.rsort.vals[1], &p.rsort.hist,
+                             &p.rsort.sort.status, &p.rsort.sort.offsets, &p.rsort.sort.counters})
         b->release();
     delete sc;
 }
@@ -691,6 +694,10 @@ WideTrace scene_trace(const pbrtm_scene *sc) {
     W.num_prims = bvh->n;
     W.max_depth = bvh->wide.max_depth;
     W.spill_per_ray = W.max_depth > kLdsStack ? W.max_depth - kLdsStack + 1 : 0;
+    for (int a = 0; a < 3; ++a) {
+        W.lo[a] = bvh->bounds[a];
+        W.hi[a] = bvh->bounds[3 + a];
+    }
     return W;
 }
 
@@ -759,6 +766,8 @@ int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *par
         stats->rays = rs.rays;
         stats->shadow_rays = rs.shadow_rays;
         stats->paths = rs.paths;
+        stats->ray_order = rs.sorted;
+        stats->reserved = 0;
     }
     return PBRTM_OK;
 }

@@ -122,6 +122,7 @@ void srgb_matrices(float xyz_from_rgb[9], float rgb_from_xyz[9], float white_xy[
 struct PathPool {
     DevBuf path_id, flags, rng, dim, lam, lam_pdf, L, beta, nee_L, scal, prev, rays, prim, thit, bary, srays, sprim,
         rec, rec_round, counters, queue, squeue, spill, job, sround;
+    RaySortScratch rsort;
     uint32_t cap = 0;
     uint64_t rec_cap = 0;
 };
@@ -129,6 +130,7 @@ struct PathPool {
 struct RenderStats {
     double ms_total = 0, ms_trace = 0, ms_shadow = 0, ms_shade = 0, ms_film = 0;
     uint64_t rounds = 0, rays = 0, shadow_rays = 0, paths = 0;
+    uint32_t sorted = 0;  // the ray-order autotuner's choice: 1 closest-hit rays sorted, 2 shadow rays sorted
 };
 
 // Renders the whole image; film_rgb (device, [h][w][3]) and film_aux (device,

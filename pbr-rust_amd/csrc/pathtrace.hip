@@ -23,6 +23,11 @@ namespace pt {
 namespace {
 
 constexpr uint32_t kBlock = 256;
+// Ray ordering autotuner: rounds kTuneFirst .. kTuneFirst + 3 alternate sorted
+// and unsorted queues (closest-hit and shadow rays separately); the cheaper
+// time per ray over each pair decides the rest of the render. The film does not
+// depend on the order.
+constexpr uint32_t kTuneFirst = 2;
 constexpr uint32_t kFlagSpecular = 1u << 16, kFlagAnyNonSpec = 1u << 17, kFlagNee = 1u << 18;
 constexpr uint32_t kDepthMask = 0xFFFFu;
 constexpr uint32_t kIdle = 0xFFFFFFFFu;
@@ -1080,16 +1085,44 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
     RenderStats rs;
     uint32_t host_cnt[16];
     float t_trace = 0, t_shadow = 0, t_shade = 0;
+    struct {
+        bool sort_closest = true, sort_shadow = true;
+        double ms[2][2] = {{0, 0}, {0, 0}};
+        uint64_t rays[2][2] = {{0, 0}, {0, 0}};
+    } tune;
     for (;;) {
         PT_CK(hipMemsetAsync(st.counters + kCntQueue, 0, 8, stream));
         PT_CK(hipEventRecord(ev[1], stream));
-        PT_CK(trace_wide(W, reinterpret_cast<const float *>(st.rays), pool, 0, pp.prim.as<uint32_t>(),
-                         pp.thit.as<float>(), pp.bary.as<float>(), err, nullptr, pp.queue.as<uint32_t>(),
-                         st.counters + kCntQueue, stream));
+        const bool tuning = P.round >= kTuneFirst && P.round < kTuneFirst + 4;
+        const bool sort_c = tuning ? ((P.round - kTuneFirst) % 2 == 0) : tune.sort_closest;
+        const bool sort_s = tuning ? ((P.round - kTuneFirst) % 2 == 0) : tune.sort_shadow;
+        if (sort_c) {  // closest-hit rays in (octant, origin Morton code) order
+            uint32_t *q = nullptr;
+            PT_CK(sort_rays(W, reinterpret_cast<const float *>(st.rays), pool, pp.prim.as<uint32_t>(),
+                            pp.thit.as<float>(), pp.bary.as<float>(), err, pp.rsort, st.counters + kCntQueue, &q,
+                            stream));
+            PT_CK(trace_wide_queued(W, reinterpret_cast<const float *>(st.rays), pool, 0, pp.prim.as<uint32_t>(),
+                                    pp.thit.as<float>(), pp.bary.as<float>(), err, q, st.counters + kCntQueue,
+                                    stream));
+        } else {
+            PT_CK(trace_wide(W, reinterpret_cast<const float *>(st.rays), pool, 0, pp.prim.as<uint32_t>(),
+                             pp.thit.as<float>(), pp.bary.as<float>(), err, nullptr, pp.queue.as<uint32_t>(),
+                             st.counters + kCntQueue, stream));
+        }
         PT_CK(hipEventRecord(ev[2], stream));
         if (spill_need) W.spill = pp.spill.as<uint2>();
-        PT_CK(trace_wide(W, reinterpret_cast<const float *>(st.srays), 2 * pool, 1, pp.sprim.as<uint32_t>(), nullptr,
-                         nullptr, err, nullptr, pp.squeue.as<uint32_t>(), st.counters + kCntSQueue, stream));
+        if (sort_s) {
+            uint32_t *q = nullptr;
+            PT_CK(sort_rays(W, reinterpret_cast<const float *>(st.srays), 2 * pool, pp.sprim.as<uint32_t>(), nullptr,
+                            nullptr, err, pp.rsort, st.counters + kCntSQueue, &q, stream));
+            PT_CK(trace_wide_queued(W, reinterpret_cast<const float *>(st.srays), 2 * pool, 1,
+                                    pp.sprim.as<uint32_t>(), nullptr, nullptr, err, q, st.counters + kCntSQueue,
+                                    stream));
+        } else {
+            PT_CK(trace_wide(W, reinterpret_cast<const float *>(st.srays), 2 * pool, 1, pp.sprim.as<uint32_t>(),
+                             nullptr, nullptr, err, nullptr, pp.squeue.as<uint32_t>(), st.counters + kCntSQueue,
+                             stream));
+        }
         PT_CK(hipEventRecord(ev[3], stream));
         if (gd) shade_grad_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);
         else shade_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);
@@ -1105,6 +1138,21 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         t_trace += a;
         t_shadow += b;
         t_shade += c;
+        if (tuning) {
+            const int k = (P.round - kTuneFirst) % 2;  // 0 sorted, 1 unsorted
+            tune.ms[0][k] += a;
+            tune.rays[0][k] += host_cnt[kCntQueue];
+            tune.ms[1][k] += b;
+            tune.rays[1][k] += host_cnt[kCntSQueue];
+            if (P.round == kTuneFirst + 3) {
+                auto faster_sorted = [&](int q) {
+                    return tune.rays[q][0] && tune.rays[q][1] &&
+                           tune.ms[q][0] / double(tune.rays[q][0]) < tune.ms[q][1] / double(tune.rays[q][1]);
+                };
+                tune.sort_closest = faster_sorted(0);
+                tune.sort_shadow = faster_sorted(1);
+            }
+        }
         rs.rays += host_cnt[kCntQueue];
         rs.shadow_rays += host_cnt[kCntSQueue];
         P.round++;
@@ -1132,6 +1180,7 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
     rs.ms_shade = t_shade;
     rs.ms_film = tf;
     rs.rounds = P.round;
+    rs.sorted = (tune.sort_closest ? 1u : 0u) | (tune.sort_shadow ? 2u : 0u);
     rs.paths = total;
     if (stats) *stats = rs;
     return hipSuccess;

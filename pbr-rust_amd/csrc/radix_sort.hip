@@ -173,6 +173,13 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_pass_kernel(
 hipError_t onesweep_sort_30(uint32_t *keys0, uint32_t *vals0, uint32_t *keys1, uint32_t *vals1,
                             uint32_t n, const uint32_t *hist, uint32_t *err, SortScratch &s,
                             hipStream_t stream, uint32_t **keys_result, uint32_t **vals_result) {
+    return onesweep_sort(keys0, vals0, keys1, vals1, n, hist, err, s, stream, keys_result, vals_result, kSortPasses);
+}
+
+hipError_t onesweep_sort(uint32_t *keys0, uint32_t *vals0, uint32_t *keys1, uint32_t *vals1, uint32_t n,
+                         const uint32_t *hist, uint32_t *err, SortScratch &s, hipStream_t stream,
+                         uint32_t **keys_result, uint32_t **vals_result, uint32_t passes) {
+    if (passes == 0 || passes > kSortPasses) return hipErrorInvalidValue;
     *keys_result = keys0;
     *vals_result = vals0;
     if (n == 0) return hipSuccess;
@@ -189,14 +196,14 @@ hipError_t onesweep_sort_30(uint32_t *keys0, uint32_t *vals0, uint32_t *keys1, u
     fill_u32_kernel<<<1, 64, 0, stream>>>(counters, 64, 0u);
     digit_offsets_kernel<<<1, kSortRadix, 0, stream>>>(hist, offsets);
     uint32_t *kin = keys0, *vin = vals0, *kout = keys1, *vout = vals1;
-    for (uint32_t p = 0; p < kSortPasses; ++p) {
+    for (uint32_t p = 0; p < passes; ++p) {
         onesweep_pass_kernel<<<tiles, kSortBlock, 0, stream>>>(
             kin, vin, kout, vout, n, p * kSortRadixBits, offsets + p * kSortRadix,
             status + uint64_t(p) * tiles * kSortRadix, counters + p, err);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    *keys_result = kin;  // an odd pass count leaves the result in buffer 1
+    *keys_result = kin;  // an odd pass count leaves the result in buffer 1, an even one in buffer 0
     *vals_result = vin;
     return hipGetLastError();
 }

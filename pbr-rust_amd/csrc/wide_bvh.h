@@ -49,6 +49,7 @@ struct WideTrace {
     uint32_t num_nodes, num_prims, max_depth;
     uint2 *spill;           // per-ray stack overflow area (max_depth - kLdsStack entries per ray), may be null
     uint32_t spill_per_ray;
+    float lo[3], hi[3];     // bounds of the whole tree (ray ordering)
 };
 
 // counts (device, nullable): adds [wide nodes visited, child box tests, triangle tests].
@@ -64,6 +65,22 @@ hipError_t trace_wide_multi(const WideTrace &B, const WideTrace *jobs, const uin
                             const float *rays, uint32_t n, int any_hit, uint32_t *prim_out, float *t_out,
                             float *bary_out, uint32_t *err, uint32_t *queue, uint32_t *queue_len,
                             hipStream_t stream);
+
+// Orders the active rays of [n] by (direction octant, Morton code of the origin
+// on a 64^3 grid over the tree's box; 20-bit keys, two radix passes) into *queue
+// (a pointer into the scratch, n words) and counts them into *queue_len (zeroed
+// by the caller); inactive rays report a miss. trace_wide_queued then traces that
+// queue.
+struct RaySortScratch {
+    DevBuf keys[2], vals[2], hist;
+    SortScratch sort;
+};
+hipError_t sort_rays(const WideTrace &B, const float *rays, uint32_t n, uint32_t *prim_out, float *t_out,
+                     float *bary_out, uint32_t *err, RaySortScratch &s, uint32_t *queue_len, uint32_t **queue,
+                     hipStream_t stream);
+hipError_t trace_wide_queued(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
+                             float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
+                             const uint32_t *queue_len, hipStream_t stream);
 
 constexpr uint32_t kLdsStack = 12;
 

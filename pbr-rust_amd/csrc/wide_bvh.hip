@@ -23,6 +23,8 @@
 #include "wave_ops.h"
 #include "wide_bvh.h"
 
+#include <algorithm>
+
 namespace pbrtm {
 
 namespace {
@@ -596,6 +598,100 @@ hipError_t trace_wide(const WideTrace &B, const float *rays, uint32_t n, int any
     else
         trace_wide_kernel<false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
                                                                   bary_out, err, nullptr);
+    return hipGetLastError();
+}
+
+namespace {
+
+__device__ __forceinline__ uint32_t spread3_6(uint32_t v) {  // 6 bits -> every third bit
+    v &= 0x3Fu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void ray_keys_kernel(const float4 *__restrict__ rays, uint32_t n, WideTrace B,
+                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                       uint32_t *__restrict__ hist, uint32_t *__restrict__ count,
+                                                       uint32_t *__restrict__ prim_out, float *__restrict__ t_out,
+                                                       float *__restrict__ bary_out) {
+    __shared__ uint32_t s_hist[kSortPasses * kSortRadix];
+    for (int i = threadIdx.x; i < int(kSortPasses * kSortRadix); i += blockDim.x) s_hist[i] = 0u;
+    __syncthreads();
+    const float sx = B.hi[0] > B.lo[0] ? 64.f / (B.hi[0] - B.lo[0]) : 0.f;
+    const float sy = B.hi[1] > B.lo[1] ? 64.f / (B.hi[1] - B.lo[1]) : 0.f;
+    const float sz = B.hi[2] > B.lo[2] ? 64.f / (B.hi[2] - B.lo[2]) : 0.f;
+    uint32_t mine = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 r0 = rays[2 * size_t(i)];
+        uint32_t key = 0xFFFFFu;  // inactive: after every active ray (active keys are < 2^19)
+        if (r0.w > 0.f) {
+            const float4 r1 = rays[2 * size_t(i) + 1];
+            const uint32_t qx = uint32_t(fminf(63.f, fmaxf(0.f, (r0.x - B.lo[0]) * sx)));
+            const uint32_t qy = uint32_t(fminf(63.f, fmaxf(0.f, (r0.y - B.lo[1]) * sy)));
+            const uint32_t qz = uint32_t(fminf(63.f, fmaxf(0.f, (r0.z - B.lo[2]) * sz)));
+            const uint32_t oct = (r1.x < 0.f ? 1u : 0u) | (r1.y < 0.f ? 2u : 0u) | (r1.z < 0.f ? 4u : 0u);
+            const uint32_t m = (spread3_6(qx) << 2) | (spread3_6(qy) << 1) | spread3_6(qz);  // 18 bits
+            key = (oct << 16) | (m >> 2);  // < 2^19: below the inactive key, two sort passes
+            ++mine;
+        } else {
+            prim_out[i] = kNoHit;
+            if (t_out) t_out[i] = 0.f;
+            if (bary_out) {
+                bary_out[3 * size_t(i)] = 0.f;
+                bary_out[3 * size_t(i) + 1] = 0.f;
+                bary_out[3 * size_t(i) + 2] = 0.f;
+            }
+        }
+        keys[i] = key;
+        vals[i] = i;
+        atomicAdd(&s_hist[key & 1023u], 1u);
+        atomicAdd(&s_hist[kSortRadix + ((key >> 10) & 1023u)], 1u);
+    }
+    const uint32_t tot = wave_sum(mine);
+    if (__lane_id() == 0 && tot) atomicAdd(count, tot);
+    __syncthreads();
+    for (int i = threadIdx.x; i < int(kSortPasses * kSortRadix); i += blockDim.x)
+        if (s_hist[i]) atomicAdd(&hist[i], s_hist[i]);
+}
+
+}  // namespace
+
+hipError_t sort_rays(const WideTrace &B, const float *rays, uint32_t n, uint32_t *prim_out, float *t_out,
+                     float *bary_out, uint32_t *err, RaySortScratch &s, uint32_t *queue_len, uint32_t **queue,
+                     hipStream_t stream) {
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = s.keys[k].reserve(size_t(n) * 4);
+        if (e == hipSuccess) e = s.vals[k].reserve(size_t(n) * 4);
+    }
+    if (e == hipSuccess) e = s.hist.reserve(kSortPasses * kSortRadix * 4);
+    if (e != hipSuccess) return e;
+    *queue = s.vals[0].as<uint32_t>();
+    if (n == 0) return hipSuccess;
+    e = hipMemsetAsync(s.hist.ptr, 0, kSortPasses * kSortRadix * 4, stream);
+    if (e != hipSuccess) return e;
+    const uint32_t grid = std::min((n + 255) / 256, 1024u);
+    ray_keys_kernel<<<grid, 256, 0, stream>>>(reinterpret_cast<const float4 *>(rays), n, B, s.keys[0].as<uint32_t>(),
+                                              s.vals[0].as<uint32_t>(), s.hist.as<uint32_t>(), queue_len, prim_out,
+                                              t_out, bary_out);
+    uint32_t *kr = nullptr, *vr = nullptr;
+    e = onesweep_sort(s.keys[0].as<uint32_t>(), s.vals[0].as<uint32_t>(), s.keys[1].as<uint32_t>(),
+                      s.vals[1].as<uint32_t>(), n, s.hist.as<uint32_t>(), err, s.sort, stream, &kr, &vr, 2);
+    *queue = vr;
+    return e;
+}
+
+hipError_t trace_wide_queued(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
+                             float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
+                             const uint32_t *queue_len, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (n + kWideBlock - 1) / kWideBlock;
+    trace_wide_kernel<false><<<grid, kWideBlock, 0, stream>>>(B, reinterpret_cast<const float4 *>(rays), queue,
+                                                              queue_len, any_hit, prim_out, t_out, bary_out, err,
+                                                              nullptr);
     return hipGetLastError();
 }
 

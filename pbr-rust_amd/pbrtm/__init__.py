@@ -77,7 +77,7 @@ class RenderParams(ctypes.Structure):
 class RenderStats(ctypes.Structure):
     _fields_ = [("ms_total", c_double), ("ms_trace", c_double), ("ms_shadow", c_double), ("ms_shade", c_double),
                 ("ms_film", c_double), ("rounds", c_uint64), ("rays", c_uint64), ("shadow_rays", c_uint64),
-                ("paths", c_uint64)]
+                ("paths", c_uint64), ("ray_order", c_uint32), ("reserved", c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
