@@ -173,6 +173,20 @@ def batch(args, ctx, pbrtm):
     finally:
         for h in handles:
             h.free()
+    # the same comparison on 64 small jobs (64..256 square), where one job alone
+    # leaves most of the GPU idle
+    small = scenes.batch_jobs(args.batch_jobs, seed=0x5A11, res=(64, 256), tris=(10_000, 100_000))
+    sh = [ctx.scene(sc) for sc, _, _ in small]
+    try:
+        sjobs = [(h, spp, depth, 0) for h, (_, spp, depth) in zip(sh, small)]
+        for h, spp, depth, _ in sjobs:
+            h.render(spp=spp, max_depth=depth)
+        small_one = sum(h.render(spp=spp, max_depth=depth)[2]["ms_total"] for h, spp, depth, _ in sjobs)
+        ctx.render_batch(sjobs)
+        small_batch = ctx.render_batch(sjobs)[3]["ms_total"]
+    finally:
+        for h in sh:
+            h.free()
     return {
         "metric": "batched render throughput (64 mixed jobs)",
         "value": round(paths / (st["ms_total"] * 1e3), 2),
@@ -189,6 +203,11 @@ def batch(args, ctx, pbrtm):
                       "film": round(st["ms_film"], 2)},
         "physical_rounds": st["rounds"], "graph_launches": st["launches"], "paths": paths,
         "scene_setup_s": round(setup_s, 2),
+        "solo_jobs": st["solo_jobs"],
+        "small_jobs": {"ms_one_by_one": round(small_one, 2), "ms_batch": round(small_batch, 2),
+                       "speedup_vs_one_by_one": round(small_one / small_batch, 3),
+                       "workload": "64 jobs: square resolution U[64, 256], spp in {1, 4, 16, 64}, depth 1..8, "
+                                   "soup rooms of 1e4..1e5 triangles, seed 0x5A11"},
         "config": {"workload": "64 jobs: square resolution U[64, 1024], spp in {1, 4, 16, 64}, max depth 1..8, "
                                "soup_room scenes of 1e4..1e6 triangles (log-uniform)", "seed": 0xBA7C4,
                    "dtype": "f32"},

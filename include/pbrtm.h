@@ -264,13 +264,15 @@ typedef struct pbrtm_batch_params {
     uint32_t pool_size;           /* slots shared by the batch; 0 = 2^22 (capped at the batch's path count) */
     uint32_t rounds_per_launch;   /* rounds per graph launch between done checks; 0 = 8 */
     uint32_t flags;               /* PBRTM_DEVICE_PTRS (films are device pointers), PBRTM_BATCH_NO_GRAPH */
-    uint32_t reserved;
+    uint32_t solo_paths;          /* jobs with at least this many paths render alone, before the shared
+                                     pool (they fill the GPU by themselves); 0 = 2^21, 0xFFFFFFFF = never */
 } pbrtm_batch_params;
 
 typedef struct pbrtm_batch_stats {
-    double ms_total, ms_rounds, ms_schedule, ms_film;
+    double ms_total, ms_rounds, ms_schedule, ms_film;  /* ms_rounds .. ms_film: the shared pool's */
     uint64_t rounds, rays, shadow_rays, paths;
     uint32_t pool_size, launches;
+    uint32_t solo_jobs, reserved;
 } pbrtm_batch_stats;
 
 int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pbrtm_batch_params *params,

@@ -914,7 +914,9 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
     CK(ctx, ctx->dev_err.reserve(64), "alloc");
     CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream), "memset");
     pbrtm::pt::BatchStats bs;
-    CK(ctx, pbrtm::pt::render_batch(bj, pool, rpl, !(params->flags & PBRTM_BATCH_NO_GRAPH), ctx->batch_pool,
+    const uint64_t solo = params->solo_paths == 0 ? (2u << 20)
+                          : params->solo_paths == 0xFFFFFFFFu ? ~uint64_t(0) : params->solo_paths;
+    CK(ctx, pbrtm::pt::render_batch(bj, pool, rpl, !(params->flags & PBRTM_BATCH_NO_GRAPH), solo, ctx->batch_pool,
                                     ctx->batch_jobs, ctx->dev_err.as<uint32_t>(), &bs, ctx->stream),
        "render batch");
     uint32_t derr = 0;
@@ -943,6 +945,8 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
         stats->paths = bs.paths;
         stats->pool_size = uint32_t(std::min<uint64_t>(pool, paths));
         stats->launches = bs.launches;
+        stats->solo_jobs = bs.solo;
+        stats->reserved = 0;
     }
     return PBRTM_OK;
 }

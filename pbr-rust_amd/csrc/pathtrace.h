@@ -158,16 +158,19 @@ struct BatchJob {
 struct BatchStats {
     double ms_total = 0, ms_rounds = 0, ms_schedule = 0, ms_film = 0;
     uint64_t rounds = 0, rays = 0, shadow_rays = 0, paths = 0;
-    uint32_t launches = 0;
+    uint32_t launches = 0, solo = 0;
     std::vector<uint32_t> job_rounds;  // each job's round count in its one-by-one schedule
 };
 
-// Renders every job in one shared pool of `pool` slots; `rounds_per_launch`
-// rounds are enqueued (as one captured graph when use_graph) between checks
-// of the device's done flag. Each job's film equals its one-by-one render
-// with pool size vpool.
+// Renders the jobs with at least solo_paths paths one after another with the
+// single-scene wavefront (such a job fills the GPU by itself, and mixing it with
+// others only shares the caches between trees), then every other job in one
+// shared pool of `pool` slots; `rounds_per_launch` rounds are enqueued (as one
+// captured graph when use_graph) between checks of the device's done flag.
+// Each job's film equals its one-by-one render with pool size vpool.
 hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32_t rounds_per_launch, bool use_graph,
-                        PathPool &pp, DevBuf &jobbuf, uint32_t *err, BatchStats *stats, hipStream_t stream);
+                        uint64_t solo_paths, PathPool &pp, DevBuf &jobbuf, uint32_t *err, BatchStats *stats,
+                        hipStream_t stream);
 
 }  // namespace pt
 }  // namespace pbrtm

@@ -1052,6 +1052,7 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         if ((e = (x)) != hipSuccess) return e; \
     } while (0)
     PT_CK(reserve_pool(pp, pool, total, false));
+    PT_CK(reserve_ray_sort(pp.rsort, 2 * pool));  // sized once: no reallocation while rounds are in flight
     WideTrace W = W0;
     const uint32_t spill_need = W.spill_per_ray;
     if (spill_need) {
@@ -1187,8 +1188,11 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
 #undef PT_CK
 }
 
-hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32_t rounds_per_launch, bool use_graph,
-                        PathPool &pp, DevBuf &jobbuf, uint32_t *err, BatchStats *stats, hipStream_t stream) {
+namespace {
+
+// Every job of `jobs` in one shared pool (the batched wavefront).
+hipError_t render_shared(const std::vector<BatchJob> &jobs, uint32_t pool, uint32_t rounds_per_launch, bool use_graph,
+                         PathPool &pp, DevBuf &jobbuf, uint32_t *err, BatchStats *stats, hipStream_t stream) {
     const uint32_t nj = uint32_t(jobs.size());
     if (nj == 0 || pool == 0 || rounds_per_launch == 0) return hipErrorInvalidValue;
     std::vector<JobDev> jd(nj);
@@ -1223,6 +1227,7 @@ hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32
         if ((e = (x)) != hipSuccess) return e; \
     } while (0)
     PT_CK(reserve_pool(pp, pool, total, true));
+    PT_CK(reserve_ray_sort(pp.rsort, 2 * pool));  // the captured rounds allocate nothing
     WideTrace W{};
     W.spill_per_ray = spill_need;
     if (spill_need) {
@@ -1269,13 +1274,23 @@ hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32
         for (uint32_t k = 0; k < rounds_per_launch; ++k) {
             round_begin_kernel<<<1, 64, 0, stream>>>(st.counters);
             if ((e2 = hipGetLastError()) != hipSuccess) return e2;
-            if ((e2 = trace_wide_multi(W, d_jw, st.job, 0, reinterpret_cast<const float *>(st.rays), pool, 0,
-                                       pp.prim.as<uint32_t>(), pp.thit.as<float>(), pp.bary.as<float>(), err,
-                                       pp.queue.as<uint32_t>(), st.counters + kCntQueue, stream)) != hipSuccess)
+            // both ray classes in (job, octant, origin Morton code) order
+            uint32_t *q = nullptr;
+            if ((e2 = sort_rays(W, reinterpret_cast<const float *>(st.rays), pool, pp.prim.as<uint32_t>(),
+                                pp.thit.as<float>(), pp.bary.as<float>(), err, pp.rsort, st.counters + kCntQueue, &q,
+                                stream, st.job, 0)) != hipSuccess)
                 return e2;
-            if ((e2 = trace_wide_multi(W, d_jw, st.job, 1, reinterpret_cast<const float *>(st.srays), 2 * pool, 1,
-                                       pp.sprim.as<uint32_t>(), nullptr, nullptr, err, pp.squeue.as<uint32_t>(),
-                                       st.counters + kCntSQueue, stream)) != hipSuccess)
+            if ((e2 = trace_wide_multi_queued(W, d_jw, st.job, 0, reinterpret_cast<const float *>(st.rays), pool, 0,
+                                              pp.prim.as<uint32_t>(), pp.thit.as<float>(), pp.bary.as<float>(), err,
+                                              q, st.counters + kCntQueue, stream)) != hipSuccess)
+                return e2;
+            if ((e2 = sort_rays(W, reinterpret_cast<const float *>(st.srays), 2 * pool, pp.sprim.as<uint32_t>(),
+                                nullptr, nullptr, err, pp.rsort, st.counters + kCntSQueue, &q, stream, st.job, 1)) !=
+                hipSuccess)
+                return e2;
+            if ((e2 = trace_wide_multi_queued(W, d_jw, st.job, 1, reinterpret_cast<const float *>(st.srays),
+                                              2 * pool, 1, pp.sprim.as<uint32_t>(), nullptr, nullptr, err, q,
+                                              st.counters + kCntSQueue, stream)) != hipSuccess)
                 return e2;
             shade_multi_kernel<<<grid, kBlock, 0, stream>>>(st, P);
             round_end_kernel<<<1, 64, 0, stream>>>(st.counters);
@@ -1345,6 +1360,70 @@ hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32
     }
     return hipSuccess;
 #undef PT_CK
+}
+
+}  // namespace
+
+hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32_t rounds_per_launch, bool use_graph,
+                        uint64_t solo_paths, PathPool &pp, DevBuf &jobbuf, uint32_t *err, BatchStats *stats,
+                        hipStream_t stream) {
+    const uint32_t nj = uint32_t(jobs.size());
+    if (nj == 0) return hipErrorInvalidValue;
+    hipEvent_t ev[2];
+    hipError_t e;
+    for (auto &x : ev)
+        if ((e = hipEventCreate(&x)) != hipSuccess) return e;
+    struct Guard {
+        hipEvent_t *e;
+        ~Guard() {
+            hipEventDestroy(e[0]);
+            hipEventDestroy(e[1]);
+        }
+    } guard{ev};
+    BatchStats out;
+    out.job_rounds.assign(nj, 0);
+    std::vector<BatchJob> shared;
+    std::vector<uint32_t> shared_idx;
+    if ((e = hipEventRecord(ev[0], stream)) != hipSuccess) return e;
+    for (uint32_t j = 0; j < nj; ++j) {
+        const BatchJob &b = jobs[j];
+        const uint64_t n = uint64_t(b.S.width) * b.S.height * b.spp;
+        if (n >= solo_paths) {  // fills the pool by itself: the single-scene wavefront, in its own pool order
+            RenderStats rs;
+            e = render(b.S, b.W, b.spp, b.max_depth, b.vpool, pp, b.film_rgb, b.film_aux, err, &rs, stream);
+            if (e != hipSuccess) return e;
+            out.job_rounds[j] = uint32_t(rs.rounds);
+            out.rounds += rs.rounds;
+            out.rays += rs.rays;
+            out.shadow_rays += rs.shadow_rays;
+            out.paths += rs.paths;
+            out.solo += 1;
+        } else {
+            shared.push_back(b);
+            shared_idx.push_back(j);
+        }
+    }
+    if (!shared.empty()) {
+        BatchStats bs;
+        e = render_shared(shared, pool, rounds_per_launch, use_graph, pp, jobbuf, err, &bs, stream);
+        if (e != hipSuccess) return e;
+        for (size_t k = 0; k < shared.size(); ++k) out.job_rounds[shared_idx[k]] = bs.job_rounds[k];
+        out.rounds += bs.rounds;
+        out.rays += bs.rays;
+        out.shadow_rays += bs.shadow_rays;
+        out.paths += bs.paths;
+        out.launches = bs.launches;
+        out.ms_rounds = bs.ms_rounds;
+        out.ms_schedule = bs.ms_schedule;
+        out.ms_film = bs.ms_film;
+    }
+    if ((e = hipEventRecord(ev[1], stream)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+    float t = 0;
+    hipEventElapsedTime(&t, ev[0], ev[1]);
+    out.ms_total = t;
+    if (stats) *stats = out;
+    return hipSuccess;
 }
 
 }  // namespace pt

@@ -45,6 +45,8 @@ struct SortScratch {
 hipError_t onesweep_sort_30(uint32_t *keys0, uint32_t *vals0, uint32_t *keys1, uint32_t *vals1,
                             uint32_t n, const uint32_t *hist, uint32_t *err, SortScratch &s,
                             hipStream_t stream, uint32_t **keys_result, uint32_t **vals_result);
+// Sizes the sort's scratch for n keys (so that later calls allocate nothing).
+hipError_t onesweep_reserve(SortScratch &s, uint32_t n);
 // The same over the low 10 * passes bits (passes <= 3); hist holds all three digit histograms.
 hipError_t onesweep_sort(uint32_t *keys0, uint32_t *vals0, uint32_t *keys1, uint32_t *vals1, uint32_t n,
                          const uint32_t *hist, uint32_t *err, SortScratch &s, hipStream_t stream,

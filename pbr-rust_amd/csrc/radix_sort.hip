@@ -170,6 +170,15 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_pass_kernel(
 
 }  // namespace
 
+hipError_t onesweep_reserve(SortScratch &s, uint32_t n) {
+    const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+    const uint64_t status_words = uint64_t(tiles) * kSortRadix * kSortPasses;
+    hipError_t e = s.status.reserve(status_words * sizeof(uint32_t));
+    if (e == hipSuccess) e = s.offsets.reserve(kSortPasses * kSortRadix * sizeof(uint32_t));
+    if (e == hipSuccess) e = s.counters.reserve(64 * sizeof(uint32_t));
+    return e;
+}
+
 hipError_t onesweep_sort_30(uint32_t *keys0, uint32_t *vals0, uint32_t *keys1, uint32_t *vals1,
                             uint32_t n, const uint32_t *hist, uint32_t *err, SortScratch &s,
                             hipStream_t stream, uint32_t **keys_result, uint32_t **vals_result) {
@@ -185,9 +194,7 @@ hipError_t onesweep_sort(uint32_t *keys0, uint32_t *vals0, uint32_t *keys1, uint
     if (n == 0) return hipSuccess;
     const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
     const uint64_t status_words = uint64_t(tiles) * kSortRadix * kSortPasses;
-    hipError_t e = s.status.reserve(status_words * sizeof(uint32_t));
-    if (e == hipSuccess) e = s.offsets.reserve(kSortPasses * kSortRadix * sizeof(uint32_t));
-    if (e == hipSuccess) e = s.counters.reserve(64 * sizeof(uint32_t));
+    hipError_t e = onesweep_reserve(s, n);
     if (e != hipSuccess) return e;
     uint32_t *status = s.status.as<uint32_t>();
     uint32_t *offsets = s.offsets.as<uint32_t>();

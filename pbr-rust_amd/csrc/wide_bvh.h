@@ -75,9 +75,17 @@ struct RaySortScratch {
     DevBuf keys[2], vals[2], hist;
     SortScratch sort;
 };
+// ray_job (nullable): rays of a batch, ray i of job ray_job[i >> job_shift]; the
+// job then leads the key (three radix passes). Every step is a kernel (no memset),
+// so that a graph may capture the call once reserve_ray_sort has sized the scratch.
 hipError_t sort_rays(const WideTrace &B, const float *rays, uint32_t n, uint32_t *prim_out, float *t_out,
                      float *bary_out, uint32_t *err, RaySortScratch &s, uint32_t *queue_len, uint32_t **queue,
-                     hipStream_t stream);
+                     hipStream_t stream, const uint32_t *ray_job = nullptr, uint32_t job_shift = 0);
+hipError_t reserve_ray_sort(RaySortScratch &s, uint32_t n);
+hipError_t trace_wide_multi_queued(const WideTrace &B, const WideTrace *jobs, const uint32_t *ray_job,
+                                   uint32_t job_shift, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
+                                   float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
+                                   const uint32_t *queue_len, hipStream_t stream);
 hipError_t trace_wide_queued(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
                              float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
                              const uint32_t *queue_len, hipStream_t stream);

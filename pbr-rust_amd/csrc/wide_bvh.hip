@@ -537,8 +537,26 @@ __global__ __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(5, 8
     float *__restrict__ t_out, float *__restrict__ bary_out, uint32_t *__restrict__ err,
     const WideTrace *__restrict__ jw, const uint32_t *__restrict__ ray_job, uint32_t job_shift) {
     __shared__ uint2 s_stack[kLdsStack][kWideBlock];
-    trace_wide_body<false, true>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err, nullptr, jw,
-                                 ray_job, job_shift, s_stack);
+    const uint32_t qi = blockIdx.x * kWideBlock + threadIdx.x;
+    if (qi >= *queue_len) return;
+    // rays sorted by job make most waves single-job: then the tree is wave-uniform
+    // (scalar loads, the single-tree code path)
+    const uint32_t j = ray_job[queue[qi] >> job_shift];
+    const uint32_t j0 = __builtin_amdgcn_readfirstlane(j);
+    if (__all(j == j0)) {
+        WideTrace U = B;
+        U.nodes = jw[j0].nodes;
+        U.order = jw[j0].order;
+        U.tri_leaf = jw[j0].tri_leaf;
+        U.prim_order = jw[j0].prim_order;
+        U.num_nodes = jw[j0].num_nodes;
+        U.num_prims = jw[j0].num_prims;
+        trace_wide_body<false, false>(U, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err, nullptr,
+                                      nullptr, nullptr, 0, s_stack);
+    } else {
+        trace_wide_body<false, true>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err, nullptr, jw,
+                                     ray_job, job_shift, s_stack);
+    }
 }
 
 }  // namespace
@@ -612,11 +630,15 @@ __device__ __forceinline__ uint32_t spread3_6(uint32_t v) {  // 6 bits -> every 
     return v;
 }
 
+// MULTI: rays of a batch; the ray's job (low 10 bits) leads the key so that a
+// wave's rays share a tree, and the keys take three radix passes.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void ray_keys_kernel(const float4 *__restrict__ rays, uint32_t n, WideTrace B,
                                                        uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                        uint32_t *__restrict__ hist, uint32_t *__restrict__ count,
                                                        uint32_t *__restrict__ prim_out, float *__restrict__ t_out,
-                                                       float *__restrict__ bary_out) {
+                                                       float *__restrict__ bary_out,
+                                                       const uint32_t *__restrict__ ray_job, uint32_t job_shift) {
     __shared__ uint32_t s_hist[kSortPasses * kSortRadix];
     for (int i = threadIdx.x; i < int(kSortPasses * kSortRadix); i += blockDim.x) s_hist[i] = 0u;
     __syncthreads();
@@ -626,7 +648,8 @@ __global__ __launch_bounds__(256) void ray_keys_kernel(const float4 *__restrict_
     uint32_t mine = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float4 r0 = rays[2 * size_t(i)];
-        uint32_t key = 0xFFFFFu;  // inactive: after every active ray (active keys are < 2^19)
+        // inactive: after every active ray (active keys are < 2^19, or < 2^30 with jobs)
+        uint32_t key = MULTI ? 0x3FFFFFFFu : 0xFFFFFu;
         if (r0.w > 0.f) {
             const float4 r1 = rays[2 * size_t(i) + 1];
             const uint32_t qx = uint32_t(fminf(63.f, fmaxf(0.f, (r0.x - B.lo[0]) * sx)));
@@ -635,6 +658,7 @@ __global__ __launch_bounds__(256) void ray_keys_kernel(const float4 *__restrict_
             const uint32_t oct = (r1.x < 0.f ? 1u : 0u) | (r1.y < 0.f ? 2u : 0u) | (r1.z < 0.f ? 4u : 0u);
             const uint32_t m = (spread3_6(qx) << 2) | (spread3_6(qy) << 1) | spread3_6(qz);  // 18 bits
             key = (oct << 16) | (m >> 2);  // < 2^19: below the inactive key, two sort passes
+            if (MULTI) key |= (ray_job[i >> job_shift] & 1023u) << 19;
             ++mine;
         } else {
             prim_out[i] = kNoHit;
@@ -649,6 +673,7 @@ __global__ __launch_bounds__(256) void ray_keys_kernel(const float4 *__restrict_
         vals[i] = i;
         atomicAdd(&s_hist[key & 1023u], 1u);
         atomicAdd(&s_hist[kSortRadix + ((key >> 10) & 1023u)], 1u);
+        if (MULTI) atomicAdd(&s_hist[2 * kSortRadix + ((key >> 20) & 1023u)], 1u);
     }
     const uint32_t tot = wave_sum(mine);
     if (__lane_id() == 0 && tot) atomicAdd(count, tot);
@@ -657,31 +682,59 @@ __global__ __launch_bounds__(256) void ray_keys_kernel(const float4 *__restrict_
         if (s_hist[i]) atomicAdd(&hist[i], s_hist[i]);
 }
 
+__global__ void zero_hist_kernel(uint32_t *h) {
+    for (uint32_t i = threadIdx.x; i < kSortPasses * kSortRadix; i += blockDim.x) h[i] = 0u;
+}
+
 }  // namespace
 
-hipError_t sort_rays(const WideTrace &B, const float *rays, uint32_t n, uint32_t *prim_out, float *t_out,
-                     float *bary_out, uint32_t *err, RaySortScratch &s, uint32_t *queue_len, uint32_t **queue,
-                     hipStream_t stream) {
+hipError_t reserve_ray_sort(RaySortScratch &s, uint32_t n) {
     hipError_t e = hipSuccess;
     for (int k = 0; k < 2 && e == hipSuccess; ++k) {
         e = s.keys[k].reserve(size_t(n) * 4);
         if (e == hipSuccess) e = s.vals[k].reserve(size_t(n) * 4);
     }
     if (e == hipSuccess) e = s.hist.reserve(kSortPasses * kSortRadix * 4);
+    if (e == hipSuccess) e = onesweep_reserve(s.sort, n);
+    return e;
+}
+
+hipError_t sort_rays(const WideTrace &B, const float *rays, uint32_t n, uint32_t *prim_out, float *t_out,
+                     float *bary_out, uint32_t *err, RaySortScratch &s, uint32_t *queue_len, uint32_t **queue,
+                     hipStream_t stream, const uint32_t *ray_job, uint32_t job_shift) {
+    hipError_t e = reserve_ray_sort(s, n);
     if (e != hipSuccess) return e;
     *queue = s.vals[0].as<uint32_t>();
     if (n == 0) return hipSuccess;
-    e = hipMemsetAsync(s.hist.ptr, 0, kSortPasses * kSortRadix * 4, stream);
-    if (e != hipSuccess) return e;
+    zero_hist_kernel<<<1, 256, 0, stream>>>(s.hist.as<uint32_t>());  // a kernel, so that a graph may capture it
     const uint32_t grid = std::min((n + 255) / 256, 1024u);
-    ray_keys_kernel<<<grid, 256, 0, stream>>>(reinterpret_cast<const float4 *>(rays), n, B, s.keys[0].as<uint32_t>(),
-                                              s.vals[0].as<uint32_t>(), s.hist.as<uint32_t>(), queue_len, prim_out,
-                                              t_out, bary_out);
+    const float4 *r = reinterpret_cast<const float4 *>(rays);
+    if (ray_job)
+        ray_keys_kernel<true><<<grid, 256, 0, stream>>>(r, n, B, s.keys[0].as<uint32_t>(), s.vals[0].as<uint32_t>(),
+                                                        s.hist.as<uint32_t>(), queue_len, prim_out, t_out, bary_out,
+                                                        ray_job, job_shift);
+    else
+        ray_keys_kernel<false><<<grid, 256, 0, stream>>>(r, n, B, s.keys[0].as<uint32_t>(), s.vals[0].as<uint32_t>(),
+                                                         s.hist.as<uint32_t>(), queue_len, prim_out, t_out,
+                                                         bary_out, nullptr, 0);
     uint32_t *kr = nullptr, *vr = nullptr;
     e = onesweep_sort(s.keys[0].as<uint32_t>(), s.vals[0].as<uint32_t>(), s.keys[1].as<uint32_t>(),
-                      s.vals[1].as<uint32_t>(), n, s.hist.as<uint32_t>(), err, s.sort, stream, &kr, &vr, 2);
+                      s.vals[1].as<uint32_t>(), n, s.hist.as<uint32_t>(), err, s.sort, stream, &kr, &vr,
+                      ray_job ? 3u : 2u);
     *queue = vr;
     return e;
+}
+
+hipError_t trace_wide_multi_queued(const WideTrace &B, const WideTrace *jobs, const uint32_t *ray_job,
+                                   uint32_t job_shift, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
+                                   float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
+                                   const uint32_t *queue_len, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (n + kWideBlock - 1) / kWideBlock;
+    trace_wide_multi_kernel<<<grid, kWideBlock, 0, stream>>>(B, reinterpret_cast<const float4 *>(rays), queue,
+                                                             queue_len, any_hit, prim_out, t_out, bary_out, err,
+                                                             jobs, ray_job, job_shift);
+    return hipGetLastError();
 }
 
 hipError_t trace_wide_queued(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,

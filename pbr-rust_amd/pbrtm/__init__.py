@@ -98,13 +98,13 @@ class Job(ctypes.Structure):
 
 class BatchParams(ctypes.Structure):
     _fields_ = [("pool_size", c_uint32), ("rounds_per_launch", c_uint32), ("flags", c_uint32),
-                ("reserved", c_uint32)]
+                ("solo_paths", c_uint32)]
 
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("ms_total", c_double), ("ms_rounds", c_double), ("ms_schedule", c_double), ("ms_film", c_double),
                 ("rounds", c_uint64), ("rays", c_uint64), ("shadow_rays", c_uint64), ("paths", c_uint64),
-                ("pool_size", c_uint32), ("launches", c_uint32)]
+                ("pool_size", c_uint32), ("launches", c_uint32), ("solo_jobs", c_uint32), ("reserved", c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -258,7 +258,7 @@ class Context:
     def scene(self, sc) -> "Scene":
         return Scene(self, sc)
 
-    def render_batch(self, jobs, pool_size=0, rounds_per_launch=0, graph=True, aux=False):
+    def render_batch(self, jobs, pool_size=0, rounds_per_launch=0, graph=True, aux=False, solo_paths=0):
         """jobs: [(Scene, spp, max_depth, pool_size)]. Returns ([film_rgb], [film_aux or None],
         [one-by-one round count per job], stats dict)."""
         arr = (Job * len(jobs))()
@@ -272,7 +272,7 @@ class Context:
             arr[k].params = RenderParams(spp, depth, vpool, 0)
             arr[k].film_rgb = _ptr(rgb)
             arr[k].film_aux = _ptr(ax)
-        bp = BatchParams(pool_size, rounds_per_launch, 0 if graph else BATCH_NO_GRAPH, 0)
+        bp = BatchParams(pool_size, rounds_per_launch, 0 if graph else BATCH_NO_GRAPH, solo_paths)
         st = BatchStats()
         self.check(self.lib.pbrtm_render_batch(self.h, arr, len(jobs), ctypes.byref(bp), ctypes.byref(st)))
         return rgbs, auxs, [arr[k].rounds for k in range(len(jobs))], st.as_dict()

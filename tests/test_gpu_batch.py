@@ -51,13 +51,22 @@ def rendered(ctx):
         h.free()
 
 
-@pytest.mark.parametrize("pool,graph,rpl", [(0, True, 0), (777, True, 3), (4096, False, 0), (1 << 16, True, 1)])
-def test_batch_equals_one_by_one(ctx, rendered, pool, graph, rpl):
+@pytest.mark.parametrize("pool,graph,rpl,solo", [(0, True, 0, 0), (777, True, 3, 0xFFFFFFFF), (4096, False, 0, 20000),
+                                                   (1 << 16, True, 1, 1)])
+def test_batch_equals_one_by_one(ctx, rendered, pool, graph, rpl, solo):
+    """solo: jobs with at least that many paths render alone before the shared pool
+    (0 = the 2^21 default, none here; 20000 = some; 1 = all)."""
     jobs, handles, ref = rendered
     spec = [(handles[id(sc)], spp, depth, vpool) for sc, spp, depth, vpool in jobs]
-    log(f"batch of {len(spec)} jobs, pool {pool}, graph {graph}, rounds per launch {rpl}")
-    rgbs, auxs, rounds, st = ctx.render_batch(spec, pool_size=pool, rounds_per_launch=rpl, graph=graph, aux=True)
-    log(f"  {st['ms_total']:.2f} ms, physical rounds {st['rounds']}, launches {st['launches']}, pool {st['pool_size']}")
+    log(f"batch of {len(spec)} jobs, pool {pool}, graph {graph}, rounds per launch {rpl}, solo {solo}")
+    rgbs, auxs, rounds, st = ctx.render_batch(spec, pool_size=pool, rounds_per_launch=rpl, graph=graph, aux=True,
+                                              solo_paths=solo)
+    log(f"  {st['ms_total']:.2f} ms, physical rounds {st['rounds']}, launches {st['launches']}, pool {st['pool_size']}, "
+        f"solo jobs {st['solo_jobs']}")
+    if solo == 1:
+        assert st["solo_jobs"] == len(spec)
+    elif solo == 20000:
+        assert 0 < st["solo_jobs"] < len(spec)
     for k, ((rgb0, aux0, st0), rgb, aux, r) in enumerate(zip(ref, rgbs, auxs, rounds)):
         assert r == st0["rounds"], (k, r, st0["rounds"])
         assert np.array_equal(aux.view(np.uint32), aux0.view(np.uint32)), k
