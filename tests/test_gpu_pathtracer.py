@@ -189,3 +189,24 @@ def test_cli_renders_fixture_png(tmp_path):
     img = np.asarray(Image.open(out).convert("RGBA"))
     orgb, _, _ = O.render(scenes.cornell34(256, 256), threads=16)
     assert np.array_equal(img, O.srgb8(orgb))
+
+
+def test_edge_shapes(ctx):
+    """One row, one column and one pixel at the largest spp; a scene without emitters."""
+    base = scenes.cornell34(8, 8)
+    for w, h, spp, depth in ((37, 1, 4, 5), (1, 23, 9, 3), (1, 1, 1024, 6)):
+        sc = dict(base, width=w, height=h)
+        rgb, aux, orgb, oaux = render_both(ctx, sc, spp, depth)
+        compare(aux, oaux, f"aux {w}x{h}x{spp}")
+        compare(rgb, orgb, f"film {w}x{h}x{spp}")
+    dark = scenes.cornell34(16, 16)
+    dark.update(lights=[], tri_light=np.full(len(dark["tri"]), -1, np.int32))
+    rgb, aux, orgb, oaux = render_both(ctx, dark, 4, 5)
+    assert not np.any(rgb) and not np.any(orgb)
+
+
+def test_pool_larger_than_image_and_tiny_pool(ctx):
+    sc = scenes.cornell34(20, 12)
+    for pool in (1, 7, 1 << 22):
+        rgb, aux, orgb, oaux = render_both(ctx, sc, 4, 4, pool=pool)
+        compare(rgb, orgb, f"film pool {pool}")
