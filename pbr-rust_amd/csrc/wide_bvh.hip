@@ -250,7 +250,7 @@ __global__ __launch_bounds__(kCollapseBlock) __attribute__((amdgpu_waves_per_eu(
     for (uint32_t oct = 0; oct < 8; ++oct) {
         uint32_t st = 8u;  // root = opened node 0
         int sp = 1;
-        uint32_t pm = 0xFFFFFFFFu;
+        uint32_t pm = 0xCCCCCCCCu;  // unused positions: 12 (a byte-permute selector that yields 0)
         int p = 0;
         while (sp > 0) {
             const uint32_t c = st & 0xFu;
@@ -460,13 +460,17 @@ __device__ __forceinline__ bool wide_step(WideRay &R, const WideTree &T, const W
         const float e = box_entry_nf(np[0], np[1], np[2], fp[0], fp[1], fp[2], R.o, R.inv);
         hs |= (e < R.best_t) ? (1u << s) : 0u;
     }
-    // ... then in depth-first position order
-    uint32_t hm = 0;
-#pragma unroll
-    for (int p = 0; p < kWide; ++p) {
-        const uint32_t s = (perm >> (4 * p)) & 0xFu;
-        hm |= (s < kWide && ((hs >> s) & 1u)) ? (1u << p) : 0u;
-    }
+    // ... then in depth-first position order: hm bit p = hs bit (slot at position p).
+    // The hit bits become bytes (slot s -> byte s of {d1, d0}), a byte permute
+    // gathers them by the position's slot code (an unused position's code 12
+    // yields 0), and a multiply packs each 4 bytes back into 4 bits.
+    const uint32_t d0 = ((hs & 0xFu) * 0x00204081u) & 0x01010101u;
+    const uint32_t d1 = ((hs >> 4) * 0x00204081u) & 0x01010101u;
+    const uint32_t plo = perm & 0xFFFFu, phi = perm >> 16;
+    const uint32_t sel0 = (plo & 0xFu) | ((plo & 0xF0u) << 4) | ((plo & 0xF00u) << 8) | ((plo & 0xF000u) << 12);
+    const uint32_t sel1 = (phi & 0xFu) | ((phi & 0xF0u) << 4) | ((phi & 0xF00u) << 8) | ((phi & 0xF000u) << 12);
+    const uint32_t g0 = __builtin_amdgcn_perm(d1, d0, sel0), g1 = __builtin_amdgcn_perm(d1, d0, sel1);
+    const uint32_t hm = (((g0 * 0x01020408u) >> 24) & 0xFu) | ((((g1 * 0x01020408u) >> 24) & 0xFu) << 4);
     uint32_t pmask = R.pmask & hm;
     const float t_at_visit = R.best_t;
     while (pmask) {
