@@ -33,7 +33,7 @@ TRI_BYTES = 48       # leaf-ordered triangle, 9 floats in a 48-byte slot
 # HBM bytes per trace_wide_kernel launch of this workload (mean of the primary and
 # bounce launches): rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors
 # read off a 256 MiB elementwise copy (--calibrate-copy).
-TRAFFIC_BYTES_PER_LAUNCH = 1_549_529_088
+TRAFFIC_BYTES_PER_LAUNCH = 1_986_019_152
 RAY_BYTES = 32
 BUILD_BYTES_PER_TRI = 216  # algorithmic bytes of an HLBVH build per triangle
 
