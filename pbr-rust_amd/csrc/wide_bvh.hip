@@ -544,40 +544,18 @@ __device__ __forceinline__ void add_counts(unsigned long long *counts, uint32_t 
     }
 }
 
-// One ray per lane. MULTI: every ray selects its own tree, jw[ray_job[i >> job_shift]]
-// (jobs of a batched render); the spill area stays the launch's.
-template <bool COUNT, bool MULTI>
-__device__ __forceinline__ void trace_wide_body(const WideTrace &B, const float4 *__restrict__ rays,
-                                                const uint32_t *__restrict__ queue,
-                                                const uint32_t *__restrict__ queue_len, int any_hit,
-                                                uint32_t *__restrict__ prim_out, float *__restrict__ t_out,
-                                                float *__restrict__ bary_out, uint32_t *__restrict__ err,
-                                                unsigned long long *__restrict__ counts,
-                                                const WideTrace *__restrict__ jw,
-                                                const uint32_t *__restrict__ ray_job, uint32_t job_shift,
-                                                uint2 (*s_stack)[kWideBlock]) {
-    const uint32_t qi = blockIdx.x * kWideBlock + threadIdx.x;
-    if (qi >= *queue_len) return;
-    const uint32_t i = queue[qi];
-    const WideTree T = tree_of(MULTI ? jw[ray_job[i >> job_shift]] : B);
-    uint32_t visits = 0, boxes = 0, tests = 0;
-    WideRay R;
-    if (wide_ray_start(R, rays, i))
-        while (wide_step<COUNT>(R, T, B, any_hit, qi, err, s_stack, visits, boxes, tests)) {
-        }
-    add_counts<COUNT>(counts, visits, boxes, tests);
-    wide_ray_finish(R, i, prim_out, t_out, bary_out);
-}
-
-// Persistent single-tree traversal: a grid that fills the GPU once; every wave
-// walks its own sequence of queue slots (row r of the queue, wave w: slots
-// r * G * 64 + w * 64 + lane, G = waves in the grid) and a lane whose ray has
-// finished takes the wave's next slot as soon as kRefill lanes are idle, so that
-// the waves keep most lanes busy instead of waiting for their slowest ray. Each
-// ray is traced exactly as by trace_wide_body.
+// Persistent traversal: a grid that fills the GPU once; every wave walks its own
+// sequence of queue slots (row r of the queue, wave w: slots r * G * 64 + w * 64 +
+// lane, G = waves in the grid) and a lane whose ray has finished takes the wave's
+// next slot as soon as kRefill lanes are idle, so that the waves keep most lanes
+// busy instead of waiting for their slowest ray. Each ray is traced on its own
+// (wide_ray_start, wide_step until it reports the ray finished, wide_ray_finish),
+// so which lane or wave takes it changes nothing. MULTI: every ray selects its own
+// tree, jw[ray_job[i >> job_shift]] (jobs of a batched render); the spill area
+// stays the launch's.
 constexpr uint32_t kRefill = 8;
 
-template <bool COUNT>
+template <bool COUNT, bool MULTI>
 __device__ __forceinline__ void trace_wide_persistent_body(const WideTrace &B, const float4 *__restrict__ rays,
                                                            const uint32_t *__restrict__ queue,
                                                            const uint32_t *__restrict__ queue_len, int any_hit,
@@ -585,12 +563,14 @@ __device__ __forceinline__ void trace_wide_persistent_body(const WideTrace &B, c
                                                            float *__restrict__ t_out, float *__restrict__ bary_out,
                                                            uint32_t *__restrict__ err,
                                                            unsigned long long *__restrict__ counts,
+                                                           const WideTrace *__restrict__ jw,
+                                                           const uint32_t *__restrict__ ray_job, uint32_t job_shift,
                                                            uint2 (*s_stack)[kWideBlock]) {
     const uint32_t n = *queue_len;
     const uint32_t lane = __lane_id();
     const uint64_t stride = uint64_t(gridDim.x) * kWideBlock;  // queue slots per row
     const uint64_t base = uint64_t(blockIdx.x) * kWideBlock + (threadIdx.x & ~uint32_t(kWave - 1));
-    const WideTree T = tree_of(B);
+    WideTree T = tree_of(B);  // MULTI: the tree of the lane's current ray
     uint32_t visits = 0, boxes = 0, tests = 0;
     WideRay R;
     bool live = false;
@@ -606,6 +586,7 @@ __device__ __forceinline__ void trace_wide_persistent_body(const WideTrace &B, c
                 if (slot < n) {
                     qi = uint32_t(slot);
                     i = queue[qi];
+                    if (MULTI) T = tree_of(jw[ray_job[i >> job_shift]]);
                     live = wide_ray_start(R, rays, i);
                     if (!live) wide_ray_finish(R, i, prim_out, t_out, bary_out);
                 }
@@ -634,8 +615,8 @@ __global__ __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(5, 8
     float *__restrict__ t_out, float *__restrict__ bary_out, uint32_t *__restrict__ err,
     unsigned long long *__restrict__ counts) {
     __shared__ uint2 s_stack[kLdsStack][kWideBlock];
-    trace_wide_persistent_body<COUNT>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err, counts,
-                                      s_stack);
+    trace_wide_persistent_body<COUNT, false>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err,
+                                             counts, nullptr, nullptr, 0, s_stack);
 }
 
 __global__ __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void trace_wide_multi_kernel(
@@ -644,26 +625,8 @@ __global__ __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(5, 8
     float *__restrict__ t_out, float *__restrict__ bary_out, uint32_t *__restrict__ err,
     const WideTrace *__restrict__ jw, const uint32_t *__restrict__ ray_job, uint32_t job_shift) {
     __shared__ uint2 s_stack[kLdsStack][kWideBlock];
-    const uint32_t qi = blockIdx.x * kWideBlock + threadIdx.x;
-    if (qi >= *queue_len) return;
-    // rays sorted by job make most waves single-job: then the tree is wave-uniform
-    // (scalar loads, the single-tree code path)
-    const uint32_t j = ray_job[queue[qi] >> job_shift];
-    const uint32_t j0 = __builtin_amdgcn_readfirstlane(j);
-    if (__all(j == j0)) {
-        WideTrace U = B;
-        U.nodes = jw[j0].nodes;
-        U.order = jw[j0].order;
-        U.tri_leaf = jw[j0].tri_leaf;
-        U.prim_order = jw[j0].prim_order;
-        U.num_nodes = jw[j0].num_nodes;
-        U.num_prims = jw[j0].num_prims;
-        trace_wide_body<false, false>(U, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err, nullptr,
-                                      nullptr, nullptr, 0, s_stack);
-    } else {
-        trace_wide_body<false, true>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err, nullptr, jw,
-                                     ray_job, job_shift, s_stack);
-    }
+    trace_wide_persistent_body<false, true>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err,
+                                            nullptr, jw, ray_job, job_shift, s_stack);
 }
 
 }  // namespace
@@ -709,33 +672,45 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
     return hipGetLastError();
 }
 
-// Workgroups of the persistent traversal: as many as are resident on the device
-// at once (both instances have the same footprint), at most one per 256 rays.
-static uint32_t persistent_grid(uint32_t n) {
-    static thread_local int cached_dev = -1;
-    static thread_local uint32_t cached = 0;
+// Workgroups of a persistent traversal kernel: as many as are resident on the
+// device at once, at most one per 256 rays. Computed once per (kernel, device).
+template <typename Kernel>
+static uint32_t persistent_grid(Kernel kernel, uint32_t n) {
+    struct Entry {
+        const void *kernel;
+        int device;
+        uint32_t blocks;
+    };
+    static thread_local Entry cache[8] = {};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-    if (dev != cached_dev) {
+    const void *key = reinterpret_cast<const void *>(kernel);
+    uint32_t blocks = 0;
+    for (const Entry &e : cache)
+        if (e.kernel == key && e.device == dev) blocks = e.blocks;
+    if (blocks == 0) {
         int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_wide_kernel<false>, kWideBlock, 0) !=
-                hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWideBlock, 0) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu <= 0 ||
             cus <= 0) {
             per_cu = 1;
             cus = 256;
         }
-        cached = uint32_t(per_cu) * uint32_t(cus);
-        cached_dev = dev;
+        blocks = uint32_t(per_cu) * uint32_t(cus);
+        for (Entry &e : cache)
+            if (!e.kernel) {
+                e = Entry{key, dev, blocks};
+                break;
+            }
     }
-    return std::max(1u, std::min(cached, (n + kWideBlock - 1) / kWideBlock));
+    return std::max(1u, std::min(blocks, (n + kWideBlock - 1) / kWideBlock));
 }
 
 hipError_t trace_wide(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
                       float *t_out, float *bary_out, uint32_t *err, unsigned long long *counts, uint32_t *queue,
                       uint32_t *queue_len, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = persistent_grid(n);
+    const uint32_t grid = persistent_grid(trace_wide_kernel<false>, n);
     const uint32_t cgrid = (n + kCompactBlock * kCompactItems - 1) / (kCompactBlock * kCompactItems);
     const float4 *r = reinterpret_cast<const float4 *>(rays);
     compact_rays_kernel<<<cgrid, kCompactBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
@@ -859,7 +834,7 @@ hipError_t trace_wide_multi_queued(const WideTrace &B, const WideTrace *jobs, co
                                    float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
                                    const uint32_t *queue_len, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = (n + kWideBlock - 1) / kWideBlock;
+    const uint32_t grid = persistent_grid(trace_wide_multi_kernel, n);
     trace_wide_multi_kernel<<<grid, kWideBlock, 0, stream>>>(B, reinterpret_cast<const float4 *>(rays), queue,
                                                              queue_len, any_hit, prim_out, t_out, bary_out, err,
                                                              jobs, ray_job, job_shift);
@@ -870,7 +845,7 @@ hipError_t trace_wide_queued(const WideTrace &B, const float *rays, uint32_t n, 
                              float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
                              const uint32_t *queue_len, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = persistent_grid(n);
+    const uint32_t grid = persistent_grid(trace_wide_kernel<false>, n);
     trace_wide_kernel<false><<<grid, kWideBlock, 0, stream>>>(B, reinterpret_cast<const float4 *>(rays), queue,
                                                               queue_len, any_hit, prim_out, t_out, bary_out, err,
                                                               nullptr);
@@ -882,7 +857,7 @@ hipError_t trace_wide_multi(const WideTrace &B, const WideTrace *jobs, const uin
                             float *bary_out, uint32_t *err, uint32_t *queue, uint32_t *queue_len,
                             hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = (n + kWideBlock - 1) / kWideBlock;
+    const uint32_t grid = persistent_grid(trace_wide_multi_kernel, n);
     const uint32_t cgrid = (n + kCompactBlock * kCompactItems - 1) / (kCompactBlock * kCompactItems);
     const float4 *r = reinterpret_cast<const float4 *>(rays);
     compact_rays_kernel<<<cgrid, kCompactBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
