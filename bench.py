@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--pt-spp", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-render measurement")
     ap.add_argument("--batch-jobs", type=int, default=64)
-    ap.add_argument("--batch-pools", type=str, default="2097152,8388608",
+    ap.add_argument("--batch-pools", type=str, default="8388608,16777216,33554432",
                     help="shared pool sizes the batch is tuned over")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiable-render measurement")
     ap.add_argument("--grad-steps", type=int, default=2)

@@ -261,11 +261,12 @@ typedef struct pbrtm_job {
 #define PBRTM_BATCH_NO_GRAPH 2u    /* enqueue each round's kernels instead of replaying a captured graph */
 
 typedef struct pbrtm_batch_params {
-    uint32_t pool_size;           /* slots shared by the batch; 0 = 2^22 (capped at the batch's path count) */
+    uint32_t pool_size;           /* slots shared by the batch; 0 = 2^24 (capped at the batch's path count) */
     uint32_t rounds_per_launch;   /* rounds per graph launch between done checks; 0 = 8 */
     uint32_t flags;               /* PBRTM_DEVICE_PTRS (films are device pointers), PBRTM_BATCH_NO_GRAPH */
     uint32_t solo_paths;          /* jobs with at least this many paths render alone, before the shared
-                                     pool (they fill the GPU by themselves); 0 = 2^21, 0xFFFFFFFF = never */
+                                     pool (they fill the GPU by themselves); 0 or 0xFFFFFFFF = never (the
+                                     default: one shared pool measured fastest on mixed jobs) */
 } pbrtm_batch_params;
 
 typedef struct pbrtm_batch_stats {

@@ -909,13 +909,13 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
             if (jb.film_aux) f += wh * 4;
         }
     }
-    const uint32_t pool = params->pool_size ? params->pool_size : (4u << 20);
+    const uint32_t pool = params->pool_size ? params->pool_size : (16u << 20);
     const uint32_t rpl = params->rounds_per_launch ? params->rounds_per_launch : 8u;
     CK(ctx, ctx->dev_err.reserve(64), "alloc");
     CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream), "memset");
     pbrtm::pt::BatchStats bs;
-    const uint64_t solo = params->solo_paths == 0 ? (2u << 20)
-                          : params->solo_paths == 0xFFFFFFFFu ? ~uint64_t(0) : params->solo_paths;
+    const uint64_t solo =
+        (params->solo_paths == 0 || params->solo_paths == 0xFFFFFFFFu) ? ~uint64_t(0) : params->solo_paths;
     CK(ctx, pbrtm::pt::render_batch(bj, pool, rpl, !(params->flags & PBRTM_BATCH_NO_GRAPH), solo, ctx->batch_pool,
                                     ctx->batch_jobs, ctx->dev_err.as<uint32_t>(), &bs, ctx->stream),
        "render batch");

@@ -55,7 +55,7 @@ def rendered(ctx):
                                                    (1 << 16, True, 1, 1)])
 def test_batch_equals_one_by_one(ctx, rendered, pool, graph, rpl, solo):
     """solo: jobs with at least that many paths render alone before the shared pool
-    (0 = the 2^21 default, none here; 20000 = some; 1 = all)."""
+    (0 = the default, never; 20000 = some; 1 = all)."""
     jobs, handles, ref = rendered
     spec = [(handles[id(sc)], spp, depth, vpool) for sc, spp, depth, vpool in jobs]
     log(f"batch of {len(spec)} jobs, pool {pool}, graph {graph}, rounds per launch {rpl}, solo {solo}")
@@ -63,7 +63,9 @@ def test_batch_equals_one_by_one(ctx, rendered, pool, graph, rpl, solo):
                                               solo_paths=solo)
     log(f"  {st['ms_total']:.2f} ms, physical rounds {st['rounds']}, launches {st['launches']}, pool {st['pool_size']}, "
         f"solo jobs {st['solo_jobs']}")
-    if solo == 1:
+    if solo in (0, 0xFFFFFFFF):
+        assert st["solo_jobs"] == 0
+    elif solo == 1:
         assert st["solo_jobs"] == len(spec)
     elif solo == 20000:
         assert 0 < st["solo_jobs"] < len(spec)
