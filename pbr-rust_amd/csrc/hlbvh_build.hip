@@ -285,12 +285,19 @@ __global__ __launch_bounds__(256) void task_count_kernel(const WorkItem *items, 
     if (i < nitems) counts[i] = (items[i].end - items[i].begin + kChunk - 1) / kChunk;
 }
 
-__global__ __launch_bounds__(256) void task_fill_kernel(const WorkItem *items, uint32_t nitems,
-                                                        const uint32_t *task_off, uint2 *tasks) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nitems) return;
-    const uint32_t nc = (items[i].end - items[i].begin + kChunk - 1) / kChunk;
-    for (uint32_t c = 0; c < nc; ++c) tasks[task_off[i] + c] = make_uint2(i, c);
+// One thread per task: its item is the last one whose first task is <= t (every
+// item has at least one chunk, so task_off is strictly increasing).
+__global__ __launch_bounds__(256) void task_fill_kernel(uint32_t nitems, const uint32_t *task_off, uint32_t ntasks,
+                                                        uint2 *tasks) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntasks) return;
+    uint32_t lo = 0, hi = nitems;  // task_off[lo] <= t < task_off[hi] (task_off[nitems] = ntasks)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (task_off[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    tasks[t] = make_uint2(lo, t - task_off[lo]);
 }
 
 // Block reduce of 6 order-mapped mins and 6 maxes, merged into global atomics.
@@ -651,7 +658,7 @@ __device__ uint32_t wave_build(const BuildArgs &A, const Rec *src, uint32_t begi
         v[8] = active ? f2ord(b.lo.z) : 0xFFFFFFFFu;
         v[9] = active ? f2ord(b.hi.x) : 0u; v[10] = active ? f2ord(b.hi.y) : 0u; v[11] = active ? f2ord(b.hi.z) : 0u;
 #pragma unroll
-        for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? wave_min_u32(v[k]) : wave_max_u32(v[k]);
+        for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? wave_min_u32_all(v[k]) : wave_max_u32_all(v[k]);
         const Box cb = acc_box(v);
         const Box fb = acc_box(v + 6);
         const int axis = box_max_dim(cb);
@@ -691,7 +698,7 @@ __device__ uint32_t wave_build(const BuildArgs &A, const Rec *src, uint32_t begi
                 const float mid = sel3(box_centroid(cb), axis);
                 left = active && sel3(c, axis) < mid;
             }
-            lp = wave_sum(left ? r.b : 0u);
+            lp = wave_sum_all(left ? r.b : 0u);
         } else {
             const float mid = sel3(box_centroid(cb), axis);
             left = active && sel3(c, axis) < mid;
@@ -925,7 +932,7 @@ static hipError_t run_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, 
         uint32_t *tlp = S.task_lprims.as<uint32_t>();
         Acc *acc = S.acc.as<Acc>();
         Split *splits = S.splits.as<Split>();
-        task_fill_kernel<<<(ncur + 255) / 256, 256, 0, stream>>>(cur, ncur, task_off, tasks);
+        task_fill_kernel<<<(ntasks + 255) / 256, 256, 0, stream>>>(ncur, task_off, ntasks, tasks);
         acc_init_kernel<<<ncur, 256, 0, stream>>>(acc, ncur);
         chunk_reduce_kernel<<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, acc);
         if (TOP) chunk_buckets_kernel<<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, acc, A.err);

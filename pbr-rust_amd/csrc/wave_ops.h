@@ -42,6 +42,29 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
+// Reductions for code where all 64 lanes are active and every lane needs the
+// result: DPP exchanges within rows of 16 lanes (quad permutes, half-row and row
+// mirrors), then the four row results read as scalars. No LDS traffic.
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_reduce_rows(uint32_t v, Op op) {
+    v = op(v, uint32_t(__builtin_amdgcn_update_dpp(int(v), int(v), 0xB1, 0xF, 0xF, false)));   // lanes 1,0,3,2
+    v = op(v, uint32_t(__builtin_amdgcn_update_dpp(int(v), int(v), 0x4E, 0xF, 0xF, false)));   // lanes 2,3,0,1
+    v = op(v, uint32_t(__builtin_amdgcn_update_dpp(int(v), int(v), 0x141, 0xF, 0xF, false)));  // half-row mirror
+    v = op(v, uint32_t(__builtin_amdgcn_update_dpp(int(v), int(v), 0x140, 0xF, 0xF, false)));  // row mirror
+    const uint32_t r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const uint32_t r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return op(op(r0, r1), op(r2, r3));
+}
+__device__ __forceinline__ uint32_t wave_min_u32_all(uint32_t v) {
+    return wave_reduce_rows(v, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
+}
+__device__ __forceinline__ uint32_t wave_max_u32_all(uint32_t v) {
+    return wave_reduce_rows(v, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
+}
+__device__ __forceinline__ uint32_t wave_sum_all(uint32_t v) {
+    return wave_reduce_rows(v, [](uint32_t a, uint32_t b) { return a + b; });
+}
+
 // Exclusive scan across a workgroup of BLOCK threads; returns the thread's
 // exclusive prefix and writes the block total to *total (all threads).
 template <int BLOCK>
