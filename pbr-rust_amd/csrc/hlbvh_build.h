@@ -16,6 +16,8 @@ enum CounterSlot : int {
     kCntTasks = 6,
     kCntTreelets = 7,
     kCntErr = 8,
+    kCntLevelsTop = 9,
+    kCntLevelsBot = 10,
     kNumCounters = 16,
 };
 

@@ -265,9 +265,12 @@ __global__ void root_kernel(BuildArgs A, uint32_t num_treelets, uint32_t n) {
 
 // ------------------------------------------------- level step for large nodes
 
-__global__ __launch_bounds__(256) void acc_init_kernel(Acc *acc, uint32_t nitems) {
+// Level kernels read the level's item count (*nitems) and task count (*ntasks)
+// from device memory, so that several levels are enqueued without reading them
+// back; grids cover host-side upper bounds and the excess exits.
+__global__ __launch_bounds__(256) void acc_init_kernel(Acc *acc, const uint32_t *nitems) {
     const uint32_t i = blockIdx.x;
-    if (i >= nitems) return;
+    if (i >= *nitems) return;
     uint32_t *w = reinterpret_cast<uint32_t *>(&acc[i]);
     for (int k = threadIdx.x; k < kAccWords; k += blockDim.x) {
         uint32_t v = 0u;
@@ -280,18 +283,20 @@ __global__ __launch_bounds__(256) void acc_init_kernel(Acc *acc, uint32_t nitems
     }
 }
 
-__global__ __launch_bounds__(256) void task_count_kernel(const WorkItem *items, uint32_t nitems, uint32_t *counts) {
+__global__ __launch_bounds__(256) void task_count_kernel(const WorkItem *items, const uint32_t *nitems,
+                                                         uint32_t bound, uint32_t *counts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nitems) counts[i] = (items[i].end - items[i].begin + kChunk - 1) / kChunk;
+    if (i < bound) counts[i] = i < *nitems ? (items[i].end - items[i].begin + kChunk - 1) / kChunk : 0u;
 }
 
 // One thread per task: its item is the last one whose first task is <= t (every
 // item has at least one chunk, so task_off is strictly increasing).
-__global__ __launch_bounds__(256) void task_fill_kernel(uint32_t nitems, const uint32_t *task_off, uint32_t ntasks,
+__global__ __launch_bounds__(256) void task_fill_kernel(const uint32_t *nitems, uint32_t bound,
+                                                        const uint32_t *task_off, const uint32_t *ntasks,
                                                         uint2 *tasks) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntasks) return;
-    uint32_t lo = 0, hi = nitems;  // task_off[lo] <= t < task_off[hi] (task_off[nitems] = ntasks)
+    if (t >= *ntasks) return;
+    uint32_t lo = 0, hi = min(*nitems, bound);  // task_off holds bound entries  // task_off[lo] <= t < task_off[hi] (task_off[nitems] = ntasks)
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) / 2;
         if (task_off[mid] <= t) lo = mid;
@@ -322,8 +327,9 @@ __device__ __forceinline__ void block_merge_boxes(uint32_t v[12], uint32_t *dst)
     }
 }
 
-__global__ __launch_bounds__(kChunkBlock) void chunk_reduce_kernel(const uint2 *tasks, const WorkItem *items,
-                                                                   Rec *const *recs, Acc *acc) {
+__global__ __launch_bounds__(kChunkBlock) void chunk_reduce_kernel(const uint2 *tasks, const uint32_t *ntasks,
+                                                                   const WorkItem *items, Rec *const *recs, Acc *acc) {
+    if (blockIdx.x >= *ntasks) return;
     const uint2 task = tasks[blockIdx.x];
     const WorkItem it = items[task.x];
     const Rec *R = recs[it.buf];
@@ -343,10 +349,12 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_reduce_kernel(const uint2 *
 }
 
 // SAH bucket statistics per chunk (hlbvh.cu:690-706).
-__global__ __launch_bounds__(kChunkBlock) void chunk_buckets_kernel(const uint2 *tasks, const WorkItem *items,
-                                                                    Rec *const *recs, Acc *acc, uint32_t *err) {
+__global__ __launch_bounds__(kChunkBlock) void chunk_buckets_kernel(const uint2 *tasks, const uint32_t *ntasks,
+                                                                    const WorkItem *items, Rec *const *recs, Acc *acc,
+                                                                    uint32_t *err) {
     __shared__ uint32_t s_cnt[kNumBuckets];
     __shared__ uint32_t s_box[kNumBuckets][6];
+    if (blockIdx.x >= *ntasks) return;
     const uint2 task = tasks[blockIdx.x];
     const WorkItem it = items[task.x];
     const Rec *R = recs[it.buf];
@@ -400,11 +408,11 @@ __device__ __forceinline__ float shfl_up_f(float v, int d) { return __shfl_up(v,
 __device__ __forceinline__ float shfl_down_f(float v, int d) { return __shfl_down(v, d, kWave); }
 
 template <bool TOP>
-__global__ __launch_bounds__(64) void decide_kernel(const WorkItem *items, uint32_t nitems, const Acc *acc,
+__global__ __launch_bounds__(64) void decide_kernel(const WorkItem *items, const uint32_t *nitems, const Acc *acc,
                                                     Split *splits) {
     __shared__ float s_cost[kNumBuckets];
     const uint32_t item = TOP ? blockIdx.x : blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= nitems) return;
+    if (item >= *nitems) return;
     const Acc &a = acc[item];
     const Box cb = acc_box(a.cb);
     Split sp{};
@@ -478,11 +486,19 @@ __device__ __forceinline__ bool goes_left(const Split &sp, const Box &b) {
 }
 
 template <bool TOP>
-__global__ __launch_bounds__(kChunkBlock) void chunk_count_kernel(const uint2 *tasks, const WorkItem *items,
-                                                                  Rec *const *recs, const Split *splits,
-                                                                  uint32_t *task_left, uint32_t *task_lprims) {
+__global__ __launch_bounds__(kChunkBlock) void chunk_count_kernel(const uint2 *tasks, const uint32_t *ntasks,
+                                                                  const WorkItem *items, Rec *const *recs,
+                                                                  const Split *splits, uint32_t *task_left,
+                                                                  uint32_t *task_lprims) {
     __shared__ uint32_t s_w[kChunkBlock / kWave];
     __shared__ uint32_t s_p[kChunkBlock / kWave];
+    if (blockIdx.x >= *ntasks) {  // past the level's tasks: zeros for the scans over the bound
+        if (threadIdx.x == 0) {
+            task_left[blockIdx.x] = 0u;
+            task_lprims[blockIdx.x] = 0u;
+        }
+        return;
+    }
     const uint2 task = tasks[blockIdx.x];
     const WorkItem it = items[task.x];
     const Split sp = splits[task.x];
@@ -507,12 +523,13 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_count_kernel(const uint2 *t
 
 // Stable partition of one chunk into the other buffer.
 template <bool TOP>
-__global__ __launch_bounds__(kChunkBlock) void chunk_scatter_kernel(const uint2 *tasks, const WorkItem *items,
-                                                                    Rec *const *recs, const Split *splits,
-                                                                    const uint32_t *task_off,
+__global__ __launch_bounds__(kChunkBlock) void chunk_scatter_kernel(const uint2 *tasks, const uint32_t *ntasks,
+                                                                    const WorkItem *items, Rec *const *recs,
+                                                                    const Split *splits, const uint32_t *task_off,
                                                                     const uint32_t *left_off) {
     __shared__ uint32_t s_waves[kChunkBlock / kWave];
     const uint32_t t = blockIdx.x;
+    if (t >= *ntasks) return;
     const uint2 task = tasks[t];
     const WorkItem it = items[task.x];
     const Split sp = splits[task.x];
@@ -551,13 +568,15 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_scatter_kernel(const uint2 
 }
 
 template <bool TOP>
-__global__ __launch_bounds__(64) void children_kernel(BuildArgs A, const WorkItem *items, uint32_t nitems,
+__global__ __launch_bounds__(64) void children_kernel(BuildArgs A, const WorkItem *items, const uint32_t *nitems,
                                                       const Split *splits, const Acc *acc,
                                                       const uint32_t *task_off, const uint32_t *left_off,
                                                       const uint32_t *lprims_off, WorkItem *next_large,
-                                                      uint32_t *next_counter) {
+                                                      uint32_t *next_counter, uint32_t *levels) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nitems) return;
+    const uint32_t n = *nitems;
+    if (i == 0 && n > 0) atomicAdd(levels, 1u);  // a level that had items
+    if (i >= n) return;
     const WorkItem it = items[i];
     const Split sp = splits[i];
     const uint32_t size = it.end - it.begin;
@@ -906,50 +925,72 @@ static hipError_t read_counters(BuildScratch &S, hipStream_t stream) {
     return hipStreamSynchronize(stream);
 }
 
+// Levels enqueued between two reads of the item count. Grids between reads use
+// upper bounds: a level has at most twice the previous level's large items
+// (each splits in two), at most elements / (kSmallNode + 1) of them (they are
+// disjoint and larger than kSmallNode), and at most elements / kChunk + items
+// chunk tasks.
+constexpr int kLevelsPerSync = 4;
+
 template <bool TOP>
 static hipError_t run_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, Rec *const *recs, int cnt0,
-                             int cnt1, WorkItem *list0, WorkItem *list1, uint32_t *levels) {
+                             int cnt1, WorkItem *list0, WorkItem *list1, uint32_t elements, uint32_t list_cap,
+                             uint32_t *levels) {
     WorkItem *cur = list0, *nxt = list1;
     int ccur = cnt0, cnxt = cnt1;
+    uint32_t *const counters = S.counters.as<uint32_t>();
+    uint32_t *const ntasks = counters + kCntTasks;
+    const uint32_t item_cap = std::min(list_cap, elements / (kSmallNode + 1) + 1);
     PBRTM_TRY(read_counters(S, stream));
-    uint32_t ncur = S.host_counters[ccur];
-    while (ncur > 0) {
-        if (++*levels > 100000) return hipErrorUnknown;  // every level strictly shrinks its nodes
-        PBRTM_TRY(S.acc.reserve(size_t(ncur) * sizeof(Acc)));
-        PBRTM_TRY(S.splits.reserve(size_t(ncur) * sizeof(Split)));
-        PBRTM_TRY(S.task_off.reserve(size_t(ncur + 1) * sizeof(uint32_t)));
+    uint32_t known = S.host_counters[ccur];  // exact item count of the next level
+    uint32_t enqueued = 0;
+    while (known > 0) {
+        // bounds for the next kLevelsPerSync levels, and scratch sized for them
+        const uint32_t nb_max = uint32_t(std::min<uint64_t>(uint64_t(known) << (kLevelsPerSync - 1), item_cap));
+        const uint32_t tb_max = elements / kChunk + nb_max + 1;
+        PBRTM_TRY(S.acc.reserve(size_t(nb_max) * sizeof(Acc)));
+        PBRTM_TRY(S.splits.reserve(size_t(nb_max) * sizeof(Split)));
+        PBRTM_TRY(S.task_off.reserve(size_t(nb_max + 1) * sizeof(uint32_t)));
+        PBRTM_TRY(S.tasks.reserve(size_t(tb_max) * sizeof(uint2)));
+        PBRTM_TRY(S.task_left.reserve(size_t(tb_max + 1) * sizeof(uint32_t)));
+        PBRTM_TRY(S.task_lprims.reserve(size_t(tb_max + 1) * sizeof(uint32_t)));
+        PBRTM_TRY(S.scan_scratch.reserve(size_t(tb_max / 4096 + 2) * sizeof(uint32_t)));
         uint32_t *task_off = S.task_off.as<uint32_t>();
-        task_count_kernel<<<(ncur + 255) / 256, 256, 0, stream>>>(cur, ncur, task_off);
-        PBRTM_TRY(exclusive_scan_u32(task_off, task_off, ncur, S.counters.as<uint32_t>() + kCntTasks,
-                                     S.scan_scratch, stream));
-        PBRTM_TRY(read_counters(S, stream));
-        const uint32_t ntasks = S.host_counters[kCntTasks];
-        PBRTM_TRY(S.tasks.reserve(size_t(ntasks) * sizeof(uint2)));
-        PBRTM_TRY(S.task_left.reserve(size_t(ntasks + 1) * sizeof(uint32_t)));
-        PBRTM_TRY(S.task_lprims.reserve(size_t(ntasks + 1) * sizeof(uint32_t)));
         uint2 *tasks = S.tasks.as<uint2>();
         uint32_t *tleft = S.task_left.as<uint32_t>();
         uint32_t *tlp = S.task_lprims.as<uint32_t>();
         Acc *acc = S.acc.as<Acc>();
         Split *splits = S.splits.as<Split>();
-        task_fill_kernel<<<(ntasks + 255) / 256, 256, 0, stream>>>(ncur, task_off, ntasks, tasks);
-        acc_init_kernel<<<ncur, 256, 0, stream>>>(acc, ncur);
-        chunk_reduce_kernel<<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, acc);
-        if (TOP) chunk_buckets_kernel<<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, acc, A.err);
-        decide_kernel<TOP><<<TOP ? ncur : (ncur + 63) / 64, 64, 0, stream>>>(cur, ncur, acc, splits);
-        chunk_count_kernel<TOP><<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, splits, tleft, tlp);
-        PBRTM_TRY(exclusive_scan_u32(tleft, tleft, ntasks, tleft + ntasks, S.scan_scratch, stream));
-        if (TOP) PBRTM_TRY(exclusive_scan_u32(tlp, tlp, ntasks, tlp + ntasks, S.scan_scratch, stream));
-        chunk_scatter_kernel<TOP><<<ntasks, kChunkBlock, 0, stream>>>(tasks, cur, recs, splits, task_off, tleft);
-        zero_counters_kernel<<<1, 64, 0, stream>>>(S.counters.as<uint32_t>() + cnxt, 1);
-        children_kernel<TOP><<<(ncur + 63) / 64, 64, 0, stream>>>(A, cur, ncur, splits, acc, task_off, tleft, tlp,
-                                                                  nxt, S.counters.as<uint32_t>() + cnxt);
-        PBRTM_TRY(hipGetLastError());
+        uint32_t nb = known;
+        for (int k = 0; k < kLevelsPerSync; ++k) {
+            if (++enqueued > 100000) return hipErrorUnknown;  // every level strictly shrinks its nodes
+            const uint32_t tb = elements / kChunk + nb + 1;
+            const uint32_t *ncur = counters + ccur;
+            task_count_kernel<<<(nb + 255) / 256, 256, 0, stream>>>(cur, ncur, nb, task_off);
+            PBRTM_TRY(exclusive_scan_u32(task_off, task_off, nb, ntasks, S.scan_scratch, stream));
+            task_fill_kernel<<<(tb + 255) / 256, 256, 0, stream>>>(ncur, nb, task_off, ntasks, tasks);
+            acc_init_kernel<<<nb, 256, 0, stream>>>(acc, ncur);
+            chunk_reduce_kernel<<<tb, kChunkBlock, 0, stream>>>(tasks, ntasks, cur, recs, acc);
+            if (TOP) chunk_buckets_kernel<<<tb, kChunkBlock, 0, stream>>>(tasks, ntasks, cur, recs, acc, A.err);
+            decide_kernel<TOP><<<TOP ? nb : (nb + 63) / 64, 64, 0, stream>>>(cur, ncur, acc, splits);
+            chunk_count_kernel<TOP><<<tb, kChunkBlock, 0, stream>>>(tasks, ntasks, cur, recs, splits, tleft, tlp);
+            PBRTM_TRY(exclusive_scan_u32(tleft, tleft, tb, tleft + tb, S.scan_scratch, stream));
+            if (TOP) PBRTM_TRY(exclusive_scan_u32(tlp, tlp, tb, tlp + tb, S.scan_scratch, stream));
+            chunk_scatter_kernel<TOP><<<tb, kChunkBlock, 0, stream>>>(tasks, ntasks, cur, recs, splits, task_off,
+                                                                      tleft);
+            zero_counters_kernel<<<1, 64, 0, stream>>>(counters + cnxt, 1);
+            children_kernel<TOP><<<(nb + 63) / 64, 64, 0, stream>>>(A, cur, ncur, splits, acc, task_off, tleft, tlp,
+                                                                    nxt, counters + cnxt,
+                                                                    counters + (TOP ? kCntLevelsTop : kCntLevelsBot));
+            PBRTM_TRY(hipGetLastError());
+            std::swap(cur, nxt);
+            std::swap(ccur, cnxt);
+            nb = std::min(2 * nb, item_cap);
+        }
         PBRTM_TRY(read_counters(S, stream));
-        std::swap(cur, nxt);
-        std::swap(ccur, cnxt);
-        ncur = S.host_counters[ccur];
+        known = S.host_counters[ccur];
     }
+    *levels = S.host_counters[TOP ? kCntLevelsTop : kCntLevelsBot];
     return hipSuccess;
 }
 
@@ -1050,7 +1091,7 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
 
     root_kernel<<<1, 64, 0, stream>>>(A, D, n);
     PBRTM_TRY(run_levels<true>(A, S, stream, trec_ptrs, kCntLargeTop0, kCntLargeTop1, A.large_top[0],
-                               A.large_top[1], &out.top_levels));
+                               A.large_top[1], D, A.cap_large_top, &out.top_levels));
     PBRTM_TRY(read_counters(S, stream));
     out.small_top = S.host_counters[kCntSmallTop];
     if (out.small_top > 0)
@@ -1058,7 +1099,7 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
             A, A.small_top, out.small_top);
     if (times) hipEventRecord(times->ev[3], stream);
     PBRTM_TRY(run_levels<false>(A, S, stream, prec_ptrs, kCntLargeBot0, kCntLargeBot1, A.large_bot[0],
-                                A.large_bot[1], &out.bottom_levels));
+                                A.large_bot[1], n, A.cap_large_bot, &out.bottom_levels));
     PBRTM_TRY(read_counters(S, stream));
     out.small_bottom = S.host_counters[kCntSmallBot];
     if (out.small_bottom > 0)
