@@ -112,14 +112,14 @@ __global__ void wide_root_kernel(const BNode *bn, Job *jobs, uint32_t *counters,
 
 constexpr int kCollapseBlock = 64;
 
-// Frontier state lives in registers: every array below is indexed only with
-// compile-time indices (fully unrolled loops with selects).
-__global__ __launch_bounds__(kCollapseBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void wide_collapse_kernel(
-    const BNode *__restrict__ bn, const Job *__restrict__ jobs, uint32_t njobs, Job *__restrict__ next,
-    uint32_t *__restrict__ counters, WNode *__restrict__ wn, uint32_t *__restrict__ order, uint32_t cap,
-    uint32_t *__restrict__ err) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < njobs;
+// One wide node per lane. Frontier state lives in registers: every array below
+// is indexed only with compile-time indices (fully unrolled loops with selects).
+// All 64 lanes of the wave take part (live or not): the child reservation is one
+// atomic per wave.
+__device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const Job *__restrict__ jobs, uint32_t i,
+                                             bool live, Job *__restrict__ next, uint32_t *__restrict__ counters,
+                                             WNode *__restrict__ wn, uint32_t *__restrict__ order, uint32_t cap,
+                                             uint32_t *__restrict__ err) {
     const Job job = live ? jobs[i] : Job{0, 0, 0, 0};
     uint32_t fid[kWide];      // frontier binary ids, left to right
     float farea[kWide];       // surface area, or -1 for a leaf / unused slot
@@ -272,6 +272,30 @@ __global__ __launch_bounds__(kCollapseBlock) __attribute__((amdgpu_waves_per_eu(
         order[job.wnode * 8 + oct] = pm;
     }
     atomicMax(&counters[3], job.depth);
+}
+
+// One level of the collapse: counters[0] jobs (read on the device), their
+// children appended to `next` (counters[2]). A grid of resident waves walks the
+// jobs 64 at a time.
+__global__ __launch_bounds__(kCollapseBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void wide_collapse_kernel(
+    const BNode *__restrict__ bn, const Job *__restrict__ jobs, Job *__restrict__ next,
+    uint32_t *__restrict__ counters, WNode *__restrict__ wn, uint32_t *__restrict__ order, uint32_t cap,
+    uint32_t *__restrict__ err) {
+    const uint32_t njobs = counters[0];
+    const uint32_t stride = gridDim.x * kCollapseBlock;
+    for (uint32_t base = blockIdx.x * kCollapseBlock; base < njobs; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        collapse_one(bn, jobs, i, i < njobs, next, counters, wn, order, cap, err);
+    }
+}
+
+// Between two levels: the next level's job count becomes the current one, a
+// level that had jobs is counted (counters[4]).
+__global__ void collapse_advance_kernel(uint32_t *counters) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (counters[0] > 0) counters[4] += 1;
+    counters[0] = counters[2];
+    counters[2] = 0;
 }
 
 // Entry distance of one child box, or +inf when the slab test fails for a
@@ -644,29 +668,42 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
     Job *jobs[2] = {reinterpret_cast<Job *>(scratch.ptr),
                     reinterpret_cast<Job *>(static_cast<char *>(scratch.ptr) + job_bytes)};
     uint32_t *counters = reinterpret_cast<uint32_t *>(static_cast<char *>(scratch.ptr) + 2 * job_bytes);
-    // counters: [0] jobs this level, [1] wide nodes used, [2] jobs next level, [3] max depth
+    // counters: [0] jobs this level, [1] wide nodes used, [2] jobs next level, [3] max depth,
+    // [4] levels that had jobs
     e = hipMemsetAsync(counters, 0, 64, stream);
     if (e != hipSuccess) return e;
     WNode *wn = out.nodes.as<WNode>();
     uint32_t *order = out.order.as<uint32_t>();
     wide_root_kernel<<<1, 64, 0, stream>>>(bnodes, jobs[0], counters, wn);
-    uint32_t host[4];
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wide_collapse_kernel, kCollapseBlock, 0) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu <= 0 ||
+        cus <= 0) {
+        per_cu = 4;
+        cus = 256;
+    }
+    const uint32_t resident = uint32_t(per_cu) * uint32_t(cus);
+    // levels are enqueued four at a time between reads of the job count; a level
+    // past the last one finds no jobs and exits at once
+    uint32_t host[5];
+    uint32_t enqueued = 0;
     int cur = 0;
     for (;;) {
         e = hipMemcpyAsync(host, counters, sizeof(host), hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return e;
-        const uint32_t nj = host[0];
-        if (nj == 0) break;
-        if (++out.collapse_levels > 100000) return hipErrorUnknown;
-        e = hipMemsetAsync(counters + 2, 0, 4, stream);
-        if (e != hipSuccess) return e;
-        wide_collapse_kernel<<<(nj + kCollapseBlock - 1) / kCollapseBlock, kCollapseBlock, 0, stream>>>(
-            bnodes, jobs[cur], nj, jobs[1 - cur], counters, wn, order, cap, err);
-        e = hipMemcpyAsync(counters, counters + 2, 4, hipMemcpyDeviceToDevice, stream);
-        if (e != hipSuccess) return e;
-        cur = 1 - cur;
+        if (host[0] == 0) break;
+        for (int k = 0; k < 4; ++k) {
+            if (++enqueued > 100000) return hipErrorUnknown;
+            wide_collapse_kernel<<<resident, kCollapseBlock, 0, stream>>>(bnodes, jobs[cur], jobs[1 - cur], counters,
+                                                                          wn, order, cap, err);
+            collapse_advance_kernel<<<1, 64, 0, stream>>>(counters);
+            cur = 1 - cur;
+        }
     }
+    out.collapse_levels = host[4];
     out.num_nodes = host[1];
     out.max_depth = std::max(host[3], 1u);
     return hipGetLastError();
