@@ -694,16 +694,37 @@ __device__ uint32_t wave_build(const BuildArgs &A, const Rec *src, uint32_t begi
             if (active) L.bkt[lane] = bk;
             wave_lds_sync();
             const float total_sa = box_area(fb);
-            if (lane < kNumBuckets - 1) {  // lane k prices the split after bucket k
-                Box bl = box_empty(), br = box_empty();
-                uint32_t cl = 0, cr = 0;
-                for (uint32_t j = lo; j < hi; ++j) {
+            // lane k prices the split after bucket k. From 16 elements on, lanes k and
+            // k + 23 each take half of the elements and the halves are merged (exact
+            // min / max and integer sums: the same boxes and counts)
+            constexpr uint32_t kSplits = kNumBuckets - 1;
+            const bool halves = m >= 16;
+            const uint32_t kk = lane < kSplits ? lane : lane - kSplits;
+            const bool pricing = lane < (halves ? 2 * kSplits : kSplits);
+            const uint32_t mid = halves ? (lo + hi) / 2 : hi;
+            const uint32_t j0 = lane < kSplits ? lo : mid, j1 = lane < kSplits ? mid : hi;
+            Box bl = box_empty(), br = box_empty();
+            uint32_t cl = 0, cr = 0;
+            if (pricing)
+                for (uint32_t j = j0; j < j1; ++j) {
                     const Rec q = L.rec[j];
-                    if (L.bkt[j] <= lane) { bl = box_union(bl, rec_box(q)); cl += q.b; }
+                    if (L.bkt[j] <= kk) { bl = box_union(bl, rec_box(q)); cl += q.b; }
                     else { br = box_union(br, rec_box(q)); cr += q.b; }
                 }
-                L.cost[lane] = 0.125 + (float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa;
+            if (halves) {  // all lanes: lane k takes lane k + 23's half
+                const int src = int(lane + kSplits) & (kWave - 1);
+                const Box ol{{__shfl(bl.lo.x, src, kWave), __shfl(bl.lo.y, src, kWave), __shfl(bl.lo.z, src, kWave)},
+                             {__shfl(bl.hi.x, src, kWave), __shfl(bl.hi.y, src, kWave), __shfl(bl.hi.z, src, kWave)}};
+                const Box orr{{__shfl(br.lo.x, src, kWave), __shfl(br.lo.y, src, kWave), __shfl(br.lo.z, src, kWave)},
+                              {__shfl(br.hi.x, src, kWave), __shfl(br.hi.y, src, kWave), __shfl(br.hi.z, src, kWave)}};
+                const uint32_t ocl = __shfl(cl, src, kWave), ocr = __shfl(cr, src, kWave);
+                bl = box_union(bl, ol);
+                br = box_union(br, orr);
+                cl += ocl;
+                cr += ocr;
             }
+            if (lane < kSplits)
+                L.cost[lane] = 0.125 + (float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa;
             wave_lds_sync();
             uint32_t split = 0;
             float best = L.cost[0];
