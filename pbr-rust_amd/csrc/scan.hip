@@ -83,10 +83,56 @@ __global__ __launch_bounds__(kScanBlock) void scan_down_kernel(const uint32_t *i
     }
 }
 
+// Small inputs: one workgroup walks the tiles in order with a running carry
+// (one launch instead of three; in place is fine, a tile is read before it is
+// written).
+__global__ __launch_bounds__(kScanBlock) void scan_single_kernel(const uint32_t *in, uint32_t *out, uint32_t n,
+                                                                 uint32_t *__restrict__ total) {
+    __shared__ uint32_t s_tile[kScanTile + kScanTile / 32];
+    __shared__ uint32_t s_waves[kScanBlock / kWave];
+    auto pad = [](uint32_t i) { return i + (i >> 5); };
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += kScanTile) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const uint32_t j = k * kScanBlock + threadIdx.x;
+            s_tile[pad(j)] = base + j < n ? in[base + j] : 0u;
+        }
+        __syncthreads();
+        uint32_t v[kScanItems];
+        uint32_t run = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const uint32_t t = s_tile[pad(threadIdx.x * kScanItems + k)];
+            v[k] = run;
+            run += t;
+        }
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_add<kScanBlock>(run, s_waves, &tot) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) s_tile[pad(threadIdx.x * kScanItems + k)] = v[k] + ex;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const uint32_t j = k * kScanBlock + threadIdx.x;
+            if (base + j < n) out[base + j] = s_tile[pad(j)];
+        }
+        carry += tot;
+        __syncthreads();  // s_tile and s_waves are reused by the next tile
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+constexpr uint32_t kScanSingleMax = 16 * kScanTile;
+
 }  // namespace
 
 hipError_t exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total,
                               DevBuf &scratch, hipStream_t stream) {
+    if (n <= kScanSingleMax) {
+        scan_single_kernel<<<1, kScanBlock, 0, stream>>>(in, out, n, total);
+        return hipGetLastError();
+    }
     const uint32_t nb = n == 0 ? 0 : (n + kScanTile - 1) / kScanTile;
     hipError_t e = scratch.reserve(size_t(nb + 1) * sizeof(uint32_t));
     if (e != hipSuccess) return e;
