@@ -268,25 +268,28 @@ __global__ void root_kernel(BuildArgs A, uint32_t num_treelets, uint32_t n) {
 // Level kernels read the level's item count (*nitems) and task count (*ntasks)
 // from device memory, so that several levels are enqueued without reading them
 // back; grids cover host-side upper bounds and the excess exits.
-__global__ __launch_bounds__(256) void acc_init_kernel(Acc *acc, const uint32_t *nitems) {
-    const uint32_t i = blockIdx.x;
-    if (i >= *nitems) return;
-    uint32_t *w = reinterpret_cast<uint32_t *>(&acc[i]);
-    for (int k = threadIdx.x; k < kAccWords; k += blockDim.x) {
-        uint32_t v = 0u;
-        if (k < 3 || (k >= 6 && k < 9)) v = 0xFFFFFFFFu;  // cb.lo, fb.lo
-        if (k >= 12 + int(kNumBuckets)) {
-            int j = (k - 12 - kNumBuckets) % 6;
-            v = j < 3 ? 0xFFFFFFFFu : 0u;
-        }
-        w[k] = v;
-    }
-}
 
-__global__ __launch_bounds__(256) void task_count_kernel(const WorkItem *items, const uint32_t *nitems,
-                                                         uint32_t bound, uint32_t *counts) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < bound) counts[i] = i < *nitems ? (items[i].end - items[i].begin + kChunk - 1) / kChunk : 0u;
+// Start of a level, one thread per accumulator word of the `bound` item slots:
+// chunk counts (zero past the level's items, for the scan over the bound), the
+// items' accumulators set to their identities, and the next level's list
+// counter zeroed (no kernel of this level reads it).
+__global__ __launch_bounds__(256) void level_setup_kernel(const WorkItem *items, const uint32_t *nitems,
+                                                          uint32_t bound, uint32_t *counts, Acc *acc,
+                                                          uint32_t *next_counter) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) *next_counter = 0u;
+    const uint32_t i = g / kAccWords, k = g % kAccWords;
+    if (i >= bound) return;
+    const uint32_t n = *nitems;
+    if (k == 0) counts[i] = i < n ? (items[i].end - items[i].begin + kChunk - 1) / kChunk : 0u;
+    if (i >= n) return;
+    uint32_t v = 0u;
+    if (k < 3 || (k >= 6 && k < 9)) v = 0xFFFFFFFFu;  // cb.lo, fb.lo
+    if (k >= 12 + kNumBuckets) {
+        const uint32_t j = (k - 12 - kNumBuckets) % 6;
+        v = j < 3 ? 0xFFFFFFFFu : 0u;
+    }
+    reinterpret_cast<uint32_t *>(&acc[i])[k] = v;
 }
 
 // One thread per task: its item is the last one whose first task is <= t (every
@@ -987,10 +990,10 @@ static hipError_t run_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, 
             if (++enqueued > 100000) return hipErrorUnknown;  // every level strictly shrinks its nodes
             const uint32_t tb = elements / kChunk + nb + 1;
             const uint32_t *ncur = counters + ccur;
-            task_count_kernel<<<(nb + 255) / 256, 256, 0, stream>>>(cur, ncur, nb, task_off);
+            level_setup_kernel<<<uint32_t((uint64_t(nb) * kAccWords + 255) / 256), 256, 0, stream>>>(
+                cur, ncur, nb, task_off, acc, counters + cnxt);
             PBRTM_TRY(exclusive_scan_u32(task_off, task_off, nb, ntasks, S.scan_scratch, stream));
             task_fill_kernel<<<(tb + 255) / 256, 256, 0, stream>>>(ncur, nb, task_off, ntasks, tasks);
-            acc_init_kernel<<<nb, 256, 0, stream>>>(acc, ncur);
             chunk_reduce_kernel<<<tb, kChunkBlock, 0, stream>>>(tasks, ntasks, cur, recs, acc);
             if (TOP) chunk_buckets_kernel<<<tb, kChunkBlock, 0, stream>>>(tasks, ntasks, cur, recs, acc, A.err);
             decide_kernel<TOP><<<TOP ? nb : (nb + 63) / 64, 64, 0, stream>>>(cur, ncur, acc, splits);
@@ -999,7 +1002,6 @@ static hipError_t run_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, 
             if (TOP) PBRTM_TRY(exclusive_scan_u32(tlp, tlp, tb, tlp + tb, S.scan_scratch, stream));
             chunk_scatter_kernel<TOP><<<tb, kChunkBlock, 0, stream>>>(tasks, ntasks, cur, recs, splits, task_off,
                                                                       tleft);
-            zero_counters_kernel<<<1, 64, 0, stream>>>(counters + cnxt, 1);
             children_kernel<TOP><<<(nb + 63) / 64, 64, 0, stream>>>(A, cur, ncur, splits, acc, task_off, tleft, tlp,
                                                                     nxt, counters + cnxt,
                                                                     counters + (TOP ? kCntLevelsTop : kCntLevelsBot));
