@@ -193,8 +193,29 @@ typedef struct pbrtm_render_params {
     uint32_t spp;       /* samples per pixel, a square (stratified) */
     uint32_t max_depth; /* Integrator "path" "integer maxdepth" (default 5) */
     uint32_t pool_size; /* path slots in flight; 0 = 2^21 as the reference (wavefront_path.cu:18) */
-    uint32_t flags;     /* PBRTM_DEVICE_PTRS: film_rgb / film_aux are device pointers */
+    uint32_t flags;     /* PBRTM_DEVICE_PTRS: film_rgb / film_aux are device pointers;
+                           one PBRTM_INTEGRATOR_* value (default: the wavefront path integrator) */
 } pbrtm_render_params;
+
+/* pbrtm_render_params.flags: the integrator (SceneBuilder::build_integrator,
+ * scene_builder.cu:289-313; MegakernelIntegrator::create, megakernel_integrator.cu:82-111).
+ * The megakernel integrators add each pixel's samples to the film in sample order
+ * (megakernel_integrator.cu:14-42); here they run on the same path pool. Only
+ * pbrtm_render takes them; pbrtm_render_grad and pbrtm_render_batch render with
+ * the wavefront path integrator. */
+#define PBRTM_INTEGRATOR_MASK 0xF0u
+#define PBRTM_INTEGRATOR_PATH 0x00u              /* "path", "volpath": WavefrontPathIntegrator */
+#define PBRTM_INTEGRATOR_MEGAKERNEL_PATH 0x10u   /* "megakernelpath": MegakernelPathIntegrator
+                                                    (integrators/megakernel_path.cu:35-213) */
+#define PBRTM_INTEGRATOR_AMBIENT_OCCLUSION 0x20u /* "ambientocclusion": AmbientOcclusionIntegrator
+                                                    (integrators/ambient_occlusion.cu:9-58) */
+#define PBRTM_INTEGRATOR_SURFACE_NORMAL 0x30u    /* "surfacenormal": SurfaceNormalIntegrator
+                                                    (integrators/surface_normal.cu:8-32) */
+
+/* The PBRTM_INTEGRATOR_* value of an integrator name as the reference's
+ * --integrator option and Integrator directive take it; PBRTM_ERR_UNSUPPORTED
+ * for names the library does not render (bdpt, mlt, ...). */
+int pbrtm_integrator_from_name(const char *name, uint32_t *flags_out);
 
 typedef struct pbrtm_render_stats {
     double ms_total, ms_trace, ms_shadow, ms_shade, ms_film;

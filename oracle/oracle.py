@@ -169,6 +169,8 @@ def _pt_sigs(lib):
     sig = {
         "orc_render": (c_int, [POINTER(OrcScene), c_int, p, p, p]),
         "orc_render_paths": (c_int, [POINTER(OrcScene), c_int, p, p, p]),
+        "orc_render_path_L": (c_int, [POINTER(OrcScene), c_int, p, p]),
+        "orc_render_megakernel": (c_int, [POINTER(OrcScene), c_int, c_int, p, p, p, p]),
         "orc_rgb_to_sigmoid_jacobian": (c_int, [p, p]),
         "orc_render_grad": (c_int, [POINTER(OrcScene), c_int, p, p, POINTER(ctypes.c_double), p, p, p, c_int]),
         "orc_rgb_to_sigmoid": (c_int, [p, p]),
@@ -270,6 +272,48 @@ def render(sc, pool_size=2 * 1024 * 1024, threads=8):
         raise RuntimeError(f"orc_render failed ({rc})")
     return rgb, aux, {"rounds": int(st[0]), "rays": int(st[1]), "shadow_rays": int(st[2]), "paths": int(st[3]),
                       "render_ms": int(st[4]) / 1e3}
+
+
+INTEGRATORS = {"megakernelpath": 1, "ambientocclusion": 2, "surfacenormal": 3}
+
+
+def render_megakernel(sc, integrator, threads=8, path_L=False):
+    """Render a scene dict with a megakernel integrator ("megakernelpath",
+    "ambientocclusion", "surfacenormal"); returns (film_rgb, aux, stats) and, with
+    path_L, each path's spectral radiance [spp][h][w][4] as a fourth item."""
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        _pt_sigs(lib)
+        lib._pt_ready = True
+    s, keep = _scene_struct(sc, 1)
+    h, w = sc["height"], sc["width"]
+    rgb = np.zeros((h, w, 3), np.float32)
+    aux = np.zeros((h, w, 4), np.float32)
+    pl = np.zeros((sc["spp"], h, w, 4), np.float32) if path_L else None
+    st = np.zeros(5, np.uint64)
+    rc = lib.orc_render_megakernel(ctypes.byref(s), INTEGRATORS[integrator], threads, _p(rgb), _p(aux), _p(pl),
+                                   _p(st))
+    if rc != 0:
+        raise RuntimeError(f"orc_render_megakernel failed ({rc})")
+    stats = {"rays": int(st[1]), "shadow_rays": int(st[2]), "paths": int(st[3]), "render_ms": int(st[4]) / 1e3}
+    return (rgb, aux, stats, pl) if path_L else (rgb, aux, stats)
+
+
+def render_path_L(sc, pool_size=2 * 1024 * 1024, threads=8):
+    """The wavefront render's per-path spectral radiance and wavelength pdfs, each
+    [spp][h][w][4] (path id = sample * W * H + pixel)."""
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        _pt_sigs(lib)
+        lib._pt_ready = True
+    s, keep = _scene_struct(sc, pool_size)
+    shape = (sc["spp"], sc["height"], sc["width"], 4)
+    L = np.zeros(shape, np.float32)
+    pdf = np.zeros(shape, np.float32)
+    rc = lib.orc_render_path_L(ctypes.byref(s), threads, _p(L), _p(pdf))
+    if rc != 0:
+        raise RuntimeError(f"orc_render_path_L failed ({rc})")
+    return L, pdf
 
 
 def render_paths(sc, pool_size=2 * 1024 * 1024, threads=8):

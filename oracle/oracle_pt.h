@@ -66,6 +66,19 @@ int orc_render(const orc_scene *scene, int threads, float *film_rgb, float *aux,
 int orc_render_paths(const orc_scene *scene, int threads, uint32_t *path_start, uint32_t *path_finish,
                      uint64_t *stats);
 
+/* The wavefront render's per-path spectral radiance and wavelength pdfs
+ * ([W*H*spp][4] by path id; path_pdf nullable). */
+int orc_render_path_L(const orc_scene *scene, int threads, float *path_L, float *path_pdf);
+
+/* The megakernel integrators (base/megakernel_integrator.cu:14-147): integrator
+ * 1 = "megakernelpath" (integrators/megakernel_path.cu:35-213), 2 =
+ * "ambientocclusion" (ambient_occlusion.cu:25-58), 3 = "surfacenormal"
+ * (surface_normal.cu:17-32). Every pixel's samples in sample order, each added
+ * to the film as it finishes; film_rgb / aux as orc_render; path_L (nullable)
+ * as orc_render_path_L; pool_size unused. */
+int orc_render_megakernel(const orc_scene *scene, int integrator, int threads, float *film_rgb, float *aux,
+                          float *path_L, uint64_t *stats);
+
 /* Loss and gradient of the path-traced film (FP64 path replay): loss = sum over
  * pixels and channels of (film - target)^2; d_rgb [num_materials][3] is
  * dLoss/d(diffuse albedo RGB) (0 for other materials), d_scale [num_lights]

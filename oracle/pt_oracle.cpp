@@ -1698,6 +1698,232 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
     }
 }
 
+// ------------------------------------------------------------ the megakernel integrators
+// MegakernelIntegrator (base/megakernel_integrator.cu:14-42, :82-147): one thread
+// per pixel renders that pixel's samples in sample order, one whole path at a
+// time, and adds each to the film as it finishes.
+constexpr int kIntMegakernelPath = 1, kIntAmbientOcclusion = 2, kIntSurfaceNormal = 3;
+
+struct MkScene {
+    const Scene &sc;
+    uint32_t max_depth;
+    float ao_scale;  // AmbientOcclusionIntegrator::create: 1 / illuminant.to_photometric(Y)
+    Sigmoid sn[3];   // SurfaceNormalIntegrator::init: RGBAlbedoSpectrum of RGB 0.01 e_c
+};
+struct MkCount {
+    uint64_t rays = 0, shadow = 0;
+};
+
+const float kInf = std::numeric_limits<float>::infinity();
+
+inline bool mk_closest(const Scene &sc, V3 o, V3 d, uint32_t *prim, TriHit *h, MkCount &n) {
+    float t, b[3];
+    n.rays++;
+    int rc = orc_trace_ray(sc.bvh, &o.x, &d.x, kInf, 0, prim, &t, b);
+    if (rc < 0) throw std::runtime_error("oracle traversal stack overflow");
+    *h = TriHit{b[0], b[1], b[2], t};
+    return rc == 1;
+}
+inline bool mk_any(const Scene &sc, V3 o, V3 d, float t_max, MkCount &n) {
+    uint32_t prim;
+    n.shadow++;
+    int rc = orc_trace_ray(sc.bvh, &o.x, &d.x, t_max, 1, &prim, nullptr, nullptr);
+    if (rc < 0) throw std::runtime_error("oracle traversal stack overflow");
+    return rc == 1;
+}
+
+// AmbientOcclusionIntegrator::li (integrators/ambient_occlusion.cu:25-58).
+S4 ao_li(const MkScene &m, V3 ro, V3 rd, const Lambda &lam, Sampler &smp, MkCount &n) {
+    uint32_t prim;
+    TriHit h;
+    if (!mk_closest(m.sc, ro, rd, &prim, &h, n)) return s4(0.0f);
+    const Isect is = make_isect(m.sc, prim, h, rd);
+    const V3 normal = face_forward(is.ns, neg(rd));
+    float u0, u1;
+    smp.get_2d(&u0, &u1);
+    const V3 lw = sample_cosine_hemisphere(u0, u1);
+    const float pdf = std::fabs(lw.z) / kPi;  // cosine_hemisphere_pdf (sampling.h:12-14)
+    if (pdf == 0.0f) return s4(0.0f);
+    V3 fx, fy;
+    coordinate_system(normal, &fx, &fy);  // Frame::from_z (frame.h:24-29)
+    const V3 wi = Frame{fx, fy, normal}.from_local(lw);
+    if (mk_any(m.sc, offset_ray_origin(is.pi, is.n, wi), wi, kInf, n)) return s4(0.0f);
+    return sample_pl(m.sc.sp->d65, lam) * (m.ao_scale * dot(normal, wi) / (kPi * pdf));
+}
+
+// SurfaceNormalIntegrator::li (integrators/surface_normal.cu:17-32); Vector3::softmax
+// (vector3.h:134-137) with exp10 evaluated in double and rounded once to float.
+S4 sn_li(const MkScene &m, V3 ro, V3 rd, const Lambda &lam, MkCount &n) {
+    uint32_t prim;
+    TriHit h;
+    if (!mk_closest(m.sc, ro, rd, &prim, &h, n)) return s4(0.0f);
+    const Isect is = make_isect(m.sc, prim, h, rd);
+    const V3 v = normalize(face_forward(is.ns, neg(rd)));
+    const float e[3] = {float(::exp10(double(v.x))), float(::exp10(double(v.y))), float(::exp10(double(v.z)))};
+    const float s = e[0] + e[1] + e[2];
+    const float c[3] = {e[0] / s, e[1] / s, e[2] / s};
+    return (sample_sigmoid(m.sn[0], lam) * c[0] + sample_sigmoid(m.sn[1], lam) * c[1]) +
+           sample_sigmoid(m.sn[2], lam) * c[2];
+}
+
+// MegakernelPathIntegrator::evaluate_li and sample_ld (integrators/megakernel_path.cu:35-213).
+S4 mk_path_li(const MkScene &m, V3 ro, V3 rd, Lambda &lam, Sampler &smp, MkCount &n) {
+    const Scene &sc = m.sc;
+    S4 L = s4(0.0f), beta = s4(1.0f);
+    bool specular = true;
+    uint32_t depth = 0;
+    float pdf_bsdf = NAN, eta_scale = 1;
+    LightCtx prev{};
+    for (;;) {
+        uint32_t prim;
+        TriHit h;
+        if (!mk_closest(sc, ro, rd, &prim, &h, n)) break;  // no infinite lights
+        const Tri &t = sc.tris[prim];
+        const Isect is = make_isect(sc, prim, h, rd);
+        if (t.light >= 0) {
+            const S4 Le = light_L(sc, sc.lights[t.light], is.n, neg(rd), lam);
+            if (positive(Le)) {
+                if (depth == 0 || specular) {
+                    L = L + beta * Le;
+                } else {
+                    const float pdf_light = sc.light_pmf[t.light] * tri_pdf(t, prev, rd);
+                    L = L + (beta * power_heuristic(pdf_bsdf, pdf_light)) * Le;
+                }
+            }
+        }
+        const Bsdf b = make_bsdf(sc, is, lam);  // before the depth test: may end the secondary wavelengths
+        if (depth++ == m.max_depth) break;
+        const uint32_t flags = bsdf_flags(b);
+        if (non_specular(flags)) {
+            LightCtx ctx{is.pi, is.n, is.ns};
+            const bool refl = (flags & 1u) != 0, trans = (flags & 2u) != 0;
+            if (refl && !trans) ctx.pi = make_p3fi(offset_ray_origin(is.pi, is.n, is.wo), {0, 0, 0});
+            else if (trans && !refl) ctx.pi = make_p3fi(offset_ray_origin(is.pi, is.n, neg(is.wo)), {0, 0, 0});
+            const float u = smp.get_1d();
+            float ul0, ul1;
+            smp.get_2d(&ul0, &ul1);
+            if (!sc.lights.empty()) {
+                const uint32_t li = search_cdf(u, sc.select_cdf.data(), uint32_t(sc.lights.size()));
+                const Light &Lg = sc.lights[li];
+                ShapeSample ss;
+                const bool ok = tri_sample(sc.tris[Lg.tri], ctx, ul0, ul1, &ss);
+                const V3 cpt = p3fi_mid(ctx.pi), sp = p3fi_mid(ss.pi);
+                if (ok && !(ss.pdf == 0 || sqlen(sub(sp, cpt)) == 0)) {
+                    const V3 wi = normalize(sub(sp, cpt));
+                    const S4 Le = light_L(sc, Lg, ss.n, neg(wi), lam);
+                    if (positive(Le) && ss.pdf != 0) {
+                        const S4 f = bsdf_f(b, is.wo, wi) * abs_dot(wi, is.ns);
+                        if (positive(f)) {
+                            // IntegratorBase::unoccluded: p -> p_light, then p_light -> p
+                            V3 so[2], sd[2];
+                            spawn_ray_to(is.pi, is.n, ss.pi, ss.n, &so[0], &sd[0]);
+                            spawn_ray_to(ss.pi, ss.n, is.pi, is.n, &so[1], &sd[1]);
+                            bool occ = false;
+                            for (int r = 0; r < 2 && !occ; ++r) occ = mk_any(sc, so[r], sd[r], 0.6f, n);
+                            if (!occ) {
+                                const float pdf_light = sc.light_pmf[li] * ss.pdf;
+                                const float wl = power_heuristic(pdf_light, bsdf_pdf(b, is.wo, wi));
+                                L = L + beta * (((Le * wl) * f) / pdf_light);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        const V3 wo = neg(rd);
+        const float uc = smp.get_1d();
+        float u0, u1;
+        smp.get_2d(&u0, &u1);
+        BS bs;
+        if (!bsdf_sample(b, wo, uc, u0, u1, &bs)) break;
+        beta = beta * ((bs.f * abs_dot(bs.wi, is.ns)) / bs.pdf);
+        pdf_bsdf = bs.pdf;
+        specular = bs.specular;
+        if (bs.transmission) eta_scale *= sqrf(bs.eta);
+        prev = LightCtx{is.pi, is.n, is.ns};
+        ro = offset_ray_origin(is.pi, is.n, bs.wi);
+        rd = bs.wi;
+        if (depth > 8) {  // Russian roulette: a sample is drawn only when beta's max is below 1
+            const S4 rr = beta * eta_scale;
+            if (max_comp(rr) < 1) {
+                const float q = clampf(1 - max_comp(rr), 0, 0.95f);
+                if (smp.get_1d() < q) break;
+                beta = beta / (1 - q);
+            }
+        }
+    }
+    return L;
+}
+
+// path_L (nullable, [W*H*spp][4] by path id = sample * W * H + pixel): each
+// sample's radiance before the sensor conversion.
+void render_mk(const orc_scene &d, int integrator, int threads, float *film_rgb, float *aux, float *path_L,
+               uint64_t *stats) {
+    Scene sc;
+    build_scene(sc, d);
+    const uint32_t W = d.width, H = d.height;
+    const uint32_t spd = uint32_t(std::sqrt(double(d.spp)));
+    if (spd * spd != d.spp || d.spp == 0) throw std::runtime_error("spp must be a square");
+    MkScene m{sc, d.max_depth, 1.0f / sc.sp->d65.inner(sc.sp->cie[1]), {}};
+    for (int c = 0; c < 3; ++c) {
+        float rgb[3] = {0, 0, 0};
+        rgb[c] = 0.01f;
+        m.sn[c] = sc.sp->table.lookup(rgb);
+    }
+    const uint64_t wh = uint64_t(W) * H;
+    std::vector<float> rgb_sum(wh * 3, 0.f), w_sum(wh, 0.f);
+    std::vector<MkCount> counts(wh);
+    const auto t_start = std::chrono::steady_clock::now();
+    parallel(wh, threads, [&](size_t px) {
+        MkCount &n = counts[px];
+        for (uint32_t s = 0; s < d.spp; ++s) {
+            Sampler smp;
+            smp.spd = spd;
+            smp.start(uint32_t(px), s, 0);
+            float u0, u1;
+            smp.get_2d(&u0, &u1);
+            float fx, fy, fw;
+            sc.filter.sample(u0, u1, &fx, &fy, &fw);
+            const float rx = (float(px % W) + fx) + 0.5f, ry = (float(px / W) + fy) + 0.5f;
+            Lambda lam = sample_visible(smp.get_1d());
+            const V3 ro = xf_point(sc.cam.render_from_camera.m, {0, 0, 0});
+            const V3 rd = xf_vector(sc.cam.render_from_camera.m, normalize(xf_point(sc.cam.camera_from_raster.m, {rx, ry, 0})));
+            S4 L;
+            if (integrator == kIntAmbientOcclusion) L = ao_li(m, ro, rd, lam, smp, n);
+            else if (integrator == kIntSurfaceNormal) L = sn_li(m, ro, rd, lam, n);
+            else L = mk_path_li(m, ro, rd, lam, smp, n);
+            if (path_L)
+                for (int i = 0; i < kNS; ++i) path_L[4 * (uint64_t(s) * wh + px) + i] = L.v[i];
+            float rgb[3];
+            sensor_rgb(sc, L, lam, rgb);
+            if (fw == 0) continue;
+            for (int c = 0; c < 3; ++c) rgb_sum[3 * px + c] += fw * rgb[c];
+            w_sum[px] += fw;
+        }
+    });
+    for (uint64_t px = 0; px < wh; ++px) {
+        float rgb[3] = {rgb_sum[3 * px], rgb_sum[3 * px + 1], rgb_sum[3 * px + 2]};
+        if (w_sum[px] != 0)
+            for (int c = 0; c < 3; ++c) rgb[c] /= w_sum[px];
+        auto o = m3_apply(sc.out_from_sensor, rgb);
+        for (int c = 0; c < 3; ++c) film_rgb[3 * px + c] = o[c];
+        if (aux) {
+            for (int c = 0; c < 3; ++c) aux[4 * px + c] = rgb_sum[3 * px + c];
+            aux[4 * px + 3] = w_sum[px];
+        }
+    }
+    if (stats) {
+        MkCount tot;
+        for (const MkCount &c : counts) tot.rays += c.rays, tot.shadow += c.shadow;
+        stats[0] = 0;
+        stats[1] = tot.rays;
+        stats[2] = tot.shadow;
+        stats[3] = wh * d.spp;
+        stats[4] = uint64_t(std::chrono::duration_cast<std::chrono::microseconds>(
+                                std::chrono::steady_clock::now() - t_start).count());
+    }
+}
+
 }  // namespace
 }  // namespace orc
 
@@ -1707,6 +1933,38 @@ int orc_render(const orc_scene *scene, int threads, float *film_rgb, float *aux,
     if (!scene || !film_rgb || !scene->tri || !scene->tri_material || scene->pool_size == 0) return -1;
     try {
         orc::render(*scene, threads, film_rgb, aux, stats);
+    } catch (const std::exception &) {
+        return -2;
+    }
+    return 0;
+}
+
+int orc_render_megakernel(const orc_scene *scene, int integrator, int threads, float *film_rgb, float *aux,
+                          float *path_L, uint64_t *stats) {
+    if (!scene || !film_rgb || !scene->tri || !scene->tri_material) return -1;
+    if (integrator < orc::kIntMegakernelPath || integrator > orc::kIntSurfaceNormal) return -1;
+    try {
+        orc::render_mk(*scene, integrator, threads, film_rgb, aux, path_L, stats);
+    } catch (const std::exception &) {
+        return -2;
+    }
+    return 0;
+}
+
+int orc_render_path_L(const orc_scene *scene, int threads, float *path_L, float *path_pdf) {
+    if (!scene || !path_L || !scene->tri || !scene->tri_material || scene->pool_size == 0) return -1;
+    try {
+        std::vector<float> rgb(size_t(scene->width) * scene->height * 3);
+        std::vector<orc::S4> L, pdf;
+        orc::GradIO gio;
+        gio.Lfin = &L;
+        gio.pdffin = &pdf;
+        orc::render(*scene, threads, rgb.data(), nullptr, nullptr, nullptr, nullptr, &gio);
+        for (size_t i = 0; i < L.size(); ++i)
+            for (int k = 0; k < orc::kNS; ++k) {
+                path_L[4 * i + k] = L[i].v[k];
+                if (path_pdf) path_pdf[4 * i + k] = pdf[i].v[k];
+            }
     } catch (const std::exception &) {
         return -2;
     }

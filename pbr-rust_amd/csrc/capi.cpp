@@ -671,6 +671,8 @@ int pbrtm_scene_create(pbrtm_ctx *ctx, const pbrtm_scene_desc *desc, pbrtm_scene
     std::memcpy(D.out_from_sensor, H.out_from_sensor, sizeof(D.out_from_sensor));
     D.width = H.width;
     D.height = H.height;
+    D.ao_scale = H.ao_scale;
+    std::memcpy(D.sn_c, H.sn_c, sizeof(D.sn_c));
     // The aggregate over the render-space triangles (SceneBuilder::preprocess, scene_builder.cu:829-838).
     const int rc = pbrtm_bvh_build(ctx, sc->tri.as<float>(), 3 * desc->num_tris, nullptr, desc->num_tris,
                                    PBRTM_DEVICE_PTRS, &sc->bvh, nullptr);
@@ -714,12 +716,31 @@ const char *check_render_params(const pbrtm_scene *sc, const pbrtm_render_params
         return "at most 2^32 - 1 - pool_size paths per render";
     *code = PBRTM_ERR_INVALID;
     if (params->max_depth > 0xFFFE) return "max_depth too large";
+    if ((params->flags & PBRTM_INTEGRATOR_MASK) > PBRTM_INTEGRATOR_SURFACE_NORMAL) return "unknown integrator";
     return "";
 }
 
 }  // namespace
 
 extern "C" {
+
+int pbrtm_integrator_from_name(const char *name, uint32_t *flags_out) {
+    if (!name || !flags_out) return PBRTM_ERR_INVALID;
+    static const struct {
+        const char *name;
+        uint32_t flags;
+    } kNames[] = {{"path", PBRTM_INTEGRATOR_PATH},
+                  {"volpath", PBRTM_INTEGRATOR_PATH},
+                  {"megakernelpath", PBRTM_INTEGRATOR_MEGAKERNEL_PATH},
+                  {"ambientocclusion", PBRTM_INTEGRATOR_AMBIENT_OCCLUSION},
+                  {"surfacenormal", PBRTM_INTEGRATOR_SURFACE_NORMAL}};
+    for (const auto &k : kNames)
+        if (std::strcmp(name, k.name) == 0) {
+            *flags_out = k.flags;
+            return PBRTM_OK;
+        }
+    return PBRTM_ERR_UNSUPPORTED;
+}
 
 int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *params, float *film_rgb,
                  float *film_aux, pbrtm_render_stats *stats) {
@@ -747,7 +768,9 @@ int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *par
     CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream), "memset");
     pbrtm::pt::RenderStats rs;
     CK(ctx, pbrtm::pt::render(sc->dev, W, spp, params->max_depth, pool, sc->pool, d_rgb, d_aux,
-                              ctx->dev_err.as<uint32_t>(), &rs, ctx->stream), "render");
+                              ctx->dev_err.as<uint32_t>(), &rs, ctx->stream, nullptr, nullptr, nullptr,
+                              (params->flags & PBRTM_INTEGRATOR_MASK) >> 4),
+       "render");
     uint32_t derr = 0;
     CK(ctx, hipMemcpyAsync(&derr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "download");
     if (!dev) {
@@ -779,6 +802,8 @@ int pbrtm_render_grad(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params
     int code = 0;
     const char *why = check_render_params(sc, params, &code);
     if (*why) return fail(ctx, code, why);
+    if (params->flags & PBRTM_INTEGRATOR_MASK)
+        return fail(ctx, PBRTM_ERR_UNSUPPORTED, "gradients render with the wavefront path integrator only");
     const pbrtm::pt::SceneHost &H = sc->host;
     const uint32_t ng = uint32_t(H.group_scale.size());
     if (ng > 255) return fail(ctx, PBRTM_ERR_UNSUPPORTED, "gradients support at most 255 emitters");
@@ -879,6 +904,8 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
         int code = 0;
         const char *why = check_render_params(jb.scene, &jb.params, &code);
         if (*why) return fail(ctx, code, why);
+        if (jb.params.flags & PBRTM_INTEGRATOR_MASK)
+            return fail(ctx, PBRTM_ERR_UNSUPPORTED, "batched jobs render with the wavefront path integrator only");
         if (jb.params.max_depth > 4000) return fail(ctx, PBRTM_ERR_UNSUPPORTED, "batched jobs allow max_depth <= 4000");
         const uint64_t wh = uint64_t(jb.scene->host.width) * jb.scene->host.height;
         film_floats += wh * (jb.film_aux ? 7 : 3);

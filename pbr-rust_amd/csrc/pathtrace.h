@@ -62,7 +62,12 @@ struct SceneDev {
     uint32_t filt_n;
     float out_from_sensor[9];
     uint32_t width, height;
+    float ao_scale;  // ambient occlusion: 1 / the colour space illuminant's to_photometric(Y)
+    float sn_c[9];   // surface normal: sigmoid coefficients of RGBAlbedoSpectrum(0.01 e_c), c = r, g, b
 };
+
+// Integrators render() runs (SceneBuilder::build_integrator, scene_builder.cu:289-313).
+enum Integrator : uint32_t { kIntPath = 0, kIntMegakernelPath = 1, kIntAmbientOcclusion = 2, kIntSurfaceNormal = 3 };
 
 // Gradient replay state (device pointers).
 struct GradDev {
@@ -102,6 +107,8 @@ struct SceneHost {
     uint32_t filt_n;
     float out_from_sensor[9];
     uint32_t width, height;
+    float ao_scale;
+    float sn_c[9];
     std::vector<float> mat_rgb;      // per material [3]: the diffuse albedo RGB (gradients)
     std::vector<float> group_scale;  // per emitter: the scene's "scale" parameter (gradients)
 };
@@ -140,7 +147,7 @@ struct RenderStats {
 hipError_t render(const SceneDev &S, const WideTrace &W, uint32_t spp, uint32_t max_depth, uint32_t pool,
                   PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
                   hipStream_t stream, float4 *lfin = nullptr, float4 *pfin = nullptr,
-                  const GradDev *gd = nullptr);
+                  const GradDev *gd = nullptr, uint32_t integrator = kIntPath);
 
 // Loss sum((film - target)^2) per pixel into loss_px, and adj [h][w][3] =
 // dLoss/d(rgb_sum / weight_sum) through out_from_sensor (divided by the

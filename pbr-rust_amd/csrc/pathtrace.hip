@@ -415,7 +415,10 @@ __device__ __forceinline__ float dot4(const S4 &a, const S4 &b) {
     return v;
 }
 
-template <bool MULTI, bool GRAD>
+// MODE is the integrator: kIntPath is the wavefront path integrator; the others
+// run the megakernel integrators (base/megakernel_integrator.cu:82-147) on the
+// same pool, one round per path vertex.
+template <bool MULTI, bool GRAD, uint32_t MODE = kIntPath>
 __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, const Params &P, uint32_t i,
                                            uint32_t pid, uint32_t lid, uint32_t spd, uint32_t max_depth,
                                            uint32_t round, float4 *erec, float4 *grec) {
@@ -455,19 +458,63 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
         lpdf = f4s(st.lam_pdf[i]);
         scal = st.scal[i];
     }
-    // Russian roulette (wavefront_path.cu:121-138), only past 8 bounces.
+    // Russian roulette (wavefront_path.cu:121-138), only past 8 bounces. The
+    // wavefront draws its sample always; the megakernel path integrator only
+    // when beta's largest value is below 1 (megakernel_path.cu:142-153).
     if (live && !terminate && depth > 8) {
-        const float u = smp.get_1d();
+        const float u = MODE == kIntMegakernelPath ? 0.f : smp.get_1d();
         const S4 rr = s4scale(beta, scal.z);
         if (s4max(rr) < 1) {
             const float q = clampf(1 - s4max(rr), 0, 0.95f);
-            if (u < q) {
+            if ((MODE == kIntMegakernelPath ? smp.get_1d() : u) < q) {
                 beta = s4(0.0f);
                 terminate = true;
             } else {
                 beta = s4div(beta, 1 - q);
             }
         }
+    }
+    // Emission with MIS at this vertex (wavefront_path.cu:179-196,
+    // megakernel_path.cu:81-99).
+    auto add_emission = [&](const Tri &tri, uint32_t light, f3 n, f3 rd) {
+        const DLight Ld = S.lights[light - 1];
+        const S4 Le = light_L(S, Ld, n, neg3(rd), lam);
+        if (!s4pos(Le)) return;
+        if (depth == 0 || (fl & kFlagSpecular)) {
+            L = s4add(L, s4mul(beta, Le));
+            if (GRAD) erec[1] = make_float4(dot4(gadj, s4mul(beta, Le)), 0.f, __uint_as_float(Ld.group), 0.f);
+        } else {
+            const float4 q0 = st.prev[3 * size_t(i)], q1 = st.prev[3 * size_t(i) + 1], q2 = st.prev[3 * size_t(i) + 2];
+            const P3i ppi{{q0.x, q0.y}, {q0.z, q0.w}, {q1.x, q1.y}};
+            const f3 pn = {q1.z, q1.w, q2.x}, pns = {q2.y, q2.z, q2.w};
+            const float pdf_light = S.light_pmf[light - 1] * tri_pdf(tri, ppi, pn, pns, rd);
+            const float wl = power_heuristic(scal.y, pdf_light);
+            L = s4add(L, s4mul(s4scale(beta, wl), Le));
+            if (GRAD) {  // MIS weight attached: dC/dpmf = -2 C (1 - w) / pmf
+                const double G = dot4(gadj, s4mul(s4scale(beta, wl), Le));
+                erec[1] = make_float4(float(G * (2.0 * double(wl) - 1.0)), float(-2.0 * G * (1.0 - double(wl))),
+                                      __uint_as_float(Ld.group), 0.f);
+            }
+        }
+    };
+    // The megakernel path integrator ends a path at depth max_depth after the
+    // vertex's emission and get_bsdf (megakernel_path.cu:81-111): no light
+    // sample, no BSDF sample. get_bsdf of a dielectric with a spectral eta still
+    // ends the secondary wavelengths (dielectric_material.cu:47-50).
+    if (MODE == kIntMegakernelPath && live && !terminate && depth == max_depth) {
+        const Tri tri = load_tri(S, prim);
+        const uint2 info = S.tri_info[prim];
+        const TriHit th{st.bary[3 * size_t(i)], st.bary[3 * size_t(i) + 1], st.bary[3 * size_t(i) + 2], st.thit[i]};
+        const HitGeom g = hit_geom(th, tri.p0, tri.p1, tri.p2);
+        const float4 r1 = st.rays[2 * size_t(i) + 1];
+        const uint32_t light = info.y & 0x7FFFFFFFu;
+        if (light) add_emission(tri, light, tri.flip ? neg3(g.n) : g.n, f3{r1.x, r1.y, r1.z});
+        const DMat m = S.mats[info.x];
+        if (m.type == kDielectric && m.eta_n && !(lpdf.v[1] == 0 && lpdf.v[2] == 0 && lpdf.v[3] == 0)) {
+            lpdf.v[1] = lpdf.v[2] = lpdf.v[3] = 0;
+            lpdf.v[0] /= kNS;
+        }
+        terminate = true;
     }
     // 2. finished paths: film record, then a new path in this slot
     const uint64_t lanes = __ballot(terminate);
@@ -480,7 +527,8 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
     if (terminate) {
         const float3 rgb = sensor_rgb(S, L, lam, lpdf);
         st.rec[pid] = make_float4(rgb.x, rgb.y, rgb.z, scal.x);
-        st.rec_round[pid] = MULTI ? round - st.sround[i] : round;
+        // the megakernel integrators add each pixel's samples in sample order
+        st.rec_round[pid] = MODE != kIntPath ? 0u : (MULTI ? round - st.sround[i] : round);
         if (st.lfin) {
             st.lfin[pid] = s4f(L);
             st.pfin[pid] = s4f(lpdf);
@@ -520,31 +568,60 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
     const P3i ipi{g.x, g.y, g.z};
     const f3 n = tri.flip ? neg3(g.n) : g.n;
     const f3 ns = n;
-    const f3 wo_n = norm3(neg3(rd));
-    // emission with MIS (wavefront_path.cu:179-196)
-    const uint32_t light = info.y & 0x7FFFFFFFu;
-    if (light) {
-        const DLight Ld = S.lights[light - 1];
-        const S4 Le = light_L(S, Ld, n, neg3(rd), lam);
-        if (s4pos(Le)) {
-            if (depth == 0 || (fl & kFlagSpecular)) {
-                L = s4add(L, s4mul(beta, Le));
-                if (GRAD) erec[1] = make_float4(dot4(gadj, s4mul(beta, Le)), 0.f, __uint_as_float(Ld.group), 0.f);
-            } else {
-                const float4 q0 = st.prev[3 * size_t(i)], q1 = st.prev[3 * size_t(i) + 1], q2 = st.prev[3 * size_t(i) + 2];
-                const P3i ppi{{q0.x, q0.y}, {q0.z, q0.w}, {q1.x, q1.y}};
-                const f3 pn = {q1.z, q1.w, q2.x}, pns = {q2.y, q2.z, q2.w};
-                const float pdf_light = S.light_pmf[light - 1] * tri_pdf(tri, ppi, pn, pns, rd);
-                const float wl = power_heuristic(scal.y, pdf_light);
-                L = s4add(L, s4mul(s4scale(beta, wl), Le));
-                if (GRAD) {  // MIS weight attached: dC/dpmf = -2 C (1 - w) / pmf
-                    const double G = dot4(gadj, s4mul(s4scale(beta, wl), Le));
-                    erec[1] = make_float4(float(G * (2.0 * double(wl) - 1.0)), float(-2.0 * G * (1.0 - double(wl))),
-                                          __uint_as_float(Ld.group), 0.f);
-                }
+    if (MODE == kIntAmbientOcclusion || MODE == kIntSurfaceNormal) {
+        // One vertex, no emission; the path ends next round (its ray is retired).
+        const f3 nf = dot3(ns, neg3(rd)) < 0.0f ? neg3(ns) : ns;  // Vector3::face_forward
+        if (MODE == kIntSurfaceNormal) {
+            // SurfaceNormalIntegrator::li (surface_normal.cu:17-32): softmax of the
+            // normal (exp10 in double, rounded once) weighting three albedo spectra
+            const f3 v = norm3(nf);
+            const float e0 = float(::exp10(double(v.x))), e1 = float(::exp10(double(v.y))),
+                        e2 = float(::exp10(double(v.z)));
+            const float es = e0 + e1 + e2;
+            const float c0 = e0 / es, c1 = e1 / es, c2 = e2 / es;
+#pragma unroll
+            for (int k = 0; k < kNS; ++k) {
+                const float l = lam.v[k];
+                L.v[k] = (sigmoid_eval(S.sn_c[0], S.sn_c[1], S.sn_c[2], l) * c0 +
+                          sigmoid_eval(S.sn_c[3], S.sn_c[4], S.sn_c[5], l) * c1) +
+                         sigmoid_eval(S.sn_c[6], S.sn_c[7], S.sn_c[8], l) * c2;
             }
+        } else {
+            // AmbientOcclusionIntegrator::li (ambient_occlusion.cu:25-58): a cosine
+            // sample around the face-forwarded normal, its any-hit ray traced next
+            // round with t_max = infinity
+            float u0, u1;
+            smp.get_2d(&u0, &u1);
+            const f3 lw = cosine_hemisphere(u0, u1);
+            const float pdf = fabsf(lw.z) / kPi;
+            bool ao = false;
+            if (pdf != 0.0f) {
+                f3 bx, by;
+                basis(nf, &bx, &by);
+                const f3 wi = add3(add3(mul3(lw.x, bx), mul3(lw.y, by)), mul3(lw.z, nf));
+                const float k = S.ao_scale * dot3(nf, wi) / (kPi * pdf);
+                S4 d;
+#pragma unroll
+                for (int q = 0; q < kNS; ++q)
+                    d.v[q] = spec_eval_uni(S, S.d65_off, S.d65_n, S.d65_uni, S.d65_inv_step, lam.v[q]) * k;
+                st.nee_L[i] = s4f(d);
+                store_ray(st.srays + 4 * size_t(i), offset_ray_origin(ipi, n, wi), __builtin_inff(), wi);
+                st.srays[4 * size_t(i) + 2].w = 0.f;
+                ao = true;
+            }
+            if (ao) fl |= kFlagNee;
+            else st.srays[4 * size_t(i)].w = 0.f;
         }
+        st.rays[2 * size_t(i)].w = 0.f;
+        st.flags[i] = (fl & ~kDepthMask) | ((depth + 1) & kDepthMask);
+        st.L[i] = s4f(L);
+        st.rng[i] = smp.rng.state;
+        st.dim[i] = smp.dim;
+        return;
     }
+    const f3 wo_n = norm3(neg3(rd));
+    const uint32_t light = info.y & 0x7FFFFFFFu;
+    if (light) add_emission(tri, light, n, rd);
     fl = (fl & ~kDepthMask) | ((depth + 1) & kDepthMask);
     // 3. material: BSDF (interaction.cu:77-89, bsdf.cu), sample_ld, sample_f
     const DMat m = S.mats[info.x];
@@ -678,10 +755,11 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
     st.dim[i] = smp.dim;
 }
 
+template <uint32_t MODE>
 __global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Params P) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pid = i < P.pool ? st.path_id[i] : kIdle;
-    shade_impl<false, false>(S, st, P, i, pid, pid, P.spd, P.max_depth, P.round, nullptr, nullptr);
+    shade_impl<false, false, MODE>(S, st, P, i, pid, pid, P.spd, P.max_depth, P.round, nullptr, nullptr);
 }
 
 // The gradient replay of one round (same paths as the forward render).
@@ -1040,7 +1118,8 @@ State make_state(PathPool &pp, uint32_t *err) {
 
 hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t max_depth, uint32_t pool,
                   PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
-                  hipStream_t stream, float4 *lfin, float4 *pfin, const GradDev *gd) {
+                  hipStream_t stream, float4 *lfin, float4 *pfin, const GradDev *gd, uint32_t integrator) {
+    if (integrator > kIntSurfaceNormal || (gd && integrator != kIntPath)) return hipErrorInvalidValue;
     const uint64_t total = uint64_t(S.width) * S.height * spp;
     const uint32_t spd = uint32_t(sqrt(double(spp)));
     if (spd * spd != spp || spp == 0 || spp > kMaxSpp) return hipErrorInvalidValue;
@@ -1079,6 +1158,7 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
     PT_CK(hipMemsetAsync(st.counters, 0, 64, stream));
     uint32_t first = uint32_t(std::min<uint64_t>(total, pool));
     PT_CK(hipMemcpyAsync(st.counters + kCntNextId, &first, 4, hipMemcpyHostToDevice, stream));
+    uint64_t next_before = first;  // paths started before the current round
     PT_CK(hipEventRecord(ev[0], stream));
     if (gd) init_pool_kernel<true><<<grid, kBlock, 0, stream>>>(S, st, P);
     else init_pool_kernel<false><<<grid, kBlock, 0, stream>>>(S, st, P);
@@ -1126,12 +1206,23 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         }
         PT_CK(hipEventRecord(ev[3], stream));
         if (gd) shade_grad_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);
-        else shade_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);
+        else if (integrator == kIntMegakernelPath) shade_kernel<kIntMegakernelPath><<<grid, kBlock, 0, stream>>>(S, st, P);
+        else if (integrator == kIntAmbientOcclusion)
+            shade_kernel<kIntAmbientOcclusion><<<grid, kBlock, 0, stream>>>(S, st, P);
+        else if (integrator == kIntSurfaceNormal) shade_kernel<kIntSurfaceNormal><<<grid, kBlock, 0, stream>>>(S, st, P);
+        else shade_kernel<kIntPath><<<grid, kBlock, 0, stream>>>(S, st, P);
         PT_CK(hipGetLastError());
         PT_CK(hipEventRecord(ev[4], stream));
         PT_CK(hipMemcpyAsync(host_cnt, st.counters, 32, hipMemcpyDeviceToHost, stream));
         PT_CK(hipStreamSynchronize(stream));
-        if (host_cnt[kCntQueue] == 0) break;  // no live path was left: the round does not count
+        // Done when no ray was traced and no path could start: every live slot then
+        // ended this round. A wavefront path always has a ray, so there this round
+        // had no live path and does not count; the megakernel integrators' single-
+        // vertex paths retire their ray a round before they end, and that last
+        // round's shadow rays and film records do count.
+        const bool done = host_cnt[kCntQueue] == 0 && next_before >= total;
+        if (done && integrator == kIntPath) break;
+        next_before = host_cnt[kCntNextId];
         float a, b, c;
         hipEventElapsedTime(&a, ev[1], ev[2]);
         hipEventElapsedTime(&b, ev[2], ev[3]);
@@ -1157,6 +1248,7 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         rs.rays += host_cnt[kCntQueue];
         rs.shadow_rays += host_cnt[kCntSQueue];
         P.round++;
+        if (done) break;
         if (P.round > 100000) return hipErrorUnknown;
     }
     PT_CK(hipEventRecord(ev[5], stream));

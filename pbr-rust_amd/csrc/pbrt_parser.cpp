@@ -180,7 +180,7 @@ struct pbrtm_pbrt_file {
     std::vector<uint8_t> tri_reverse;
     std::vector<pbrtm_material> materials;
     std::vector<pbrtm_area_light> lights;
-    uint32_t spp = 4, max_depth = 5;
+    uint32_t spp = 4, max_depth = 5, integrator = PBRTM_INTEGRATOR_PATH;
     std::string filename, error;
 };
 
@@ -432,11 +432,13 @@ class Builder {
             } else if (k == "Integrator") {
                 const std::string type = read_str(T, i, "Integrator");
                 Params P = read_params(T, i);
-                if (type != "path" && type != "volpath")
+                // scene_builder.cu:289-313: "path"/"volpath" is the wavefront integrator,
+                // the megakernel ones go to MegakernelIntegrator::create
+                if (pbrtm_integrator_from_name(type.c_str(), &F.integrator) != PBRTM_OK)
                     throw std::runtime_error(where(t) + "integrator \"" + type + "\" is not supported");
                 max_depth = P.get_int("maxdepth", 5);
                 if (P.get_bool("regularize", false)) throw std::runtime_error(where(t) + "regularize is not supported");
-                P.check_used("Integrator", {"lightsampler"});
+                P.check_used("Integrator", {"lightsampler", "cossample", "maxdistance"});
             } else if (k == "PixelFilter") {
                 const std::string type = read_str(T, i, "PixelFilter");
                 Params P = read_params(T, i);
@@ -601,6 +603,7 @@ int pbrtm_pbrt_render_params(const pbrtm_pbrt_file *f, pbrtm_render_params *out)
     *out = pbrtm_render_params{};
     out->spp = f->spp;
     out->max_depth = f->max_depth;
+    out->flags = f->integrator;
     return PBRTM_OK;
 }
 

@@ -829,6 +829,17 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
         // output_rgb_from_sensor_rgb = rgb_from_xyz * identity (rgb_film.cu:67)
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) S.out_from_sensor[3 * i + j] = G.rgb_from_xyz.m[i][j];
+        // AmbientOcclusionIntegrator::create (ambient_occlusion.cu:9-23): the colour
+        // space illuminant and 1 / its photometric integral
+        S.ao_scale = 1.0f / G.d65.dot(G.cie[1]);
+        // SurfaceNormalIntegrator::init (surface_normal.h:17-23): albedo spectra of
+        // RGB (0.01, 0, 0), (0, 0.01, 0), (0, 0, 0.01)
+        for (int c = 0; c < 3; ++c) {
+            float rgb[3] = {0.f, 0.f, 0.f};
+            rgb[c] = 0.01f;
+            const auto k = G.table.lookup(rgb);
+            for (int q = 0; q < 3; ++q) S.sn_c[3 * c + q] = k[q];
+        }
     } catch (const std::exception &e) {
         return e.what();
     }

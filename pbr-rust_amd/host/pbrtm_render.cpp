@@ -1,8 +1,10 @@
 // Command-line renderer over the C ABI, with the reference's options
 // (scene/command_line_option.h, main.cu): pbrtm-render [--spp N]
-// [--integrator path] [--outfile out.png] scene.pbrt
-// Parses the file, builds the scene on device 0, renders with the wavefront
-// path integrator and writes the PNG (the Film filename unless --outfile).
+// [--integrator path|volpath|megakernelpath|ambientocclusion|surfacenormal]
+// [--outfile out.png] scene.pbrt
+// Parses the file, builds the scene on device 0, renders with the file's
+// Integrator (or --integrator, which overrides it as in scene_builder.cu:366-371)
+// and writes the PNG (the Film filename unless --outfile).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -12,7 +14,7 @@
 #include "../../include/pbrtm.h"
 
 int main(int argc, char **argv) {
-    std::string input, outfile, integrator = "path";
+    std::string input, outfile, integrator;
     int spp = -1;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -36,7 +38,8 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "please provide a PBRT input file:\n$ pbrtm-render example.pbrt\n");
         return 1;
     }
-    if (integrator != "path" && integrator != "volpath") {
+    uint32_t integrator_flags = PBRTM_INTEGRATOR_PATH;
+    if (!integrator.empty() && pbrtm_integrator_from_name(integrator.c_str(), &integrator_flags) != PBRTM_OK) {
         std::fprintf(stderr, "integrator `%s` is not supported\n", integrator.c_str());
         return 1;
     }
@@ -48,6 +51,7 @@ int main(int argc, char **argv) {
     }
     pbrtm_render_params params;
     pbrtm_pbrt_render_params(file, &params);
+    if (!integrator.empty()) params.flags = (params.flags & ~PBRTM_INTEGRATOR_MASK) | integrator_flags;
     if (spp > 0) {
         int s = 1;
         while ((s + 1) * (s + 1) <= spp) ++s;

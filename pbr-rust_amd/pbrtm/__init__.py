@@ -35,9 +35,12 @@ EXPORTS = (
     "pbrtm_gen_camera_rays", "pbrtm_scene_create", "pbrtm_scene_free", "pbrtm_render", "pbrtm_rgb_to_sigmoid",
     "pbrtm_srgb_matrices", "pbrtm_srgb8", "pbrtm_write_png", "pbrtm_pbrt_parse", "pbrtm_pbrt_desc",
     "pbrtm_pbrt_render_params", "pbrtm_pbrt_output", "pbrtm_pbrt_free", "pbrtm_render_batch",
-    "pbrtm_render_grad",
+    "pbrtm_render_grad", "pbrtm_integrator_from_name",
 )
 BATCH_NO_GRAPH = 2
+# pbrtm_render_params.flags: the integrator
+INTEGRATORS = {"path": 0x00, "volpath": 0x00, "megakernelpath": 0x10, "ambientocclusion": 0x20,
+               "surfacenormal": 0x30}
 
 
 class BuildStats(ctypes.Structure):
@@ -149,6 +152,7 @@ def load(path: str = LIB_PATH):
         "pbrtm_scene_create": (c_int, [p, POINTER(SceneDesc), POINTER(p)]),
         "pbrtm_scene_free": (None, [p]),
         "pbrtm_render": (c_int, [p, p, POINTER(RenderParams), p, p, POINTER(RenderStats)]),
+        "pbrtm_integrator_from_name": (c_int, [ctypes.c_char_p, POINTER(c_uint32)]),
         "pbrtm_rgb_to_sigmoid": (c_int, [p, p]),
         "pbrtm_srgb_matrices": (c_int, [p, p, p]),
         "pbrtm_srgb8": (c_int, [p, c_uint64, p]),
@@ -426,6 +430,8 @@ class PbrtFile:
         p = RenderParams()
         self.lib.pbrtm_pbrt_render_params(h, ctypes.byref(p))
         self.spp, self.max_depth = p.spp, p.max_depth
+        flags = p.flags & 0xF0
+        self.integrator = next(k for k, v in INTEGRATORS.items() if v == flags and k != "volpath")
         self.output = self.lib.pbrtm_pbrt_output(h).decode()
 
     def free(self):
@@ -507,12 +513,12 @@ class Scene:
         except Exception:
             pass
 
-    def render_grad(self, target, spp=4, max_depth=5, pool_size=0):
+    def render_grad(self, target, spp=4, max_depth=5, pool_size=0, integrator="path"):
         """Loss sum((film - target)^2) and its gradient (pbrtm_render_grad): returns
         (loss, d_albedo [num_materials][3], d_scale [num_lights], film, stats dict)."""
         tgt = np.ascontiguousarray(target, np.float32)
         assert tgt.shape == (self.height, self.width, 3)
-        p = RenderParams(spp, max_depth, pool_size, 0)
+        p = RenderParams(spp, max_depth, pool_size, INTEGRATORS[integrator])
         film = np.zeros((self.height, self.width, 3), np.float32)
         d_alb = np.zeros((self.num_materials, 3), np.float32)
         d_sc = np.zeros(max(1, self.num_lights), np.float32)
@@ -522,9 +528,11 @@ class Scene:
                                                       ctypes.byref(loss), _ptr(d_alb), _ptr(d_sc), ctypes.byref(st)))
         return loss.value, d_alb, d_sc[:self.num_lights], film, st.as_dict()
 
-    def render(self, spp=4, max_depth=5, pool_size=0, film_rgb=None, film_aux=None, device_ptrs=False):
-        """Returns (film_rgb [h][w][3], film_aux [h][w][4], stats dict)."""
-        p = RenderParams(spp, max_depth, pool_size, DEVICE_PTRS if device_ptrs else 0)
+    def render(self, spp=4, max_depth=5, pool_size=0, film_rgb=None, film_aux=None, device_ptrs=False,
+               integrator="path"):
+        """Returns (film_rgb [h][w][3], film_aux [h][w][4], stats dict). integrator: "path"
+        (wavefront), "megakernelpath", "ambientocclusion" or "surfacenormal"."""
+        p = RenderParams(spp, max_depth, pool_size, (DEVICE_PTRS if device_ptrs else 0) | INTEGRATORS[integrator])
         st = RenderStats()
         if not device_ptrs:
             film_rgb = np.zeros((self.height, self.width, 3), np.float32)

@@ -2,6 +2,7 @@
 scene the tests build in Python, transforms compose as the reference's
 graphics state does (CTM = CTM * T), and unsupported input is an error, not a
 silent approximation."""
+import ctypes
 import math
 import os
 
@@ -98,3 +99,26 @@ def test_camera_must_be_a_lookat(tmp_path):
     head = 'Sampler "stratified" "integer pixelsamples" 10\nFilm "rgb"\nWorldBegin\n'
     f = pbrtm.PbrtFile(write(tmp_path, TRI, head))
     assert f.spp == 9  # rounded down to a square, as the stratified sampler needs
+
+
+@pytest.mark.parametrize("name,flags", [("path", 0x00), ("volpath", 0x00), ("megakernelpath", 0x10),
+                                        ("ambientocclusion", 0x20), ("surfacenormal", 0x30)])
+def test_integrator_names(tmp_path, name, flags):
+    """scene_builder.cu:289-313: "volpath" is the wavefront "path"; the megakernel
+    integrators by their MegakernelIntegrator::create names."""
+    lib = pbrtm.load()
+    out = ctypes.c_uint32(0xFFFF)
+    assert lib.pbrtm_integrator_from_name(name.encode(), ctypes.byref(out)) == pbrtm.OK and out.value == flags
+    head = f'Integrator "{name}" "integer maxdepth" [3]\nFilm "rgb"\nWorldBegin\n'
+    f = pbrtm.PbrtFile(write(tmp_path, TRI, head))
+    assert pbrtm.INTEGRATORS[f.integrator] == flags and f.max_depth == 3
+
+
+@pytest.mark.parametrize("name", ["bdpt", "mlt", "Path", ""])
+def test_unsupported_integrators(tmp_path, name):
+    lib = pbrtm.load()
+    out = ctypes.c_uint32(0)
+    assert lib.pbrtm_integrator_from_name(name.encode(), ctypes.byref(out)) == pbrtm.ERR_UNSUPPORTED
+    if name:
+        with pytest.raises(pbrtm.PbrtmError, match="integrator"):
+            pbrtm.PbrtFile(write(tmp_path, TRI, f'Integrator "{name}"\nFilm "rgb"\nWorldBegin\n'))
