@@ -15,7 +15,9 @@ displaced-sphere grid (2,097,218 triangles; diffuse, rough copper and glass;
 64 emissive triangles) at 1920x1080, 64 spp stratified, max depth 8:
 Msamples/s = camera samples / render time, the scene and its HLBVH resident
 in HBM before the timed renders, with the oracle's render of the same
-workload at 128x72 as its CPU baseline.
+workload at 128x72 as its CPU baseline. The "integrators" object measures the
+megakernel integrators (megakernelpath, ambientocclusion) on the same scene at
+16 spp the same way.
 """
 import argparse
 import json
@@ -134,6 +136,49 @@ def pathtracer(args, ctx, pbrtm):
             "sample": f"oracle wavefront render of the same scene at {cw}x{ch}, {spp} spp, depth {depth} "
                       f"({cw * ch * spp} samples, {ost['render_ms']:.0f} ms, set-up and BVH build excluded)",
         }
+    return out
+
+
+def integrators(args, ctx, pbrtm):
+    """Msamples/s of the megakernel integrators (megakernelpath at depth 8,
+    ambientocclusion) on the same sphere grid at 1920x1080, 16 spp, each next to
+    the oracle's sample-by-sample render of the same scene at 128x72."""
+    import numpy as np
+    from pbrtm import scenes
+
+    W, H, spp, depth = 1920, 1080, 16, 8
+    sc = scenes.sphere_grid(width=W, height=H, spp=spp, max_depth=depth)
+    out = {}
+    scene = ctx.scene(sc)
+    try:
+        for name in ("megakernelpath", "ambientocclusion"):
+            scene.render(spp=spp, max_depth=depth, integrator=name)  # warm-up
+            runs = [scene.render(spp=spp, max_depth=depth, integrator=name)[2] for _ in range(args.pt_steps)]
+            ms = float(np.mean([r["ms_total"] for r in runs]))
+            last = runs[-1]
+            out[name] = {
+                "value": round(W * H * spp / (ms * 1e3), 2), "unit": "Msamples/s", "ms_per_image": round(ms, 2),
+                "rounds": last["rounds"], "rays": last["rays"], "shadow_rays": last["shadow_rays"],
+                "mrays_per_s": round((last["rays"] + last["shadow_rays"]) / (ms * 1e3), 1),
+            }
+    finally:
+        scene.free()
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        cw, ch = 128, 72
+        small = scenes.sphere_grid(width=cw, height=ch, spp=spp, max_depth=depth)
+        for name in out:
+            _, _, ost = oracle.render_megakernel(small, name, threads=16)
+            out[name]["cpu_baseline"] = {
+                "value": round(cw * ch * spp / (ost["render_ms"] * 1e3), 4), "unit": "Msamples/s", "cores": 16,
+                "kind": "port",
+                "sample": f"oracle megakernel render of the same scene at {cw}x{ch}, {spp} spp ({cw * ch * spp} "
+                          f"samples, {ost['render_ms']:.0f} ms, set-up and BVH build excluded)",
+            }
+    out["config"] = {"workload": "sphere grid 32x32 UV spheres, 2,097,218 tris, 64 emitters", "width": W,
+                     "height": H, "spp": spp, "max_depth": depth, "dtype": "f32"}
     return out
 
 
@@ -391,6 +436,7 @@ def main():
         del d_tri, d_rays, d_prim, d_t, d_bary, d_bounce, d_prim2, d_t2
         torch.cuda.empty_cache()
         line["pathtracer"] = pathtracer(args, ctx, pbrtm)
+        line["integrators"] = integrators(args, ctx, pbrtm)
     if not args.no_batch:
         line["batch"] = batch(args, ctx, pbrtm)
     if not args.no_grad:
