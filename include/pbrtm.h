@@ -171,6 +171,17 @@ typedef struct pbrtm_area_light { /* AreaLightSource "diffuse" (lights/diffuse_a
     uint32_t two_sided;           /* "bool twosided" */
 } pbrtm_area_light;
 
+typedef struct pbrtm_infinite_light { /* LightSource "infinite" without "filename"
+                                         (lights/uniform_infinite_light.cu:4-43) */
+    float rgb[3];                 /* "rgb L" (an RGBIlluminantSpectrum) */
+    uint32_t has_rgb;             /* 0: no "L", the colour space illuminant itself */
+    float scale;                  /* "float scale" (default 1) */
+    float illuminance;            /* "float illuminance" (<= 0: not given) */
+    uint32_t light_index;         /* its place in the scene's light list: the number of
+                                     emissive triangles created before it (the reference keeps
+                                     LightSource and area lights in file order) */
+} pbrtm_infinite_light;
+
 typedef struct pbrtm_scene_desc {
     float eye[3], look[3], up[3]; /* LookAt */
     float fov;                    /* Camera "perspective" "float fov" (degrees) */
@@ -187,6 +198,8 @@ typedef struct pbrtm_scene_desc {
     uint32_t num_xforms;          /* object-to-world matrices (the CTM at each Shape) */
     const float *xforms;          /* [num_xforms][4][4] row-major; may be NULL */
     const uint32_t *tri_xform;    /* [n] index into xforms, UINT32_MAX = identity; NULL = all identity */
+    uint32_t num_infinite_lights; /* at most 4 */
+    const pbrtm_infinite_light *infinite_lights;
 } pbrtm_scene_desc;
 
 typedef struct pbrtm_render_params {

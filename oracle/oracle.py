@@ -152,13 +152,19 @@ class OrcLight(ctypes.Structure):
     _fields_ = [("rgb", c_float * 3), ("scale", c_float), ("two_sided", c_uint32)]
 
 
+class OrcInfiniteLight(ctypes.Structure):
+    _fields_ = [("rgb", c_float * 3), ("has_rgb", c_uint32), ("scale", c_float), ("illuminance", c_float),
+                ("light_index", c_uint32)]
+
+
 class OrcScene(ctypes.Structure):
     _fields_ = [("eye", c_float * 3), ("look", c_float * 3), ("up", c_float * 3), ("fov", c_float),
                 ("width", c_uint32), ("height", c_uint32), ("spp", c_uint32), ("max_depth", c_uint32),
                 ("pool_size", c_uint32), ("num_tris", c_uint64), ("tri", c_void_p), ("tri_material", c_void_p),
                 ("tri_light", c_void_p), ("tri_reverse", c_void_p), ("num_materials", c_uint32),
                 ("materials", POINTER(OrcMaterial)), ("num_lights", c_uint32), ("lights", POINTER(OrcLight)),
-                ("num_xforms", c_uint32), ("xforms", c_void_p), ("tri_xform", c_void_p), ("select_scale", c_void_p)]
+                ("num_xforms", c_uint32), ("xforms", c_void_p), ("tri_xform", c_void_p), ("select_scale", c_void_p),
+                ("num_infinite_lights", c_uint32), ("infinite_lights", POINTER(OrcInfiniteLight))]
 
 
 METALS = {"Ag": 0, "Al": 1, "Au": 2, "Cu": 3}
@@ -253,7 +259,18 @@ def _scene_struct(sc, pool_size):
     sel = sc.get("select_scale")
     sel = None if sel is None else np.ascontiguousarray(sel, dtype=np.float32)
     s.select_scale = _p(sel)
-    return s, (tri, mat, lig, rev, mats, lights, xf, txf, sel, marr)
+    inf = sc.get("infinite_lights", [])
+    infl = (OrcInfiniteLight * max(1, len(inf)))()
+    for i, l in enumerate(inf):
+        if l.get("rgb") is not None:
+            infl[i].rgb[:] = list(l["rgb"])
+            infl[i].has_rgb = 1
+        infl[i].scale = l.get("scale", 1.0)
+        infl[i].illuminance = l.get("illuminance", -1.0)
+        infl[i].light_index = l.get("light_index", 0)
+    s.num_infinite_lights = len(inf)
+    s.infinite_lights = infl
+    return s, (tri, mat, lig, rev, mats, lights, xf, txf, sel, marr, infl)
 
 
 def render(sc, pool_size=2 * 1024 * 1024, threads=8):

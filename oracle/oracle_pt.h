@@ -32,6 +32,14 @@ typedef struct orc_light {  /* AreaLightSource "diffuse" */
     uint32_t two_sided;
 } orc_light;
 
+typedef struct orc_infinite_light { /* LightSource "infinite" (uniform_infinite_light.cu:4-43) */
+    float rgb[3];           /* "rgb L" */
+    uint32_t has_rgb;       /* 0: no "L", the colour space illuminant */
+    float scale;
+    float illuminance;      /* <= 0: not given */
+    uint32_t light_index;   /* place in the light list: emissive triangles created before it */
+} orc_infinite_light;
+
 typedef struct orc_scene {
     float eye[3], look[3], up[3]; /* LookAt */
     float fov;                    /* Camera "perspective" "float fov" */
@@ -54,6 +62,8 @@ typedef struct orc_scene {
     const float *select_scale;    /* [num_lights] nullable: emitter scales the light choice's cdf
                                      is built from (test knob: freezes the choice under a change of
                                      the scales; pmf values in the estimator stay the scene's) */
+    uint32_t num_infinite_lights;
+    const orc_infinite_light *infinite_lights;
 } orc_scene;
 
 /* film_rgb: [height][width][3] = RGBFilm::get_pixel_rgb; aux (nullable):

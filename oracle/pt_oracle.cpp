@@ -449,6 +449,7 @@ struct Light {
     float rgb_scale;              // RGBIlluminantSpectrum::scale
     Sigmoid rsp;
     bool two_sided;
+    bool infinite = false;        // UniformInfiniteLight (tri unused; no "L": rgb_scale 1, rsp = 1)
 };
 
 struct Scene {
@@ -462,6 +463,8 @@ struct Scene {
     std::vector<Light> lights;
     std::vector<float> light_pmf, light_cdf;
     std::vector<float> select_cdf;  // the cdf the NEE light choice searches (= light_cdf unless overridden)
+    std::vector<uint32_t> infinite;  // indices of the infinite lights, in light order
+    float inf_dist = 0;             // 2 x the scene's bounding-sphere radius
     Spectra *sp;
     PLSpectrum metals[8], glass_bk7;
     M3 out_from_sensor;
@@ -573,6 +576,55 @@ void build_scene(Scene &sc, const orc_scene &d) {
             sc.lights.push_back(l);
         }
     }
+    // Infinite lights join the light list at their place in file order (scene_builder.cu:494, :637).
+    if (d.num_infinite_lights) {
+        if (!d.infinite_lights) throw std::runtime_error("infinite lights missing");
+        std::vector<Light> area = std::move(sc.lights), all;
+        std::vector<int32_t> remap(area.size());
+        for (size_t a = 0; a <= area.size(); ++a) {
+            for (uint32_t q = 0; q < d.num_infinite_lights; ++q) {
+                const orc_infinite_light &I = d.infinite_lights[q];
+                const size_t at = std::min<size_t>(I.light_index, area.size());
+                if (at != a) continue;
+                // UniformInfiniteLight::create (uniform_infinite_light.cu:4-30)
+                Light l{};
+                l.infinite = true;
+                l.two_sided = true;
+                l.tri = 0xFFFFFFFFu;
+                float scale = I.scale / sc.sp->d65.inner(sc.sp->cie[1]);
+                if (I.illuminance > 0) scale *= I.illuminance / kPi;
+                l.scale = scale;
+                if (I.has_rgb) {
+                    l.rgb_scale = float(2.0 * double(std::max(std::max(I.rgb[0], I.rgb[1]), I.rgb[2])));
+                    float c[3] = {0, 0, 0};
+                    if (l.rgb_scale > 0.0f)
+                        for (int w = 0; w < 3; ++w) c[w] = std::max(0.f, I.rgb[w] / l.rgb_scale);
+                    l.rsp = sc.sp->table.lookup(c);
+                } else {
+                    l.rgb_scale = 1;
+                    l.rsp = Sigmoid{0, 0, std::numeric_limits<float>::infinity()};
+                }
+                sc.infinite.push_back(uint32_t(all.size()));
+                all.push_back(l);
+            }
+            if (a < area.size()) {
+                remap[a] = int32_t(all.size());
+                all.push_back(area[a]);
+            }
+        }
+        sc.lights = std::move(all);
+        for (Tri &t : sc.tris)
+            if (t.light >= 0) t.light = remap[t.light];
+        // Bounds3::bounding_sphere of the aggregate's bounds (uniform_infinite_light.h:17-19)
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t i = 0; i < sc.soup.size(); i += 3)
+            for (int c = 0; c < 3; ++c) lo[c] = std::min(lo[c], sc.soup[i + c]), hi[c] = std::max(hi[c], sc.soup[i + c]);
+        const V3 ctr{(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2, (lo[2] + hi[2]) / 2};
+        const bool inside = ctr.x >= lo[0] && ctr.x <= hi[0] && ctr.y >= lo[1] && ctr.y <= hi[1] && ctr.z >= lo[2] &&
+                            ctr.z <= hi[2];
+        const float radius = inside ? length(sub(ctr, V3{hi[0], hi[1], hi[2]})) : 0.0f;
+        sc.inf_dist = 2 * radius;
+    }
     orc_build_info info;
     if (orc_bvh_build(sc.soup.data(), d.num_tris, &sc.bvh, &info) != 0) throw std::runtime_error("oracle BVH build");
     // PowerLightSampler::create (power_light_sampler.cu:7-45) + Distribution1D::build.
@@ -584,9 +636,15 @@ void build_scene(Scene &sc, const orc_scene &d) {
         for (const Light &l : sc.lights) {
             S4 Ls;
             for (int i = 0; i < kNS; ++i) Ls.v[i] = l.rgb_scale * l.rsp(lam.l[i]);
-            S4 L = (sample_pl(sc.sp->d65, lam) * Ls) * l.scale;
-            float k = kPi * float(l.two_sided ? 2 : 1) * tri_area(sc.tris[l.tri]);
-            S4 phi = L * k;
+            S4 phi;
+            if (l.infinite) {  // UniformInfiniteLight::phi: 4 pi^2 R^2 scale L(lambda)
+                const float R = sc.inf_dist / 2;
+                phi = (sample_pl(sc.sp->d65, lam) * Ls) * (4 * kPi * kPi * (R * R) * l.scale);
+            } else {
+                S4 L = (sample_pl(sc.sp->d65, lam) * Ls) * l.scale;
+                float k = kPi * float(l.two_sided ? 2 : 1) * tri_area(sc.tris[l.tri]);
+                phi = L * k;
+            }
             w.push_back(average(safe_div(phi, pdf)));
         }
         double sum = 0.0;
@@ -599,6 +657,7 @@ void build_scene(Scene &sc, const orc_scene &d) {
         for (size_t i = 1; i < n; ++i) sc.light_cdf[i] = sc.light_cdf[i - 1] + sc.light_pmf[i];
         sc.select_cdf = sc.light_cdf;
         if (d.select_scale) {
+            if (!sc.infinite.empty()) throw std::runtime_error("select_scale with infinite lights");
             double ssum = 0.0;
             std::vector<double> ws(n);
             for (size_t i = 0; i < n; ++i) {
@@ -1255,6 +1314,30 @@ inline float power_heuristic(float f, float g) {  // sampling.h:259-269 with nf 
     return sqrf(ff) / (sqrf(ff) + sqrf(gg));
 }
 
+// UniformInfiniteLight::sample_li (lights/uniform_infinite_light.cu:51-60) with
+// sample_uniform_sphere / uniform_sphere_pdf (util/sampling.h:203-216).
+inline V3 sample_uniform_sphere(float u0, float u1) {
+    const float z = 1 - 2 * u0;
+    const float r = safe_sqrt(1 - sqrf(z));
+    const float phi = 2 * kPi * u1;
+    return {r * tcos(phi), r * tsin(phi), z};
+}
+const float kUniformSpherePdf = float(1.0 / (4.0 * double(kPi)));
+
+// The infinite lights' emission on a path that ends with a positive beta (the
+// wavefront's control_logic, wavefront_path.cu:140-160) or escapes (megakernel,
+// megakernel_path.cu:58-75): pdf_li with an incomplete pdf allowed is 0, so the
+// MIS weight of a non-specular bounce is power_heuristic(pdf_bsdf, 0).
+inline S4 add_infinite(const Scene &sc, S4 L, const S4 &beta, const Lambda &lam, uint32_t depth, bool specular,
+                       float pdf_bsdf) {
+    for (uint32_t li : sc.infinite) {
+        const S4 Le = light_L(sc, sc.lights[li], V3{0, 0, 0}, V3{0, 0, 0}, lam);
+        if (depth == 0 || specular) L = L + beta * Le;
+        else L = L + (beta * power_heuristic(pdf_bsdf, sc.light_pmf[li] * 0.0f)) * Le;
+    }
+    return L;
+}
+
 // Ray::spawn_ray_to (base/ray.h:45-51)
 inline void spawn_ray_to(const P3fi &from, V3 n_from, const P3fi &to, V3 n_to, V3 *o, V3 *d) {
     V3 pf = offset_ray_origin(from, n_from, sub(p3fi_mid(to), p3fi_mid(from)));
@@ -1359,6 +1442,7 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
     uint64_t next_id = 0, n_rays = 0, n_shadow = 0, n_done = 0, round = 0;
     uint32_t gen_round = 0;  // the round whose ray cast first traces the paths gen() starts
     const bool replay = gio && gio->A;
+    if (replay && !sc.infinite.empty()) throw std::runtime_error("gradients with infinite lights");
     if (replay) {
         gio->d_coef.assign(size_t(d.num_materials) * 3, 0.0);
         gio->a.assign(d.num_lights, 0.0);
@@ -1505,6 +1589,8 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                 }
             }
             if (terminate) {
+                if (positive(p.beta) && !sc.infinite.empty())
+                    p.L = add_infinite(sc, p.L, p.beta, p.lam, p.depth, p.specular, p.pdf_bsdf);
                 FilmRec r;
                 r.pixel = p.pixel;
                 r.sample = p.sample;
@@ -1595,7 +1681,24 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                 float u = p.smp.get_1d();
                 float ul0, ul1;
                 p.smp.get_2d(&ul0, &ul1);
-                if (!sc.lights.empty()) {
+                if (!sc.lights.empty() && sc.lights[search_cdf(u, sc.select_cdf.data(), uint32_t(sc.lights.size()))].infinite) {
+                    const uint32_t li = search_cdf(u, sc.select_cdf.data(), uint32_t(sc.lights.size()));
+                    const V3 wi = sample_uniform_sphere(ul0, ul1);
+                    const S4 Le = light_L(sc, sc.lights[li], V3{0, 0, 0}, V3{0, 0, 0}, p.lam);
+                    if (positive(Le)) {
+                        const S4 f = bsdf_f(b, is.wo, wi) * abs_dot(wi, is.ns);
+                        if (positive(f)) {
+                            const float pdf_light = sc.light_pmf[li] * kUniformSpherePdf;
+                            const float wl = power_heuristic(pdf_light, bsdf_pdf(b, is.wo, wi));
+                            p.nee = true;
+                            p.nee_L = p.beta * (((Le * wl) * f) / pdf_light);
+                            // the light point: ctx.p() + wi * 2R, an exact point without a normal
+                            const P3fi pl = make_p3fi(add(p3fi_mid(ctx.pi), scale(sc.inf_dist, wi)), {0, 0, 0});
+                            spawn_ray_to(is.pi, is.n, pl, V3{0, 0, 0}, &p.so[0], &p.sd[0]);
+                            spawn_ray_to(pl, V3{0, 0, 0}, is.pi, is.n, &p.so[1], &p.sd[1]);
+                        }
+                    }
+                } else if (!sc.lights.empty()) {
                     uint32_t li = search_cdf(u, sc.select_cdf.data(), uint32_t(sc.lights.size()));
                     float pmf = sc.light_pmf[li];
                     const Light &L = sc.lights[li];
@@ -1777,7 +1880,10 @@ S4 mk_path_li(const MkScene &m, V3 ro, V3 rd, Lambda &lam, Sampler &smp, MkCount
     for (;;) {
         uint32_t prim;
         TriHit h;
-        if (!mk_closest(sc, ro, rd, &prim, &h, n)) break;  // no infinite lights
+        if (!mk_closest(sc, ro, rd, &prim, &h, n)) {
+            L = add_infinite(sc, L, beta, lam, depth, specular, pdf_bsdf);
+            break;
+        }
         const Tri &t = sc.tris[prim];
         const Isect is = make_isect(sc, prim, h, rd);
         if (t.light >= 0) {
@@ -1802,7 +1908,27 @@ S4 mk_path_li(const MkScene &m, V3 ro, V3 rd, Lambda &lam, Sampler &smp, MkCount
             const float u = smp.get_1d();
             float ul0, ul1;
             smp.get_2d(&ul0, &ul1);
-            if (!sc.lights.empty()) {
+            if (!sc.lights.empty() && sc.lights[search_cdf(u, sc.select_cdf.data(), uint32_t(sc.lights.size()))].infinite) {
+                const uint32_t li = search_cdf(u, sc.select_cdf.data(), uint32_t(sc.lights.size()));
+                const V3 wi = sample_uniform_sphere(ul0, ul1);
+                const S4 Le = light_L(sc, sc.lights[li], V3{0, 0, 0}, V3{0, 0, 0}, lam);
+                if (positive(Le)) {
+                    const S4 f = bsdf_f(b, is.wo, wi) * abs_dot(wi, is.ns);
+                    if (positive(f)) {
+                        const P3fi pl = make_p3fi(add(p3fi_mid(ctx.pi), scale(sc.inf_dist, wi)), {0, 0, 0});
+                        V3 so[2], sd[2];
+                        spawn_ray_to(is.pi, is.n, pl, V3{0, 0, 0}, &so[0], &sd[0]);
+                        spawn_ray_to(pl, V3{0, 0, 0}, is.pi, is.n, &so[1], &sd[1]);
+                        bool occ = false;
+                        for (int r = 0; r < 2 && !occ; ++r) occ = mk_any(sc, so[r], sd[r], 0.6f, n);
+                        if (!occ) {
+                            const float pdf_light = sc.light_pmf[li] * kUniformSpherePdf;
+                            const float wl = power_heuristic(pdf_light, bsdf_pdf(b, is.wo, wi));
+                            L = L + beta * (((Le * wl) * f) / pdf_light);
+                        }
+                    }
+                }
+            } else if (!sc.lights.empty()) {
                 const uint32_t li = search_cdf(u, sc.select_cdf.data(), uint32_t(sc.lights.size()));
                 const Light &Lg = sc.lights[li];
                 ShapeSample ss;

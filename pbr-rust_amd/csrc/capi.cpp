@@ -673,6 +673,9 @@ int pbrtm_scene_create(pbrtm_ctx *ctx, const pbrtm_scene_desc *desc, pbrtm_scene
     D.height = H.height;
     D.ao_scale = H.ao_scale;
     std::memcpy(D.sn_c, H.sn_c, sizeof(D.sn_c));
+    D.num_inf = H.num_inf;
+    std::memcpy(D.inf_idx, H.inf_idx, sizeof(D.inf_idx));
+    D.inf_dist = H.inf_dist;
     // The aggregate over the render-space triangles (SceneBuilder::preprocess, scene_builder.cu:829-838).
     const int rc = pbrtm_bvh_build(ctx, sc->tri.as<float>(), 3 * desc->num_tris, nullptr, desc->num_tris,
                                    PBRTM_DEVICE_PTRS, &sc->bvh, nullptr);
@@ -804,6 +807,8 @@ int pbrtm_render_grad(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params
     if (*why) return fail(ctx, code, why);
     if (params->flags & PBRTM_INTEGRATOR_MASK)
         return fail(ctx, PBRTM_ERR_UNSUPPORTED, "gradients render with the wavefront path integrator only");
+    if (sc->host.num_inf)
+        return fail(ctx, PBRTM_ERR_UNSUPPORTED, "gradients cover area lights only (the scene has an infinite light)");
     const pbrtm::pt::SceneHost &H = sc->host;
     const uint32_t ng = uint32_t(H.group_scale.size());
     if (ng > 255) return fail(ctx, PBRTM_ERR_UNSUPPORTED, "gradients support at most 255 emitters");

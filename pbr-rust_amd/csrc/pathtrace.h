@@ -27,8 +27,11 @@ struct DMat {  // 48 bytes
     uint32_t pad;
 };
 
-struct DLight {  // 32 bytes: a DiffuseAreaLight on one triangle
-    uint32_t tri;
+constexpr uint32_t kInfiniteLight = 0xFFFFFFFFu;  // DLight::tri of a UniformInfiniteLight
+constexpr uint32_t kMaxInfinite = 4;
+
+struct DLight {  // 32 bytes: a DiffuseAreaLight on one triangle, or a UniformInfiniteLight
+    uint32_t tri;     // kInfiniteLight: an infinite light (two_sided set; no "L": rgb_scale 1, c2 = inf)
     float scale;      // "scale" / to_photometric(L)
     float rgb_scale;  // RGBIlluminantSpectrum scale = 2 max(rgb)
     float c0, c1, c2; // sigmoid of rgb / rgb_scale
@@ -64,6 +67,8 @@ struct SceneDev {
     uint32_t width, height;
     float ao_scale;  // ambient occlusion: 1 / the colour space illuminant's to_photometric(Y)
     float sn_c[9];   // surface normal: sigmoid coefficients of RGBAlbedoSpectrum(0.01 e_c), c = r, g, b
+    uint32_t num_inf, inf_idx[kMaxInfinite];  // the infinite lights' indices into lights, in light order
+    float inf_dist;  // 2 x the scene's bounding-sphere radius (UniformInfiniteLight::sample_li)
 };
 
 // Integrators render() runs (SceneBuilder::build_integrator, scene_builder.cu:289-313).
@@ -109,6 +114,8 @@ struct SceneHost {
     uint32_t width, height;
     float ao_scale;
     float sn_c[9];
+    uint32_t num_inf, inf_idx[kMaxInfinite];
+    float inf_dist;
     std::vector<float> mat_rgb;      // per material [3]: the diffuse albedo RGB (gradients)
     std::vector<float> group_scale;  // per emitter: the scene's "scale" parameter (gradients)
 };

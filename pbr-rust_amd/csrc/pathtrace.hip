@@ -525,6 +525,17 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
         base = __shfl(base, leader, kWave);
     }
     if (terminate) {
+        // Infinite lights (wavefront_path.cu:140-160): every path that ends with a
+        // positive beta, escaped or not; the megakernel path integrator only on an
+        // escape (megakernel_path.cu:58-75). pdf_li allowing an incomplete pdf is 0.
+        if (MODE <= kIntMegakernelPath && S.num_inf && (MODE == kIntPath ? s4pos(beta) : prim == kNoHit)) {
+            for (uint32_t k = 0; k < S.num_inf; ++k) {
+                const uint32_t li = S.inf_idx[k];
+                const S4 Le = light_L(S, S.lights[li], f3{0.f, 0.f, 0.f}, f3{0.f, 0.f, 0.f}, lam);
+                if (depth == 0 || (fl & kFlagSpecular)) L = s4add(L, s4mul(beta, Le));
+                else L = s4add(L, s4mul(s4scale(beta, power_heuristic(scal.y, S.light_pmf[li] * 0.0f)), Le));
+            }
+        }
         const float3 rgb = sensor_rgb(S, L, lam, lpdf);
         st.rec[pid] = make_float4(rgb.x, rgb.y, rgb.z, scal.x);
         // the megakernel integrators add each pixel's samples in sample order
@@ -688,15 +699,32 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
             const uint32_t li = search_cdf(u, S.light_cdf, S.num_lights);
             const float pmf = S.light_pmf[li];
             const DLight Lg = S.lights[li];
-            const Tri lt = load_tri(S, Lg.tri);
-            ShapeSample ss;
-            const bool ok = tri_sample(lt, cpi, ns, ul0, ul1, &ss);
             const f3 cpt = p3i_mid(cpi);
-            const f3 sp = p3i_mid(ss.pi);
-            if (ok && !(ss.pdf == 0 || sqlen3(sub3(sp, cpt)) == 0)) {
-                const f3 wi = norm3(sub3(sp, cpt));
-                const S4 Le = light_L(S, Lg, ss.n, neg3(wi), lam);
-                if (s4pos(Le) && ss.pdf != 0) {
+            ShapeSample ss;
+            f3 wi = {0.f, 0.f, 0.f};
+            S4 Le = s4(0.f);
+            bool ok;
+            if (Lg.tri == kInfiniteLight) {
+                // UniformInfiniteLight::sample_li (uniform_infinite_light.cu:51-60): the
+                // light point is ctx.p() + wi 2R, exact and without a normal
+                wi = uniform_sphere(ul0, ul1);
+                ss.pdf = kUniformSpherePdf;
+                ss.pi = p3i_exact(add3(cpt, mul3(S.inf_dist, wi)));
+                ss.n = f3{0.f, 0.f, 0.f};
+                Le = light_L(S, Lg, ss.n, ss.n, lam);
+                ok = s4pos(Le);
+            } else {
+                ok = tri_sample(load_tri(S, Lg.tri), cpi, ns, ul0, ul1, &ss);
+                const f3 sp = p3i_mid(ss.pi);
+                ok = ok && !(ss.pdf == 0 || sqlen3(sub3(sp, cpt)) == 0);
+                if (ok) {
+                    wi = norm3(sub3(sp, cpt));
+                    Le = light_L(S, Lg, ss.n, neg3(wi), lam);
+                    ok = s4pos(Le) && ss.pdf != 0;
+                }
+            }
+            {
+                if (ok) {
                     // BSDF::f and BSDF::pdf (bsdf.cu:44-93) in the shading frame
                     const f3 wil = {dot3(wi, fx), dot3(wi, fy), dot3(wi, ns)};
                     const f3 wol = {dot3(wo_n, fx), dot3(wo_n, fy), dot3(wo_n, ns)};

@@ -7,7 +7,8 @@
 // PixelFilter "mitchell", WorldBegin, AttributeBegin/End, Translate, Scale,
 // Rotate, Transform, ConcatTransform, Identity, CoordSysTransform, ReverseOrientation,
 // Material / MakeNamedMaterial / NamedMaterial ("diffuse", "conductor",
-// "dielectric"), AreaLightSource "diffuse", Shape "trianglemesh", Include.
+// "dielectric"), AreaLightSource "diffuse", LightSource "infinite" (uniform),
+// Shape "trianglemesh", Include.
 // Anything else that changes the image is reported as unsupported rather than
 // silently approximated.
 #include <cctype>
@@ -180,6 +181,7 @@ struct pbrtm_pbrt_file {
     std::vector<uint8_t> tri_reverse;
     std::vector<pbrtm_material> materials;
     std::vector<pbrtm_area_light> lights;
+    std::vector<pbrtm_infinite_light> infinite_lights;
     uint32_t spp = 4, max_depth = 5, integrator = PBRTM_INTEGRATOR_PATH;
     std::string filename, error;
 };
@@ -218,6 +220,8 @@ class Builder {
         d.num_xforms = uint32_t(F.xforms.size() / 16);
         d.xforms = F.xforms.empty() ? nullptr : F.xforms.data();
         d.tri_xform = F.tri_xform.data();
+        d.num_infinite_lights = uint32_t(F.infinite_lights.size());
+        d.infinite_lights = F.infinite_lights.empty() ? nullptr : F.infinite_lights.data();
         const int s = int(std::sqrt(double(spp_asked)));  // stratified: spp rounded to a square (scene_builder.cu:283-287)
         F.spp = uint32_t(s * s);
         F.max_depth = uint32_t(max_depth);
@@ -550,6 +554,27 @@ class Builder {
                 P.check_used("AreaLightSource \"diffuse\"");
                 gs.has_light = true;
                 gs.light = L;
+            } else if (k == "LightSource") {
+                // parse_light_source (scene_builder.cu:480-495) -> Light::create: only the
+                // uniform "infinite" light (lights/uniform_infinite_light.cu:4-30)
+                const std::string type = read_str(T, i, "LightSource");
+                Params P = read_params(T, i);
+                if (type != "infinite") throw std::runtime_error(where(t) + "LightSource \"" + type + "\" is not supported");
+                if (P.find("filename")) throw std::runtime_error(where(t) + "image infinite lights are not supported");
+                pbrtm_infinite_light L{};
+                if (Param *l = P.find("L", "rgb")) {
+                    if (l->nums.size() != 3) throw std::runtime_error(where(t) + "rgb L needs 3 values");
+                    for (int c = 0; c < 3; ++c) L.rgb[c] = float(l->nums[c]);
+                    L.has_rgb = 1;
+                }
+                L.scale = P.get_float("scale", 1.f);
+                L.illuminance = P.get_float("illuminance", -1.f);
+                uint32_t emissive = 0;  // area lights are created per triangle as shapes are parsed
+                for (int32_t v : F.tri_light) emissive += v >= 0 ? 1u : 0u;
+                L.light_index = emissive;
+                if (F.infinite_lights.size() >= 4) throw std::runtime_error(where(t) + "at most 4 infinite lights");
+                P.check_used("LightSource \"infinite\"");
+                F.infinite_lights.push_back(L);
             } else if (k == "Shape") {
                 const std::string type = read_str(T, i, "Shape");
                 Params P = read_params(T, i);

@@ -291,6 +291,15 @@ PBRTM_HD f3 cosine_hemisphere(float u0, float u1) {
     return {dx, dy, sqrtf(smax(0.0f, zz))};
 }
 
+// util/sampling.h:203-216 (UniformInfiniteLight::sample_li)
+PBRTM_HD f3 uniform_sphere(float u0, float u1) {
+    const float z = 1 - 2 * u0;
+    const float r = sqrtf(smax(0.0f, 1 - z * z));
+    const float phi = 2 * kPi * u1;
+    return {r * tcos(phi), r * tsin(phi), z};
+}
+constexpr float kUniformSpherePdf = float(1.0 / (4.0 * double(kPi)));
+
 // ------------------------------------------------------------ spherical triangles
 // shapes/triangle.h:25-27
 PBRTM_HD float spherical_triangle_area(f3 a, f3 b, f3 c) {

@@ -8,6 +8,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -763,6 +764,66 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
             }
             S.tri_info[i] = make_uint2(d.tri_material[i], light | (flip ? 0x80000000u : 0u));
         }
+        // Infinite lights join the list at their place in file order (the reference pushes
+        // LightSource lights and per-triangle area lights as it parses, scene_builder.cu:494,
+        // :637); UniformInfiniteLight::create (uniform_infinite_light.cu:4-30).
+        S.num_inf = 0;
+        S.inf_dist = 0.f;
+        if (d.num_infinite_lights) {
+            if (!d.infinite_lights) return "infinite lights missing";
+            if (d.num_infinite_lights > kMaxInfinite) return "at most 4 infinite lights";
+            std::vector<DLight> area = std::move(S.lights);
+            S.lights.clear();
+            std::vector<uint32_t> remap(area.size());
+            for (size_t a = 0; a <= area.size(); ++a) {
+                for (uint32_t q = 0; q < d.num_infinite_lights; ++q) {
+                    const pbrtm_infinite_light &I = d.infinite_lights[q];
+                    if (std::min<size_t>(I.light_index, area.size()) != a) continue;
+                    DLight dl{};
+                    dl.tri = kInfiniteLight;
+                    dl.two_sided = 1u;
+                    dl.group = 0xFFFFFFFFu;
+                    float scale = I.scale / G.d65.dot(G.cie[1]);
+                    if (I.illuminance > 0) scale *= I.illuminance / kPiF;
+                    dl.scale = scale;
+                    if (I.has_rgb) {
+                        dl.rgb_scale = float(2.0 * double(std::max(std::max(I.rgb[0], I.rgb[1]), I.rgb[2])));
+                        float c[3] = {0, 0, 0};
+                        if (dl.rgb_scale > 0.0f)
+                            for (int w = 0; w < 3; ++w) c[w] = std::max(0.f, I.rgb[w] / dl.rgb_scale);
+                        auto k = G.table.lookup(c);
+                        dl.c0 = k[0], dl.c1 = k[1], dl.c2 = k[2];
+                    } else {  // the colour space illuminant itself: a sigmoid that is 1 everywhere
+                        dl.rgb_scale = 1.f;
+                        dl.c0 = dl.c1 = 0.f;
+                        dl.c2 = std::numeric_limits<float>::infinity();
+                    }
+                    S.inf_idx[S.num_inf++] = uint32_t(S.lights.size());
+                    S.lights.push_back(dl);
+                }
+                if (a < area.size()) {
+                    remap[a] = uint32_t(S.lights.size());
+                    S.lights.push_back(area[a]);
+                }
+            }
+            for (uint64_t i = 0; i < n; ++i) {
+                const uint32_t l = S.tri_info[i].y & 0x7FFFFFFFu;
+                if (l) S.tri_info[i].y = (S.tri_info[i].y & 0x80000000u) | (remap[l - 1] + 1);
+            }
+            // Bounds3::bounding_sphere of the aggregate's bounds (uniform_infinite_light.h:17-19)
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (size_t i = 0; i < S.tri.size(); i += 3)
+                for (int c = 0; c < 3; ++c) lo[c] = std::min(lo[c], S.tri[i + c]), hi[c] = std::max(hi[c], S.tri[i + c]);
+            float ctr[3];
+            bool inside = true;
+            for (int c = 0; c < 3; ++c) {
+                ctr[c] = (lo[c] + hi[c]) / 2;
+                inside = inside && ctr[c] >= lo[c] && ctr[c] <= hi[c];
+            }
+            const float dx = ctr[0] - hi[0], dy = ctr[1] - hi[1], dz = ctr[2] - hi[2];
+            const float radius = inside ? std::sqrt(dx * dx + dy * dy + dz * dz) : 0.0f;
+            S.inf_dist = 2 * radius;
+        }
         // power light sampler (power_light_sampler.cu:7-45, distribution_1d.h:46-70)
         S.light_pmf.clear();
         S.light_cdf.clear();
@@ -777,16 +838,23 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
             }
             std::vector<float> w;
             for (const DLight &l : S.lights) {
-                const float *t = &S.tri[9 * size_t(l.tri)];
-                const V p0{t[0], t[1], t[2]}, p1{t[3], t[4], t[5]}, p2{t[6], t[7], t[8]};
-                const float area = vlen(vcross(vsub(p1, p0), vsub(p2, p0))) * 0.5f;
-                const float k = kPiF * float(l.two_sided ? 2 : 1) * area;
+                const bool inf = l.tri == kInfiniteLight;
+                float k = 0.f;
+                if (!inf) {
+                    const float *t = &S.tri[9 * size_t(l.tri)];
+                    const V p0{t[0], t[1], t[2]}, p1{t[3], t[4], t[5]}, p2{t[6], t[7], t[8]};
+                    const float area = vlen(vcross(vsub(p1, p0), vsub(p2, p0))) * 0.5f;
+                    k = kPiF * float(l.two_sided ? 2 : 1) * area;
+                } else {  // UniformInfiniteLight::phi: 4 pi^2 R^2 scale L(lambda)
+                    const float R = S.inf_dist / 2;
+                    k = 4 * kPiF * kPiF * (R * R) * l.scale;
+                }
                 float sum = 0;
                 for (int i = 0; i < 4; ++i) {
                     const float poly = std::fma(lam[i], std::fma(lam[i], l.c0, l.c1), l.c2);
                     const float sg = std::isinf(poly) ? (poly > 0 ? 1.f : 0.f) : .5f + poly / (2 * std::sqrt(1 + poly * poly));
-                    const float L = (G.d65.at(lam[i]) * (l.rgb_scale * sg)) * l.scale;
-                    const float phi = L * k;
+                    const float phi = inf ? (G.d65.at(lam[i]) * (l.rgb_scale * sg)) * k
+                                          : ((G.d65.at(lam[i]) * (l.rgb_scale * sg)) * l.scale) * k;
                     sum += pdf[i] == 0.0f ? 0.0f : phi / pdf[i];
                 }
                 w.push_back(sum / 4);

@@ -64,13 +64,19 @@ class AreaLight(ctypes.Structure):
     _fields_ = [("rgb", c_float * 3), ("scale", c_float), ("two_sided", c_uint32)]
 
 
+class InfiniteLight(ctypes.Structure):
+    _fields_ = [("rgb", c_float * 3), ("has_rgb", c_uint32), ("scale", c_float), ("illuminance", c_float),
+                ("light_index", c_uint32)]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("eye", c_float * 3), ("look", c_float * 3), ("up", c_float * 3), ("fov", c_float),
                 ("width", c_uint32), ("height", c_uint32), ("num_tris", c_uint64), ("tri", c_void_p),
                 ("tri_material", c_void_p), ("tri_light", c_void_p), ("tri_reverse", c_void_p),
                 ("num_materials", c_uint32), ("materials", POINTER(Material)), ("num_lights", c_uint32),
                 ("lights", POINTER(AreaLight)), ("num_xforms", c_uint32), ("xforms", c_void_p),
-                ("tri_xform", c_void_p)]
+                ("tri_xform", c_void_p), ("num_infinite_lights", c_uint32),
+                ("infinite_lights", POINTER(InfiniteLight))]
 
 
 class RenderParams(ctypes.Structure):
@@ -412,7 +418,20 @@ def _scene_desc_tail(sc, d, keep):
     d.num_xforms = len(xf)
     d.xforms = _ptr(xf) if len(xf) else None
     d.tri_xform = _ptr(txf)
-    return d, keep + (xf, txf)
+    # "infinite_lights": [{"rgb": (r, g, b) or None (the colour space illuminant), "scale",
+    # "illuminance", "light_index": emissive triangles created before it}]
+    inf = sc.get("infinite_lights", [])
+    infl = (InfiniteLight * max(1, len(inf)))()
+    for i, l in enumerate(inf):
+        if l.get("rgb") is not None:
+            infl[i].rgb[:] = list(l["rgb"])
+            infl[i].has_rgb = 1
+        infl[i].scale = l.get("scale", 1.0)
+        infl[i].illuminance = l.get("illuminance", -1.0)
+        infl[i].light_index = l.get("light_index", 0)
+    d.num_infinite_lights = len(inf)
+    d.infinite_lights = infl
+    return d, keep + (xf, txf, infl)
 
 
 class PbrtFile:
@@ -480,6 +499,11 @@ class PbrtFile:
         if d.num_xforms:
             out["xforms"] = arr(d.xforms, c_float, 16 * d.num_xforms, np.float32).reshape(-1, 4, 4)
             out["tri_xform"] = arr(d.tri_xform, c_uint32, n, np.uint32)
+        if d.num_infinite_lights:
+            out["infinite_lights"] = [
+                {"rgb": tuple(l.rgb) if l.has_rgb else None, "scale": l.scale, "illuminance": l.illuminance,
+                 "light_index": l.light_index}
+                for l in (d.infinite_lights[i] for i in range(d.num_infinite_lights))]
         return out
 
 

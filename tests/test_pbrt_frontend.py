@@ -82,7 +82,8 @@ def test_named_materials_lights_and_defaults(tmp_path):
 @pytest.mark.parametrize("body,msg", [
     ('Shape "sphere" "float radius" 1\n', "sphere"),
     ('Material "coateddiffuse"\n', "coateddiffuse"),
-    ('LightSource "infinite"\n', "LightSource"),
+    ('LightSource "point"\n', "LightSource"),
+    ('LightSource "infinite" "string filename" "sky.exr"\n', "image infinite"),
     ('Shape "trianglemesh" "integer indices" [0 1 2] "point3 P" [0 0 0 1 0 0 0 1 0] "normal N" [0 0 1 0 0 1 0 0 1]\n',
      "normal N"),
     ('Shape "trianglemesh" "integer indices" [0 1 5] "point3 P" [0 0 0 1 0 0 0 1 0]\n', "out of range"),
@@ -122,3 +123,16 @@ def test_unsupported_integrators(tmp_path, name):
     if name:
         with pytest.raises(pbrtm.PbrtmError, match="integrator"):
             pbrtm.PbrtFile(write(tmp_path, TRI, f'Integrator "{name}"\nFilm "rgb"\nWorldBegin\n'))
+
+
+def test_infinite_light_place_in_light_order(tmp_path):
+    """LightSource "infinite" joins the light list where it appears: after the
+    emissive triangles created before it (scene_builder.cu:494, :637)."""
+    body = ('AreaLightSource "diffuse" "rgb L" [1 1 1]\n' + TRI +
+            'LightSource "infinite" "rgb L" [0.2 0.4 0.9] "float scale" 3\n' +
+            'LightSource "infinite"\n')
+    d = pbrtm.PbrtFile(write(tmp_path, body)).to_dict()
+    inf = d["infinite_lights"]
+    assert [l["light_index"] for l in inf] == [1, 1]
+    assert inf[0]["rgb"] == tuple(np.float32([0.2, 0.4, 0.9])) and inf[0]["scale"] == 3 and inf[0]["illuminance"] < 0
+    assert inf[1]["rgb"] is None and inf[1]["scale"] == 1
