@@ -104,6 +104,13 @@ def pathtracer(args, ctx, pbrtm):
             _, _, st = scene.render(spp=spp, max_depth=depth)
             runs.append(st)
         wall = (time.perf_counter() - t0) / args.pt_steps
+        # the same render with a 2^22-slot pool: each pixel then sums the same samples
+        # in another round order, so it is reported beside the reference-pool figure,
+        # not as it
+        big = 4 << 20
+        scene.render(spp=spp, max_depth=depth, pool_size=big)
+        ms_big = float(np.mean([scene.render(spp=spp, max_depth=depth, pool_size=big)[2]["ms_total"]
+                                for _ in range(args.pt_steps)]))
     finally:
         scene.free()
     ms = float(np.mean([r["ms_total"] for r in runs]))
@@ -120,6 +127,8 @@ def pathtracer(args, ctx, pbrtm):
         "rounds": last["rounds"], "rays": last["rays"], "shadow_rays": last["shadow_rays"],
         "ray_order": {"closest_sorted": bool(last["ray_order"] & 1), "shadow_sorted": bool(last["ray_order"] & 2)},
         "mrays_per_s": round((last["rays"] + last["shadow_rays"]) / (ms * 1e3), 1),
+        "pool_4m": {"value": round(W * H * spp / (ms_big * 1e3), 2), "ms_per_image": round(ms_big, 2),
+                    "note": "2^22 path slots instead of the reference's 2^21: each pixel sums the same samples in another order"},
         "config": {"workload": "sphere grid 32x32 UV spheres, 2,097,218 tris, 64 emitters", "width": W,
                    "height": H, "spp": spp, "max_depth": depth, "seed": 0xC0FFEE, "dtype": "f32"},
     }

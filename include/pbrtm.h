@@ -205,7 +205,10 @@ typedef struct pbrtm_scene_desc {
 typedef struct pbrtm_render_params {
     uint32_t spp;       /* samples per pixel, a square (stratified) */
     uint32_t max_depth; /* Integrator "path" "integer maxdepth" (default 5) */
-    uint32_t pool_size; /* path slots in flight; 0 = 2^21 as the reference (wavefront_path.cu:18) */
+    uint32_t pool_size; /* path slots in flight; 0 = 2^21 as the reference (wavefront_path.cu:18) for
+                           the wavefront integrator, whose film order depends on it; for the megakernel
+                           integrators (film in sample order) 2^22 (megakernelpath) or 2^23 slots,
+                           at most the render's path count */
     uint32_t flags;     /* PBRTM_DEVICE_PTRS: film_rgb / film_aux are device pointers;
                            one PBRTM_INTEGRATOR_* value (default: the wavefront path integrator) */
 } pbrtm_render_params;
