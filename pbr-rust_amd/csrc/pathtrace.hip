@@ -992,6 +992,7 @@ __global__ __launch_bounds__(kFilmWaves * kWave) void film_kernel(SceneDev S, co
                                                                   float *film_rgb, float *film_aux) {
     __shared__ uint32_t s_key[kFilmWaves][kMaxSpp];
     __shared__ uint16_t s_order[kFilmWaves][kMaxSpp];
+    __shared__ float4 s_rec[kFilmWaves][kWave];
     const uint32_t wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const uint32_t wh = S.width * S.height;
     const uint32_t px = blockIdx.x * kFilmWaves + wv;
@@ -1008,16 +1009,26 @@ __global__ __launch_bounds__(kFilmWaves * kWave) void film_kernel(SceneDev S, co
         s_order[wv][rank] = uint16_t(s);
     }
     __builtin_amdgcn_wave_barrier();
-    if (lane != 0) return;
+    // The records are gathered 64 at a time by all lanes (each from its own sample
+    // row), then lane 0 adds them in rank order from LDS.
     float sum[3] = {0.f, 0.f, 0.f}, wsum = 0.f;
-    for (uint32_t j = 0; j < spp; ++j) {
-        const float4 r = rec[size_t(s_order[wv][j]) * wh + px];
-        if (r.w == 0) continue;
-        sum[0] += r.w * r.x;
-        sum[1] += r.w * r.y;
-        sum[2] += r.w * r.z;
-        wsum += r.w;
+    for (uint32_t c = 0; c < spp; c += kWave) {
+        if (c + lane < spp) s_rec[wv][lane] = rec[size_t(s_order[wv][c + lane]) * wh + px];
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+            const uint32_t m = spp - c < kWave ? spp - c : kWave;
+            for (uint32_t j = 0; j < m; ++j) {
+                const float4 r = s_rec[wv][j];
+                if (r.w == 0) continue;
+                sum[0] += r.w * r.x;
+                sum[1] += r.w * r.y;
+                sum[2] += r.w * r.z;
+                wsum += r.w;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
     }
+    if (lane != 0) return;
     float rgb[3] = {sum[0], sum[1], sum[2]};
     if (wsum != 0)
         for (int c = 0; c < 3; ++c) rgb[c] /= wsum;
