@@ -341,6 +341,10 @@ void pbrtm_pbrt_free(pbrtm_pbrt_file *file);
  * written as an RGBA8 PNG. pbrtm_srgb8 returns the 8-bit pixels ([n][4]). */
 int pbrtm_srgb8(const float *rgb, uint64_t n, uint8_t *rgba_out);
 int pbrtm_write_png(const float *rgb, uint32_t width, uint32_t height, const char *path);
+/* The same linear output-space film, unencoded, as a little-endian PFM (no
+ * counterpart in the reference, whose films leave only as 8-bit PNG): for
+ * comparing images at full precision. */
+int pbrtm_write_pfm(const float *rgb, uint32_t width, uint32_t height, const char *path);
 
 #ifdef __cplusplus
 }

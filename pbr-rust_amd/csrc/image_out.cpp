@@ -96,4 +96,16 @@ int pbrtm_write_png(const float *rgb, uint32_t width, uint32_t height, const cha
     return (w == png.size() && rc == 0) ? PBRTM_OK : PBRTM_ERR_INVALID;
 }
 
+int pbrtm_write_pfm(const float *rgb, uint32_t width, uint32_t height, const char *path) {
+    if (!rgb || !path || width == 0 || height == 0) return PBRTM_ERR_INVALID;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return PBRTM_ERR_INVALID;
+    // "PF": three channels; a negative scale marks little-endian floats; rows bottom-up
+    bool ok = std::fprintf(f, "PF\n%u %u\n-1.0\n", width, height) > 0;
+    for (uint32_t y = height; ok && y-- > 0;)
+        ok = std::fwrite(rgb + size_t(y) * width * 3, sizeof(float), size_t(width) * 3, f) == size_t(width) * 3;
+    const int rc = std::fclose(f);
+    return (ok && rc == 0) ? PBRTM_OK : PBRTM_ERR_INVALID;
+}
+
 }  // extern "C"

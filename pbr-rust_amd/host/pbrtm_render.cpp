@@ -4,7 +4,8 @@
 // [--outfile out.png] scene.pbrt
 // Parses the file, builds the scene on device 0, renders with the file's
 // Integrator (or --integrator, which overrides it as in scene_builder.cu:366-371)
-// and writes the PNG (the Film filename unless --outfile).
+// and writes the PNG (the Film filename unless --outfile; a .pfm name writes the
+// linear film as floats).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -75,7 +76,10 @@ int main(int argc, char **argv) {
         rc = pbrtm_render(ctx, scene, &params, rgb.data(), nullptr, &st);
     }
     if (rc != PBRTM_OK) std::fprintf(stderr, "error: %s\n", pbrtm_last_error(ctx));
-    if (rc == PBRTM_OK) rc = pbrtm_write_png(rgb.data(), desc->width, desc->height, outfile.c_str());
+    const bool pfm = outfile.size() > 4 && outfile.compare(outfile.size() - 4, 4, ".pfm") == 0;
+    if (rc == PBRTM_OK)
+        rc = pfm ? pbrtm_write_pfm(rgb.data(), desc->width, desc->height, outfile.c_str())
+                 : pbrtm_write_png(rgb.data(), desc->width, desc->height, outfile.c_str());
     if (rc == PBRTM_OK)
         std::printf("%ux%u, %u spp, maxdepth %u: %.2f ms (%.1f Msamples/s), %llu rays, %llu shadow rays -> %s\n",
                     desc->width, desc->height, params.spp, params.max_depth, st.ms_total,

@@ -35,7 +35,7 @@ EXPORTS = (
     "pbrtm_gen_camera_rays", "pbrtm_scene_create", "pbrtm_scene_free", "pbrtm_render", "pbrtm_rgb_to_sigmoid",
     "pbrtm_srgb_matrices", "pbrtm_srgb8", "pbrtm_write_png", "pbrtm_pbrt_parse", "pbrtm_pbrt_desc",
     "pbrtm_pbrt_render_params", "pbrtm_pbrt_output", "pbrtm_pbrt_free", "pbrtm_render_batch",
-    "pbrtm_render_grad", "pbrtm_integrator_from_name",
+    "pbrtm_render_grad", "pbrtm_integrator_from_name", "pbrtm_write_pfm",
 )
 BATCH_NO_GRAPH = 2
 # pbrtm_render_params.flags: the integrator
@@ -163,6 +163,7 @@ def load(path: str = LIB_PATH):
         "pbrtm_srgb_matrices": (c_int, [p, p, p]),
         "pbrtm_srgb8": (c_int, [p, c_uint64, p]),
         "pbrtm_write_png": (c_int, [p, c_uint32, c_uint32, c_char_p]),
+        "pbrtm_write_pfm": (c_int, [p, c_uint32, c_uint32, c_char_p]),
         "pbrtm_pbrt_parse": (c_int, [c_char_p, POINTER(p), c_char_p, c_uint64]),
         "pbrtm_pbrt_desc": (POINTER(SceneDesc), [p]),
         "pbrtm_pbrt_render_params": (c_int, [p, POINTER(RenderParams)]),
@@ -329,6 +330,16 @@ def srgb8(rgb) -> np.ndarray:
     if rc != OK:
         raise PbrtmError(rc, "srgb8")
     return out
+
+
+def write_pfm(rgb, path):
+    """Write a linear RGB film [h][w][3] as a little-endian float PFM (host only)."""
+    lib = load()
+    a = np.ascontiguousarray(rgb, np.float32)
+    h, w = a.shape[:2]
+    rc = lib.pbrtm_write_pfm(_ptr(a), w, h, os.fsencode(path))
+    if rc != OK:
+        raise PbrtmError(rc, f"write_pfm {path}")
 
 
 def write_png(rgb, path):

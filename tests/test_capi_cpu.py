@@ -6,6 +6,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 import oracle as O
 import pbrtm
@@ -78,3 +79,20 @@ def test_cpp_interface_compiles_and_fails_cleanly_without_gpu():
     if pbrtm.device_count() == 0:
         out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
         assert out.returncode == 1 and "pbrtm_create failed" in out.stderr
+
+
+def test_pfm_round_trip(tmp_path):
+    """pbrtm_write_pfm: header "PF", width height, scale -1 (little-endian), rows
+    bottom-up; the floats come back bit for bit."""
+    import numpy as np
+    rgb = np.random.default_rng(7).standard_normal((5, 7, 3)).astype(np.float32)
+    rgb[0, 0] = [np.inf, -0.0, 1e-42]
+    p = tmp_path / "f.pfm"
+    pbrtm.write_pfm(rgb, str(p))
+    data = p.read_bytes()
+    head = b"PF\n7 5\n-1.0\n"
+    assert data[:len(head)] == head and len(data) == len(head) + rgb.size * 4
+    back = np.frombuffer(data[len(head):], "<f4").reshape(5, 7, 3)[::-1]
+    assert np.array_equal(back.view(np.uint32), rgb.view(np.uint32))
+    with pytest.raises(pbrtm.PbrtmError):
+        pbrtm.write_pfm(rgb, str(tmp_path / "missing" / "f.pfm"))
