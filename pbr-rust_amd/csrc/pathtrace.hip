@@ -934,16 +934,29 @@ __global__ __launch_bounds__(kSchedBlock) void schedule_kernel(const JobDev *job
     uint64_t r = 0;
     uint32_t my_max = 0;
     for (;;) {
-        for (uint64_t k = lo + threadIdx.x; k < hi; k += kSchedBlock) {
-            const uint32_t len = rr[k];
-            if (len > lmax) {
+        // wave-uniform trips; the finishing rounds of a wave's 64 paths are counted
+        // per distinct value (at most max_depth + 1 of them), one LDS atomic each
+        for (uint64_t b = lo; b < hi; b += kSchedBlock) {
+            const uint64_t k = b + threadIdx.x;
+            bool ok = k < hi;
+            const uint32_t len = ok ? rr[k] : 0u;
+            if (ok && len > lmax) {
                 s_bad = 1;
-                continue;
+                ok = false;
             }
             const uint32_t f = uint32_t(r) + len;
-            rr[k] = f;
-            my_max = f > my_max ? f : my_max;
-            atomicAdd(&ring[f % kRing], 1u);
+            if (ok) {
+                rr[k] = f;
+                my_max = f > my_max ? f : my_max;
+            }
+            uint64_t pending = __ballot(ok);
+            while (pending) {
+                const int leader = __ffsll((unsigned long long)pending) - 1;
+                const uint32_t fl = __shfl(f, leader, kWave);
+                const uint64_t same = __ballot(ok && f == fl);
+                if (lane_id() == uint32_t(leader)) atomicAdd(&ring[fl % kRing], uint32_t(__popcll(same)));
+                pending &= ~same;
+            }
         }
         __syncthreads();
         const uint32_t t = ring[r % kRing];
