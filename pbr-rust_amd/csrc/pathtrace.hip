@@ -1199,14 +1199,24 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         P.gd = *gd;
     }
     const uint32_t grid = (pool + kBlock - 1) / kBlock;
-    hipEvent_t ev[6];
+    // ev[0]: start, ev[1]: end of the rounds, ev[2]: end; per round slot (rounds
+    // alternate between two): before the closest-hit trace, after it, after the
+    // shadow trace, after shading, after the counter read-back
+    constexpr int kEv = 3 + 2 * 5;
+    hipEvent_t ev[kEv];
     for (auto &x : ev) PT_CK(hipEventCreate(&x));
+    uint32_t *hc = nullptr;  // pinned: the counters of two rounds
+    PT_CK(hipHostMalloc(reinterpret_cast<void **>(&hc), 2 * 16 * sizeof(uint32_t), hipHostMallocDefault));
     struct EvGuard {
         hipEvent_t *e;
+        uint32_t *h;
+        hipStream_t s;
         ~EvGuard() {
-            for (int k = 0; k < 6; ++k) hipEventDestroy(e[k]);
+            hipStreamSynchronize(s);  // no read-back may still target h
+            for (int k = 0; k < kEv; ++k) hipEventDestroy(e[k]);
+            hipHostFree(h);
         }
-    } guard{ev};
+    } guard{ev, hc, stream};
     PT_CK(hipMemsetAsync(st.counters, 0, 64, stream));
     uint32_t first = uint32_t(std::min<uint64_t>(total, pool));
     PT_CK(hipMemcpyAsync(st.counters + kCntNextId, &first, 4, hipMemcpyHostToDevice, stream));
@@ -1216,19 +1226,25 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
     else init_pool_kernel<false><<<grid, kBlock, 0, stream>>>(S, st, P);
     PT_CK(hipGetLastError());
     RenderStats rs;
-    uint32_t host_cnt[16];
     float t_trace = 0, t_shadow = 0, t_shade = 0;
     struct {
         bool sort_closest = true, sort_shadow = true;
         double ms[2][2] = {{0, 0}, {0, 0}};
         uint64_t rays[2][2] = {{0, 0}, {0, 0}};
     } tune;
-    for (;;) {
+    // One round: both traces and the shading kernel, then the counters to the host.
+    // Round r + 1 is enqueued before round r's counters are read, so the device
+    // does not wait for the host between rounds; when round r turns out to be the
+    // last, round r + 1 finds no live path and does nothing.
+    auto enqueue = [&](uint32_t round) -> hipError_t {
+        hipError_t e;
+        hipEvent_t *E = ev + 3 + 5 * (round & 1);
+        P.round = round;
         PT_CK(hipMemsetAsync(st.counters + kCntQueue, 0, 8, stream));
-        PT_CK(hipEventRecord(ev[1], stream));
-        const bool tuning = P.round >= kTuneFirst && P.round < kTuneFirst + 4;
-        const bool sort_c = tuning ? ((P.round - kTuneFirst) % 2 == 0) : tune.sort_closest;
-        const bool sort_s = tuning ? ((P.round - kTuneFirst) % 2 == 0) : tune.sort_shadow;
+        PT_CK(hipEventRecord(E[0], stream));
+        const bool tuning = round >= kTuneFirst && round < kTuneFirst + 4;
+        const bool sort_c = tuning ? ((round - kTuneFirst) % 2 == 0) : tune.sort_closest;
+        const bool sort_s = tuning ? ((round - kTuneFirst) % 2 == 0) : tune.sort_shadow;
         if (sort_c) {  // closest-hit rays in (octant, origin Morton code) order
             uint32_t *q = nullptr;
             PT_CK(sort_rays(W, reinterpret_cast<const float *>(st.rays), pool, pp.prim.as<uint32_t>(),
@@ -1242,7 +1258,7 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
                              pp.thit.as<float>(), pp.bary.as<float>(), err, nullptr, pp.queue.as<uint32_t>(),
                              st.counters + kCntQueue, stream));
         }
-        PT_CK(hipEventRecord(ev[2], stream));
+        PT_CK(hipEventRecord(E[1], stream));
         if (spill_need) W.spill = pp.spill.as<uint2>();
         if (sort_s) {
             uint32_t *q = nullptr;
@@ -1256,7 +1272,7 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
                              nullptr, nullptr, err, nullptr, pp.squeue.as<uint32_t>(), st.counters + kCntSQueue,
                              stream));
         }
-        PT_CK(hipEventRecord(ev[3], stream));
+        PT_CK(hipEventRecord(E[2], stream));
         if (gd) shade_grad_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);
         else if (integrator == kIntMegakernelPath) shade_kernel<kIntMegakernelPath><<<grid, kBlock, 0, stream>>>(S, st, P);
         else if (integrator == kIntAmbientOcclusion)
@@ -1264,9 +1280,18 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         else if (integrator == kIntSurfaceNormal) shade_kernel<kIntSurfaceNormal><<<grid, kBlock, 0, stream>>>(S, st, P);
         else shade_kernel<kIntPath><<<grid, kBlock, 0, stream>>>(S, st, P);
         PT_CK(hipGetLastError());
-        PT_CK(hipEventRecord(ev[4], stream));
-        PT_CK(hipMemcpyAsync(host_cnt, st.counters, 32, hipMemcpyDeviceToHost, stream));
-        PT_CK(hipStreamSynchronize(stream));
+        PT_CK(hipEventRecord(E[3], stream));
+        PT_CK(hipMemcpyAsync(hc + 16 * (round & 1), st.counters, 32, hipMemcpyDeviceToHost, stream));
+        PT_CK(hipEventRecord(E[4], stream));
+        return hipSuccess;
+    };
+    PT_CK(enqueue(0));
+    uint32_t r = 0;
+    for (;;) {
+        PT_CK(enqueue(r + 1));
+        hipEvent_t *E = ev + 3 + 5 * (r & 1);
+        PT_CK(hipEventSynchronize(E[4]));
+        const uint32_t *host_cnt = hc + 16 * (r & 1);
         // Done when no ray was traced and no path could start: every live slot then
         // ended this round. A wavefront path always has a ray, so there this round
         // had no live path and does not count; the megakernel integrators' single-
@@ -1276,19 +1301,19 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         if (done && integrator == kIntPath) break;
         next_before = host_cnt[kCntNextId];
         float a, b, c;
-        hipEventElapsedTime(&a, ev[1], ev[2]);
-        hipEventElapsedTime(&b, ev[2], ev[3]);
-        hipEventElapsedTime(&c, ev[3], ev[4]);
+        hipEventElapsedTime(&a, E[0], E[1]);
+        hipEventElapsedTime(&b, E[1], E[2]);
+        hipEventElapsedTime(&c, E[2], E[3]);
         t_trace += a;
         t_shadow += b;
         t_shade += c;
-        if (tuning) {
-            const int k = (P.round - kTuneFirst) % 2;  // 0 sorted, 1 unsorted
+        if (r >= kTuneFirst && r < kTuneFirst + 4) {
+            const int k = (r - kTuneFirst) % 2;  // 0 sorted, 1 unsorted
             tune.ms[0][k] += a;
             tune.rays[0][k] += host_cnt[kCntQueue];
             tune.ms[1][k] += b;
             tune.rays[1][k] += host_cnt[kCntSQueue];
-            if (P.round == kTuneFirst + 3) {
+            if (r == kTuneFirst + 3) {
                 auto faster_sorted = [&](int q) {
                     return tune.rays[q][0] && tune.rays[q][1] &&
                            tune.ms[q][0] / double(tune.rays[q][0]) < tune.ms[q][1] / double(tune.rays[q][1]);
@@ -1299,11 +1324,12 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         }
         rs.rays += host_cnt[kCntQueue];
         rs.shadow_rays += host_cnt[kCntSQueue];
-        P.round++;
+        ++r;
         if (done) break;
-        if (P.round > 100000) return hipErrorUnknown;
+        if (r > 100000) return hipErrorUnknown;
     }
-    PT_CK(hipEventRecord(ev[5], stream));
+    P.round = r;
+    PT_CK(hipEventRecord(ev[1], stream));
     const uint32_t wh = S.width * S.height;
     if (gd && gd->ncoef) {
         reduce_shards_kernel<<<(gd->ncoef + kBlock - 1) / kBlock, kBlock, 0, stream>>>(gd->d_coef, gd->ncoef);
@@ -1314,11 +1340,11 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
             S, st.rec, st.rec_round, spp, film_rgb, film_aux);
         PT_CK(hipGetLastError());
     }
-    PT_CK(hipEventRecord(ev[4], stream));
+    PT_CK(hipEventRecord(ev[2], stream));
     PT_CK(hipStreamSynchronize(stream));
     float tf = 0, tt = 0;
-    hipEventElapsedTime(&tf, ev[5], ev[4]);
-    hipEventElapsedTime(&tt, ev[0], ev[4]);
+    hipEventElapsedTime(&tf, ev[1], ev[2]);
+    hipEventElapsedTime(&tt, ev[0], ev[2]);
     rs.ms_total = tt;
     rs.ms_trace = t_trace;
     rs.ms_shadow = t_shadow;
