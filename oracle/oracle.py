@@ -170,6 +170,93 @@ class OrcScene(ctypes.Structure):
 METALS = {"Ag": 0, "Al": 1, "Au": 2, "Cu": 3}
 
 
+class OrcBxdf(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_uint32), ("r", c_float * 4), ("eta4", c_float * 4), ("k4", c_float * 4),
+                ("eta", c_float), ("ax", c_float), ("ay", c_float)]
+
+
+def bxdf(kind, r=(0, 0, 0, 0), eta4=(0, 0, 0, 0), k4=(0, 0, 0, 0), eta=1.0, ax=0.0, ay=None):
+    """A BxDF of the oracle in its local frame: kind 'diffuse', 'conductor' or 'dielectric'."""
+    t = {"diffuse": 0, "conductor": 1, "dielectric": 2}[kind]
+    return OrcBxdf(t, (c_float * 4)(*r), (c_float * 4)(*eta4), (c_float * 4)(*k4), eta, ax,
+                   ax if ay is None else ay)
+
+
+def _dirs(a):
+    return np.ascontiguousarray(np.broadcast_to(a, np.shape(a)), np.float32).reshape(-1, 3)
+
+
+def bxdf_flags(b):
+    return int(_pt().orc_bxdf_flags(ctypes.byref(b)))
+
+
+def bxdf_eval(b, wo, wi):
+    """BSDF::f (4 wavelengths) and BSDF::pdf for direction pairs [n][3]."""
+    wo, wi = _dirs(wo), _dirs(wi)
+    n = len(wo)
+    f = np.zeros((n, 4), np.float32)
+    pdf = np.zeros(n, np.float32)
+    _pt().orc_bxdf_eval(ctypes.byref(b), n, _p(wo), _p(wi), _p(f), _p(pdf))
+    return f, pdf
+
+
+def bxdf_sample(b, wo, u3):
+    """BSDF::sample_f: f [n][4], wi [n][3], pdf, eta, flags (1 ok, 2 specular, 4 transmission)."""
+    wo = _dirs(wo)
+    u3 = np.ascontiguousarray(u3, np.float32).reshape(-1, 3)
+    n = len(wo)
+    f = np.zeros((n, 4), np.float32)
+    wi = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    eta = np.zeros(n, np.float32)
+    flags = np.zeros(n, np.uint32)
+    _pt().orc_bxdf_sample(ctypes.byref(b), n, _p(wo), _p(u3), _p(f), _p(wi), _p(pdf), _p(eta), _p(flags))
+    return f, wi, pdf, eta, flags
+
+
+def fr_dielectric(cos_i, eta):
+    return float(_pt().orc_fr_dielectric(float(cos_i), float(eta)))
+
+
+def refract(wi, n, eta):
+    """(transmitted direction, relative IOR), or None at total internal reflection."""
+    wi = np.ascontiguousarray(wi, np.float32)
+    n = np.ascontiguousarray(n, np.float32)
+    wt = np.zeros(3, np.float32)
+    etap = c_float(0)
+    if not _pt().orc_refract(_p(wi), _p(n), float(eta), ctypes.byref(etap), _p(wt)):
+        return None
+    return wt, etap.value
+
+
+def tr_eval(ax, ay, w, wm):
+    """Trowbridge-Reitz [n][4]: D(wm), Lambda(w), G1(w), D_w(wm)."""
+    w, wm = _dirs(w), _dirs(wm)
+    out = np.zeros((len(w), 4), np.float32)
+    _pt().orc_tr_eval(float(ax), float(ay), len(w), _p(w), _p(wm), _p(out))
+    return out
+
+
+def tr_sample(ax, ay, w, u2):
+    w = _dirs(w)
+    u2 = np.ascontiguousarray(u2, np.float32).reshape(-1, 2)
+    out = np.zeros((len(w), 3), np.float32)
+    _pt().orc_tr_sample(float(ax), float(ay), len(w), _p(w), _p(u2), _p(out))
+    return out
+
+
+def light_pmf(sc):
+    """The power light sampler's pmf in light order, and each light's triangle (-1: infinite)."""
+    s, keep = _scene_struct(sc, 1)
+    cap = 4096
+    pmf = np.zeros(cap, np.float32)
+    tri = np.zeros(cap, np.int32)
+    cnt = ctypes.c_uint32(0)
+    if _pt().orc_light_pmf(ctypes.byref(s), _p(pmf), _p(tri), cap, ctypes.byref(cnt)) != 0:
+        raise RuntimeError("light_pmf")
+    return pmf[:cnt.value].copy(), tri[:cnt.value].copy()
+
+
 def _pt_sigs(lib):
     p = c_void_p
     sig = {
@@ -191,6 +278,14 @@ def _pt_sigs(lib):
         "orc_invert_spherical_triangle": (c_int, [p, p, p, p]),
         "orc_fr_complex": (c_float, [c_float, c_float, c_float]),
         "orc_srgb8": (c_int, [p, c_uint64, p]),
+        "orc_bxdf_flags": (ctypes.c_uint32, [POINTER(OrcBxdf)]),
+        "orc_bxdf_eval": (c_int, [POINTER(OrcBxdf), c_uint64, p, p, p, p]),
+        "orc_bxdf_sample": (c_int, [POINTER(OrcBxdf), c_uint64, p, p, p, p, p, p, p]),
+        "orc_fr_dielectric": (c_float, [c_float, c_float]),
+        "orc_refract": (c_int, [p, p, c_float, POINTER(c_float), p]),
+        "orc_tr_eval": (c_int, [c_float, c_float, c_uint64, p, p, p]),
+        "orc_tr_sample": (c_int, [c_float, c_float, c_uint64, p, p, p]),
+        "orc_light_pmf": (c_int, [POINTER(OrcScene), p, p, ctypes.c_uint32, POINTER(ctypes.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -107,7 +107,8 @@ int orc_rgb_to_sigmoid(const float rgb[3], float out3[3]);
 int orc_rgb_to_sigmoid_jacobian(const float rgb[3], double J9[9]); /* dc_q/drgb_k at [3q + k] */
 int orc_srgb_matrices(float xyz_from_rgb9[9], float rgb_from_xyz9[9], float white_xy2[2]);
 int orc_filter_sample(const float u[2], float out3[3]);
-float orc_spectrum_eval(int which, float lambda); /* 0-2 CIE XYZ, 3 D65 (normalised), 4+ metals eta/k */
+float orc_spectrum_eval(int which, float lambda); /* 0-2 CIE XYZ, 3 D65 (normalised), 4-11 eta/k of Ag, Al,
+                                                     Au, Cu, 12 glass-BK7 eta */
 int orc_sample_visible(float u, float lam4[4], float pdf4[4]);
 /* A stratified sampler started at (pixel, sample, dimension 0); kinds[i] = 1
  * draws get_1d, 2 draws get_2d; returns the number of floats written. */
@@ -117,6 +118,40 @@ int orc_camera_ray(const orc_scene *scene, float px, float py, float out6[6]);
 int orc_spherical_triangle(const float v9[9], const float p[3], const float u[2], float b3[3], float *pdf);
 int orc_invert_spherical_triangle(const float v9[9], const float p[3], const float w3[3], float u2[2]);
 float orc_fr_complex(float cos_i, float eta, float k);
+/* Shading functions in the local frame (shading normal = +z), one direction
+ * pair per entry, for the tests that pin them to the mathematics. A BxDF:
+ * type 0 diffuse (r = the sampled reflectance at 4 wavelengths), 1 conductor
+ * (eta4, k4), 2 dielectric (eta; 0 means 1). ax, ay are the Trowbridge-Reitz
+ * alphas after any roughness remapping; a rough lobe clamps them to >= 1e-4
+ * as the material does. flags: BxDFFlags (reflection 1, transmission 2,
+ * diffuse 4, glossy 8, specular 16). */
+typedef struct orc_bxdf {
+    uint32_t type;
+    float r[4];
+    float eta4[4], k4[4];
+    float eta;
+    float ax, ay;
+} orc_bxdf;
+uint32_t orc_bxdf_flags(const orc_bxdf *b);
+/* BSDF::f and BSDF::pdf of (wo, wi): f4 [n][4], pdf [n]. */
+int orc_bxdf_eval(const orc_bxdf *b, uint64_t n, const float *wo, const float *wi, float *f4, float *pdf);
+/* BSDF::sample_f of wo with (uc, u0, u1) = u3[i]: f4, wi, pdf, eta (the relative
+ * IOR of a transmission, else 1); flags[i] bit 0 = a sample was returned,
+ * bit 1 specular, bit 2 transmission. */
+int orc_bxdf_sample(const orc_bxdf *b, uint64_t n, const float *wo, const float *u3, float *f4, float *wi,
+                    float *pdf, float *eta, uint32_t *flags);
+float orc_fr_dielectric(float cos_i, float eta);
+/* refract(wi, n, eta): returns 1 and the transmitted direction and relative
+ * IOR, or 0 at total internal reflection. */
+int orc_refract(const float wi[3], const float n[3], float eta, float *etap, float wt[3]);
+/* Trowbridge-Reitz: out4[i] = D(wm), Lambda(w), G1(w), D_w(wm) (the visible
+ * normal density); orc_tr_sample: Sample_wm(w, u). */
+int orc_tr_eval(float ax, float ay, uint64_t n, const float *w, const float *wm, float *out4);
+int orc_tr_sample(float ax, float ay, uint64_t n, const float *w, const float *u2, float *wm);
+/* The power light sampler's pmf over the scene's lights in light order, and
+ * each light's emissive triangle (-1 = an infinite light). */
+int orc_light_pmf(const orc_scene *scene, float *pmf, int32_t *tri, uint32_t cap, uint32_t *count);
+
 /* Film::write_to_png's per-channel encoding: linear RGB [n][3] -> sRGB RGBA8 [n][4]. */
 int orc_srgb8(const float *rgb, uint64_t n, uint8_t *rgba);
 

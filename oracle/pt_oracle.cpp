@@ -2201,6 +2201,11 @@ float orc_spectrum_eval(int which, float lambda) {
         for (int i = 0; i < 8; ++i) metals.push_back(orc::pl_from_interleaved(d[i], n[i]));
     });
     if (which >= 4 && which < 12) return metals[which - 4](lambda);
+    if (which == 12) {
+        static const orc::PLSpectrum bk7 = orc::pl_from_interleaved(pbrtm_data::kGlassBK7Eta,
+                                                                    pbrtm_data::kGlassBK7EtaLen);
+        return bk7(lambda);
+    }
     return NAN;
 }
 
@@ -2253,6 +2258,100 @@ int orc_invert_spherical_triangle(const float v9[9], const float p[3], const flo
 }
 
 float orc_fr_complex(float cos_i, float eta, float k) { return orc::fr_complex(cos_i, orc::Cx{eta, k}); }
+
+// ---- shading functions one at a time, in the local frame (shading normal +z),
+// for the mathematical pins of tests/test_oracle_bsdf.py.
+static orc::Bsdf local_bsdf(const orc_bxdf &d) {
+    orc::Bsdf b{};
+    b.fr = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    b.type = d.type;
+    b.tr = orc::TR{d.ax, d.ay};
+    if (!b.tr.smooth()) b.tr = orc::TR{std::max(d.ax, 1e-4f), std::max(d.ay, 1e-4f)};
+    for (int i = 0; i < orc::kNS; ++i) {
+        b.r.v[i] = orc::clampf(d.r[i], 0.0f, 1.0f);
+        b.eta.v[i] = d.eta4[i];
+        b.k.v[i] = d.k4[i];
+    }
+    b.diffuse_set = orc::positive(b.r);
+    b.eta_d = d.eta == 0 ? 1.0f : d.eta;
+    return b;
+}
+
+uint32_t orc_bxdf_flags(const orc_bxdf *d) { return orc::bsdf_flags(local_bsdf(*d)); }
+
+int orc_bxdf_eval(const orc_bxdf *d, uint64_t n, const float *wo, const float *wi, float *f4, float *pdf) {
+    const orc::Bsdf b = local_bsdf(*d);
+    for (uint64_t i = 0; i < n; ++i) {
+        const orc::V3 o{wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]}, w{wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]};
+        const orc::S4 f = orc::bsdf_f(b, o, w);
+        for (int k = 0; k < 4; ++k) f4[4 * i + k] = f.v[k];
+        pdf[i] = orc::bsdf_pdf(b, o, w);
+    }
+    return 0;
+}
+
+int orc_bxdf_sample(const orc_bxdf *d, uint64_t n, const float *wo, const float *u3, float *f4, float *wi,
+                    float *pdf, float *eta, uint32_t *flags) {
+    const orc::Bsdf b = local_bsdf(*d);
+    for (uint64_t i = 0; i < n; ++i) {
+        const orc::V3 o{wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]};
+        orc::BS bs{};
+        const bool ok = orc::bsdf_sample(b, o, u3[3 * i], u3[3 * i + 1], u3[3 * i + 2], &bs);
+        flags[i] = ok ? (1u | (bs.specular ? 2u : 0u) | (bs.transmission ? 4u : 0u)) : 0u;
+        for (int k = 0; k < 4; ++k) f4[4 * i + k] = ok ? bs.f.v[k] : 0.f;
+        wi[3 * i] = ok ? bs.wi.x : 0.f;
+        wi[3 * i + 1] = ok ? bs.wi.y : 0.f;
+        wi[3 * i + 2] = ok ? bs.wi.z : 0.f;
+        pdf[i] = ok ? bs.pdf : 0.f;
+        eta[i] = ok ? bs.eta : 0.f;
+    }
+    return 0;
+}
+
+float orc_fr_dielectric(float cos_i, float eta) { return orc::fr_dielectric(cos_i, eta); }
+
+int orc_refract(const float wi[3], const float n[3], float eta, float *etap, float wt[3]) {
+    orc::V3 t{};
+    if (!orc::refract({wi[0], wi[1], wi[2]}, {n[0], n[1], n[2]}, eta, etap, &t)) return 0;
+    wt[0] = t.x, wt[1] = t.y, wt[2] = t.z;
+    return 1;
+}
+
+int orc_tr_eval(float ax, float ay, uint64_t n, const float *w, const float *wm, float *out4) {
+    const orc::TR tr{ax, ay};
+    for (uint64_t i = 0; i < n; ++i) {
+        const orc::V3 a{w[3 * i], w[3 * i + 1], w[3 * i + 2]}, m{wm[3 * i], wm[3 * i + 1], wm[3 * i + 2]};
+        out4[4 * i] = tr.D(m);
+        out4[4 * i + 1] = tr.lambda(a);
+        out4[4 * i + 2] = tr.G1(a);
+        out4[4 * i + 3] = tr.D(a, m);
+    }
+    return 0;
+}
+
+int orc_tr_sample(float ax, float ay, uint64_t n, const float *w, const float *u2, float *wm) {
+    const orc::TR tr{ax, ay};
+    for (uint64_t i = 0; i < n; ++i) {
+        const orc::V3 m = tr.sample_wm({w[3 * i], w[3 * i + 1], w[3 * i + 2]}, u2[2 * i], u2[2 * i + 1]);
+        wm[3 * i] = m.x, wm[3 * i + 1] = m.y, wm[3 * i + 2] = m.z;
+    }
+    return 0;
+}
+
+int orc_light_pmf(const orc_scene *d, float *pmf, int32_t *tri, uint32_t cap, uint32_t *count) {
+    try {
+        orc::Scene sc;
+        orc::build_scene(sc, *d);
+        *count = uint32_t(sc.lights.size());
+        for (size_t i = 0; i < sc.lights.size() && i < cap; ++i) {
+            pmf[i] = sc.light_pmf[i];
+            tri[i] = sc.lights[i].infinite ? -1 : int32_t(sc.lights[i].tri);
+        }
+    } catch (const std::exception &) {
+        return -1;
+    }
+    return 0;
+}
 
 // SRGBColorEncoding::from_linear (spectrum_util/color_encoding.h:7-60).
 int orc_srgb8(const float *rgb, uint64_t n, uint8_t *rgba) {
