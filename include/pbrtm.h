@@ -118,6 +118,20 @@ int pbrtm_bvh_export_canonical(const pbrtm_bvh *bvh, uint8_t *buf, uint64_t *len
 int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_hit, uint32_t flags,
                         uint32_t *prim_out, float *t_out, float *bary_out, uint64_t *stats_out);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pbrtm_bvh_intersect with PBRTM_DEVICE_PTRS (8-wide
+ * traversal, no statistics), but the ray queue, the device error word and the
+ * traversal's stack spill live in a fixed library scratch array (32 MiB + 256 B
+ * per device, held until the process ends) instead of buffers it allocates.
+ * rays, prim_out, t_out and bary_out are device pointers (t_out, bary_out
+ * nullable); results equal pbrtm_bvh_intersect's bit for bit. At most 2^22 rays
+ * per call (PBRTM_ERR_INVALID beyond), and the tree's stack spill for n rays must
+ * fit 16 MiB (trees deeper than 12 wide levels; PBRTM_ERR_INVALID otherwise).
+ * Calls that overlap on one device share the scratch and race: serialise them. */
+int pbrtm_bvh_intersect_noalloc(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_hit, uint32_t *prim_out,
+                                float *t_out, float *bary_out);
+
 /* From closest hits (prim, barycentrics), one cosine-weighted diffuse bounce
  * per ray, as integrators/ambient_occlusion.cu:36-52 spawns it: u = the first
  * two PCG32 draws of sequence = ray index, frame around the face-forwarded

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -119,6 +120,54 @@ double event_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// Scratch of pbrtm_bvh_intersect_noalloc: a file-scope device array, so that the
+// entry point allocates nothing. Layout: a 256-byte header (word 0: device
+// error bits, word 1: ray queue length; cleared by a kernel on every call),
+// the ray queue (one word per ray), then the traversal's stack spill area.
+// Each device holds its own copy for the life of the process; every call on a
+// device shares it, so calls that overlap on one device race.
+constexpr uint64_t kNoAllocMaxRays = 1ull << 22;
+constexpr uint64_t kNoAllocHeader = 256;
+constexpr uint64_t kNoAllocSpillBytes = 16ull << 20;
+constexpr uint64_t kNoAllocBytes = kNoAllocHeader + kNoAllocMaxRays * 4 + kNoAllocSpillBytes;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+// The array's device address per device, taken once with that device current
+// (by pbrtm_create, or by the first no-alloc call there).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<uint8_t *>(p);
+    return hipSuccess;
+}
+
+// Device address of this device's scratch (the device must be current), or null.
+uint8_t *noalloc_scratch_of(int device) {
+    if (noalloc_scratch_init(device) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+__global__ void zero_words_kernel(uint32_t *w, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) w[i] = 0u;
+}
+
+__global__ void fill_miss_kernel(uint32_t n, uint32_t *prim, float *t, float *bary) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    prim[i] = kNoHit;
+    if (t) t[i] = 0.f;
+    if (bary) bary[3 * size_t(i)] = bary[3 * size_t(i) + 1] = bary[3 * size_t(i) + 2] = 0.f;
+}
+
 uint32_t canon_bits(float f) {
     uint32_t u;
     std::memcpy(&u, &f, 4);
@@ -156,6 +205,8 @@ int pbrtm_create(pbrtm_ctx **out, int device) {
         return PBRTM_ERR_HIP;
     }
     for (auto &e : ctx->ev) hipEventCreate(&e);
+    // the no-alloc entry point's scratch address on this device (checked again there)
+    (void)noalloc_scratch_init(device);
     *out = ctx;
     return PBRTM_OK;
 }
@@ -448,6 +499,51 @@ int pbrtm_bvh_intersect(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_h
     CK(ctx, hipMemcpyAsync(&derr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "download");
     if (stats_out)
         CK(ctx, hipMemcpyAsync(stats_out, d_counts, 24, hipMemcpyDeviceToHost, ctx->stream), "download");
+    CK(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (derr) return fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr));
+    return PBRTM_OK;
+}
+
+int pbrtm_bvh_intersect_noalloc(pbrtm_bvh *bvh, const float *rays, uint64_t n, int any_hit, uint32_t *prim_out,
+                                float *t_out, float *bary_out) {
+    if (!bvh || !rays || !prim_out) return PBRTM_ERR_INVALID;
+    pbrtm_ctx *ctx = bvh->ctx;
+    if (n > kNoAllocMaxRays)
+        return fail(ctx, PBRTM_ERR_INVALID,
+                    "pbrtm_bvh_intersect_noalloc: at most " + std::to_string(kNoAllocMaxRays) + " rays per call");
+    if (n == 0) return PBRTM_OK;
+    CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    uint8_t *base = noalloc_scratch_of(ctx->device);
+    if (!base) return fail(ctx, PBRTM_ERR_HIP, "pbrtm_bvh_intersect_noalloc: scratch symbol address unavailable");
+    uint32_t *words = reinterpret_cast<uint32_t *>(base);  // [0] device error bits, [1] queue length
+    uint32_t *queue = reinterpret_cast<uint32_t *>(base + kNoAllocHeader);
+    uint2 *spill = reinterpret_cast<uint2 *>(base + kNoAllocHeader + kNoAllocMaxRays * sizeof(uint32_t));
+    zero_words_kernel<<<1, 64, 0, ctx->stream>>>(words, kNoAllocHeader / sizeof(uint32_t));
+    CK(ctx, hipGetLastError(), "clear scratch");
+    if (bvh->n == 0) {
+        fill_miss_kernel<<<uint32_t((n + 255) / 256), 256, 0, ctx->stream>>>(uint32_t(n), prim_out, t_out, bary_out);
+        CK(ctx, hipGetLastError(), "fill misses");
+    } else {
+        WideTrace W{};
+        W.nodes = bvh->wide.nodes.as<WNode>();
+        W.order = bvh->wide.order.as<uint32_t>();
+        W.tri_leaf = bvh->tri_leaf.as<float>();
+        W.prim_order = bvh->dev.prim_order.as<uint32_t>();
+        W.num_nodes = bvh->wide.num_nodes;
+        W.num_prims = bvh->n;
+        W.max_depth = bvh->wide.max_depth;
+        W.spill_per_ray = W.max_depth > kLdsStack ? W.max_depth - kLdsStack + 1 : 0;
+        if (uint64_t(W.spill_per_ray) * n * sizeof(uint2) > kNoAllocSpillBytes)
+            return fail(ctx, PBRTM_ERR_INVALID,
+                        "pbrtm_bvh_intersect_noalloc: this tree (" + std::to_string(W.max_depth) +
+                            " wide levels) needs a stack spill area larger than the " +
+                            std::to_string(kNoAllocSpillBytes) + "-byte scratch for " + std::to_string(n) + " rays");
+        W.spill = W.spill_per_ray ? spill : nullptr;
+        CK(ctx, trace_wide(W, rays, uint32_t(n), any_hit, prim_out, t_out, bary_out, words, nullptr, queue,
+                           words + 1, ctx->stream), "trace");
+    }
+    uint32_t derr = 0;
+    CK(ctx, hipMemcpyAsync(&derr, words, 4, hipMemcpyDeviceToHost, ctx->stream), "download");
     CK(ctx, hipStreamSynchronize(ctx->stream), "sync");
     if (derr) return fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr));
     return PBRTM_OK;

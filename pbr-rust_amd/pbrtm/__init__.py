@@ -35,7 +35,7 @@ EXPORTS = (
     "pbrtm_gen_camera_rays", "pbrtm_scene_create", "pbrtm_scene_free", "pbrtm_render", "pbrtm_rgb_to_sigmoid",
     "pbrtm_srgb_matrices", "pbrtm_srgb8", "pbrtm_write_png", "pbrtm_pbrt_parse", "pbrtm_pbrt_desc",
     "pbrtm_pbrt_render_params", "pbrtm_pbrt_output", "pbrtm_pbrt_free", "pbrtm_render_batch",
-    "pbrtm_render_grad", "pbrtm_integrator_from_name", "pbrtm_write_pfm",
+    "pbrtm_render_grad", "pbrtm_integrator_from_name", "pbrtm_write_pfm", "pbrtm_bvh_intersect_noalloc",
 )
 BATCH_NO_GRAPH = 2
 # pbrtm_render_params.flags: the integrator
@@ -152,6 +152,7 @@ def load(path: str = LIB_PATH):
         "pbrtm_bvh_bounds": (c_int, [p, p]),
         "pbrtm_bvh_export_canonical": (c_int, [p, p, POINTER(c_uint64)]),
         "pbrtm_bvh_intersect": (c_int, [p, p, c_uint64, c_int, c_uint32, p, p, p, p]),
+        "pbrtm_bvh_intersect_noalloc": (c_int, [p, p, c_uint64, c_int, p, p, p]),
         "pbrtm_ao_bounce": (c_int, [p, p, p, p, c_uint64, c_uint32, p]),
         "pbrtm_gen_soup": (c_int, [c_uint64, c_uint64, c_float, c_float, c_int, p]),
         "pbrtm_gen_camera_rays": (c_int, [c_uint32, c_uint32, p, p, p, c_float, p]),
@@ -620,6 +621,13 @@ class Bvh:
         flags = (DEVICE_PTRS if device_ptrs else 0) | (TRACE_BINARY if binary else 0)
         self.ctx.check(self.ctx.lib.pbrtm_bvh_intersect(self.h, _ptr(rays), n, int(any_hit), flags, _ptr(prim),
                                                         _ptr(t), _ptr(bary), _ptr(stats)))
+        return prim, t, bary
+
+    def intersect_noalloc(self, rays, prim, t=None, bary=None, any_hit=False):
+        """pbrtm_bvh_intersect_noalloc: device tensors in and out, no device
+        allocation inside the library (its fixed scratch array instead)."""
+        self.ctx.check(self.ctx.lib.pbrtm_bvh_intersect_noalloc(self.h, _ptr(rays), rays.shape[0], int(any_hit),
+                                                                _ptr(prim), _ptr(t), _ptr(bary)))
         return prim, t, bary
 
     def ao_bounce(self, rays, prim, bary, device_ptrs=False, out=None):

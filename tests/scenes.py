@@ -27,3 +27,44 @@ def cornell_emitters(width=32, height=32, spp=16, max_depth=4):
                 {"rgb": (0.3, 0.5, 1.0), "scale": 9.0, "two_sided": True}],
     )
     return sc
+
+
+def deep_chain(levels=36, per_level=8):
+    """A triangle set whose 8-wide tree is deeper than the traversal's 12-entry LDS
+    stack, so that closest-hit rays along +x spill their stack to memory.
+
+    Cluster k (k = 1..levels) holds `per_level` triangles with bounding-box centres
+    at x = 0.5 * 2^-k * (1 - 0.04 j), y = z = 0.5. Midpoint splits of the centroid
+    bounds on x (the only axis with extent) then peel one cluster per binary level.
+    Cluster k spans 0.8 * 0.9^k in y and z, more than every deeper cluster, so the
+    greedy wide collapse (largest surface area first) opens cluster k's own nodes
+    before the deeper chain: one wide level per cluster. A ray along +x through the
+    middle enters the deeper chain first (near child) and pushes cluster k's slots
+    at every level."""
+    import numpy as np
+
+    tri = []
+    for k in range(1, levels + 1):
+        c0 = 0.5 * 2.0 ** -k
+        a = 0.4 * 0.9 ** k
+        for j in range(per_level):
+            c = c0 * (1.0 - 0.04 * j)
+            d = 0.01 * c0
+            tri.append([[c - d, 0.5 - a, 0.5 - 0.3 * a], [c + d, 0.5 + a, 0.5 - a], [c, 0.5 + 0.2 * a, 0.5 + a]])
+    return np.ascontiguousarray(np.array(tri, np.float32))
+
+
+def chain_rays(n, seed=3, x0=-1e-12):
+    """Rays from x = x0 towards +x through the chain's middle, with small tilts.
+    From x0 = -1e-12 the clusters' hit distances differ (no exact t ties); from
+    x0 = -1 every cluster below 2^-25 is hit at t = 1.0 exactly."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0] = x0
+    rays[:, 1:3] = 0.5 + rng.uniform(-0.01, 0.01, (n, 2))
+    rays[:, 3] = np.inf
+    d = np.stack([np.ones(n), rng.uniform(-0.02, 0.02, n), rng.uniform(-0.02, 0.02, n)], 1)
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    return rays
