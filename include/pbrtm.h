@@ -196,6 +196,22 @@ typedef struct pbrtm_infinite_light { /* LightSource "infinite" without "filenam
                                      LightSource and area lights in file order) */
 } pbrtm_infinite_light;
 
+#define PBRTM_SHAPE_SPHERE 1u   /* Shape "sphere" (shapes/sphere.cu) */
+#define PBRTM_SHAPE_DISK 2u     /* Shape "disk" (shapes/disk.cu) */
+#define PBRTM_SHAPE_INSTANCE 3u /* a triangle of an ObjectInstance: TransformedPrimitive
+                                   (primitives/transformed_primitive.cu, scene_builder.cu:775-810) */
+
+typedef struct pbrtm_shape {
+    uint32_t type;        /* PBRTM_SHAPE_* */
+    uint32_t xform;       /* sphere, disk: the CTM at the Shape; instance: the CTM at the ObjectInstance.
+                             An index into xforms / xforms_inv, UINT32_MAX = identity */
+    float radius;         /* "float radius" (default 1) */
+    float zmin, zmax;     /* sphere "zmin" / "zmax" (defaults -radius / radius) */
+    float phimax;         /* "phimax" in degrees (default 360) */
+    float height;         /* disk "height" (default 0) */
+    float innerradius;    /* disk "innerradius" (default 0) */
+} pbrtm_shape;
+
 typedef struct pbrtm_scene_desc {
     float eye[3], look[3], up[3]; /* LookAt */
     float fov;                    /* Camera "perspective" "float fov" (degrees) */
@@ -214,6 +230,17 @@ typedef struct pbrtm_scene_desc {
     const uint32_t *tri_xform;    /* [n] index into xforms, UINT32_MAX = identity; NULL = all identity */
     uint32_t num_infinite_lights; /* at most 4 */
     const pbrtm_infinite_light *infinite_lights;
+    /* Analytic shapes and instancing. The primitive list stays the triangle list above,
+     * in the front end's order (which orders the aggregate and the lights); entry i is
+     * shapes[tri_shape[i]] when tri_shape[i] != UINT32_MAX: a sphere or a disk (its
+     * vertices are ignored; its material, light and ReverseOrientation are entry i's),
+     * or a triangle of an object instance (its vertices, placed by tri_xform[i] as the
+     * definition's CTM placed them, are the instanced triangle; instances cannot emit). */
+    const float *xforms_inv;      /* [num_xforms][4][4]: each CTM's inverse as the front end composed it
+                                     (Transform keeps both); NULL = inverted here */
+    uint32_t num_shapes;
+    const pbrtm_shape *shapes;
+    const uint32_t *tri_shape;    /* [n]; NULL = every entry is a triangle */
 } pbrtm_scene_desc;
 
 typedef struct pbrtm_render_params {

@@ -60,6 +60,9 @@ void parallel_for(uint64_t n, int threads, const std::function<void(uint64_t, ui
 struct orc_bvh {
     const float *tri = nullptr;  // caller-owned, must outlive the handle
     uint64_t n = 0;
+    orc_prim_test custom_test = nullptr;  // primitives with custom[i] != 0 are tested by this
+    void *custom_user = nullptr;
+    const uint8_t *custom = nullptr;
     std::vector<Prim> prims;     // Morton-sorted, then partitioned in place
     std::vector<Node> nodes;
     int top_nodes = 0;
@@ -337,9 +340,14 @@ int trace(const orc_bvh *bvh, V3 o, V3 d, float t_max, bool any_hit, uint32_t *p
                 uint32_t pid = bvh->prims[i].idx;
                 TriHit h;
                 st->tris++;
-                if (!intersect_triangle(o, d, best_t, tri_point(bvh->tri, pid, 0),
-                                        tri_point(bvh->tri, pid, 1), tri_point(bvh->tri, pid, 2), &h))
+                if (bvh->custom && bvh->custom[pid]) {
+                    float t, b3[3];
+                    if (!bvh->custom_test(bvh->custom_user, pid, &o.x, &d.x, best_t, &t, b3)) continue;
+                    h = TriHit{b3[0], b3[1], b3[2], t};
+                } else if (!intersect_triangle(o, d, best_t, tri_point(bvh->tri, pid, 0), tri_point(bvh->tri, pid, 1),
+                                               tri_point(bvh->tri, pid, 2), &h)) {
                     continue;
+                }
                 *prim = pid;
                 *hit = h;
                 if (any_hit) return 1;
@@ -432,6 +440,12 @@ int orc_bvh_build(const float *tri, uint64_t n, orc_bvh **out, orc_build_info *i
 const char *orc_bvh_error(const orc_bvh *bvh) { return bvh ? bvh->error.c_str() : "null handle"; }
 
 void orc_bvh_free(orc_bvh *bvh) { delete bvh; }
+
+void orc_bvh_set_custom(orc_bvh *bvh, orc_prim_test fn, void *user, const uint8_t *custom) {
+    bvh->custom_test = fn;
+    bvh->custom_user = user;
+    bvh->custom = custom;
+}
 
 int orc_bvh_export_canonical(const orc_bvh *bvh, uint8_t *buf, uint64_t *len) {
     std::vector<uint8_t> out;

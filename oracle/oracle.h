@@ -47,6 +47,12 @@ int orc_ao_bounce(const float *tri, const float *rays, const uint32_t *prim, con
 uint32_t orc_pcg32_sequence(uint64_t seq, uint64_t offset, int mode, int64_t advance, uint32_t *out,
                             uint64_t count);
 uint64_t orc_murmur64a(const uint8_t *key, uint64_t len, uint64_t seed);
+/* Primitives that are not triangles (analytic shapes, instances): custom[i] != 0
+ * routes primitive i's test to fn, which returns 1 on a hit with t < or = t_max
+ * as the shape defines it, writing t and three words for the hit record. */
+typedef int (*orc_prim_test)(void *user, uint32_t prim, const float o[3], const float d[3], float t_max, float *t,
+                             float b3[3]);
+void orc_bvh_set_custom(orc_bvh *bvh, orc_prim_test fn, void *user, const uint8_t *custom);
 /* One ray through the oracle tree (closest or any hit); 1 = hit, 0 = miss, -1 = stack overflow. */
 int orc_trace_ray(const orc_bvh *bvh, const float o[3], const float d[3], float t_max, int any_hit,
                   uint32_t *prim, float *t, float *bary3);

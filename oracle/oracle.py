@@ -157,6 +157,11 @@ class OrcInfiniteLight(ctypes.Structure):
                 ("light_index", c_uint32)]
 
 
+class OrcShape(ctypes.Structure):
+    _fields_ = [("type", c_uint32), ("xform", c_uint32), ("radius", c_float), ("zmin", c_float), ("zmax", c_float),
+                ("phimax", c_float), ("height", c_float), ("innerradius", c_float)]
+
+
 class OrcScene(ctypes.Structure):
     _fields_ = [("eye", c_float * 3), ("look", c_float * 3), ("up", c_float * 3), ("fov", c_float),
                 ("width", c_uint32), ("height", c_uint32), ("spp", c_uint32), ("max_depth", c_uint32),
@@ -164,7 +169,12 @@ class OrcScene(ctypes.Structure):
                 ("tri_light", c_void_p), ("tri_reverse", c_void_p), ("num_materials", c_uint32),
                 ("materials", POINTER(OrcMaterial)), ("num_lights", c_uint32), ("lights", POINTER(OrcLight)),
                 ("num_xforms", c_uint32), ("xforms", c_void_p), ("tri_xform", c_void_p), ("select_scale", c_void_p),
-                ("num_infinite_lights", c_uint32), ("infinite_lights", POINTER(OrcInfiniteLight))]
+                ("num_infinite_lights", c_uint32), ("infinite_lights", POINTER(OrcInfiniteLight)),
+                ("xforms_inv", c_void_p), ("num_shapes", c_uint32), ("shapes", POINTER(OrcShape)),
+                ("tri_shape", c_void_p)]
+
+
+SHAPE_KINDS = {"sphere": 1, "disk": 2, "instance": 3}
 
 
 METALS = {"Ag": 0, "Al": 1, "Au": 2, "Cu": 3}
@@ -286,6 +296,10 @@ def _pt_sigs(lib):
         "orc_tr_eval": (c_int, [c_float, c_float, c_uint64, p, p, p]),
         "orc_tr_sample": (c_int, [c_float, c_float, c_uint64, p, p, p]),
         "orc_light_pmf": (c_int, [POINTER(OrcScene), p, p, ctypes.c_uint32, POINTER(ctypes.c_uint32)]),
+        "orc_shape_intersect": (c_int, [POINTER(OrcShape), p, p, c_int, p, c_uint64, p, p, p, p]),
+        "orc_shape_sample": (c_int, [POINTER(OrcShape), p, p, c_int, p, p, c_uint64, p, p, p, p]),
+        "orc_shape_pdf": (c_int, [POINTER(OrcShape), p, p, c_int, p, p, c_uint64, p]),
+        "orc_directed": (c_int, [c_int, c_float, c_float, p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -365,7 +379,27 @@ def _scene_struct(sc, pool_size):
         infl[i].light_index = l.get("light_index", 0)
     s.num_infinite_lights = len(inf)
     s.infinite_lights = infl
-    return s, (tri, mat, lig, rev, mats, lights, xf, txf, sel, marr, infl)
+    shp = sc.get("shapes") or []
+    shapes = (OrcShape * max(1, len(shp)))()
+    for i, q in enumerate(shp):
+        r = q.get("radius", 1.0)
+        shapes[i].type = SHAPE_KINDS[q["type"]]
+        shapes[i].xform = 0xFFFFFFFF if q.get("xform") is None else q["xform"]
+        shapes[i].radius = r
+        shapes[i].zmin = q.get("zmin", -r)
+        shapes[i].zmax = q.get("zmax", r)
+        shapes[i].phimax = q.get("phimax", 360.0)
+        shapes[i].height = q.get("height", 0.0)
+        shapes[i].innerradius = q.get("innerradius", 0.0)
+    tsh = sc.get("tri_shape")
+    tsh = None if tsh is None else np.ascontiguousarray(tsh, dtype=np.uint32)
+    xinv = sc.get("xforms_inv")
+    xinv = None if xinv is None else np.ascontiguousarray(xinv, dtype=np.float32).reshape(-1, 16)
+    s.num_shapes = len(shp)
+    s.shapes = shapes
+    s.tri_shape = _p(tsh)
+    s.xforms_inv = _p(xinv)
+    return s, (tri, mat, lig, rev, mats, lights, xf, txf, sel, marr, infl, shapes, tsh, xinv)
 
 
 def render(sc, pool_size=2 * 1024 * 1024, threads=8):
@@ -567,6 +601,66 @@ def invert_spherical_triangle(v, p, w):
     u = np.zeros(2, np.float32)
     _pt().orc_invert_spherical_triangle(_p(v), _p(p), _p(w), _p(u))
     return u
+
+
+def _shape(kind, radius=1.0, zmin=None, zmax=None, phimax=360.0, height=0.0, innerradius=0.0):
+    q = OrcShape()
+    q.type = SHAPE_KINDS[kind]
+    q.radius = radius
+    q.zmin = -radius if zmin is None else zmin
+    q.zmax = radius if zmax is None else zmax
+    q.phimax, q.height, q.innerradius = phimax, height, innerradius
+    return q
+
+
+def _xf(m):
+    m = np.ascontiguousarray(m, np.float32).reshape(4, 4)
+    return m, np.ascontiguousarray(np.linalg.inv(m.astype(np.float64)), np.float32)
+
+
+def shape_intersect(kind, m, rays, reverse=False, inv=None, **params):
+    """Closest hits of rays [n][8] with one shape placed by render_from_object m
+    (inverse `inv`, default the FP64 inverse rounded): (hit, t, point, normal)."""
+    q = _shape(kind, **params)
+    m, inv0 = _xf(m)
+    inv = inv0 if inv is None else np.ascontiguousarray(inv, np.float32)
+    r = np.ascontiguousarray(rays, np.float32)
+    n = len(r)
+    t, pp, nn, hit = np.zeros(n, np.float32), np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), \
+        np.zeros(n, np.uint8)
+    _pt().orc_shape_intersect(ctypes.byref(q), _p(m), _p(inv), int(reverse), _p(r), n, _p(t), _p(pp), _p(nn), _p(hit))
+    return hit.astype(bool), t, pp, nn
+
+
+def shape_sample(kind, m, ctx, u, reverse=False, **params):
+    """Shape::sample(ctx, u): ctx [n][9] = (p, n, ns), u [n][2] -> (ok, point, normal, pdf)."""
+    q = _shape(kind, **params)
+    m, inv = _xf(m)
+    c = np.ascontiguousarray(ctx, np.float32)
+    uu = np.ascontiguousarray(u, np.float32)
+    n = len(c)
+    pp, nn, pdf, ok = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros(n, np.float32), \
+        np.zeros(n, np.uint8)
+    _pt().orc_shape_sample(ctypes.byref(q), _p(m), _p(inv), int(reverse), _p(c), _p(uu), n, _p(pp), _p(nn), _p(pdf),
+                           _p(ok))
+    return ok.astype(bool), pp, nn, pdf
+
+
+def shape_pdf(kind, m, ctx, wi, reverse=False, **params):
+    q = _shape(kind, **params)
+    m, inv = _xf(m)
+    c = np.ascontiguousarray(ctx, np.float32)
+    w = np.ascontiguousarray(wi, np.float32)
+    pdf = np.zeros(len(c), np.float32)
+    _pt().orc_shape_pdf(ctypes.byref(q), _p(m), _p(inv), int(reverse), _p(c), _p(w), len(c), _p(pdf))
+    return pdf
+
+
+def directed(op, a, b=0.0):
+    """[round down, round up] of a*b ("mul"), a/b ("div"), sqrt(a) ("sqrt"), a+b ("add")."""
+    out = np.zeros(2, np.float32)
+    _pt().orc_directed({"mul": 0, "div": 1, "sqrt": 2, "add": 3}[op], float(a), float(b), _p(out))
+    return float(out[0]), float(out[1])
 
 
 def fr_complex(cos_i, eta, k):

@@ -40,6 +40,12 @@ typedef struct orc_infinite_light { /* LightSource "infinite" (uniform_infinite_
     uint32_t light_index;   /* place in the light list: emissive triangles created before it */
 } orc_infinite_light;
 
+typedef struct orc_shape {  /* Shape "sphere" (1), "disk" (2), or an ObjectInstance's triangle (3) */
+    uint32_t type;
+    uint32_t xform;         /* the CTM (instance: at the ObjectInstance): index into xforms, UINT32_MAX = identity */
+    float radius, zmin, zmax, phimax /* degrees */, height, innerradius;
+} orc_shape;
+
 typedef struct orc_scene {
     float eye[3], look[3], up[3]; /* LookAt */
     float fov;                    /* Camera "perspective" "float fov" */
@@ -64,6 +70,10 @@ typedef struct orc_scene {
                                      the scales; pmf values in the estimator stay the scene's) */
     uint32_t num_infinite_lights;
     const orc_infinite_light *infinite_lights;
+    const float *xforms_inv;      /* [num_xforms] inverses as composed by the front end; nullable */
+    uint32_t num_shapes;
+    const orc_shape *shapes;
+    const uint32_t *tri_shape;    /* [n] UINT32_MAX = triangle, else an index into shapes; nullable */
 } orc_scene;
 
 /* film_rgb: [height][width][3] = RGBFilm::get_pixel_rgb; aux (nullable):
@@ -151,6 +161,18 @@ int orc_tr_sample(float ax, float ay, uint64_t n, const float *w, const float *u
 /* The power light sampler's pmf over the scene's lights in light order, and
  * each light's emissive triangle (-1 = an infinite light). */
 int orc_light_pmf(const orc_scene *scene, float *pmf, int32_t *tri, uint32_t cap, uint32_t *count);
+
+/* Analytic shapes one at a time (tests), given render_from_object (m16) and its
+ * inverse (inv16), row-major: closest hits of rays [n][8] (t, render-space point
+ * and normal), Shape::sample(ctx, u) and Shape::pdf(ctx, wi) with ctx [n][9] =
+ * (p, n, ns); and the directed-rounding primitives (op 0 mul, 1 div, 2 sqrt, 3 add). */
+int orc_shape_intersect(const orc_shape *sh, const float *m16, const float *inv16, int reverse, const float *rays,
+                        uint64_t n, float *t_out, float *p_out, float *n_out, uint8_t *hit_out);
+int orc_shape_sample(const orc_shape *sh, const float *m16, const float *inv16, int reverse, const float *ctx,
+                     const float *u, uint64_t n, float *p_out, float *n_out, float *pdf_out, uint8_t *ok_out);
+int orc_shape_pdf(const orc_shape *sh, const float *m16, const float *inv16, int reverse, const float *ctx,
+                  const float *wi, uint64_t n, float *pdf_out);
+int orc_directed(int op, float a, float b, float out[2]);
 
 /* Film::write_to_png's per-channel encoding: linear RGB [n][3] -> sRGB RGBA8 [n][4]. */
 int orc_srgb8(const float *rgb, uint64_t n, uint8_t *rgba);

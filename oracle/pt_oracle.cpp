@@ -23,6 +23,7 @@
 #include "oracle.h"
 #include "oracle_math.h"
 #include "oracle_pt.h"
+#include "oracle_quadric.h"
 #include "oracle_rng.h"
 #include "oracle_spectra.h"
 #include "oracle_tri.h"
@@ -465,6 +466,9 @@ struct Scene {
     std::vector<float> select_cdf;  // the cdf the NEE light choice searches (= light_cdf unless overridden)
     std::vector<uint32_t> infinite;  // indices of the infinite lights, in light order
     float inf_dist = 0;             // 2 x the scene's bounding-sphere radius
+    std::vector<quad::Quadric> quads;  // analytic primitives (spheres, disks, instanced triangles)
+    std::vector<uint32_t> prim_quad;   // per primitive: quads index + 1, 0 = triangle (empty: none)
+    std::vector<uint8_t> is_quad;      // the oracle BVH's custom-test flags
     Spectra *sp;
     PLSpectrum metals[8], glass_bk7;
     M3 out_from_sensor;
@@ -486,6 +490,111 @@ inline S4 light_L(const Scene &sc, const Light &L, V3 n, V3 w, const Lambda &lam
     S4 s;
     for (int i = 0; i < kNS; ++i) s.v[i] = L.rgb_scale * L.rsp(lam.l[i]);
     return (sample_pl(sc.sp->d65, lam) * s) * L.scale;
+}
+
+// Transform::is_identity (transform.h:219-239)
+inline bool xf_is_identity(const Xf &x) {
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k)
+            if (x.m.m[i][k] != x.inv.m[i][k] || x.m.m[i][k] != (i == k ? 1.f : 0.f)) return false;
+    return true;
+}
+// Transform::operator()(Bounds3f) (transform.h:323-349): translation, then per
+// output axis the smaller / larger of the two scaled input planes.
+inline B3 xf_bounds(const M4 &M, const B3 &b) {
+    B3 r;
+    for (int i = 0; i < 3; ++i) r.lo[i] = r.hi[i] = M.m[i][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const float a = M.m[i][k] * b.lo[k], c = M.m[i][k] * b.hi[k];
+            r.lo[i] += std::min(a, c);
+            r.hi[i] += std::max(a, c);
+        }
+    return r;
+}
+// SquareMatrix<3>::determinant (squared_matrix.cu:8-17) of the upper 3x3 < 0.
+inline bool xf_swaps_handedness(const M4 &M) {
+    const auto &m = M.m;
+    const float minor12 = dop(m[1][1], m[2][2], m[1][2], m[2][1]);
+    const float minor02 = dop(m[1][0], m[2][2], m[1][2], m[2][0]);
+    const float minor01 = dop(m[1][0], m[2][1], m[1][1], m[2][0]);
+    return std::fma(m[0][2], minor01, dop(m[0][0], minor12, m[0][1], minor02)) < 0;
+}
+
+// Entry i is an analytic shape or an instance's triangle (Shape::create,
+// shapes/sphere.cu:8-40, shapes/disk.cu:6-25; ObjectInstance, scene_builder.cu:775-810).
+// Its oracle-BVH entry is its bounds as the triangle (lo, hi, lo).
+void add_quadric(Scene &sc, const orc_scene &d, uint64_t i, const Xf &render_from_world) {
+    const orc_shape &sh = d.shapes[d.tri_shape[i]];
+    if (d.tri_shape[i] >= d.num_shapes) throw std::runtime_error("shape index out of range");
+    Xf ctm{m4_diag(1, 1, 1), m4_diag(1, 1, 1)};
+    if (sh.xform != 0xFFFFFFFFu) {
+        if (sh.xform >= d.num_xforms) throw std::runtime_error("shape transform index out of range");
+        std::memcpy(ctm.m.m, d.xforms + 16 * size_t(sh.xform), sizeof(ctm.m.m));
+        if (d.xforms_inv) std::memcpy(ctm.inv.m, d.xforms_inv + 16 * size_t(sh.xform), sizeof(ctm.inv.m));
+        else ctm.inv = m4_inverse(ctm.m);
+    }
+    Xf x;
+    Tri &t = sc.tris[i];
+    quad::Quadric q{};
+    q.type = sh.type;
+    q.reverse = t.flip;
+    if (sh.type == 1 || sh.type == 2) {
+        x = xf_mul(render_from_world, ctm);
+    } else if (sh.type == 3) {
+        if (t.light >= 0) throw std::runtime_error("area lights with object instancing");
+        x = xf_mul(xf_mul(render_from_world, ctm), xf_inverse(render_from_world));
+        if (xf_is_identity(x)) return;  // the definition's triangle itself
+    } else {
+        throw std::runtime_error("unknown shape type");
+    }
+    for (const M4 *mm : {&x.m, &x.inv})
+        if (mm->m[3][0] != 0 || mm->m[3][1] != 0 || mm->m[3][2] != 0 || mm->m[3][3] != 1)
+            throw std::runtime_error("projective shape transform");
+    std::memcpy(q.rfo.m, x.m.m, sizeof(q.rfo.m));
+    std::memcpy(q.rfo.inv, x.inv.m, sizeof(q.rfo.inv));
+    const float phi_max = float(double(kPi) / 180.0 * double(clampf(sh.phimax, 0, 360)));
+    B3 b;
+    if (sh.type == 1) {
+        const float r = sh.radius;
+        q.radius = r;
+        q.z_min = clampf(std::min(sh.zmin, sh.zmax), -r, r);
+        q.z_max = clampf(std::max(sh.zmin, sh.zmax), -r, r);
+        q.theta_z_min = float(std::acos(double(clampf(q.z_min / r, -1, 1))));
+        q.theta_z_max = float(std::acos(double(clampf(q.z_max / r, -1, 1))));
+        q.phi_max = phi_max;
+        q.flip = q.reverse;  // transform_swaps_handedness is never set by Sphere::init
+        b = xf_bounds(x.m, B3{{-r, -r, q.z_min}, {r, r, q.z_max}});
+    } else if (sh.type == 2) {
+        q.height = sh.height;
+        q.radius = sh.radius;
+        q.inner_radius = sh.innerradius;
+        q.phi_max = phi_max;
+        q.flip = q.reverse ^ xf_swaps_handedness(x.m);
+        b = xf_bounds(x.m, B3{{-sh.radius, -sh.radius, sh.height}, {sh.radius, sh.radius, sh.height}});
+    } else {
+        for (int v = 0; v < 3; ++v) q.p[v] = t.p[v];
+        q.flip = q.reverse;
+        b = xf_bounds(x.m, b3_of_tri(t.p[0], t.p[1], t.p[2]));
+    }
+    t.p[0] = b.lo, t.p[1] = b.hi, t.p[2] = b.lo;
+    for (int v = 0; v < 3; ++v)
+        for (int c = 0; c < 3; ++c) sc.soup[9 * i + 3 * v + c] = t.p[v][c];
+    if (sc.prim_quad.empty()) sc.prim_quad.assign(d.num_tris, 0);
+    sc.quads.push_back(q);
+    sc.prim_quad[i] = uint32_t(sc.quads.size());
+}
+
+int quadric_test(void *user, uint32_t prim, const float o[3], const float d[3], float t_max, float *t,
+                 float b3[3]) {
+    const Scene &sc = *static_cast<const Scene *>(user);
+    quad::QuadricHit h;
+    if (!quad::quadric_intersect(sc.quads[sc.prim_quad[prim] - 1], V3{o[0], o[1], o[2]}, V3{d[0], d[1], d[2]}, t_max,
+                                 &h, kPi))
+        return 0;
+    *t = h.t;
+    b3[0] = h.p.x, b3[1] = h.p.y, b3[2] = h.p.z;
+    return 1;
 }
 
 void build_scene(Scene &sc, const orc_scene &d) {
@@ -560,6 +669,7 @@ void build_scene(Scene &sc, const orc_scene &d) {
         if (t.material >= d.num_materials) throw std::runtime_error("material index out of range");
         t.light = d.tri_light ? d.tri_light[i] : -1;
         t.flip = d.tri_reverse ? d.tri_reverse[i] != 0 : false;
+        if (d.tri_shape && d.tri_shape[i] != 0xFFFFFFFFu) add_quadric(sc, d, i, sc.cam.render_from_world);
         if (t.light >= 0) {
             if (uint32_t(t.light) >= d.num_lights) throw std::runtime_error("light index out of range");
             t.light = int32_t(sc.lights.size());
@@ -627,6 +737,11 @@ void build_scene(Scene &sc, const orc_scene &d) {
     }
     orc_build_info info;
     if (orc_bvh_build(sc.soup.data(), d.num_tris, &sc.bvh, &info) != 0) throw std::runtime_error("oracle BVH build");
+    if (!sc.quads.empty()) {
+        sc.is_quad.assign(d.num_tris, 0);
+        for (uint64_t i = 0; i < d.num_tris; ++i) sc.is_quad[i] = sc.prim_quad[i] != 0;
+        orc_bvh_set_custom(sc.bvh, &quadric_test, &sc, sc.is_quad.data());
+    }
     // PowerLightSampler::create (power_light_sampler.cu:7-45) + Distribution1D::build.
     if (!sc.lights.empty()) {
         Lambda lam = sample_visible(0.5f);
@@ -642,7 +757,10 @@ void build_scene(Scene &sc, const orc_scene &d) {
                 phi = (sample_pl(sc.sp->d65, lam) * Ls) * (4 * kPi * kPi * (R * R) * l.scale);
             } else {
                 S4 L = (sample_pl(sc.sp->d65, lam) * Ls) * l.scale;
-                float k = kPi * float(l.two_sided ? 2 : 1) * tri_area(sc.tris[l.tri]);
+                const float area = sc.prim_quad.empty() || !sc.prim_quad[l.tri]
+                                       ? tri_area(sc.tris[l.tri])
+                                       : quad::quadric_area(sc.quads[sc.prim_quad[l.tri] - 1]);
+                float k = kPi * float(l.two_sided ? 2 : 1) * area;
                 phi = L * k;
             }
             w.push_back(average(safe_div(phi, pdf)));
@@ -702,6 +820,17 @@ inline V3 p3fi_mid(const P3fi &p) { return p3fi_point(p); }
 
 // shapes/triangle.cu:330-470 for meshes without normals or uv.
 inline Isect make_isect(const Scene &sc, uint32_t prim, const TriHit &h, V3 ray_d) {
+    if (!sc.prim_quad.empty() && sc.prim_quad[prim]) {
+        const quad::QSurf q = quad::quadric_surface(sc.quads[sc.prim_quad[prim] - 1], V3{h.b0, h.b1, h.b2});
+        Isect is;
+        is.pi = q.pi;
+        is.wo = normalize(neg(ray_d));
+        is.n = q.n;
+        is.ns = q.ns;
+        is.dpdu = q.dpdu;
+        is.tri = prim;
+        return is;
+    }
     const Tri &t = sc.tris[prim];
     V3 p0 = t.p[0], p1 = t.p[1], p2 = t.p[2];
     // uv = (0,0), (1,0), (1,1): duv02 = (-1,-1), duv12 = (0,-1), determinant 1.
@@ -1308,6 +1437,128 @@ inline float tri_pdf(const Tri &t, const LightCtx &ctx, V3 wi) {
     return pdf;
 }
 
+// ------------------------------------------------------------ sphere and disk emitters
+// Sphere::sample(u) (shapes/sphere.cu:152-181): uniform area sampling.
+inline void sphere_sample_area(const quad::Quadric &s, float u0, float u1, ShapeSample *ss) {
+    const float z = 1 - 2 * u0;  // sample_uniform_sphere (util/sampling.h:203-211)
+    const float r = safe_sqrt(1 - sqrf(z));
+    const float phi = 2 * kPi * u1;
+    const V3 w = {r * tcos(phi), r * tsin(phi), z};
+    V3 pObj = add(V3{0, 0, 0}, scale(s.radius, w));
+    pObj = scale(s.radius / std::sqrt(sqrf(pObj.x - 0) + sqrf(pObj.y - 0) + sqrf(pObj.z - 0)), pObj);
+    const V3 pObjError = scale(gammaf(5), vabs(pObj));
+    V3 n = normalize(quad::tf_normal(s.rfo.inv, pObj));
+    if (s.reverse) n = scale(-1.f, n);
+    ss->pi = quad::to_p3fi(quad::tf_point(s.rfo.m, quad::iv3_of(pObj, pObjError)));
+    ss->n = n;
+    ss->pdf = float(1) / quad::sphere_area(s);
+}
+inline float sqdist(V3 a, V3 b) { return sqrf(a.x - b.x) + sqrf(a.y - b.y) + sqrf(a.z - b.z); }
+// Sphere::sample(ctx, u) (shapes/sphere.cu:183-255).
+inline bool sphere_sample(const quad::Quadric &s, const LightCtx &ctx, float u0, float u1, ShapeSample *ss) {
+    const V3 pCenter = quad::tf_point(s.rfo.m, V3{0, 0, 0});
+    const V3 cp = p3fi_mid(ctx.pi);
+    const V3 pOrigin = ctx_offset_origin(ctx, sub(pCenter, cp));
+    if (sqdist(pOrigin, pCenter) <= sqrf(s.radius)) {
+        sphere_sample_area(s, u0, u1, ss);
+        const V3 sp = p3fi_mid(ss->pi);
+        V3 wi = sub(sp, cp);
+        if (sqlen(wi) == 0) return false;
+        wi = normalize(wi);
+        ss->pdf /= abs_dot(ss->n, neg(wi)) / sqdist(cp, sp);
+        return !std::isinf(ss->pdf);
+    }
+    const float sinThetaMax = s.radius / std::sqrt(sqdist(cp, pCenter));
+    const float sin2ThetaMax = sqrf(sinThetaMax);
+    const float cosThetaMax = safe_sqrt(1 - sin2ThetaMax);
+    float oneMinusCosThetaMax = 1 - cosThetaMax;
+    float cosTheta = (cosThetaMax - 1) * u0 + 1;
+    float sin2Theta = 1 - sqrf(cosTheta);
+    if (sin2ThetaMax < 0.00068523f /* sin^2(1.5 deg) */) {
+        sin2Theta = sin2ThetaMax * u0;
+        cosTheta = std::sqrt(1 - sin2Theta);
+        oneMinusCosThetaMax = sin2ThetaMax / 2;
+    }
+    const float cosAlpha = sin2Theta / sinThetaMax + cosTheta * safe_sqrt(1 - sin2Theta / sqrf(sinThetaMax));
+    const float sinAlpha = safe_sqrt(1 - sqrf(cosAlpha));
+    const float phi = u1 * 2 * kPi;
+    const V3 w = {clampf(sinAlpha, -1, 1) * tcos(phi), clampf(sinAlpha, -1, 1) * tsin(phi), clampf(cosAlpha, -1, 1)};
+    V3 fx, fy;
+    const V3 fz = normalize(sub(pCenter, cp));
+    coordinate_system(fz, &fx, &fy);  // Frame::from_z
+    const V3 mw = neg(w);
+    V3 n = add(add(scale(mw.x, fx), scale(mw.y, fy)), scale(mw.z, fz));
+    const V3 p = add(pCenter, scale(s.radius, n));
+    if (s.reverse) n = scale(-1.f, n);
+    ss->pi = make_p3fi(p, scale(gammaf(5), vabs(p)));
+    ss->n = n;
+    ss->pdf = 1 / (2 * kPi * oneMinusCosThetaMax);
+    return true;
+}
+// Disk::sample(u) and sample(ctx, u) (shapes/disk.cu:39-94).
+inline bool disk_sample(const quad::Quadric &s, const LightCtx &ctx, float u0, float u1, ShapeSample *ss) {
+    float dx = 0, dy = 0;  // sample_uniform_disk_concentric (util/sampling.h:61-83)
+    const float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f;
+    if (!(ox == 0 && oy == 0)) {
+        const float pi_over_4 = float(kPi / 4.0);
+        float r, theta;
+        if (std::fabs(ox) > std::fabs(oy)) {
+            r = ox;
+            theta = pi_over_4 * (oy / ox);
+        } else {
+            r = oy;
+            theta = float(kPi / 2.0) - pi_over_4 * (ox / oy);
+        }
+        dx = r * tcos(theta);
+        dy = r * tsin(theta);
+    }
+    const V3 pObj = {dx * s.radius, dy * s.radius, s.height};
+    ss->pi = quad::to_p3fi(quad::tf_point(s.rfo.m, quad::iv3_exact(pObj)));
+    V3 n = normalize(quad::tf_normal(s.rfo.inv, V3{0, 0, 1}));
+    if (s.reverse) n = scale(-1.f, n);
+    ss->n = n;
+    ss->pdf = float(1.0 / quad::disk_area(s));
+    const V3 cp = p3fi_mid(ctx.pi), sp = p3fi_mid(ss->pi);
+    V3 wi = sub(sp, cp);
+    if (sqlen(wi) == 0) return false;
+    wi = normalize(wi);
+    ss->pdf /= abs_dot(ss->n, neg(wi)) / sqdist(cp, sp);
+    return !std::isinf(ss->pdf);
+}
+// Sphere::pdf (shapes/sphere.cu:257-297), Disk::pdf (shapes/disk.cu:20-37).
+inline float quadric_pdf(const quad::Quadric &s, const LightCtx &ctx, V3 wi) {
+    const V3 cp = p3fi_mid(ctx.pi);
+    if (s.type == 1) {
+        const V3 pCenter = quad::tf_point(s.rfo.m, V3{0, 0, 0});
+        if (sqdist(ctx_offset_origin(ctx, sub(pCenter, cp)), pCenter) > sqrf(s.radius)) {
+            const float sin2ThetaMax = s.radius * s.radius / sqdist(cp, pCenter);
+            const float cosThetaMax = safe_sqrt(1 - sin2ThetaMax);
+            float oneMinusCosThetaMax = 1 - cosThetaMax;
+            if (sin2ThetaMax < 0.00068523f) oneMinusCosThetaMax = sin2ThetaMax / 2;
+            return float(1.0 / (2.0 * kPi * oneMinusCosThetaMax));
+        }
+    }
+    quad::QuadricHit h;
+    const V3 o = ctx_offset_origin(ctx, wi);
+    if (!quad::quadric_intersect(s, o, wi, std::numeric_limits<float>::infinity(), &h, kPi)) return 0;
+    const quad::QSurf g = quad::quadric_surface(s, h.p);
+    float pdf = float((1.0 / quad::quadric_area(s)) / (abs_dot(g.n, neg(wi)) / sqdist(cp, p3fi_mid(g.pi))));
+    if (std::isinf(pdf)) pdf = 0;
+    return pdf;
+}
+// DiffuseAreaLight::sample_li / pdf_li through the emitter's shape (diffuse_area_light.cu:48-74).
+inline bool emitter_sample(const Scene &sc, uint32_t prim, const LightCtx &ctx, float u0, float u1, ShapeSample *ss) {
+    if (!sc.prim_quad.empty() && sc.prim_quad[prim]) {
+        const quad::Quadric &q = sc.quads[sc.prim_quad[prim] - 1];
+        return q.type == 1 ? sphere_sample(q, ctx, u0, u1, ss) : disk_sample(q, ctx, u0, u1, ss);
+    }
+    return tri_sample(sc.tris[prim], ctx, u0, u1, ss);
+}
+inline float emitter_pdf(const Scene &sc, uint32_t prim, const LightCtx &ctx, V3 wi) {
+    if (!sc.prim_quad.empty() && sc.prim_quad[prim]) return quadric_pdf(sc.quads[sc.prim_quad[prim] - 1], ctx, wi);
+    return tri_pdf(sc.tris[prim], ctx, wi);
+}
+
 inline float power_heuristic(float f, float g) {  // sampling.h:259-269 with nf = ng = 1
     float ff = 1 * f, gg = 1 * g;
     if (std::isinf(sqrf(ff))) return 1;
@@ -1619,7 +1870,7 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                         p.L = p.L + p.beta * Le;
                         if (replay) gio->a[group_of(t.light)] += dotg(p.g, p.beta * Le);
                     } else {
-                        float pdf_light = sc.light_pmf[t.light] * tri_pdf(t, p.prev, p.rd);
+                        float pdf_light = sc.light_pmf[t.light] * emitter_pdf(sc, p.prim, p.prev, p.rd);
                         float wl = power_heuristic(p.pdf_bsdf, pdf_light);
                         p.L = p.L + (p.beta * wl) * Le;
                         if (replay) {  // emission with MIS: a += G (2w - 1), k += -2 G (1 - w)
@@ -1702,9 +1953,8 @@ void render(const orc_scene &d, int threads, float *film_rgb, float *aux, uint64
                     uint32_t li = search_cdf(u, sc.select_cdf.data(), uint32_t(sc.lights.size()));
                     float pmf = sc.light_pmf[li];
                     const Light &L = sc.lights[li];
-                    const Tri &lt = sc.tris[L.tri];
                     ShapeSample ss;
-                    bool ok = tri_sample(lt, ctx, ul0, ul1, &ss);
+                    bool ok = emitter_sample(sc, L.tri, ctx, ul0, ul1, &ss);
                     V3 cpt = p3fi_mid(ctx.pi);
                     V3 sp = p3fi_mid(ss.pi);
                     if (ok && !(ss.pdf == 0 || sqlen(sub(sp, cpt)) == 0)) {
@@ -1892,7 +2142,7 @@ S4 mk_path_li(const MkScene &m, V3 ro, V3 rd, Lambda &lam, Sampler &smp, MkCount
                 if (depth == 0 || specular) {
                     L = L + beta * Le;
                 } else {
-                    const float pdf_light = sc.light_pmf[t.light] * tri_pdf(t, prev, rd);
+                    const float pdf_light = sc.light_pmf[t.light] * emitter_pdf(sc, prim, prev, rd);
                     L = L + (beta * power_heuristic(pdf_bsdf, pdf_light)) * Le;
                 }
             }
@@ -1932,7 +2182,7 @@ S4 mk_path_li(const MkScene &m, V3 ro, V3 rd, Lambda &lam, Sampler &smp, MkCount
                 const uint32_t li = search_cdf(u, sc.select_cdf.data(), uint32_t(sc.lights.size()));
                 const Light &Lg = sc.lights[li];
                 ShapeSample ss;
-                const bool ok = tri_sample(sc.tris[Lg.tri], ctx, ul0, ul1, &ss);
+                const bool ok = emitter_sample(sc, Lg.tri, ctx, ul0, ul1, &ss);
                 const V3 cpt = p3fi_mid(ctx.pi), sp = p3fi_mid(ss.pi);
                 if (ok && !(ss.pdf == 0 || sqlen(sub(sp, cpt)) == 0)) {
                     const V3 wi = normalize(sub(sp, cpt));
@@ -2279,21 +2529,23 @@ static orc::Bsdf local_bsdf(const orc_bxdf &d) {
 
 uint32_t orc_bxdf_flags(const orc_bxdf *d) { return orc::bsdf_flags(local_bsdf(*d)); }
 
+static int test_threads() { return int(std::max(1u, std::min(16u, std::thread::hardware_concurrency()))); }
+
 int orc_bxdf_eval(const orc_bxdf *d, uint64_t n, const float *wo, const float *wi, float *f4, float *pdf) {
     const orc::Bsdf b = local_bsdf(*d);
-    for (uint64_t i = 0; i < n; ++i) {
+    orc::parallel(n, test_threads(), [&](size_t i) {
         const orc::V3 o{wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]}, w{wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]};
         const orc::S4 f = orc::bsdf_f(b, o, w);
         for (int k = 0; k < 4; ++k) f4[4 * i + k] = f.v[k];
         pdf[i] = orc::bsdf_pdf(b, o, w);
-    }
+    });
     return 0;
 }
 
 int orc_bxdf_sample(const orc_bxdf *d, uint64_t n, const float *wo, const float *u3, float *f4, float *wi,
                     float *pdf, float *eta, uint32_t *flags) {
     const orc::Bsdf b = local_bsdf(*d);
-    for (uint64_t i = 0; i < n; ++i) {
+    orc::parallel(n, test_threads(), [&](size_t i) {
         const orc::V3 o{wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]};
         orc::BS bs{};
         const bool ok = orc::bsdf_sample(b, o, u3[3 * i], u3[3 * i + 1], u3[3 * i + 2], &bs);
@@ -2304,7 +2556,7 @@ int orc_bxdf_sample(const orc_bxdf *d, uint64_t n, const float *wo, const float 
         wi[3 * i + 2] = ok ? bs.wi.z : 0.f;
         pdf[i] = ok ? bs.pdf : 0.f;
         eta[i] = ok ? bs.eta : 0.f;
-    }
+    });
     return 0;
 }
 
@@ -2354,6 +2606,104 @@ int orc_light_pmf(const orc_scene *d, float *pmf, int32_t *tri, uint32_t cap, ui
 }
 
 // SRGBColorEncoding::from_linear (spectrum_util/color_encoding.h:7-60).
+// ---- analytic shapes one at a time (tests): a shape with render_from_object m16
+// and its inverse inv16 (row-major), reverse = ReverseOrientation.
+static orc::quad::Quadric test_quadric(const orc_shape *sh, const float *m16, const float *inv16, int reverse) {
+    using namespace orc;
+    quad::Quadric q{};
+    q.type = sh->type;
+    std::memcpy(q.rfo.m, m16, sizeof(q.rfo.m));
+    std::memcpy(q.rfo.inv, inv16, sizeof(q.rfo.inv));
+    q.reverse = reverse != 0;
+    const float phi_max = float(double(kPi) / 180.0 * double(clampf(sh->phimax, 0, 360)));
+    const float r = sh->radius;
+    q.radius = r;
+    q.phi_max = phi_max;
+    if (sh->type == 1) {
+        q.z_min = clampf(std::min(sh->zmin, sh->zmax), -r, r);
+        q.z_max = clampf(std::max(sh->zmin, sh->zmax), -r, r);
+        q.theta_z_min = float(std::acos(double(clampf(q.z_min / r, -1, 1))));
+        q.theta_z_max = float(std::acos(double(clampf(q.z_max / r, -1, 1))));
+        q.flip = q.reverse;
+    } else {
+        q.height = sh->height;
+        q.inner_radius = sh->innerradius;
+        M4 m;
+        std::memcpy(m.m, m16, sizeof(m.m));
+        q.flip = q.reverse ^ xf_swaps_handedness(m);
+    }
+    return q;
+}
+
+/* Closest hit of each ray [n][8] with the shape: t (0 on a miss), the render-space hit
+ * point and normal; returns the hit count. */
+int orc_shape_intersect(const orc_shape *sh, const float *m16, const float *inv16, int reverse, const float *rays,
+                        uint64_t n, float *t_out, float *p_out, float *n_out, uint8_t *hit_out) {
+    using namespace orc;
+    const quad::Quadric q = test_quadric(sh, m16, inv16, reverse);
+    int hits = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const float *r = rays + 8 * i;
+        quad::QuadricHit h;
+        const bool hit = quad::quadric_intersect(q, V3{r[0], r[1], r[2]}, V3{r[4], r[5], r[6]}, r[3], &h, kPi);
+        hit_out[i] = hit;
+        t_out[i] = hit ? h.t : 0.f;
+        if (hit) {
+            const quad::QSurf g = quad::quadric_surface(q, h.p);
+            const V3 pm = p3fi_mid(g.pi);
+            for (int c = 0; c < 3; ++c) p_out[3 * i + c] = pm[c], n_out[3 * i + c] = g.n[c];
+            ++hits;
+        }
+    }
+    return hits;
+}
+
+/* Shape::sample(ctx, u) for reference points ctx [n][9] (p, n, ns) and u [n][2]:
+ * the sampled point, its normal and solid-angle pdf; ok = 0 where sampling fails. */
+int orc_shape_sample(const orc_shape *sh, const float *m16, const float *inv16, int reverse, const float *ctx,
+                     const float *u, uint64_t n, float *p_out, float *n_out, float *pdf_out, uint8_t *ok_out) {
+    using namespace orc;
+    const quad::Quadric q = test_quadric(sh, m16, inv16, reverse);
+    for (uint64_t i = 0; i < n; ++i) {
+        const float *c = ctx + 9 * i;
+        LightCtx lc{make_p3fi(V3{c[0], c[1], c[2]}, {0, 0, 0}), V3{c[3], c[4], c[5]}, V3{c[6], c[7], c[8]}};
+        ShapeSample ss;
+        const bool ok = q.type == 1 ? sphere_sample(q, lc, u[2 * i], u[2 * i + 1], &ss)
+                                    : disk_sample(q, lc, u[2 * i], u[2 * i + 1], &ss);
+        ok_out[i] = ok;
+        const V3 pm = ok ? p3fi_mid(ss.pi) : V3{0, 0, 0};
+        for (int k = 0; k < 3; ++k) p_out[3 * i + k] = pm[k], n_out[3 * i + k] = ok ? ss.n[k] : 0.f;
+        pdf_out[i] = ok ? ss.pdf : 0.f;
+    }
+    return 0;
+}
+
+/* Shape::pdf(ctx, wi) for reference points ctx [n][9] and directions wi [n][3]. */
+int orc_shape_pdf(const orc_shape *sh, const float *m16, const float *inv16, int reverse, const float *ctx,
+                  const float *wi, uint64_t n, float *pdf_out) {
+    using namespace orc;
+    const quad::Quadric q = test_quadric(sh, m16, inv16, reverse);
+    for (uint64_t i = 0; i < n; ++i) {
+        const float *c = ctx + 9 * i;
+        LightCtx lc{make_p3fi(V3{c[0], c[1], c[2]}, {0, 0, 0}), V3{c[3], c[4], c[5]}, V3{c[6], c[7], c[8]}};
+        pdf_out[i] = quadric_pdf(q, lc, V3{wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]});
+    }
+    return 0;
+}
+
+/* The directed-rounding primitives of the interval arithmetic: out = [down, up] of
+ * a * b (op 0), a / b (op 1), sqrt(a) (op 2), a + b (op 3). */
+int orc_directed(int op, float a, float b, float out[2]) {
+    using namespace orc::quad;
+    switch (op) {
+    case 0: out[0] = mul_rd(a, b), out[1] = mul_ru(a, b); return 0;
+    case 1: out[0] = div_rd(a, b), out[1] = div_ru(a, b); return 0;
+    case 2: out[0] = sqrt_rd(a), out[1] = sqrt_ru(a); return 0;
+    case 3: out[0] = orc::add_round_down(a, b), out[1] = orc::add_round_up(a, b); return 0;
+    }
+    return -1;
+}
+
 int orc_srgb8(const float *rgb, uint64_t n, uint8_t *rgba) {
     auto to_srgb = [](float v) -> float {
         if (v <= 0.0031308f) return 12.92f * v;

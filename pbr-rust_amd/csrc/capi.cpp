@@ -667,7 +667,7 @@ struct pbrtm_scene {
     int device = 0;
     pbrtm_bvh *bvh = nullptr;
     pbrtm::pt::SceneHost host;
-    pbrtm::DevBuf tri, tri_info, mats, lights, pmf, cdf, spec_lam, spec_val, filt;
+    pbrtm::DevBuf tri, tri_info, mats, lights, pmf, cdf, spec_lam, spec_val, filt, shapes, prim_shape;
     pbrtm::pt::SceneDev dev{};
     pbrtm::pt::PathPool pool;
     pbrtm::DevBuf film_rgb, film_aux;
@@ -681,7 +681,8 @@ void pbrtm_scene_free(pbrtm_scene *sc) {
     hipSetDevice(sc->device);
     if (sc->bvh) pbrtm_bvh_free(sc->bvh);
     for (pbrtm::DevBuf *b : {&sc->tri, &sc->tri_info, &sc->mats, &sc->lights, &sc->pmf, &sc->cdf, &sc->spec_lam,
-                             &sc->spec_val, &sc->filt, &sc->film_rgb, &sc->film_aux, &sc->g_lfin, &sc->g_pfin,
+                             &sc->spec_val, &sc->filt, &sc->shapes, &sc->prim_shape, &sc->film_rgb, &sc->film_aux,
+                             &sc->g_lfin, &sc->g_pfin,
                              &sc->g_target, &sc->g_adj, &sc->g_coef, &sc->g_acc, &sc->g_slot})
         b->release();
     auto &p = sc->pool;
@@ -729,6 +730,8 @@ int pbrtm_scene_create(pbrtm_ctx *ctx, const pbrtm_scene_desc *desc, pbrtm_scene
                           &H.filt_marg_cdf})
         filt.insert(filt.end(), v->begin(), v->end());
     if (e == hipSuccess) e = up(sc->filt, filt.data(), filt.size() * 4);
+    if (e == hipSuccess && !H.shapes.empty()) e = up(sc->shapes, H.shapes.data(), H.shapes.size() * sizeof(pbrtm::pt::DShape));
+    if (e == hipSuccess && !H.prim_shape.empty()) e = up(sc->prim_shape, H.prim_shape.data(), H.prim_shape.size() * 4);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return bail(hip_fail(ctx, e, "scene upload"));
     pbrtm::pt::SceneDev &D = sc->dev;
@@ -772,10 +775,19 @@ int pbrtm_scene_create(pbrtm_ctx *ctx, const pbrtm_scene_desc *desc, pbrtm_scene
     D.num_inf = H.num_inf;
     std::memcpy(D.inf_idx, H.inf_idx, sizeof(D.inf_idx));
     D.inf_dist = H.inf_dist;
-    // The aggregate over the render-space triangles (SceneBuilder::preprocess, scene_builder.cu:829-838).
+    D.shapes = H.shapes.empty() ? nullptr : sc->shapes.as<pbrtm::pt::DShape>();
+    D.prim_shape = H.shapes.empty() ? nullptr : sc->prim_shape.as<uint32_t>();
+    // The aggregate over the render-space primitives (SceneBuilder::preprocess,
+    // scene_builder.cu:829-838): triangles, and analytic primitives through their boxes.
     const int rc = pbrtm_bvh_build(ctx, sc->tri.as<float>(), 3 * desc->num_tris, nullptr, desc->num_tris,
                                    PBRTM_DEVICE_PTRS, &sc->bvh, nullptr);
     if (rc != PBRTM_OK) return bail(rc);
+    if (D.shapes) {
+        e = pbrtm::pt::tag_leaf_shapes(sc->bvh->tri_leaf.as<float>(), sc->bvh->dev.prim_order.as<uint32_t>(),
+                                       D.prim_shape, uint32_t(desc->num_tris), ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return bail(hip_fail(ctx, e, "shape tags"));
+    }
     *out = sc;
     return PBRTM_OK;
 }
@@ -794,6 +806,7 @@ WideTrace scene_trace(const pbrtm_scene *sc) {
     W.num_nodes = bvh->wide.num_nodes;
     W.num_prims = bvh->n;
     W.max_depth = bvh->wide.max_depth;
+    W.shapes = sc->dev.shapes;
     W.spill_per_ray = W.max_depth > kLdsStack ? W.max_depth - kLdsStack + 1 : 0;
     for (int a = 0; a < 3; ++a) {
         W.lo[a] = bvh->bounds[a];

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "pbrtm_internal.h"
+#include "pbrtm_shapes.h"
 #include "wide_bvh.h"
 
 namespace pbrtm {
@@ -69,6 +70,8 @@ struct SceneDev {
     float sn_c[9];   // surface normal: sigmoid coefficients of RGBAlbedoSpectrum(0.01 e_c), c = r, g, b
     uint32_t num_inf, inf_idx[kMaxInfinite];  // the infinite lights' indices into lights, in light order
     float inf_dist;  // 2 x the scene's bounding-sphere radius (UniformInfiniteLight::sample_li)
+    const DShape *shapes;        // analytic primitives (spheres, disks, instanced triangles); null if none
+    const uint32_t *prim_shape;  // per primitive: index into shapes + 1, 0 = a triangle; null if none
 };
 
 // Integrators render() runs (SceneBuilder::build_integrator, scene_builder.cu:289-313).
@@ -116,6 +119,8 @@ struct SceneHost {
     float sn_c[9];
     uint32_t num_inf, inf_idx[kMaxInfinite];
     float inf_dist;
+    std::vector<DShape> shapes;      // analytic primitives
+    std::vector<uint32_t> prim_shape;  // per primitive: shape index + 1, 0 = triangle (empty: no shapes)
     std::vector<float> mat_rgb;      // per material [3]: the diffuse albedo RGB (gradients)
     std::vector<float> group_scale;  // per emitter: the scene's "scale" parameter (gradients)
 };
@@ -132,6 +137,8 @@ void rgb_to_sigmoid_jacobian(const float rgb[3], double J[9]);
 // out[m] = J(rgb[m])^T coef[m] for n RGBs (multi-threaded).
 void rgb_to_sigmoid_jacobian_transpose(const float *rgb, const float *coef, size_t n, float *out);
 void srgb_matrices(float xyz_from_rgb[9], float rgb_from_xyz[9], float white_xy[2]);
+// SquareMatrix<4>::inverse (euclidean_space/squared_matrix.cu), row-major; false if singular.
+bool mat4_inverse_ref(const float in[16], float out[16]);
 
 struct PathPool {
     DevBuf path_id, flags, rng, dim, lam, lam_pdf, L, beta, nee_L, scal, prev, rays, prim, thit, bary, srays, sprim,
@@ -155,6 +162,11 @@ hipError_t render(const SceneDev &S, const WideTrace &W, uint32_t spp, uint32_t 
                   PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
                   hipStream_t stream, float4 *lfin = nullptr, float4 *pfin = nullptr,
                   const GradDev *gd = nullptr, uint32_t integrator = kIntPath);
+
+// Marks the leaf positions of analytic primitives for the traversal: the word after
+// a position's nine coordinates receives prim_shape[prim_order[k]] (shape index + 1).
+hipError_t tag_leaf_shapes(float *tri_leaf, const uint32_t *prim_order, const uint32_t *prim_shape, uint32_t n,
+                           hipStream_t stream);
 
 // Loss sum((film - target)^2) per pixel into loss_px, and adj [h][w][3] =
 // dLoss/d(rgb_sum / weight_sum) through out_from_sensor (divided by the

@@ -11,6 +11,7 @@
 // keep their film contribution per path id and the film kernel adds them per
 // pixel in the reference's order (round, then sample).
 #include "pathtrace.h"
+#include "pbrtm_shapes.h"
 #include "pt_device.h"
 #include "wave_ops.h"
 
@@ -294,6 +295,55 @@ __device__ __forceinline__ void spawn_ray_to(const P3i &from, f3 n_from, const P
     *d = sub3(pt, pf);
 }
 
+// The surface interaction at a closest hit: a triangle (shapes/triangle.cu:330-470,
+// default uv, no shading normals), or an analytic primitive of S.shapes (spheres,
+// disks, instanced triangles), whose hit point or base barycentrics the traversal
+// left in the barycentric slots.
+__device__ __forceinline__ Surf surface_at(const SceneDev &S, uint32_t prim, const TriHit &th) {
+    if (S.prim_shape) {
+        const uint32_t k = S.prim_shape[prim];
+        if (k) return shape_surf(S.shapes[k - 1], f3{th.b0, th.b1, th.b2});
+    }
+    const Tri tri = load_tri(S, prim);
+    const f3 dp02 = sub3(tri.p0, tri.p2), dp12 = sub3(tri.p1, tri.p2);
+    f3 dpdu = {dop(-1.f, dp02.x, -1.f, dp12.x), dop(-1.f, dp02.y, -1.f, dp12.y), dop(-1.f, dp02.z, -1.f, dp12.z)};
+    f3 dpdv = {dop(-1.f, dp12.x, 0.f, dp02.x), dop(-1.f, dp12.y, 0.f, dp02.y), dop(-1.f, dp12.z, 0.f, dp02.z)};
+    dpdu = mul3(1.0f, dpdu);
+    dpdv = mul3(1.0f, dpdv);
+    if (sqlen3(cross3(dpdu, dpdv)) == 0) basis(norm3(cross3(sub3(tri.p2, tri.p0), sub3(tri.p1, tri.p0))), &dpdu, &dpdv);
+    const HitGeom g = hit_geom(th, tri.p0, tri.p1, tri.p2);
+    const f3 n = tri.flip ? neg3(g.n) : g.n;
+    return Surf{P3i{g.x, g.y, g.z}, n, n, dpdu};
+}
+
+// DiffuseAreaLight::pdf_li (lights/diffuse_area_light.cu:69-74) through the emitter's shape.
+__device__ __forceinline__ float prim_light_pdf(const SceneDev &S, uint32_t prim, const P3i &cpi, f3 cn, f3 cns,
+                                                f3 wi) {
+    if (S.prim_shape) {
+        const uint32_t k = S.prim_shape[prim];
+        if (k) return shape_light_pdf(S.shapes[k - 1], cpi, cn, wi);
+    }
+    return tri_pdf(load_tri(S, prim), cpi, cn, cns, wi);
+}
+
+// Shape::sample(ctx, u) of the emitter's shape (DiffuseAreaLight::sample_li,
+// lights/diffuse_area_light.cu:48-67).
+__device__ __forceinline__ bool prim_light_sample(const SceneDev &S, uint32_t prim, const P3i &cpi, f3 cn, f3 cns,
+                                                  float u0, float u1, ShapeSample *ss) {
+    if (S.prim_shape) {
+        const uint32_t k = S.prim_shape[prim];
+        if (k) {
+            QSample q;
+            if (!shape_light_sample(S.shapes[k - 1], cpi, cn, u0, u1, &q)) return false;
+            ss->pi = q.pi;
+            ss->n = q.n;
+            ss->pdf = q.pdf;
+            return true;
+        }
+    }
+    return tri_sample(load_tri(S, prim), cpi, cns, u0, u1, ss);
+}
+
 __device__ __forceinline__ void store_ray(float4 *r, f3 o, float tmax, f3 d) {
     r[0] = make_float4(o.x, o.y, o.z, tmax);
     r[1] = make_float4(d.x, d.y, d.z, 0.f);
@@ -476,7 +526,7 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
     }
     // Emission with MIS at this vertex (wavefront_path.cu:179-196,
     // megakernel_path.cu:81-99).
-    auto add_emission = [&](const Tri &tri, uint32_t light, f3 n, f3 rd) {
+    auto add_emission = [&](uint32_t prim, uint32_t light, f3 n, f3 rd) {
         const DLight Ld = S.lights[light - 1];
         const S4 Le = light_L(S, Ld, n, neg3(rd), lam);
         if (!s4pos(Le)) return;
@@ -487,7 +537,7 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
             const float4 q0 = st.prev[3 * size_t(i)], q1 = st.prev[3 * size_t(i) + 1], q2 = st.prev[3 * size_t(i) + 2];
             const P3i ppi{{q0.x, q0.y}, {q0.z, q0.w}, {q1.x, q1.y}};
             const f3 pn = {q1.z, q1.w, q2.x}, pns = {q2.y, q2.z, q2.w};
-            const float pdf_light = S.light_pmf[light - 1] * tri_pdf(tri, ppi, pn, pns, rd);
+            const float pdf_light = S.light_pmf[light - 1] * prim_light_pdf(S, prim, ppi, pn, pns, rd);
             const float wl = power_heuristic(scal.y, pdf_light);
             L = s4add(L, s4mul(s4scale(beta, wl), Le));
             if (GRAD) {  // MIS weight attached: dC/dpmf = -2 C (1 - w) / pmf
@@ -502,13 +552,11 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
     // sample, no BSDF sample. get_bsdf of a dielectric with a spectral eta still
     // ends the secondary wavelengths (dielectric_material.cu:47-50).
     if (MODE == kIntMegakernelPath && live && !terminate && depth == max_depth) {
-        const Tri tri = load_tri(S, prim);
         const uint2 info = S.tri_info[prim];
         const TriHit th{st.bary[3 * size_t(i)], st.bary[3 * size_t(i) + 1], st.bary[3 * size_t(i) + 2], st.thit[i]};
-        const HitGeom g = hit_geom(th, tri.p0, tri.p1, tri.p2);
         const float4 r1 = st.rays[2 * size_t(i) + 1];
         const uint32_t light = info.y & 0x7FFFFFFFu;
-        if (light) add_emission(tri, light, tri.flip ? neg3(g.n) : g.n, f3{r1.x, r1.y, r1.z});
+        if (light) add_emission(prim, light, surface_at(S, prim, th).n, f3{r1.x, r1.y, r1.z});
         const DMat m = S.mats[info.x];
         if (m.type == kDielectric && m.eta_n && !(lpdf.v[1] == 0 && lpdf.v[2] == 0 && lpdf.v[3] == 0)) {
             lpdf.v[1] = lpdf.v[2] = lpdf.v[3] = 0;
@@ -565,20 +613,11 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
     if (!live) return;
     const float4 r1 = st.rays[2 * size_t(i) + 1];
     const f3 rd = {r1.x, r1.y, r1.z};
-    const Tri tri = load_tri(S, prim);
     const uint2 info = S.tri_info[prim];
-    // surface interaction (shapes/triangle.cu:330-470, default uv, no shading normals)
-    const f3 dp02 = sub3(tri.p0, tri.p2), dp12 = sub3(tri.p1, tri.p2);
-    f3 dpdu = {dop(-1.f, dp02.x, -1.f, dp12.x), dop(-1.f, dp02.y, -1.f, dp12.y), dop(-1.f, dp02.z, -1.f, dp12.z)};
-    f3 dpdv = {dop(-1.f, dp12.x, 0.f, dp02.x), dop(-1.f, dp12.y, 0.f, dp02.y), dop(-1.f, dp12.z, 0.f, dp02.z)};
-    dpdu = mul3(1.0f, dpdu);
-    dpdv = mul3(1.0f, dpdv);
-    if (sqlen3(cross3(dpdu, dpdv)) == 0) basis(norm3(cross3(sub3(tri.p2, tri.p0), sub3(tri.p1, tri.p0))), &dpdu, &dpdv);
     const TriHit th{st.bary[3 * size_t(i)], st.bary[3 * size_t(i) + 1], st.bary[3 * size_t(i) + 2], st.thit[i]};
-    const HitGeom g = hit_geom(th, tri.p0, tri.p1, tri.p2);
-    const P3i ipi{g.x, g.y, g.z};
-    const f3 n = tri.flip ? neg3(g.n) : g.n;
-    const f3 ns = n;
+    const Surf sf = surface_at(S, prim, th);
+    const P3i ipi = sf.pi;
+    const f3 n = sf.n, ns = sf.ns, dpdu = sf.dpdu;
     if (MODE == kIntAmbientOcclusion || MODE == kIntSurfaceNormal) {
         // One vertex, no emission; the path ends next round (its ray is retired).
         const f3 nf = dot3(ns, neg3(rd)) < 0.0f ? neg3(ns) : ns;  // Vector3::face_forward
@@ -632,7 +671,7 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
     }
     const f3 wo_n = norm3(neg3(rd));
     const uint32_t light = info.y & 0x7FFFFFFFu;
-    if (light) add_emission(tri, light, n, rd);
+    if (light) add_emission(prim, light, n, rd);
     fl = (fl & ~kDepthMask) | ((depth + 1) & kDepthMask);
     // 3. material: BSDF (interaction.cu:77-89, bsdf.cu), sample_ld, sample_f
     const DMat m = S.mats[info.x];
@@ -714,7 +753,7 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
                 Le = light_L(S, Lg, ss.n, ss.n, lam);
                 ok = s4pos(Le);
             } else {
-                ok = tri_sample(load_tri(S, Lg.tri), cpi, ns, ul0, ul1, &ss);
+                ok = prim_light_sample(S, Lg.tri, cpi, n, ns, ul0, ul1, &ss);
                 const f3 sp = p3i_mid(ss.pi);
                 ok = ok && !(ss.pdf == 0 || sqlen3(sub3(sp, cpt)) == 0);
                 if (ok) {
@@ -1168,6 +1207,22 @@ State make_state(PathPool &pp, uint32_t *err) {
 
 }  // namespace
 
+__global__ __launch_bounds__(256) void tag_leaf_kernel(float *__restrict__ tri_leaf,
+                                                       const uint32_t *__restrict__ prim_order,
+                                                       const uint32_t *__restrict__ prim_shape, uint32_t n) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t p = prim_order[k];
+    tri_leaf[12 * size_t(k) + 9] = __uint_as_float(p < n ? prim_shape[p] : 0u);
+}
+
+hipError_t tag_leaf_shapes(float *tri_leaf, const uint32_t *prim_order, const uint32_t *prim_shape, uint32_t n,
+                           hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    tag_leaf_kernel<<<(n + 255) / 256, 256, 0, stream>>>(tri_leaf, prim_order, prim_shape, n);
+    return hipGetLastError();
+}
+
 hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t max_depth, uint32_t pool,
                   PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
                   hipStream_t stream, float4 *lfin, float4 *pfin, const GradDev *gd, uint32_t integrator) {
@@ -1400,6 +1455,7 @@ hipError_t render_shared(const std::vector<BatchJob> &jobs, uint32_t pool, uint3
     PT_CK(reserve_ray_sort(pp.rsort, 2 * pool));  // the captured rounds allocate nothing
     WideTrace W{};
     W.spill_per_ray = spill_need;
+    for (uint32_t j = 0; j < nj && !W.shapes; ++j) W.shapes = jw[j].shapes;  // the analytic-primitive kernels
     if (spill_need) {
         PT_CK(pp.spill.reserve(size_t(pool) * 2 * spill_need * sizeof(uint2)));
         W.spill = pp.spill.as<uint2>();

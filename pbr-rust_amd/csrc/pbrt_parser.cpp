@@ -8,7 +8,8 @@
 // Rotate, Transform, ConcatTransform, Identity, CoordSysTransform, ReverseOrientation,
 // Material / MakeNamedMaterial / NamedMaterial ("diffuse", "conductor",
 // "dielectric"), AreaLightSource "diffuse", LightSource "infinite" (uniform),
-// Shape "trianglemesh", Include.
+// Shape "trianglemesh" / "sphere" / "disk", ObjectBegin / ObjectEnd /
+// ObjectInstance (triangle meshes), Include.
 // Anything else that changes the image is reported as unsupported rather than
 // silently approximated.
 #include <cctype>
@@ -24,6 +25,7 @@
 #include <vector>
 
 #include "../../include/pbrtm.h"
+#include "pathtrace.h"
 
 namespace {
 
@@ -160,8 +162,19 @@ struct Material {
     pbrtm_material m;
 };
 
+M4 transpose(const M4 &a) {
+    M4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r.m[4 * i + j] = a.m[4 * j + i];
+    return r;
+}
+
+// The CTM keeps its inverse as the reference's Transform does (m, inv_m): products
+// compose inverses in reverse order; Translate, Scale and Rotate have their own
+// (transform.h:61-132), a general matrix the 4x4 inverse (squared_matrix.cu).
 struct GState {
     M4 ctm = ident();
+    M4 ctm_inv = ident();
     bool ctm_identity = true;
     int material = -1;  // index into materials; -1 = the default diffuse 0.5
     bool has_light = false;
@@ -175,8 +188,9 @@ constexpr float kPiF = 3.14159274101257324f;
 
 struct pbrtm_pbrt_file {
     pbrtm_scene_desc desc{};
-    std::vector<float> tri, xforms;
-    std::vector<uint32_t> tri_material, tri_xform;
+    std::vector<float> tri, xforms, xforms_inv;
+    std::vector<uint32_t> tri_material, tri_xform, tri_shape;
+    std::vector<pbrtm_shape> shapes;
     std::vector<int32_t> tri_light;
     std::vector<uint8_t> tri_reverse;
     std::vector<pbrtm_material> materials;
@@ -219,7 +233,11 @@ class Builder {
         d.lights = F.lights.empty() ? nullptr : F.lights.data();
         d.num_xforms = uint32_t(F.xforms.size() / 16);
         d.xforms = F.xforms.empty() ? nullptr : F.xforms.data();
+        d.xforms_inv = F.xforms_inv.empty() ? nullptr : F.xforms_inv.data();
         d.tri_xform = F.tri_xform.data();
+        d.num_shapes = uint32_t(F.shapes.size());
+        d.shapes = F.shapes.empty() ? nullptr : F.shapes.data();
+        d.tri_shape = F.shapes.empty() ? nullptr : F.tri_shape.data();
         d.num_infinite_lights = uint32_t(F.infinite_lights.size());
         d.infinite_lights = F.infinite_lights.empty() ? nullptr : F.infinite_lights.data();
         const int s = int(std::sqrt(double(spp_asked)));  // stratified: spp rounded to a square (scene_builder.cu:283-287)
@@ -232,7 +250,7 @@ class Builder {
     GState gs;
     std::vector<GState> stack;
     std::map<std::string, int> named_materials;
-    std::map<std::string, M4> named_coords;
+    std::map<std::string, std::pair<M4, M4>> named_coords;
     bool world = false, have_film = false, camera_set = false, lookat_only = true;
     float eye[3] = {0, 0, 0}, look[3] = {0, 0, 1}, up[3] = {0, 1, 0};
     float fov = 90.f;
@@ -284,11 +302,51 @@ class Builder {
         return T[i++].s;
     }
 
-    void concat(const M4 &t) {
+    void concat(const M4 &t, const M4 &t_inv) {
         gs.ctm = mul(gs.ctm, t);
+        gs.ctm_inv = mul(t_inv, gs.ctm_inv);
         gs.ctm_identity = false;
         if (!world) lookat_only = false;
     }
+    static M4 inverse_of(const M4 &m) {
+        M4 r;
+        if (!pbrtm::pt::mat4_inverse_ref(m.m, r.m)) throw std::runtime_error("singular transform matrix");
+        return r;
+    }
+    // The CTM as a transform index (UINT32_MAX = identity).
+    uint32_t ctm_index() {
+        if (gs.ctm_identity) return 0xFFFFFFFFu;
+        const uint32_t xf = uint32_t(F.xforms.size() / 16);
+        F.xforms.insert(F.xforms.end(), gs.ctm.m, gs.ctm.m + 16);
+        F.xforms_inv.insert(F.xforms_inv.end(), gs.ctm_inv.m, gs.ctm_inv.m + 16);
+        return xf;
+    }
+    // One primitive slot (the triangle list orders the aggregate and the lights).
+    void add_slot(const float *v9, uint32_t mat, uint32_t xf, uint32_t shape) {
+        F.tri.insert(F.tri.end(), v9, v9 + 9);
+        F.tri_material.push_back(mat);
+        F.tri_xform.push_back(xf);
+        F.tri_reverse.push_back(gs.reverse ? 1 : 0);
+        F.tri_shape.push_back(shape);
+        if (gs.has_light) {
+            F.tri_light.push_back(int32_t(F.lights.size()));
+            F.lights.push_back(gs.light);
+        } else {
+            F.tri_light.push_back(-1);
+        }
+    }
+    // A triangle of an ObjectBegin block, kept until ObjectInstance.
+    struct ProtoTri {
+        float v[9];
+        uint32_t mat, xf;
+        bool reverse;
+    };
+    struct Definition {
+        std::vector<ProtoTri> tris;
+    };
+    std::map<std::string, Definition> definitions;
+    Definition *active_def = nullptr;
+    std::string active_name;
 
     pbrtm_material make_material(const std::string &type, Params &P, const Tok &at) {
         pbrtm_material m{};
@@ -358,6 +416,29 @@ class Builder {
     int default_mat = -1;
 
     void shape(const std::string &type, Params &P, const Tok &at) {
+        if (type == "sphere" || type == "disk") {
+            // Sphere::create (shapes/sphere.cu:8-22), Disk::create (shapes/disk.cu:6-25)
+            if (active_def) throw std::runtime_error(where(at) + "only triangle meshes are supported in ObjectBegin");
+            pbrtm_shape q{};
+            q.radius = P.get_float("radius", 1.f);
+            q.phimax = P.get_float("phimax", 360.f);
+            if (type == "sphere") {
+                q.type = PBRTM_SHAPE_SPHERE;
+                q.zmin = P.get_float("zmin", -q.radius);
+                q.zmax = P.get_float("zmax", q.radius);
+            } else {
+                q.type = PBRTM_SHAPE_DISK;
+                q.height = P.get_float("height", 0.f);
+                q.innerradius = P.get_float("innerradius", 0.f);
+            }
+            P.check_used("Shape \"" + type + "\"", {"alpha"});
+            q.xform = ctm_index();
+            F.shapes.push_back(q);
+            const float zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            add_slot(zero, uint32_t(gs.material >= 0 ? gs.material : default_material()), q.xform,
+                     uint32_t(F.shapes.size() - 1));
+            return;
+        }
         if (type != "trianglemesh") throw std::runtime_error(where(at) + "shape \"" + type + "\" is not supported");
         Param *idx = P.find("indices", "integer");
         Param *pts = P.find("P", "point3");
@@ -374,28 +455,47 @@ class Builder {
         if (ind.size() % 3) throw std::runtime_error(where(at) + "indices are not a triangle list");
         P.check_used("Shape \"trianglemesh\"");
         const long nv = long(pts->nums.size() / 3);
-        uint32_t xf = 0xFFFFFFFFu;
-        if (!gs.ctm_identity) {
-            xf = uint32_t(F.xforms.size() / 16);
-            F.xforms.insert(F.xforms.end(), gs.ctm.m, gs.ctm.m + 16);
-        }
+        if (active_def && gs.has_light)
+            throw std::runtime_error(where(at) + "area lights are not supported with object instancing");
+        const uint32_t xf = ctm_index();
         const uint32_t mat = uint32_t(gs.material >= 0 ? gs.material : default_material());
         for (size_t t = 0; t < ind.size(); t += 3) {
+            float v9[9];
             for (int v = 0; v < 3; ++v) {
                 const long k = ind[t + v];
                 if (k < 0 || k >= nv) throw std::runtime_error(where(at) + "vertex index out of range");
-                for (int c = 0; c < 3; ++c) F.tri.push_back(float(pts->nums[3 * k + c]));
+                for (int c = 0; c < 3; ++c) v9[3 * v + c] = float(pts->nums[3 * k + c]);
             }
-            F.tri_material.push_back(mat);
-            F.tri_xform.push_back(xf);
-            F.tri_reverse.push_back(gs.reverse ? 1 : 0);
-            if (gs.has_light) {
-                F.tri_light.push_back(int32_t(F.lights.size()));
-                F.lights.push_back(gs.light);
+            if (active_def) {
+                ProtoTri p;
+                std::memcpy(p.v, v9, sizeof(v9));
+                p.mat = mat, p.xf = xf, p.reverse = gs.reverse;
+                active_def->tris.push_back(p);
             } else {
-                F.tri_light.push_back(-1);
+                add_slot(v9, mat, xf, 0xFFFFFFFFu);
             }
         }
+    }
+
+    // ObjectInstance (scene_builder.cu:775-810): the definition's primitives with
+    // render_from_instance = render_from_world * CTM * world_from_render; the library
+    // adds them as they are when that is the identity, else as TransformedPrimitives.
+    void instance(const std::string &name, const Tok &at) {
+        auto it = definitions.find(name);
+        if (it == definitions.end()) throw std::runtime_error(where(at) + "object \"" + name + "\" not found");
+        if (it->second.tris.empty()) return;
+        pbrtm_shape q{};
+        q.type = PBRTM_SHAPE_INSTANCE;
+        q.xform = ctm_index();
+        F.shapes.push_back(q);
+        const uint32_t sk = uint32_t(F.shapes.size() - 1);
+        const GState saved = gs;
+        gs.has_light = false;
+        for (const ProtoTri &p : it->second.tris) {
+            gs.reverse = p.reverse;
+            add_slot(p.v, p.mat, p.xf, sk);
+        }
+        gs = saved;
     }
 
     void run(const std::vector<Tok> &T, const std::string &dir) {
@@ -450,15 +550,16 @@ class Builder {
                     throw std::runtime_error(where(t) + "only the default Mitchell filter is supported");
             } else if (k == "WorldBegin") {
                 world = true;
-                gs.ctm = ident();
+                gs.ctm = gs.ctm_inv = ident();
                 gs.ctm_identity = true;
-                named_coords["world"] = gs.ctm;
+                named_coords["world"] = {gs.ctm, gs.ctm_inv};
             } else if (k == "AttributeBegin" || k == "TransformBegin") {
                 stack.push_back(gs);
             } else if (k == "AttributeEnd" || k == "TransformEnd") {
                 if (stack.empty()) throw std::runtime_error(where(t) + "unmatched " + k);
                 if (k == "TransformEnd") {
                     gs.ctm = stack.back().ctm;
+                    gs.ctm_inv = stack.back().ctm_inv;
                     gs.ctm_identity = stack.back().ctm_identity;
                 } else {
                     gs = stack.back();
@@ -466,14 +567,16 @@ class Builder {
                 stack.pop_back();
             } else if (k == "Translate") {
                 const auto v = read_nums(T, i, 3);
-                M4 m = ident();
+                M4 m = ident(), mi = ident();
                 m.m[3] = float(v[0]), m.m[7] = float(v[1]), m.m[11] = float(v[2]);
-                concat(m);
+                mi.m[3] = -float(v[0]), mi.m[7] = -float(v[1]), mi.m[11] = -float(v[2]);
+                concat(m, mi);
             } else if (k == "Scale") {
                 const auto v = read_nums(T, i, 3);
-                M4 m = ident();
+                M4 m = ident(), mi = ident();
                 m.m[0] = float(v[0]), m.m[5] = float(v[1]), m.m[10] = float(v[2]);
-                concat(m);
+                mi.m[0] = float(1.0) / float(v[0]), mi.m[5] = float(1.0) / float(v[1]), mi.m[10] = float(1.0) / float(v[2]);
+                concat(m, mi);
             } else if (k == "Rotate") {
                 // Transform::rotate (transform.h:61-88); sin and cos rounded from double
                 const auto v = read_nums(T, i, 4);
@@ -492,7 +595,7 @@ class Builder {
                 m.m[8] = float(ax * az * (1.0 - c) - ay * s);
                 m.m[9] = float(ay * az * (1.0 - c) + ax * s);
                 m.m[10] = float(az * az + (1.0 - az * az) * c);
-                concat(m);
+                concat(m, transpose(m));
             } else if (k == "Transform" || k == "ConcatTransform") {
                 const auto v = read_nums(T, i, 16);
                 M4 m;  // the file lists the matrix column-major (scene_builder.cu: transpose)
@@ -500,22 +603,24 @@ class Builder {
                     for (int c = 0; c < 4; ++c) m.m[4 * r + c] = float(v[4 * c + r]);
                 if (k == "Transform") {
                     gs.ctm = m;
+                    gs.ctm_inv = inverse_of(m);
                     gs.ctm_identity = false;
                     if (!world) lookat_only = false;
                 } else {
-                    concat(m);
+                    concat(m, inverse_of(m));
                 }
             } else if (k == "Identity") {
-                gs.ctm = ident();
+                gs.ctm = gs.ctm_inv = ident();
                 gs.ctm_identity = true;
             } else if (k == "CoordSysTransform") {
                 const std::string name = read_str(T, i, "CoordSysTransform");
                 auto it = named_coords.find(name);
                 if (it == named_coords.end()) throw std::runtime_error(where(t) + "coordinate system \"" + name + "\" not available");
-                gs.ctm = it->second;
+                gs.ctm = it->second.first;
+                gs.ctm_inv = it->second.second;
                 gs.ctm_identity = name == "world";
             } else if (k == "CoordinateSystem") {
-                named_coords[read_str(T, i, "CoordinateSystem")] = gs.ctm;
+                named_coords[read_str(T, i, "CoordinateSystem")] = {gs.ctm, gs.ctm_inv};
             } else if (k == "ReverseOrientation") {
                 gs.reverse = !gs.reverse;
             } else if (k == "Material") {
@@ -580,6 +685,24 @@ class Builder {
                 Params P = read_params(T, i);
                 if (!world) throw std::runtime_error(where(t) + "Shape before WorldBegin");
                 shape(type, P, t);
+            } else if (k == "ObjectBegin") {
+                // scene_builder.cu:742-756: the graphics state is pushed, shapes go to the definition
+                const std::string name = read_str(T, i, "ObjectBegin");
+                if (active_def) throw std::runtime_error(where(t) + "ObjectBegin called inside of instance definition");
+                stack.push_back(gs);
+                active_name = name;
+                definitions[name] = Definition{};
+                active_def = &definitions[name];
+            } else if (k == "ObjectEnd") {
+                if (!active_def) throw std::runtime_error(where(t) + "ObjectEnd called before an instance defined");
+                active_def = nullptr;
+                if (stack.empty()) throw std::runtime_error(where(t) + "unmatched ObjectEnd");
+                gs = stack.back();
+                stack.pop_back();
+            } else if (k == "ObjectInstance") {
+                const std::string name = read_str(T, i, "ObjectInstance");
+                if (active_def) throw std::runtime_error(where(t) + "ObjectInstance inside of instance definition");
+                instance(name, t);
             } else if (k == "Include" || k == "Import") {
                 const std::string f = read_str(T, i, "Include");
                 parse_file(f.size() && f[0] == '/' ? f : dir + f);

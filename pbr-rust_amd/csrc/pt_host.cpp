@@ -588,11 +588,118 @@ void rgb_to_sigmoid(const float rgb[3], float out[3]) {
     out[0] = c[0], out[1] = c[1], out[2] = c[2];
 }
 
+bool mat4_inverse_ref(const float in[16], float out[16]) {
+    Mat4 m;
+    std::memcpy(m.m, in, sizeof(m.m));
+    try {
+        const Mat4 r = mat4_inv(m);
+        std::memcpy(out, r.m, sizeof(r.m));
+    } catch (const std::exception &) {
+        return false;
+    }
+    return true;
+}
+
 void srgb_matrices(float a[9], float b[9], float w[2]) {
     const Globals &g = globals();
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) a[3 * i + j] = g.xyz_from_rgb.m[i][j], b[3 * i + j] = g.rgb_from_xyz.m[i][j];
     w[0] = g.white[0], w[1] = g.white[1];
+}
+
+// SquareMatrix<3>::determinant of the upper 3x3 (squared_matrix.cu:8-17) < 0:
+// Transform::swaps_handedness (transform.h:208-217).
+bool swaps_handedness(const Mat4 &M) {
+    const auto &m = M.m;
+    const float minor12 = dop_h(m[1][1], m[2][2], m[1][2], m[2][1]);
+    const float minor02 = dop_h(m[1][0], m[2][2], m[1][2], m[2][0]);
+    const float minor01 = dop_h(m[1][0], m[2][1], m[1][1], m[2][0]);
+    return std::fma(m[0][2], minor01, dop_h(m[0][0], minor12, m[0][1], minor02)) < 0;
+}
+bool affine(const Mat4 &M) { return M.m[3][0] == 0 && M.m[3][1] == 0 && M.m[3][2] == 0 && M.m[3][3] == 1; }
+bool is_identity(const Xform &x) {  // Transform::is_identity (transform.h:219-239)
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k)
+            if (x.m.m[i][k] != x.inv.m[i][k] || x.m.m[i][k] != (i == k ? 1.f : 0.f)) return false;
+    return true;
+}
+// Transform::operator()(Bounds3f) (transform.h:323-349).
+void xbounds(const Mat4 &M, const float lo[3], const float hi[3], float olo[3], float ohi[3]) {
+    const auto &m = M.m;
+    for (int i = 0; i < 3; ++i) olo[i] = ohi[i] = m[i][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const float a = m[i][k] * lo[k], b = m[i][k] * hi[k];
+            olo[i] += std::min(a, b);
+            ohi[i] += std::max(a, b);
+        }
+}
+
+// Primitive i of the description is an analytic shape or an instanced triangle:
+// its DShape, and in its triangle slot the box corners lo, hi, lo (the build and
+// the aggregate's bounds see exactly Primitive::bounds). Instances whose
+// render_from_instance is the identity stay plain triangles, as the reference adds
+// the definition's primitives directly (scene_builder.cu:775-810).
+template <typename CtmOf>
+std::string add_shape(const pbrtm_shape &sh, const pbrtm_scene_desc &d, uint64_t i, const Xform &rw,
+                      const Xform &wr, CtmOf ctm_of, SceneHost &S) {
+    DShape q{};
+    q.type = sh.type;
+    const bool reverse = d.tri_reverse && d.tri_reverse[i];
+    q.reverse = reverse ? 1u : 0u;
+    Xform x;
+    if (sh.type == PBRTM_SHAPE_SPHERE || sh.type == PBRTM_SHAPE_DISK) {
+        x = xmul(rw, ctm_of(sh.xform));  // render_from_object = render_from_world * CTM
+    } else if (sh.type == PBRTM_SHAPE_INSTANCE) {
+        if (d.tri_light && d.tri_light[i] >= 0) return "area lights are not supported with object instancing";
+        x = xmul(xmul(rw, ctm_of(sh.xform)), wr);  // render_from_instance (scene_builder.cu:785-786)
+        if (is_identity(x)) return "";
+    } else {
+        return "unknown shape type";
+    }
+    if (!affine(x.m) || !affine(x.inv)) return "projective shape transforms are not supported";
+    std::memcpy(q.m, x.m.m, sizeof(q.m));
+    std::memcpy(q.inv, x.inv.m, sizeof(q.inv));
+    float blo[3], bhi[3];
+    const float phimax = float(double(kPiF) / 180.0 * double(std::min(std::max(sh.phimax, 0.f), 360.f)));
+    if (sh.type == PBRTM_SHAPE_SPHERE) {  // Sphere::init (shapes/sphere.cu:24-40)
+        const float r = sh.radius;
+        q.radius = r;
+        q.zmin = std::min(std::max(std::min(sh.zmin, sh.zmax), -r), r);
+        q.zmax = std::min(std::max(std::max(sh.zmin, sh.zmax), -r), r);
+        q.theta_zmin = float(std::acos(double(std::min(std::max(q.zmin / r, -1.f), 1.f))));
+        q.theta_zmax = float(std::acos(double(std::min(std::max(q.zmax / r, -1.f), 1.f))));
+        q.phimax = phimax;
+        q.flip = q.reverse;  // Sphere::init leaves transform_swaps_handedness false
+        const float lo[3] = {-r, -r, q.zmin}, hi[3] = {r, r, q.zmax};
+        xbounds(x.m, lo, hi, blo, bhi);
+    } else if (sh.type == PBRTM_SHAPE_DISK) {  // Disk::create (shapes/disk.cu:6-25)
+        q.radius = sh.radius;
+        q.height = sh.height;
+        q.inner = sh.innerradius;
+        q.phimax = phimax;
+        q.flip = (reverse != swaps_handedness(x.m)) ? 1u : 0u;
+        const float lo[3] = {-sh.radius, -sh.radius, sh.height}, hi[3] = {sh.radius, sh.radius, sh.height};
+        xbounds(x.m, lo, hi, blo, bhi);
+    } else {  // TransformedPrimitive::bounds (transformed_primitive.cu:10-13) of the triangle's box
+        for (int k = 0; k < 9; ++k) q.p[k] = S.tri[9 * i + k];
+        q.flip = q.reverse;
+        float lo[3], hi[3];
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(std::min(q.p[c], q.p[3 + c]), q.p[6 + c]);
+            hi[c] = std::max(std::max(q.p[c], q.p[3 + c]), q.p[6 + c]);
+        }
+        xbounds(x.m, lo, hi, blo, bhi);
+    }
+    for (int c = 0; c < 3; ++c) {
+        S.tri[9 * i + c] = blo[c];
+        S.tri[9 * i + 3 + c] = bhi[c];
+        S.tri[9 * i + 6 + c] = blo[c];
+    }
+    if (S.prim_shape.empty()) S.prim_shape.assign(d.num_tris, 0u);
+    S.shapes.push_back(q);
+    S.prim_shape[i] = uint32_t(S.shapes.size());
+    return "";
 }
 
 std::string build_scene_host(const void *vdesc, SceneHost &S) {
@@ -733,6 +840,20 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
         S.tri.resize(n * 9);
         S.tri_info.resize(n);
         S.lights.clear();
+        S.shapes.clear();
+        S.prim_shape.clear();
+        if (d.tri_shape && d.num_shapes && !d.shapes) return "shapes missing";
+        // the CTM's inverse as the front end composed it, else the reference's 4x4 inverse
+        auto ctm_of = [&](uint32_t k) -> Xform {
+            Xform x{mat4_diag(1, 1, 1), mat4_diag(1, 1, 1)};
+            if (k == 0xFFFFFFFFu) return x;
+            if (k >= d.num_xforms) throw std::runtime_error("shape transform index out of range");
+            std::memcpy(x.m.m, d.xforms + 16 * size_t(k), sizeof(x.m.m));
+            if (d.xforms_inv) std::memcpy(x.inv.m, d.xforms_inv + 16 * size_t(k), sizeof(x.inv.m));
+            else x.inv = mat4_inv(x.m);
+            return x;
+        };
+        const Xform world_from_render = xinv(render_from_world);
         for (uint64_t i = 0; i < n; ++i) {
             const uint32_t xk = d.tri_xform ? d.tri_xform[i] : 0xFFFFFFFFu;
             if (xk != 0xFFFFFFFFu && xk >= d.num_xforms) return "triangle transform index out of range";
@@ -742,6 +863,12 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
                 S.tri[9 * i + 3 * v] = p.x, S.tri[9 * i + 3 * v + 1] = p.y, S.tri[9 * i + 3 * v + 2] = p.z;
             }
             if (d.tri_material[i] >= d.num_materials) return "triangle material index out of range";
+            const uint32_t sk = d.tri_shape ? d.tri_shape[i] : 0xFFFFFFFFu;
+            if (sk != 0xFFFFFFFFu) {
+                if (sk >= d.num_shapes) return "shape index out of range";
+                const std::string e = add_shape(d.shapes[sk], d, i, render_from_world, world_from_render, ctm_of, S);
+                if (!e.empty()) return e;
+            }
             const bool flip = d.tri_reverse && d.tri_reverse[i];
             uint32_t light = 0;
             const int32_t li = d.tri_light ? d.tri_light[i] : -1;
@@ -843,7 +970,12 @@ std::string build_scene_host(const void *vdesc, SceneHost &S) {
                 if (!inf) {
                     const float *t = &S.tri[9 * size_t(l.tri)];
                     const V p0{t[0], t[1], t[2]}, p1{t[3], t[4], t[5]}, p2{t[6], t[7], t[8]};
-                    const float area = vlen(vcross(vsub(p1, p0), vsub(p2, p0))) * 0.5f;
+                    float area = vlen(vcross(vsub(p1, p0), vsub(p2, p0))) * 0.5f;
+                    const uint32_t sk = S.prim_shape.empty() ? 0u : S.prim_shape[l.tri];
+                    if (sk) {  // Sphere::area, Disk::area (shapes/sphere.h:31-34, shapes/disk.h:19-22)
+                        const DShape &q = S.shapes[sk - 1];
+                        area = q.type == kShapeSphere ? sphere_area(q) : disk_area(q);
+                    }
                     k = kPiF * float(l.two_sided ? 2 : 1) * area;
                 } else {  // UniformInfiniteLight::phi: 4 pi^2 R^2 scale L(lambda)
                     const float R = S.inf_dist / 2;

@@ -6,6 +6,10 @@
 
 namespace pbrtm {
 
+namespace pt {
+struct DShape;
+}
+
 constexpr int kWide = 8;
 constexpr uint32_t kWideLeaf = 0x80000000u;
 constexpr uint32_t kWideEmpty = 0xFFFFFFFFu;
@@ -46,6 +50,7 @@ struct WideTrace {
     const uint32_t *order;
     const float *tri_leaf;
     const uint32_t *prim_order;
+    const pt::DShape *shapes;  // analytic primitives (leaf pad word = index + 1), null for triangle-only trees
     uint32_t num_nodes, num_prims, max_depth;
     uint2 *spill;           // per-ray stack overflow area (max_depth - kLdsStack entries per ray), may be null
     uint32_t spill_per_ray;

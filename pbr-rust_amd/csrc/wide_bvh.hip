@@ -15,12 +15,17 @@
 // box inclusion, so skipping the collapsed inner binary boxes changes no
 // decision: the sequence of triangle tests, and therefore every hit (ties
 // included), equals the binary traversal's. Quantized boxes only contain the
-// exact ones, so they add box entries whose subtrees hold no triangle the ray
-// hits before the current best t (the slab test is conservative): extra node
-// visits and triangle tests, never a different hit or tie winner. Each visit
-// costs one 128-byte node instead of up to seven dependent 32-byte fetches.
+// exact ones, so they add box entries and triangle tests the binary traversal
+// would not make. Those can only find a hit the reference misses when the hit
+// lies on the exact leaf box's boundary (a tie with the current best t on a box
+// face, or a rounding of the entry distance): the first hit in a leaf is
+// therefore kept only if the leaf's exact box (the union of its primitives'
+// boxes) passes the slab test with the t at which the leaf was entered, exactly
+// as the binary traversal tests it. Each visit costs one 128-byte node instead
+// of up to seven dependent 32-byte fetches.
 #include "pbrtm_geometry.h"
 #include "wave_ops.h"
+#include "pbrtm_shapes.h"
 #include "wide_bvh.h"
 
 #include <algorithm>
@@ -390,11 +395,28 @@ struct WideTree {
     const uint32_t *order;
     const float *tri_leaf;
     const uint32_t *prim_order;
+    const pt::DShape *shapes;
     uint32_t num_nodes, num_prims;
 };
 
 __device__ __forceinline__ WideTree tree_of(const WideTrace &B) {
-    return WideTree{B.nodes, B.order, B.tri_leaf, B.prim_order, B.num_nodes, B.num_prims};
+    return WideTree{B.nodes, B.order, B.tri_leaf, B.prim_order, B.shapes, B.num_nodes, B.num_prims};
+}
+
+// The exact box of the binary leaf at positions [first, first + cnt) (the union of
+// its primitives' boxes: the leaf-ordered vertices, or an analytic primitive's
+// box stored as the corners lo, hi, lo) against the ray with t_max (bounds3.h:146-195).
+__device__ __forceinline__ bool leaf_box_hit(const WideTree &T, uint32_t first, uint32_t cnt,
+                                                        const WideRay &R, float t_max) {
+    Box b = box_empty();
+    for (uint32_t k = first; k < first + cnt; ++k) {
+        const float4 *t4 = reinterpret_cast<const float4 *>(T.tri_leaf) + 3ull * k;
+        const float4 q0 = t4[0], q1 = t4[1], q2 = t4[2];
+        b = box_union(b, tri_box(f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, f3{q1.z, q1.w, q2.x}));
+    }
+    const float lo[3] = {b.lo.x, b.lo.y, b.lo.z}, hi[3] = {b.hi.x, b.hi.y, b.hi.z};
+    const int neg[3] = {int(R.oct & 1u), int((R.oct >> 1) & 1u), int((R.oct >> 2) & 1u)};
+    return box_hit(lo, hi, R.o, R.inv, neg, t_max);
 }
 
 // Loads ray i; true when it has an interval to trace (t_max > 0).
@@ -429,7 +451,7 @@ __device__ __forceinline__ void wide_ray_finish(const WideRay &R, uint32_t i, ui
 // intersected on the spot, the first interior child is descended into, the
 // rest of the node is pushed), or pops the stack. Returns false once the ray
 // is finished. qi: the ray's queue slot, which owns the ray's spill area.
-template <bool COUNT>
+template <bool COUNT, bool SHAPES>
 __device__ __forceinline__ bool wide_step(WideRay &R, const WideTree &T, const WideTrace &B, int any_hit,
                                           uint32_t qi, uint32_t *__restrict__ err,
                                           uint2 (*s_stack)[kWideBlock], uint32_t &visits, uint32_t &boxes,
@@ -516,13 +538,27 @@ __device__ __forceinline__ bool wide_step(WideRay &R, const WideTree &T, const W
                 atomicOr(err, uint32_t(kErrStack));
                 return false;
             }
+            const float t_leaf = R.best_t;
             for (uint32_t k = first; k < first + cnt; ++k) {
                 const float4 *t4 = reinterpret_cast<const float4 *>(T.tri_leaf) + 3ull * k;
                 const float4 q0 = t4[0], q1 = t4[1], q2 = t4[2];
                 if (COUNT) ++tests;
                 TriHit h;
-                if (tri_hit(R.o, R.d, R.best_t, f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, f3{q1.z, q1.w, q2.x},
-                            &h)) {
+                bool hit;
+                if (SHAPES && __float_as_uint(q2.y) != 0u) {
+                    pt::QHit qh;
+                    hit = pt::shape_hit(T.shapes, __float_as_uint(q2.y) - 1u, R.o, R.d, R.best_t, &qh);
+                    h = TriHit{qh.p.x, qh.p.y, qh.p.z, qh.t};
+                } else {
+                    hit = tri_hit(R.o, R.d, R.best_t, f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y},
+                                  f3{q1.z, q1.w, q2.x}, &h);
+                }
+                if (hit) {
+                    // The first hit in this leaf: would the binary traversal have entered
+                    // the leaf? Its exact box (the union of its primitives' boxes) against
+                    // the t at entry; the quantized slot box only contains it. A rare check
+                    // (hits are few) that keeps ties on box faces the reference's.
+                    if (R.best_t == t_leaf && !leaf_box_hit(T, first, cnt, R, t_leaf)) break;
                     R.best = T.prim_order[k];
                     R.hit = h;
                     R.best_t = h.t;
@@ -579,7 +615,7 @@ __device__ __forceinline__ void add_counts(unsigned long long *counts, uint32_t 
 // stays the launch's.
 constexpr uint32_t kRefill = 8;
 
-template <bool COUNT, bool MULTI>
+template <bool COUNT, bool MULTI, bool SHAPES>
 __device__ __forceinline__ void trace_wide_persistent_body(const WideTrace &B, const float4 *__restrict__ rays,
                                                            const uint32_t *__restrict__ queue,
                                                            const uint32_t *__restrict__ queue_len, int any_hit,
@@ -621,7 +657,7 @@ __device__ __forceinline__ void trace_wide_persistent_body(const WideTrace &B, c
         }
         if (idle == ~0ull) break;  // nothing in flight and nothing left
         if (live) {
-            live = wide_step<COUNT>(R, T, B, any_hit, qi, err, s_stack, visits, boxes, tests);
+            live = wide_step<COUNT, SHAPES>(R, T, B, any_hit, qi, err, s_stack, visits, boxes, tests);
             if (!live) wide_ray_finish(R, i, prim_out, t_out, bary_out);
         }
     }
@@ -632,25 +668,28 @@ __device__ __forceinline__ void trace_wide_persistent_body(const WideTrace &B, c
 // persistent single-tree traversal runs fastest at 5 waves per SIMD with refills
 // at 8 idle lanes (6 waves: -10 %; 16 lanes: -4 %), the per-ray-tree one at 5;
 // the LDS stack (24 KiB per block) allows 6.
-template <bool COUNT>
+// SHAPES: trees with analytic primitives (a leaf position whose pad word holds a
+// shape index + 1); triangle-only trees take the instance without that branch.
+template <bool COUNT, bool SHAPES>
 __global__ __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void trace_wide_kernel(
     WideTrace B, const float4 *__restrict__ rays, const uint32_t *__restrict__ queue,
     const uint32_t *__restrict__ queue_len, int any_hit, uint32_t *__restrict__ prim_out,
     float *__restrict__ t_out, float *__restrict__ bary_out, uint32_t *__restrict__ err,
     unsigned long long *__restrict__ counts) {
     __shared__ uint2 s_stack[kLdsStack][kWideBlock];
-    trace_wide_persistent_body<COUNT, false>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err,
-                                             counts, nullptr, nullptr, 0, s_stack);
+    trace_wide_persistent_body<COUNT, false, SHAPES>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out,
+                                                     err, counts, nullptr, nullptr, 0, s_stack);
 }
 
+template <bool SHAPES>
 __global__ __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void trace_wide_multi_kernel(
     WideTrace B, const float4 *__restrict__ rays, const uint32_t *__restrict__ queue,
     const uint32_t *__restrict__ queue_len, int any_hit, uint32_t *__restrict__ prim_out,
     float *__restrict__ t_out, float *__restrict__ bary_out, uint32_t *__restrict__ err,
     const WideTrace *__restrict__ jw, const uint32_t *__restrict__ ray_job, uint32_t job_shift) {
     __shared__ uint2 s_stack[kLdsStack][kWideBlock];
-    trace_wide_persistent_body<false, true>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out, err,
-                                            nullptr, jw, ray_job, job_shift, s_stack);
+    trace_wide_persistent_body<false, true, SHAPES>(B, rays, queue, queue_len, any_hit, prim_out, t_out, bary_out,
+                                                    err, nullptr, jw, ray_job, job_shift, s_stack);
 }
 
 }  // namespace
@@ -743,21 +782,56 @@ static uint32_t persistent_grid(Kernel kernel, uint32_t n) {
     return std::max(1u, std::min(blocks, (n + kWideBlock - 1) / kWideBlock));
 }
 
+// The persistent traversal over a compacted queue; the instance with the
+// analytic-primitive branch when the tree has analytic primitives.
+static hipError_t trace_wide_queued_counts(const WideTrace &B, const float4 *r, uint32_t n, int any_hit,
+                                           uint32_t *prim_out, float *t_out, float *bary_out, uint32_t *err,
+                                           unsigned long long *counts, const uint32_t *queue,
+                                           const uint32_t *queue_len, hipStream_t stream) {
+    const uint32_t grid = persistent_grid(trace_wide_kernel<false, false>, n);
+    if (B.shapes) {
+        if (counts)
+            trace_wide_kernel<true, true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
+                                                                           t_out, bary_out, err, counts);
+        else
+            trace_wide_kernel<false, true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit,
+                                                                            prim_out, t_out, bary_out, err, nullptr);
+    } else if (counts) {
+        trace_wide_kernel<true, false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
+                                                                        t_out, bary_out, err, counts);
+    } else {
+        trace_wide_kernel<false, false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
+                                                                         t_out, bary_out, err, nullptr);
+    }
+    return hipGetLastError();
+}
+
+// B.shapes non-null: some job's tree has analytic primitives.
+static hipError_t launch_multi(const WideTrace &B, const WideTrace *jobs, const uint32_t *ray_job,
+                               uint32_t job_shift, const float4 *r, uint32_t n, int any_hit, uint32_t *prim_out,
+                               float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
+                               const uint32_t *queue_len, hipStream_t stream) {
+    const uint32_t grid = persistent_grid(trace_wide_multi_kernel<false>, n);
+    if (B.shapes)
+        trace_wide_multi_kernel<true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
+                                                                       t_out, bary_out, err, jobs, ray_job,
+                                                                       job_shift);
+    else
+        trace_wide_multi_kernel<false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
+                                                                        t_out, bary_out, err, jobs, ray_job,
+                                                                        job_shift);
+    return hipGetLastError();
+}
+
 hipError_t trace_wide(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
                       float *t_out, float *bary_out, uint32_t *err, unsigned long long *counts, uint32_t *queue,
                       uint32_t *queue_len, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = persistent_grid(trace_wide_kernel<false>, n);
     const uint32_t cgrid = (n + kCompactBlock * kCompactItems - 1) / (kCompactBlock * kCompactItems);
     const float4 *r = reinterpret_cast<const float4 *>(rays);
     compact_rays_kernel<<<cgrid, kCompactBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
-    if (counts)
-        trace_wide_kernel<true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
-                                                                 bary_out, err, counts);
-    else
-        trace_wide_kernel<false><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
-                                                                  bary_out, err, nullptr);
-    return hipGetLastError();
+    return trace_wide_queued_counts(B, r, n, any_hit, prim_out, t_out, bary_out, err, counts, queue, queue_len,
+                                    stream);
 }
 
 namespace {
@@ -871,22 +945,16 @@ hipError_t trace_wide_multi_queued(const WideTrace &B, const WideTrace *jobs, co
                                    float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
                                    const uint32_t *queue_len, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = persistent_grid(trace_wide_multi_kernel, n);
-    trace_wide_multi_kernel<<<grid, kWideBlock, 0, stream>>>(B, reinterpret_cast<const float4 *>(rays), queue,
-                                                             queue_len, any_hit, prim_out, t_out, bary_out, err,
-                                                             jobs, ray_job, job_shift);
-    return hipGetLastError();
+    return launch_multi(B, jobs, ray_job, job_shift, reinterpret_cast<const float4 *>(rays), n, any_hit, prim_out,
+                        t_out, bary_out, err, queue, queue_len, stream);
 }
 
 hipError_t trace_wide_queued(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
                              float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
                              const uint32_t *queue_len, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = persistent_grid(trace_wide_kernel<false>, n);
-    trace_wide_kernel<false><<<grid, kWideBlock, 0, stream>>>(B, reinterpret_cast<const float4 *>(rays), queue,
-                                                              queue_len, any_hit, prim_out, t_out, bary_out, err,
-                                                              nullptr);
-    return hipGetLastError();
+    return trace_wide_queued_counts(B, reinterpret_cast<const float4 *>(rays), n, any_hit, prim_out, t_out, bary_out,
+                                    err, nullptr, queue, queue_len, stream);
 }
 
 hipError_t trace_wide_multi(const WideTrace &B, const WideTrace *jobs, const uint32_t *ray_job, uint32_t job_shift,
@@ -894,13 +962,11 @@ hipError_t trace_wide_multi(const WideTrace &B, const WideTrace *jobs, const uin
                             float *bary_out, uint32_t *err, uint32_t *queue, uint32_t *queue_len,
                             hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = persistent_grid(trace_wide_multi_kernel, n);
     const uint32_t cgrid = (n + kCompactBlock * kCompactItems - 1) / (kCompactBlock * kCompactItems);
     const float4 *r = reinterpret_cast<const float4 *>(rays);
     compact_rays_kernel<<<cgrid, kCompactBlock, 0, stream>>>(r, n, queue, queue_len, prim_out, t_out, bary_out);
-    trace_wide_multi_kernel<<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out, t_out,
-                                                             bary_out, err, jobs, ray_job, job_shift);
-    return hipGetLastError();
+    return launch_multi(B, jobs, ray_job, job_shift, r, n, any_hit, prim_out, t_out, bary_out, err, queue, queue_len,
+                        stream);
 }
 
 }  // namespace pbrtm

@@ -69,6 +69,14 @@ class InfiniteLight(ctypes.Structure):
                 ("light_index", c_uint32)]
 
 
+class Shape(ctypes.Structure):
+    _fields_ = [("type", c_uint32), ("xform", c_uint32), ("radius", c_float), ("zmin", c_float), ("zmax", c_float),
+                ("phimax", c_float), ("height", c_float), ("innerradius", c_float)]
+
+
+SHAPES = {"sphere": 1, "disk": 2, "instance": 3}
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("eye", c_float * 3), ("look", c_float * 3), ("up", c_float * 3), ("fov", c_float),
                 ("width", c_uint32), ("height", c_uint32), ("num_tris", c_uint64), ("tri", c_void_p),
@@ -76,7 +84,8 @@ class SceneDesc(ctypes.Structure):
                 ("num_materials", c_uint32), ("materials", POINTER(Material)), ("num_lights", c_uint32),
                 ("lights", POINTER(AreaLight)), ("num_xforms", c_uint32), ("xforms", c_void_p),
                 ("tri_xform", c_void_p), ("num_infinite_lights", c_uint32),
-                ("infinite_lights", POINTER(InfiniteLight))]
+                ("infinite_lights", POINTER(InfiniteLight)), ("xforms_inv", c_void_p), ("num_shapes", c_uint32),
+                ("shapes", POINTER(Shape)), ("tri_shape", c_void_p)]
 
 
 class RenderParams(ctypes.Structure):
@@ -443,7 +452,30 @@ def _scene_desc_tail(sc, d, keep):
         infl[i].light_index = l.get("light_index", 0)
     d.num_infinite_lights = len(inf)
     d.infinite_lights = infl
-    return d, keep + (xf, txf, infl)
+    # analytic shapes: "shapes": [{"type": "sphere" | "disk" | "instance", "xform": index or None,
+    # "radius", "zmin", "zmax", "phimax" (degrees), "height", "innerradius"}], "tri_shape": [n]
+    # (0xFFFFFFFF = a triangle), "xforms_inv": [num_xforms][4][4] (optional)
+    shp = sc.get("shapes") or []
+    shapes = (Shape * max(1, len(shp)))()
+    for i, q in enumerate(shp):
+        r = q.get("radius", 1.0)
+        shapes[i].type = SHAPES[q["type"]]
+        shapes[i].xform = 0xFFFFFFFF if q.get("xform") is None else q["xform"]
+        shapes[i].radius = r
+        shapes[i].zmin = q.get("zmin", -r)
+        shapes[i].zmax = q.get("zmax", r)
+        shapes[i].phimax = q.get("phimax", 360.0)
+        shapes[i].height = q.get("height", 0.0)
+        shapes[i].innerradius = q.get("innerradius", 0.0)
+    tsh = sc.get("tri_shape")
+    tsh = None if tsh is None else np.ascontiguousarray(tsh, dtype=np.uint32)
+    xinv = sc.get("xforms_inv")
+    xinv = None if xinv is None else np.ascontiguousarray(xinv, dtype=np.float32).reshape(-1, 16)
+    d.num_shapes = len(shp)
+    d.shapes = shapes
+    d.tri_shape = _ptr(tsh)
+    d.xforms_inv = _ptr(xinv)
+    return d, keep + (xf, txf, infl, shapes, tsh, xinv)
 
 
 class PbrtFile:
@@ -511,6 +543,16 @@ class PbrtFile:
         if d.num_xforms:
             out["xforms"] = arr(d.xforms, c_float, 16 * d.num_xforms, np.float32).reshape(-1, 4, 4)
             out["tri_xform"] = arr(d.tri_xform, c_uint32, n, np.uint32)
+            if d.xforms_inv:
+                out["xforms_inv"] = arr(d.xforms_inv, c_float, 16 * d.num_xforms, np.float32).reshape(-1, 4, 4)
+        if d.num_shapes:
+            kinds = {v: k for k, v in SHAPES.items()}
+            out["shapes"] = [
+                {"type": kinds[q.type], "xform": None if q.xform == 0xFFFFFFFF else q.xform, "radius": q.radius,
+                 "zmin": q.zmin, "zmax": q.zmax, "phimax": q.phimax, "height": q.height,
+                 "innerradius": q.innerradius}
+                for q in (d.shapes[i] for i in range(d.num_shapes))]
+            out["tri_shape"] = arr(d.tri_shape, c_uint32, n, np.uint32)
         if d.num_infinite_lights:
             out["infinite_lights"] = [
                 {"rgb": tuple(l.rgb) if l.has_rgb else None, "scale": l.scale, "illuminance": l.illuminance,

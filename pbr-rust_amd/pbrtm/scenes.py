@@ -28,6 +28,15 @@ every triangle given its own diffuse material (albedo U[0.1, 0.9]^3 from a
 grid, scales U[4, 8]); perturbed() scales every albedo and emitter scale by
 1 + U[-a, a] (albedos clamped to [0.01, 0.99]) to make the target.
 
+shapes_room: cornell34 with analytic shapes (Shape "sphere" / "disk") and object
+instancing: a smooth glass sphere (the example scene's), a rough copper partial
+sphere (zmin, zmax, phimax) under a rotation, a diffuse sphere with
+ReverseOrientation, an emissive sphere and an emissive disk facing down, a
+diffuse annular disk under a mirroring transform (Scale -1: the disk's normal
+flips with the handedness), and the short box instanced twice more through
+ObjectInstance transforms (rotation + translation). Each transform carries its
+inverse as composed from the directives (Translate, Rotate, Scale).
+
 batch_jobs: the mixed batch of the batched-render benchmark (per job: a square
 resolution uniform in [lo, hi], spp from {1, 4, 16, 64}, max depth 1..8 and a
 soup_room of 10^4..10^6 triangles, log-uniform), drawn from one PCG32 stream.
@@ -288,3 +297,114 @@ def perturbed(sc, seed=0x7A27, amount=0.1):
     out["albedo"] = np.clip(sc["albedo"] * f, 0.01, 0.99).astype(np.float32)
     out["lights"] = [dict(l, scale=l["scale"] * (1.0 + amount * (2.0 * rng.uniform() - 1.0))) for l in sc["lights"]]
     return out
+
+
+def _translate(x, y, z):
+    m, mi = np.eye(4, dtype=np.float32), np.eye(4, dtype=np.float32)
+    m[:3, 3] = (x, y, z)
+    mi[:3, 3] = (-np.float32(x), -np.float32(y), -np.float32(z))
+    return m, mi
+
+
+def _scale(x, y, z):
+    m = np.diag([x, y, z, 1.0]).astype(np.float32)
+    mi = np.diag([np.float32(1.0) / np.float32(x), np.float32(1.0) / np.float32(y), np.float32(1.0) / np.float32(z),
+                  1.0]).astype(np.float32)
+    return m, mi
+
+
+def _rotate(deg, ax, ay, az):
+    """Transform::rotate (transform.h:61-88): sin and cos rounded from double; the
+    inverse is the transpose."""
+    rad = np.float32(np.float64(np.float32(np.pi)) / 180.0 * np.float64(np.float32(deg)))
+    s, c = np.float32(np.sin(np.float64(rad))), np.float32(np.cos(np.float64(rad)))
+    a = np.array([ax, ay, az], np.float32)
+    a = a / np.sqrt(np.float32(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]))
+    x, y, z = (np.float64(v) for v in a)
+    s, c = np.float64(s), np.float64(c)
+    m = np.eye(4, dtype=np.float32)
+    m[0, :3] = (x * x + (1.0 - x * x) * c, x * y * (1.0 - c) - z * s, x * z * (1.0 - c) + y * s)
+    m[1, :3] = (x * y * (1.0 - c) + z * s, y * y + (1.0 - y * y) * c, y * z * (1.0 - c) - x * s)
+    m[2, :3] = (x * z * (1.0 - c) - y * s, y * z * (1.0 - c) + x * s, z * z + (1.0 - z * z) * c)
+    return m, np.ascontiguousarray(m.T)
+
+
+def _fma_mul(a, b):
+    """SquareMatrix<4>::operator* (squared_matrix.h:98-109): an FMA chain per entry."""
+    r = np.zeros((4, 4), np.float32)
+    for i in range(4):
+        for j in range(4):
+            acc = np.float32(0)
+            for k in range(4):
+                acc = np.float32(np.float64(a[i, k]) * np.float64(b[k, j]) + np.float64(acc))
+            r[i, j] = acc
+    return r
+
+
+def _compose(*xfs):
+    """CTM = x0 * x1 * ...; its inverse composed in reverse order, as Transform does."""
+    m, mi = xfs[0]
+    for x, xi in xfs[1:]:
+        m, mi = _fma_mul(m, x), _fma_mul(xi, mi)
+    return m, mi
+
+
+def shapes_room(width=64, height=64, spp=4, max_depth=5):
+    sc = cornell34(width, height, spp=spp, max_depth=max_depth)
+    tri = [t for t in np.asarray(sc["tri"], np.float32)]
+    mats = list(np.asarray(sc["tri_material"], np.uint32))
+    lig = list(np.asarray(sc["tri_light"], np.int32))
+    materials = list(sc["materials"]) + [
+        {"type": "dielectric", "eta": 1.5, "roughness": 0.0, "remap": False},  # 4 glass
+        {"type": "conductor", "metal": "Cu", "roughness": 0.05, "remap": True},  # 5 rough copper
+        {"type": "diffuse", "rgb": (0.2, 0.35, 0.8)},  # 6 blue
+    ]
+    lights = list(sc["lights"]) + [{"rgb": (1.0, 0.6, 0.3), "scale": 8.0, "two_sided": False},
+                                   {"rgb": (0.4, 0.7, 1.0), "scale": 5.0, "two_sided": False}]
+    n0 = len(tri)
+    rev = [0] * n0
+    xforms, shapes = [], []
+    tri_xform = [0xFFFFFFFF] * n0
+    tri_shape = [0xFFFFFFFF] * n0
+
+    def add_xf(x):
+        xforms.append(x)
+        return len(xforms) - 1
+
+    def add_shape(q, mat, light=-1, reverse=0):
+        shapes.append(q)
+        tri.append(np.zeros(9, np.float32))
+        mats.append(mat)
+        lig.append(light)
+        rev.append(reverse)
+        tri_xform.append(q.get("xform") if q.get("xform") is not None else 0xFFFFFFFF)
+        tri_shape.append(len(shapes) - 1)
+
+    add_shape({"type": "sphere", "radius": 75.0, "xform": add_xf(_translate(400.0, 80.0, 150.0))}, 4)
+    add_shape({"type": "sphere", "radius": 60.0, "zmin": -35.0, "zmax": 50.0, "phimax": 290.0,
+               "xform": add_xf(_compose(_translate(430.0, 330.0 + 60.0, 350.0), _rotate(-70.0, 1.0, 0.3, 0.0)))}, 5)
+    add_shape({"type": "sphere", "radius": 40.0, "xform": add_xf(_translate(70.0, 45.0, 120.0))}, 6, reverse=1)
+    add_shape({"type": "sphere", "radius": 18.0, "xform": add_xf(_translate(90.0, 430.0, 300.0))}, 0, light=1)
+    add_shape({"type": "disk", "radius": 45.0,
+               "xform": add_xf(_compose(_translate(420.0, 548.0, 110.0), _rotate(90.0, 1.0, 0.0, 0.0)))}, 0, light=2)
+    add_shape({"type": "disk", "radius": 70.0, "innerradius": 25.0, "phimax": 300.0, "height": 2.0,
+               "xform": add_xf(_compose(_translate(300.0, 520.0, 480.0), _scale(1.0, -1.0, 1.0),
+                                        _rotate(90.0, 1.0, 0.0, 0.0)))}, 6)
+    # the short box (cornell34 entries 22..33) instanced twice
+    boxes = list(range(22, 34))
+    for xf in (_compose(_translate(80.0, 0.0, 300.0), _rotate(25.0, 0.0, 1.0, 0.0), _scale(0.5, 0.8, 0.5)),
+               _compose(_translate(330.0, 220.0, 60.0), _rotate(-15.0, 0.0, 0.0, 1.0), _scale(0.35, 0.3, 0.35))):
+        shapes.append({"type": "instance", "xform": add_xf(xf)})
+        for b in boxes:
+            tri.append(np.asarray(sc["tri"], np.float32)[b])
+            mats.append(6)
+            lig.append(-1)
+            rev.append(0)
+            tri_xform.append(0xFFFFFFFF)
+            tri_shape.append(len(shapes) - 1)
+    sc.update(
+        tri=np.asarray(tri, np.float32).reshape(-1, 9), tri_material=np.asarray(mats, np.uint32),
+        tri_light=np.asarray(lig, np.int32), tri_reverse=np.asarray(rev, np.uint8), materials=materials,
+        lights=lights, xforms=np.stack([m for m, _ in xforms]), xforms_inv=np.stack([mi for _, mi in xforms]),
+        tri_xform=np.asarray(tri_xform, np.uint32), shapes=shapes, tri_shape=np.asarray(tri_shape, np.uint32))
+    return sc

@@ -1,6 +1,6 @@
 """Scene descriptions shared by the oracle and the library in the tests (the
 generators live in the package: pbrtm/scenes.py)."""
-from pbrtm.scenes import GREY, GREEN, RED, batch_jobs, box, cornell34, quad, soup_room, sphere_grid  # noqa: F401
+from pbrtm.scenes import GREY, GREEN, RED, batch_jobs, box, cornell34, quad, shapes_room, soup_room, sphere_grid  # noqa: F401
 
 
 def cornell_emitters(width=32, height=32, spp=16, max_depth=4):

@@ -140,6 +140,57 @@ def test_intersect_and_bounce_parity_1m(ctx):
     assert np.array_equal(pa != O.NO_HIT, qa != O.NO_HIT)
 
 
+def test_headline_config_parity_10m(ctx):
+    """The benchmark's exact workload: 10M triangles of seed 0x5EED, the canonical
+    tree, then closest hits (prim, t, barycentrics) of the 1920x1080 primary rays
+    and of their cosine bounce, all equal to the oracle (hlbvh.cu:384-624, :231-289)."""
+    n = 10_000_000
+    log(f"headline: soup {n}")
+    tri = soup(n, 0x5EED)
+    bvh = ctx.build(tri)
+    log(f"headline: stats {bvh.stats.as_dict()}")
+    ob = O.OracleBvh(tri)
+    log("headline: canonical trees")
+    assert bvh.export_canonical() == ob.export_canonical()
+    rays = pbrtm.gen_camera_rays(1920, 1080)
+    log("headline: primary rays")
+    prim, t, bary = bvh.intersect(rays)
+    op, ot, obary, _ = ob.intersect(rays, threads=16)
+    assert (op != O.NO_HIT).mean() > 0.1
+    assert np.array_equal(prim, op) and np.array_equal(t, ot) and np.array_equal(bary, obary)
+    log("headline: bounce rays")
+    br = bvh.ao_bounce(rays, prim, bary)
+    p2, t2, b2 = bvh.intersect(br)
+    q2, u2, c2, _ = ob.intersect(br, threads=16)
+    assert np.array_equal(p2, q2) and np.array_equal(t2, u2) and np.array_equal(b2, c2)
+    bvh.free()
+
+
+def test_deep_tree_spills_and_matches_brute_force(ctx):
+    """A tree deeper than the 12-entry LDS stack (wide depth > 12): rays along the
+    chain push one entry per wide level, so the traversal's memory spill area is
+    used; hits equal brute force (no exact t ties from x = -1e-12) and the oracle's
+    traversal (ties resolved by traversal order, from x = -1)."""
+    import scenes
+
+    tri = scenes.deep_chain()
+    bvh = ctx.build(tri)
+    log(f"deep chain stats {bvh.stats.as_dict()}")
+    assert bvh.stats.wide_depth > 12
+    assert bvh.export_canonical() == O.OracleBvh(tri).export_canonical()
+    rays = scenes.chain_rays(4096)
+    prim, t, _ = bvh.intersect(rays)
+    bp, bt = O.brute_intersect(tri, rays)
+    assert (bp != O.NO_HIT).all()
+    assert np.array_equal(prim, bp) and np.array_equal(t, bt)
+    far = scenes.chain_rays(4096, seed=4, x0=-1.0)
+    prim, t, bary = bvh.intersect(far)
+    op, ot, ob, _ = O.OracleBvh(tri).intersect(far)
+    assert np.array_equal(prim, op) and np.array_equal(t, ot) and np.array_equal(bary, ob)
+    pa, _, _ = bvh.intersect(far, any_hit=True, want_bary=False)
+    assert np.array_equal(pa != O.NO_HIT, op != O.NO_HIT)
+
+
 def test_intersect_brute_force_small(ctx):
     tri = soup(3000, 21)
     rng = np.random.default_rng(1)

@@ -80,7 +80,10 @@ def test_named_materials_lights_and_defaults(tmp_path):
 
 
 @pytest.mark.parametrize("body,msg", [
-    ('Shape "sphere" "float radius" 1\n', "sphere"),
+    ('Shape "cylinder" "float radius" 1\n', "cylinder"),
+    ('ObjectBegin "a"\nShape "sphere" "float radius" 1\nObjectEnd\n', "ObjectBegin"),
+    ('ObjectInstance "missing"\n', "not found"),
+    ('Shape "sphere" "float radius" 1 "float thetamax" 2\n', "thetamax"),
     ('Material "coateddiffuse"\n', "coateddiffuse"),
     ('LightSource "point"\n', "LightSource"),
     ('LightSource "infinite" "string filename" "sky.exr"\n', "image infinite"),
