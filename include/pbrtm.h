@@ -260,6 +260,10 @@ typedef struct pbrtm_render_params {
  * (megakernel_integrator.cu:14-42); here they run on the same path pool. Only
  * pbrtm_render takes them; pbrtm_render_grad and pbrtm_render_batch render with
  * the wavefront path integrator. */
+/* pbrtm_render_params.flags: trace with the counting instance of the traversal and
+ * report its totals in pbrtm_render_stats (same films and ray counts; for byte
+ * accounting, not for timing). */
+#define PBRTM_RENDER_COUNT 0x100u
 #define PBRTM_INTEGRATOR_MASK 0xF0u
 #define PBRTM_INTEGRATOR_PATH 0x00u              /* "path", "volpath": WavefrontPathIntegrator */
 #define PBRTM_INTEGRATOR_MEGAKERNEL_PATH 0x10u   /* "megakernelpath": MegakernelPathIntegrator
@@ -279,6 +283,9 @@ typedef struct pbrtm_render_stats {
     uint64_t rounds, rays, shadow_rays, paths;
     uint32_t ray_order;  /* the ray-order autotuner's choice: bit 0 closest-hit rays sorted, bit 1 shadow rays */
     uint32_t reserved;
+    /* with PBRTM_RENDER_COUNT: [wide nodes visited, child box tests, primitive tests]
+       summed over the closest-hit and the shadow traversals (else zero) */
+    uint64_t closest_counts[3], shadow_counts[3];
 } pbrtm_render_stats;
 
 /* Builds the scene tables and its aggregate on the context's device. Replaces

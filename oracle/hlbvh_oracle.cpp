@@ -9,10 +9,13 @@
 // atomics (hlbvh.cu:65, :167, :796), so only its tree of primitive SETS is
 // deterministic. This oracle builds that set-tree and exports it in a canonical
 // preorder form; the GPU build must reproduce the same bytes.
+#include <algorithm>
 #include <atomic>
-#include <functional>
-#include <string>
 #include <cstdio>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -43,9 +46,27 @@ struct Prim {          // hlbvh.h:14-19 MortonPrimitive
 
 struct Node {          // hlbvh.h:33-74, with explicit children
     B3 bounds;
-    int axis = -1;     // -1 = leaf
-    int left = -1, right = -1;
-    uint32_t first = 0, count = 0;  // leaf range in prim order; subtree prim count
+    int axis;          // -1 = leaf
+    int left, right;
+    uint32_t first, count;  // leaf range in prim order; subtree prim count
+};
+
+// Node storage: ids are handed out by an atomic counter, as the reference's
+// node_count.fetch_add(2) (hlbvh.cu:796), so concurrent subtree builds can
+// allocate; a build over n primitives uses at most 2n - 1 nodes.
+struct NodeArena {
+    std::unique_ptr<Node[]> p;
+    uint64_t cap = 0;
+    std::atomic<uint64_t> used{0};
+    void reset(uint64_t c) {
+        p.reset(new Node[c]);  // trivially constructible: the pages are touched when used
+        cap = c;
+        used = 0;
+    }
+    Node &operator[](uint64_t i) { return p[i]; }
+    const Node &operator[](uint64_t i) const { return p[i]; }
+    uint64_t size() const { return used.load(); }
+    bool empty() const { return size() == 0; }
 };
 
 struct Treelet {       // hlbvh.h:21-25
@@ -64,7 +85,7 @@ struct orc_bvh {
     void *custom_user = nullptr;
     const uint8_t *custom = nullptr;
     std::vector<Prim> prims;     // Morton-sorted, then partitioned in place
-    std::vector<Node> nodes;
+    NodeArena nodes;
     int top_nodes = 0;
     std::string error;
 };
@@ -91,17 +112,25 @@ void parallel_for(uint64_t n, int threads, const std::function<void(uint64_t, ui
     for (auto &th : pool) th.join();
 }
 
-// hlbvh.cu:101-141 and the host centroid-bounds loop :428-446.
-void compute_morton(const float *tri, uint64_t n, std::vector<Prim> &prims) {
+// hlbvh.cu:101-141 and the host centroid-bounds loop :428-446. With threads > 1
+// the centroid bounds are reduced per chunk; min and max do not depend on the
+// order (up to the sign of a zero, which reaches no Morton code).
+void compute_morton(const float *tri, uint64_t n, std::vector<Prim> &prims, int threads = 1) {
     prims.resize(n);
     B3 cb = b3_empty();
-    for (uint64_t i = 0; i < n; ++i) {
-        Prim &p = prims[i];
-        p.idx = uint32_t(i);
-        p.bounds = b3_of_tri(tri_point(tri, i, 0), tri_point(tri, i, 1), tri_point(tri, i, 2));
-        p.centroid = b3_centroid(p.bounds);
-        cb = b3_add_point(cb, p.centroid);
-    }
+    std::mutex mu;
+    parallel_for(n, threads, [&](uint64_t b, uint64_t e) {
+        B3 part = b3_empty();
+        for (uint64_t i = b; i < e; ++i) {
+            Prim &p = prims[i];
+            p.idx = uint32_t(i);
+            p.bounds = b3_of_tri(tri_point(tri, i, 0), tri_point(tri, i, 1), tri_point(tri, i, 2));
+            p.centroid = b3_centroid(p.bounds);
+            part = b3_add_point(part, p.centroid);
+        }
+        std::lock_guard<std::mutex> g(mu);
+        cb = b3_add_point(b3_add_point(cb, part.lo), part.hi);
+    });
     int max_dim = b3_max_dimension(cb);
     float radius = (cb.hi[max_dim] - cb.lo[max_dim]) / 2;
     V3 lo = cb.lo, hi = cb.hi;
@@ -112,33 +141,92 @@ void compute_morton(const float *tri, uint64_t n, std::vector<Prim> &prims) {
         hi[d] = center + radius;
     }
     B3 adj{vmin(lo, hi), vmax(lo, hi)};
-    for (uint64_t i = 0; i < n; ++i) {
-        V3 o = b3_offset(adj, prims[i].centroid);
-        V3 s = scale(float(1 << 10), o);
-        prims[i].code = encode_morton3(f2u_sat(s.x), f2u_sat(s.y), f2u_sat(s.z));
-    }
+    parallel_for(n, threads, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) {
+            V3 o = b3_offset(adj, prims[i].centroid);
+            V3 s = scale(float(1 << 10), o);
+            prims[i].code = encode_morton3(f2u_sat(s.x), f2u_sat(s.y), f2u_sat(s.z));
+        }
+    });
 }
 
 // Canonical order: full 30-bit code, ties by input index. It refines the
-// reference's treelet-only counting sort (hlbvh.cu:54-80, :457-502).
-void canonical_sort(std::vector<Prim> &prims) {
-    std::stable_sort(prims.begin(), prims.end(),
-                     [](const Prim &a, const Prim &b) { return a.code < b.code; });
+// reference's treelet-only counting sort (hlbvh.cu:54-80, :457-502). Keys are
+// (code << 32 | index), distributed over 1024 buckets by the top 10 code bits,
+// each bucket sorted on its own; prims[i].idx == i on entry.
+void canonical_sort(std::vector<Prim> &prims, int threads = 1) {
+    const uint64_t n = prims.size();
+    constexpr int kBits = 10, kBuckets = 1 << kBits;
+    const int parts = std::max(1, threads);
+    std::vector<uint64_t> hist(uint64_t(parts) * kBuckets, 0);
+    auto range = [&](int t) { return std::make_pair(n * t / parts, n * (t + 1) / parts); };
+    auto bucket = [](uint32_t code) { return code >> (30 - kBits); };
+    auto run = [&](const std::function<void(int)> &fn) {
+        std::vector<std::thread> pool;
+        for (int t = 1; t < parts; ++t) pool.emplace_back(fn, t);
+        fn(0);
+        for (auto &th : pool) th.join();
+    };
+    run([&](int t) {
+        auto [b, e] = range(t);
+        for (uint64_t i = b; i < e; ++i) ++hist[uint64_t(t) * kBuckets + bucket(prims[i].code)];
+    });
+    std::vector<uint64_t> start(kBuckets + 1, 0);
+    uint64_t acc = 0;
+    for (int k = 0; k < kBuckets; ++k) {  // bucket-major, part-minor offsets
+        start[k] = acc;
+        for (int t = 0; t < parts; ++t) {
+            const uint64_t c = hist[uint64_t(t) * kBuckets + k];
+            hist[uint64_t(t) * kBuckets + k] = acc;
+            acc += c;
+        }
+    }
+    start[kBuckets] = acc;
+    std::unique_ptr<uint64_t[]> keys(new uint64_t[n]);
+    run([&](int t) {
+        auto [b, e] = range(t);
+        uint64_t *off = hist.data() + uint64_t(t) * kBuckets;
+        for (uint64_t i = b; i < e; ++i)
+            keys[off[bucket(prims[i].code)]++] = (uint64_t(prims[i].code) << 32) | prims[i].idx;
+    });
+    std::atomic<int> next{0};
+    run([&](int) {
+        for (int k; (k = next.fetch_add(1)) < kBuckets;) std::sort(keys.get() + start[k], keys.get() + start[k + 1]);
+    });
+    std::vector<Prim> sorted(n);
+    parallel_for(n, threads, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) sorted[i] = prims[uint32_t(keys[i])];
+    });
+    prims.swap(sorted);
 }
 
 struct Builder {
     orc_bvh *bvh;
     std::vector<Treelet> treelets;
-    bool failed = false;
+    std::atomic<bool> failed{false};
     std::string why;
+    std::mutex why_mu;
 
-    int new_node() {
-        bvh->nodes.emplace_back();
-        return int(bvh->nodes.size()) - 1;
+    void fail(const char *msg) {
+        std::lock_guard<std::mutex> g(why_mu);
+        if (!failed.exchange(true)) why = msg;
     }
 
-    // hlbvh.cu:649-826 — SAH over treelets, 24 buckets.
-    void build_upper(int node, const std::vector<uint32_t> &ids) {
+    // k consecutive ids (the two children of a split), fields initialised as a leaf
+    int new_nodes(int k) {
+        const uint64_t id = bvh->nodes.used.fetch_add(uint64_t(k));
+        if (id + k > bvh->nodes.cap) {  // cannot happen: a binary tree over n primitives has < 2n nodes
+            fail("node arena exhausted");
+            return 0;
+        }
+        for (int j = 0; j < k; ++j) bvh->nodes[id + j] = Node{b3_empty(), -1, -1, -1, 0u, 0u};
+        return int(id);
+    }
+
+    // hlbvh.cu:649-826 — SAH over treelets, 24 buckets. spawn > 0 lets the left
+    // subtree go to a thread of its own, as the reference hands subtrees of 64
+    // or more treelets to its thread pool (hlbvh.cu:802-824).
+    void build_upper(int node, const std::vector<uint32_t> &ids, int spawn = 0) {
         if (failed) return;
         if (ids.size() == 1) {
             const Treelet &t = treelets[ids[0]];
@@ -158,8 +246,7 @@ struct Builder {
         }
         const int axis = b3_max_dimension(cb);
         if (cb.lo[axis] == cb.hi[axis]) {  // hlbvh.cu:670-674 (fatal in the reference)
-            failed = true;
-            why = "degenerate treelet centroids";
+            fail("degenerate treelet centroids");
             return;
         }
         struct Bucket {
@@ -173,8 +260,7 @@ struct Builder {
             float c = b3_centroid(treelets[id].bounds)[axis];
             uint32_t b = f2u_sat(float(kNumBuckets) * ((c - base) / span));
             if (b > kNumBuckets) {  // hlbvh.cu:696-698
-                failed = true;
-                why = "bucket index out of range";
+                fail("bucket index out of range");
                 return;
             }
             if (b == kNumBuckets) b = kNumBuckets - 1;
@@ -220,21 +306,27 @@ struct Builder {
                 else right.push_back(id);
             }
             if (left.empty() || right.empty()) {
-                failed = true;
-                why = "treelet split failed";
+                fail("treelet split failed");
                 return;
             }
         }
-        int l = new_node();
-        int r = new_node();
+        const int l = new_nodes(2);
+        const int r = l + 1;
+        if (failed) return;
         Node &nd = bvh->nodes[node];
         nd.bounds = full;
         nd.axis = axis;
         nd.left = l;
         nd.right = r;
         nd.count = total;
-        build_upper(l, left);
-        build_upper(r, right);
+        if (spawn > 0 && left.size() >= 64 && right.size() >= 64) {
+            std::thread th([&]() { build_upper(l, left, spawn - 1); });
+            build_upper(r, right, spawn - 1);
+            th.join();
+        } else {
+            build_upper(l, left, spawn - 1);
+            build_upper(r, right, spawn - 1);
+        }
     }
 
     // hlbvh.cu:828-857 — Hoare partition on centroid[dim] < split.
@@ -268,8 +360,9 @@ struct Builder {
         B3 lb = b3_empty(), rb = b3_empty();
         for (uint32_t i = nd.first; i < mid; ++i) lb = b3_union(lb, P[i].bounds);
         for (uint32_t i = mid; i < nd.first + nd.count; ++i) rb = b3_union(rb, P[i].bounds);
-        int l = new_node();
-        int r = new_node();
+        const int l = new_nodes(2);
+        const int r = l + 1;
+        if (failed) return;
         Node &L = bvh->nodes[l];
         L.bounds = lb; L.axis = -1; L.first = nd.first; L.count = mid - nd.first;
         Node &R = bvh->nodes[r];
@@ -390,7 +483,10 @@ int orc_morton_sort(const float *tri, uint64_t n, uint32_t *codes_out, uint32_t 
 }
 
 // hlbvh.cu:384-624 — the whole build. tri is [n][3][3] floats, kept by pointer.
-int orc_bvh_build(const float *tri, uint64_t n, orc_bvh **out, orc_build_info *info) {
+// threads > 1 splits the Morton pass, the sort, the top-level SAH (subtrees on
+// threads of their own, as the reference's ThreadPool, hlbvh.cu:559-641) and the
+// per-treelet bottom splits; the tree is the same for every thread count.
+int orc_bvh_build_threads(const float *tri, uint64_t n, int threads, orc_bvh **out, orc_build_info *info) {
     auto *bvh = new orc_bvh();
     bvh->tri = tri;
     bvh->n = n;
@@ -400,8 +496,9 @@ int orc_bvh_build(const float *tri, uint64_t n, orc_bvh **out, orc_build_info *i
         bvh->error = "too many primitives";
         return -1;
     }
-    compute_morton(tri, n, bvh->prims);
-    canonical_sort(bvh->prims);
+    threads = std::max(1, threads);
+    compute_morton(tri, n, bvh->prims, threads);
+    canonical_sort(bvh->prims, threads);
 
     Builder b{bvh};
     for (uint32_t i = 0; i < n;) {  // treelet runs of equal code >> 9 (hlbvh.cu:504-543)
@@ -417,15 +514,23 @@ int orc_bvh_build(const float *tri, uint64_t n, orc_bvh **out, orc_build_info *i
     }
     std::vector<uint32_t> ids(b.treelets.size());
     for (uint32_t i = 0; i < ids.size(); ++i) ids[i] = i;
-    bvh->nodes.reserve(4 * n);
-    b.new_node();
-    b.build_upper(0, ids);
+    bvh->nodes.reset(2 * n);
+    b.new_nodes(1);
+    int spawn = 0;
+    while (threads > 1 && (1 << spawn) < 4 * threads) ++spawn;  // up to ~4 subtrees per thread
+    b.build_upper(0, ids, spawn);
     if (b.failed) {
         bvh->error = b.why;
         return -2;
     }
     bvh->top_nodes = int(bvh->nodes.size());
-    for (int i = 0; i < bvh->top_nodes; ++i) b.build_bottom(i);
+    parallel_for(uint64_t(bvh->top_nodes), threads, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t i = lo; i < hi; ++i) b.build_bottom(int(i));
+    });
+    if (b.failed) {
+        bvh->error = b.why;
+        return -2;
+    }
     if (info) {
         info->num_treelets = b.treelets.size();
         info->top_nodes = bvh->top_nodes;
@@ -435,6 +540,10 @@ int orc_bvh_build(const float *tri, uint64_t n, orc_bvh **out, orc_build_info *i
         info->max_treelet = mx;
     }
     return 0;
+}
+
+int orc_bvh_build(const float *tri, uint64_t n, orc_bvh **out, orc_build_info *info) {
+    return orc_bvh_build_threads(tri, n, 1, out, info);
 }
 
 const char *orc_bvh_error(const orc_bvh *bvh) { return bvh ? bvh->error.c_str() : "null handle"; }

@@ -34,6 +34,8 @@ typedef struct orc_build_info {
 int orc_morton_codes(const float *tri, uint64_t n, uint32_t *codes_by_input);
 int orc_morton_sort(const float *tri, uint64_t n, uint32_t *codes_out, uint32_t *order_out);
 int orc_bvh_build(const float *tri, uint64_t n, orc_bvh **out, orc_build_info *info);
+/* the same tree, built with that many host threads (the CPU baseline's build) */
+int orc_bvh_build_threads(const float *tri, uint64_t n, int threads, orc_bvh **out, orc_build_info *info);
 const char *orc_bvh_error(const orc_bvh *bvh);
 void orc_bvh_free(orc_bvh *bvh);
 int orc_bvh_export_canonical(const orc_bvh *bvh, uint8_t *buf, uint64_t *len);

@@ -36,6 +36,7 @@ def load():
         "orc_morton_codes": (c_int, [p, c_uint64, p]),
         "orc_morton_sort": (c_int, [p, c_uint64, p, p]),
         "orc_bvh_build": (c_int, [p, c_uint64, POINTER(p), POINTER(BuildInfo)]),
+        "orc_bvh_build_threads": (c_int, [p, c_uint64, c_int, POINTER(p), POINTER(BuildInfo)]),
         "orc_bvh_error": (c_char_p, [p]),
         "orc_bvh_free": (None, [p]),
         "orc_bvh_export_canonical": (c_int, [p, p, POINTER(c_uint64)]),
@@ -72,12 +73,13 @@ def morton_sort(tri: np.ndarray):
 
 
 class OracleBvh:
-    def __init__(self, tri: np.ndarray):
+    def __init__(self, tri: np.ndarray, threads: int = 1):
         self.tri = np.ascontiguousarray(tri, np.float32)
         self.lib = load()
         self.h = c_void_p()
         self.info = BuildInfo()
-        rc = self.lib.orc_bvh_build(_p(self.tri), self.tri.shape[0], ctypes.byref(self.h), ctypes.byref(self.info))
+        rc = self.lib.orc_bvh_build_threads(_p(self.tri), self.tri.shape[0], int(threads), ctypes.byref(self.h),
+                                            ctypes.byref(self.info))
         if rc != 0:
             msg = self.lib.orc_bvh_error(self.h).decode()
             self.lib.orc_bvh_free(self.h)

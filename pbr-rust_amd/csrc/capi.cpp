@@ -889,15 +889,20 @@ int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *par
         }
     }
     const WideTrace W = scene_trace(sc);
-    CK(ctx, ctx->dev_err.reserve(64), "alloc");
-    CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream), "memset");
+    const bool count = (params->flags & PBRTM_RENDER_COUNT) != 0;
+    CK(ctx, ctx->dev_err.reserve(128), "alloc");
+    CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, count ? 128 : 4, ctx->stream), "memset");
+    unsigned long long *d_counts = count ? reinterpret_cast<unsigned long long *>(ctx->dev_err.as<uint8_t>() + 64)
+                                         : nullptr;
     pbrtm::pt::RenderStats rs;
     CK(ctx, pbrtm::pt::render(sc->dev, W, spp, params->max_depth, pool, sc->pool, d_rgb, d_aux,
                               ctx->dev_err.as<uint32_t>(), &rs, ctx->stream, nullptr, nullptr, nullptr,
-                              (params->flags & PBRTM_INTEGRATOR_MASK) >> 4),
+                              (params->flags & PBRTM_INTEGRATOR_MASK) >> 4, d_counts),
        "render");
     uint32_t derr = 0;
+    unsigned long long hcounts[6] = {0, 0, 0, 0, 0, 0};
     CK(ctx, hipMemcpyAsync(&derr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+    if (count) CK(ctx, hipMemcpyAsync(hcounts, d_counts, sizeof(hcounts), hipMemcpyDeviceToHost, ctx->stream), "download");
     if (!dev) {
         CK(ctx, hipMemcpyAsync(film_rgb, d_rgb, wh * 12, hipMemcpyDeviceToHost, ctx->stream), "download");
         if (film_aux) CK(ctx, hipMemcpyAsync(film_aux, d_aux, wh * 16, hipMemcpyDeviceToHost, ctx->stream), "download");
@@ -916,6 +921,10 @@ int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *par
         stats->paths = rs.paths;
         stats->ray_order = rs.sorted;
         stats->reserved = 0;
+        for (int k = 0; k < 3; ++k) {
+            stats->closest_counts[k] = hcounts[k];
+            stats->shadow_counts[k] = hcounts[3 + k];
+        }
     }
     return PBRTM_OK;
 }
@@ -1023,7 +1032,9 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
                        pbrtm_batch_stats *stats) {
     if (!ctx || !jobs || !params || njobs == 0) return PBRTM_ERR_INVALID;
     const bool dev = params->flags & PBRTM_DEVICE_PTRS;
-    uint64_t film_floats = 0, paths = 0;
+    uint64_t film_floats = 0, paths = 0, shared_paths = 0;
+    const uint64_t solo =
+        (params->solo_paths == 0 || params->solo_paths == 0xFFFFFFFFu) ? ~uint64_t(0) : params->solo_paths;
     for (uint32_t j = 0; j < njobs; ++j) {
         const pbrtm_job &jb = jobs[j];
         if (!jb.scene || !jb.film_rgb) return fail(ctx, PBRTM_ERR_INVALID, "job without scene or film");
@@ -1037,9 +1048,12 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
         const uint64_t wh = uint64_t(jb.scene->host.width) * jb.scene->host.height;
         film_floats += wh * (jb.film_aux ? 7 : 3);
         paths += wh * jb.params.spp;
+        if (wh * jb.params.spp < solo) shared_paths += wh * jb.params.spp;  // the jobs of the shared pool
     }
-    const uint64_t slots = std::min<uint64_t>(params->pool_size ? params->pool_size : (4u << 20), paths);
-    if (paths + slots >= 0xFFFFFFFFull)
+    // the shared pool's slots: pool_size, or 2^24, capped at its jobs' path count (as render_batch does)
+    const uint32_t pool = params->pool_size ? params->pool_size : (16u << 20);
+    const uint64_t slots = std::min<uint64_t>(pool, shared_paths);
+    if (shared_paths + slots >= 0xFFFFFFFFull || paths >= 0xFFFFFFFFull)
         return fail(ctx, PBRTM_ERR_UNSUPPORTED, "at most 2^32 - 1 - pool_size paths per batch");
     CK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     if (!dev) CK(ctx, ctx->batch_films.reserve(film_floats * 4), "alloc films");
@@ -1063,13 +1077,10 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
             if (jb.film_aux) f += wh * 4;
         }
     }
-    const uint32_t pool = params->pool_size ? params->pool_size : (16u << 20);
     const uint32_t rpl = params->rounds_per_launch ? params->rounds_per_launch : 8u;
     CK(ctx, ctx->dev_err.reserve(64), "alloc");
     CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream), "memset");
     pbrtm::pt::BatchStats bs;
-    const uint64_t solo =
-        (params->solo_paths == 0 || params->solo_paths == 0xFFFFFFFFu) ? ~uint64_t(0) : params->solo_paths;
     CK(ctx, pbrtm::pt::render_batch(bj, pool, rpl, !(params->flags & PBRTM_BATCH_NO_GRAPH), solo, ctx->batch_pool,
                                     ctx->batch_jobs, ctx->dev_err.as<uint32_t>(), &bs, ctx->stream),
        "render batch");
@@ -1097,7 +1108,7 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
         stats->rays = bs.rays;
         stats->shadow_rays = bs.shadow_rays;
         stats->paths = bs.paths;
-        stats->pool_size = uint32_t(std::min<uint64_t>(pool, paths));
+        stats->pool_size = uint32_t(slots);  // 0 when every job rendered alone
         stats->launches = bs.launches;
         stats->solo_jobs = bs.solo;
         stats->reserved = 0;

@@ -158,10 +158,14 @@ struct RenderStats {
 // [h][w][4], nullable) receive RGBFilm::get_pixel_rgb and (rgb_sum, weight_sum).
 // lfin / pfin (nullable, per path id) receive each path's final L and wavelength
 // pdfs; with gd the render is a gradient replay (film_rgb may then be null).
+// counts (device, nullable, 6 words zeroed by the caller): the traversal's totals,
+// [visits, box tests, primitive tests] of the closest-hit then the shadow rays,
+// from the counting instance of the traversal kernel.
 hipError_t render(const SceneDev &S, const WideTrace &W, uint32_t spp, uint32_t max_depth, uint32_t pool,
                   PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
                   hipStream_t stream, float4 *lfin = nullptr, float4 *pfin = nullptr,
-                  const GradDev *gd = nullptr, uint32_t integrator = kIntPath);
+                  const GradDev *gd = nullptr, uint32_t integrator = kIntPath,
+                  unsigned long long *counts = nullptr);
 
 // Marks the leaf positions of analytic primitives for the traversal: the word after
 // a position's nine coordinates receives prim_shape[prim_order[k]] (shape index + 1).

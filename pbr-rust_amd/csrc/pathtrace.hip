@@ -1225,7 +1225,8 @@ hipError_t tag_leaf_shapes(float *tri_leaf, const uint32_t *prim_order, const ui
 
 hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t max_depth, uint32_t pool,
                   PathPool &pp, float *film_rgb, float *film_aux, uint32_t *err, RenderStats *stats,
-                  hipStream_t stream, float4 *lfin, float4 *pfin, const GradDev *gd, uint32_t integrator) {
+                  hipStream_t stream, float4 *lfin, float4 *pfin, const GradDev *gd, uint32_t integrator,
+                  unsigned long long *counts) {
     if (integrator > kIntSurfaceNormal || (gd && integrator != kIntPath)) return hipErrorInvalidValue;
     const uint64_t total = uint64_t(S.width) * S.height * spp;
     const uint32_t spd = uint32_t(sqrt(double(spp)));
@@ -1307,10 +1308,10 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
                             stream));
             PT_CK(trace_wide_queued(W, reinterpret_cast<const float *>(st.rays), pool, 0, pp.prim.as<uint32_t>(),
                                     pp.thit.as<float>(), pp.bary.as<float>(), err, q, st.counters + kCntQueue,
-                                    stream));
+                                    stream, counts));
         } else {
             PT_CK(trace_wide(W, reinterpret_cast<const float *>(st.rays), pool, 0, pp.prim.as<uint32_t>(),
-                             pp.thit.as<float>(), pp.bary.as<float>(), err, nullptr, pp.queue.as<uint32_t>(),
+                             pp.thit.as<float>(), pp.bary.as<float>(), err, counts, pp.queue.as<uint32_t>(),
                              st.counters + kCntQueue, stream));
         }
         PT_CK(hipEventRecord(E[1], stream));
@@ -1321,11 +1322,11 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
                             nullptr, err, pp.rsort, st.counters + kCntSQueue, &q, stream));
             PT_CK(trace_wide_queued(W, reinterpret_cast<const float *>(st.srays), 2 * pool, 1,
                                     pp.sprim.as<uint32_t>(), nullptr, nullptr, err, q, st.counters + kCntSQueue,
-                                    stream));
+                                    stream, counts ? counts + 3 : nullptr));
         } else {
             PT_CK(trace_wide(W, reinterpret_cast<const float *>(st.srays), 2 * pool, 1, pp.sprim.as<uint32_t>(),
-                             nullptr, nullptr, err, nullptr, pp.squeue.as<uint32_t>(), st.counters + kCntSQueue,
-                             stream));
+                             nullptr, nullptr, err, counts ? counts + 3 : nullptr, pp.squeue.as<uint32_t>(),
+                             st.counters + kCntSQueue, stream));
         }
         PT_CK(hipEventRecord(E[2], stream));
         if (gd) shade_grad_kernel<<<grid, kBlock, 0, stream>>>(S, st, P);

@@ -91,9 +91,10 @@ hipError_t trace_wide_multi_queued(const WideTrace &B, const WideTrace *jobs, co
                                    uint32_t job_shift, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
                                    float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
                                    const uint32_t *queue_len, hipStream_t stream);
+// counts (device, nullable): adds [nodes visited, child box tests, primitive tests].
 hipError_t trace_wide_queued(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
                              float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
-                             const uint32_t *queue_len, hipStream_t stream);
+                             const uint32_t *queue_len, hipStream_t stream, unsigned long long *counts = nullptr);
 
 constexpr uint32_t kLdsStack = 12;
 

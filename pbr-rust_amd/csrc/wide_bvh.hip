@@ -951,10 +951,10 @@ hipError_t trace_wide_multi_queued(const WideTrace &B, const WideTrace *jobs, co
 
 hipError_t trace_wide_queued(const WideTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
                              float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
-                             const uint32_t *queue_len, hipStream_t stream) {
+                             const uint32_t *queue_len, hipStream_t stream, unsigned long long *counts) {
     if (n == 0) return hipSuccess;
     return trace_wide_queued_counts(B, reinterpret_cast<const float4 *>(rays), n, any_hit, prim_out, t_out, bary_out,
-                                    err, nullptr, queue, queue_len, stream);
+                                    err, counts, queue, queue_len, stream);
 }
 
 hipError_t trace_wide_multi(const WideTrace &B, const WideTrace *jobs, const uint32_t *ray_job, uint32_t job_shift,

@@ -38,6 +38,7 @@ EXPORTS = (
     "pbrtm_render_grad", "pbrtm_integrator_from_name", "pbrtm_write_pfm", "pbrtm_bvh_intersect_noalloc",
 )
 BATCH_NO_GRAPH = 2
+RENDER_COUNT = 0x100
 # pbrtm_render_params.flags: the integrator
 INTEGRATORS = {"path": 0x00, "volpath": 0x00, "megakernelpath": 0x10, "ambientocclusion": 0x20,
                "surfacenormal": 0x30}
@@ -95,10 +96,14 @@ class RenderParams(ctypes.Structure):
 class RenderStats(ctypes.Structure):
     _fields_ = [("ms_total", c_double), ("ms_trace", c_double), ("ms_shadow", c_double), ("ms_shade", c_double),
                 ("ms_film", c_double), ("rounds", c_uint64), ("rays", c_uint64), ("shadow_rays", c_uint64),
-                ("paths", c_uint64), ("ray_order", c_uint32), ("reserved", c_uint32)]
+                ("paths", c_uint64), ("ray_order", c_uint32), ("reserved", c_uint32),
+                ("closest_counts", c_uint64 * 3), ("shadow_counts", c_uint64 * 3)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        out = {name: getattr(self, name) for name, _ in self._fields_}
+        out["closest_counts"] = list(self.closest_counts)
+        out["shadow_counts"] = list(self.shadow_counts)
+        return out
 
 
 class GradStats(ctypes.Structure):
@@ -607,10 +612,13 @@ class Scene:
         return loss.value, d_alb, d_sc[:self.num_lights], film, st.as_dict()
 
     def render(self, spp=4, max_depth=5, pool_size=0, film_rgb=None, film_aux=None, device_ptrs=False,
-               integrator="path"):
+               integrator="path", count=False):
         """Returns (film_rgb [h][w][3], film_aux [h][w][4], stats dict). integrator: "path"
-        (wavefront), "megakernelpath", "ambientocclusion" or "surfacenormal"."""
-        p = RenderParams(spp, max_depth, pool_size, (DEVICE_PTRS if device_ptrs else 0) | INTEGRATORS[integrator])
+        (wavefront), "megakernelpath", "ambientocclusion" or "surfacenormal". count=True
+        traces with the counting traversal: stats["closest_counts"] / ["shadow_counts"]
+        then hold [nodes visited, box tests, primitive tests]."""
+        p = RenderParams(spp, max_depth, pool_size, (DEVICE_PTRS if device_ptrs else 0) | INTEGRATORS[integrator]
+                         | (RENDER_COUNT if count else 0))
         st = RenderStats()
         if not device_ptrs:
             film_rgb = np.zeros((self.height, self.width, 3), np.float32)

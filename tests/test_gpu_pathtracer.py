@@ -210,3 +210,27 @@ def test_pool_larger_than_image_and_tiny_pool(ctx):
     for pool in (1, 7, 1 << 22):
         rgb, aux, orgb, oaux = render_both(ctx, sc, 4, 4, pool=pool)
         compare(rgb, orgb, f"film pool {pool}")
+
+
+def test_counting_render_is_the_same_render(ctx):
+    """The counting traversal (used to price a render's traversal bytes) changes no
+    film bit and no ray count, and its totals are those of a real traversal."""
+    sc = scenes.sphere_grid(width=48, height=32, spp=4)
+    s = ctx.scene(sc)
+    try:
+        rgb, aux, st = s.render(spp=4, max_depth=6)
+        crgb, caux, cst = s.render(spp=4, max_depth=6, count=True)
+    finally:
+        s.free()
+    log(f"counts closest {cst['closest_counts']} shadow {cst['shadow_counts']} "
+        f"rays {cst['rays']} shadow rays {cst['shadow_rays']}")
+    assert np.array_equal(rgb.view(np.uint32), crgb.view(np.uint32))
+    assert np.array_equal(aux.view(np.uint32), caux.view(np.uint32))
+    assert (st["rays"], st["shadow_rays"]) == (cst["rays"], cst["shadow_rays"])
+    assert st["closest_counts"] == [0, 0, 0] and st["shadow_counts"] == [0, 0, 0]
+    for key, n in (("closest_counts", cst["rays"]), ("shadow_counts", cst["shadow_rays"])):
+        visits, boxes, tris = cst[key]
+        # eight child boxes per visit, plus the re-tests of boxes popped after a
+        # leaf of the same node shortened the ray
+        assert n > 0 and visits >= n // 2 and tris > 0, key
+        assert 8 * visits <= boxes <= 16 * visits, key

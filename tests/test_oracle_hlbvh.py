@@ -97,6 +97,19 @@ def test_tree_equals_python_restatement_clustered():
     assert b.export_canonical() == hlbvh_py.build_canonical(tri)
 
 
+@pytest.mark.parametrize("threads", [2, 5, 16])
+def test_threaded_build_is_the_same_tree(threads):
+    """The CPU baseline's threaded build (parallel Morton pass and sort, subtrees of
+    the top-level SAH on threads of their own, parallel bottom splits) exports the
+    single-threaded build's bytes, clustered duplicates included."""
+    tri = soup(200_000, seed=11)
+    tri[1000:1400] = tri[999]
+    one = O.OracleBvh(tri)
+    many = O.OracleBvh(tri, threads=threads)
+    assert many.info.total_nodes == one.info.total_nodes and many.info.top_nodes == one.info.top_nodes
+    assert many.export_canonical() == one.export_canonical()
+
+
 def test_tree_invariants_and_split_rules():
     tri = soup(30000, seed=11)
     b = O.OracleBvh(tri)
