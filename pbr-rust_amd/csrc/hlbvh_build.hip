@@ -639,6 +639,10 @@ __global__ __launch_bounds__(64) void children_kernel(BuildArgs A, const WorkIte
 constexpr int kFinBlock = 256;
 constexpr int kFinWaves = kFinBlock / kWave;
 
+// Top-level nodes of at most this many treelets are split by one lane each
+// (after the wave has split the larger ones cooperatively).
+constexpr uint32_t kLaneNode = 8;
+
 // Per-wave LDS working set of the finisher.
 struct WaveLds {
     Rec rec[kSmallNode];
@@ -647,6 +651,9 @@ struct WaveLds {
     uint32_t bkt[kSmallNode];
     float cost[kNumBuckets];
     uint2 leaf[kSmallNode];  // top level: (node id, wave-local index) of single-treelet leaves
+    uint32_t acc[kNumBuckets][7];    // top level: bucket prim count and order-mapped box
+    uint32_t leaf_node[kSmallNode];  // top level: node id of the single-treelet leaf at each position
+    uint4 small[kSmallNode];         // top level: (lo, hi, node id) of nodes left to the lane phase
 };
 
 // Builds the whole subtree of one node with m0 <= 64 elements inside one wave.
@@ -865,6 +872,276 @@ __device__ void lane_build_bottom(const BuildArgs &A, uint32_t first, uint32_t c
     }
 }
 
+// SAH subtree of one top-level node of 2..kLaneNode treelets, built by one lane
+// over the wave's LDS records [lo, hi) (which no other lane touches). The split
+// positions between two occupied buckets all give the same partition and the
+// same cost, so the candidates are the occupied buckets but the last, in
+// increasing order; the first strict minimum among them is the reference's
+// first strict minimum over all 23 (hlbvh.cu:711-741). Each candidate's sides
+// are unions over the node's elements (exact min / max, integer counts), the
+// same boxes and counts as the reference's bucket unions. The partition is in
+// place and unstable: the order of treelets inside a node changes no later
+// decision. Pending nodes (at most three: the smaller child is taken next) sit
+// in a register stack.
+__device__ __forceinline__ void lane_sah_small(const BuildArgs &A, WaveLds &L, uint32_t lo, uint32_t hi, uint32_t id) {
+    uint32_t st_r[4] = {0, 0, 0, 0}, st_id[4] = {0, 0, 0, 0};
+    int sp = 0;
+    for (;;) {
+        const uint32_t m = hi - lo;
+        if (m == 1) {
+            L.leaf_node[lo] = id;
+            if (sp == 0) return;
+            --sp;
+            lo = st_r[0] & 0xFFu;
+            hi = st_r[0] >> 8;
+            id = st_id[0];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { st_r[k] = st_r[k + 1]; st_id[k] = st_id[k + 1]; }
+            continue;
+        }
+        uint32_t v[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
+        for (uint32_t j = lo; j < hi; ++j) {
+            const Box b = rec_box(L.rec[j]);
+            const f3 c = box_centroid(b);
+            v[0] = min(v[0], f2ord(c.x)); v[1] = min(v[1], f2ord(c.y)); v[2] = min(v[2], f2ord(c.z));
+            v[3] = max(v[3], f2ord(c.x)); v[4] = max(v[4], f2ord(c.y)); v[5] = max(v[5], f2ord(c.z));
+            v[6] = min(v[6], f2ord(b.lo.x)); v[7] = min(v[7], f2ord(b.lo.y)); v[8] = min(v[8], f2ord(b.lo.z));
+            v[9] = max(v[9], f2ord(b.hi.x)); v[10] = max(v[10], f2ord(b.hi.y)); v[11] = max(v[11], f2ord(b.hi.z));
+        }
+        const Box cb = acc_box(v);
+        const Box fb = acc_box(v + 6);
+        const int axis = box_max_dim(cb);
+        if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) {
+            atomicOr(A.err, uint32_t(kErrDegenerateCentroids));
+            return;
+        }
+        const float base = sel3(cb.lo, axis);
+        const float span = sel3(cb.hi, axis) - sel3(cb.lo, axis);
+        uint64_t bpack = 0;  // 5 bits per element
+        uint32_t occ = 0;
+        for (uint32_t j = lo; j < hi; ++j) {
+            const uint32_t bk = bucket_of(sel3(box_centroid(rec_box(L.rec[j])), axis), base, span);
+            bpack |= uint64_t(bk) << (5 * (j - lo));
+            occ |= 1u << bk;
+        }
+        const float total_sa = box_area(fb);
+        const uint32_t last = 31u - __builtin_clz(occ);
+        uint32_t cand = occ & ~(1u << last);
+        uint32_t split = 0, lp = 0;
+        float best = 0.f;
+        bool first = true;
+        while (cand) {
+            const uint32_t sb = __builtin_ctz(cand);
+            cand &= cand - 1;
+            Box bl = box_empty(), br = box_empty();
+            uint32_t cl = 0, cr = 0;
+            for (uint32_t j = lo; j < hi; ++j) {
+                const Rec q = L.rec[j];
+                if (uint32_t((bpack >> (5 * (j - lo))) & 31u) <= sb) { bl = box_union(bl, rec_box(q)); cl += q.b; }
+                else { br = box_union(br, rec_box(q)); cr += q.b; }
+            }
+            const float cost = 0.125 + (float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa;
+            if (first || cost < best) {
+                best = cost;
+                split = sb;
+                lp = cl;
+                first = false;
+            }
+        }
+        // two-pointer partition on bucket <= split
+        uint32_t i = lo, j = hi;
+        while (i < j) {
+            if (uint32_t((bpack >> (5 * (i - lo))) & 31u) <= split) {
+                ++i;
+            } else {
+                --j;
+                const Rec ri = L.rec[i];
+                L.rec[i] = L.rec[j];
+                L.rec[j] = ri;
+                const uint64_t mi = 31ull << (5 * (i - lo)), mj = 31ull << (5 * (j - lo));
+                const uint64_t bi = (bpack & mi) >> (5 * (i - lo)), bj = (bpack & mj) >> (5 * (j - lo));
+                bpack = (bpack & ~(mi | mj)) | (bj << (5 * (i - lo))) | (bi << (5 * (j - lo)));
+            }
+        }
+        const uint32_t nl = i - lo;
+        if (nl == 0 || nl == m) {  // unreachable: the first and last buckets are both occupied
+            atomicOr(A.err, uint32_t(kErrSplit));
+            return;
+        }
+        write_interior(A.nodes, id, fb, lp, uint32_t(axis));
+        // children: left [lo, lo + nl) as id + 1, right [lo + nl, hi) as id + 2 lp; the
+        // smaller one next, the larger one pending
+        const uint32_t llo = lo, lhi = lo + nl, lid = id + 1;
+        const uint32_t rlo = lo + nl, rhi = hi, rid = id + 2 * lp;
+        const bool left_first = nl <= m - nl;
+        const uint32_t push_r = left_first ? (rlo | (rhi << 8)) : (llo | (lhi << 8));
+        const uint32_t push_id = left_first ? rid : lid;
+#pragma unroll
+        for (int k = 3; k > 0; --k) { st_r[k] = st_r[k - 1]; st_id[k] = st_id[k - 1]; }
+        st_r[0] = push_r;
+        st_id[0] = push_id;
+        ++sp;
+        if (sp > 4) {  // cannot happen for <= 8 elements taken smaller side first
+            atomicOr(A.err, uint32_t(kErrSplit));
+            return;
+        }
+        lo = left_first ? llo : rlo;
+        hi = left_first ? lhi : rhi;
+        id = left_first ? lid : rid;
+    }
+}
+
+// SAH subtree of one top-level node of m0 <= 64 treelets inside one wave
+// (hlbvh.cu:649-826). Nodes larger than kLaneNode are split by the whole wave:
+// element per lane, bounds by wave reductions, bucket statistics by LDS atomics
+// (exact min / max of order-mapped floats, integer counts), then, as at the level
+// path, lane k holds bucket k and inclusive prefix and suffix unions give it the
+// cost of the split after bucket k (hlbvh.cu:711-731); the lowest lane holding
+// the least cost is the first strict minimum (hlbvh.cu:734-741). Smaller nodes
+// are collected and split one per lane (lane_sah_small). Every position ends
+// with the node id of its single-treelet leaf in L.leaf_node.
+__device__ __forceinline__ void wave_build_top(const BuildArgs &A, const Rec *src, uint32_t begin, uint32_t m0, uint32_t node0,
+                               WaveLds &L) {
+    const uint32_t lane = __lane_id();
+    if (lane < m0) L.rec[lane] = src[begin + lane];
+    if (lane == 0) L.stack[0] = make_uint4(0, m0, node0, 0);
+    int sp = 1;
+    uint32_t nsmall = 0;
+    if (m0 <= kLaneNode) {  // straight to the lane phase
+        sp = 0;
+        if (lane == 0) L.small[0] = make_uint4(0, m0, node0, 0);
+        nsmall = 1;
+    }
+    wave_lds_sync();
+    while (sp > 0) {
+        --sp;
+        const uint4 nd = L.stack[sp];
+        const uint32_t lo = nd.x, hi = nd.y, id = nd.z, m = hi - lo;
+        const bool active = lane >= lo && lane < hi;
+        Rec r;
+        if (active) r = L.rec[lane];
+        const Box b = active ? rec_box(r) : box_empty();
+        const f3 c = box_centroid(b);
+        uint32_t v[12];
+        v[0] = active ? f2ord(c.x) : 0xFFFFFFFFu; v[1] = active ? f2ord(c.y) : 0xFFFFFFFFu;
+        v[2] = active ? f2ord(c.z) : 0xFFFFFFFFu;
+        v[3] = active ? f2ord(c.x) : 0u; v[4] = active ? f2ord(c.y) : 0u; v[5] = active ? f2ord(c.z) : 0u;
+        v[6] = active ? f2ord(b.lo.x) : 0xFFFFFFFFu; v[7] = active ? f2ord(b.lo.y) : 0xFFFFFFFFu;
+        v[8] = active ? f2ord(b.lo.z) : 0xFFFFFFFFu;
+        v[9] = active ? f2ord(b.hi.x) : 0u; v[10] = active ? f2ord(b.hi.y) : 0u; v[11] = active ? f2ord(b.hi.z) : 0u;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? wave_min_u32_all(v[k]) : wave_max_u32_all(v[k]);
+        const Box cb = acc_box(v);
+        const Box fb = acc_box(v + 6);
+        const int axis = box_max_dim(cb);
+        if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) {
+            if (lane == 0) atomicOr(A.err, uint32_t(kErrDegenerateCentroids));
+            continue;
+        }
+        const float base = sel3(cb.lo, axis);
+        const float span = sel3(cb.hi, axis) - sel3(cb.lo, axis);
+        const uint32_t bk = bucket_of(sel3(c, axis), base, span);
+        if (lane < kNumBuckets) {
+            L.acc[lane][0] = 0u;
+#pragma unroll
+            for (int k = 1; k < 7; ++k) L.acc[lane][k] = k < 4 ? 0xFFFFFFFFu : 0u;
+        }
+        wave_lds_sync();
+        if (active) {
+            atomicAdd(&L.acc[bk][0], r.b);
+            atomicMin(&L.acc[bk][1], f2ord(b.lo.x)); atomicMin(&L.acc[bk][2], f2ord(b.lo.y));
+            atomicMin(&L.acc[bk][3], f2ord(b.lo.z));
+            atomicMax(&L.acc[bk][4], f2ord(b.hi.x)); atomicMax(&L.acc[bk][5], f2ord(b.hi.y));
+            atomicMax(&L.acc[bk][6], f2ord(b.hi.z));
+        }
+        wave_lds_sync();
+        // lane k: bucket k; an empty bucket is the empty box
+        const bool in = lane < kNumBuckets;
+        const uint32_t cnt = in ? L.acc[lane][0] : 0u;
+        Box bb = box_empty();
+        if (in && cnt > 0)
+            bb = {{ord2f(L.acc[lane][1]), ord2f(L.acc[lane][2]), ord2f(L.acc[lane][3])},
+                  {ord2f(L.acc[lane][4]), ord2f(L.acc[lane][5]), ord2f(L.acc[lane][6])}};
+        float pl[6] = {bb.lo.x, bb.lo.y, bb.lo.z, bb.hi.x, bb.hi.y, bb.hi.z};
+        float sf[6] = {bb.lo.x, bb.lo.y, bb.lo.z, bb.hi.x, bb.hi.y, bb.hi.z};
+        uint32_t pc = cnt, sc = cnt;
+#pragma unroll
+        for (int d = 1; d < 32; d <<= 1) {
+            const bool up = int(lane) >= d;
+            const bool dn = int(lane) + d < int(kNumBuckets);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const float u = __shfl_up(pl[k], d, kWave);
+                const float w = __shfl_down(sf[k], d, kWave);
+                if (up) pl[k] = k < 3 ? smin(u, pl[k]) : smax(u, pl[k]);
+                if (dn) sf[k] = k < 3 ? smin(sf[k], w) : smax(sf[k], w);
+            }
+            const uint32_t uc = __shfl_up(pc, d, kWave), wc = __shfl_down(sc, d, kWave);
+            if (up) pc += uc;
+            if (dn) sc += wc;
+        }
+        float r6[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) r6[k] = __shfl_down(sf[k], 1, kWave);
+        const uint32_t rc = __shfl_down(sc, 1, kWave);
+        const float total_sa = box_area(fb);
+        uint32_t key = 0xFFFFFFFFu;
+        if (lane < kNumBuckets - 1) {
+            const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
+            const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
+            const float cost = 0.125 + (float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa;
+            key = f2ord(cost);
+        }
+        const uint32_t kmin = wave_min_u32_all(key);
+        const uint32_t split = uint32_t(__builtin_ctzll(__ballot(lane < kNumBuckets - 1 && key == kmin)));
+        uint32_t lp = __shfl(pc, int(split), kWave);
+        bool left = active && bk <= split;
+        const uint64_t lm = __ballot(left);
+        if (lm == 0 || __popcll(lm) == m) {  // midpoint fallback (hlbvh.cu:758-777)
+            const float mid = sel3(box_centroid(cb), axis);
+            left = active && sel3(c, axis) < mid;
+            lp = wave_sum_all(left ? r.b : 0u);
+        }
+        const uint64_t lmask = __ballot(left);
+        const uint64_t rmask = __ballot(active && !left);
+        const uint32_t Lc = __popcll(lmask);
+        if (Lc == 0 || Lc == m) {
+            if (lane == 0) atomicOr(A.err, uint32_t(kErrSplit));
+            continue;
+        }
+        if (active) {
+            const uint32_t dst = left ? lo + __popcll(lmask & lanemask_lt())
+                                      : lo + Lc + __popcll(rmask & lanemask_lt());
+            L.tmp[dst] = r;
+        }
+        wave_lds_sync();
+        if (active) L.rec[lane] = L.tmp[lane];
+        if (lane == 0) write_interior(A.nodes, id, fb, lp, axis);
+        const uint4 ch[2] = {make_uint4(lo, lo + Lc, id + 1, 0), make_uint4(lo + Lc, hi, id + 2 * lp, 0)};
+        for (int k = 1; k >= 0; --k) {
+            const uint4 cn = ch[k];
+            const uint32_t cs = cn.y - cn.x;
+            if (cs == 1) {
+                if (lane == 0) L.leaf_node[cn.x] = cn.z;
+            } else if (cs <= kLaneNode) {
+                if (lane == 0) L.small[nsmall] = cn;
+                ++nsmall;
+            } else {
+                if (lane == 0) L.stack[sp] = cn;
+                ++sp;
+            }
+        }
+        wave_lds_sync();
+    }
+    for (uint32_t k = lane; k < nsmall; k += kWave) {
+        const uint4 nd = L.small[k];
+        lane_sah_small(A, L, nd.x, nd.y, nd.z);
+    }
+    wave_lds_sync();
+}
+
 // Top-level finisher: one wave per node of <= 64 treelets. After the SAH
 // subtree, each single-treelet leaf becomes the root of a midpoint subtree over
 // its primitives, built in the same wave when it has <= 64 of them (the common
@@ -877,13 +1154,13 @@ __global__ __launch_bounds__(kFinBlock) void finish_top_kernel(BuildArgs A, cons
     if (wid >= nitems) return;
     WaveLds &L = s_lds[w];
     const WorkItem it = items[wid];
-    const uint32_t nleaf = wave_build<true>(A, pick(A.trec, it.buf), it.begin, it.end - it.begin, it.node, it.prims, L);
-    // lane k keeps leaf k: node id and the treelet's primitive range
+    const uint32_t nleaf = it.end - it.begin;
+    wave_build_top(A, pick(A.trec, it.buf), it.begin, nleaf, it.node, L);
+    // lane k keeps the leaf at position k: node id and the treelet's primitive range
     uint32_t my_node = 0, my_first = 0, my_count = 0;
     if (lane < nleaf) {
-        const uint2 lf = L.leaf[lane];
-        const Rec t = L.rec[lf.y];
-        my_node = lf.x;
+        const Rec t = L.rec[lane];
+        my_node = L.leaf_node[lane];
         my_first = t.a;
         my_count = t.b;
         if (my_count == 1) {

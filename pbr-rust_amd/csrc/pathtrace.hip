@@ -822,8 +822,12 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
     st.dim[i] = smp.dim;
 }
 
+// The path-integrator shading instances need about 172 VGPRs unbounded (2 waves
+// per SIMD); held to 168 (3 waves) they spill 8 bytes per lane, which is faster.
+#define PBRTM_SHADE_WAVES __attribute__((amdgpu_waves_per_eu(3)))
+
 template <uint32_t MODE>
-__global__ __launch_bounds__(kBlock) void shade_kernel(SceneDev S, State st, Params P) {
+__global__ __launch_bounds__(kBlock) PBRTM_SHADE_WAVES void shade_kernel(SceneDev S, State st, Params P) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pid = i < P.pool ? st.path_id[i] : kIdle;
     shade_impl<false, false, MODE>(S, st, P, i, pid, pid, P.spd, P.max_depth, P.round, nullptr, nullptr);
@@ -906,7 +910,7 @@ __global__ __launch_bounds__(kBlock) void shade_grad_kernel(SceneDev S, State st
 
 // Batched render: each slot reads its job's scene and parameters; the round
 // number comes from the device counter so the round can be replayed in a graph.
-__global__ __launch_bounds__(kBlock) void shade_multi_kernel(State st, Params P) {
+__global__ __launch_bounds__(kBlock) PBRTM_SHADE_WAVES void shade_multi_kernel(State st, Params P) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pid = i < P.pool ? st.path_id[i] : kIdle;
     const uint32_t j = pid != kIdle ? st.job[i] : 0u;

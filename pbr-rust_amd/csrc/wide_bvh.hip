@@ -65,7 +65,7 @@ __device__ __forceinline__ void encode_node(WNode *w, const float ulo[3], const 
         ex |= (uint32_t(e) & 0xFFu) << (8 * a);
     }
     uint64_t plo[3] = {0, 0, 0}, phi[3] = {0, 0, 0};
-#pragma unroll 1
+#pragma unroll
     for (int s = 0; s < kWide; ++s) {
         float l[3] = {0.f, 0.f, 0.f}, h[3] = {0.f, 0.f, 0.f};
         if (s < n) box(s, l, h);
@@ -128,9 +128,10 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
     const Job job = live ? jobs[i] : Job{0, 0, 0, 0};
     uint32_t fid[kWide];      // frontier binary ids, left to right
     float farea[kWide];       // surface area, or -1 for a leaf / unused slot
+    uint32_t fwa[kWide], fwb[kWide];  // the frontier nodes' a and b words (children, or leaf range)
     uint32_t oid[kWide - 1], ol[kWide - 1], orr[kWide - 1], oax[kWide - 1];  // opened nodes
 #pragma unroll
-    for (int k = 0; k < kWide; ++k) { fid[k] = 0u; farea[k] = -1.0f; }
+    for (int k = 0; k < kWide; ++k) { fid[k] = 0u; farea[k] = -1.0f; fwa[k] = fwb[k] = 0u; }
 #pragma unroll
     for (int k = 0; k < kWide - 1; ++k) { oid[k] = 0xFFFFFFFFu; ol[k] = orr[k] = oax[k] = 0u; }
     int nF = 0, nE = 0;
@@ -141,40 +142,82 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
         const BNode c0 = bn[ol[0]], c1 = bn[orr[0]];
         fid[0] = ol[0]; farea[0] = bnode_leaf(c0) ? -1.0f : box_area(bnode_box(c0));
         fid[1] = orr[0]; farea[1] = bnode_leaf(c1) ? -1.0f : box_area(bnode_box(c1));
+        fwa[0] = c0.a; fwb[0] = c0.b;
+        fwa[1] = c1.a; fwb[1] = c1.b;
         nF = 2;
-        while (nF < kWide) {  // open the frontier interior node of largest surface area
-            int best = -1;
-            float best_area = -1.0f;
+        while (nF < kWide) {
+            // open at once the (at most 8 - nF) interior frontier nodes of largest
+            // surface area, ties to the left: each step is one round of independent
+            // fetches (the opened nodes' children), and a full node takes about
+            // log2(8) steps instead of seven dependent ones
+            const int cap = kWide - nF;
+            bool op[kWide];
+            int nopen = 0;
 #pragma unroll
-            for (int k = 0; k < kWide; ++k)
-                if (farea[k] > best_area) { best_area = farea[k]; best = k; }
-            if (best < 0) break;
-            uint32_t bid = 0;
+            for (int q = 0; q < kWide; ++q) {
+                int rank = 0;
 #pragma unroll
-            for (int k = 0; k < kWide; ++k) bid = (k == best) ? fid[k] : bid;
-            const BNode c = bn[bid];
-            const uint32_t cl = c.a, cr = c.b & 0x3FFFFFFFu;
-            const BNode nl = bn[cl], nr = bn[cr];
-#pragma unroll
-            for (int k = 0; k < kWide - 1; ++k) {
-                const bool here = k == nE;
-                oid[k] = here ? bid : oid[k];
-                ol[k] = here ? cl : ol[k];
-                orr[k] = here ? cr : orr[k];
-                oax[k] = here ? (c.b >> 30) : oax[k];
+                for (int j = 0; j < kWide; ++j)
+                    rank += (farea[j] > farea[q] || (farea[j] == farea[q] && j < q)) ? 1 : 0;
+                op[q] = farea[q] >= 0.0f && rank < cap;
+                nopen += op[q] ? 1 : 0;
             }
-            ++nE;
-            const float al = bnode_leaf(nl) ? -1.0f : box_area(bnode_box(nl));
-            const float ar = bnode_leaf(nr) ? -1.0f : box_area(bnode_box(nr));
+            if (nopen == 0) break;
+            float la[kWide], ra[kWide];
+            uint32_t lw0[kWide], lw1[kWide], rw0[kWide], rw1[kWide];
 #pragma unroll
-            for (int k = kWide - 1; k >= 0; --k) {  // shift right of best by one, then place the children
-                const uint32_t prev_id = k > 0 ? fid[k - 1] : 0u;
-                const float prev_a = k > 0 ? farea[k - 1] : -1.0f;
-                if (k > best + 1) { fid[k] = prev_id; farea[k] = prev_a; }
-                else if (k == best + 1) { fid[k] = cr; farea[k] = ar; }
-                else if (k == best) { fid[k] = cl; farea[k] = al; }
+            for (int q = 0; q < kWide; ++q) {
+                la[q] = ra[q] = -1.0f;
+                lw0[q] = lw1[q] = rw0[q] = rw1[q] = 0u;
+                if (op[q]) {
+                    const BNode nl = bn[fwa[q]], nr = bn[fwb[q] & 0x3FFFFFFFu];
+                    la[q] = bnode_leaf(nl) ? -1.0f : box_area(bnode_box(nl));
+                    ra[q] = bnode_leaf(nr) ? -1.0f : box_area(bnode_box(nr));
+                    lw0[q] = nl.a; lw1[q] = nl.b;
+                    rw0[q] = nr.a; rw1[q] = nr.b;
+                }
             }
-            ++nF;
+            uint32_t nfid[kWide], nwa[kWide], nwb[kWide];
+            float nar[kWide];
+#pragma unroll
+            for (int k = 0; k < kWide; ++k) { nfid[k] = 0u; nwa[k] = nwb[k] = 0u; nar[k] = -1.0f; }
+            int pos = 0;
+#pragma unroll
+            for (int q = 0; q < kWide; ++q) {
+                if (q < nF) {
+                    const bool o = op[q];
+                    const uint32_t cl = fwa[q], cr = fwb[q] & 0x3FFFFFFFu;
+                    if (o) {  // the opened node joins the opened list
+#pragma unroll
+                        for (int k = 0; k < kWide - 1; ++k) {
+                            const bool here = k == nE;
+                            oid[k] = here ? fid[q] : oid[k];
+                            ol[k] = here ? cl : ol[k];
+                            orr[k] = here ? cr : orr[k];
+                            oax[k] = here ? (fwb[q] >> 30) : oax[k];
+                        }
+                        ++nE;
+                    }
+#pragma unroll
+                    for (int k = 0; k < kWide; ++k) {
+                        if (k == pos) {
+                            nfid[k] = o ? cl : fid[q];
+                            nar[k] = o ? la[q] : farea[q];
+                            nwa[k] = o ? lw0[q] : fwa[q];
+                            nwb[k] = o ? lw1[q] : fwb[q];
+                        } else if (o && k == pos + 1) {
+                            nfid[k] = cr;
+                            nar[k] = ra[q];
+                            nwa[k] = rw0[q];
+                            nwb[k] = rw1[q];
+                        }
+                    }
+                    pos += o ? 2 : 1;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kWide; ++k) { fid[k] = nfid[k]; farea[k] = nar[k]; fwa[k] = nwa[k]; fwb[k] = nwb[k]; }
+            nF = pos;
         }
     }
     // interior children need wide node indices: one atomic per wave
@@ -199,7 +242,6 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
 #pragma unroll
     for (int s = 0; s < kWide; ++s) {
         if (s < nF) {
-            const BNode c = bn[fid[s]];
             if (farea[s] >= 0.0f) {
                 const uint32_t idx = base + k_int;
                 next[jbase + k_int] = Job{fid[s], idx, job.depth + 1, 0};
@@ -207,35 +249,25 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
                 refw[s][0] = idx;
                 refw[s][1] = 0u;
             } else {
-                refw[s][0] = c.a;  // kLeafFlag | first (same bit as kWideLeaf)
-                refw[s][1] = c.b;
+                refw[s][0] = fwa[s];  // kLeafFlag | first (same bit as kWideLeaf)
+                refw[s][1] = fwb[s];
             }
         } else {
             refw[s][0] = kWideEmpty;
             refw[s][1] = 0u;
         }
     }
-    // child boxes re-read from the binary nodes (cached) rather than held in registers;
-    // their union is the collapsed node's own box
-    auto fidx = [&](int s) {
-        uint32_t v = 0;
-#pragma unroll
-        for (int k = 0; k < kWide; ++k) v = (k == s) ? fid[k] : v;
-        return v;
-    };
     WNode *w = wn + job.wnode;
-    const BNode root = bn[job.bnode];
-    encode_node(w, root.lo, root.hi, [&](int s, float *l, float *h) {
-        const BNode c = bn[fidx(s)];
-        for (int a = 0; a < 3; ++a) l[a] = c.lo[a], h[a] = c.hi[a];
-    }, nF);
     uint4 *wr = reinterpret_cast<uint4 *>(w->ref);
 #pragma unroll
     for (int k = 0; k < 4; ++k) wr[k] = make_uint4(refw[2 * k][0], refw[2 * k][1], refw[2 * k + 1][0], refw[2 * k + 1][1]);
     // depth-first, near-first slot order per octant (the binary traversal's order).
     // Codes: 0..7 = frontier slot, 8..14 = opened node; the DFS stack is eight
     // 4-bit codes packed in one register.
-    uint32_t lcode[kWide - 1], rcode[kWide - 1];
+    // per opened node k: its left and right codes in byte k of lr (left in the low
+    // nibble) and its split axis in bits 2k..2k+1 of axes
+    uint64_t lr = 0;
+    uint32_t axes = 0;
 #pragma unroll
     for (int k = 0; k < kWide - 1; ++k) {
         uint32_t lc = 0xFu, rc = 0xFu;
@@ -249,9 +281,11 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
             if (s < nF && fid[s] == ol[k]) lc = s;
             if (s < nF && fid[s] == orr[k]) rc = s;
         }
-        lcode[k] = lc;
-        rcode[k] = rc;
+        lr |= uint64_t(lc | (rc << 4)) << (8 * k);
+        axes |= (oax[k] & 3u) << (2 * k);
     }
+    uint32_t *ow = order + size_t(job.wnode) * 8;
+#pragma unroll 1
     for (uint32_t oct = 0; oct < 8; ++oct) {
         uint32_t st = 8u;  // root = opened node 0
         int sp = 1;
@@ -262,27 +296,38 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
             st >>= 4;
             --sp;
             if (c >= 8u) {
-                uint32_t l = 0, r = 0, ax = 0;
-#pragma unroll
-                for (int k = 0; k < kWide - 1; ++k)
-                    if (uint32_t(k) == c - 8u) { l = lcode[k]; r = rcode[k]; ax = oax[k]; }
+                const uint32_t e = uint32_t(lr >> (8 * (c - 8u)));
+                const uint32_t l = e & 0xFu, r = (e >> 4) & 0xFu, ax = (axes >> (2 * (c - 8u))) & 3u;
                 const bool neg = (oct >> ax) & 1u;
-                st = (st << 8) | ((neg ? r : l) & 0xFu) | (((neg ? l : r) & 0xFu) << 4);  // near on top
+                st = (st << 8) | (neg ? r : l) | ((neg ? l : r) << 4);  // near on top
                 sp += 2;
             } else {
                 pm = (pm & ~(0xFu << (4 * p))) | (c << (4 * p));
                 ++p;
             }
         }
-        order[job.wnode * 8 + oct] = pm;
+        ow[oct] = pm;
     }
-    atomicMax(&counters[3], job.depth);
+    // child boxes re-read from the binary nodes (cached), all eight fetches issued
+    // together (after the order words, whose registers are then free); their union
+    // is the collapsed node's own box
+    float cbl[kWide][3], cbh[kWide][3];
+    const BNode root = bn[job.bnode];
+#pragma unroll
+    for (int s = 0; s < kWide; ++s) {
+        const BNode c = bn[s < nF ? fid[s] : fid[0]];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { cbl[s][a] = c.lo[a]; cbh[s][a] = c.hi[a]; }
+    }
+    encode_node(w, root.lo, root.hi, [&](int s, float *l, float *h) {
+        for (int a = 0; a < 3; ++a) l[a] = cbl[s][a], h[a] = cbh[s][a];
+    }, nF);
 }
 
 // One level of the collapse: counters[0] jobs (read on the device), their
 // children appended to `next` (counters[2]). A grid of resident waves walks the
 // jobs 64 at a time.
-__global__ __launch_bounds__(kCollapseBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void wide_collapse_kernel(
+__global__ __launch_bounds__(kCollapseBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void wide_collapse_kernel(
     const BNode *__restrict__ bn, const Job *__restrict__ jobs, Job *__restrict__ next,
     uint32_t *__restrict__ counters, WNode *__restrict__ wn, uint32_t *__restrict__ order, uint32_t cap,
     uint32_t *__restrict__ err) {
@@ -707,7 +752,7 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
     Job *jobs[2] = {reinterpret_cast<Job *>(scratch.ptr),
                     reinterpret_cast<Job *>(static_cast<char *>(scratch.ptr) + job_bytes)};
     uint32_t *counters = reinterpret_cast<uint32_t *>(static_cast<char *>(scratch.ptr) + 2 * job_bytes);
-    // counters: [0] jobs this level, [1] wide nodes used, [2] jobs next level, [3] max depth,
+    // counters: [0] jobs this level, [1] wide nodes used, [2] jobs next level, [3] unused,
     // [4] levels that had jobs
     e = hipMemsetAsync(counters, 0, 64, stream);
     if (e != hipSuccess) return e;
@@ -744,7 +789,9 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
     }
     out.collapse_levels = host[4];
     out.num_nodes = host[1];
-    out.max_depth = std::max(host[3], 1u);
+    // every job of a level has the same depth, so the depth of the wide tree is
+    // the number of levels that had jobs (a leaf root: one level, no jobs)
+    out.max_depth = std::max(host[4], 1u);
     return hipGetLastError();
 }
 

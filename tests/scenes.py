@@ -29,18 +29,23 @@ def cornell_emitters(width=32, height=32, spp=16, max_depth=4):
     return sc
 
 
-def deep_chain(levels=36, per_level=8):
+def deep_chain(levels=48, per_level=16):
     """A triangle set whose 8-wide tree is deeper than the traversal's 12-entry LDS
     stack, so that closest-hit rays along +x spill their stack to memory.
 
     Cluster k (k = 1..levels) holds `per_level` triangles with bounding-box centres
-    at x = 0.5 * 2^-k * (1 - 0.04 j), y = z = 0.5. Midpoint splits of the centroid
-    bounds on x (the only axis with extent) then peel one cluster per binary level.
+    at x = 0.5 * 2^-k * (1 - 0.03 j), y = z = 0.5: clusters do not interleave and no
+    two centres coincide, so every leaf holds one triangle (the order of the
+    triangles inside a multi-primitive leaf is left open by the reference, whose
+    treelet sort places them with atomics, and with exact t ties that order picks
+    the hit). Midpoint splits of the centroid bounds on x (the only axis with
+    extent) then peel one cluster per binary level.
     Cluster k spans 0.8 * 0.9^k in y and z, more than every deeper cluster, so the
-    greedy wide collapse (largest surface area first) opens cluster k's own nodes
-    before the deeper chain: one wide level per cluster. A ray along +x through the
-    middle enters the deeper chain first (near child) and pushes cluster k's slots
-    at every level."""
+    wide collapse (largest surface area first) fills most of a wide node with
+    cluster k's own nodes and reaches only a few clusters down the chain per wide
+    level: the wide tree is 16 levels deep. A ray along +x through the middle
+    enters the deeper chain first (near child) and pushes cluster slots at every
+    level."""
     import numpy as np
 
     tri = []
@@ -48,7 +53,7 @@ def deep_chain(levels=36, per_level=8):
         c0 = 0.5 * 2.0 ** -k
         a = 0.4 * 0.9 ** k
         for j in range(per_level):
-            c = c0 * (1.0 - 0.04 * j)
+            c = c0 * (1.0 - 0.03 * j)
             d = 0.01 * c0
             tri.append([[c - d, 0.5 - a, 0.5 - 0.3 * a], [c + d, 0.5 + a, 0.5 - a], [c, 0.5 + 0.2 * a, 0.5 + a]])
     return np.ascontiguousarray(np.array(tri, np.float32))
