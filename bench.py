@@ -100,8 +100,6 @@ def parse():
     ap.add_argument("--pt-spp", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-render measurement")
     ap.add_argument("--batch-jobs", type=int, default=64)
-    ap.add_argument("--batch-pools", type=str, default="8388608,16777216,33554432",
-                    help="shared pool sizes the batch is tuned over")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiable-render measurement")
     ap.add_argument("--grad-steps", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -224,6 +222,7 @@ def pathtracer(args, ctx, pbrtm, host):
                       for k in ("ms_trace", "ms_shadow", "ms_shade", "ms_film")},
         "rounds": last["rounds"], "rays": last["rays"], "shadow_rays": last["shadow_rays"],
         "ray_order": {"closest_sorted": bool(last["ray_order"] & 1), "shadow_sorted": bool(last["ray_order"] & 2)},
+        "grid_quarters": last["grid_quarters"],
         "mrays_per_s": round((last["rays"] + last["shadow_rays"]) / (ms * 1e3), 1),
         "pool_4m": {"value": round(W * H * spp / (ms_big * 1e3), 2), "ms_per_image": round(ms_big, 2),
                     "note": "2^22 path slots instead of the reference's 2^21: each pixel sums the same samples in another order"},
@@ -314,23 +313,27 @@ def batch(args, ctx, pbrtm):
     try:
         for h, spp, depth, _ in jobs:  # warm-up: per-scene pools and records
             h.render(spp=spp, max_depth=depth)
-        ref, one_ms = [], 0.0
+        ref, one_ms, per_job = [], 0.0, []
         t0 = time.perf_counter()
         for h, spp, depth, _ in jobs:
             rgb, _, st = h.render(spp=spp, max_depth=depth)
             ref.append((rgb, st["rounds"]))
             one_ms += st["ms_total"]
+            per_job.append((st["ms_total"], st["rays"] + st["shadow_rays"]))
         one_wall = (time.perf_counter() - t0) * 1e3
-        tune = {}
-        for pool in (int(x) for x in args.batch_pools.split(",")):
-            ctx.render_batch(jobs, pool_size=pool)  # warm-up at this pool size
-            rgbs, _, rounds, st = ctx.render_batch(jobs, pool_size=pool)
-            same = all(np.array_equal(a.view(np.uint32), r[0].view(np.uint32)) and n == r[1]
-                       for a, n, r in zip(rgbs, rounds, ref))
-            tune[pool] = (st, same)
-        best = min(tune, key=lambda k: tune[k][0]["ms_total"])
-        st = tune[best][0]
-        wall = st["ms_total"]
+        # the library's autotuner: the first calls each try one (pool size, traversal
+        # grid) candidate, later calls use the fastest; every call's films are checked
+        tune, same_all = [], True
+        for _ in range(6):
+            rgbs, _, rounds, st = ctx.render_batch(jobs, autotune=True)
+            same_all &= all(np.array_equal(a.view(np.uint32), r[0].view(np.uint32)) and n == r[1]
+                            for a, n, r in zip(rgbs, rounds, ref))
+            tune.append(st)
+            if st["tune_state"] == 2:
+                break
+        assert tune[-1]["tune_state"] == 2
+        st = tune[-1]
+        best = st["pool_size"]
     finally:
         for h in handles:
             h.free()
@@ -348,6 +351,10 @@ def batch(args, ctx, pbrtm):
     finally:
         for h in sh:
             h.free()
+    # The bound on any batching: every job's rays at the best ray rate one job alone
+    # reaches (the largest jobs fill the GPU one by one), against the one-by-one sum.
+    peak = max(r / ms for ms, r in per_job if ms > 0)
+    ideal_ms = sum(r for _, r in per_job) / peak
     return {
         "metric": "batched render throughput (64 mixed jobs)",
         "value": round(paths / (st["ms_total"] * 1e3), 2),
@@ -357,9 +364,15 @@ def batch(args, ctx, pbrtm):
         "ms_one_by_one": round(one_ms, 2),
         "wall_ms_one_by_one": round(one_wall, 2),
         "speedup_vs_one_by_one": round(one_ms / st["ms_total"], 3),
-        "films_identical": all(v[1] for v in tune.values()),
+        "speedup_bound": {"value": round(one_ms / ideal_ms, 3), "ideal_ms": round(ideal_ms, 2),
+                          "peak_mrays_per_s": round(peak / 1e3, 1),
+                          "note": "all jobs' rays at the best single-job ray rate: no batching of these jobs can "
+                                  "beat one-by-one by more than this"},
+        "films_identical": bool(same_all),
         "pool_size": best,
-        "tuning_ms": {str(k): round(v[0]["ms_total"], 2) for k, v in tune.items()},
+        "grid_quarters": st["grid_quarters"],
+        "autotune": [{"pool_size": t["pool_size"], "grid_quarters": t["grid_quarters"], "ms": round(t["ms_total"], 2),
+                      "state": "tried" if t["tune_state"] == 1 else "chosen"} for t in tune],
         "stages_ms": {"rounds": round(st["ms_rounds"], 2), "schedule": round(st["ms_schedule"], 2),
                       "film": round(st["ms_film"], 2)},
         "physical_rounds": st["rounds"], "graph_launches": st["launches"], "paths": paths,
@@ -367,11 +380,11 @@ def batch(args, ctx, pbrtm):
         "solo_jobs": st["solo_jobs"],
         "small_jobs": {"ms_one_by_one": round(small_one, 2), "ms_batch": round(small_batch, 2),
                        "speedup_vs_one_by_one": round(small_one / small_batch, 3),
-                       "workload": "64 jobs: square resolution U[64, 256], spp in {1, 4, 16, 64}, depth 1..8, "
-                                   "soup rooms of 1e4..1e5 triangles, seed 0x5A11"},
-        "config": {"workload": "64 jobs: square resolution U[64, 1024], spp in {1, 4, 16, 64}, max depth 1..8, "
-                               "soup_room scenes of 1e4..1e6 triangles (log-uniform)", "seed": 0xBA7C4,
-                   "dtype": "f32"},
+                       "workload": "64 jobs: width and height U[64, 256], spp in {1, 4, 16, 64}, depth 1..8, "
+                                   "the same scenes with 1e4..1e5 triangles, seed 0x5A11"},
+        "config": {"workload": "64 jobs: width and height each U[64, 1024], spp in {1, 4, 16, 64}, max depth "
+                               "U[1, 8]; scene = HLBVH benchmark soup of 1e4..1e6 triangles (log-uniform) under "
+                               "one emissive quad, all diffuse albedo 0.5", "seed": 0xBA7C, "dtype": "f32"},
     }
 
 

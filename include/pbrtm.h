@@ -282,7 +282,7 @@ typedef struct pbrtm_render_stats {
     double ms_total, ms_trace, ms_shadow, ms_shade, ms_film;
     uint64_t rounds, rays, shadow_rays, paths;
     uint32_t ray_order;  /* the ray-order autotuner's choice: bit 0 closest-hit rays sorted, bit 1 shadow rays */
-    uint32_t reserved;
+    uint32_t grid_quarters;  /* the occupancy autotuner's choice: traversal grid in quarters of the maximum */
     /* with PBRTM_RENDER_COUNT: [wide nodes visited, child box tests, primitive tests]
        summed over the closest-hit and the shadow traversals (else zero) */
     uint64_t closest_counts[3], shadow_counts[3];
@@ -344,6 +344,14 @@ typedef struct pbrtm_job {
 
 /* pbrtm_batch_params.flags */
 #define PBRTM_BATCH_NO_GRAPH 2u    /* enqueue each round's kernels instead of replaying a captured graph */
+/* Autotune the shared pool size (queue capacity: 2^23, 2^24, 2^25 slots) and the
+ * persistent traversal grid (4, 3, 2 quarters of the resident workgroups) from
+ * measured time, across calls: the context remembers each batch (the same
+ * scenes, spp, depths and pool sizes, in order); its first calls each try one
+ * candidate (the pool sizes at the full grid, then the best pool at smaller
+ * grids) and later calls use the fastest. pool_size is then ignored. Films do not
+ * depend on either choice. */
+#define PBRTM_BATCH_AUTOTUNE 4u
 
 typedef struct pbrtm_batch_params {
     uint32_t pool_size;           /* slots shared by the batch; 0 = 2^24 (capped at the batch's path count) */
@@ -358,7 +366,10 @@ typedef struct pbrtm_batch_stats {
     double ms_total, ms_rounds, ms_schedule, ms_film;  /* ms_rounds .. ms_film: the shared pool's */
     uint64_t rounds, rays, shadow_rays, paths;
     uint32_t pool_size, launches;
-    uint32_t solo_jobs, reserved;
+    uint32_t solo_jobs;
+    uint32_t grid_quarters;  /* the shared pool's traversal grid, in quarters of the resident maximum */
+    uint32_t tune_state;     /* PBRTM_BATCH_AUTOTUNE: 1 = this call tried a candidate, 2 = the tuned choice */
+    uint32_t reserved;
 } pbrtm_batch_stats;
 
 int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pbrtm_batch_params *params,

@@ -20,6 +20,15 @@
 
 using namespace pbrtm;
 
+// PBRTM_BATCH_AUTOTUNE state of one batch signature: the candidates tried and
+// their measured times.
+struct BatchTune {
+    uint64_t key = 0;
+    int tried = 0;
+    uint32_t pool[6] = {}, quarters[6] = {};
+    double ms[6] = {};
+};
+
 struct pbrtm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -29,6 +38,7 @@ struct pbrtm_ctx {
     DevBuf batch_jobs, batch_films;
     hipEvent_t ev[8] = {};
     std::string error;
+    std::vector<BatchTune> batch_tune;
 };
 
 struct pbrtm_bvh {
@@ -920,7 +930,7 @@ int pbrtm_render(pbrtm_ctx *ctx, pbrtm_scene *sc, const pbrtm_render_params *par
         stats->shadow_rays = rs.shadow_rays;
         stats->paths = rs.paths;
         stats->ray_order = rs.sorted;
-        stats->reserved = 0;
+        stats->grid_quarters = rs.grid_quarters;
         for (int k = 0; k < 3; ++k) {
             stats->closest_counts[k] = hcounts[k];
             stats->shadow_counts[k] = hcounts[3 + k];
@@ -1050,8 +1060,51 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
         paths += wh * jb.params.spp;
         if (wh * jb.params.spp < solo) shared_paths += wh * jb.params.spp;  // the jobs of the shared pool
     }
+    // PBRTM_BATCH_AUTOTUNE: this batch's tuning record (FNV-1a over its jobs)
+    BatchTune *tune = nullptr;
+    uint32_t quarters = 0, tune_state = 0, tune_pool = 0;
+    if (params->flags & PBRTM_BATCH_AUTOTUNE) {
+        uint64_t key = 1469598103934665603ull;
+        auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
+        mix(njobs);
+        mix(solo);
+        for (uint32_t j = 0; j < njobs; ++j) {
+            mix(reinterpret_cast<uintptr_t>(jobs[j].scene));
+            mix(jobs[j].params.spp);
+            mix(jobs[j].params.max_depth);
+            mix(jobs[j].params.pool_size);
+        }
+        for (auto &t : ctx->batch_tune)
+            if (t.key == key) tune = &t;
+        if (!tune) {
+            ctx->batch_tune.emplace_back();
+            tune = &ctx->batch_tune.back();
+            tune->key = key;
+        }
+        const uint32_t pools[3] = {8u << 20, 16u << 20, 32u << 20};
+        auto best_of = [&](int upto) {
+            int b = 0;
+            for (int k = 1; k < upto; ++k)
+                if (tune->ms[k] < tune->ms[b]) b = k;
+            return b;
+        };
+        if (tune->tried < 3) {  // pool sizes at the full grid
+            tune_pool = pools[tune->tried];
+            quarters = 4;
+            tune_state = 1;
+        } else if (tune->tried < 5) {  // the best pool at 3 and 2 quarters of the grid
+            tune_pool = tune->pool[best_of(3)];
+            quarters = tune->tried == 3 ? 3u : 2u;
+            tune_state = 1;
+        } else {
+            const int b = best_of(5);
+            tune_pool = tune->pool[b];
+            quarters = tune->quarters[b];
+            tune_state = 2;
+        }
+    }
     // the shared pool's slots: pool_size, or 2^24, capped at its jobs' path count (as render_batch does)
-    const uint32_t pool = params->pool_size ? params->pool_size : (16u << 20);
+    const uint32_t pool = tune ? tune_pool : (params->pool_size ? params->pool_size : (16u << 20));
     const uint64_t slots = std::min<uint64_t>(pool, shared_paths);
     if (shared_paths + slots >= 0xFFFFFFFFull || paths >= 0xFFFFFFFFull)
         return fail(ctx, PBRTM_ERR_UNSUPPORTED, "at most 2^32 - 1 - pool_size paths per batch");
@@ -1082,7 +1135,7 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
     CK(ctx, hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream), "memset");
     pbrtm::pt::BatchStats bs;
     CK(ctx, pbrtm::pt::render_batch(bj, pool, rpl, !(params->flags & PBRTM_BATCH_NO_GRAPH), solo, ctx->batch_pool,
-                                    ctx->batch_jobs, ctx->dev_err.as<uint32_t>(), &bs, ctx->stream),
+                                    ctx->batch_jobs, ctx->dev_err.as<uint32_t>(), &bs, ctx->stream, quarters),
        "render batch");
     uint32_t derr = 0;
     CK(ctx, hipMemcpyAsync(&derr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "download");
@@ -1099,6 +1152,12 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
     CK(ctx, hipStreamSynchronize(ctx->stream), "sync");
     if (derr) return fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr));
     for (uint32_t j = 0; j < njobs; ++j) jobs[j].rounds = bs.job_rounds[j];
+    if (tune && tune_state == 1) {
+        const int k = tune->tried++;
+        tune->pool[k] = pool;
+        tune->quarters[k] = quarters;
+        tune->ms[k] = bs.ms_total;
+    }
     if (stats) {
         stats->ms_total = bs.ms_total;
         stats->ms_rounds = bs.ms_rounds;
@@ -1111,6 +1170,8 @@ int pbrtm_render_batch(pbrtm_ctx *ctx, pbrtm_job *jobs, uint32_t njobs, const pb
         stats->pool_size = uint32_t(slots);  // 0 when every job rendered alone
         stats->launches = bs.launches;
         stats->solo_jobs = bs.solo;
+        stats->grid_quarters = quarters ? quarters : 4u;
+        stats->tune_state = tune_state;
         stats->reserved = 0;
     }
     return PBRTM_OK;

@@ -152,6 +152,7 @@ struct RenderStats {
     double ms_total = 0, ms_trace = 0, ms_shadow = 0, ms_shade = 0, ms_film = 0;
     uint64_t rounds = 0, rays = 0, shadow_rays = 0, paths = 0;
     uint32_t sorted = 0;  // the ray-order autotuner's choice: 1 closest-hit rays sorted, 2 shadow rays sorted
+    uint32_t grid_quarters = 4;  // the occupancy autotuner's choice: traversal grid in quarters of the maximum
 };
 
 // Renders the whole image; film_rgb (device, [h][w][3]) and film_aux (device,
@@ -198,9 +199,11 @@ struct BatchStats {
 // shared pool of `pool` slots; `rounds_per_launch` rounds are enqueued (as one
 // captured graph when use_graph) between checks of the device's done flag.
 // Each job's film equals its one-by-one render with pool size vpool.
+// grid_quarters: the shared pool's persistent traversal grids, in quarters of the
+// resident maximum (0 = all).
 hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32_t rounds_per_launch, bool use_graph,
                         uint64_t solo_paths, PathPool &pp, DevBuf &jobbuf, uint32_t *err, BatchStats *stats,
-                        hipStream_t stream);
+                        hipStream_t stream, uint32_t grid_quarters = 0);
 
 }  // namespace pt
 }  // namespace pbrtm

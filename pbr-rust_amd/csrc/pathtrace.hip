@@ -29,6 +29,12 @@ constexpr uint32_t kBlock = 256;
 // time per ray over each pair decides the rest of the render. The film does not
 // depend on the order.
 constexpr uint32_t kTuneFirst = 2;
+// Traversal occupancy autotuner: from round kOccFirst (after the ray order is
+// settled), six rounds give the persistent traversal grids 4, 3, 2, 4, 3, 2
+// quarters of the resident maximum; the least time per traced ray (closest-hit
+// and shadow together) over each pair decides the rest of the render.
+constexpr uint32_t kOccFirst = kTuneFirst + 5;
+constexpr uint32_t kOccQuarters[3] = {4, 3, 2};
 constexpr uint32_t kFlagSpecular = 1u << 16, kFlagAnyNonSpec = 1u << 17, kFlagNee = 1u << 18;
 constexpr uint32_t kDepthMask = 0xFFFFu;
 constexpr uint32_t kIdle = 0xFFFFFFFFu;
@@ -1291,6 +1297,9 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         bool sort_closest = true, sort_shadow = true;
         double ms[2][2] = {{0, 0}, {0, 0}};
         uint64_t rays[2][2] = {{0, 0}, {0, 0}};
+        uint32_t quarters = 4;
+        double occ_ms[3] = {0, 0, 0};
+        uint64_t occ_rays[3] = {0, 0, 0};
     } tune;
     // One round: both traces and the shading kernel, then the counters to the host.
     // Round r + 1 is enqueued before round r's counters are read, so the device
@@ -1305,6 +1314,8 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         const bool tuning = round >= kTuneFirst && round < kTuneFirst + 4;
         const bool sort_c = tuning ? ((round - kTuneFirst) % 2 == 0) : tune.sort_closest;
         const bool sort_s = tuning ? ((round - kTuneFirst) % 2 == 0) : tune.sort_shadow;
+        const bool occ_tuning = round >= kOccFirst && round < kOccFirst + 6;
+        W.grid_quarters = occ_tuning ? kOccQuarters[(round - kOccFirst) % 3] : tune.quarters;
         if (sort_c) {  // closest-hit rays in (octant, origin Morton code) order
             uint32_t *q = nullptr;
             PT_CK(sort_rays(W, reinterpret_cast<const float *>(st.rays), pool, pp.prim.as<uint32_t>(),
@@ -1382,6 +1393,19 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
                 tune.sort_shadow = faster_sorted(1);
             }
         }
+        if (r >= kOccFirst && r < kOccFirst + 6) {
+            const int k = (r - kOccFirst) % 3;
+            tune.occ_ms[k] += a + b;
+            tune.occ_rays[k] += uint64_t(host_cnt[kCntQueue]) + host_cnt[kCntSQueue];
+            if (r == kOccFirst + 5) {
+                int best = 0;
+                for (int q = 1; q < 3; ++q)
+                    if (tune.occ_rays[q] && tune.occ_rays[best] &&
+                        tune.occ_ms[q] / double(tune.occ_rays[q]) < tune.occ_ms[best] / double(tune.occ_rays[best]))
+                        best = q;
+                tune.quarters = kOccQuarters[best];
+            }
+        }
         rs.rays += host_cnt[kCntQueue];
         rs.shadow_rays += host_cnt[kCntSQueue];
         ++r;
@@ -1412,6 +1436,7 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
     rs.ms_film = tf;
     rs.rounds = P.round;
     rs.sorted = (tune.sort_closest ? 1u : 0u) | (tune.sort_shadow ? 2u : 0u);
+    rs.grid_quarters = tune.quarters;
     rs.paths = total;
     if (stats) *stats = rs;
     return hipSuccess;
@@ -1422,7 +1447,8 @@ namespace {
 
 // Every job of `jobs` in one shared pool (the batched wavefront).
 hipError_t render_shared(const std::vector<BatchJob> &jobs, uint32_t pool, uint32_t rounds_per_launch, bool use_graph,
-                         PathPool &pp, DevBuf &jobbuf, uint32_t *err, BatchStats *stats, hipStream_t stream) {
+                         PathPool &pp, DevBuf &jobbuf, uint32_t *err, BatchStats *stats, hipStream_t stream,
+                         uint32_t grid_quarters) {
     const uint32_t nj = uint32_t(jobs.size());
     if (nj == 0 || pool == 0 || rounds_per_launch == 0) return hipErrorInvalidValue;
     std::vector<JobDev> jd(nj);
@@ -1460,6 +1486,7 @@ hipError_t render_shared(const std::vector<BatchJob> &jobs, uint32_t pool, uint3
     PT_CK(reserve_ray_sort(pp.rsort, 2 * pool));  // the captured rounds allocate nothing
     WideTrace W{};
     W.spill_per_ray = spill_need;
+    W.grid_quarters = grid_quarters;
     for (uint32_t j = 0; j < nj && !W.shapes; ++j) W.shapes = jw[j].shapes;  // the analytic-primitive kernels
     if (spill_need) {
         PT_CK(pp.spill.reserve(size_t(pool) * 2 * spill_need * sizeof(uint2)));
@@ -1597,7 +1624,7 @@ hipError_t render_shared(const std::vector<BatchJob> &jobs, uint32_t pool, uint3
 
 hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32_t rounds_per_launch, bool use_graph,
                         uint64_t solo_paths, PathPool &pp, DevBuf &jobbuf, uint32_t *err, BatchStats *stats,
-                        hipStream_t stream) {
+                        hipStream_t stream, uint32_t grid_quarters) {
     const uint32_t nj = uint32_t(jobs.size());
     if (nj == 0) return hipErrorInvalidValue;
     hipEvent_t ev[2];
@@ -1636,7 +1663,7 @@ hipError_t render_batch(const std::vector<BatchJob> &jobs, uint32_t pool, uint32
     }
     if (!shared.empty()) {
         BatchStats bs;
-        e = render_shared(shared, pool, rounds_per_launch, use_graph, pp, jobbuf, err, &bs, stream);
+        e = render_shared(shared, pool, rounds_per_launch, use_graph, pp, jobbuf, err, &bs, stream, grid_quarters);
         if (e != hipSuccess) return e;
         for (size_t k = 0; k < shared.size(); ++k) out.job_rounds[shared_idx[k]] = bs.job_rounds[k];
         out.rounds += bs.rounds;

@@ -42,7 +42,14 @@ struct TriHit {
 __device__ __forceinline__ float maxc(f3 a) { return smax(smax(a.x, a.y), a.z); }
 
 // Watertight ray/triangle test of shapes/triangle.cu:217-328 (pbrt-v4).
-__device__ __forceinline__ bool tri_hit(f3 ro, f3 rd, float t_max, f3 p0, f3 p1, f3 p2, TriHit *out) {
+// robust (nullable): set when the hit is clear of every box test a BVH traversal
+// makes with this t_max: the ray is inside each edge by more than the edge
+// functions' error bound deltaE, so the exact ray meets the triangle, and the hit
+// lies more than the t error bound deltaT (and the slab distances' own rounding)
+// before t_max, so every box holding the triangle has its entry distance below
+// t_max (its far distances are widened, bounds3.h:159-180).
+__device__ __forceinline__ bool tri_hit(f3 ro, f3 rd, float t_max, f3 p0, f3 p1, f3 p2, TriHit *out,
+                                        bool *robust = nullptr) {
     if (sqlen3(cross3(sub3(p2, p0), sub3(p1, p0))) == 0.0f) return false;
     f3 a = sub3(p0, ro), b = sub3(p1, ro), c = sub3(p2, ro);
     const f3 ad = abs3(rd);
@@ -84,6 +91,10 @@ __device__ __forceinline__ bool tri_hit(f3 ro, f3 rd, float t_max, f3 p0, f3 p1,
     const float maxE = maxc(abs3(f3{e0, e1, e2}));
     const float deltaT = 3 * (gamma_n(3) * maxE * maxZt + deltaE * maxZt + deltaZ * maxE) * fabsf(invDet);
     if (t <= deltaT) return false;
+    if (robust) {
+        const float minE = fminf(fminf(fabsf(e0), fabsf(e1)), fabsf(e2));
+        *robust = minE > deltaE && (t + deltaT) * (1.0f + 8.0f * gamma_n(3)) < t_max;
+    }
     out->b0 = e0 * invDet;
     out->b1 = e1 * invDet;
     out->b2 = e2 * invDet;

@@ -55,6 +55,7 @@ struct WideTrace {
     uint2 *spill;           // per-ray stack overflow area (max_depth - kLdsStack entries per ray), may be null
     uint32_t spill_per_ray;
     float lo[3], hi[3];     // bounds of the whole tree (ray ordering)
+    uint32_t grid_quarters; // persistent traversal grid: this many quarters of the resident maximum (0 = all)
 };
 
 // counts (device, nullable): adds [wide nodes visited, child box tests, triangle tests].

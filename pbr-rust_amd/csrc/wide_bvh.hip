@@ -145,79 +145,45 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
         fwa[0] = c0.a; fwb[0] = c0.b;
         fwa[1] = c1.a; fwb[1] = c1.b;
         nF = 2;
-        while (nF < kWide) {
-            // open at once the (at most 8 - nF) interior frontier nodes of largest
-            // surface area, ties to the left: each step is one round of independent
-            // fetches (the opened nodes' children), and a full node takes about
-            // log2(8) steps instead of seven dependent ones
-            const int cap = kWide - nF;
-            bool op[kWide];
-            int nopen = 0;
+        while (nF < kWide) {  // open the frontier interior node of largest surface area
+            int best = -1;
+            float best_area = -1.0f;
 #pragma unroll
-            for (int q = 0; q < kWide; ++q) {
-                int rank = 0;
+            for (int k = 0; k < kWide; ++k)
+                if (farea[k] > best_area) { best_area = farea[k]; best = k; }
+            if (best < 0) break;
+            // the opened node's child ids are already in registers: one dependent
+            // fetch (its two children, independent of each other) per step
+            uint32_t bid = 0, ba = 0, bb = 0;
 #pragma unroll
-                for (int j = 0; j < kWide; ++j)
-                    rank += (farea[j] > farea[q] || (farea[j] == farea[q] && j < q)) ? 1 : 0;
-                op[q] = farea[q] >= 0.0f && rank < cap;
-                nopen += op[q] ? 1 : 0;
+            for (int k = 0; k < kWide; ++k) {
+                bid = (k == best) ? fid[k] : bid;
+                ba = (k == best) ? fwa[k] : ba;
+                bb = (k == best) ? fwb[k] : bb;
             }
-            if (nopen == 0) break;
-            float la[kWide], ra[kWide];
-            uint32_t lw0[kWide], lw1[kWide], rw0[kWide], rw1[kWide];
+            const uint32_t cl = ba, cr = bb & 0x3FFFFFFFu;
+            const BNode nl = bn[cl], nr = bn[cr];
 #pragma unroll
-            for (int q = 0; q < kWide; ++q) {
-                la[q] = ra[q] = -1.0f;
-                lw0[q] = lw1[q] = rw0[q] = rw1[q] = 0u;
-                if (op[q]) {
-                    const BNode nl = bn[fwa[q]], nr = bn[fwb[q] & 0x3FFFFFFFu];
-                    la[q] = bnode_leaf(nl) ? -1.0f : box_area(bnode_box(nl));
-                    ra[q] = bnode_leaf(nr) ? -1.0f : box_area(bnode_box(nr));
-                    lw0[q] = nl.a; lw1[q] = nl.b;
-                    rw0[q] = nr.a; rw1[q] = nr.b;
-                }
+            for (int k = 0; k < kWide - 1; ++k) {
+                const bool here = k == nE;
+                oid[k] = here ? bid : oid[k];
+                ol[k] = here ? cl : ol[k];
+                orr[k] = here ? cr : orr[k];
+                oax[k] = here ? (bb >> 30) : oax[k];
             }
-            uint32_t nfid[kWide], nwa[kWide], nwb[kWide];
-            float nar[kWide];
+            ++nE;
+            const float al = bnode_leaf(nl) ? -1.0f : box_area(bnode_box(nl));
+            const float ar = bnode_leaf(nr) ? -1.0f : box_area(bnode_box(nr));
 #pragma unroll
-            for (int k = 0; k < kWide; ++k) { nfid[k] = 0u; nwa[k] = nwb[k] = 0u; nar[k] = -1.0f; }
-            int pos = 0;
-#pragma unroll
-            for (int q = 0; q < kWide; ++q) {
-                if (q < nF) {
-                    const bool o = op[q];
-                    const uint32_t cl = fwa[q], cr = fwb[q] & 0x3FFFFFFFu;
-                    if (o) {  // the opened node joins the opened list
-#pragma unroll
-                        for (int k = 0; k < kWide - 1; ++k) {
-                            const bool here = k == nE;
-                            oid[k] = here ? fid[q] : oid[k];
-                            ol[k] = here ? cl : ol[k];
-                            orr[k] = here ? cr : orr[k];
-                            oax[k] = here ? (fwb[q] >> 30) : oax[k];
-                        }
-                        ++nE;
-                    }
-#pragma unroll
-                    for (int k = 0; k < kWide; ++k) {
-                        if (k == pos) {
-                            nfid[k] = o ? cl : fid[q];
-                            nar[k] = o ? la[q] : farea[q];
-                            nwa[k] = o ? lw0[q] : fwa[q];
-                            nwb[k] = o ? lw1[q] : fwb[q];
-                        } else if (o && k == pos + 1) {
-                            nfid[k] = cr;
-                            nar[k] = ra[q];
-                            nwa[k] = rw0[q];
-                            nwb[k] = rw1[q];
-                        }
-                    }
-                    pos += o ? 2 : 1;
-                }
+            for (int k = kWide - 1; k >= 0; --k) {  // shift right of best by one, then place the children
+                const uint32_t prev_id = k > 0 ? fid[k - 1] : 0u;
+                const float prev_a = k > 0 ? farea[k - 1] : -1.0f;
+                const uint32_t prev_wa = k > 0 ? fwa[k - 1] : 0u, prev_wb = k > 0 ? fwb[k - 1] : 0u;
+                if (k > best + 1) { fid[k] = prev_id; farea[k] = prev_a; fwa[k] = prev_wa; fwb[k] = prev_wb; }
+                else if (k == best + 1) { fid[k] = cr; farea[k] = ar; fwa[k] = nr.a; fwb[k] = nr.b; }
+                else if (k == best) { fid[k] = cl; farea[k] = al; fwa[k] = nl.a; fwb[k] = nl.b; }
             }
-#pragma unroll
-            for (int k = 0; k < kWide; ++k) { fid[k] = nfid[k]; farea[k] = nar[k]; fwa[k] = nwa[k]; fwb[k] = nwb[k]; }
-            nF = pos;
+            ++nF;
         }
     }
     // interior children need wide node indices: one atomic per wave
@@ -589,21 +555,24 @@ __device__ __forceinline__ bool wide_step(WideRay &R, const WideTree &T, const W
                 const float4 q0 = t4[0], q1 = t4[1], q2 = t4[2];
                 if (COUNT) ++tests;
                 TriHit h;
-                bool hit;
+                bool hit, robust = false;
                 if (SHAPES && __float_as_uint(q2.y) != 0u) {
                     pt::QHit qh;
                     hit = pt::shape_hit(T.shapes, __float_as_uint(q2.y) - 1u, R.o, R.d, R.best_t, &qh);
                     h = TriHit{qh.p.x, qh.p.y, qh.p.z, qh.t};
                 } else {
                     hit = tri_hit(R.o, R.d, R.best_t, f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y},
-                                  f3{q1.z, q1.w, q2.x}, &h);
+                                  f3{q1.z, q1.w, q2.x}, &h, &robust);
                 }
                 if (hit) {
                     // The first hit in this leaf: would the binary traversal have entered
                     // the leaf? Its exact box (the union of its primitives' boxes) against
-                    // the t at entry; the quantized slot box only contains it. A rare check
-                    // (hits are few) that keeps ties on box faces the reference's.
-                    if (R.best_t == t_leaf && !leaf_box_hit(T, first, cnt, R, t_leaf)) break;
+                    // the t at entry; the quantized slot box only contains it. A robust
+                    // triangle hit (tri_hit: clear of the edges and of t_max by its error
+                    // bounds, where t_max is the t at entry) passes every box holding the
+                    // triangle; only the others (ties on box faces, edge and grazing hits,
+                    // analytic shapes) assemble the leaf's box from its primitives.
+                    if (R.best_t == t_leaf && !robust && !leaf_box_hit(T, first, cnt, R, t_leaf)) break;
                     R.best = T.prim_order[k];
                     R.hit = h;
                     R.best_t = h.t;
@@ -796,9 +765,10 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
 }
 
 // Workgroups of a persistent traversal kernel: as many as are resident on the
-// device at once, at most one per 256 rays. Computed once per (kernel, device).
+// device at once (quarters/4 of them when quarters is 1..3), at most one per 256
+// rays. The resident count is computed once per (kernel, device).
 template <typename Kernel>
-static uint32_t persistent_grid(Kernel kernel, uint32_t n) {
+static uint32_t persistent_grid(Kernel kernel, uint32_t n, uint32_t quarters) {
     struct Entry {
         const void *kernel;
         int device;
@@ -826,6 +796,7 @@ static uint32_t persistent_grid(Kernel kernel, uint32_t n) {
                 break;
             }
     }
+    if (quarters >= 1 && quarters < 4) blocks = std::max(1u, blocks * quarters / 4);
     return std::max(1u, std::min(blocks, (n + kWideBlock - 1) / kWideBlock));
 }
 
@@ -835,7 +806,7 @@ static hipError_t trace_wide_queued_counts(const WideTrace &B, const float4 *r, 
                                            uint32_t *prim_out, float *t_out, float *bary_out, uint32_t *err,
                                            unsigned long long *counts, const uint32_t *queue,
                                            const uint32_t *queue_len, hipStream_t stream) {
-    const uint32_t grid = persistent_grid(trace_wide_kernel<false, false>, n);
+    const uint32_t grid = persistent_grid(trace_wide_kernel<false, false>, n, B.grid_quarters);
     if (B.shapes) {
         if (counts)
             trace_wide_kernel<true, true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
@@ -858,7 +829,7 @@ static hipError_t launch_multi(const WideTrace &B, const WideTrace *jobs, const 
                                uint32_t job_shift, const float4 *r, uint32_t n, int any_hit, uint32_t *prim_out,
                                float *t_out, float *bary_out, uint32_t *err, const uint32_t *queue,
                                const uint32_t *queue_len, hipStream_t stream) {
-    const uint32_t grid = persistent_grid(trace_wide_multi_kernel<false>, n);
+    const uint32_t grid = persistent_grid(trace_wide_multi_kernel<false>, n, B.grid_quarters);
     if (B.shapes)
         trace_wide_multi_kernel<true><<<grid, kWideBlock, 0, stream>>>(B, r, queue, queue_len, any_hit, prim_out,
                                                                        t_out, bary_out, err, jobs, ray_job,

@@ -38,6 +38,7 @@ EXPORTS = (
     "pbrtm_render_grad", "pbrtm_integrator_from_name", "pbrtm_write_pfm", "pbrtm_bvh_intersect_noalloc",
 )
 BATCH_NO_GRAPH = 2
+BATCH_AUTOTUNE = 4
 RENDER_COUNT = 0x100
 # pbrtm_render_params.flags: the integrator
 INTEGRATORS = {"path": 0x00, "volpath": 0x00, "megakernelpath": 0x10, "ambientocclusion": 0x20,
@@ -96,7 +97,7 @@ class RenderParams(ctypes.Structure):
 class RenderStats(ctypes.Structure):
     _fields_ = [("ms_total", c_double), ("ms_trace", c_double), ("ms_shadow", c_double), ("ms_shade", c_double),
                 ("ms_film", c_double), ("rounds", c_uint64), ("rays", c_uint64), ("shadow_rays", c_uint64),
-                ("paths", c_uint64), ("ray_order", c_uint32), ("reserved", c_uint32),
+                ("paths", c_uint64), ("ray_order", c_uint32), ("grid_quarters", c_uint32),
                 ("closest_counts", c_uint64 * 3), ("shadow_counts", c_uint64 * 3)]
 
     def as_dict(self):
@@ -127,7 +128,8 @@ class BatchParams(ctypes.Structure):
 class BatchStats(ctypes.Structure):
     _fields_ = [("ms_total", c_double), ("ms_rounds", c_double), ("ms_schedule", c_double), ("ms_film", c_double),
                 ("rounds", c_uint64), ("rays", c_uint64), ("shadow_rays", c_uint64), ("paths", c_uint64),
-                ("pool_size", c_uint32), ("launches", c_uint32), ("solo_jobs", c_uint32), ("reserved", c_uint32)]
+                ("pool_size", c_uint32), ("launches", c_uint32), ("solo_jobs", c_uint32), ("grid_quarters", c_uint32),
+                ("tune_state", c_uint32), ("reserved", c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -284,9 +286,11 @@ class Context:
     def scene(self, sc) -> "Scene":
         return Scene(self, sc)
 
-    def render_batch(self, jobs, pool_size=0, rounds_per_launch=0, graph=True, aux=False, solo_paths=0):
+    def render_batch(self, jobs, pool_size=0, rounds_per_launch=0, graph=True, aux=False, solo_paths=0,
+                     autotune=False):
         """jobs: [(Scene, spp, max_depth, pool_size)]. Returns ([film_rgb], [film_aux or None],
-        [one-by-one round count per job], stats dict)."""
+        [one-by-one round count per job], stats dict). autotune=True: the library picks the
+        shared pool size and traversal grid from measured time across calls of the same batch."""
         arr = (Job * len(jobs))()
         rgbs, auxs = [], []
         for k, (scene, spp, depth, vpool) in enumerate(jobs):
@@ -298,7 +302,8 @@ class Context:
             arr[k].params = RenderParams(spp, depth, vpool, 0)
             arr[k].film_rgb = _ptr(rgb)
             arr[k].film_aux = _ptr(ax)
-        bp = BatchParams(pool_size, rounds_per_launch, 0 if graph else BATCH_NO_GRAPH, solo_paths)
+        bp = BatchParams(pool_size, rounds_per_launch, (0 if graph else BATCH_NO_GRAPH) | (BATCH_AUTOTUNE if autotune else 0),
+                         solo_paths)
         st = BatchStats()
         self.check(self.lib.pbrtm_render_batch(self.h, arr, len(jobs), ctypes.byref(bp), ctypes.byref(st)))
         return rgbs, auxs, [arr[k].rounds for k in range(len(jobs))], st.as_dict()

@@ -37,9 +37,14 @@ flips with the handedness), and the short box instanced twice more through
 ObjectInstance transforms (rotation + translation). Each transform carries its
 inverse as composed from the directives (Translate, Rotate, Scale).
 
-batch_jobs: the mixed batch of the batched-render benchmark (per job: a square
-resolution uniform in [lo, hi], spp from {1, 4, 16, 64}, max depth 1..8 and a
-soup_room of 10^4..10^6 triangles, log-uniform), drawn from one PCG32 stream.
+batch_jobs: the mixed batch of the batched-render benchmark, drawn from one PCG32
+stream (seed 0xBA7C): per job a width and a height each uniform in [lo, hi], spp
+from {1, 4, 16, 64}, max depth uniform in 1..8 and a batch_scene of n triangles,
+n log-uniform in [10^4, 10^6].
+
+batch_scene: the HLBVH benchmark's triangle soup (centroids in the unit cube,
+log-normal edges) seen by its camera, all diffuse with albedo 0.5, lit by one
+emissive quad above the cube (facing down, rgb 1 scale 8).
 """
 import numpy as np
 
@@ -245,17 +250,34 @@ def soup_room(n_tris, width=256, height=256, seed=0x5EED, spp=4, max_depth=5):
     }
 
 
-def batch_jobs(n_jobs=64, seed=0xBA7C4, res=(64, 1024), tris=(10_000, 1_000_000)):
+def batch_scene(n_tris, width, height, seed=0x5EED, spp=4, max_depth=5):
+    from . import gen_soup
+    soup = gen_soup(n_tris, seed=seed).reshape(-1, 9)
+    light = quad((0.25, 1.5, 0.25), (0.75, 1.5, 0.25), (0.75, 1.5, 0.75), (0.25, 1.5, 0.75))
+    tri = np.concatenate([soup, np.array(light, np.float32)]).astype(np.float32)
+    tri_light = np.full(len(tri), -1, np.int32)
+    tri_light[-2:] = 0
+    return {
+        "eye": (0.5, 0.5, -1.5), "look": (0.5, 0.5, 0.5), "up": (0.0, 1.0, 0.0), "fov": 45.0,
+        "width": width, "height": height, "spp": spp, "max_depth": max_depth,
+        "tri": np.ascontiguousarray(tri), "tri_material": np.zeros(len(tri), np.uint32), "tri_light": tri_light,
+        "materials": [{"type": "diffuse", "rgb": (0.5, 0.5, 0.5)}], "lights": [{"rgb": (1.0, 1.0, 1.0), "scale": 8.0}],
+    }
+
+
+def batch_jobs(n_jobs=64, seed=0xBA7C, res=(64, 1024), tris=(10_000, 1_000_000)):
     """[(scene dict, spp, max_depth)] of the mixed batch."""
     rng = Pcg32(seed)
     out = []
+    span = res[1] - res[0]
     for k in range(n_jobs):
-        r = res[0] + min(int(rng.uniform() * (res[1] - res[0] + 1)), res[1] - res[0])
+        w = res[0] + min(int(rng.uniform() * (span + 1)), span)
+        h = res[0] + min(int(rng.uniform() * (span + 1)), span)
         spp = (1, 4, 16, 64)[min(int(rng.uniform() * 4), 3)]
         depth = 1 + min(int(rng.uniform() * 8), 7)
         lo, hi = np.log(tris[0]), np.log(tris[1])
         n = int(round(float(np.exp(lo + (hi - lo) * rng.uniform()))))
-        out.append((soup_room(n, r, r, seed=seed + 17 * k + 1, spp=spp, max_depth=depth), spp, depth))
+        out.append((batch_scene(n, w, h, seed=seed + 17 * k + 1, spp=spp, max_depth=depth), spp, depth))
     return out
 
 

@@ -41,9 +41,9 @@ def deep_chain(levels=48, per_level=16):
     the hit). Midpoint splits of the centroid bounds on x (the only axis with
     extent) then peel one cluster per binary level.
     Cluster k spans 0.8 * 0.9^k in y and z, more than every deeper cluster, so the
-    wide collapse (largest surface area first) fills most of a wide node with
-    cluster k's own nodes and reaches only a few clusters down the chain per wide
-    level: the wide tree is 16 levels deep. A ray along +x through the middle
+    greedy wide collapse (largest surface area first) fills a wide node with
+    cluster k's own nodes before it opens the deeper chain: the wide tree is 40
+    levels deep. A ray along +x through the middle
     enters the deeper chain first (near child) and pushes cluster slots at every
     level."""
     import numpy as np

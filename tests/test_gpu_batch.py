@@ -95,3 +95,27 @@ def test_batch_errors(ctx, rendered):
         ctx.render_batch([(h, 4, 4001, 0)])
     with pytest.raises(pbrtm.PbrtmError):
         ctx.render_batch([])
+
+
+def test_batch_autotune_settles_on_the_fastest(ctx, rendered):
+    """PBRTM_BATCH_AUTOTUNE: five calls each try one (pool size, traversal grid)
+    candidate (three pool sizes at the full grid, then the fastest pool at 3 and 2
+    quarters of it), the sixth and later use the fastest measured; every call's
+    films equal the one-by-one films. Another batch (a different job list) starts
+    its own tuning."""
+    jobs, handles, ref = rendered
+    spec = [(handles[id(sc)], spp, depth, vpool) for sc, spp, depth, vpool in jobs]
+    seen = []
+    for call in range(7):
+        rgbs, _, rounds, st = ctx.render_batch(spec, autotune=True)
+        log(f"call {call}: pool {st['pool_size']} quarters {st['grid_quarters']} state {st['tune_state']} "
+            f"{st['ms_total']:.2f} ms")
+        for k, ((rgb0, _, st0), rgb, r) in enumerate(zip(ref, rgbs, rounds)):
+            assert r == st0["rounds"] and np.array_equal(rgb.view(np.uint32), rgb0.view(np.uint32)), (call, k)
+        seen.append((st["pool_size"], st["grid_quarters"], st["tune_state"], st["ms_total"]))
+    assert [s[2] for s in seen] == [1, 1, 1, 1, 1, 2, 2]
+    assert [s[1] for s in seen[:3]] == [4, 4, 4] and [s[1] for s in seen[3:5]] == [3, 2]
+    fastest = min(seen[:5], key=lambda s: s[3])
+    assert seen[5][:2] == fastest[:2] and seen[6][:2] == fastest[:2]
+    _, _, _, st = ctx.render_batch(spec[:2], autotune=True)
+    assert st["tune_state"] == 1

@@ -228,6 +228,7 @@ def test_counting_render_is_the_same_render(ctx):
     assert np.array_equal(aux.view(np.uint32), caux.view(np.uint32))
     assert (st["rays"], st["shadow_rays"]) == (cst["rays"], cst["shadow_rays"])
     assert st["closest_counts"] == [0, 0, 0] and st["shadow_counts"] == [0, 0, 0]
+    assert st["grid_quarters"] in (2, 3, 4) and cst["grid_quarters"] in (2, 3, 4)
     for key, n in (("closest_counts", cst["rays"]), ("shadow_counts", cst["shadow_rays"])):
         visits, boxes, tris = cst[key]
         # eight child boxes per visit, plus the re-tests of boxes popped after a
