@@ -18,6 +18,10 @@ enum CounterSlot : int {
     kCntErr = 8,
     kCntLevelsTop = 9,
     kCntLevelsBot = 10,
+    kCntTopItems0 = 11,  // fused top level: nodes and chunk tasks of the two level parities
+    kCntTopItems1 = 12,
+    kCntTopTasks0 = 13,
+    kCntTopTasks1 = 14,
     kNumCounters = 16,
 };
 
@@ -38,6 +42,7 @@ struct BuildArgs {
 struct BuildScratch {
     DevBuf counters, hist, keys[2], vals[2], prec[2], trec[2];
     DevBuf lists, acc, splits, tasks, task_off, task_left, task_lprims, rec_ptrs;
+    DevBuf top_items, top_acc, top_tasks, top_hist, top_left, top_bkt;  // fused top level
     DevBuf scan_scratch;
     SortScratch sort;
     uint32_t *host_counters = nullptr;

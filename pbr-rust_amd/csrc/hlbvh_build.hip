@@ -634,6 +634,353 @@ __global__ __launch_bounds__(64) void children_kernel(BuildArgs A, const WorkIte
     }
 }
 
+
+// ------------------------------------------- fused top level (> 64 treelets)
+//
+// Four launches per level instead of the generic level path's eleven, and two
+// passes over the treelet records instead of four:
+//  1. top_bucket_kernel, one workgroup per 2048-treelet chunk of a node: each
+//     treelet's SAH bucket (hlbvh.cu:690-706), kept as one byte for the scatter,
+//     the chunk's treelet count per bucket, and the bucket prim counts and boxes
+//     merged into the node's accumulator.
+//  2. top_decide_kernel, one wave per node: prices the 23 splits (prefix and
+//     suffix unions over the buckets, first strict minimum: hlbvh.cu:711-741) and
+//     turns the chunks' counts into each chunk's offset among the left side.
+//  3. top_scatter_kernel: the stable partition of each chunk into the other
+//     buffer by its bucket bytes, merging the two children's centroid and full
+//     boxes into the node's child boxes on the way.
+//  4. top_children_kernel, one thread per node: the interior node, then each child
+//     becomes a bottom-level root (one treelet), a node for the wave finisher
+//     (<= 64 treelets) or a node of the next level, whose boxes come from the
+//     child boxes and whose chunk tasks it appends.
+constexpr uint32_t kTopInvalid = 0xFFFFFFFFu;
+
+struct TopItem {
+    uint32_t begin, end, node, prims;
+    uint32_t buf, first_task, ntasks, pad;
+};
+
+struct TopAcc {
+    uint32_t cb[6], fb[6];        // the node's centroid and full box (order-mapped), from its parent
+    uint32_t bcnt[kNumBuckets];   // prims per bucket
+    uint32_t bbox[kNumBuckets][6];
+    uint32_t child[2][12];        // the children's centroid and full boxes, merged by the scatter
+    uint32_t done;                // chunks finished with their buckets
+    uint32_t split;               // last bucket on the left side, kTopInvalid until priced
+    uint32_t lp, lcount, axis;    // left prims, left treelets, split axis
+    uint32_t pad[3];
+};
+
+struct TopLevel {
+    TopItem *items[2];
+    TopAcc *acc[2];
+    uint2 *tasks[2];
+    uint32_t *hist;   // per task: treelets per bucket
+    uint32_t *left;   // per task: treelets of the node's left side in earlier chunks
+    uint8_t *bkt;     // per treelet position: its bucket at the current level
+    uint32_t cap_items, cap_tasks, num;  // list capacities; treelet count
+};
+
+// A level's task and node counts, held to the list capacities (an overflow is
+// reported by the kernel that appended; the build fails, reading nothing past the lists).
+__device__ __forceinline__ uint32_t top_count(const uint32_t *counters, int slot, uint32_t cap) {
+    return min(counters[slot], cap);
+}
+
+__device__ __forceinline__ bool top_item_ok(const TopLevel &TL, const TopItem &it) {
+    return it.begin < it.end && it.end <= TL.num && it.buf < 2u;
+}
+
+__device__ __forceinline__ void top_acc_init(TopAcc &a, const uint32_t box12[12]) {
+    for (int k = 0; k < 12; ++k) (k < 6 ? a.cb[k] : a.fb[k - 6]) = box12[k];
+    for (int k = 0; k < int(kNumBuckets); ++k) {
+        a.bcnt[k] = 0u;
+        for (int j = 0; j < 6; ++j) a.bbox[k][j] = j < 3 ? 0xFFFFFFFFu : 0u;
+    }
+    for (int c = 0; c < 2; ++c)
+        for (int k = 0; k < 12; ++k) a.child[c][k] = (k % 6) < 3 ? 0xFFFFFFFFu : 0u;
+    a.done = 0u;
+    a.split = kTopInvalid;
+    a.lp = a.lcount = a.axis = 0u;
+}
+
+// The root node of the top level: its item, chunk tasks and accumulator (boxes
+// identities, merged next by top_root_box_kernel).
+__global__ void top_root_kernel(TopLevel TL, uint32_t num_treelets, uint32_t n, uint32_t *counters) {
+    if (blockIdx.x != 0) return;
+    const uint32_t ntasks = (num_treelets + kChunk - 1) / kChunk;
+    for (uint32_t c = threadIdx.x; c < ntasks; c += blockDim.x) TL.tasks[0][c] = make_uint2(0u, c);
+    if (threadIdx.x == 0) {
+        TL.items[0][0] = TopItem{0u, num_treelets, 0u, n, 0u, 0u, ntasks, 0u};
+        uint32_t id[12];
+        for (int k = 0; k < 12; ++k) id[k] = (k % 6) < 3 ? 0xFFFFFFFFu : 0u;
+        top_acc_init(TL.acc[0][0], id);
+        counters[kCntTopItems0] = 1u;
+        counters[kCntTopTasks0] = ntasks;
+        counters[kCntTopItems1] = 0u;
+        counters[kCntTopTasks1] = 0u;
+    }
+}
+
+// The root's centroid and full box over all treelet records.
+__global__ __launch_bounds__(kChunkBlock) void top_root_box_kernel(const Rec *__restrict__ R, uint32_t m,
+                                                                   TopLevel TL) {
+    uint32_t v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
+    for (uint32_t i = blockIdx.x * kChunkBlock + threadIdx.x; i < m; i += gridDim.x * kChunkBlock) {
+        const Box b = rec_box(R[i]);
+        const f3 c = box_centroid(b);
+        v[0] = min(v[0], f2ord(c.x)); v[1] = min(v[1], f2ord(c.y)); v[2] = min(v[2], f2ord(c.z));
+        v[3] = max(v[3], f2ord(c.x)); v[4] = max(v[4], f2ord(c.y)); v[5] = max(v[5], f2ord(c.z));
+        v[6] = min(v[6], f2ord(b.lo.x)); v[7] = min(v[7], f2ord(b.lo.y)); v[8] = min(v[8], f2ord(b.lo.z));
+        v[9] = max(v[9], f2ord(b.hi.x)); v[10] = max(v[10], f2ord(b.hi.y)); v[11] = max(v[11], f2ord(b.hi.z));
+    }
+    block_merge_boxes(v, TL.acc[0][0].cb);  // cb[6] and fb[6] are contiguous
+}
+
+__global__ __launch_bounds__(kChunkBlock) void top_bucket_kernel(TopLevel TL, uint32_t p, Rec *const *recs,
+                                                                 uint32_t *counters, uint32_t *err) {
+    __shared__ uint32_t s_cnt[kNumBuckets], s_pr[kNumBuckets], s_box[kNumBuckets][6];
+    const uint32_t q = 1u - p;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the next level's lists start empty
+        counters[kCntTopItems0 + q] = 0u;
+        counters[kCntTopTasks0 + q] = 0u;
+    }
+    const uint32_t t = blockIdx.x;
+    if (t >= top_count(counters, kCntTopTasks0 + p, TL.cap_tasks)) return;
+    const uint2 task = TL.tasks[p][t];
+    if (task.x >= TL.cap_items) return;
+    const TopItem it = TL.items[p][task.x];
+    if (!top_item_ok(TL, it)) return;
+    TopAcc &A = TL.acc[p][task.x];
+    const Rec *R = recs[it.buf];
+    const Box cb = acc_box(A.cb);
+    const int axis = box_max_dim(cb);
+    const float base = sel3(cb.lo, axis);
+    const float span = sel3(cb.hi, axis) - sel3(cb.lo, axis);
+    if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) {  // the node stays unpriced: no children
+        if (threadIdx.x == 0) atomicOr(err, uint32_t(kErrDegenerateCentroids));
+        return;
+    }
+    for (int k = threadIdx.x; k < int(kNumBuckets * 8); k += kChunkBlock) {
+        if (k < int(kNumBuckets)) s_cnt[k] = 0u;
+        else if (k < int(2 * kNumBuckets)) s_pr[k - kNumBuckets] = 0u;
+        else {
+            const int j = k - 2 * kNumBuckets;
+            (&s_box[0][0])[j] = (j % 6) < 3 ? 0xFFFFFFFFu : 0u;
+        }
+    }
+    __syncthreads();
+    const uint32_t s = it.begin + task.y * kChunk, e = min(it.end, s + kChunk);
+    for (uint32_t i = s + threadIdx.x; i < e; i += kChunkBlock) {
+        const Rec r = R[i];
+        const Box b = rec_box(r);
+        const uint32_t k = bucket_of(sel3(box_centroid(b), axis), base, span);
+        TL.bkt[i] = uint8_t(k);
+        atomicAdd(&s_cnt[k], 1u);
+        atomicAdd(&s_pr[k], r.b);
+        atomicMin(&s_box[k][0], f2ord(b.lo.x)); atomicMin(&s_box[k][1], f2ord(b.lo.y));
+        atomicMin(&s_box[k][2], f2ord(b.lo.z));
+        atomicMax(&s_box[k][3], f2ord(b.hi.x)); atomicMax(&s_box[k][4], f2ord(b.hi.y));
+        atomicMax(&s_box[k][5], f2ord(b.hi.z));
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < int(kNumBuckets * 7); k += kChunkBlock) {
+        if (k < int(kNumBuckets)) {
+            TL.hist[size_t(t) * kNumBuckets + k] = s_cnt[k];
+            if (s_pr[k]) atomicAdd(&A.bcnt[k], s_pr[k]);
+        } else {
+            const int j = k - kNumBuckets;
+            if (s_cnt[j / 6] == 0u) continue;
+            const uint32_t v = (&s_box[0][0])[j];
+            if ((j % 6) < 3) atomicMin(&(&A.bbox[0][0])[j], v);
+            else atomicMax(&(&A.bbox[0][0])[j], v);
+        }
+    }
+}
+
+// One wave per node: prices the 23 splits from the node's bucket counts and boxes
+// (lane k = bucket k, as decide_kernel), then turns its chunks' per-bucket counts
+// into each chunk's offset among the left-side treelets.
+__global__ __launch_bounds__(64) void top_decide_kernel(TopLevel TL, uint32_t p, const uint32_t *counters,
+                                                        uint32_t *err) {
+    const uint32_t item = blockIdx.x;
+    if (item >= top_count(counters, kCntTopItems0 + p, TL.cap_items)) return;
+    const TopItem it = TL.items[p][item];
+    if (!top_item_ok(TL, it) || uint64_t(it.first_task) + it.ntasks > TL.cap_tasks) return;
+    TopAcc &A = TL.acc[p][item];
+    const Box cb = acc_box(A.cb);
+    const int axis = box_max_dim(cb);
+    if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) return;  // reported by top_bucket_kernel; no children
+    const int lane = threadIdx.x;
+    const bool in = lane < int(kNumBuckets);
+    const uint32_t cnt = in ? A.bcnt[lane] : 0u;
+    // an untouched bucket holds the atomic identities: use the empty box
+    const Box b = cnt > 0 ? acc_box(A.bbox[lane]) : box_empty();
+    float pl[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+    float sf[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+    uint32_t pc = cnt, sc = cnt;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        const bool up = lane >= d;
+        const bool dn = lane + d < int(kNumBuckets);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const float u = __shfl_up(pl[k], d, kWave);
+            const float w = __shfl_down(sf[k], d, kWave);
+            if (up) pl[k] = k < 3 ? smin(u, pl[k]) : smax(u, pl[k]);
+            if (dn) sf[k] = k < 3 ? smin(sf[k], w) : smax(sf[k], w);
+        }
+        const uint32_t uc = __shfl_up(pc, d, kWave), wc = __shfl_down(sc, d, kWave);
+        if (up) pc += uc;
+        if (dn) sc += wc;
+    }
+    float r6[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r6[k] = __shfl_down(sf[k], 1, kWave);
+    const uint32_t rc = __shfl_down(sc, 1, kWave);
+    const float total_sa = box_area(acc_box(A.fb));
+    uint32_t key = 0xFFFFFFFFu;
+    if (lane < int(kNumBuckets) - 1) {
+        const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
+        const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
+        const float cost = 0.125 + (float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa;
+        key = f2ord(cost);
+    }
+    const uint32_t kmin = wave_min_u32(key);
+    const uint32_t split = uint32_t(__builtin_ctzll(__ballot(lane < int(kNumBuckets) - 1 && key == kmin)));
+    const uint32_t lp = __shfl(pc, int(split), kWave), rp = __shfl(rc, int(split), kWave);
+    // both end buckets hold a treelet (the least and the greatest centroid), so
+    // neither side is empty; the midpoint fallback (hlbvh.cu:758-777) is unreachable
+    if (lp == 0u || rp == 0u) {
+        if (lane == 0) atomicOr(err, uint32_t(kErrSplit));
+        return;
+    }
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < it.ntasks; c0 += kWave) {
+        const uint32_t c = c0 + lane;
+        uint32_t l = 0;
+        if (c < it.ntasks) {
+            const uint32_t *h = TL.hist + size_t(it.first_task + c) * kNumBuckets;
+            for (uint32_t k = 0; k <= split; ++k) l += h[k];
+        }
+        const uint32_t inc = wave_inclusive_add(l);
+        if (c < it.ntasks) TL.left[it.first_task + c] = carry + inc - l;
+        carry += __shfl(inc, kWave - 1, kWave);
+    }
+    if (lane == 0) {
+        A.lcount = carry;
+        A.lp = lp;
+        A.axis = uint32_t(axis);
+        A.split = split;
+    }
+}
+
+__global__ __launch_bounds__(kChunkBlock) void top_scatter_kernel(TopLevel TL, uint32_t p, Rec *const *recs,
+                                                                  const uint32_t *counters) {
+    __shared__ uint32_t s_waves[kChunkBlock / kWave];
+    const uint32_t t = blockIdx.x;
+    if (t >= top_count(counters, kCntTopTasks0 + p, TL.cap_tasks)) return;
+    const uint2 task = TL.tasks[p][t];
+    if (task.x >= TL.cap_items) return;
+    const TopItem it = TL.items[p][task.x];
+    if (!top_item_ok(TL, it)) return;
+    TopAcc &A = TL.acc[p][task.x];
+    const uint32_t split = A.split;
+    if (split == kTopInvalid) return;
+    const Rec *R = recs[it.buf];
+    Rec *W = recs[1 - it.buf];
+    const uint32_t s = it.begin + task.y * kChunk, e = min(it.end, s + kChunk);
+    const uint32_t left_total = A.lcount;
+    const uint32_t left_before = TL.left[t];
+    const uint32_t right_before = (s - it.begin) - left_before;
+    Rec r[kChunkItems];
+    uint32_t mask = 0, nleft = 0;
+    uint32_t v[2][12];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[c][k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
+#pragma unroll
+    for (int k = 0; k < kChunkItems; ++k) {
+        const uint32_t i = s + threadIdx.x * kChunkItems + k;
+        if (i < e) {
+            r[k] = R[i];
+            const bool left = TL.bkt[i] <= split;
+            if (left) { mask |= 1u << k; ++nleft; }
+            const Box b = rec_box(r[k]);
+            const f3 c = box_centroid(b);
+#pragma unroll
+            for (int sd = 0; sd < 2; ++sd) {
+                if ((sd == 0) != left) continue;
+                uint32_t *w = v[sd];
+                w[0] = min(w[0], f2ord(c.x)); w[1] = min(w[1], f2ord(c.y)); w[2] = min(w[2], f2ord(c.z));
+                w[3] = max(w[3], f2ord(c.x)); w[4] = max(w[4], f2ord(c.y)); w[5] = max(w[5], f2ord(c.z));
+                w[6] = min(w[6], f2ord(b.lo.x)); w[7] = min(w[7], f2ord(b.lo.y)); w[8] = min(w[8], f2ord(b.lo.z));
+                w[9] = max(w[9], f2ord(b.hi.x)); w[10] = max(w[10], f2ord(b.hi.y)); w[11] = max(w[11], f2ord(b.hi.z));
+            }
+        }
+    }
+    const uint32_t nvalid = (s + threadIdx.x * kChunkItems >= e)
+                                ? 0u
+                                : min(uint32_t(kChunkItems), e - (s + threadIdx.x * kChunkItems));
+    uint32_t tot;
+    const uint32_t lpre = block_exclusive_add<kChunkBlock>(nleft, s_waves, &tot);
+    const uint32_t rpre = threadIdx.x * kChunkItems - lpre;
+    uint32_t li = 0, ri = 0;
+#pragma unroll
+    for (int k = 0; k < kChunkItems; ++k) {
+        if (uint32_t(k) >= nvalid) break;
+        if (mask & (1u << k)) W[it.begin + left_before + lpre + li++] = r[k];
+        else W[it.begin + left_total + right_before + rpre + ri++] = r[k];
+    }
+    block_merge_boxes(v[0], A.child[0]);
+    __syncthreads();  // block_merge_boxes reuses its LDS staging
+    block_merge_boxes(v[1], A.child[1]);
+}
+
+__global__ __launch_bounds__(64) void top_children_kernel(BuildArgs A, TopLevel TL, uint32_t p, uint32_t *levels) {
+    const uint32_t q = 1u - p;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t *counters = A.counters;
+    const uint32_t n = top_count(counters, kCntTopItems0 + p, TL.cap_items);
+    if (i == 0 && n > 0) atomicAdd(levels, 1u);
+    if (i >= n) return;
+    const TopItem it = TL.items[p][i];
+    const TopAcc &Ac = TL.acc[p][i];
+    if (Ac.split == kTopInvalid || !top_item_ok(TL, it)) return;
+    const uint32_t size = it.end - it.begin, L = Ac.lcount, lp = Ac.lp;
+    if (L == 0 || L == size) {  // unreachable (both end buckets are occupied)
+        atomicOr(A.err, uint32_t(kErrSplit));
+        return;
+    }
+    write_interior(A.nodes, it.node, acc_box(Ac.fb), lp, Ac.axis);
+    const uint32_t nb = 1 - it.buf;
+    const WorkItem ch[2] = {{it.begin, it.begin + L, it.node + 1, lp, nb, {0, 0, 0}},
+                            {it.begin + L, it.end, it.node + 2 * lp, it.prims - lp, nb, {0, 0, 0}}};
+    for (int c = 0; c < 2; ++c) {
+        const WorkItem cw = ch[c];
+        const uint32_t cs = cw.end - cw.begin;
+        if (cs == 1) {  // a single treelet: the root of a bottom-level build over its primitives
+            emit_bottom_root(A, A.trec[nb][cw.begin], cw.node);
+        } else if (cs <= kSmallNode) {
+            push_item(A.small_top, counters + kCntSmallTop, A.cap_small_top, cw, A.err);
+        } else {
+            const uint32_t slot = atomicAdd(counters + kCntTopItems0 + q, 1u);
+            const uint32_t nt = (cs + kChunk - 1) / kChunk;
+            const uint32_t first = atomicAdd(counters + kCntTopTasks0 + q, nt);
+            if (slot >= TL.cap_items || uint64_t(first) + nt > TL.cap_tasks) {
+                atomicOr(A.err, uint32_t(kErrCapacity));
+                continue;
+            }
+            TL.items[q][slot] = TopItem{cw.begin, cw.end, cw.node, cw.prims, nb, first, nt, 0u};
+            top_acc_init(TL.acc[q][slot], Ac.child[c]);
+            for (uint32_t k = 0; k < nt; ++k) TL.tasks[q][first + k] = make_uint2(slot, k);
+        }
+    }
+}
+
 // --------------------------------------------- wave-per-node subtree finisher
 
 constexpr int kFinBlock = 256;
@@ -1294,6 +1641,60 @@ static hipError_t run_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, 
     return hipSuccess;
 }
 
+
+// The fused top level: three launches per level, levels enqueued kLevelsPerSync
+// at a time between reads of the node count (grids cover upper bounds; the
+// kernels read the level's counts on the device).
+static hipError_t run_top_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, Rec *const *recs,
+                                 uint32_t num_treelets, uint32_t n, uint32_t *levels) {
+    uint32_t *const counters = S.counters.as<uint32_t>();
+    const uint32_t cap_items = num_treelets / (kSmallNode + 1) + 2;
+    const uint32_t cap_tasks = num_treelets / kChunk + cap_items + 1;
+    PBRTM_TRY(S.top_items.reserve(2 * size_t(cap_items) * sizeof(TopItem)));
+    PBRTM_TRY(S.top_acc.reserve(2 * size_t(cap_items) * sizeof(TopAcc)));
+    PBRTM_TRY(S.top_tasks.reserve(2 * size_t(cap_tasks) * sizeof(uint2)));
+    PBRTM_TRY(S.top_hist.reserve(size_t(cap_tasks) * kNumBuckets * sizeof(uint32_t)));
+    PBRTM_TRY(S.top_left.reserve(size_t(cap_tasks) * sizeof(uint32_t)));
+    PBRTM_TRY(S.top_bkt.reserve(size_t(num_treelets)));
+    TopLevel TL{};
+    TL.items[0] = S.top_items.as<TopItem>();
+    TL.items[1] = TL.items[0] + cap_items;
+    TL.acc[0] = S.top_acc.as<TopAcc>();
+    TL.acc[1] = TL.acc[0] + cap_items;
+    TL.tasks[0] = S.top_tasks.as<uint2>();
+    TL.tasks[1] = TL.tasks[0] + cap_tasks;
+    TL.hist = S.top_hist.as<uint32_t>();
+    TL.left = S.top_left.as<uint32_t>();
+    TL.bkt = S.top_bkt.as<uint8_t>();
+    TL.cap_items = cap_items;
+    TL.cap_tasks = cap_tasks;
+    TL.num = num_treelets;
+    top_root_kernel<<<1, 256, 0, stream>>>(TL, num_treelets, n, counters);
+    const uint32_t rgrid = std::min<uint32_t>((num_treelets + kChunkBlock - 1) / kChunkBlock, 1024u);
+    top_root_box_kernel<<<rgrid, kChunkBlock, 0, stream>>>(recs[0], num_treelets, TL);
+    PBRTM_TRY(hipGetLastError());
+    uint32_t known = 1, p = 0, enqueued = 0;
+    const uint32_t item_cap = cap_items;
+    while (known > 0) {
+        uint32_t nb = known;
+        for (int k = 0; k < kLevelsPerSync; ++k) {
+            if (++enqueued > 100000) return hipErrorUnknown;  // every level strictly shrinks its nodes
+            const uint32_t tb = num_treelets / kChunk + nb + 1;
+            top_bucket_kernel<<<tb, kChunkBlock, 0, stream>>>(TL, p, recs, counters, A.err);
+            top_decide_kernel<<<nb, 64, 0, stream>>>(TL, p, counters, A.err);
+            top_scatter_kernel<<<tb, kChunkBlock, 0, stream>>>(TL, p, recs, counters);
+            top_children_kernel<<<(nb + 63) / 64, 64, 0, stream>>>(A, TL, p, counters + kCntLevelsTop);
+            PBRTM_TRY(hipGetLastError());
+            p = 1 - p;
+            nb = std::min(2 * nb, item_cap);
+        }
+        PBRTM_TRY(read_counters(S, stream));
+        known = S.host_counters[kCntTopItems0 + p];
+    }
+    *levels = S.host_counters[kCntLevelsTop];
+    return hipSuccess;
+}
+
 hipError_t morton_sort_device(const float *d_tri, uint32_t n, BuildScratch &S, uint32_t **keys_out,
                               uint32_t **vals_out, hipStream_t stream) {
     PBRTM_TRY(S.counters.reserve(kNumCounters * sizeof(uint32_t)));
@@ -1389,9 +1790,11 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     Rec *const *trec_ptrs = S.rec_ptrs.as<Rec *>();
     Rec *const *prec_ptrs = S.rec_ptrs.as<Rec *>() + 2;
 
-    root_kernel<<<1, 64, 0, stream>>>(A, D, n);
-    PBRTM_TRY(run_levels<true>(A, S, stream, trec_ptrs, kCntLargeTop0, kCntLargeTop1, A.large_top[0],
-                               A.large_top[1], D, A.cap_large_top, &out.top_levels));
+    if (D > kSmallNode) {  // the fused top level down to nodes of <= 64 treelets
+        PBRTM_TRY(run_top_levels(A, S, stream, trec_ptrs, D, n, &out.top_levels));
+    } else {
+        root_kernel<<<1, 64, 0, stream>>>(A, D, n);
+    }
     PBRTM_TRY(read_counters(S, stream));
     out.small_top = S.host_counters[kCntSmallTop];
     if (out.small_top > 0)
@@ -1416,7 +1819,8 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
 void BuildScratch::release() {
     DevBuf *all[] = {&counters, &hist, &keys[0], &keys[1], &vals[0], &vals[1], &prec[0], &prec[1],
                      &trec[0], &trec[1], &lists, &acc, &splits, &tasks, &task_off, &task_left,
-                     &task_lprims, &rec_ptrs, &scan_scratch, &sort.status, &sort.offsets, &sort.counters};
+                     &task_lprims, &rec_ptrs, &scan_scratch, &sort.status, &sort.offsets, &sort.counters,
+                     &top_items, &top_acc, &top_tasks, &top_hist, &top_left, &top_bkt};
     for (DevBuf *b : all) b->release();
     if (host_counters) hipHostFree(host_counters);
     host_counters = nullptr;
