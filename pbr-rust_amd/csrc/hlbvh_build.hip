@@ -459,7 +459,7 @@ __global__ __launch_bounds__(64) void decide_kernel(const WorkItem *items, const
     if (lane < int(kNumBuckets) - 1) {
         const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
         const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
-        s_cost[lane] = 0.125 + (float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa;
+        s_cost[lane] = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
     }
     __syncthreads();
     uint32_t split = 0;
@@ -671,6 +671,12 @@ struct TopAcc {
     uint32_t pad[3];
 };
 
+static_assert(offsetof(TopAcc, bcnt) == 12 * 4 && offsetof(TopAcc, bbox) == (12 + kNumBuckets) * 4 &&
+                  offsetof(TopAcc, child) == (12 + 7 * kNumBuckets) * 4 &&
+                  offsetof(TopAcc, done) == (12 + 7 * kNumBuckets + 24) * 4 &&
+                  offsetof(TopAcc, split) == (12 + 7 * kNumBuckets + 25) * 4,
+              "TopAcc word layout");
+
 struct TopLevel {
     TopItem *items[2];
     TopAcc *acc[2];
@@ -845,7 +851,7 @@ __global__ __launch_bounds__(64) void top_decide_kernel(TopLevel TL, uint32_t p,
     if (lane < int(kNumBuckets) - 1) {
         const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
         const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
-        const float cost = 0.125 + (float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa;
+        const float cost = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
         key = f2ord(cost);
     }
     const uint32_t kmin = wave_min_u32(key);
@@ -940,44 +946,83 @@ __global__ __launch_bounds__(kChunkBlock) void top_scatter_kernel(TopLevel TL, u
     block_merge_boxes(v[1], A.child[1]);
 }
 
+// One thread per node: the interior node, then its children. Next-level nodes are
+// appended with one atomic per wave (for their slots and for their chunk tasks);
+// each gets its boxes, no split yet and its tasks here, its bucket and child
+// accumulators from top_acc_reset_kernel.
 __global__ __launch_bounds__(64) void top_children_kernel(BuildArgs A, TopLevel TL, uint32_t p, uint32_t *levels) {
     const uint32_t q = 1u - p;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t *counters = A.counters;
     const uint32_t n = top_count(counters, kCntTopItems0 + p, TL.cap_items);
     if (i == 0 && n > 0) atomicAdd(levels, 1u);
-    if (i >= n) return;
-    const TopItem it = TL.items[p][i];
-    const TopAcc &Ac = TL.acc[p][i];
-    if (Ac.split == kTopInvalid || !top_item_ok(TL, it)) return;
-    const uint32_t size = it.end - it.begin, L = Ac.lcount, lp = Ac.lp;
-    if (L == 0 || L == size) {  // unreachable (both end buckets are occupied)
-        atomicOr(A.err, uint32_t(kErrSplit));
-        return;
-    }
-    write_interior(A.nodes, it.node, acc_box(Ac.fb), lp, Ac.axis);
-    const uint32_t nb = 1 - it.buf;
-    const WorkItem ch[2] = {{it.begin, it.begin + L, it.node + 1, lp, nb, {0, 0, 0}},
-                            {it.begin + L, it.end, it.node + 2 * lp, it.prims - lp, nb, {0, 0, 0}}};
-    for (int c = 0; c < 2; ++c) {
-        const WorkItem cw = ch[c];
-        const uint32_t cs = cw.end - cw.begin;
-        if (cs == 1) {  // a single treelet: the root of a bottom-level build over its primitives
-            emit_bottom_root(A, A.trec[nb][cw.begin], cw.node);
-        } else if (cs <= kSmallNode) {
-            push_item(A.small_top, counters + kCntSmallTop, A.cap_small_top, cw, A.err);
-        } else {
-            const uint32_t slot = atomicAdd(counters + kCntTopItems0 + q, 1u);
-            const uint32_t nt = (cs + kChunk - 1) / kChunk;
-            const uint32_t first = atomicAdd(counters + kCntTopTasks0 + q, nt);
-            if (slot >= TL.cap_items || uint64_t(first) + nt > TL.cap_tasks) {
-                atomicOr(A.err, uint32_t(kErrCapacity));
-                continue;
+    bool ok = i < n;
+    TopItem it{};
+    uint32_t L = 0, lp = 0;
+    if (ok) {
+        it = TL.items[p][i];
+        const TopAcc &Ac = TL.acc[p][i];
+        ok = Ac.split != kTopInvalid && top_item_ok(TL, it);
+        if (ok) {
+            L = Ac.lcount;
+            lp = Ac.lp;
+            if (L == 0 || L == it.end - it.begin) {  // unreachable (both end buckets are occupied)
+                atomicOr(A.err, uint32_t(kErrSplit));
+                ok = false;
+            } else {
+                write_interior(A.nodes, it.node, acc_box(Ac.fb), lp, Ac.axis);
             }
-            TL.items[q][slot] = TopItem{cw.begin, cw.end, cw.node, cw.prims, nb, first, nt, 0u};
-            top_acc_init(TL.acc[q][slot], Ac.child[c]);
-            for (uint32_t k = 0; k < nt; ++k) TL.tasks[q][first + k] = make_uint2(slot, k);
         }
+    }
+    const uint32_t nb = 1 - it.buf;
+    for (int c = 0; c < 2; ++c) {  // every lane of the wave takes part (the appends ballot)
+        WorkItem cw{};
+        if (ok) cw = c == 0 ? WorkItem{it.begin, it.begin + L, it.node + 1, lp, nb, {0, 0, 0}}
+                            : WorkItem{it.begin + L, it.end, it.node + 2 * lp, it.prims - lp, nb, {0, 0, 0}};
+        const uint32_t cs = cw.end - cw.begin;
+        const bool large = ok && cs > kSmallNode;
+        if (ok && cs == 1) emit_bottom_root(A, A.trec[nb][cw.begin], cw.node);
+        wave_push(ok && cs > 1 && cs <= kSmallNode, cw, A.small_top, counters + kCntSmallTop, A.cap_small_top, A.err);
+        const uint32_t nt = large ? (cs + kChunk - 1) / kChunk : 0u;
+        const uint32_t inc_i = wave_inclusive_add(large ? 1u : 0u), inc_t = wave_inclusive_add(nt);
+        uint32_t base_i = 0, base_t = 0;
+        if (__lane_id() == kWave - 1 && inc_i > 0) {
+            base_i = atomicAdd(counters + kCntTopItems0 + q, inc_i);
+            base_t = atomicAdd(counters + kCntTopTasks0 + q, inc_t);
+        }
+        base_i = __shfl(base_i, kWave - 1, kWave);
+        base_t = __shfl(base_t, kWave - 1, kWave);
+        if (!large) continue;
+        const uint32_t slot = base_i + inc_i - 1, first = base_t + inc_t - nt;
+        if (slot >= TL.cap_items || uint64_t(first) + nt > TL.cap_tasks) {
+            atomicOr(A.err, uint32_t(kErrCapacity));
+            continue;
+        }
+        TL.items[q][slot] = TopItem{cw.begin, cw.end, cw.node, cw.prims, nb, first, nt, 0u};
+        TopAcc &Nx = TL.acc[q][slot];
+        const TopAcc &Ac = TL.acc[p][i];
+        for (int k = 0; k < 6; ++k) {
+            Nx.cb[k] = Ac.child[c][k];
+            Nx.fb[k] = Ac.child[c][6 + k];
+        }
+        Nx.split = kTopInvalid;
+        for (uint32_t k = 0; k < nt; ++k) TL.tasks[q][first + k] = make_uint2(slot, k);
+    }
+}
+
+// The bucket and child accumulators of the next level's nodes: identities.
+__global__ __launch_bounds__(256) void top_acc_reset_kernel(TopLevel TL, uint32_t q, const uint32_t *counters) {
+    constexpr uint32_t kFirst = 12, kLast = 12 + 7 * kNumBuckets + 24;  // bcnt .. child
+    constexpr uint32_t kSpan = kLast - kFirst;
+    const uint32_t n = top_count(counters, kCntTopItems0 + q, TL.cap_items);
+    const uint64_t total = uint64_t(n) * kSpan;
+    for (uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+         g += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t item = uint32_t(g / kSpan), k = kFirst + uint32_t(g % kSpan);
+        uint32_t v = 0u;
+        if (k >= 12 + kNumBuckets && k < 12 + 7 * kNumBuckets) v = ((k - 12 - kNumBuckets) % 6) < 3 ? 0xFFFFFFFFu : 0u;
+        else if (k >= 12 + 7 * kNumBuckets) v = ((k - 12 - 7 * kNumBuckets) % 6) < 3 ? 0xFFFFFFFFu : 0u;
+        reinterpret_cast<uint32_t *>(&TL.acc[q][item])[k] = v;
     }
 }
 
@@ -1081,7 +1126,7 @@ __device__ uint32_t wave_build(const BuildArgs &A, const Rec *src, uint32_t begi
                 cr += ocr;
             }
             if (lane < kSplits)
-                L.cost[lane] = 0.125 + (float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa;
+                L.cost[lane] = sah_cost((float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa);
             wave_lds_sync();
             uint32_t split = 0;
             float best = L.cost[0];
@@ -1289,7 +1334,7 @@ __device__ __forceinline__ void lane_sah_small(const BuildArgs &A, WaveLds &L, u
                 if (uint32_t((bpack >> (5 * (j - lo))) & 31u) <= sb) { bl = box_union(bl, rec_box(q)); cl += q.b; }
                 else { br = box_union(br, rec_box(q)); cr += q.b; }
             }
-            const float cost = 0.125 + (float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa;
+            const float cost = sah_cost((float(cl) * box_area(bl) + float(cr) * box_area(br)) / total_sa);
             if (first || cost < best) {
                 best = cost;
                 split = sb;
@@ -1438,7 +1483,7 @@ __device__ __forceinline__ void wave_build_top(const BuildArgs &A, const Rec *sr
         if (lane < kNumBuckets - 1) {
             const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
             const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
-            const float cost = 0.125 + (float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa;
+            const float cost = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
             key = f2ord(cost);
         }
         const uint32_t kmin = wave_min_u32_all(key);
@@ -1684,6 +1729,7 @@ static hipError_t run_top_levels(BuildArgs &A, BuildScratch &S, hipStream_t stre
             top_decide_kernel<<<nb, 64, 0, stream>>>(TL, p, counters, A.err);
             top_scatter_kernel<<<tb, kChunkBlock, 0, stream>>>(TL, p, recs, counters);
             top_children_kernel<<<(nb + 63) / 64, 64, 0, stream>>>(A, TL, p, counters + kCntLevelsTop);
+            top_acc_reset_kernel<<<1024, 256, 0, stream>>>(TL, 1 - p, counters);
             PBRTM_TRY(hipGetLastError());
             p = 1 - p;
             nb = std::min(2 * nb, item_cap);

@@ -75,10 +75,18 @@ PBRTM_HD int box_max_dim(Box b) {
     if (d.y > d.z) return 1;
     return 2;
 }
+// Bounds3::surface_area computes 2.0 * (FP32 sum) in double and rounds back; doubling
+// is exact in both formats (an overflow gives +inf in both), so the FP32 product is
+// the same value.
 PBRTM_HD float box_area(Box b) {
     f3 d = sub3(b.hi, b.lo);
-    return float(2.0 * double(d.x * d.y + d.x * d.z + d.y * d.z));
+    return 2.0f * (d.x * d.y + d.x * d.z + d.y * d.z);
 }
+
+// The SAH cost 0.125 + x (hlbvh.cu:711-731 adds the double 0.125 to an FP32 x and
+// rounds to FP32): below 2^50 the double sum is exact, so its one rounding is the
+// FP32 sum's; above, 0.125 is lost in both.
+PBRTM_HD float sah_cost(float x) { return 0.125f + x; }
 PBRTM_HD Box tri_box(f3 p0, f3 p1, f3 p2) {
     Box b{p0, p0};
     b.lo = min3(b.lo, p1); b.hi = max3(b.hi, p1);
