@@ -41,7 +41,7 @@ struct BuildArgs {
 // Scratch reused across builds on one context (grown, never shrunk).
 struct BuildScratch {
     DevBuf counters, hist, keys[2], vals[2], prec[2], trec[2];
-    DevBuf lists, acc, splits, tasks, task_off, task_left, task_lprims, rec_ptrs;
+    DevBuf lists, acc, splits, tasks, task_off, task_left, task_lprims;
     DevBuf top_items, top_acc, top_tasks, top_hist, top_left, top_bkt;  // fused top level
     DevBuf scan_scratch;
     SortScratch sort;

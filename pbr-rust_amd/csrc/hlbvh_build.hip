@@ -57,6 +57,12 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 __device__ __forceinline__ Rec *pick(Rec *const (&pair)[2], uint32_t b) { return b ? pair[1] : pair[0]; }
 
+// The two ping-pong record buffers of one level kind, passed by value as a kernel
+// argument (the host never reads device memory to find them).
+struct RecPair {
+    Rec *buf[2];
+};
+
 __device__ __forceinline__ Box acc_box(const uint32_t *w) {
     return {{ord2f(w[0]), ord2f(w[1]), ord2f(w[2])}, {ord2f(w[3]), ord2f(w[4]), ord2f(w[5])}};
 }
@@ -331,11 +337,11 @@ __device__ __forceinline__ void block_merge_boxes(uint32_t v[12], uint32_t *dst)
 }
 
 __global__ __launch_bounds__(kChunkBlock) void chunk_reduce_kernel(const uint2 *tasks, const uint32_t *ntasks,
-                                                                   const WorkItem *items, Rec *const *recs, Acc *acc) {
+                                                                   const WorkItem *items, RecPair recs, Acc *acc) {
     if (blockIdx.x >= *ntasks) return;
     const uint2 task = tasks[blockIdx.x];
     const WorkItem it = items[task.x];
-    const Rec *R = recs[it.buf];
+    const Rec *R = pick(recs.buf, it.buf);
     const uint32_t s = it.begin + task.y * kChunk, e = min(it.end, s + kChunk);
     uint32_t v[12];
 #pragma unroll
@@ -353,14 +359,14 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_reduce_kernel(const uint2 *
 
 // SAH bucket statistics per chunk (hlbvh.cu:690-706).
 __global__ __launch_bounds__(kChunkBlock) void chunk_buckets_kernel(const uint2 *tasks, const uint32_t *ntasks,
-                                                                    const WorkItem *items, Rec *const *recs, Acc *acc,
+                                                                    const WorkItem *items, RecPair recs, Acc *acc,
                                                                     uint32_t *err) {
     __shared__ uint32_t s_cnt[kNumBuckets];
     __shared__ uint32_t s_box[kNumBuckets][6];
     if (blockIdx.x >= *ntasks) return;
     const uint2 task = tasks[blockIdx.x];
     const WorkItem it = items[task.x];
-    const Rec *R = recs[it.buf];
+    const Rec *R = pick(recs.buf, it.buf);
     for (int k = threadIdx.x; k < int(kNumBuckets * 7); k += kChunkBlock) {
         if (k < int(kNumBuckets)) s_cnt[k] = 0u;
         else {
@@ -490,7 +496,7 @@ __device__ __forceinline__ bool goes_left(const Split &sp, const Box &b) {
 
 template <bool TOP>
 __global__ __launch_bounds__(kChunkBlock) void chunk_count_kernel(const uint2 *tasks, const uint32_t *ntasks,
-                                                                  const WorkItem *items, Rec *const *recs,
+                                                                  const WorkItem *items, RecPair recs,
                                                                   const Split *splits, uint32_t *task_left,
                                                                   uint32_t *task_lprims) {
     __shared__ uint32_t s_w[kChunkBlock / kWave];
@@ -505,7 +511,7 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_count_kernel(const uint2 *t
     const uint2 task = tasks[blockIdx.x];
     const WorkItem it = items[task.x];
     const Split sp = splits[task.x];
-    const Rec *R = recs[it.buf];
+    const Rec *R = pick(recs.buf, it.buf);
     const uint32_t s = it.begin + task.y * kChunk, e = min(it.end, s + kChunk);
     uint32_t cnt = 0, prims = 0;
     for (uint32_t i = s + threadIdx.x; i < e; i += kChunkBlock) {
@@ -527,7 +533,7 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_count_kernel(const uint2 *t
 // Stable partition of one chunk into the other buffer.
 template <bool TOP>
 __global__ __launch_bounds__(kChunkBlock) void chunk_scatter_kernel(const uint2 *tasks, const uint32_t *ntasks,
-                                                                    const WorkItem *items, Rec *const *recs,
+                                                                    const WorkItem *items, RecPair recs,
                                                                     const Split *splits, const uint32_t *task_off,
                                                                     const uint32_t *left_off) {
     __shared__ uint32_t s_waves[kChunkBlock / kWave];
@@ -536,8 +542,8 @@ __global__ __launch_bounds__(kChunkBlock) void chunk_scatter_kernel(const uint2 
     const uint2 task = tasks[t];
     const WorkItem it = items[task.x];
     const Split sp = splits[task.x];
-    const Rec *R = recs[it.buf];
-    Rec *W = recs[1 - it.buf];
+    const Rec *R = pick(recs.buf, it.buf);
+    Rec *W = pick(recs.buf, 1 - it.buf);
     const uint32_t first_task = task_off[task.x];
     const uint32_t nchunks = (it.end - it.begin + kChunk - 1) / kChunk;
     const uint32_t left_total = left_off[first_task + nchunks] - left_off[first_task];
@@ -745,7 +751,7 @@ __global__ __launch_bounds__(kChunkBlock) void top_root_box_kernel(const Rec *__
     block_merge_boxes(v, TL.acc[0][0].cb);  // cb[6] and fb[6] are contiguous
 }
 
-__global__ __launch_bounds__(kChunkBlock) void top_bucket_kernel(TopLevel TL, uint32_t p, Rec *const *recs,
+__global__ __launch_bounds__(kChunkBlock) void top_bucket_kernel(TopLevel TL, uint32_t p, RecPair recs,
                                                                  uint32_t *counters, uint32_t *err) {
     __shared__ uint32_t s_cnt[kNumBuckets], s_pr[kNumBuckets], s_box[kNumBuckets][6];
     const uint32_t q = 1u - p;
@@ -760,7 +766,7 @@ __global__ __launch_bounds__(kChunkBlock) void top_bucket_kernel(TopLevel TL, ui
     const TopItem it = TL.items[p][task.x];
     if (!top_item_ok(TL, it)) return;
     TopAcc &A = TL.acc[p][task.x];
-    const Rec *R = recs[it.buf];
+    const Rec *R = pick(recs.buf, it.buf);
     const Box cb = acc_box(A.cb);
     const int axis = box_max_dim(cb);
     const float base = sel3(cb.lo, axis);
@@ -883,7 +889,7 @@ __global__ __launch_bounds__(64) void top_decide_kernel(TopLevel TL, uint32_t p,
     }
 }
 
-__global__ __launch_bounds__(kChunkBlock) void top_scatter_kernel(TopLevel TL, uint32_t p, Rec *const *recs,
+__global__ __launch_bounds__(kChunkBlock) void top_scatter_kernel(TopLevel TL, uint32_t p, RecPair recs,
                                                                   const uint32_t *counters) {
     __shared__ uint32_t s_waves[kChunkBlock / kWave];
     const uint32_t t = blockIdx.x;
@@ -895,8 +901,8 @@ __global__ __launch_bounds__(kChunkBlock) void top_scatter_kernel(TopLevel TL, u
     TopAcc &A = TL.acc[p][task.x];
     const uint32_t split = A.split;
     if (split == kTopInvalid) return;
-    const Rec *R = recs[it.buf];
-    Rec *W = recs[1 - it.buf];
+    const Rec *R = pick(recs.buf, it.buf);
+    Rec *W = pick(recs.buf, 1 - it.buf);
     const uint32_t s = it.begin + task.y * kChunk, e = min(it.end, s + kChunk);
     const uint32_t left_total = A.lcount;
     const uint32_t left_before = TL.left[t];
@@ -1626,7 +1632,7 @@ static hipError_t read_counters(BuildScratch &S, hipStream_t stream) {
 constexpr int kLevelsPerSync = 4;
 
 template <bool TOP>
-static hipError_t run_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, Rec *const *recs, int cnt0,
+static hipError_t run_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, RecPair recs, int cnt0,
                              int cnt1, WorkItem *list0, WorkItem *list1, uint32_t elements, uint32_t list_cap,
                              uint32_t *levels) {
     WorkItem *cur = list0, *nxt = list1;
@@ -1690,7 +1696,7 @@ static hipError_t run_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, 
 // The fused top level: three launches per level, levels enqueued kLevelsPerSync
 // at a time between reads of the node count (grids cover upper bounds; the
 // kernels read the level's counts on the device).
-static hipError_t run_top_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, Rec *const *recs,
+static hipError_t run_top_levels(BuildArgs &A, BuildScratch &S, hipStream_t stream, RecPair recs,
                                  uint32_t num_treelets, uint32_t n, uint32_t *levels) {
     uint32_t *const counters = S.counters.as<uint32_t>();
     const uint32_t cap_items = num_treelets / (kSmallNode + 1) + 2;
@@ -1716,7 +1722,7 @@ static hipError_t run_top_levels(BuildArgs &A, BuildScratch &S, hipStream_t stre
     TL.num = num_treelets;
     top_root_kernel<<<1, 256, 0, stream>>>(TL, num_treelets, n, counters);
     const uint32_t rgrid = std::min<uint32_t>((num_treelets + kChunkBlock - 1) / kChunkBlock, 1024u);
-    top_root_box_kernel<<<rgrid, kChunkBlock, 0, stream>>>(recs[0], num_treelets, TL);
+    top_root_box_kernel<<<rgrid, kChunkBlock, 0, stream>>>(A.trec[0], num_treelets, TL);
     PBRTM_TRY(hipGetLastError());
     uint32_t known = 1, p = 0, enqueued = 0;
     const uint32_t item_cap = cap_items;
@@ -1830,14 +1836,10 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     A.large_bot[1] = lp;
     A.counters = counters;
     A.err = counters + kCntErr;
-    PBRTM_TRY(S.rec_ptrs.reserve(4 * sizeof(Rec *)));
-    Rec *ptrs[4] = {A.trec[0], A.trec[1], A.prec[0], A.prec[1]};
-    PBRTM_TRY(hipMemcpyAsync(S.rec_ptrs.ptr, ptrs, sizeof(ptrs), hipMemcpyHostToDevice, stream));
-    Rec *const *trec_ptrs = S.rec_ptrs.as<Rec *>();
-    Rec *const *prec_ptrs = S.rec_ptrs.as<Rec *>() + 2;
+    const RecPair trecs{{A.trec[0], A.trec[1]}}, precs{{A.prec[0], A.prec[1]}};
 
     if (D > kSmallNode) {  // the fused top level down to nodes of <= 64 treelets
-        PBRTM_TRY(run_top_levels(A, S, stream, trec_ptrs, D, n, &out.top_levels));
+        PBRTM_TRY(run_top_levels(A, S, stream, trecs, D, n, &out.top_levels));
     } else {
         root_kernel<<<1, 64, 0, stream>>>(A, D, n);
     }
@@ -1847,7 +1849,7 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
         finish_top_kernel<<<(out.small_top + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
             A, A.small_top, out.small_top);
     if (times) hipEventRecord(times->ev[3], stream);
-    PBRTM_TRY(run_levels<false>(A, S, stream, prec_ptrs, kCntLargeBot0, kCntLargeBot1, A.large_bot[0],
+    PBRTM_TRY(run_levels<false>(A, S, stream, precs, kCntLargeBot0, kCntLargeBot1, A.large_bot[0],
                                 A.large_bot[1], n, A.cap_large_bot, &out.bottom_levels));
     PBRTM_TRY(read_counters(S, stream));
     out.small_bottom = S.host_counters[kCntSmallBot];
@@ -1865,7 +1867,7 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
 void BuildScratch::release() {
     DevBuf *all[] = {&counters, &hist, &keys[0], &keys[1], &vals[0], &vals[1], &prec[0], &prec[1],
                      &trec[0], &trec[1], &lists, &acc, &splits, &tasks, &task_off, &task_left,
-                     &task_lprims, &rec_ptrs, &scan_scratch, &sort.status, &sort.offsets, &sort.counters,
+                     &task_lprims, &scan_scratch, &sort.status, &sort.offsets, &sort.counters,
                      &top_items, &top_acc, &top_tasks, &top_hist, &top_left, &top_bkt};
     for (DevBuf *b : all) b->release();
     if (host_counters) hipHostFree(host_counters);

@@ -210,6 +210,21 @@ def test_intersect_brute_force_small(ctx):
     assert np.array_equal(t, bt)
 
 
+def test_build_size_sequence_in_one_context():
+    """Builds of growing and shrinking treelet counts in one fresh context, so that
+    the scratch buffers are reallocated between builds (treelet records growing
+    while the top-level lists keep their size, and the reverse), each equal to the
+    oracle's tree."""
+    c = pbrtm.Context(0)
+    try:
+        for k, n in enumerate((40, 100, 180, 250, 90, 3000, 200, 70_000, 130, 66, 1000)):
+            tri = soup(n, 100 + k)
+            log(f"build {k}: {n} triangles")
+            assert c.build(tri).export_canonical() == O.OracleBvh(tri).export_canonical(), n
+    finally:
+        c.close()
+
+
 def test_empty_aggregate(ctx):
     tri = np.zeros((0, 3, 3), np.float32)
     bvh = ctx.build(tri)
