@@ -22,7 +22,9 @@ enum CounterSlot : int {
     kCntTopItems1 = 12,
     kCntTopTasks0 = 13,
     kCntTopTasks1 = 14,
-    kNumCounters = 16,
+    kCntLaneBot0 = 15,   // treelets of 2..8 and 9..16 primitives for the one-lane builder
+    kCntLaneBot1 = 16,
+    kNumCounters = 17,
 };
 
 // Everything the build kernels write, passed by value as a kernel argument.
@@ -33,7 +35,9 @@ struct BuildArgs {
     Rec *prec[2];
     WorkItem *small_top, *small_bot;
     WorkItem *large_top[2], *large_bot[2];
+    WorkItem *lane_bot[2];
     uint32_t cap_small_top, cap_small_bot, cap_large_top, cap_large_bot;
+    uint32_t cap_lane_bot[2];
     uint32_t *counters;
     uint32_t *err;
 };

@@ -1195,10 +1195,11 @@ __device__ uint32_t wave_build(const BuildArgs &A, const Rec *src, uint32_t begi
     return nleaf;
 }
 
-// Midpoint subtree of one small treelet built serially by one lane (the common
-// case: a few primitives per treelet). Records are partitioned in place in the
-// primitive buffer, which this treelet owns; the pending-node stack lives in
-// four 64-bit registers as 16-bit entries (lo, hi, node offset).
+// Treelets of 2..16 primitives are built by lane_bottom_kernel, one lane per
+// treelet, with the treelet's boxes held in registers: one batch of loads, then
+// no memory reads. Treelets of up to kLaneSmall primitives and those of up to
+// kLaneBuild go to two lists (the 8-slot variant needs half the registers).
+constexpr uint32_t kLaneSmall = 8;
 constexpr uint32_t kLaneBuild = 16;
 
 __device__ __forceinline__ Rec load_rec(const Rec *p) {
@@ -1210,63 +1211,123 @@ __device__ __forceinline__ Rec load_rec(const Rec *p) {
     return r;
 }
 
-__device__ void lane_build_bottom(const BuildArgs &A, uint32_t first, uint32_t count, uint32_t node0) {
-    Rec *R = A.prec[0] + first;
-    uint64_t st[4] = {0, 0, 0, 0};
-    int sp = 0;
-    auto push = [&](uint32_t lo, uint32_t hi, uint32_t off) {
-        st[3] = (st[3] << 16) | (st[2] >> 48);
-        st[2] = (st[2] << 16) | (st[1] >> 48);
-        st[1] = (st[1] << 16) | (st[0] >> 48);
-        st[0] = (st[0] << 16) | uint64_t(lo | (hi << 5) | (off << 10));
-        ++sp;
-    };
-    push(0, count, 0);
+__device__ __forceinline__ uint32_t nib(uint64_t w, uint32_t p) { return uint32_t(w >> (4 * p)) & 15u; }
+
+// Midpoint subtree of one treelet (hlbvh.cu:828-857): at each node the centroid
+// box's widest axis and its middle, a two-pointer partition on centroid < middle,
+// and a leaf of all the node's primitives when one side is empty. Slot k holds the
+// treelet's k-th primitive as loaded; `at` maps each position of the treelet's
+// range to its slot (4 bits a position) and carries the partitions, so positions,
+// leaf ranges and the primitive order are those of partitioning the records in
+// place. Pending nodes sit in four 64-bit registers as 16-bit entries (lo, hi,
+// node offset); a treelet of c primitives keeps at most c of them.
+// Box and primitive id of slot s, selected with compile-time indices only (the
+// slot arrays stay in registers).
+template <int MAXC>
+__device__ __forceinline__ Box slot_box(const float (&lo)[MAXC][3], const float (&hi)[MAXC][3],
+                                        const uint32_t (&pid)[MAXC], uint32_t s, uint32_t *id) {
+    Box b{{lo[0][0], lo[0][1], lo[0][2]}, {hi[0][0], hi[0][1], hi[0][2]}};
+    uint32_t q = pid[0];
+#pragma unroll
+    for (int k = 1; k < MAXC; ++k) {
+        const bool m = s == uint32_t(k);
+        b.lo.x = m ? lo[k][0] : b.lo.x; b.lo.y = m ? lo[k][1] : b.lo.y; b.lo.z = m ? lo[k][2] : b.lo.z;
+        b.hi.x = m ? hi[k][0] : b.hi.x; b.hi.y = m ? hi[k][1] : b.hi.y; b.hi.z = m ? hi[k][2] : b.hi.z;
+        q = m ? pid[k] : q;
+    }
+    *id = q;
+    return b;
+}
+
+template <int MAXC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXC <= 8 ? 3 : 2))) void lane_bottom_kernel(
+    BuildArgs A, const WorkItem *items, uint32_t nitems) {
+    const uint32_t t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= nitems) return;
+    const WorkItem it = items[t];
+    const uint32_t first = it.begin, count = it.end - it.begin, node0 = it.node;
+    if (count < 2 || count > uint32_t(MAXC)) {  // the list holds 2..MAXC only
+        atomicOr(A.err, uint32_t(kErrCapacity));
+        return;
+    }
+    float lo[MAXC][3], hi[MAXC][3];
+    uint32_t pid[MAXC];
+#pragma unroll
+    for (int k = 0; k < MAXC; ++k) {
+        Rec r{};
+        if (uint32_t(k) < count) r = load_rec(A.prec[0] + first + k);
+        lo[k][0] = r.lo[0]; lo[k][1] = r.lo[1]; lo[k][2] = r.lo[2];
+        hi[k][0] = r.hi[0]; hi[k][1] = r.hi[1]; hi[k][2] = r.hi[2];
+        pid[k] = r.a;
+    }
+    uint64_t at = 0xFEDCBA9876543210ull;
+    uint64_t st0 = uint64_t(0u | (count << 5)), st1 = 0, st2 = 0, st3 = 0;  // the root: [0, count), offset 0
+    int sp = 1;
     while (sp > 0) {
-        const uint32_t e = uint32_t(st[0] & 0xFFFFu);
-        st[0] = (st[0] >> 16) | (st[1] << 48);
-        st[1] = (st[1] >> 16) | (st[2] << 48);
-        st[2] = (st[2] >> 16) | (st[3] << 48);
-        st[3] >>= 16;
+        const uint32_t e = uint32_t(st0 & 0xFFFFu);
+        st0 = (st0 >> 16) | (st1 << 48);
+        st1 = (st1 >> 16) | (st2 << 48);
+        st2 = (st2 >> 16) | (st3 << 48);
+        st3 >>= 16;
         --sp;
-        const uint32_t lo = e & 31u, hi = (e >> 5) & 31u, off = e >> 10;
+        const uint32_t l = e & 31u, h = (e >> 5) & 31u, off = e >> 10;
         const uint32_t id = node0 + off;
-        if (hi - lo == 1) {
-            const Rec q = load_rec(R + lo);
-            write_leaf(A.nodes, id, rec_box(q), first + lo, 1);
-            A.prim_order[first + lo] = q.a;
+        if (h - l == 1) {
+            uint32_t q;
+            const Box b = slot_box<MAXC>(lo, hi, pid, nib(at, l), &q);
+            write_leaf(A.nodes, id, b, first + l, 1);
+            A.prim_order[first + l] = q;
             continue;
         }
+        uint32_t set = 0;
+        for (uint32_t p = l; p < h; ++p) set |= 1u << nib(at, p);
         Box cb = box_empty(), fb = box_empty();
-        for (uint32_t k = lo; k < hi; ++k) {
-            const Box b = rec_box(load_rec(R + k));
+#pragma unroll
+        for (int k = 0; k < MAXC; ++k) {
+            if (!((set >> k) & 1u)) continue;
+            const Box b{{lo[k][0], lo[k][1], lo[k][2]}, {hi[k][0], hi[k][1], hi[k][2]}};
             cb = box_add(cb, box_centroid(b));
             fb = box_union(fb, b);
         }
         const int axis = box_max_dim(cb);
         const float mid = sel3(box_centroid(cb), axis);
-        // two-pointer partition on centroid[axis] < mid (hlbvh.cu:828-857 predicate)
-        uint32_t i = lo, j = hi;
+        uint32_t left = 0;
+#pragma unroll
+        for (int k = 0; k < MAXC; ++k) {
+            const float c = sel3(box_centroid(Box{{lo[k][0], lo[k][1], lo[k][2]}, {hi[k][0], hi[k][1], hi[k][2]}}), axis);
+            left |= (((set >> k) & 1u) && c < mid) ? (1u << k) : 0u;
+        }
+        // two-pointer partition of the positions [l, h)
+        uint32_t i = l, j = h;
         while (i < j) {
-            const Rec ri = load_rec(R + i);
-            if (sel3(box_centroid(rec_box(ri)), axis) < mid) {
+            if ((left >> nib(at, i)) & 1u) {
                 ++i;
             } else {
                 --j;
-                const Rec rj = load_rec(R + j);
-                R[i] = rj;
-                R[j] = ri;
+                const uint64_t si = nib(at, i), sj = nib(at, j);
+                at &= ~((15ull << (4 * i)) | (15ull << (4 * j)));
+                at |= (sj << (4 * i)) | (si << (4 * j));
             }
         }
-        const uint32_t L = i - lo;
-        if (L == 0 || L == hi - lo) {  // all centroids on one side: a multi-primitive leaf
-            write_leaf(A.nodes, id, fb, first + lo, hi - lo);
-            for (uint32_t k = lo; k < hi; ++k) A.prim_order[first + k] = R[k].a;
+        const uint32_t L = i - l;
+        if (L == 0 || L == h - l) {  // all centroids on one side: a multi-primitive leaf
+            write_leaf(A.nodes, id, fb, first + l, h - l);
+            for (uint32_t p = l; p < h; ++p) {
+                uint32_t q;
+                slot_box<MAXC>(lo, hi, pid, nib(at, p), &q);
+                A.prim_order[first + p] = q;
+            }
             continue;
         }
         write_interior(A.nodes, id, fb, L, axis);
-        push(lo + L, hi, off + 2 * L);
-        push(lo, lo + L, off + 1);
+        // push the right child, then the left (taken next)
+        const uint64_t er = uint64_t((l + L) | (h << 5) | ((off + 2 * L) << 10));
+        const uint64_t el = uint64_t(l | ((l + L) << 5) | ((off + 1) << 10));
+        st3 = (st3 << 32) | (st2 >> 32);
+        st2 = (st2 << 32) | (st1 >> 32);
+        st1 = (st1 << 32) | (st0 >> 32);
+        st0 = (st0 << 32) | (er << 16) | el;
+        sp += 2;
     }
 }
 
@@ -1542,8 +1603,8 @@ __device__ __forceinline__ void wave_build_top(const BuildArgs &A, const Rec *sr
 
 // Top-level finisher: one wave per node of <= 64 treelets. After the SAH
 // subtree, each single-treelet leaf becomes the root of a midpoint subtree over
-// its primitives, built in the same wave when it has <= 64 of them (the common
-// case) and queued for the level-synchronous path otherwise.
+// its primitives: a leaf here when it has one, else queued by size for
+// lane_bottom_kernel (<= 16), finish_bottom_kernel (<= 64) or the level path.
 __global__ __launch_bounds__(kFinBlock) void finish_top_kernel(BuildArgs A, const WorkItem *items, uint32_t nitems) {
     __shared__ WaveLds s_lds[kFinWaves];
     const int w = threadIdx.x / kWave;
@@ -1554,7 +1615,8 @@ __global__ __launch_bounds__(kFinBlock) void finish_top_kernel(BuildArgs A, cons
     const WorkItem it = items[wid];
     const uint32_t nleaf = it.end - it.begin;
     wave_build_top(A, pick(A.trec, it.buf), it.begin, nleaf, it.node, L);
-    // lane k keeps the leaf at position k: node id and the treelet's primitive range
+    // lane k keeps the leaf at position k: a single primitive becomes the leaf;
+    // larger treelets go to the bottom-level lists by size
     uint32_t my_node = 0, my_first = 0, my_count = 0;
     if (lane < nleaf) {
         const Rec t = L.rec[lane];
@@ -1570,20 +1632,13 @@ __global__ __launch_bounds__(kFinBlock) void finish_top_kernel(BuildArgs A, cons
                       WorkItem{my_first, my_first + my_count, my_node, my_count, 0, {0, 0, 0}}, A.err);
         }
     }
-    // small treelets: one lane each, all in parallel
-    if (lane < nleaf && my_count >= 2 && my_count <= kLaneBuild) lane_build_bottom(A, my_first, my_count, my_node);
-    wave_lds_sync();
-    // medium treelets: the whole wave, one after another
-    uint64_t medium = __ballot(lane < nleaf && my_count > kLaneBuild && my_count <= kSmallNode);
-    while (medium) {
-        const int k = __builtin_ctzll(medium);
-        medium &= medium - 1;
-        const uint32_t node = __shfl(my_node, k, kWave);
-        const uint32_t first = __shfl(my_first, k, kWave);
-        const uint32_t count = __shfl(my_count, k, kWave);
-        wave_build<false>(A, A.prec[0], first, count, node, count, L);
-        wave_lds_sync();
-    }
+    const WorkItem tw{my_first, my_first + my_count, my_node, my_count, 0, {0, 0, 0}};
+    const bool real = lane < nleaf && my_count >= 2;
+    wave_push(real && my_count <= kLaneSmall, tw, A.lane_bot[0], A.counters + kCntLaneBot0, A.cap_lane_bot[0], A.err);
+    wave_push(real && my_count > kLaneSmall && my_count <= kLaneBuild, tw, A.lane_bot[1], A.counters + kCntLaneBot1,
+              A.cap_lane_bot[1], A.err);
+    wave_push(real && my_count > kLaneBuild && my_count <= kSmallNode, tw, A.small_bot, A.counters + kCntSmallBot,
+              A.cap_small_bot, A.err);
 }
 
 // Bottom-level finisher for nodes of <= 64 primitives left by the level path.
@@ -1824,8 +1879,11 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     A.cap_large_top = D / 64 + 64;
     A.cap_small_bot = n / 2 + 64;
     A.cap_large_bot = n / 64 + 64;
-    const size_t list_items =
-        size_t(A.cap_small_top) + A.cap_small_bot + 2 * size_t(A.cap_large_top) + 2 * size_t(A.cap_large_bot);
+    // disjoint treelets of >= 2 and >= 9 primitives
+    A.cap_lane_bot[0] = std::min(D, n / 2) + 64;
+    A.cap_lane_bot[1] = std::min(D, n / (kLaneSmall + 1)) + 64;
+    const size_t list_items = size_t(A.cap_small_top) + A.cap_small_bot + 2 * size_t(A.cap_large_top) +
+                              2 * size_t(A.cap_large_bot) + A.cap_lane_bot[0] + A.cap_lane_bot[1];
     PBRTM_TRY(S.lists.reserve(list_items * sizeof(WorkItem)));
     WorkItem *lp = S.lists.as<WorkItem>();
     A.small_top = lp; lp += A.cap_small_top;
@@ -1833,7 +1891,9 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     A.large_top[0] = lp; lp += A.cap_large_top;
     A.large_top[1] = lp; lp += A.cap_large_top;
     A.large_bot[0] = lp; lp += A.cap_large_bot;
-    A.large_bot[1] = lp;
+    A.large_bot[1] = lp; lp += A.cap_large_bot;
+    A.lane_bot[0] = lp; lp += A.cap_lane_bot[0];
+    A.lane_bot[1] = lp;
     A.counters = counters;
     A.err = counters + kCntErr;
     const RecPair trecs{{A.trec[0], A.trec[1]}}, precs{{A.prec[0], A.prec[1]}};
@@ -1849,6 +1909,13 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
         finish_top_kernel<<<(out.small_top + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
             A, A.small_top, out.small_top);
     if (times) hipEventRecord(times->ev[3], stream);
+    PBRTM_TRY(read_counters(S, stream));
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t m = std::min(S.host_counters[kCntLaneBot0 + k], A.cap_lane_bot[k]);
+        if (m == 0) continue;
+        if (k == 0) lane_bottom_kernel<kLaneSmall><<<(m + 63) / 64, 64, 0, stream>>>(A, A.lane_bot[0], m);
+        else lane_bottom_kernel<kLaneBuild><<<(m + 63) / 64, 64, 0, stream>>>(A, A.lane_bot[1], m);
+    }
     PBRTM_TRY(run_levels<false>(A, S, stream, precs, kCntLargeBot0, kCntLargeBot1, A.large_bot[0],
                                 A.large_bot[1], n, A.cap_large_bot, &out.bottom_levels));
     PBRTM_TRY(read_counters(S, stream));
