@@ -1213,14 +1213,6 @@ __device__ __forceinline__ Rec load_rec(const Rec *p) {
 
 __device__ __forceinline__ uint32_t nib(uint64_t w, uint32_t p) { return uint32_t(w >> (4 * p)) & 15u; }
 
-// Midpoint subtree of one treelet (hlbvh.cu:828-857): at each node the centroid
-// box's widest axis and its middle, a two-pointer partition on centroid < middle,
-// and a leaf of all the node's primitives when one side is empty. Slot k holds the
-// treelet's k-th primitive as loaded; `at` maps each position of the treelet's
-// range to its slot (4 bits a position) and carries the partitions, so positions,
-// leaf ranges and the primitive order are those of partitioning the records in
-// place. Pending nodes sit in four 64-bit registers as 16-bit entries (lo, hi,
-// node offset); a treelet of c primitives keeps at most c of them.
 // Box and primitive id of slot s, selected with compile-time indices only (the
 // slot arrays stay in registers).
 template <int MAXC>
@@ -1239,13 +1231,50 @@ __device__ __forceinline__ Box slot_box(const float (&lo)[MAXC][3], const float 
     return b;
 }
 
+constexpr int kLaneBlock = 256;
+
+// Midpoint subtree of one treelet (hlbvh.cu:828-857): at each node the centroid
+// box's widest axis and its middle, a two-pointer partition on centroid < middle,
+// and a leaf of all the node's primitives when one side is empty. Slot k holds the
+// treelet's k-th primitive as loaded; `at` maps each position of the treelet's
+// range to its slot (4 bits a position) and carries the partitions, so positions,
+// leaf ranges and the primitive order are those of partitioning the records in
+// place. Pending nodes (two primitives or more; a single one is written as its
+// leaf at once) sit in four 64-bit registers as 16-bit entries (lo, hi, node
+// offset); a treelet of c primitives keeps fewer than c of them.
 template <int MAXC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXC <= 8 ? 3 : 2))) void lane_bottom_kernel(
+__global__ __launch_bounds__(kLaneBlock) __attribute__((amdgpu_waves_per_eu(MAXC <= 8 ? 3 : 2))) void lane_bottom_kernel(
     BuildArgs A, const WorkItem *items, uint32_t nitems) {
-    const uint32_t t = blockIdx.x * 64 + threadIdx.x;
-    if (t >= nitems) return;
-    const WorkItem it = items[t];
-    const uint32_t first = it.begin, count = it.end - it.begin, node0 = it.node;
+    // The workgroup's treelets ordered by size (a counting sort in LDS), so that
+    // each wave's lanes take similar numbers of steps.
+    __shared__ uint32_t s_bin[MAXC + 2];
+    __shared__ uint4 s_item[kLaneBlock];
+    const uint32_t t = blockIdx.x * kLaneBlock + threadIdx.x;
+    uint4 mine = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t key = MAXC + 1;  // past the end: no treelet
+    if (t < nitems) {
+        const WorkItem it = items[t];
+        mine = make_uint4(it.begin, it.end - it.begin, it.node, 0u);
+        key = min(mine.y, uint32_t(MAXC + 1));
+    }
+    if (threadIdx.x < MAXC + 2) s_bin[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t rank_in_bin = atomicAdd(&s_bin[key], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int k = 0; k < MAXC + 2; ++k) {
+            const uint32_t c = s_bin[k];
+            s_bin[k] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    s_item[s_bin[key] + rank_in_bin] = mine;
+    __syncthreads();
+    const uint4 it = s_item[threadIdx.x];
+    const uint32_t first = it.x, count = it.y, node0 = it.z;
+    if (count == 0) return;  // past the list's end (sorted last)
     if (count < 2 || count > uint32_t(MAXC)) {  // the list holds 2..MAXC only
         atomicOr(A.err, uint32_t(kErrCapacity));
         return;
@@ -1271,14 +1300,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXC <= 8 ? 
         st3 >>= 16;
         --sp;
         const uint32_t l = e & 31u, h = (e >> 5) & 31u, off = e >> 10;
-        const uint32_t id = node0 + off;
-        if (h - l == 1) {
-            uint32_t q;
-            const Box b = slot_box<MAXC>(lo, hi, pid, nib(at, l), &q);
-            write_leaf(A.nodes, id, b, first + l, 1);
-            A.prim_order[first + l] = q;
-            continue;
-        }
+        const uint32_t id = node0 + off;  // a node of >= 2 primitives
         uint32_t set = 0;
         for (uint32_t p = l; p < h; ++p) set |= 1u << nib(at, p);
         Box cb = box_empty(), fb = box_empty();
@@ -1320,14 +1342,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXC <= 8 ? 
             continue;
         }
         write_interior(A.nodes, id, fb, L, axis);
-        // push the right child, then the left (taken next)
-        const uint64_t er = uint64_t((l + L) | (h << 5) | ((off + 2 * L) << 10));
-        const uint64_t el = uint64_t(l | ((l + L) << 5) | ((off + 1) << 10));
-        st3 = (st3 << 32) | (st2 >> 32);
-        st2 = (st2 << 32) | (st1 >> 32);
-        st1 = (st1 << 32) | (st0 >> 32);
-        st0 = (st0 << 32) | (er << 16) | el;
-        sp += 2;
+        // a child of one primitive is its leaf now; the others are pushed, the
+        // right one first (the left is taken next)
+        const uint32_t cl[2] = {l, l + L}, ch[2] = {l + L, h}, co[2] = {off + 1, off + 2 * L};
+#pragma unroll
+        for (int c = 1; c >= 0; --c) {
+            if (ch[c] - cl[c] == 1) {
+                uint32_t q;
+                const Box b = slot_box<MAXC>(lo, hi, pid, nib(at, cl[c]), &q);
+                write_leaf(A.nodes, node0 + co[c], b, first + cl[c], 1);
+                A.prim_order[first + cl[c]] = q;
+            } else {
+                st3 = (st3 << 16) | (st2 >> 48);
+                st2 = (st2 << 16) | (st1 >> 48);
+                st1 = (st1 << 16) | (st0 >> 48);
+                st0 = (st0 << 16) | uint64_t(cl[c] | (ch[c] << 5) | (co[c] << 10));
+                ++sp;
+            }
+        }
     }
 }
 
@@ -1913,8 +1945,9 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     for (int k = 0; k < 2; ++k) {
         const uint32_t m = std::min(S.host_counters[kCntLaneBot0 + k], A.cap_lane_bot[k]);
         if (m == 0) continue;
-        if (k == 0) lane_bottom_kernel<kLaneSmall><<<(m + 63) / 64, 64, 0, stream>>>(A, A.lane_bot[0], m);
-        else lane_bottom_kernel<kLaneBuild><<<(m + 63) / 64, 64, 0, stream>>>(A, A.lane_bot[1], m);
+        const uint32_t grid = (m + kLaneBlock - 1) / kLaneBlock;
+        if (k == 0) lane_bottom_kernel<kLaneSmall><<<grid, kLaneBlock, 0, stream>>>(A, A.lane_bot[0], m);
+        else lane_bottom_kernel<kLaneBuild><<<grid, kLaneBlock, 0, stream>>>(A, A.lane_bot[1], m);
     }
     PBRTM_TRY(run_levels<false>(A, S, stream, precs, kCntLargeBot0, kCntLargeBot1, A.large_bot[0],
                                 A.large_bot[1], n, A.cap_large_bot, &out.bottom_levels));
