@@ -889,9 +889,14 @@ __global__ __launch_bounds__(64) void top_decide_kernel(TopLevel TL, uint32_t p,
     }
 }
 
+// Striped over the chunk (item k*256 + thread), so that every load and store
+// instruction covers consecutive records: a treelet's rank among the chunk's
+// left side is the left count of the rows before its own, plus that of the
+// earlier waves and lanes of its row.
 __global__ __launch_bounds__(kChunkBlock) void top_scatter_kernel(TopLevel TL, uint32_t p, RecPair recs,
                                                                   const uint32_t *counters) {
-    __shared__ uint32_t s_waves[kChunkBlock / kWave];
+    constexpr int kWaves = kChunkBlock / kWave;
+    __shared__ uint32_t s_rows[kChunkItems][kWaves];  // left treelets per row and wave
     const uint32_t t = blockIdx.x;
     if (t >= top_count(counters, kCntTopTasks0 + p, TL.cap_tasks)) return;
     const uint2 task = TL.tasks[p][t];
@@ -907,8 +912,9 @@ __global__ __launch_bounds__(kChunkBlock) void top_scatter_kernel(TopLevel TL, u
     const uint32_t left_total = A.lcount;
     const uint32_t left_before = TL.left[t];
     const uint32_t right_before = (s - it.begin) - left_before;
+    const int w = threadIdx.x / kWave;
     Rec r[kChunkItems];
-    uint32_t mask = 0, nleft = 0;
+    uint32_t lbits = 0;
     uint32_t v[2][12];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
@@ -916,36 +922,47 @@ __global__ __launch_bounds__(kChunkBlock) void top_scatter_kernel(TopLevel TL, u
         for (int k = 0; k < 12; ++k) v[c][k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
 #pragma unroll
     for (int k = 0; k < kChunkItems; ++k) {
-        const uint32_t i = s + threadIdx.x * kChunkItems + k;
+        const uint32_t i = s + k * kChunkBlock + threadIdx.x;
+        bool left = false;
         if (i < e) {
             r[k] = R[i];
-            const bool left = TL.bkt[i] <= split;
-            if (left) { mask |= 1u << k; ++nleft; }
+            left = TL.bkt[i] <= split;
             const Box b = rec_box(r[k]);
             const f3 c = box_centroid(b);
 #pragma unroll
             for (int sd = 0; sd < 2; ++sd) {
                 if ((sd == 0) != left) continue;
-                uint32_t *w = v[sd];
-                w[0] = min(w[0], f2ord(c.x)); w[1] = min(w[1], f2ord(c.y)); w[2] = min(w[2], f2ord(c.z));
-                w[3] = max(w[3], f2ord(c.x)); w[4] = max(w[4], f2ord(c.y)); w[5] = max(w[5], f2ord(c.z));
-                w[6] = min(w[6], f2ord(b.lo.x)); w[7] = min(w[7], f2ord(b.lo.y)); w[8] = min(w[8], f2ord(b.lo.z));
-                w[9] = max(w[9], f2ord(b.hi.x)); w[10] = max(w[10], f2ord(b.hi.y)); w[11] = max(w[11], f2ord(b.hi.z));
+                uint32_t *q = v[sd];
+                q[0] = min(q[0], f2ord(c.x)); q[1] = min(q[1], f2ord(c.y)); q[2] = min(q[2], f2ord(c.z));
+                q[3] = max(q[3], f2ord(c.x)); q[4] = max(q[4], f2ord(c.y)); q[5] = max(q[5], f2ord(c.z));
+                q[6] = min(q[6], f2ord(b.lo.x)); q[7] = min(q[7], f2ord(b.lo.y)); q[8] = min(q[8], f2ord(b.lo.z));
+                q[9] = max(q[9], f2ord(b.hi.x)); q[10] = max(q[10], f2ord(b.hi.y)); q[11] = max(q[11], f2ord(b.hi.z));
             }
         }
+        lbits |= left ? (1u << k) : 0u;
+        const uint64_t m = __ballot(left);
+        if (__lane_id() == 0) s_rows[k][w] = __popcll(m);
     }
-    const uint32_t nvalid = (s + threadIdx.x * kChunkItems >= e)
-                                ? 0u
-                                : min(uint32_t(kChunkItems), e - (s + threadIdx.x * kChunkItems));
-    uint32_t tot;
-    const uint32_t lpre = block_exclusive_add<kChunkBlock>(nleft, s_waves, &tot);
-    const uint32_t rpre = threadIdx.x * kChunkItems - lpre;
-    uint32_t li = 0, ri = 0;
+    __syncthreads();
+    uint32_t row_base = 0;  // left treelets in the rows before k
 #pragma unroll
     for (int k = 0; k < kChunkItems; ++k) {
-        if (uint32_t(k) >= nvalid) break;
-        if (mask & (1u << k)) W[it.begin + left_before + lpre + li++] = r[k];
-        else W[it.begin + left_total + right_before + rpre + ri++] = r[k];
+        uint32_t row_total = 0, before = 0;
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ++ww) {
+            const uint32_t c = s_rows[k][ww];
+            row_total += c;
+            before += ww < w ? c : 0u;
+        }
+        const bool left = (lbits >> k) & 1u;
+        const uint64_t m = __ballot(left);
+        const uint32_t pos = k * kChunkBlock + threadIdx.x;  // position in the chunk
+        const uint32_t lefts_before = row_base + before + __popcll(m & lanemask_lt());
+        if (s + pos < e) {
+            if (left) W[it.begin + left_before + lefts_before] = r[k];
+            else W[it.begin + left_total + right_before + (pos - lefts_before)] = r[k];
+        }
+        row_base += row_total;
     }
     block_merge_boxes(v[0], A.child[0]);
     __syncthreads();  // block_merge_boxes reuses its LDS staging
