@@ -93,6 +93,9 @@ int pbrtm_morton_sort(pbrtm_ctx *ctx, const float *tri, uint64_t n, uint32_t fla
  * outlive the handle. n == 0 builds an empty aggregate (hlbvh.cu:392-395). */
 int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint32_t *idx, uint64_t nt,
                     uint32_t flags, pbrtm_bvh **out, pbrtm_build_stats *stats);
+/* The context keeps the freed aggregate's device buffers (one set, released by
+ * pbrtm_destroy) and hands them to its next build: building, using and freeing
+ * in a loop allocates no device memory after the first pass. */
 void pbrtm_bvh_free(pbrtm_bvh *bvh);
 
 /* Bounds of the whole aggregate: lo.xyz, hi.xyz. Replaces HLBVH::bounds()

@@ -29,6 +29,21 @@ struct BatchTune {
     double ms[6] = {};
 };
 
+// Device buffers of the last aggregate freed on a context, handed to its next
+// build (one set), so that rebuilding in a loop allocates nothing.
+struct SpareBvh {
+    DevBuf nodes, prim_order, tri_leaf, wnodes, worder;
+};
+
+// dst takes src's allocation (dst's own is released first).
+static void take_buf(DevBuf &dst, DevBuf &src) {
+    dst.release();
+    dst.ptr = src.ptr;
+    dst.bytes = src.bytes;
+    src.ptr = nullptr;
+    src.bytes = 0;
+}
+
 struct pbrtm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -39,6 +54,7 @@ struct pbrtm_ctx {
     hipEvent_t ev[8] = {};
     std::string error;
     std::vector<BatchTune> batch_tune;
+    SpareBvh spare;
 };
 
 struct pbrtm_bvh {
@@ -226,6 +242,9 @@ void pbrtm_destroy(pbrtm_ctx *ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
+    for (DevBuf *b : {&ctx->spare.nodes, &ctx->spare.prim_order, &ctx->spare.tri_leaf, &ctx->spare.wnodes,
+                      &ctx->spare.worder})
+        b->release();
     ctx->stage_in.release();
     ctx->stage_a.release();
     ctx->stage_b.release();
@@ -285,6 +304,11 @@ int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint3
     bvh->ctx = ctx;
     bvh->device = ctx->device;
     bvh->n = uint32_t(nt);
+    take_buf(bvh->dev.nodes, ctx->spare.nodes);  // kept if large enough, else reallocated
+    take_buf(bvh->dev.prim_order, ctx->spare.prim_order);
+    take_buf(bvh->tri_leaf, ctx->spare.tri_leaf);
+    take_buf(bvh->wide.nodes, ctx->spare.wnodes);
+    take_buf(bvh->wide.order, ctx->spare.worder);
     const float mx = 3.402823466e+38f;
     const float empty[6] = {mx, mx, mx, -mx, -mx, -mx};
     std::memcpy(bvh->bounds, empty, sizeof(empty));
@@ -375,6 +399,14 @@ int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint3
 void pbrtm_bvh_free(pbrtm_bvh *bvh) {
     if (!bvh) return;
     hipSetDevice(bvh->device);  // hipFree waits for work that uses the memory
+    if (bvh->ctx) {  // the context's next build reuses these (its work is ordered after this one's on the stream)
+        SpareBvh &s = bvh->ctx->spare;
+        take_buf(s.nodes, bvh->dev.nodes);
+        take_buf(s.prim_order, bvh->dev.prim_order);
+        take_buf(s.tri_leaf, bvh->tri_leaf);
+        take_buf(s.wnodes, bvh->wide.nodes);
+        take_buf(s.worder, bvh->wide.order);
+    }
     bvh->dev.nodes.release();
     bvh->dev.prim_order.release();
     bvh->tri_owned.release();

@@ -784,18 +784,54 @@ __global__ __launch_bounds__(kChunkBlock) void top_bucket_kernel(TopLevel TL, ui
         }
     }
     __syncthreads();
+    // Treelets next to each other in Morton order mostly share a bucket: each wave
+    // reduces the lanes of its first two buckets across the wave and one lane
+    // merges each result into LDS (instead of many same-address atomics); lanes
+    // of further buckets (deep levels, where a wave spans many) merge their own.
     const uint32_t s = it.begin + task.y * kChunk, e = min(it.end, s + kChunk);
-    for (uint32_t i = s + threadIdx.x; i < e; i += kChunkBlock) {
-        const Rec r = R[i];
-        const Box b = rec_box(r);
-        const uint32_t k = bucket_of(sel3(box_centroid(b), axis), base, span);
-        TL.bkt[i] = uint8_t(k);
-        atomicAdd(&s_cnt[k], 1u);
-        atomicAdd(&s_pr[k], r.b);
-        atomicMin(&s_box[k][0], f2ord(b.lo.x)); atomicMin(&s_box[k][1], f2ord(b.lo.y));
-        atomicMin(&s_box[k][2], f2ord(b.lo.z));
-        atomicMax(&s_box[k][3], f2ord(b.hi.x)); atomicMax(&s_box[k][4], f2ord(b.hi.y));
-        atomicMax(&s_box[k][5], f2ord(b.hi.z));
+    for (uint32_t i0 = s; i0 < e; i0 += kChunkBlock) {  // the same trip count in every thread
+        const uint32_t i = i0 + threadIdx.x;
+        const bool valid = i < e;
+        Rec r{};
+        uint32_t k = 0;
+        if (valid) {
+            r = R[i];
+            k = bucket_of(sel3(box_centroid(rec_box(r)), axis), base, span);
+            TL.bkt[i] = uint8_t(k);
+        }
+        uint64_t todo = __ballot(valid);
+        for (int round = 0; round < 2 && todo; ++round) {
+            const int lead = __builtin_ctzll(todo);
+            const uint32_t kb = __shfl(k, lead, kWave);
+            const bool in = valid && k == kb;
+            const uint64_t peers = __ballot(in);
+            todo &= ~peers;
+            const uint32_t pr = wave_sum_all(in ? r.b : 0u);
+            uint32_t bx[6];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                bx[d] = wave_min_u32_all(in ? f2ord(r.lo[d]) : 0xFFFFFFFFu);
+                bx[3 + d] = wave_max_u32_all(in ? f2ord(r.hi[d]) : 0u);
+            }
+            if (int(__lane_id()) == lead) {
+                atomicAdd(&s_cnt[kb], uint32_t(__popcll(peers)));
+                atomicAdd(&s_pr[kb], pr);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    atomicMin(&s_box[kb][d], bx[d]);
+                    atomicMax(&s_box[kb][3 + d], bx[3 + d]);
+                }
+            }
+        }
+        if ((todo >> __lane_id()) & 1u) {
+            atomicAdd(&s_cnt[k], 1u);
+            atomicAdd(&s_pr[k], r.b);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                atomicMin(&s_box[k][d], f2ord(r.lo[d]));
+                atomicMax(&s_box[k][3 + d], f2ord(r.hi[d]));
+            }
+        }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < int(kNumBuckets * 7); k += kChunkBlock) {
