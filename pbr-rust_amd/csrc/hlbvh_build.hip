@@ -1092,7 +1092,7 @@ constexpr int kFinWaves = kFinBlock / kWave;
 
 // Top-level nodes of at most this many treelets are split by one lane each
 // (after the wave has split the larger ones cooperatively).
-constexpr uint32_t kLaneNode = 8;
+constexpr uint32_t kLaneNode = 4;
 
 // Per-wave LDS working set of the finisher.
 struct WaveLds {
