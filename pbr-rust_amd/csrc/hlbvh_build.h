@@ -24,7 +24,8 @@ enum CounterSlot : int {
     kCntTopTasks1 = 14,
     kCntLaneBot0 = 15,   // treelets of 2..8 and 9..16 primitives for the one-lane builder
     kCntLaneBot1 = 16,
-    kNumCounters = 17,
+    kCntMidTop = 17,     // top-level nodes of 65..2048 treelets for the workgroup finisher
+    kNumCounters = 18,
 };
 
 // Everything the build kernels write, passed by value as a kernel argument.
@@ -36,8 +37,10 @@ struct BuildArgs {
     WorkItem *small_top, *small_bot;
     WorkItem *large_top[2], *large_bot[2];
     WorkItem *lane_bot[2];
+    WorkItem *mid_top;
     uint32_t cap_small_top, cap_small_bot, cap_large_top, cap_large_bot;
     uint32_t cap_lane_bot[2];
+    uint32_t cap_mid_top;
     uint32_t *counters;
     uint32_t *err;
 };

@@ -25,6 +25,7 @@ namespace pbrtm {
 namespace {
 
 constexpr int kChunk = 2048;       // elements per workgroup task
+constexpr uint32_t kMidNode = kChunk;  // top-level nodes up to this size finish in one workgroup
 constexpr int kChunkBlock = 256;
 constexpr int kChunkItems = kChunk / kChunkBlock;
 constexpr int kAccWords = 192;     // per-item accumulator (see Acc below)
@@ -266,7 +267,7 @@ __global__ void root_kernel(BuildArgs A, uint32_t num_treelets, uint32_t n) {
     }
     WorkItem it{0, num_treelets, 0, n, 0, {0, 0, 0}};
     if (num_treelets <= kSmallNode) push_item(A.small_top, A.counters + kCntSmallTop, A.cap_small_top, it, A.err);
-    else push_item(A.large_top[0], A.counters + kCntLargeTop0, A.cap_large_top, it, A.err);
+    else push_item(A.mid_top, A.counters + kCntMidTop, A.cap_mid_top, it, A.err);
 }
 
 // ------------------------------------------------- level step for large nodes
@@ -1039,9 +1040,10 @@ __global__ __launch_bounds__(64) void top_children_kernel(BuildArgs A, TopLevel 
         if (ok) cw = c == 0 ? WorkItem{it.begin, it.begin + L, it.node + 1, lp, nb, {0, 0, 0}}
                             : WorkItem{it.begin + L, it.end, it.node + 2 * lp, it.prims - lp, nb, {0, 0, 0}};
         const uint32_t cs = cw.end - cw.begin;
-        const bool large = ok && cs > kSmallNode;
+        const bool large = ok && cs > kMidNode;
         if (ok && cs == 1) emit_bottom_root(A, A.trec[nb][cw.begin], cw.node);
         wave_push(ok && cs > 1 && cs <= kSmallNode, cw, A.small_top, counters + kCntSmallTop, A.cap_small_top, A.err);
+        wave_push(ok && cs > kSmallNode && cs <= kMidNode, cw, A.mid_top, counters + kCntMidTop, A.cap_mid_top, A.err);
         const uint32_t nt = large ? (cs + kChunk - 1) / kChunk : 0u;
         const uint32_t inc_i = wave_inclusive_add(large ? 1u : 0u), inc_t = wave_inclusive_add(nt);
         uint32_t base_i = 0, base_t = 0;
@@ -1082,6 +1084,253 @@ __global__ __launch_bounds__(256) void top_acc_reset_kernel(TopLevel TL, uint32_
         if (k >= 12 + kNumBuckets && k < 12 + 7 * kNumBuckets) v = ((k - 12 - kNumBuckets) % 6) < 3 ? 0xFFFFFFFFu : 0u;
         else if (k >= 12 + 7 * kNumBuckets) v = ((k - 12 - 7 * kNumBuckets) % 6) < 3 ? 0xFFFFFFFFu : 0u;
         reinterpret_cast<uint32_t *>(&TL.acc[q][item])[k] = v;
+    }
+}
+
+// --------------------------------- workgroup finisher (65..2048 treelets)
+//
+// A top-level node of at most kMidNode treelets is split, together with its
+// whole subtree down to nodes of <= 64 treelets, by one workgroup: per split a
+// bounds pass, a bucket pass (hlbvh.cu:690-706), the price of the 23 splits by
+// one wave (first strict minimum, hlbvh.cu:711-741) and a stable partition into
+// the other record buffer, synchronised inside the workgroup. The deep levels of
+// the level path (thousands of small nodes, a few launches each) become one launch.
+constexpr int kMidStack = 16;  // pending nodes: the larger child waits, so at most log2(2048 / 65) + 1
+
+// Order-mapped min (k % 6 < 3) / max of 12 words over the workgroup into dst (LDS);
+// every thread of the workgroup calls it. Leaves a barrier behind.
+__device__ __forceinline__ void block_box_lds(uint32_t v[12], uint32_t (*s_part)[12], uint32_t *dst) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? wave_min_u32_all(v[k]) : wave_max_u32_all(v[k]);
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0)
+        for (int k = 0; k < 12; ++k) s_part[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        uint32_t r = s_part[0][k];
+        for (int ww = 1; ww < kChunkBlock / kWave; ++ww) r = (k % 6) < 3 ? min(r, s_part[ww][k]) : max(r, s_part[ww][k]);
+        dst[k] = r;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A, uint32_t nitems) {
+    constexpr int kWaves = kChunkBlock / kWave;
+    __shared__ uint4 s_stack[kMidStack];  // (begin, end, node id, prims | buffer << 31)
+    __shared__ uint32_t s_sp;
+    __shared__ uint8_t s_bkt[kMidNode];   // bucket of each position of the node
+    __shared__ uint32_t s_cnt[kNumBuckets], s_pr[kNumBuckets], s_box[kNumBuckets][6];
+    __shared__ uint32_t s_part[kWaves][12];
+    __shared__ uint32_t s_nb[12];  // the node's centroid and full box
+    __shared__ uint32_t s_rows[kChunkItems][kWaves];
+    __shared__ uint32_t s_dec[3];  // split (kTopInvalid: none), left primitives, left treelets
+    if (blockIdx.x >= nitems) return;
+    if (threadIdx.x == 0) {
+        const WorkItem it = A.mid_top[blockIdx.x];
+        s_stack[0] = make_uint4(it.begin, it.end, it.node, it.prims | (it.buf << 31));
+        s_sp = 1;
+    }
+    __syncthreads();
+    for (;;) {
+        const uint32_t sp = s_sp;  // read by every thread after the last barrier: uniform
+        if (sp == 0) break;
+        const uint4 nd = s_stack[sp - 1];
+        const uint32_t begin = nd.x, end = nd.y, node = nd.z, prims = nd.w & 0x7FFFFFFFu, buf = nd.w >> 31;
+        const uint32_t m = end - begin;
+        const Rec *R = pick(A.trec, buf);
+        Rec *W = pick(A.trec, 1 - buf);
+        // bounds
+        uint32_t v[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
+        for (uint32_t p = threadIdx.x; p < m; p += kChunkBlock) {
+            const Box b = rec_box(R[begin + p]);
+            const f3 c = box_centroid(b);
+            v[0] = min(v[0], f2ord(c.x)); v[1] = min(v[1], f2ord(c.y)); v[2] = min(v[2], f2ord(c.z));
+            v[3] = max(v[3], f2ord(c.x)); v[4] = max(v[4], f2ord(c.y)); v[5] = max(v[5], f2ord(c.z));
+            v[6] = min(v[6], f2ord(b.lo.x)); v[7] = min(v[7], f2ord(b.lo.y)); v[8] = min(v[8], f2ord(b.lo.z));
+            v[9] = max(v[9], f2ord(b.hi.x)); v[10] = max(v[10], f2ord(b.hi.y)); v[11] = max(v[11], f2ord(b.hi.z));
+        }
+        if (threadIdx.x < kNumBuckets) {
+            s_cnt[threadIdx.x] = 0u;
+            s_pr[threadIdx.x] = 0u;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) s_box[threadIdx.x][j] = j < 3 ? 0xFFFFFFFFu : 0u;
+        }
+        block_box_lds(v, s_part, s_nb);  // its barriers also order every thread's read of the stack before the pop
+        if (threadIdx.x == 0) s_sp = sp - 1;
+        const Box cb = acc_box(s_nb), fb = acc_box(s_nb + 6);
+        const int axis = box_max_dim(cb);
+        if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) {  // the node stays unbuilt
+            if (threadIdx.x == 0) atomicOr(A.err, uint32_t(kErrDegenerateCentroids));
+            __syncthreads();
+            continue;
+        }
+        const float base = sel3(cb.lo, axis);
+        const float span = sel3(cb.hi, axis) - sel3(cb.lo, axis);
+        // buckets: the first two of each wave reduced across the wave, the rest per lane
+        for (uint32_t p0 = 0; p0 < m; p0 += kChunkBlock) {  // the same trip count in every thread
+            const uint32_t p = p0 + threadIdx.x;
+            const bool valid = p < m;
+            Rec r{};
+            uint32_t k = 0;
+            if (valid) {
+                r = R[begin + p];
+                k = bucket_of(sel3(box_centroid(rec_box(r)), axis), base, span);
+                s_bkt[p] = uint8_t(k);
+            }
+            uint64_t todo = __ballot(valid);
+            for (int round = 0; round < 2 && todo; ++round) {
+                const int lead = __builtin_ctzll(todo);
+                const uint32_t kb = __shfl(k, lead, kWave);
+                const bool in = valid && k == kb;
+                const uint64_t peers = __ballot(in);
+                todo &= ~peers;
+                const uint32_t pr = wave_sum_all(in ? r.b : 0u);
+                uint32_t bx[6];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    bx[d] = wave_min_u32_all(in ? f2ord(r.lo[d]) : 0xFFFFFFFFu);
+                    bx[3 + d] = wave_max_u32_all(in ? f2ord(r.hi[d]) : 0u);
+                }
+                if (int(__lane_id()) == lead) {
+                    atomicAdd(&s_cnt[kb], uint32_t(__popcll(peers)));
+                    atomicAdd(&s_pr[kb], pr);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        atomicMin(&s_box[kb][d], bx[d]);
+                        atomicMax(&s_box[kb][3 + d], bx[3 + d]);
+                    }
+                }
+            }
+            if ((todo >> __lane_id()) & 1u) {
+                atomicAdd(&s_cnt[k], 1u);
+                atomicAdd(&s_pr[k], r.b);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    atomicMin(&s_box[k][d], f2ord(r.lo[d]));
+                    atomicMax(&s_box[k][3 + d], f2ord(r.hi[d]));
+                }
+            }
+        }
+        __syncthreads();
+        // the price of the 23 splits (lane k = bucket k), as top_decide_kernel
+        if (threadIdx.x < kWave) {
+            const int lane = threadIdx.x;
+            const bool in = lane < int(kNumBuckets);
+            const uint32_t cnt = in ? s_pr[lane] : 0u, tcnt = in ? s_cnt[lane] : 0u;
+            // an untouched bucket holds the atomic identities: use the empty box
+            Box b = box_empty();
+            if (in && tcnt > 0)
+                b = {{ord2f(s_box[lane][0]), ord2f(s_box[lane][1]), ord2f(s_box[lane][2])},
+                     {ord2f(s_box[lane][3]), ord2f(s_box[lane][4]), ord2f(s_box[lane][5])}};
+            float pl[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+            float sf[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+            uint32_t pc = cnt, sc = cnt, tc = tcnt;
+#pragma unroll
+            for (int d = 1; d < 32; d <<= 1) {
+                const bool up = lane >= d;
+                const bool dn = lane + d < int(kNumBuckets);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    const float u = __shfl_up(pl[k], d, kWave);
+                    const float w = __shfl_down(sf[k], d, kWave);
+                    if (up) pl[k] = k < 3 ? smin(u, pl[k]) : smax(u, pl[k]);
+                    if (dn) sf[k] = k < 3 ? smin(sf[k], w) : smax(sf[k], w);
+                }
+                const uint32_t uc = __shfl_up(pc, d, kWave), wc = __shfl_down(sc, d, kWave);
+                const uint32_t ut = __shfl_up(tc, d, kWave);
+                if (up) pc += uc, tc += ut;
+                if (dn) sc += wc;
+            }
+            float r6[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) r6[k] = __shfl_down(sf[k], 1, kWave);
+            const uint32_t rc = __shfl_down(sc, 1, kWave);
+            const float total_sa = box_area(fb);
+            uint32_t key = 0xFFFFFFFFu;
+            if (lane < int(kNumBuckets) - 1) {
+                const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
+                const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
+                const float cost = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
+                key = f2ord(cost);
+            }
+            const uint32_t kmin = wave_min_u32(key);
+            const uint32_t split = uint32_t(__builtin_ctzll(__ballot(lane < int(kNumBuckets) - 1 && key == kmin)));
+            const uint32_t lp = __shfl(pc, int(split), kWave), rp = __shfl(rc, int(split), kWave);
+            const uint32_t lt = __shfl(tc, int(split), kWave);
+            if (lane == 0) {
+                // both end buckets hold a treelet, so neither side is empty (as top_decide_kernel)
+                s_dec[0] = (lp == 0u || rp == 0u) ? kTopInvalid : split;
+                s_dec[1] = lp;
+                s_dec[2] = lt;
+            }
+        }
+        __syncthreads();
+        const uint32_t split = s_dec[0], lp = s_dec[1], L = s_dec[2];
+        if (split == kTopInvalid || L == 0u || L == m) {
+            if (threadIdx.x == 0) atomicOr(A.err, uint32_t(kErrSplit));
+            __syncthreads();
+            continue;
+        }
+        // stable partition into the other buffer, striped rows (position k * 256 + thread)
+        const int w = threadIdx.x / kWave;
+        uint32_t lbits = 0;
+#pragma unroll
+        for (int k = 0; k < kChunkItems; ++k) {
+            const uint32_t p = k * kChunkBlock + threadIdx.x;
+            const bool left = p < m && s_bkt[p] <= split;
+            lbits |= left ? (1u << k) : 0u;
+            const uint64_t bm = __ballot(left);
+            if (__lane_id() == 0) s_rows[k][w] = __popcll(bm);
+        }
+        __syncthreads();
+        uint32_t row_base = 0;
+#pragma unroll
+        for (int k = 0; k < kChunkItems; ++k) {
+            uint32_t row_total = 0, before = 0;
+#pragma unroll
+            for (int ww = 0; ww < kWaves; ++ww) {
+                const uint32_t c = s_rows[k][ww];
+                row_total += c;
+                before += ww < w ? c : 0u;
+            }
+            const bool left = (lbits >> k) & 1u;
+            const uint64_t bm = __ballot(left);
+            const uint32_t p = k * kChunkBlock + threadIdx.x;
+            const uint32_t lb = row_base + before + __popcll(bm & lanemask_lt());
+            if (p < m) {
+                const Rec rec = R[begin + p];
+                if (left) W[begin + lb] = rec;
+                else W[begin + L + (p - lb)] = rec;
+            }
+            row_base += row_total;
+        }
+        __syncthreads();  // the partitioned records are visible to thread 0 below
+        if (threadIdx.x == 0) {
+            write_interior(A.nodes, node, fb, lp, uint32_t(axis));
+            const uint32_t nb = 1 - buf;
+            const WorkItem cl{begin, begin + L, node + 1, lp, nb, {0, 0, 0}};
+            const WorkItem cr{begin + L, end, node + 2 * lp, prims - lp, nb, {0, 0, 0}};
+            const bool right_big = m - L > L;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {  // the larger child waits below the smaller one
+                const WorkItem cw = (j == 0) == right_big ? cr : cl;
+                const uint32_t cs = cw.end - cw.begin;
+                if (cs == 1) {
+                    emit_bottom_root(A, A.trec[nb][cw.begin], cw.node);
+                } else if (cs <= kSmallNode) {
+                    push_item(A.small_top, A.counters + kCntSmallTop, A.cap_small_top, cw, A.err);
+                } else if (s_sp < uint32_t(kMidStack)) {
+                    s_stack[s_sp] = make_uint4(cw.begin, cw.end, cw.node, cw.prims | (nb << 31));
+                    s_sp = s_sp + 1;
+                } else {
+                    atomicOr(A.err, uint32_t(kErrCapacity));
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1967,8 +2216,10 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     // disjoint treelets of >= 2 and >= 9 primitives
     A.cap_lane_bot[0] = std::min(D, n / 2) + 64;
     A.cap_lane_bot[1] = std::min(D, n / (kLaneSmall + 1)) + 64;
+    A.cap_mid_top = D / (kSmallNode + 1) + 64;  // disjoint nodes of > 64 treelets
     const size_t list_items = size_t(A.cap_small_top) + A.cap_small_bot + 2 * size_t(A.cap_large_top) +
-                              2 * size_t(A.cap_large_bot) + A.cap_lane_bot[0] + A.cap_lane_bot[1];
+                              2 * size_t(A.cap_large_bot) + A.cap_lane_bot[0] + A.cap_lane_bot[1] +
+                              A.cap_mid_top;
     PBRTM_TRY(S.lists.reserve(list_items * sizeof(WorkItem)));
     WorkItem *lp = S.lists.as<WorkItem>();
     A.small_top = lp; lp += A.cap_small_top;
@@ -1978,17 +2229,23 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     A.large_bot[0] = lp; lp += A.cap_large_bot;
     A.large_bot[1] = lp; lp += A.cap_large_bot;
     A.lane_bot[0] = lp; lp += A.cap_lane_bot[0];
-    A.lane_bot[1] = lp;
+    A.lane_bot[1] = lp; lp += A.cap_lane_bot[1];
+    A.mid_top = lp;
     A.counters = counters;
     A.err = counters + kCntErr;
     const RecPair trecs{{A.trec[0], A.trec[1]}}, precs{{A.prec[0], A.prec[1]}};
 
-    if (D > kSmallNode) {  // the fused top level down to nodes of <= 64 treelets
+    if (D > kMidNode) {  // the fused top level down to nodes of <= 2048 treelets
         PBRTM_TRY(run_top_levels(A, S, stream, trecs, D, n, &out.top_levels));
     } else {
         root_kernel<<<1, 64, 0, stream>>>(A, D, n);
     }
     PBRTM_TRY(read_counters(S, stream));
+    const uint32_t nmid = std::min(S.host_counters[kCntMidTop], A.cap_mid_top);
+    if (nmid > 0) {  // their subtrees down to nodes of <= 64 treelets
+        mid_top_kernel<<<nmid, kChunkBlock, 0, stream>>>(A, nmid);
+        PBRTM_TRY(read_counters(S, stream));
+    }
     out.small_top = S.host_counters[kCntSmallTop];
     if (out.small_top > 0)
         finish_top_kernel<<<(out.small_top + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
