@@ -1087,55 +1087,40 @@ __global__ __launch_bounds__(256) void top_acc_reset_kernel(TopLevel TL, uint32_
     }
 }
 
-// --------------------------------- workgroup finisher (65..2048 treelets)
+// ---------------------------------------- wave finisher for 65..2048 treelets
 //
 // A top-level node of at most kMidNode treelets is split, together with its
-// whole subtree down to nodes of <= 64 treelets, by one workgroup: per split a
-// bounds pass, a bucket pass (hlbvh.cu:690-706), the price of the 23 splits by
-// one wave (first strict minimum, hlbvh.cu:711-741) and a stable partition into
-// the other record buffer, synchronised inside the workgroup. The deep levels of
-// the level path (thousands of small nodes, a few launches each) become one launch.
+// whole subtree down to nodes of <= 64 treelets, by one wave, one node after
+// another from a stack in LDS: per split a bounds pass, a bucket pass
+// (hlbvh.cu:690-706), the price of the 23 splits (first strict minimum,
+// hlbvh.cu:711-741) and a stable partition into the other record buffer, 64
+// positions a row. The waves of a workgroup work independently (no workgroup
+// barriers). The deep levels of the level path (thousands of small nodes, a few
+// launches each) become this one launch.
 constexpr int kMidStack = 16;  // pending nodes: the larger child waits, so at most log2(2048 / 65) + 1
+constexpr int kMidWaves = kChunkBlock / kWave;
 
-// Order-mapped min (k % 6 < 3) / max of 12 words over the workgroup into dst (LDS);
-// every thread of the workgroup calls it. Leaves a barrier behind.
-__device__ __forceinline__ void block_box_lds(uint32_t v[12], uint32_t (*s_part)[12], uint32_t *dst) {
-#pragma unroll
-    for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? wave_min_u32_all(v[k]) : wave_max_u32_all(v[k]);
-    const int w = threadIdx.x / kWave;
-    if (__lane_id() == 0)
-        for (int k = 0; k < 12; ++k) s_part[w][k] = v[k];
-    __syncthreads();
-    if (threadIdx.x < 12) {
-        const int k = threadIdx.x;
-        uint32_t r = s_part[0][k];
-        for (int ww = 1; ww < kChunkBlock / kWave; ++ww) r = (k % 6) < 3 ? min(r, s_part[ww][k]) : max(r, s_part[ww][k]);
-        dst[k] = r;
-    }
-    __syncthreads();
-}
+struct MidLds {
+    uint4 stack[kMidStack];  // (begin, end, node id, prims | buffer << 31)
+    uint32_t cnt[kNumBuckets], pr[kNumBuckets], box[kNumBuckets][6];
+    uint8_t bkt[kMidNode];  // bucket of each position of the node
+};
 
 __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A, uint32_t nitems) {
-    constexpr int kWaves = kChunkBlock / kWave;
-    __shared__ uint4 s_stack[kMidStack];  // (begin, end, node id, prims | buffer << 31)
-    __shared__ uint32_t s_sp;
-    __shared__ uint8_t s_bkt[kMidNode];   // bucket of each position of the node
-    __shared__ uint32_t s_cnt[kNumBuckets], s_pr[kNumBuckets], s_box[kNumBuckets][6];
-    __shared__ uint32_t s_part[kWaves][12];
-    __shared__ uint32_t s_nb[12];  // the node's centroid and full box
-    __shared__ uint32_t s_rows[kChunkItems][kWaves];
-    __shared__ uint32_t s_dec[3];  // split (kTopInvalid: none), left primitives, left treelets
-    if (blockIdx.x >= nitems) return;
-    if (threadIdx.x == 0) {
-        const WorkItem it = A.mid_top[blockIdx.x];
-        s_stack[0] = make_uint4(it.begin, it.end, it.node, it.prims | (it.buf << 31));
-        s_sp = 1;
+    __shared__ MidLds s_lds[kMidWaves];
+    const uint32_t wid = blockIdx.x * kMidWaves + threadIdx.x / kWave;
+    if (wid >= nitems) return;  // the whole wave
+    MidLds &L = s_lds[threadIdx.x / kWave];
+    const uint32_t lane = __lane_id();
+    {
+        const WorkItem it = A.mid_top[wid];
+        if (lane == 0) L.stack[0] = make_uint4(it.begin, it.end, it.node, it.prims | (it.buf << 31));
     }
-    __syncthreads();
-    for (;;) {
-        const uint32_t sp = s_sp;  // read by every thread after the last barrier: uniform
-        if (sp == 0) break;
-        const uint4 nd = s_stack[sp - 1];
+    int sp = 1;  // uniform
+    wave_lds_sync();
+    while (sp > 0) {
+        --sp;
+        const uint4 nd = L.stack[sp];
         const uint32_t begin = nd.x, end = nd.y, node = nd.z, prims = nd.w & 0x7FFFFFFFu, buf = nd.w >> 31;
         const uint32_t m = end - begin;
         const Rec *R = pick(A.trec, buf);
@@ -1144,7 +1129,7 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A, uint3
         uint32_t v[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
-        for (uint32_t p = threadIdx.x; p < m; p += kChunkBlock) {
+        for (uint32_t p = lane; p < m; p += kWave) {
             const Box b = rec_box(R[begin + p]);
             const f3 c = box_centroid(b);
             v[0] = min(v[0], f2ord(c.x)); v[1] = min(v[1], f2ord(c.y)); v[2] = min(v[2], f2ord(c.z));
@@ -1152,33 +1137,33 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A, uint3
             v[6] = min(v[6], f2ord(b.lo.x)); v[7] = min(v[7], f2ord(b.lo.y)); v[8] = min(v[8], f2ord(b.lo.z));
             v[9] = max(v[9], f2ord(b.hi.x)); v[10] = max(v[10], f2ord(b.hi.y)); v[11] = max(v[11], f2ord(b.hi.z));
         }
-        if (threadIdx.x < kNumBuckets) {
-            s_cnt[threadIdx.x] = 0u;
-            s_pr[threadIdx.x] = 0u;
 #pragma unroll
-            for (int j = 0; j < 6; ++j) s_box[threadIdx.x][j] = j < 3 ? 0xFFFFFFFFu : 0u;
-        }
-        block_box_lds(v, s_part, s_nb);  // its barriers also order every thread's read of the stack before the pop
-        if (threadIdx.x == 0) s_sp = sp - 1;
-        const Box cb = acc_box(s_nb), fb = acc_box(s_nb + 6);
+        for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? wave_min_u32_all(v[k]) : wave_max_u32_all(v[k]);
+        const Box cb = acc_box(v), fb = acc_box(v + 6);
         const int axis = box_max_dim(cb);
         if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) {  // the node stays unbuilt
-            if (threadIdx.x == 0) atomicOr(A.err, uint32_t(kErrDegenerateCentroids));
-            __syncthreads();
+            if (lane == 0) atomicOr(A.err, uint32_t(kErrDegenerateCentroids));
             continue;
         }
         const float base = sel3(cb.lo, axis);
         const float span = sel3(cb.hi, axis) - sel3(cb.lo, axis);
-        // buckets: the first two of each wave reduced across the wave, the rest per lane
-        for (uint32_t p0 = 0; p0 < m; p0 += kChunkBlock) {  // the same trip count in every thread
-            const uint32_t p = p0 + threadIdx.x;
+        if (lane < kNumBuckets) {
+            L.cnt[lane] = 0u;
+            L.pr[lane] = 0u;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) L.box[lane][j] = j < 3 ? 0xFFFFFFFFu : 0u;
+        }
+        wave_lds_sync();
+        // buckets: the first two of each row reduced across the wave, the rest per lane
+        for (uint32_t p0 = 0; p0 < m; p0 += kWave) {
+            const uint32_t p = p0 + lane;
             const bool valid = p < m;
             Rec r{};
             uint32_t k = 0;
             if (valid) {
                 r = R[begin + p];
                 k = bucket_of(sel3(box_centroid(rec_box(r)), axis), base, span);
-                s_bkt[p] = uint8_t(k);
+                L.bkt[p] = uint8_t(k);
             }
             uint64_t todo = __ballot(valid);
             for (int round = 0; round < 2 && todo; ++round) {
@@ -1194,126 +1179,99 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A, uint3
                     bx[d] = wave_min_u32_all(in ? f2ord(r.lo[d]) : 0xFFFFFFFFu);
                     bx[3 + d] = wave_max_u32_all(in ? f2ord(r.hi[d]) : 0u);
                 }
-                if (int(__lane_id()) == lead) {
-                    atomicAdd(&s_cnt[kb], uint32_t(__popcll(peers)));
-                    atomicAdd(&s_pr[kb], pr);
+                if (int(lane) == lead) {
+                    L.cnt[kb] += uint32_t(__popcll(peers));
+                    L.pr[kb] += pr;
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        atomicMin(&s_box[kb][d], bx[d]);
-                        atomicMax(&s_box[kb][3 + d], bx[3 + d]);
+                        L.box[kb][d] = min(L.box[kb][d], bx[d]);
+                        L.box[kb][3 + d] = max(L.box[kb][3 + d], bx[3 + d]);
                     }
                 }
+                wave_lds_sync();
             }
-            if ((todo >> __lane_id()) & 1u) {
-                atomicAdd(&s_cnt[k], 1u);
-                atomicAdd(&s_pr[k], r.b);
+            if ((todo >> lane) & 1u) {
+                atomicAdd(&L.cnt[k], 1u);
+                atomicAdd(&L.pr[k], r.b);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    atomicMin(&s_box[k][d], f2ord(r.lo[d]));
-                    atomicMax(&s_box[k][3 + d], f2ord(r.hi[d]));
+                    atomicMin(&L.box[k][d], f2ord(r.lo[d]));
+                    atomicMax(&L.box[k][3 + d], f2ord(r.hi[d]));
                 }
             }
+            wave_lds_sync();
         }
-        __syncthreads();
         // the price of the 23 splits (lane k = bucket k), as top_decide_kernel
-        if (threadIdx.x < kWave) {
-            const int lane = threadIdx.x;
-            const bool in = lane < int(kNumBuckets);
-            const uint32_t cnt = in ? s_pr[lane] : 0u, tcnt = in ? s_cnt[lane] : 0u;
-            // an untouched bucket holds the atomic identities: use the empty box
-            Box b = box_empty();
-            if (in && tcnt > 0)
-                b = {{ord2f(s_box[lane][0]), ord2f(s_box[lane][1]), ord2f(s_box[lane][2])},
-                     {ord2f(s_box[lane][3]), ord2f(s_box[lane][4]), ord2f(s_box[lane][5])}};
-            float pl[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
-            float sf[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
-            uint32_t pc = cnt, sc = cnt, tc = tcnt;
+        const bool in = lane < kNumBuckets;
+        const uint32_t cnt = in ? L.pr[lane] : 0u, tcnt = in ? L.cnt[lane] : 0u;
+        // an untouched bucket holds the identities: use the empty box
+        Box b = box_empty();
+        if (in && tcnt > 0)
+            b = {{ord2f(L.box[lane][0]), ord2f(L.box[lane][1]), ord2f(L.box[lane][2])},
+                 {ord2f(L.box[lane][3]), ord2f(L.box[lane][4]), ord2f(L.box[lane][5])}};
+        float pl[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+        float sf[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+        uint32_t pc = cnt, sc = cnt, tc = tcnt;
 #pragma unroll
-            for (int d = 1; d < 32; d <<= 1) {
-                const bool up = lane >= d;
-                const bool dn = lane + d < int(kNumBuckets);
+        for (int d = 1; d < 32; d <<= 1) {
+            const bool up = int(lane) >= d;
+            const bool dn = int(lane) + d < int(kNumBuckets);
 #pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                    const float u = __shfl_up(pl[k], d, kWave);
-                    const float w = __shfl_down(sf[k], d, kWave);
-                    if (up) pl[k] = k < 3 ? smin(u, pl[k]) : smax(u, pl[k]);
-                    if (dn) sf[k] = k < 3 ? smin(sf[k], w) : smax(sf[k], w);
-                }
-                const uint32_t uc = __shfl_up(pc, d, kWave), wc = __shfl_down(sc, d, kWave);
-                const uint32_t ut = __shfl_up(tc, d, kWave);
-                if (up) pc += uc, tc += ut;
-                if (dn) sc += wc;
+            for (int k = 0; k < 6; ++k) {
+                const float u = __shfl_up(pl[k], d, kWave);
+                const float w = __shfl_down(sf[k], d, kWave);
+                if (up) pl[k] = k < 3 ? smin(u, pl[k]) : smax(u, pl[k]);
+                if (dn) sf[k] = k < 3 ? smin(sf[k], w) : smax(sf[k], w);
             }
-            float r6[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) r6[k] = __shfl_down(sf[k], 1, kWave);
-            const uint32_t rc = __shfl_down(sc, 1, kWave);
-            const float total_sa = box_area(fb);
-            uint32_t key = 0xFFFFFFFFu;
-            if (lane < int(kNumBuckets) - 1) {
-                const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
-                const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
-                const float cost = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
-                key = f2ord(cost);
-            }
-            const uint32_t kmin = wave_min_u32(key);
-            const uint32_t split = uint32_t(__builtin_ctzll(__ballot(lane < int(kNumBuckets) - 1 && key == kmin)));
-            const uint32_t lp = __shfl(pc, int(split), kWave), rp = __shfl(rc, int(split), kWave);
-            const uint32_t lt = __shfl(tc, int(split), kWave);
-            if (lane == 0) {
-                // both end buckets hold a treelet, so neither side is empty (as top_decide_kernel)
-                s_dec[0] = (lp == 0u || rp == 0u) ? kTopInvalid : split;
-                s_dec[1] = lp;
-                s_dec[2] = lt;
-            }
+            const uint32_t uc = __shfl_up(pc, d, kWave), wc = __shfl_down(sc, d, kWave);
+            const uint32_t ut = __shfl_up(tc, d, kWave);
+            if (up) pc += uc, tc += ut;
+            if (dn) sc += wc;
         }
-        __syncthreads();
-        const uint32_t split = s_dec[0], lp = s_dec[1], L = s_dec[2];
-        if (split == kTopInvalid || L == 0u || L == m) {
-            if (threadIdx.x == 0) atomicOr(A.err, uint32_t(kErrSplit));
-            __syncthreads();
+        float r6[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) r6[k] = __shfl_down(sf[k], 1, kWave);
+        const uint32_t rc = __shfl_down(sc, 1, kWave);
+        const float total_sa = box_area(fb);
+        uint32_t key = 0xFFFFFFFFu;
+        if (lane < kNumBuckets - 1) {
+            const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
+            const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
+            const float cost = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
+            key = f2ord(cost);
+        }
+        const uint32_t kmin = wave_min_u32_all(key);
+        const uint32_t split = uint32_t(__builtin_ctzll(__ballot(lane < kNumBuckets - 1 && key == kmin)));
+        const uint32_t lp = __shfl(pc, int(split), kWave), rp = __shfl(rc, int(split), kWave);
+        const uint32_t L2 = __shfl(tc, int(split), kWave);  // left treelets
+        // both end buckets hold a treelet, so neither side is empty (as top_decide_kernel)
+        if (lp == 0u || rp == 0u || L2 == 0u || L2 == m) {
+            if (lane == 0) atomicOr(A.err, uint32_t(kErrSplit));
             continue;
         }
-        // stable partition into the other buffer, striped rows (position k * 256 + thread)
-        const int w = threadIdx.x / kWave;
-        uint32_t lbits = 0;
-#pragma unroll
-        for (int k = 0; k < kChunkItems; ++k) {
-            const uint32_t p = k * kChunkBlock + threadIdx.x;
-            const bool left = p < m && s_bkt[p] <= split;
-            lbits |= left ? (1u << k) : 0u;
+        // stable partition into the other buffer, one row of 64 positions at a time
+        uint32_t lefts = 0;  // left positions in the rows before
+        for (uint32_t p0 = 0; p0 < m; p0 += kWave) {
+            const uint32_t p = p0 + lane;
+            const bool left = p < m && L.bkt[p] <= split;
             const uint64_t bm = __ballot(left);
-            if (__lane_id() == 0) s_rows[k][w] = __popcll(bm);
-        }
-        __syncthreads();
-        uint32_t row_base = 0;
-#pragma unroll
-        for (int k = 0; k < kChunkItems; ++k) {
-            uint32_t row_total = 0, before = 0;
-#pragma unroll
-            for (int ww = 0; ww < kWaves; ++ww) {
-                const uint32_t c = s_rows[k][ww];
-                row_total += c;
-                before += ww < w ? c : 0u;
-            }
-            const bool left = (lbits >> k) & 1u;
-            const uint64_t bm = __ballot(left);
-            const uint32_t p = k * kChunkBlock + threadIdx.x;
-            const uint32_t lb = row_base + before + __popcll(bm & lanemask_lt());
             if (p < m) {
+                const uint32_t lb = lefts + __popcll(bm & lanemask_lt());
                 const Rec rec = R[begin + p];
                 if (left) W[begin + lb] = rec;
-                else W[begin + L + (p - lb)] = rec;
+                else W[begin + L2 + (p - lb)] = rec;
             }
-            row_base += row_total;
+            lefts += __popcll(bm);
         }
-        __syncthreads();  // the partitioned records are visible to thread 0 below
-        if (threadIdx.x == 0) {
+        // the children: lane 0 reads a single treelet back from W (written by this wave)
+        __threadfence_block();
+        if (lane == 0) {
             write_interior(A.nodes, node, fb, lp, uint32_t(axis));
             const uint32_t nb = 1 - buf;
-            const WorkItem cl{begin, begin + L, node + 1, lp, nb, {0, 0, 0}};
-            const WorkItem cr{begin + L, end, node + 2 * lp, prims - lp, nb, {0, 0, 0}};
-            const bool right_big = m - L > L;
+            const WorkItem cl{begin, begin + L2, node + 1, lp, nb, {0, 0, 0}};
+            const WorkItem cr{begin + L2, end, node + 2 * lp, prims - lp, nb, {0, 0, 0}};
+            const bool right_big = m - L2 > L2;
+            int top = sp;
 #pragma unroll
             for (int j = 0; j < 2; ++j) {  // the larger child waits below the smaller one
                 const WorkItem cw = (j == 0) == right_big ? cr : cl;
@@ -1322,15 +1280,16 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A, uint3
                     emit_bottom_root(A, A.trec[nb][cw.begin], cw.node);
                 } else if (cs <= kSmallNode) {
                     push_item(A.small_top, A.counters + kCntSmallTop, A.cap_small_top, cw, A.err);
-                } else if (s_sp < uint32_t(kMidStack)) {
-                    s_stack[s_sp] = make_uint4(cw.begin, cw.end, cw.node, cw.prims | (nb << 31));
-                    s_sp = s_sp + 1;
+                } else if (top < kMidStack) {
+                    L.stack[top++] = make_uint4(cw.begin, cw.end, cw.node, cw.prims | (nb << 31));
                 } else {
                     atomicOr(A.err, uint32_t(kErrCapacity));
                 }
             }
+            sp = top;
         }
-        __syncthreads();
+        sp = __shfl(sp, 0, kWave);
+        wave_lds_sync();
     }
 }
 
@@ -1546,7 +1505,9 @@ constexpr int kLaneBlock = 256;
 // offset); a treelet of c primitives keeps fewer than c of them.
 template <int MAXC>
 __global__ __launch_bounds__(kLaneBlock) __attribute__((amdgpu_waves_per_eu(MAXC <= 8 ? 3 : 2))) void lane_bottom_kernel(
-    BuildArgs A, const WorkItem *items, uint32_t nitems) {
+    BuildArgs A, const WorkItem *items, const uint32_t *list_count, uint32_t cap) {
+    const uint32_t nitems = min(*list_count, cap);  // the grid covers the list's capacity
+    if (blockIdx.x * kLaneBlock >= nitems) return;  // the whole workgroup
     // The workgroup's treelets ordered by size (a counting sort in LDS), so that
     // each wave's lanes take similar numbers of steps.
     __shared__ uint32_t s_bin[MAXC + 2];
@@ -2115,9 +2076,14 @@ static hipError_t run_top_levels(BuildArgs &A, BuildScratch &S, hipStream_t stre
     PBRTM_TRY(hipGetLastError());
     uint32_t known = 1, p = 0, enqueued = 0;
     const uint32_t item_cap = cap_items;
+    // the first batch covers the fewest levels that can bring every node down to
+    // kMidNode treelets (halving at best), so that few count reads follow
+    int batch = kLevelsPerSync;
+    for (uint64_t c = kMidNode; c < num_treelets; c <<= 1) ++batch;
+    batch = std::max(kLevelsPerSync, batch - 3);
     while (known > 0) {
         uint32_t nb = known;
-        for (int k = 0; k < kLevelsPerSync; ++k) {
+        for (int k = 0; k < batch; ++k) {
             if (++enqueued > 100000) return hipErrorUnknown;  // every level strictly shrinks its nodes
             const uint32_t tb = num_treelets / kChunk + nb + 1;
             top_bucket_kernel<<<tb, kChunkBlock, 0, stream>>>(TL, p, recs, counters, A.err);
@@ -2129,6 +2095,7 @@ static hipError_t run_top_levels(BuildArgs &A, BuildScratch &S, hipStream_t stre
             p = 1 - p;
             nb = std::min(2 * nb, item_cap);
         }
+        batch = kLevelsPerSync;
         PBRTM_TRY(read_counters(S, stream));
         known = S.host_counters[kCntTopItems0 + p];
     }
@@ -2243,7 +2210,7 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     PBRTM_TRY(read_counters(S, stream));
     const uint32_t nmid = std::min(S.host_counters[kCntMidTop], A.cap_mid_top);
     if (nmid > 0) {  // their subtrees down to nodes of <= 64 treelets
-        mid_top_kernel<<<nmid, kChunkBlock, 0, stream>>>(A, nmid);
+        mid_top_kernel<<<(nmid + kMidWaves - 1) / kMidWaves, kChunkBlock, 0, stream>>>(A, nmid);
         PBRTM_TRY(read_counters(S, stream));
     }
     out.small_top = S.host_counters[kCntSmallTop];
@@ -2251,18 +2218,16 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
         finish_top_kernel<<<(out.small_top + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
             A, A.small_top, out.small_top);
     if (times) hipEventRecord(times->ev[3], stream);
-    PBRTM_TRY(read_counters(S, stream));
-    for (int k = 0; k < 2; ++k) {
-        const uint32_t m = std::min(S.host_counters[kCntLaneBot0 + k], A.cap_lane_bot[k]);
-        if (m == 0) continue;
-        const uint32_t grid = (m + kLaneBlock - 1) / kLaneBlock;
-        if (k == 0) lane_bottom_kernel<kLaneSmall><<<grid, kLaneBlock, 0, stream>>>(A, A.lane_bot[0], m);
-        else lane_bottom_kernel<kLaneBuild><<<grid, kLaneBlock, 0, stream>>>(A, A.lane_bot[1], m);
-    }
+    // one-lane treelet subtrees: grids over the lists' capacities, counts read on the device
+    lane_bottom_kernel<kLaneSmall><<<(A.cap_lane_bot[0] + kLaneBlock - 1) / kLaneBlock, kLaneBlock, 0, stream>>>(
+        A, A.lane_bot[0], counters + kCntLaneBot0, A.cap_lane_bot[0]);
+    lane_bottom_kernel<kLaneBuild><<<(A.cap_lane_bot[1] + kLaneBlock - 1) / kLaneBlock, kLaneBlock, 0, stream>>>(
+        A, A.lane_bot[1], counters + kCntLaneBot1, A.cap_lane_bot[1]);
+    // run_levels ends on a read of the counters after its last level: the small
+    // bottom list is complete in it
     PBRTM_TRY(run_levels<false>(A, S, stream, precs, kCntLargeBot0, kCntLargeBot1, A.large_bot[0],
                                 A.large_bot[1], n, A.cap_large_bot, &out.bottom_levels));
-    PBRTM_TRY(read_counters(S, stream));
-    out.small_bottom = S.host_counters[kCntSmallBot];
+    out.small_bottom = std::min(S.host_counters[kCntSmallBot], A.cap_small_bot);
     if (out.small_bottom > 0)
         finish_bottom_kernel<<<(out.small_bottom + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
             A, A.small_bot, out.small_bottom);

@@ -738,23 +738,27 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
         cus = 256;
     }
     const uint32_t resident = uint32_t(per_cu) * uint32_t(cus);
-    // levels are enqueued four at a time between reads of the job count; a level
-    // past the last one finds no jobs and exits at once
+    // levels are enqueued in batches between reads of the job count (the first
+    // batch: about the depth of a full 8-wide tree over n leaves, plus two); a
+    // level past the last one finds no jobs and exits at once
     uint32_t host[5];
     uint32_t enqueued = 0;
     int cur = 0;
+    int batch = 2;
+    for (uint64_t c = 1; c < n; c <<= 3) ++batch;
     for (;;) {
-        e = hipMemcpyAsync(host, counters, sizeof(host), hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) return e;
-        if (host[0] == 0) break;
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < batch; ++k) {
             if (++enqueued > 100000) return hipErrorUnknown;
             wide_collapse_kernel<<<resident, kCollapseBlock, 0, stream>>>(bnodes, jobs[cur], jobs[1 - cur], counters,
                                                                           wn, order, cap, err);
             collapse_advance_kernel<<<1, 64, 0, stream>>>(counters);
             cur = 1 - cur;
         }
+        batch = 4;
+        e = hipMemcpyAsync(host, counters, sizeof(host), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return e;
+        if (host[0] == 0) break;
     }
     out.collapse_levels = host[4];
     out.num_nodes = host[1];
