@@ -1106,190 +1106,193 @@ struct MidLds {
     uint8_t bkt[kMidNode];  // bucket of each position of the node
 };
 
-__global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A, uint32_t nitems) {
+// Resident grid: each wave takes nodes wave-stride from the list, whose length
+// it reads on the device.
+__global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
     __shared__ MidLds s_lds[kMidWaves];
-    const uint32_t wid = blockIdx.x * kMidWaves + threadIdx.x / kWave;
-    if (wid >= nitems) return;  // the whole wave
     MidLds &L = s_lds[threadIdx.x / kWave];
     const uint32_t lane = __lane_id();
-    {
-        const WorkItem it = A.mid_top[wid];
-        if (lane == 0) L.stack[0] = make_uint4(it.begin, it.end, it.node, it.prims | (it.buf << 31));
-    }
-    int sp = 1;  // uniform
-    wave_lds_sync();
-    while (sp > 0) {
-        --sp;
-        const uint4 nd = L.stack[sp];
-        const uint32_t begin = nd.x, end = nd.y, node = nd.z, prims = nd.w & 0x7FFFFFFFu, buf = nd.w >> 31;
-        const uint32_t m = end - begin;
-        const Rec *R = pick(A.trec, buf);
-        Rec *W = pick(A.trec, 1 - buf);
-        // bounds
-        uint32_t v[12];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
-        for (uint32_t p = lane; p < m; p += kWave) {
-            const Box b = rec_box(R[begin + p]);
-            const f3 c = box_centroid(b);
-            v[0] = min(v[0], f2ord(c.x)); v[1] = min(v[1], f2ord(c.y)); v[2] = min(v[2], f2ord(c.z));
-            v[3] = max(v[3], f2ord(c.x)); v[4] = max(v[4], f2ord(c.y)); v[5] = max(v[5], f2ord(c.z));
-            v[6] = min(v[6], f2ord(b.lo.x)); v[7] = min(v[7], f2ord(b.lo.y)); v[8] = min(v[8], f2ord(b.lo.z));
-            v[9] = max(v[9], f2ord(b.hi.x)); v[10] = max(v[10], f2ord(b.hi.y)); v[11] = max(v[11], f2ord(b.hi.z));
+    const uint32_t nitems = min(A.counters[kCntMidTop], A.cap_mid_top);
+    for (uint32_t wid = blockIdx.x * kMidWaves + threadIdx.x / kWave; wid < nitems; wid += gridDim.x * kMidWaves) {
+        {
+            const WorkItem it = A.mid_top[wid];
+            if (lane == 0) L.stack[0] = make_uint4(it.begin, it.end, it.node, it.prims | (it.buf << 31));
         }
-#pragma unroll
-        for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? wave_min_u32_all(v[k]) : wave_max_u32_all(v[k]);
-        const Box cb = acc_box(v), fb = acc_box(v + 6);
-        const int axis = box_max_dim(cb);
-        if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) {  // the node stays unbuilt
-            if (lane == 0) atomicOr(A.err, uint32_t(kErrDegenerateCentroids));
-            continue;
-        }
-        const float base = sel3(cb.lo, axis);
-        const float span = sel3(cb.hi, axis) - sel3(cb.lo, axis);
-        if (lane < kNumBuckets) {
-            L.cnt[lane] = 0u;
-            L.pr[lane] = 0u;
-#pragma unroll
-            for (int j = 0; j < 6; ++j) L.box[lane][j] = j < 3 ? 0xFFFFFFFFu : 0u;
-        }
+        int sp = 1;  // uniform
         wave_lds_sync();
-        // buckets: the first two of each row reduced across the wave, the rest per lane
-        for (uint32_t p0 = 0; p0 < m; p0 += kWave) {
-            const uint32_t p = p0 + lane;
-            const bool valid = p < m;
-            Rec r{};
-            uint32_t k = 0;
-            if (valid) {
-                r = R[begin + p];
-                k = bucket_of(sel3(box_centroid(rec_box(r)), axis), base, span);
-                L.bkt[p] = uint8_t(k);
+        while (sp > 0) {
+            --sp;
+            const uint4 nd = L.stack[sp];
+            const uint32_t begin = nd.x, end = nd.y, node = nd.z, prims = nd.w & 0x7FFFFFFFu, buf = nd.w >> 31;
+            const uint32_t m = end - begin;
+            const Rec *R = pick(A.trec, buf);
+            Rec *W = pick(A.trec, 1 - buf);
+            // bounds
+            uint32_t v[12];
+    #pragma unroll
+            for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
+            for (uint32_t p = lane; p < m; p += kWave) {
+                const Box b = rec_box(R[begin + p]);
+                const f3 c = box_centroid(b);
+                v[0] = min(v[0], f2ord(c.x)); v[1] = min(v[1], f2ord(c.y)); v[2] = min(v[2], f2ord(c.z));
+                v[3] = max(v[3], f2ord(c.x)); v[4] = max(v[4], f2ord(c.y)); v[5] = max(v[5], f2ord(c.z));
+                v[6] = min(v[6], f2ord(b.lo.x)); v[7] = min(v[7], f2ord(b.lo.y)); v[8] = min(v[8], f2ord(b.lo.z));
+                v[9] = max(v[9], f2ord(b.hi.x)); v[10] = max(v[10], f2ord(b.hi.y)); v[11] = max(v[11], f2ord(b.hi.z));
             }
-            uint64_t todo = __ballot(valid);
-            for (int round = 0; round < 2 && todo; ++round) {
-                const int lead = __builtin_ctzll(todo);
-                const uint32_t kb = __shfl(k, lead, kWave);
-                const bool in = valid && k == kb;
-                const uint64_t peers = __ballot(in);
-                todo &= ~peers;
-                const uint32_t pr = wave_sum_all(in ? r.b : 0u);
-                uint32_t bx[6];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    bx[d] = wave_min_u32_all(in ? f2ord(r.lo[d]) : 0xFFFFFFFFu);
-                    bx[3 + d] = wave_max_u32_all(in ? f2ord(r.hi[d]) : 0u);
+    #pragma unroll
+            for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? wave_min_u32_all(v[k]) : wave_max_u32_all(v[k]);
+            const Box cb = acc_box(v), fb = acc_box(v + 6);
+            const int axis = box_max_dim(cb);
+            if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) {  // the node stays unbuilt
+                if (lane == 0) atomicOr(A.err, uint32_t(kErrDegenerateCentroids));
+                continue;
+            }
+            const float base = sel3(cb.lo, axis);
+            const float span = sel3(cb.hi, axis) - sel3(cb.lo, axis);
+            if (lane < kNumBuckets) {
+                L.cnt[lane] = 0u;
+                L.pr[lane] = 0u;
+    #pragma unroll
+                for (int j = 0; j < 6; ++j) L.box[lane][j] = j < 3 ? 0xFFFFFFFFu : 0u;
+            }
+            wave_lds_sync();
+            // buckets: the first two of each row reduced across the wave, the rest per lane
+            for (uint32_t p0 = 0; p0 < m; p0 += kWave) {
+                const uint32_t p = p0 + lane;
+                const bool valid = p < m;
+                Rec r{};
+                uint32_t k = 0;
+                if (valid) {
+                    r = R[begin + p];
+                    k = bucket_of(sel3(box_centroid(rec_box(r)), axis), base, span);
+                    L.bkt[p] = uint8_t(k);
                 }
-                if (int(lane) == lead) {
-                    L.cnt[kb] += uint32_t(__popcll(peers));
-                    L.pr[kb] += pr;
-#pragma unroll
+                uint64_t todo = __ballot(valid);
+                for (int round = 0; round < 2 && todo; ++round) {
+                    const int lead = __builtin_ctzll(todo);
+                    const uint32_t kb = __shfl(k, lead, kWave);
+                    const bool in = valid && k == kb;
+                    const uint64_t peers = __ballot(in);
+                    todo &= ~peers;
+                    const uint32_t pr = wave_sum_all(in ? r.b : 0u);
+                    uint32_t bx[6];
+    #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        L.box[kb][d] = min(L.box[kb][d], bx[d]);
-                        L.box[kb][3 + d] = max(L.box[kb][3 + d], bx[3 + d]);
+                        bx[d] = wave_min_u32_all(in ? f2ord(r.lo[d]) : 0xFFFFFFFFu);
+                        bx[3 + d] = wave_max_u32_all(in ? f2ord(r.hi[d]) : 0u);
+                    }
+                    if (int(lane) == lead) {
+                        L.cnt[kb] += uint32_t(__popcll(peers));
+                        L.pr[kb] += pr;
+    #pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            L.box[kb][d] = min(L.box[kb][d], bx[d]);
+                            L.box[kb][3 + d] = max(L.box[kb][3 + d], bx[3 + d]);
+                        }
+                    }
+                    wave_lds_sync();
+                }
+                if ((todo >> lane) & 1u) {
+                    atomicAdd(&L.cnt[k], 1u);
+                    atomicAdd(&L.pr[k], r.b);
+    #pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        atomicMin(&L.box[k][d], f2ord(r.lo[d]));
+                        atomicMax(&L.box[k][3 + d], f2ord(r.hi[d]));
                     }
                 }
                 wave_lds_sync();
             }
-            if ((todo >> lane) & 1u) {
-                atomicAdd(&L.cnt[k], 1u);
-                atomicAdd(&L.pr[k], r.b);
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    atomicMin(&L.box[k][d], f2ord(r.lo[d]));
-                    atomicMax(&L.box[k][3 + d], f2ord(r.hi[d]));
+            // the price of the 23 splits (lane k = bucket k), as top_decide_kernel
+            const bool in = lane < kNumBuckets;
+            const uint32_t cnt = in ? L.pr[lane] : 0u, tcnt = in ? L.cnt[lane] : 0u;
+            // an untouched bucket holds the identities: use the empty box
+            Box b = box_empty();
+            if (in && tcnt > 0)
+                b = {{ord2f(L.box[lane][0]), ord2f(L.box[lane][1]), ord2f(L.box[lane][2])},
+                     {ord2f(L.box[lane][3]), ord2f(L.box[lane][4]), ord2f(L.box[lane][5])}};
+            float pl[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+            float sf[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+            uint32_t pc = cnt, sc = cnt, tc = tcnt;
+    #pragma unroll
+            for (int d = 1; d < 32; d <<= 1) {
+                const bool up = int(lane) >= d;
+                const bool dn = int(lane) + d < int(kNumBuckets);
+    #pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    const float u = __shfl_up(pl[k], d, kWave);
+                    const float w = __shfl_down(sf[k], d, kWave);
+                    if (up) pl[k] = k < 3 ? smin(u, pl[k]) : smax(u, pl[k]);
+                    if (dn) sf[k] = k < 3 ? smin(sf[k], w) : smax(sf[k], w);
                 }
+                const uint32_t uc = __shfl_up(pc, d, kWave), wc = __shfl_down(sc, d, kWave);
+                const uint32_t ut = __shfl_up(tc, d, kWave);
+                if (up) pc += uc, tc += ut;
+                if (dn) sc += wc;
             }
+            float r6[6];
+    #pragma unroll
+            for (int k = 0; k < 6; ++k) r6[k] = __shfl_down(sf[k], 1, kWave);
+            const uint32_t rc = __shfl_down(sc, 1, kWave);
+            const float total_sa = box_area(fb);
+            uint32_t key = 0xFFFFFFFFu;
+            if (lane < kNumBuckets - 1) {
+                const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
+                const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
+                const float cost = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
+                key = f2ord(cost);
+            }
+            const uint32_t kmin = wave_min_u32_all(key);
+            const uint32_t split = uint32_t(__builtin_ctzll(__ballot(lane < kNumBuckets - 1 && key == kmin)));
+            const uint32_t lp = __shfl(pc, int(split), kWave), rp = __shfl(rc, int(split), kWave);
+            const uint32_t L2 = __shfl(tc, int(split), kWave);  // left treelets
+            // both end buckets hold a treelet, so neither side is empty (as top_decide_kernel)
+            if (lp == 0u || rp == 0u || L2 == 0u || L2 == m) {
+                if (lane == 0) atomicOr(A.err, uint32_t(kErrSplit));
+                continue;
+            }
+            // stable partition into the other buffer, one row of 64 positions at a time
+            uint32_t lefts = 0;  // left positions in the rows before
+            for (uint32_t p0 = 0; p0 < m; p0 += kWave) {
+                const uint32_t p = p0 + lane;
+                const bool left = p < m && L.bkt[p] <= split;
+                const uint64_t bm = __ballot(left);
+                if (p < m) {
+                    const uint32_t lb = lefts + __popcll(bm & lanemask_lt());
+                    const Rec rec = R[begin + p];
+                    if (left) W[begin + lb] = rec;
+                    else W[begin + L2 + (p - lb)] = rec;
+                }
+                lefts += __popcll(bm);
+            }
+            // the children: lane 0 reads a single treelet back from W (written by this wave)
+            __threadfence_block();
+            if (lane == 0) {
+                write_interior(A.nodes, node, fb, lp, uint32_t(axis));
+                const uint32_t nb = 1 - buf;
+                const WorkItem cl{begin, begin + L2, node + 1, lp, nb, {0, 0, 0}};
+                const WorkItem cr{begin + L2, end, node + 2 * lp, prims - lp, nb, {0, 0, 0}};
+                const bool right_big = m - L2 > L2;
+                int top = sp;
+    #pragma unroll
+                for (int j = 0; j < 2; ++j) {  // the larger child waits below the smaller one
+                    const WorkItem cw = (j == 0) == right_big ? cr : cl;
+                    const uint32_t cs = cw.end - cw.begin;
+                    if (cs == 1) {
+                        emit_bottom_root(A, A.trec[nb][cw.begin], cw.node);
+                    } else if (cs <= kSmallNode) {
+                        push_item(A.small_top, A.counters + kCntSmallTop, A.cap_small_top, cw, A.err);
+                    } else if (top < kMidStack) {
+                        L.stack[top++] = make_uint4(cw.begin, cw.end, cw.node, cw.prims | (nb << 31));
+                    } else {
+                        atomicOr(A.err, uint32_t(kErrCapacity));
+                    }
+                }
+                sp = top;
+            }
+            sp = __shfl(sp, 0, kWave);
             wave_lds_sync();
         }
-        // the price of the 23 splits (lane k = bucket k), as top_decide_kernel
-        const bool in = lane < kNumBuckets;
-        const uint32_t cnt = in ? L.pr[lane] : 0u, tcnt = in ? L.cnt[lane] : 0u;
-        // an untouched bucket holds the identities: use the empty box
-        Box b = box_empty();
-        if (in && tcnt > 0)
-            b = {{ord2f(L.box[lane][0]), ord2f(L.box[lane][1]), ord2f(L.box[lane][2])},
-                 {ord2f(L.box[lane][3]), ord2f(L.box[lane][4]), ord2f(L.box[lane][5])}};
-        float pl[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
-        float sf[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
-        uint32_t pc = cnt, sc = cnt, tc = tcnt;
-#pragma unroll
-        for (int d = 1; d < 32; d <<= 1) {
-            const bool up = int(lane) >= d;
-            const bool dn = int(lane) + d < int(kNumBuckets);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const float u = __shfl_up(pl[k], d, kWave);
-                const float w = __shfl_down(sf[k], d, kWave);
-                if (up) pl[k] = k < 3 ? smin(u, pl[k]) : smax(u, pl[k]);
-                if (dn) sf[k] = k < 3 ? smin(sf[k], w) : smax(sf[k], w);
-            }
-            const uint32_t uc = __shfl_up(pc, d, kWave), wc = __shfl_down(sc, d, kWave);
-            const uint32_t ut = __shfl_up(tc, d, kWave);
-            if (up) pc += uc, tc += ut;
-            if (dn) sc += wc;
-        }
-        float r6[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) r6[k] = __shfl_down(sf[k], 1, kWave);
-        const uint32_t rc = __shfl_down(sc, 1, kWave);
-        const float total_sa = box_area(fb);
-        uint32_t key = 0xFFFFFFFFu;
-        if (lane < kNumBuckets - 1) {
-            const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
-            const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
-            const float cost = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
-            key = f2ord(cost);
-        }
-        const uint32_t kmin = wave_min_u32_all(key);
-        const uint32_t split = uint32_t(__builtin_ctzll(__ballot(lane < kNumBuckets - 1 && key == kmin)));
-        const uint32_t lp = __shfl(pc, int(split), kWave), rp = __shfl(rc, int(split), kWave);
-        const uint32_t L2 = __shfl(tc, int(split), kWave);  // left treelets
-        // both end buckets hold a treelet, so neither side is empty (as top_decide_kernel)
-        if (lp == 0u || rp == 0u || L2 == 0u || L2 == m) {
-            if (lane == 0) atomicOr(A.err, uint32_t(kErrSplit));
-            continue;
-        }
-        // stable partition into the other buffer, one row of 64 positions at a time
-        uint32_t lefts = 0;  // left positions in the rows before
-        for (uint32_t p0 = 0; p0 < m; p0 += kWave) {
-            const uint32_t p = p0 + lane;
-            const bool left = p < m && L.bkt[p] <= split;
-            const uint64_t bm = __ballot(left);
-            if (p < m) {
-                const uint32_t lb = lefts + __popcll(bm & lanemask_lt());
-                const Rec rec = R[begin + p];
-                if (left) W[begin + lb] = rec;
-                else W[begin + L2 + (p - lb)] = rec;
-            }
-            lefts += __popcll(bm);
-        }
-        // the children: lane 0 reads a single treelet back from W (written by this wave)
-        __threadfence_block();
-        if (lane == 0) {
-            write_interior(A.nodes, node, fb, lp, uint32_t(axis));
-            const uint32_t nb = 1 - buf;
-            const WorkItem cl{begin, begin + L2, node + 1, lp, nb, {0, 0, 0}};
-            const WorkItem cr{begin + L2, end, node + 2 * lp, prims - lp, nb, {0, 0, 0}};
-            const bool right_big = m - L2 > L2;
-            int top = sp;
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {  // the larger child waits below the smaller one
-                const WorkItem cw = (j == 0) == right_big ? cr : cl;
-                const uint32_t cs = cw.end - cw.begin;
-                if (cs == 1) {
-                    emit_bottom_root(A, A.trec[nb][cw.begin], cw.node);
-                } else if (cs <= kSmallNode) {
-                    push_item(A.small_top, A.counters + kCntSmallTop, A.cap_small_top, cw, A.err);
-                } else if (top < kMidStack) {
-                    L.stack[top++] = make_uint4(cw.begin, cw.end, cw.node, cw.prims | (nb << 31));
-                } else {
-                    atomicOr(A.err, uint32_t(kErrCapacity));
-                }
-            }
-            sp = top;
-        }
-        sp = __shfl(sp, 0, kWave);
-        wave_lds_sync();
     }
 }
 
@@ -1931,9 +1934,9 @@ __global__ __launch_bounds__(kFinBlock) void finish_top_kernel(BuildArgs A, cons
     const bool real = lane < nleaf && my_count >= 2;
     wave_push(real && my_count <= kLaneSmall, tw, A.lane_bot[0], A.counters + kCntLaneBot0, A.cap_lane_bot[0], A.err);
     wave_push(real && my_count > kLaneSmall && my_count <= kLaneBuild, tw, A.lane_bot[1], A.counters + kCntLaneBot1,
-              A.cap_lane_bot[1], A.err);
+          A.cap_lane_bot[1], A.err);
     wave_push(real && my_count > kLaneBuild && my_count <= kSmallNode, tw, A.small_bot, A.counters + kCntSmallBot,
-              A.cap_small_bot, A.err);
+          A.cap_small_bot, A.err);
 }
 
 // Bottom-level finisher for nodes of <= 64 primitives left by the level path.
@@ -2207,13 +2210,23 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     } else {
         root_kernel<<<1, 64, 0, stream>>>(A, D, n);
     }
-    PBRTM_TRY(read_counters(S, stream));
-    const uint32_t nmid = std::min(S.host_counters[kCntMidTop], A.cap_mid_top);
-    if (nmid > 0) {  // their subtrees down to nodes of <= 64 treelets
-        mid_top_kernel<<<(nmid + kMidWaves - 1) / kMidWaves, kChunkBlock, 0, stream>>>(A, nmid);
-        PBRTM_TRY(read_counters(S, stream));
+    // the nodes of 65..2048 treelets: a resident grid, the list's length read on
+    // the device (bounded by what the list can hold)
+    static int mid_grid = 0;
+    if (!mid_grid) {
+        int per_cu = 0, cus = 0, dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        mid_grid = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mid_top_kernel, kChunkBlock, 0) ==
+                           hipSuccess && per_cu > 0 ? per_cu * cus : 4 * cus;
     }
-    out.small_top = S.host_counters[kCntSmallTop];
+    if (D > kSmallNode)
+        mid_top_kernel<<<std::min<uint32_t>(mid_grid, (A.cap_mid_top + kMidWaves - 1) / kMidWaves), kChunkBlock, 0,
+                         stream>>>(A);
+    // then the nodes of <= 64, a wave each (an exact grid lets the hardware balance their uneven work)
+    PBRTM_TRY(read_counters(S, stream));
+    out.small_top = std::min(S.host_counters[kCntSmallTop], A.cap_small_top);
     if (out.small_top > 0)
         finish_top_kernel<<<(out.small_top + kFinWaves - 1) / kFinWaves, kFinBlock, 0, stream>>>(
             A, A.small_top, out.small_top);
