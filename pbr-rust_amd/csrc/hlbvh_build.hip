@@ -1102,21 +1102,42 @@ constexpr int kMidWaves = kChunkBlock / kWave;
 
 struct MidLds {
     uint4 stack[kMidStack];  // (begin, end, node id, prims | buffer << 31)
+    uint32_t sbox[kMidStack][12];  // their centroid and full boxes (order-mapped)
     uint32_t cnt[kNumBuckets], pr[kNumBuckets], box[kNumBuckets][6];
     uint8_t bkt[kMidNode];  // bucket of each position of the node
 };
-
 // Resident grid: each wave takes nodes wave-stride from the list, whose length
 // it reads on the device.
+// Order-mapped min (k % 6 < 3) / max of a record's centroid and box into v.
+__device__ __forceinline__ void add_rec_bounds(uint32_t v[12], const Rec &r) {
+    const Box b = rec_box(r);
+    const f3 c = box_centroid(b);
+    v[0] = min(v[0], f2ord(c.x)); v[1] = min(v[1], f2ord(c.y)); v[2] = min(v[2], f2ord(c.z));
+    v[3] = max(v[3], f2ord(c.x)); v[4] = max(v[4], f2ord(c.y)); v[5] = max(v[5], f2ord(c.z));
+    v[6] = min(v[6], f2ord(b.lo.x)); v[7] = min(v[7], f2ord(b.lo.y)); v[8] = min(v[8], f2ord(b.lo.z));
+    v[9] = max(v[9], f2ord(b.hi.x)); v[10] = max(v[10], f2ord(b.hi.y)); v[11] = max(v[11], f2ord(b.hi.z));
+}
+
 __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
     __shared__ MidLds s_lds[kMidWaves];
     MidLds &L = s_lds[threadIdx.x / kWave];
     const uint32_t lane = __lane_id();
     const uint32_t nitems = min(A.counters[kCntMidTop], A.cap_mid_top);
     for (uint32_t wid = blockIdx.x * kMidWaves + threadIdx.x / kWave; wid < nitems; wid += gridDim.x * kMidWaves) {
-        {
+        {  // the list's node and its bounds; its descendants get theirs from the partition
             const WorkItem it = A.mid_top[wid];
-            if (lane == 0) L.stack[0] = make_uint4(it.begin, it.end, it.node, it.prims | (it.buf << 31));
+            const Rec *R = pick(A.trec, it.buf);
+            uint32_t v[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
+            for (uint32_t p = it.begin + lane; p < it.end; p += kWave) add_rec_bounds(v, R[p]);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? wave_min_u32_all(v[k]) : wave_max_u32_all(v[k]);
+            if (lane == 0) {
+                L.stack[0] = make_uint4(it.begin, it.end, it.node, it.prims | (it.buf << 31));
+#pragma unroll
+                for (int k = 0; k < 12; ++k) L.sbox[0][k] = v[k];
+            }
         }
         int sp = 1;  // uniform
         wave_lds_sync();
@@ -1127,21 +1148,7 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
             const uint32_t m = end - begin;
             const Rec *R = pick(A.trec, buf);
             Rec *W = pick(A.trec, 1 - buf);
-            // bounds
-            uint32_t v[12];
-    #pragma unroll
-            for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
-            for (uint32_t p = lane; p < m; p += kWave) {
-                const Box b = rec_box(R[begin + p]);
-                const f3 c = box_centroid(b);
-                v[0] = min(v[0], f2ord(c.x)); v[1] = min(v[1], f2ord(c.y)); v[2] = min(v[2], f2ord(c.z));
-                v[3] = max(v[3], f2ord(c.x)); v[4] = max(v[4], f2ord(c.y)); v[5] = max(v[5], f2ord(c.z));
-                v[6] = min(v[6], f2ord(b.lo.x)); v[7] = min(v[7], f2ord(b.lo.y)); v[8] = min(v[8], f2ord(b.lo.z));
-                v[9] = max(v[9], f2ord(b.hi.x)); v[10] = max(v[10], f2ord(b.hi.y)); v[11] = max(v[11], f2ord(b.hi.z));
-            }
-    #pragma unroll
-            for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? wave_min_u32_all(v[k]) : wave_max_u32_all(v[k]);
-            const Box cb = acc_box(v), fb = acc_box(v + 6);
+            const Box cb = acc_box(L.sbox[sp]), fb = acc_box(L.sbox[sp] + 6);
             const int axis = box_max_dim(cb);
             if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) {  // the node stays unbuilt
                 if (lane == 0) atomicOr(A.err, uint32_t(kErrDegenerateCentroids));
@@ -1152,18 +1159,22 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
             if (lane < kNumBuckets) {
                 L.cnt[lane] = 0u;
                 L.pr[lane] = 0u;
-    #pragma unroll
+#pragma unroll
                 for (int j = 0; j < 6; ++j) L.box[lane][j] = j < 3 ? 0xFFFFFFFFu : 0u;
             }
             wave_lds_sync();
-            // buckets: the first two of each row reduced across the wave, the rest per lane
+            // buckets, one row of 64 positions at a time (the next row's records
+            // loaded ahead): the first two buckets of a row reduced across the wave,
+            // the rest per lane
+            Rec rn{};
+            if (lane < m) rn = R[begin + lane];
             for (uint32_t p0 = 0; p0 < m; p0 += kWave) {
                 const uint32_t p = p0 + lane;
                 const bool valid = p < m;
-                Rec r{};
+                const Rec r = rn;
+                if (p + kWave < m) rn = R[begin + p + kWave];
                 uint32_t k = 0;
                 if (valid) {
-                    r = R[begin + p];
                     k = bucket_of(sel3(box_centroid(rec_box(r)), axis), base, span);
                     L.bkt[p] = uint8_t(k);
                 }
@@ -1176,7 +1187,7 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
                     todo &= ~peers;
                     const uint32_t pr = wave_sum_all(in ? r.b : 0u);
                     uint32_t bx[6];
-    #pragma unroll
+#pragma unroll
                     for (int d = 0; d < 3; ++d) {
                         bx[d] = wave_min_u32_all(in ? f2ord(r.lo[d]) : 0xFFFFFFFFu);
                         bx[3 + d] = wave_max_u32_all(in ? f2ord(r.hi[d]) : 0u);
@@ -1184,7 +1195,7 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
                     if (int(lane) == lead) {
                         L.cnt[kb] += uint32_t(__popcll(peers));
                         L.pr[kb] += pr;
-    #pragma unroll
+#pragma unroll
                         for (int d = 0; d < 3; ++d) {
                             L.box[kb][d] = min(L.box[kb][d], bx[d]);
                             L.box[kb][3 + d] = max(L.box[kb][3 + d], bx[3 + d]);
@@ -1195,7 +1206,7 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
                 if ((todo >> lane) & 1u) {
                     atomicAdd(&L.cnt[k], 1u);
                     atomicAdd(&L.pr[k], r.b);
-    #pragma unroll
+#pragma unroll
                     for (int d = 0; d < 3; ++d) {
                         atomicMin(&L.box[k][d], f2ord(r.lo[d]));
                         atomicMax(&L.box[k][3 + d], f2ord(r.hi[d]));
@@ -1214,11 +1225,11 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
             float pl[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
             float sf[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
             uint32_t pc = cnt, sc = cnt, tc = tcnt;
-    #pragma unroll
+#pragma unroll
             for (int d = 1; d < 32; d <<= 1) {
                 const bool up = int(lane) >= d;
                 const bool dn = int(lane) + d < int(kNumBuckets);
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < 6; ++k) {
                     const float u = __shfl_up(pl[k], d, kWave);
                     const float w = __shfl_down(sf[k], d, kWave);
@@ -1231,7 +1242,7 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
                 if (dn) sc += wc;
             }
             float r6[6];
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < 6; ++k) r6[k] = __shfl_down(sf[k], 1, kWave);
             const uint32_t rc = __shfl_down(sc, 1, kWave);
             const float total_sa = box_area(fb);
@@ -1245,45 +1256,65 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
             const uint32_t kmin = wave_min_u32_all(key);
             const uint32_t split = uint32_t(__builtin_ctzll(__ballot(lane < kNumBuckets - 1 && key == kmin)));
             const uint32_t lp = __shfl(pc, int(split), kWave), rp = __shfl(rc, int(split), kWave);
-            const uint32_t L2 = __shfl(tc, int(split), kWave);  // left treelets
+            const uint32_t nl = __shfl(tc, int(split), kWave);  // left treelets
             // both end buckets hold a treelet, so neither side is empty (as top_decide_kernel)
-            if (lp == 0u || rp == 0u || L2 == 0u || L2 == m) {
+            if (lp == 0u || rp == 0u || nl == 0u || nl == m) {
                 if (lane == 0) atomicOr(A.err, uint32_t(kErrSplit));
                 continue;
             }
-            // stable partition into the other buffer, one row of 64 positions at a time
+            // stable partition into the other buffer, a row at a time (records loaded
+            // ahead), with the two children's bounds
+            uint32_t cv[2][12];
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int k = 0; k < 12; ++k) cv[c][k] = ((k % 6) < 3) ? 0xFFFFFFFFu : 0u;
             uint32_t lefts = 0;  // left positions in the rows before
+            if (lane < m) rn = R[begin + lane];
             for (uint32_t p0 = 0; p0 < m; p0 += kWave) {
                 const uint32_t p = p0 + lane;
+                const Rec rec = rn;
+                if (p + kWave < m) rn = R[begin + p + kWave];
                 const bool left = p < m && L.bkt[p] <= split;
                 const uint64_t bm = __ballot(left);
                 if (p < m) {
                     const uint32_t lb = lefts + __popcll(bm & lanemask_lt());
-                    const Rec rec = R[begin + p];
                     if (left) W[begin + lb] = rec;
-                    else W[begin + L2 + (p - lb)] = rec;
+                    else W[begin + nl + (p - lb)] = rec;
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+                        if ((c == 0) == left) add_rec_bounds(cv[c], rec);
                 }
                 lefts += __popcll(bm);
             }
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int k = 0; k < 12; ++k)
+                    cv[c][k] = ((k % 6) < 3) ? wave_min_u32_all(cv[c][k]) : wave_max_u32_all(cv[c][k]);
             // the children: lane 0 reads a single treelet back from W (written by this wave)
             __threadfence_block();
             if (lane == 0) {
                 write_interior(A.nodes, node, fb, lp, uint32_t(axis));
                 const uint32_t nb = 1 - buf;
-                const WorkItem cl{begin, begin + L2, node + 1, lp, nb, {0, 0, 0}};
-                const WorkItem cr{begin + L2, end, node + 2 * lp, prims - lp, nb, {0, 0, 0}};
-                const bool right_big = m - L2 > L2;
+                const WorkItem cl{begin, begin + nl, node + 1, lp, nb, {0, 0, 0}};
+                const WorkItem cr{begin + nl, end, node + 2 * lp, prims - lp, nb, {0, 0, 0}};
+                const bool right_big = m - nl > nl;
                 int top = sp;
-    #pragma unroll
+#pragma unroll
                 for (int j = 0; j < 2; ++j) {  // the larger child waits below the smaller one
-                    const WorkItem cw = (j == 0) == right_big ? cr : cl;
+                    const bool take_right = (j == 0) == right_big;
+                    const WorkItem cw = take_right ? cr : cl;
                     const uint32_t cs = cw.end - cw.begin;
                     if (cs == 1) {
                         emit_bottom_root(A, A.trec[nb][cw.begin], cw.node);
                     } else if (cs <= kSmallNode) {
                         push_item(A.small_top, A.counters + kCntSmallTop, A.cap_small_top, cw, A.err);
                     } else if (top < kMidStack) {
-                        L.stack[top++] = make_uint4(cw.begin, cw.end, cw.node, cw.prims | (nb << 31));
+                        L.stack[top] = make_uint4(cw.begin, cw.end, cw.node, cw.prims | (nb << 31));
+#pragma unroll
+                        for (int k = 0; k < 12; ++k) L.sbox[top][k] = take_right ? cv[1][k] : cv[0][k];
+                        ++top;
                     } else {
                         atomicOr(A.err, uint32_t(kErrCapacity));
                     }
