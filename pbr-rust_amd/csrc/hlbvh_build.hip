@@ -1164,46 +1164,18 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
             }
             wave_lds_sync();
             // buckets, one row of 64 positions at a time (the next row's records
-            // loaded ahead): the first two buckets of a row reduced across the wave,
-            // the rest per lane
+            // loaded ahead), merged into the node's statistics by LDS atomics (across
+            // the wave reductions of same-bucket lanes cost more than the atomics
+            // at these node sizes)
             Rec rn{};
             if (lane < m) rn = R[begin + lane];
             for (uint32_t p0 = 0; p0 < m; p0 += kWave) {
                 const uint32_t p = p0 + lane;
-                const bool valid = p < m;
                 const Rec r = rn;
                 if (p + kWave < m) rn = R[begin + p + kWave];
-                uint32_t k = 0;
-                if (valid) {
-                    k = bucket_of(sel3(box_centroid(rec_box(r)), axis), base, span);
+                if (p < m) {
+                    const uint32_t k = bucket_of(sel3(box_centroid(rec_box(r)), axis), base, span);
                     L.bkt[p] = uint8_t(k);
-                }
-                uint64_t todo = __ballot(valid);
-                for (int round = 0; round < 2 && todo; ++round) {
-                    const int lead = __builtin_ctzll(todo);
-                    const uint32_t kb = __shfl(k, lead, kWave);
-                    const bool in = valid && k == kb;
-                    const uint64_t peers = __ballot(in);
-                    todo &= ~peers;
-                    const uint32_t pr = wave_sum_all(in ? r.b : 0u);
-                    uint32_t bx[6];
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        bx[d] = wave_min_u32_all(in ? f2ord(r.lo[d]) : 0xFFFFFFFFu);
-                        bx[3 + d] = wave_max_u32_all(in ? f2ord(r.hi[d]) : 0u);
-                    }
-                    if (int(lane) == lead) {
-                        L.cnt[kb] += uint32_t(__popcll(peers));
-                        L.pr[kb] += pr;
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) {
-                            L.box[kb][d] = min(L.box[kb][d], bx[d]);
-                            L.box[kb][3 + d] = max(L.box[kb][3 + d], bx[3 + d]);
-                        }
-                    }
-                    wave_lds_sync();
-                }
-                if ((todo >> lane) & 1u) {
                     atomicAdd(&L.cnt[k], 1u);
                     atomicAdd(&L.pr[k], r.b);
 #pragma unroll
@@ -1212,8 +1184,8 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
                         atomicMax(&L.box[k][3 + d], f2ord(r.hi[d]));
                     }
                 }
-                wave_lds_sync();
             }
+            wave_lds_sync();
             // the price of the 23 splits (lane k = bucket k), as top_decide_kernel
             const bool in = lane < kNumBuckets;
             const uint32_t cnt = in ? L.pr[lane] : 0u, tcnt = in ? L.cnt[lane] : 0u;
