@@ -964,6 +964,7 @@ __global__ void round_end_kernel(uint32_t *counters) {
 // rounds[j] receives the job's round count. Lengths never exceed
 // max_depth + 1, so finishing counts live in a ring of kRing rounds.
 constexpr uint32_t kSchedBlock = 1024, kRing = 4096;
+constexpr int kSchedPaths = 8;  // paths per thread and trip
 __global__ __launch_bounds__(kSchedBlock) void schedule_kernel(const JobDev *jobs, uint32_t *rec_round,
                                                                uint32_t *rounds, uint32_t *err) {
     __shared__ uint32_t ring[kRing];
@@ -983,28 +984,37 @@ __global__ __launch_bounds__(kSchedBlock) void schedule_kernel(const JobDev *job
     uint64_t r = 0;
     uint32_t my_max = 0;
     for (;;) {
-        // wave-uniform trips; the finishing rounds of a wave's 64 paths are counted
-        // per distinct value (at most max_depth + 1 of them), one LDS atomic each
-        for (uint64_t b = lo; b < hi; b += kSchedBlock) {
-            const uint64_t k = b + threadIdx.x;
-            bool ok = k < hi;
-            const uint32_t len = ok ? rr[k] : 0u;
-            if (ok && len > lmax) {
-                s_bad = 1;
-                ok = false;
+        // wave-uniform trips of kSchedPaths paths per thread (their lengths loaded
+        // together); the finishing rounds of a wave's paths are counted per distinct
+        // value (at most max_depth + 1 of them), one LDS atomic each
+        for (uint64_t b = lo; b < hi; b += uint64_t(kSchedBlock) * kSchedPaths) {
+            uint32_t len[kSchedPaths];
+#pragma unroll
+            for (int j = 0; j < kSchedPaths; ++j) {
+                const uint64_t k = b + uint64_t(j) * kSchedBlock + threadIdx.x;
+                len[j] = k < hi ? rr[k] : 0u;
             }
-            const uint32_t f = uint32_t(r) + len;
-            if (ok) {
-                rr[k] = f;
-                my_max = f > my_max ? f : my_max;
-            }
-            uint64_t pending = __ballot(ok);
-            while (pending) {
-                const int leader = __ffsll((unsigned long long)pending) - 1;
-                const uint32_t fl = __shfl(f, leader, kWave);
-                const uint64_t same = __ballot(ok && f == fl);
-                if (lane_id() == uint32_t(leader)) atomicAdd(&ring[fl % kRing], uint32_t(__popcll(same)));
-                pending &= ~same;
+#pragma unroll
+            for (int j = 0; j < kSchedPaths; ++j) {
+                const uint64_t k = b + uint64_t(j) * kSchedBlock + threadIdx.x;
+                bool ok = k < hi;
+                if (ok && len[j] > lmax) {
+                    s_bad = 1;
+                    ok = false;
+                }
+                const uint32_t f = uint32_t(r) + len[j];
+                if (ok) {
+                    rr[k] = f;
+                    my_max = f > my_max ? f : my_max;
+                }
+                uint64_t pending = __ballot(ok);
+                while (pending) {
+                    const int leader = __ffsll((unsigned long long)pending) - 1;
+                    const uint32_t fl = __shfl(f, leader, kWave);
+                    const uint64_t same = __ballot(ok && f == fl);
+                    if (lane_id() == uint32_t(leader)) atomicAdd(&ring[fl % kRing], uint32_t(__popcll(same)));
+                    pending &= ~same;
+                }
             }
         }
         __syncthreads();
