@@ -1111,6 +1111,69 @@ __global__ __launch_bounds__(kFilmWaves * kWave) void film_kernel(SceneDev S, co
     }
 }
 
+// film_kernel for at most kLaneFilm samples per pixel: one lane per pixel (a wave
+// per pixel would leave most lanes idle), the same (round, sample) order and the
+// same sums, one operation at a time.
+constexpr uint32_t kLaneFilm = 4;
+
+__global__ __launch_bounds__(kBlock) void film_lane_kernel(SceneDev S, const float4 *__restrict__ rec,
+                                                           const uint32_t *__restrict__ rec_round, uint32_t spp,
+                                                           float *film_rgb, float *film_aux) {
+    const uint32_t wh = S.width * S.height;
+    const uint32_t px = blockIdx.x * kBlock + threadIdx.x;
+    if (px >= wh) return;
+    uint32_t key[kLaneFilm];
+    float4 r[kLaneFilm];
+#pragma unroll
+    for (uint32_t s = 0; s < kLaneFilm; ++s) {
+        key[s] = 0u;
+        r[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (s < spp) {
+            key[s] = rec_round[size_t(s) * wh + px];
+            r[s] = rec[size_t(s) * wh + px];
+        }
+    }
+    float sum[3] = {0.f, 0.f, 0.f}, wsum = 0.f;
+#pragma unroll
+    for (uint32_t pos = 0; pos < kLaneFilm; ++pos) {
+        if (pos >= spp) break;
+        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (uint32_t s = 0; s < kLaneFilm; ++s) {
+            uint32_t rank = 0;
+#pragma unroll
+            for (uint32_t t = 0; t < kLaneFilm; ++t)
+                rank += (t < spp && (key[t] < key[s] || (key[t] == key[s] && t < s))) ? 1u : 0u;
+            if (s < spp && rank == pos) q = r[s];
+        }
+        if (q.w == 0) continue;
+        sum[0] += q.w * q.x;
+        sum[1] += q.w * q.y;
+        sum[2] += q.w * q.z;
+        wsum += q.w;
+    }
+    float rgb[3] = {sum[0], sum[1], sum[2]};
+    if (wsum != 0)
+        for (int c = 0; c < 3; ++c) rgb[c] /= wsum;
+    for (int c = 0; c < 3; ++c) film_rgb[3 * size_t(px) + c] = ip3(S.out_from_sensor + 3 * c, rgb);
+    if (film_aux) {
+        for (int c = 0; c < 3; ++c) film_aux[4 * size_t(px) + c] = sum[c];
+        film_aux[4 * size_t(px) + 3] = wsum;
+    }
+}
+
+// The film of one render: a lane per pixel up to kLaneFilm samples, else a wave.
+static void launch_film(const SceneDev &S, const float4 *rec, const uint32_t *rec_round, uint32_t spp,
+                        float *film_rgb, float *film_aux, hipStream_t stream) {
+    const uint32_t wh = S.width * S.height;
+    if (spp <= kLaneFilm)
+        film_lane_kernel<<<(wh + kBlock - 1) / kBlock, kBlock, 0, stream>>>(S, rec, rec_round, spp, film_rgb,
+                                                                            film_aux);
+    else
+        film_kernel<<<(wh + kFilmWaves - 1) / kFilmWaves, kFilmWaves * kWave, 0, stream>>>(S, rec, rec_round, spp,
+                                                                                         film_rgb, film_aux);
+}
+
 }  // namespace
 
 namespace {
@@ -1430,8 +1493,7 @@ hipError_t render(const SceneDev &S, const WideTrace &W0, uint32_t spp, uint32_t
         PT_CK(hipGetLastError());
     }
     if (film_rgb) {
-        film_kernel<<<(wh + kFilmWaves - 1) / kFilmWaves, kFilmWaves * kWave, 0, stream>>>(
-            S, st.rec, st.rec_round, spp, film_rgb, film_aux);
+        launch_film(S, st.rec, st.rec_round, spp, film_rgb, film_aux, stream);
         PT_CK(hipGetLastError());
     }
     PT_CK(hipEventRecord(ev[2], stream));
@@ -1597,9 +1659,8 @@ hipError_t render_shared(const std::vector<BatchJob> &jobs, uint32_t pool, uint3
     PT_CK(hipGetLastError());
     PT_CK(hipEventRecord(ev[2], stream));
     for (uint32_t j = 0; j < nj; ++j) {
-        const uint32_t wh = jd[j].S.width * jd[j].S.height;
-        film_kernel<<<(wh + kFilmWaves - 1) / kFilmWaves, kFilmWaves * kWave, 0, stream>>>(
-            jd[j].S, st.rec + jd[j].base, st.rec_round + jd[j].base, jd[j].spp, jobs[j].film_rgb, jobs[j].film_aux);
+        launch_film(jd[j].S, st.rec + jd[j].base, st.rec_round + jd[j].base, jd[j].spp, jobs[j].film_rgb,
+                    jobs[j].film_aux, stream);
         PT_CK(hipGetLastError());
     }
     PT_CK(hipEventRecord(ev[3], stream));
