@@ -910,9 +910,18 @@ __global__ __launch_bounds__(64) void top_decide_kernel(TopLevel TL, uint32_t p,
     for (uint32_t c0 = 0; c0 < it.ntasks; c0 += kWave) {
         const uint32_t c = c0 + lane;
         uint32_t l = 0;
-        if (c < it.ntasks) {
-            const uint32_t *h = TL.hist + size_t(it.first_task + c) * kNumBuckets;
-            for (uint32_t k = 0; k <= split; ++k) l += h[k];
+        if (c < it.ntasks) {  // the chunk's 24 counts in six independent 16-byte loads
+            const uint4 *h = reinterpret_cast<const uint4 *>(TL.hist + size_t(it.first_task + c) * kNumBuckets);
+            uint4 q[kNumBuckets / 4];
+#pragma unroll
+            for (int j = 0; j < int(kNumBuckets / 4); ++j) q[j] = h[j];
+#pragma unroll
+            for (int j = 0; j < int(kNumBuckets / 4); ++j) {
+                l += (4 * j + 0 <= int(split)) ? q[j].x : 0u;
+                l += (4 * j + 1 <= int(split)) ? q[j].y : 0u;
+                l += (4 * j + 2 <= int(split)) ? q[j].z : 0u;
+                l += (4 * j + 3 <= int(split)) ? q[j].w : 0u;
+            }
         }
         const uint32_t inc = wave_inclusive_add(l);
         if (c < it.ntasks) TL.left[it.first_task + c] = carry + inc - l;
@@ -1053,21 +1062,30 @@ __global__ __launch_bounds__(64) void top_children_kernel(BuildArgs A, TopLevel 
         }
         base_i = __shfl(base_i, kWave - 1, kWave);
         base_t = __shfl(base_t, kWave - 1, kWave);
-        if (!large) continue;
         const uint32_t slot = base_i + inc_i - 1, first = base_t + inc_t - nt;
-        if (slot >= TL.cap_items || uint64_t(first) + nt > TL.cap_tasks) {
-            atomicOr(A.err, uint32_t(kErrCapacity));
-            continue;
+        const bool fits = large && slot < TL.cap_items && uint64_t(first) + nt <= TL.cap_tasks;
+        if (large && !fits) atomicOr(A.err, uint32_t(kErrCapacity));
+        if (fits) {
+            TL.items[q][slot] = TopItem{cw.begin, cw.end, cw.node, cw.prims, nb, first, nt, 0u};
+            TopAcc &Nx = TL.acc[q][slot];
+            const TopAcc &Ac = TL.acc[p][i];
+            for (int k = 0; k < 6; ++k) {
+                Nx.cb[k] = Ac.child[c][k];
+                Nx.fb[k] = Ac.child[c][6 + k];
+            }
+            Nx.split = kTopInvalid;
         }
-        TL.items[q][slot] = TopItem{cw.begin, cw.end, cw.node, cw.prims, nb, first, nt, 0u};
-        TopAcc &Nx = TL.acc[q][slot];
-        const TopAcc &Ac = TL.acc[p][i];
-        for (int k = 0; k < 6; ++k) {
-            Nx.cb[k] = Ac.child[c][k];
-            Nx.fb[k] = Ac.child[c][6 + k];
+        // each node's chunk tasks: a few by its own lane, many (near the root: hundreds)
+        // by the whole wave
+        if (fits && nt <= 8)
+            for (uint32_t k = 0; k < nt; ++k) TL.tasks[q][first + k] = make_uint2(slot, k);
+        uint64_t todo = __ballot(fits && nt > 8);
+        while (todo) {
+            const int l = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const uint32_t f = __shfl(first, l, kWave), cnt = __shfl(nt, l, kWave), sl = __shfl(slot, l, kWave);
+            for (uint32_t k = __lane_id(); k < cnt; k += kWave) TL.tasks[q][f + k] = make_uint2(sl, k);
         }
-        Nx.split = kTopInvalid;
-        for (uint32_t k = 0; k < nt; ++k) TL.tasks[q][first + k] = make_uint2(slot, k);
     }
 }
 
