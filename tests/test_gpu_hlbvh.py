@@ -225,6 +225,26 @@ def test_build_size_sequence_in_one_context():
         c.close()
 
 
+def skewed(n, seed, power=3.0):
+    """A thin sheet whose centroids crowd towards one corner (u^3, v^3): uneven SAH
+    splits all the way down, so that nodes of 65..2048 treelets hold deep,
+    lopsided subtrees."""
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 1, 3))
+    c[:, :, :2] = c[:, :, :2] ** power
+    c[:, :, 2] = 0.5 + 0.01 * c[:, :, 2]
+    return np.ascontiguousarray((c + rng.normal(0, 2e-4, (n, 3, 3))).astype(np.float32))
+
+
+@pytest.mark.parametrize("n", [2500, 6000, 40000])
+def test_skewed_sheet_canonical_parity(ctx, n):
+    """1.8k, 3.6k and 13.9k treelets: the root inside the 65..2048-treelet finisher,
+    just above it, and several fused levels before it."""
+    tri = skewed(n, 11)
+    got = ctx.build(tri).export_canonical()
+    assert got == O.OracleBvh(tri).export_canonical()
+
+
 def test_empty_aggregate(ctx):
     tri = np.zeros((0, 3, 3), np.float32)
     bvh = ctx.build(tri)
