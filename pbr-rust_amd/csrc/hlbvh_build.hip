@@ -642,6 +642,75 @@ __global__ __launch_bounds__(64) void children_kernel(BuildArgs A, const WorkIte
 }
 
 
+
+// ------------------------------------------------ SAH price of the 23 splits
+//
+// Lane k (< 24) holds bucket k: its primitive count, treelet count and box (the
+// empty box when unused); lane k also holds bucket 23 - k in the `r` arguments.
+// Inclusive prefixes over lanes 0..23 by DPP (row shifts 1, 2, 4, 8, then row 0's
+// last lane broadcast into row 1): the forward scan gives the left side of split k
+// at lane k, the scan of the reversed buckets the right side (buckets k + 1..23)
+// at lane 22 - k. Cost of split k as hlbvh.cu:711-731, the first strict minimum as
+// :733-741; the min / max / sum of the unions are exact, so the order in which the
+// scans combine them changes no value.
+struct SplitPrice {
+    uint32_t split, left_prims, right_prims, left_treelets;
+};
+
+template <typename Op>
+__device__ __forceinline__ uint32_t dpp_prefix24(uint32_t v, uint32_t ident, Op op) {
+    v = op(v, uint32_t(__builtin_amdgcn_update_dpp(int(ident), int(v), 0x111, 0xF, 0xF, false)));  // row_shr:1
+    v = op(v, uint32_t(__builtin_amdgcn_update_dpp(int(ident), int(v), 0x112, 0xF, 0xF, false)));  // row_shr:2
+    v = op(v, uint32_t(__builtin_amdgcn_update_dpp(int(ident), int(v), 0x114, 0xF, 0xF, false)));  // row_shr:4
+    v = op(v, uint32_t(__builtin_amdgcn_update_dpp(int(ident), int(v), 0x118, 0xF, 0xF, false)));  // row_shr:8
+    v = op(v, uint32_t(__builtin_amdgcn_update_dpp(int(ident), int(v), 0x142, 0xA, 0xF, false)));  // row_bcast:15
+    return v;
+}
+
+__device__ __forceinline__ void dpp_prefix_box(uint32_t w[6]) {  // order-mapped lo (min) and hi (max)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        w[k] = dpp_prefix24(w[k], 0xFFFFFFFFu, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
+        w[3 + k] = dpp_prefix24(w[3 + k], 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
+    }
+}
+
+// Every lane of the wave calls it. Boxes are order-mapped (f2ord), identities for
+// empty buckets.
+__device__ __forceinline__ SplitPrice price_splits(uint32_t cnt, uint32_t tcnt, const uint32_t box[6],
+                                                   uint32_t rcnt, const uint32_t rbox[6], float total_sa) {
+    const uint32_t lane = __lane_id();
+    auto add = [](uint32_t a, uint32_t b) { return a + b; };
+    uint32_t pl[6], sr[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        pl[k] = box[k];
+        sr[k] = rbox[k];
+    }
+    dpp_prefix_box(pl);
+    dpp_prefix_box(sr);
+    const uint32_t pc = dpp_prefix24(cnt, 0u, add), tc = dpp_prefix24(tcnt, 0u, add);
+    const uint32_t sc = dpp_prefix24(rcnt, 0u, add);
+    const int src = int(kNumBuckets) - 2 - int(lane);  // right side of split `lane` (lanes 0..22)
+    uint32_t r6[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r6[k] = __shfl(sr[k], src < 0 ? 0 : src, kWave);
+    const uint32_t rc = __shfl(sc, src < 0 ? 0 : src, kWave);
+    uint32_t key = 0xFFFFFFFFu;
+    if (lane < kNumBuckets - 1) {
+        const Box bl = acc_box(pl), br = acc_box(r6);
+        const float cost = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
+        key = f2ord(cost);
+    }
+    const uint32_t kmin = wave_min_u32_all(key);
+    SplitPrice sp;
+    sp.split = uint32_t(__builtin_ctzll(__ballot(lane < kNumBuckets - 1 && key == kmin)));
+    sp.left_prims = __shfl(pc, int(sp.split), kWave);
+    sp.right_prims = __shfl(rc, int(sp.split), kWave);
+    sp.left_treelets = __shfl(tc, int(sp.split), kWave);
+    return sp;
+}
+
 // ------------------------------------------- fused top level (> 64 treelets)
 //
 // Four launches per level instead of the generic level path's eleven, and two
@@ -864,42 +933,17 @@ __global__ __launch_bounds__(64) void top_decide_kernel(TopLevel TL, uint32_t p,
     if (sel3(cb.lo, axis) == sel3(cb.hi, axis)) return;  // reported by top_bucket_kernel; no children
     const int lane = threadIdx.x;
     const bool in = lane < int(kNumBuckets);
-    const uint32_t cnt = in ? A.bcnt[lane] : 0u;
-    // an untouched bucket holds the atomic identities: use the empty box
-    const Box b = cnt > 0 ? acc_box(A.bbox[lane]) : box_empty();
-    float pl[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
-    float sf[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
-    uint32_t pc = cnt, sc = cnt;
+    const int fl = in ? lane : 0, rl = in ? int(kNumBuckets) - 1 - lane : 0;  // the bucket, reversed
+    const uint32_t cnt = in ? A.bcnt[fl] : 0u, rcnt = in ? A.bcnt[rl] : 0u;
+    uint32_t box[6], rbox[6];
 #pragma unroll
-    for (int d = 1; d < 32; d <<= 1) {
-        const bool up = lane >= d;
-        const bool dn = lane + d < int(kNumBuckets);
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const float u = __shfl_up(pl[k], d, kWave);
-            const float w = __shfl_down(sf[k], d, kWave);
-            if (up) pl[k] = k < 3 ? smin(u, pl[k]) : smax(u, pl[k]);
-            if (dn) sf[k] = k < 3 ? smin(sf[k], w) : smax(sf[k], w);
-        }
-        const uint32_t uc = __shfl_up(pc, d, kWave), wc = __shfl_down(sc, d, kWave);
-        if (up) pc += uc;
-        if (dn) sc += wc;
+    for (int k = 0; k < 6; ++k) {
+        // an untouched bucket holds the atomic identities
+        box[k] = cnt > 0 ? A.bbox[fl][k] : (k < 3 ? 0xFFFFFFFFu : 0u);
+        rbox[k] = rcnt > 0 ? A.bbox[rl][k] : (k < 3 ? 0xFFFFFFFFu : 0u);
     }
-    float r6[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) r6[k] = __shfl_down(sf[k], 1, kWave);
-    const uint32_t rc = __shfl_down(sc, 1, kWave);
-    const float total_sa = box_area(acc_box(A.fb));
-    uint32_t key = 0xFFFFFFFFu;
-    if (lane < int(kNumBuckets) - 1) {
-        const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
-        const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
-        const float cost = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
-        key = f2ord(cost);
-    }
-    const uint32_t kmin = wave_min_u32(key);
-    const uint32_t split = uint32_t(__builtin_ctzll(__ballot(lane < int(kNumBuckets) - 1 && key == kmin)));
-    const uint32_t lp = __shfl(pc, int(split), kWave), rp = __shfl(rc, int(split), kWave);
+    const SplitPrice price = price_splits(cnt, 0u, box, rcnt, rbox, box_area(acc_box(A.fb)));
+    const uint32_t split = price.split, lp = price.left_prims, rp = price.right_prims;
     // both end buckets hold a treelet (the least and the greatest centroid), so
     // neither side is empty; the midpoint fallback (hlbvh.cu:758-777) is unreachable
     if (lp == 0u || rp == 0u) {
@@ -1204,49 +1248,19 @@ __global__ __launch_bounds__(kChunkBlock) void mid_top_kernel(BuildArgs A) {
                 }
             }
             wave_lds_sync();
-            // the price of the 23 splits (lane k = bucket k), as top_decide_kernel
+            // the price of the 23 splits (lane k = bucket k, and bucket 23 - k reversed)
             const bool in = lane < kNumBuckets;
-            const uint32_t cnt = in ? L.pr[lane] : 0u, tcnt = in ? L.cnt[lane] : 0u;
-            // an untouched bucket holds the identities: use the empty box
-            Box b = box_empty();
-            if (in && tcnt > 0)
-                b = {{ord2f(L.box[lane][0]), ord2f(L.box[lane][1]), ord2f(L.box[lane][2])},
-                     {ord2f(L.box[lane][3]), ord2f(L.box[lane][4]), ord2f(L.box[lane][5])}};
-            float pl[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
-            float sf[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
-            uint32_t pc = cnt, sc = cnt, tc = tcnt;
+            const uint32_t fl = in ? lane : 0u, rl = in ? kNumBuckets - 1 - lane : 0u;
+            const uint32_t cnt = in ? L.pr[fl] : 0u, tcnt = in ? L.cnt[fl] : 0u, rcnt = in ? L.pr[rl] : 0u;
+            uint32_t box[6], rbox[6];
 #pragma unroll
-            for (int d = 1; d < 32; d <<= 1) {
-                const bool up = int(lane) >= d;
-                const bool dn = int(lane) + d < int(kNumBuckets);
-#pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                    const float u = __shfl_up(pl[k], d, kWave);
-                    const float w = __shfl_down(sf[k], d, kWave);
-                    if (up) pl[k] = k < 3 ? smin(u, pl[k]) : smax(u, pl[k]);
-                    if (dn) sf[k] = k < 3 ? smin(sf[k], w) : smax(sf[k], w);
-                }
-                const uint32_t uc = __shfl_up(pc, d, kWave), wc = __shfl_down(sc, d, kWave);
-                const uint32_t ut = __shfl_up(tc, d, kWave);
-                if (up) pc += uc, tc += ut;
-                if (dn) sc += wc;
+            for (int k = 0; k < 6; ++k) {  // an untouched bucket holds the identities
+                box[k] = in && L.cnt[fl] > 0 ? L.box[fl][k] : (k < 3 ? 0xFFFFFFFFu : 0u);
+                rbox[k] = in && L.cnt[rl] > 0 ? L.box[rl][k] : (k < 3 ? 0xFFFFFFFFu : 0u);
             }
-            float r6[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) r6[k] = __shfl_down(sf[k], 1, kWave);
-            const uint32_t rc = __shfl_down(sc, 1, kWave);
-            const float total_sa = box_area(fb);
-            uint32_t key = 0xFFFFFFFFu;
-            if (lane < kNumBuckets - 1) {
-                const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
-                const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
-                const float cost = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
-                key = f2ord(cost);
-            }
-            const uint32_t kmin = wave_min_u32_all(key);
-            const uint32_t split = uint32_t(__builtin_ctzll(__ballot(lane < kNumBuckets - 1 && key == kmin)));
-            const uint32_t lp = __shfl(pc, int(split), kWave), rp = __shfl(rc, int(split), kWave);
-            const uint32_t nl = __shfl(tc, int(split), kWave);  // left treelets
+            const SplitPrice price = price_splits(cnt, tcnt, box, rcnt, rbox, box_area(fb));
+            const uint32_t split = price.split, lp = price.left_prims, rp = price.right_prims;
+            const uint32_t nl = price.left_treelets;  // left treelets
             // both end buckets hold a treelet, so neither side is empty (as top_decide_kernel)
             if (lp == 0u || rp == 0u || nl == 0u || nl == m) {
                 if (lane == 0) atomicOr(A.err, uint32_t(kErrSplit));
@@ -1837,44 +1851,17 @@ __device__ __forceinline__ void wave_build_top(const BuildArgs &A, const Rec *sr
         wave_lds_sync();
         // lane k: bucket k; an empty bucket is the empty box
         const bool in = lane < kNumBuckets;
-        const uint32_t cnt = in ? L.acc[lane][0] : 0u;
-        Box bb = box_empty();
-        if (in && cnt > 0)
-            bb = {{ord2f(L.acc[lane][1]), ord2f(L.acc[lane][2]), ord2f(L.acc[lane][3])},
-                  {ord2f(L.acc[lane][4]), ord2f(L.acc[lane][5]), ord2f(L.acc[lane][6])}};
-        float pl[6] = {bb.lo.x, bb.lo.y, bb.lo.z, bb.hi.x, bb.hi.y, bb.hi.z};
-        float sf[6] = {bb.lo.x, bb.lo.y, bb.lo.z, bb.hi.x, bb.hi.y, bb.hi.z};
-        uint32_t pc = cnt, sc = cnt;
+        const uint32_t fl = in ? lane : 0u, rl = in ? kNumBuckets - 1 - lane : 0u;  // the bucket, reversed
+        const uint32_t cnt = in ? L.acc[fl][0] : 0u, rcnt = in ? L.acc[rl][0] : 0u;
+        uint32_t box[6], rbox[6];
 #pragma unroll
-        for (int d = 1; d < 32; d <<= 1) {
-            const bool up = int(lane) >= d;
-            const bool dn = int(lane) + d < int(kNumBuckets);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const float u = __shfl_up(pl[k], d, kWave);
-                const float w = __shfl_down(sf[k], d, kWave);
-                if (up) pl[k] = k < 3 ? smin(u, pl[k]) : smax(u, pl[k]);
-                if (dn) sf[k] = k < 3 ? smin(sf[k], w) : smax(sf[k], w);
-            }
-            const uint32_t uc = __shfl_up(pc, d, kWave), wc = __shfl_down(sc, d, kWave);
-            if (up) pc += uc;
-            if (dn) sc += wc;
+        for (int k = 0; k < 6; ++k) {  // an untouched bucket holds the identities
+            box[k] = cnt > 0 ? L.acc[fl][1 + k] : (k < 3 ? 0xFFFFFFFFu : 0u);
+            rbox[k] = rcnt > 0 ? L.acc[rl][1 + k] : (k < 3 ? 0xFFFFFFFFu : 0u);
         }
-        float r6[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) r6[k] = __shfl_down(sf[k], 1, kWave);
-        const uint32_t rc = __shfl_down(sc, 1, kWave);
-        const float total_sa = box_area(fb);
-        uint32_t key = 0xFFFFFFFFu;
-        if (lane < kNumBuckets - 1) {
-            const Box bl{{pl[0], pl[1], pl[2]}, {pl[3], pl[4], pl[5]}};
-            const Box br{{r6[0], r6[1], r6[2]}, {r6[3], r6[4], r6[5]}};
-            const float cost = sah_cost((float(pc) * box_area(bl) + float(rc) * box_area(br)) / total_sa);
-            key = f2ord(cost);
-        }
-        const uint32_t kmin = wave_min_u32_all(key);
-        const uint32_t split = uint32_t(__builtin_ctzll(__ballot(lane < kNumBuckets - 1 && key == kmin)));
-        uint32_t lp = __shfl(pc, int(split), kWave);
+        const SplitPrice price = price_splits(cnt, 0u, box, rcnt, rbox, box_area(fb));
+        const uint32_t split = price.split;
+        uint32_t lp = price.left_prims;
         bool left = active && bk <= split;
         const uint64_t lm = __ballot(left);
         if (lm == 0 || __popcll(lm) == m) {  // midpoint fallback (hlbvh.cu:758-777)
