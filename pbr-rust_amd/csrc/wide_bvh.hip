@@ -94,6 +94,10 @@ __device__ __forceinline__ Box bnode_box(const BNode &b) {
 }
 __device__ __forceinline__ bool bnode_leaf(const BNode &b) { return (b.a & kLeafFlag) != 0; }
 
+// Collapse counters: [1] wide nodes used, [4] levels that had jobs, [kCollapseJobs
+// + L % 3] the job count of level L (three slots in rotation).
+constexpr uint32_t kCollapseJobs = 8;
+
 __global__ void wide_root_kernel(const BNode *bn, Job *jobs, uint32_t *counters, WNode *wn) {
     if (threadIdx.x != 0) return;
     const BNode r = bn[0];
@@ -106,12 +110,12 @@ __global__ void wide_root_kernel(const BNode *bn, Job *jobs, uint32_t *counters,
             wn[0].ref[s][0] = use ? r.a : kWideEmpty;
             wn[0].ref[s][1] = use ? r.b : 0u;
         }
-        counters[1] = 1;  // nodes used
-        counters[0] = 0;  // no jobs
+        counters[1] = 1;                // nodes used
+        counters[kCollapseJobs] = 0;    // no jobs at level 0
         return;
     }
     jobs[0] = Job{0, 0, 1, 0};
-    counters[0] = 1;
+    counters[kCollapseJobs] = 1;
     counters[1] = 1;
 }
 
@@ -122,7 +126,8 @@ constexpr int kCollapseBlock = 64;
 // All 64 lanes of the wave take part (live or not): the child reservation is one
 // atomic per wave.
 __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const Job *__restrict__ jobs, uint32_t i,
-                                             bool live, Job *__restrict__ next, uint32_t *__restrict__ counters,
+                                             bool live, Job *__restrict__ next, uint32_t *__restrict__ next_count,
+                                             uint32_t *__restrict__ counters,
                                              WNode *__restrict__ wn, uint32_t *__restrict__ order, uint32_t cap,
                                              uint32_t *__restrict__ err) {
     const Job job = live ? jobs[i] : Job{0, 0, 0, 0};
@@ -194,7 +199,7 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
     uint32_t base = 0, jbase = 0;
     if (__lane_id() == kWave - 1 && incl > 0) {
         base = atomicAdd(&counters[1], incl);
-        jbase = atomicAdd(&counters[2], incl);
+        jbase = atomicAdd(next_count, incl);
     }
     base = __shfl(base, kWave - 1, kWave) + incl - n_int;
     jbase = __shfl(jbase, kWave - 1, kWave) + incl - n_int;
@@ -290,28 +295,26 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
     }, nF);
 }
 
-// One level of the collapse: counters[0] jobs (read on the device), their
-// children appended to `next` (counters[2]). A grid of resident waves walks the
-// jobs 64 at a time.
+// One level of the collapse. Job counts rotate over three counter slots: level L
+// reads its count from slot L % 3, appends its children's jobs to slot (L + 1) % 3
+// and clears slot (L + 2) % 3 (level L - 1's, read by no one any more; level L + 1
+// appends to it), so no kernel runs between levels. A grid of resident waves walks
+// the jobs 64 at a time; a level that had jobs is counted (counters[4]).
 __global__ __launch_bounds__(kCollapseBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void wide_collapse_kernel(
     const BNode *__restrict__ bn, const Job *__restrict__ jobs, Job *__restrict__ next,
     uint32_t *__restrict__ counters, WNode *__restrict__ wn, uint32_t *__restrict__ order, uint32_t cap,
-    uint32_t *__restrict__ err) {
-    const uint32_t njobs = counters[0];
+    uint32_t *__restrict__ err, uint32_t level) {
+    const uint32_t njobs = counters[kCollapseJobs + level % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[kCollapseJobs + (level + 2) % 3] = 0u;
+        if (njobs > 0) counters[4] += 1u;
+    }
+    uint32_t *next_count = counters + kCollapseJobs + (level + 1) % 3;
     const uint32_t stride = gridDim.x * kCollapseBlock;
     for (uint32_t base = blockIdx.x * kCollapseBlock; base < njobs; base += stride) {
         const uint32_t i = base + threadIdx.x;
-        collapse_one(bn, jobs, i, i < njobs, next, counters, wn, order, cap, err);
+        collapse_one(bn, jobs, i, i < njobs, next, next_count, counters, wn, order, cap, err);
     }
-}
-
-// Between two levels: the next level's job count becomes the current one, a
-// level that had jobs is counted (counters[4]).
-__global__ void collapse_advance_kernel(uint32_t *counters) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (counters[0] > 0) counters[4] += 1;
-    counters[0] = counters[2];
-    counters[2] = 0;
 }
 
 // Entry distance of one child box, or +inf when the slab test fails for a
@@ -721,8 +724,8 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
     Job *jobs[2] = {reinterpret_cast<Job *>(scratch.ptr),
                     reinterpret_cast<Job *>(static_cast<char *>(scratch.ptr) + job_bytes)};
     uint32_t *counters = reinterpret_cast<uint32_t *>(static_cast<char *>(scratch.ptr) + 2 * job_bytes);
-    // counters: [0] jobs this level, [1] wide nodes used, [2] jobs next level, [3] unused,
-    // [4] levels that had jobs
+    // counters: [1] wide nodes used, [4] levels that had jobs, [kCollapseJobs + L % 3]
+    // the job count of level L
     e = hipMemsetAsync(counters, 0, 64, stream);
     if (e != hipSuccess) return e;
     WNode *wn = out.nodes.as<WNode>();
@@ -741,24 +744,23 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
     // levels are enqueued in batches between reads of the job count (the first
     // batch: about the depth of a full 8-wide tree over n leaves, plus two); a
     // level past the last one finds no jobs and exits at once
-    uint32_t host[5];
-    uint32_t enqueued = 0;
+    uint32_t host[kCollapseJobs + 3];
+    uint32_t level = 0;
     int cur = 0;
     int batch = 2;
     for (uint64_t c = 1; c < n; c <<= 3) ++batch;
     for (;;) {
-        for (int k = 0; k < batch; ++k) {
-            if (++enqueued > 100000) return hipErrorUnknown;
+        for (int k = 0; k < batch; ++k, ++level) {
+            if (level > 100000) return hipErrorUnknown;
             wide_collapse_kernel<<<resident, kCollapseBlock, 0, stream>>>(bnodes, jobs[cur], jobs[1 - cur], counters,
-                                                                          wn, order, cap, err);
-            collapse_advance_kernel<<<1, 64, 0, stream>>>(counters);
+                                                                          wn, order, cap, err, level);
             cur = 1 - cur;
         }
         batch = 4;
         e = hipMemcpyAsync(host, counters, sizeof(host), hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return e;
-        if (host[0] == 0) break;
+        if (host[kCollapseJobs + level % 3] == 0) break;  // the next level has no jobs
     }
     out.collapse_levels = host[4];
     out.num_nodes = host[1];
