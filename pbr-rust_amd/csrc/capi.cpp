@@ -350,8 +350,7 @@ int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint3
         if (derr) return bail(fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr)));
         e = bvh->tri_leaf.reserve(nt * 48);
         if (e == hipSuccess)
-            e = make_leaf_triangles(bvh->tri, bvh->dev.prim_order.as<uint32_t>(), uint32_t(nt),
-                                    bvh->tri_leaf.as<float>(), ctx->stream);
+            e = leaf_order(ctx->scratch, bvh->dev, bvh->tri_leaf.as<float>(), ctx->stream);
         if (e == hipSuccess) e = ctx->dev_err.reserve(64);
         if (e == hipSuccess) e = hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream);
         if (e == hipSuccess)

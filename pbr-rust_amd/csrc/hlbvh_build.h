@@ -51,8 +51,10 @@ struct BuildScratch {
     DevBuf lists, acc, splits, tasks, task_off, task_left, task_lprims;
     DevBuf top_items, top_acc, top_tasks, top_hist, top_left, top_bkt;  // fused top level
     DevBuf scan_scratch;
+    DevBuf tri_sorted;  // the triangles in sorted (Morton) order, 9 floats each
     SortScratch sort;
     uint32_t *host_counters = nullptr;
+    const uint32_t *sorted_pid = nullptr;  // input index per sorted position (one of vals[])
     void release();
 };
 
@@ -77,5 +79,10 @@ hipError_t morton_sort_device(const float *d_tri, uint32_t n, BuildScratch &S, u
 
 hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, BvhDevice &out,
                               uint32_t *device_error, hipStream_t stream, BuildPhaseTimes *times);
+
+// After build_hlbvh_device on the same scratch and stream: the primitive order
+// becomes input indices (the build leaves sorted positions in it), and tri_leaf
+// ([n][12] floats) receives the triangles in leaf order.
+hipError_t leaf_order(const BuildScratch &S, BvhDevice &bvh, float *tri_leaf, hipStream_t stream);
 
 }  // namespace pbrtm

@@ -224,20 +224,61 @@ __global__ __launch_bounds__(256) void morton_kernel(const float *__restrict__ t
         if (s_hist[i]) atomicAdd(&hist[i], s_hist[i]);
 }
 
-// Sorted position -> primitive record (box of the triangle, id), and the
+// Sorted position -> primitive record (box of the triangle, its sorted position
+// i: the leaves' primitive order holds sorted positions until leaf_order_kernel
+// maps them to input indices), the triangle copied in sorted order, and the
 // treelet-start flag (code >> 9 differs from the previous position).
 __global__ __launch_bounds__(256) void gather_kernel(const float *__restrict__ tri, uint32_t n,
                                                      const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ vals, Rec *__restrict__ prec,
-                                                     uint32_t *__restrict__ flags) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t pid = vals[i] < n ? vals[i] : 0u;  // in range unless the sort failed (reported)
-    const float *t = tri + 9ull * pid;
-    f3 p0{t[0], t[1], t[2]}, p1{t[3], t[4], t[5]}, p2{t[6], t[7], t[8]};
-    prec[i] = make_rec(tri_box(p0, p1, p2), pid, 1u);
-    const uint32_t key = keys[i] >> kTreeletShift;
-    flags[i] = (i == 0 || (keys[i - 1] >> kTreeletShift) != key) ? 1u : 0u;
+                                                     float *__restrict__ tri_sorted, uint32_t *__restrict__ flags) {
+    // the block's 256 triangles are staged in LDS and written as one contiguous
+    // run (16-byte stores for a full block: 9216 bytes, 16-byte aligned)
+    __shared__ __align__(16) float s_tri[256 * 9];
+    const uint32_t i0 = blockIdx.x * 256u;
+    const uint32_t i = i0 + threadIdx.x;
+    if (i < n) {
+        const uint32_t pid = vals[i] < n ? vals[i] : 0u;  // in range unless the sort failed (reported)
+        const float *t = tri + 9ull * pid;
+        f3 p0{t[0], t[1], t[2]}, p1{t[3], t[4], t[5]}, p2{t[6], t[7], t[8]};
+        prec[i] = make_rec(tri_box(p0, p1, p2), i, 1u);
+        float *d = s_tri + 9 * threadIdx.x;
+        d[0] = p0.x; d[1] = p0.y; d[2] = p0.z;
+        d[3] = p1.x; d[4] = p1.y; d[5] = p1.z;
+        d[6] = p2.x; d[7] = p2.y; d[8] = p2.z;
+        const uint32_t key = keys[i] >> kTreeletShift;
+        flags[i] = (i == 0 || (keys[i - 1] >> kTreeletShift) != key) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (n - i0 >= 256u) {
+        float4 *dst = reinterpret_cast<float4 *>(tri_sorted + 9ull * i0);
+        const float4 *src = reinterpret_cast<const float4 *>(s_tri);
+        for (uint32_t j = threadIdx.x; j < 256u * 9u / 4u; j += 256u) dst[j] = src[j];
+    } else {
+        const uint32_t cnt = (n - i0) * 9u;
+        for (uint32_t j = threadIdx.x; j < cnt; j += 256u) tri_sorted[9ull * i0 + j] = s_tri[j];
+    }
+}
+
+// Leaf position k: the sorted position m the tree put there becomes the input
+// index vals[m], and the triangle is copied from the sorted copy (48-byte
+// stride, three 16-byte stores). A treelet's primitives are one contiguous run
+// of sorted positions, so these reads stay local where the input-order gather
+// would be scattered.
+__global__ __launch_bounds__(256) void leaf_order_kernel(const uint32_t *__restrict__ vals,
+                                                         const float *__restrict__ tri_sorted, uint32_t n,
+                                                         uint32_t *__restrict__ prim_order,
+                                                         float *__restrict__ tri_leaf) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t m0 = prim_order[k];
+    const uint32_t m = m0 < n ? m0 : 0u;  // m0 < n for every complete tree
+    prim_order[k] = vals[m];
+    const float *s = tri_sorted + 9ull * m;
+    float4 *d = reinterpret_cast<float4 *>(tri_leaf) + 3ull * k;
+    d[0] = make_float4(s[0], s[1], s[2], s[3]);
+    d[1] = make_float4(s[4], s[5], s[6], s[7]);
+    d[2] = make_float4(s[8], 0.f, 0.f, 0.f);
 }
 
 __global__ __launch_bounds__(256) void treelet_start_kernel(const uint32_t *__restrict__ flags,
@@ -2156,6 +2197,7 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     PBRTM_TRY(S.hist.reserve((kSortPasses * kSortRadix + 64) * 4));
     PBRTM_TRY(out.nodes.reserve(size_t(2 * uint64_t(n) - 1) * sizeof(BNode)));
     PBRTM_TRY(out.prim_order.reserve(size_t(n) * 4));
+    PBRTM_TRY(S.tri_sorted.reserve(size_t(n) * 36));
     uint32_t *counters = S.counters.as<uint32_t>();
     uint32_t *hist = S.hist.as<uint32_t>();
     uint32_t *cbounds = hist + kSortPasses * kSortRadix;
@@ -2168,7 +2210,8 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     uint32_t *flags = (K == S.keys[0].as<uint32_t>()) ? S.keys[1].as<uint32_t>() : S.keys[0].as<uint32_t>();
     uint32_t *tidx = (V == S.vals[0].as<uint32_t>()) ? S.vals[1].as<uint32_t>() : S.vals[0].as<uint32_t>();
     Rec *prec0 = S.prec[0].as<Rec>();
-    gather_kernel<<<grid_n, 256, 0, stream>>>(d_tri, n, K, V, prec0, flags);
+    S.sorted_pid = V;
+    gather_kernel<<<grid_n, 256, 0, stream>>>(d_tri, n, K, V, prec0, S.tri_sorted.as<float>(), flags);
     PBRTM_TRY(exclusive_scan_u32(flags, tidx, n, counters + kCntTreelets, S.scan_scratch, stream));
     PBRTM_TRY(read_counters(S, stream));
     const uint32_t D = S.host_counters[kCntTreelets];
@@ -2260,11 +2303,18 @@ hipError_t build_hlbvh_device(const float *d_tri, uint32_t n, BuildScratch &S, B
     return hipSuccess;
 }
 
+hipError_t leaf_order(const BuildScratch &S, BvhDevice &bvh, float *tri_leaf, hipStream_t stream) {
+    if (bvh.n == 0) return hipSuccess;
+    leaf_order_kernel<<<(bvh.n + 255) / 256, 256, 0, stream>>>(S.sorted_pid, S.tri_sorted.as<float>(), bvh.n,
+                                                               bvh.prim_order.as<uint32_t>(), tri_leaf);
+    return hipGetLastError();
+}
+
 void BuildScratch::release() {
     DevBuf *all[] = {&counters, &hist, &keys[0], &keys[1], &vals[0], &vals[1], &prec[0], &prec[1],
                      &trec[0], &trec[1], &lists, &acc, &splits, &tasks, &task_off, &task_left,
                      &task_lprims, &scan_scratch, &sort.status, &sort.offsets, &sort.counters,
-                     &top_items, &top_acc, &top_tasks, &top_hist, &top_left, &top_bkt};
+                     &top_items, &top_acc, &top_tasks, &top_hist, &top_left, &top_bkt, &tri_sorted};
     for (DevBuf *b : all) b->release();
     if (host_counters) hipHostFree(host_counters);
     host_counters = nullptr;

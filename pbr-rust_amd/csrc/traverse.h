@@ -21,8 +21,6 @@ struct BvhTrace {
 hipError_t trace_binary(const BvhTrace &B, const float *rays, uint32_t n, int any_hit, uint32_t *prim_out,
                         float *t_out, float *bary_out, uint32_t *err, unsigned long long *counts,
                         hipStream_t stream);
-hipError_t make_leaf_triangles(const float *tri, const uint32_t *prim_order, uint32_t n, float *tri_leaf,
-                               hipStream_t stream);
 hipError_t deindex_triangles(const float *verts, const uint32_t *idx, uint32_t nt, uint32_t nv, float *tri,
                              uint32_t *err, hipStream_t stream);
 hipError_t ao_bounce(const float *tri, uint32_t num_tri, const float *rays, const uint32_t *prim, const float *bary,

@@ -105,21 +105,6 @@ __global__ __launch_bounds__(kTraceBlock) void trace_binary_kernel(
     }
 }
 
-// Triangles of leaf position k = input triangle prim_order[k], 48-byte stride
-// (three 16-byte loads per triangle).
-__global__ __launch_bounds__(256) void leaf_triangles_kernel(const float *__restrict__ tri,
-                                                             const uint32_t *__restrict__ prim_order, uint32_t n,
-                                                             float *__restrict__ tri_leaf) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t p = prim_order[k];
-    const float *s = tri + 9ull * (p < n ? p : 0u);  // p < n for every complete tree
-    float4 *d = reinterpret_cast<float4 *>(tri_leaf) + 3ull * k;
-    d[0] = make_float4(s[0], s[1], s[2], s[3]);
-    d[1] = make_float4(s[4], s[5], s[6], s[7]);
-    d[2] = make_float4(s[8], 0.f, 0.f, 0.f);
-}
-
 __global__ __launch_bounds__(256) void deindex_kernel(const float *__restrict__ verts, const uint32_t *__restrict__ idx,
                                                       uint32_t nt, uint32_t nv, float *__restrict__ tri,
                                                       uint32_t *__restrict__ err) {
@@ -207,13 +192,6 @@ hipError_t trace_binary(const BvhTrace &B, const float *rays, uint32_t n, int an
         trace_binary_kernel<false><<<grid, kTraceBlock, 0, stream>>>(B.nodes, 2 * B.num_prims - 1, B.num_prims,
                                                                      B.tri_leaf, B.prim_order, r, n, any_hit,
                                                                      prim_out, t_out, bary_out, err, nullptr);
-    return hipGetLastError();
-}
-
-hipError_t make_leaf_triangles(const float *tri, const uint32_t *prim_order, uint32_t n, float *tri_leaf,
-                               hipStream_t stream) {
-    if (n == 0) return hipSuccess;
-    leaf_triangles_kernel<<<(n + 255) / 256, 256, 0, stream>>>(tri, prim_order, n, tri_leaf);
     return hipGetLastError();
 }
 
