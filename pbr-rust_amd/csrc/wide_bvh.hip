@@ -35,7 +35,7 @@ namespace pbrtm {
 namespace {
 
 struct Job {
-    uint32_t bnode, wnode, depth, pad;
+    uint32_t bnode, wnode, depth, prims;  // prims: primitives under the binary node
 };
 
 // 2^e for e in [-126, 127], exactly.
@@ -94,11 +94,16 @@ __device__ __forceinline__ Box bnode_box(const BNode &b) {
 }
 __device__ __forceinline__ bool bnode_leaf(const BNode &b) { return (b.a & kLeafFlag) != 0; }
 
-// Collapse counters: [1] wide nodes used, [4] levels that had jobs, [kCollapseJobs
-// + L % 3] the job count of level L (three slots in rotation).
+// Collapse counters: [4] levels that had jobs, [kCollapseJobs + L % 3] the job
+// count J(L) of level L and [kCollapseBase + L % 3] the wide index B(L) of its
+// first job (three slots each, in rotation). Level L's jobs are the wide nodes
+// B(L) .. B(L) + J(L) - 1, so the children it appends get B(L) + J(L) + their job
+// index at level L + 1, and B(L) + J(L) of the first level without jobs is the
+// number of wide nodes.
 constexpr uint32_t kCollapseJobs = 8;
+constexpr uint32_t kCollapseBase = 12;
 
-__global__ void wide_root_kernel(const BNode *bn, Job *jobs, uint32_t *counters, WNode *wn) {
+__global__ void wide_root_kernel(const BNode *bn, uint32_t n, Job *jobs, uint32_t *counters, WNode *wn) {
     if (threadIdx.x != 0) return;
     const BNode r = bn[0];
     if (bnode_leaf(r)) {  // the whole tree is one leaf: a wide root with one leaf slot
@@ -110,33 +115,51 @@ __global__ void wide_root_kernel(const BNode *bn, Job *jobs, uint32_t *counters,
             wn[0].ref[s][0] = use ? r.a : kWideEmpty;
             wn[0].ref[s][1] = use ? r.b : 0u;
         }
-        counters[1] = 1;                // nodes used
-        counters[kCollapseJobs] = 0;    // no jobs at level 0
+        counters[kCollapseJobs] = 0;  // no jobs at level 0
+        counters[kCollapseBase] = 1;  // one node, written here
         return;
     }
-    jobs[0] = Job{0, 0, 1, 0};
+    jobs[0] = Job{0, 0, 1, n};
     counters[kCollapseJobs] = 1;
-    counters[1] = 1;
+    counters[kCollapseBase] = 0;
 }
 
 constexpr int kCollapseBlock = 64;
+constexpr int kCollapseWaves = kCollapseBlock / kWave;
+// A binary subtree of P primitives holds its 2P - 1 nodes contiguously in
+// preorder (left child = id + 1, right child = id + 2 * P_left). Up to
+// kPrefetchPrims primitives (31 nodes, at most 9 lines of 128 bytes), a job
+// loads every line of its subtree at once, so that the dependent node fetches
+// of the collapse find them in cache instead of waiting on memory one by one.
+constexpr uint32_t kPrefetchPrims = 16;
+constexpr uint32_t kPrefetchLines = 9;
 
 // One wide node per lane. Frontier state lives in registers: every array below
 // is indexed only with compile-time indices (fully unrolled loops with selects).
-// All 64 lanes of the wave take part (live or not): the child reservation is one
-// atomic per wave.
+// Every thread of the workgroup (one wave) takes part, live or not: the child
+// reservation is one atomic per workgroup.
 __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const Job *__restrict__ jobs, uint32_t i,
                                              bool live, Job *__restrict__ next, uint32_t *__restrict__ next_count,
-                                             uint32_t *__restrict__ counters,
+                                             uint32_t node_base, uint32_t *s_wave, uint32_t &s_base,
                                              WNode *__restrict__ wn, uint32_t *__restrict__ order, uint32_t cap,
-                                             uint32_t *__restrict__ err) {
+                                             uint32_t num_bnodes, uint32_t *__restrict__ err) {
     const Job job = live ? jobs[i] : Job{0, 0, 0, 0};
+    uint32_t touch = 0u;
+    if (live && job.prims >= 2u && job.prims <= kPrefetchPrims) {
+        const uint32_t *words = reinterpret_cast<const uint32_t *>(bn);
+        const uint32_t last_node = min(job.bnode + 2u * job.prims - 2u, num_bnodes - 1u);
+        const uint32_t first_line = job.bnode >> 2, last_line = last_node >> 2;
+#pragma unroll
+        for (uint32_t k = 1; k < kPrefetchLines; ++k)  // line 0 holds the root, loaded next
+            if (first_line + k <= last_line) touch ^= words[size_t(first_line + k) * 32u];
+    }
     uint32_t fid[kWide];      // frontier binary ids, left to right
+    uint32_t fp[kWide];       // their primitive counts
     float farea[kWide];       // surface area, or -1 for a leaf / unused slot
     uint32_t fwa[kWide], fwb[kWide];  // the frontier nodes' a and b words (children, or leaf range)
     uint32_t oid[kWide - 1], ol[kWide - 1], orr[kWide - 1], oax[kWide - 1];  // opened nodes
 #pragma unroll
-    for (int k = 0; k < kWide; ++k) { fid[k] = 0u; farea[k] = -1.0f; fwa[k] = fwb[k] = 0u; }
+    for (int k = 0; k < kWide; ++k) { fid[k] = 0u; fp[k] = 0u; farea[k] = -1.0f; fwa[k] = fwb[k] = 0u; }
 #pragma unroll
     for (int k = 0; k < kWide - 1; ++k) { oid[k] = 0xFFFFFFFFu; ol[k] = orr[k] = oax[k] = 0u; }
     int nF = 0, nE = 0;
@@ -147,6 +170,8 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
         const BNode c0 = bn[ol[0]], c1 = bn[orr[0]];
         fid[0] = ol[0]; farea[0] = bnode_leaf(c0) ? -1.0f : box_area(bnode_box(c0));
         fid[1] = orr[0]; farea[1] = bnode_leaf(c1) ? -1.0f : box_area(bnode_box(c1));
+        fp[0] = (orr[0] - job.bnode) >> 1;
+        fp[1] = job.prims - fp[0];
         fwa[0] = c0.a; fwb[0] = c0.b;
         fwa[1] = c1.a; fwb[1] = c1.b;
         nF = 2;
@@ -159,14 +184,16 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
             if (best < 0) break;
             // the opened node's child ids are already in registers: one dependent
             // fetch (its two children, independent of each other) per step
-            uint32_t bid = 0, ba = 0, bb = 0;
+            uint32_t bid = 0, ba = 0, bb = 0, bp = 0;
 #pragma unroll
             for (int k = 0; k < kWide; ++k) {
                 bid = (k == best) ? fid[k] : bid;
                 ba = (k == best) ? fwa[k] : ba;
                 bb = (k == best) ? fwb[k] : bb;
+                bp = (k == best) ? fp[k] : bp;
             }
             const uint32_t cl = ba, cr = bb & 0x3FFFFFFFu;
+            const uint32_t pl = (cr - bid) >> 1, pr = bp - pl;
             const BNode nl = bn[cl], nr = bn[cr];
 #pragma unroll
             for (int k = 0; k < kWide - 1; ++k) {
@@ -181,28 +208,37 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
             const float ar = bnode_leaf(nr) ? -1.0f : box_area(bnode_box(nr));
 #pragma unroll
             for (int k = kWide - 1; k >= 0; --k) {  // shift right of best by one, then place the children
-                const uint32_t prev_id = k > 0 ? fid[k - 1] : 0u;
+                const uint32_t prev_id = k > 0 ? fid[k - 1] : 0u, prev_p = k > 0 ? fp[k - 1] : 0u;
                 const float prev_a = k > 0 ? farea[k - 1] : -1.0f;
                 const uint32_t prev_wa = k > 0 ? fwa[k - 1] : 0u, prev_wb = k > 0 ? fwb[k - 1] : 0u;
-                if (k > best + 1) { fid[k] = prev_id; farea[k] = prev_a; fwa[k] = prev_wa; fwb[k] = prev_wb; }
-                else if (k == best + 1) { fid[k] = cr; farea[k] = ar; fwa[k] = nr.a; fwb[k] = nr.b; }
-                else if (k == best) { fid[k] = cl; farea[k] = al; fwa[k] = nl.a; fwb[k] = nl.b; }
+                if (k > best + 1) { fid[k] = prev_id; fp[k] = prev_p; farea[k] = prev_a; fwa[k] = prev_wa; fwb[k] = prev_wb; }
+                else if (k == best + 1) { fid[k] = cr; fp[k] = pr; farea[k] = ar; fwa[k] = nr.a; fwb[k] = nr.b; }
+                else if (k == best) { fid[k] = cl; fp[k] = pl; farea[k] = al; fwa[k] = nl.a; fwb[k] = nl.b; }
             }
             ++nF;
         }
     }
-    // interior children need wide node indices: one atomic per wave
+    // interior children need job slots (and so wide node indices): one atomic per
+    // workgroup. s_wave is read only between the two barriers and written only
+    // after the second, so consecutive calls may share it.
     uint32_t n_int = 0;
 #pragma unroll
     for (int s = 0; s < kWide; ++s) n_int += (s < nF && farea[s] >= 0.0f) ? 1u : 0u;
     const uint32_t incl = wave_inclusive_add(n_int);
-    uint32_t base = 0, jbase = 0;
-    if (__lane_id() == kWave - 1 && incl > 0) {
-        base = atomicAdd(&counters[1], incl);
-        jbase = atomicAdd(next_count, incl);
+    const uint32_t wv = threadIdx.x / kWave;
+    if (__lane_id() == kWave - 1) s_wave[wv] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0, total = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < uint32_t(kCollapseWaves); ++k) {
+        const uint32_t t = s_wave[k];
+        wave_off += k < wv ? t : 0u;
+        total += t;
     }
-    base = __shfl(base, kWave - 1, kWave) + incl - n_int;
-    jbase = __shfl(jbase, kWave - 1, kWave) + incl - n_int;
+    if (threadIdx.x == 0) s_base = total > 0 ? atomicAdd(next_count, total) : 0u;
+    __syncthreads();
+    const uint32_t jbase = s_base + wave_off + incl - n_int;
+    const uint32_t base = node_base + jbase;
     if (!live) return;
     if (base + n_int > cap) {
         atomicOr(err, uint32_t(kErrCapacity));
@@ -215,7 +251,7 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
         if (s < nF) {
             if (farea[s] >= 0.0f) {
                 const uint32_t idx = base + k_int;
-                next[jbase + k_int] = Job{fid[s], idx, job.depth + 1, 0};
+                next[jbase + k_int] = Job{fid[s], idx, job.depth + 1, fp[s]};
                 ++k_int;
                 refw[s][0] = idx;
                 refw[s][1] = 0u;
@@ -293,27 +329,35 @@ __device__ __forceinline__ void collapse_one(const BNode *__restrict__ bn, const
     encode_node(w, root.lo, root.hi, [&](int s, float *l, float *h) {
         for (int a = 0; a < 3; ++a) l[a] = cbl[s][a], h[a] = cbh[s][a];
     }, nF);
+    // the prefetched words are consumed here (a depth of all ones never occurs)
+    if (touch == 0x9E3779B9u && job.depth == 0xFFFFFFFFu) atomicOr(err, 0u);
 }
 
 // One level of the collapse. Job counts rotate over three counter slots: level L
 // reads its count from slot L % 3, appends its children's jobs to slot (L + 1) % 3
 // and clears slot (L + 2) % 3 (level L - 1's, read by no one any more; level L + 1
-// appends to it), so no kernel runs between levels. A grid of resident waves walks
-// the jobs 64 at a time; a level that had jobs is counted (counters[4]).
+// appends to it), so no kernel runs between levels; B(L + 1) = B(L) + J(L) goes
+// to the base slot (L + 1) % 3 the same way. A grid of resident workgroups walks
+// the jobs kCollapseBlock at a time; a level that had jobs is counted (counters[4]).
 __global__ __launch_bounds__(kCollapseBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void wide_collapse_kernel(
     const BNode *__restrict__ bn, const Job *__restrict__ jobs, Job *__restrict__ next,
     uint32_t *__restrict__ counters, WNode *__restrict__ wn, uint32_t *__restrict__ order, uint32_t cap,
-    uint32_t *__restrict__ err, uint32_t level) {
+    uint32_t num_bnodes, uint32_t *__restrict__ err, uint32_t level) {
+    __shared__ uint32_t s_wave[kCollapseWaves];
+    __shared__ uint32_t s_base;
     const uint32_t njobs = counters[kCollapseJobs + level % 3];
+    const uint32_t node_base = counters[kCollapseBase + level % 3] + njobs;  // B(L + 1)
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         counters[kCollapseJobs + (level + 2) % 3] = 0u;
+        counters[kCollapseBase + (level + 1) % 3] = node_base;
         if (njobs > 0) counters[4] += 1u;
     }
     uint32_t *next_count = counters + kCollapseJobs + (level + 1) % 3;
     const uint32_t stride = gridDim.x * kCollapseBlock;
     for (uint32_t base = blockIdx.x * kCollapseBlock; base < njobs; base += stride) {
         const uint32_t i = base + threadIdx.x;
-        collapse_one(bn, jobs, i, i < njobs, next, next_count, counters, wn, order, cap, err);
+        collapse_one(bn, jobs, i, i < njobs, next, next_count, node_base, s_wave, s_base, wn, order, cap,
+                     num_bnodes, err);
     }
 }
 
@@ -724,13 +768,13 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
     Job *jobs[2] = {reinterpret_cast<Job *>(scratch.ptr),
                     reinterpret_cast<Job *>(static_cast<char *>(scratch.ptr) + job_bytes)};
     uint32_t *counters = reinterpret_cast<uint32_t *>(static_cast<char *>(scratch.ptr) + 2 * job_bytes);
-    // counters: [1] wide nodes used, [4] levels that had jobs, [kCollapseJobs + L % 3]
-    // the job count of level L
+    // counters: [4] levels that had jobs, [kCollapseJobs + L % 3] the job count of
+    // level L, [kCollapseBase + L % 3] the wide index of its first job
     e = hipMemsetAsync(counters, 0, 64, stream);
     if (e != hipSuccess) return e;
     WNode *wn = out.nodes.as<WNode>();
     uint32_t *order = out.order.as<uint32_t>();
-    wide_root_kernel<<<1, 64, 0, stream>>>(bnodes, jobs[0], counters, wn);
+    wide_root_kernel<<<1, 64, 0, stream>>>(bnodes, n, jobs[0], counters, wn);
     int per_cu = 0, cus = 0, dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wide_collapse_kernel, kCollapseBlock, 0) !=
@@ -744,7 +788,7 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
     // levels are enqueued in batches between reads of the job count (the first
     // batch: about the depth of a full 8-wide tree over n leaves, plus two); a
     // level past the last one finds no jobs and exits at once
-    uint32_t host[kCollapseJobs + 3];
+    uint32_t host[kCollapseBase + 3];
     uint32_t level = 0;
     int cur = 0;
     int batch = 2;
@@ -753,7 +797,7 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
         for (int k = 0; k < batch; ++k, ++level) {
             if (level > 100000) return hipErrorUnknown;
             wide_collapse_kernel<<<resident, kCollapseBlock, 0, stream>>>(bnodes, jobs[cur], jobs[1 - cur], counters,
-                                                                          wn, order, cap, err, level);
+                                                                          wn, order, cap, 2 * n - 1, err, level);
             cur = 1 - cur;
         }
         batch = 4;
@@ -763,7 +807,7 @@ hipError_t collapse_wide(const BNode *bnodes, uint32_t n, WideBvh &out, DevBuf &
         if (host[kCollapseJobs + level % 3] == 0) break;  // the next level has no jobs
     }
     out.collapse_levels = host[4];
-    out.num_nodes = host[1];
+    out.num_nodes = host[kCollapseBase + level % 3];  // B of the first level without jobs
     // every job of a level has the same depth, so the depth of the wide tree is
     // the number of levels that had jobs (a leaf root: one level, no jobs)
     out.max_depth = std::max(host[4], 1u);
