@@ -52,6 +52,11 @@ struct pbrtm_ctx {
     pbrtm::pt::PathPool batch_pool;
     DevBuf batch_jobs, batch_films;
     hipEvent_t ev[8] = {};
+    // a second stream (default priority) for build work that overlaps the wide
+    // collapse: aux_ev[0] orders it after the binary build, aux_ev[1] the
+    // collapse's end after it
+    hipStream_t aux = nullptr;
+    hipEvent_t aux_ev[2] = {};
     std::string error;
     std::vector<BatchTune> batch_tune;
     SpareBvh spare;
@@ -230,7 +235,13 @@ int pbrtm_create(pbrtm_ctx **out, int device) {
         delete ctx;
         return PBRTM_ERR_HIP;
     }
+    if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess) {
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return PBRTM_ERR_HIP;
+    }
     for (auto &e : ctx->ev) hipEventCreate(&e);
+    for (auto &e : ctx->aux_ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
     // the no-alloc entry point's scratch address on this device (checked again there)
     (void)noalloc_scratch_init(device);
     *out = ctx;
@@ -261,6 +272,9 @@ void pbrtm_destroy(pbrtm_ctx *ctx) {
                       &p.rsort.hist, &p.rsort.sort.status, &p.rsort.sort.offsets, &p.rsort.sort.counters})
         b->release();
     for (auto &e : ctx->ev) hipEventDestroy(e);
+    for (auto &e : ctx->aux_ev) hipEventDestroy(e);
+    hipStreamSynchronize(ctx->aux);
+    hipStreamDestroy(ctx->aux);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -313,6 +327,7 @@ int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint3
     const float empty[6] = {mx, mx, mx, -mx, -mx, -mx};
     std::memcpy(bvh->bounds, empty, sizeof(empty));
     auto bail = [&](int code) {
+        hipStreamSynchronize(ctx->aux);  // no leaf-order work left on buffers the context keeps
         pbrtm_bvh_free(bvh);
         return code;
     };
@@ -349,13 +364,18 @@ int pbrtm_bvh_build(pbrtm_ctx *ctx, const float *verts, uint64_t nv, const uint3
         if (e != hipSuccess) return bail(hip_fail(ctx, e, "build"));
         if (derr) return bail(fail(ctx, PBRTM_ERR_DEVICE, device_error_text(derr)));
         e = bvh->tri_leaf.reserve(nt * 48);
-        if (e == hipSuccess)
-            e = leaf_order(ctx->scratch, bvh->dev, bvh->tri_leaf.as<float>(), ctx->stream);
+        // the leaf order streams on the second stream while the collapse's first
+        // levels (a few nodes each, bound by their dependent fetches) run here
+        if (e == hipSuccess) e = hipEventRecord(ctx->aux_ev[0], ctx->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->aux, ctx->aux_ev[0], 0);
+        if (e == hipSuccess) e = leaf_order(ctx->scratch, bvh->dev, bvh->tri_leaf.as<float>(), ctx->aux);
+        if (e == hipSuccess) e = hipEventRecord(ctx->aux_ev[1], ctx->aux);
         if (e == hipSuccess) e = ctx->dev_err.reserve(64);
         if (e == hipSuccess) e = hipMemsetAsync(ctx->dev_err.ptr, 0, 4, ctx->stream);
         if (e == hipSuccess)
             e = collapse_wide(bvh->dev.nodes.as<BNode>(), uint32_t(nt), bvh->wide, ctx->collapse,
                               ctx->dev_err.as<uint32_t>(), ctx->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->aux_ev[1], 0);
         if (e == hipSuccess) e = hipEventRecord(ctx->ev[6], ctx->stream);
         uint32_t cerr = 0;
         if (e == hipSuccess) e = hipMemcpyAsync(&cerr, ctx->dev_err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream);
