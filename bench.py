@@ -55,7 +55,7 @@ FILM_RECORD_BYTES = 16 + 4  # per path: sensor RGB + filter weight, finishing ro
 # beside it, not measured by this run.
 TRAFFIC = {
     "trace_wide_kernel": (2_014_720_512, "profiles/trace_wide_hlbvh10m_1080p_pmc_traffic.md"),
-    "build": (14_257_693_200, "profiles/trace_wide_hlbvh10m_1080p_pmc_traffic.md"),
+    "build": (13_166_864_448, "profiles/trace_wide_hlbvh10m_1080p_pmc_traffic.md"),
 }
 
 
@@ -552,8 +552,8 @@ def main():
                   "traffic_gbs": round(build_traffic / (build_ms * 1e6), 1) if args.tris == 10_000_000 else None,
                   "bytes_per_build": BUILD_BYTES_PER_TRI * args.tris,
                   "algorithmic_bytes": f"{BUILD_BYTES_PER_TRI} B per triangle",
-                  "limiter": "two scattered triangle gathers (Morton order, leaf order), the wide collapse's "
-                             "dependent node fetches and the subtree finishers' per-split latency"}
+                  "limiter": "the scattered triangle gather into Morton order, the wide collapse's dependent "
+                             "node fetches and the subtree finishers' per-split latency"}
 
     line = {
         "metric": "HLBVH build ms @10M tris",
