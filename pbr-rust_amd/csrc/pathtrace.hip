@@ -570,14 +570,25 @@ __device__ __forceinline__ void shade_impl(const SceneDev &S, const State &st, c
         }
         terminate = true;
     }
-    // 2. finished paths: film record, then a new path in this slot
+    // 2. finished paths: film record, then a new path in this slot. The new path
+    // ids are reserved once per workgroup (every thread of it gets here): the ids
+    // handed out in a round are one contiguous run whatever the order of the
+    // reservations, and one shared counter serves them one at a time.
     const uint64_t lanes = __ballot(terminate);
-    uint32_t base = 0;
-    if (lanes) {
-        const int leader = __ffsll((unsigned long long)lanes) - 1;
-        if (lane_id() == leader) base = atomicAdd(&st.counters[kCntNextId], uint32_t(__popcll(lanes)));
-        base = __shfl(base, leader, kWave);
+    __shared__ uint32_t s_new[kBlock / kWave];
+    __shared__ uint32_t s_new_base;
+    const uint32_t wv = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
+    if (lane_id() == 0) s_new[wv] = uint32_t(__popcll(lanes));
+    __syncthreads();
+    uint32_t before = 0, total_new = 0;
+    for (uint32_t k = 0; k < nwaves; ++k) {
+        const uint32_t t = s_new[k];
+        before += k < wv ? t : 0u;
+        total_new += t;
     }
+    if (threadIdx.x == 0) s_new_base = total_new ? atomicAdd(&st.counters[kCntNextId], total_new) : 0u;
+    __syncthreads();
+    const uint32_t base = s_new_base + before;
     if (terminate) {
         // Infinite lights (wavefront_path.cu:140-160): every path that ends with a
         // positive beta, escaped or not; the megakernel path integrator only on an
